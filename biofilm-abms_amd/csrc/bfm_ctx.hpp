@@ -1,0 +1,142 @@
+// bfm_ctx.hpp -- context state of the C ABI (host side of libbfm.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/bfm.h"
+
+namespace bfm {
+
+// Grid parameters of one neighbor build, computed on the device.
+struct Plan {
+    double lo[3];
+    double edge;
+    double inv_edge;
+    double pre;       // reference prefilter radius thr + Ldiv + 2R
+    double thr;
+    int dims[3];
+    int reach;        // stencil half-width in bins
+    int nbins;
+    int ok;
+};
+
+// Device status bits, checked at synchronisation points.
+enum StatusBits {
+    ST_SLOT_OVERFLOW = 1,   // a cell has more neighbors than the per-cell slot capacity
+    ST_PAIR_OVERFLOW = 2,   // more pairs than max_pairs
+    ST_NAN = 4,             // non-finite velocity
+    ST_JKR_ROOT = 8,        // JKR quartic without admissible root
+    ST_QUEUE = 16           // internal candidate queue misuse (never expected)
+};
+
+struct SurrogateWeights;
+
+struct TimedLaunch {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct Tableau {
+    double A[49];
+    double b[7];
+    double bs[7];
+    int s;
+};
+
+}  // namespace bfm
+
+struct bfm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int max_cells = 0;
+    int64_t max_pairs = 0;
+    int max_bins = 0;
+    int slot_cap = 64;
+    double bin_scale = 1.0;
+    int block_threads = 256;
+
+    bfm_params p{};
+    bool params_set = false;
+    std::string err;
+
+    // device status word and counters
+    int* d_status = nullptr;
+    unsigned long long* d_counters = nullptr;   // [0] screened candidates, [1] segment tests
+
+    // staging for host-pointer entry points
+    double* d_cells = nullptr;      // max_cells x 32 (ld = max_cells)
+    double* d_cells_new = nullptr;  // max_cells x 32
+    double* d_nb = nullptr;         // max_pairs x 7
+    double* d_noise = nullptr;      // max_cells x 6
+    double* d_out6 = nullptr;       // max_cells x 6
+    double* d_err6 = nullptr;       // max_cells x 6
+    int* d_assume2d = nullptr;      // max_cells
+    double* d_ss = nullptr;         // unused spare
+
+    // binning
+    double* d_partials = nullptr;   // bbox partial reductions
+    bfm::Plan* d_plan = nullptr;
+    int* d_keys = nullptr; int* d_keys_alt = nullptr;
+    int* d_vals = nullptr; int* d_vals_alt = nullptr;
+    int* d_bin_count = nullptr;     // max_bins + 1
+    int* d_bin_off = nullptr;       // max_bins + 1
+    double* d_sorted = nullptr;     // 7 x max_cells SoA: x y z nx ny nz hl
+    int* d_sid = nullptr;           // sorted -> original index
+    int* d_skey = nullptr;          // sorted keys
+    int* d_count = nullptr;         // per-cell neighbor counts (max_cells + 1)
+    int* d_offsets = nullptr;       // exclusive scan of counts (max_cells + 1)
+    int* d_slot_j = nullptr;        // max_cells x slot_cap
+    double* d_slot_g = nullptr;     // max_cells x slot_cap x 5
+
+    // internal neighbor list (SoA)
+    int* pi = nullptr; int* pj = nullptr;
+    double* pdx = nullptr; double* pdy = nullptr; double* pdz = nullptr;
+    double* ps = nullptr; double* pt = nullptr;
+    int* d_m = nullptr;             // device pair count
+    int64_t h_m = -1;               // host copy when known
+    int list_n = 0;                 // number of cells the list refers to
+    bool list_sorted_i = false;     // rows sorted by i (i-role ranges contiguous)
+    bool transposed = false;        // ioff/joff/perm valid for the current list
+
+    // transposes (i-role offsets live in d_offsets)
+    bool ioff_valid = false;
+    int* d_joff = nullptr;                          // max_cells + 1
+    int* d_icnt = nullptr; int* d_jcnt = nullptr;   // max_cells + 1
+    int* d_perm_i = nullptr; int* d_perm_j = nullptr;  // max_pairs
+    bool use_perm_i = false;
+
+    // per-pair force magnitudes and per-cell forces
+    double* d_mag = nullptr;        // 2 x max_pairs (main, hard)
+    double* d_cc = nullptr;         // 6 x max_cells
+    double* d_k = nullptr;          // 7 x 6 x max_cells stage velocities
+    unsigned long long* d_errmax = nullptr;
+
+    // JKR table
+    double* d_tab_ov = nullptr; double* d_tab_g = nullptr; double* d_tab_r = nullptr;
+    int tab_nd = 0, tab_ng = 0;
+
+    // surrogate
+    bfm::SurrogateWeights* sur = nullptr;
+    double sur_env[4] = {0, 0, 0, 0};
+    int sur_polish = 1;
+
+    // hipcub temporary storage
+    void* d_temp = nullptr;
+    size_t temp_bytes = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<bfm::TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, int64_t>> ktimes;
+    std::vector<std::string> korder;
+
+    // last build stats
+    double last_edge = 0;
+    int last_nbins = 0;
+};

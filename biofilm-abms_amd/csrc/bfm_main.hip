@@ -1,0 +1,1596 @@
+// bfm_main.hip -- MI355X (gfx950) kernels and C ABI of the mechanical step.
+//
+// Data layout in HBM: the cells array stays column-major (each column is a
+// coalesced SoA stream, include/bfm.h). The neighbor list is SoA: int32 i, j
+// and five FP64 columns (dx, dy, dz, s, t), rows sorted (i asc, j asc).
+//
+// Pipeline of one RK stage (bfm_dev_stage):
+//   k_bbox_partial, k_plan        bounding box and uniform-grid plan (device side)
+//   k_keys                        bin id per cell + bin histogram
+//   hipcub radix sort             (bin id, cell index): deterministic bin order
+//   hipcub scan                   bin offsets
+//   k_reorder                     bin-ordered SoA copy of the geometry
+//   k_candidates                  one wavefront per 16 home cells; lanes screen
+//                                 the stencil bins, survivors are compacted into
+//                                 an LDS queue (ballot + popcount) and the
+//                                 segment-segment test runs on full wavefronts
+//   hipcub scan, k_compact        per-cell counts -> sorted (i, j) rows
+//   k_pair_forces                 Hertz / JKR magnitude per pair (+ j counts)
+//   k_fill_perm, k_sort_segments  transpose (rows where a cell is j), ascending
+//   k_velocities                  per cell: forces summed in ascending partner
+//                                 order (the reference's row order), surface
+//                                 forces, velocity solve
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "bfm_ctx.hpp"
+#include "bfm_device.hpp"
+#include "bfm_surrogate.hpp"
+
+using namespace bfm;
+
+// ---------------------------------------------------------------------------
+// error handling and launch instrumentation
+// ---------------------------------------------------------------------------
+namespace {
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+inline void hipchk(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw HipError{e, what};
+}
+#define HIPCHK(x) hipchk((x), #x)
+
+struct ArgError {
+    int code;
+    std::string msg;
+};
+
+inline void require(bool ok, int code, const std::string& msg)
+{
+    if (!ok) throw ArgError{code, msg};
+}
+
+hipEvent_t get_event(bfm_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+}
+
+struct LaunchScope {
+    bfm_ctx* c;
+    TimedLaunch tl;
+    LaunchScope(bfm_ctx* ctx, const char* name) : c(ctx)
+    {
+        if (c->timing) {
+            tl.name = name;
+            tl.a = get_event(c);
+            tl.b = get_event(c);
+            HIPCHK(hipEventRecord(tl.a, c->stream));
+        }
+    }
+    ~LaunchScope()
+    {
+        if (c->timing) {
+            if (hipEventRecord(tl.b, c->stream) == hipSuccess) c->pending.push_back(tl);
+        }
+    }
+};
+
+#define KLAUNCH(ctx, name, kern, grid, block, shm, ...)                         \
+    do {                                                                        \
+        LaunchScope _ls((ctx), (name));                                         \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shm, (ctx)->stream,   \
+                           __VA_ARGS__);                                        \
+        HIPCHK(hipGetLastError());                                              \
+    } while (0)
+
+void resolve_timing(bfm_ctx* c)
+{
+    if (c->pending.empty()) return;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& t : c->pending) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+        auto it = c->ktimes.find(t.name);
+        if (it == c->ktimes.end()) {
+            c->ktimes[t.name] = {ms, 1};
+            c->korder.push_back(t.name);
+        } else {
+            it->second.first += ms;
+            it->second.second += 1;
+        }
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+inline int grid_for(int64_t n, int block, int cap = 1 << 20)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+constexpr int kPairGrid = 2048;   // grid-stride launches over device-sized pair counts
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+namespace bfm {
+
+__device__ __forceinline__ double cellv(const double* __restrict__ cells, int ld, int i, int c)
+{
+    return cells[(int64_t)c * ld + i];
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int v, int64_t n)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x)
+        p[q] = v;
+}
+
+__global__ void k_iota(int* __restrict__ p, int n)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) p[q] = q;
+}
+
+// Block partials of min/max of the centers and max half-length.
+__global__ void __launch_bounds__(256) k_bbox_partial(const double* __restrict__ cells, int n, int ld,
+                                                      double* __restrict__ partials)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double hl = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double v = cellv(cells, ld, i, C_RX + k);
+            lo[k] = fmin(lo[k], v);
+            hi[k] = fmax(hi[k], v);
+        }
+        hl = fmax(hl, cellv(cells, ld, i, C_HALF_L));
+    }
+    __shared__ double red[7][256];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { red[k][threadIdx.x] = lo[k]; red[3 + k][threadIdx.x] = hi[k]; }
+    red[6][threadIdx.x] = hl;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+                red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + w]);
+            }
+            red[6][threadIdx.x] = fmax(red[6][threadIdx.x], red[6][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 7; ++k) partials[blockIdx.x * 7 + k] = red[k][0];
+}
+
+// Final reduction and grid plan. The search cutoff covers every pair the
+// reference can keep: |r_i - r_j| < thr + Ldiv + 2R (its prefilter) and,
+// since |d| >= |r_i - r_j| - hl_i - hl_j, |r_i - r_j| < thr + 2 max(hl).
+__global__ void k_plan(const double* __restrict__ partials, int nparts, double thr, double pre,
+                       double bin_scale, int max_bins, Plan* __restrict__ plan)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double hl = 0;
+    for (int b = 0; b < nparts; ++b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], partials[b * 7 + k]);
+            hi[k] = fmax(hi[k], partials[b * 7 + 3 + k]);
+        }
+        hl = fmax(hl, partials[b * 7 + 6]);
+    }
+    double cut = fmin(pre, thr + 2 * hl) * (1 + 1e-9) + 1e-9;
+    double edge = cut * bin_scale;
+    Plan P;
+    for (;;) {
+        int reach = (int)ceil(cut / edge - 1e-12);
+        if (reach < 1) reach = 1;
+        if (reach * edge < cut) reach++;
+        int64_t tot = 1;
+        int dims[3];
+        for (int k = 0; k < 3; ++k) {
+            double span = hi[k] - lo[k];
+            dims[k] = (int)floor(span / edge) + 1;
+            tot *= dims[k];
+        }
+        if (tot <= max_bins) {
+            for (int k = 0; k < 3; ++k) { P.lo[k] = lo[k]; P.dims[k] = dims[k]; }
+            P.reach = reach;
+            P.nbins = (int)tot;
+            break;
+        }
+        edge *= 1.25;
+    }
+    P.edge = edge;
+    P.inv_edge = 1.0 / edge;
+    P.pre = pre;
+    P.thr = thr;
+    P.ok = 1;
+    *plan = P;
+}
+
+__device__ __forceinline__ int bin_coord(double x, double lo, double edge, int dim)
+{
+    int b = (int)floor((x - lo) / edge);
+    return b < 0 ? 0 : (b >= dim ? dim - 1 : b);
+}
+
+__global__ void k_keys(const double* __restrict__ cells, int n, int ld, const Plan* __restrict__ plan,
+                       int* __restrict__ keys, int* __restrict__ vals, int* __restrict__ hist)
+{
+    const Plan P = *plan;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int bx = bin_coord(cellv(cells, ld, i, C_RX), P.lo[0], P.edge, P.dims[0]);
+        int by = bin_coord(cellv(cells, ld, i, C_RY), P.lo[1], P.edge, P.dims[1]);
+        int bz = bin_coord(cellv(cells, ld, i, C_RZ), P.lo[2], P.edge, P.dims[2]);
+        int key = (bz * P.dims[1] + by) * P.dims[0] + bx;
+        keys[i] = key;
+        vals[i] = i;
+        atomicAdd(&hist[key], 1);
+    }
+}
+
+// Bin-ordered SoA copy of the geometry the candidate kernel reads.
+__global__ void k_reorder(const double* __restrict__ cells, int n, int ld,
+                          const int* __restrict__ sorted_vals, double* __restrict__ sorted,
+                          int* __restrict__ sid, int64_t stride)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        int i = sorted_vals[q];
+        sid[q] = i;
+        sorted[0 * stride + q] = cellv(cells, ld, i, C_RX);
+        sorted[1 * stride + q] = cellv(cells, ld, i, C_RY);
+        sorted[2 * stride + q] = cellv(cells, ld, i, C_RZ);
+        sorted[3 * stride + q] = cellv(cells, ld, i, C_NX);
+        sorted[4 * stride + q] = cellv(cells, ld, i, C_NY);
+        sorted[5 * stride + q] = cellv(cells, ld, i, C_NZ);
+        sorted[6 * stride + q] = cellv(cells, ld, i, C_HALF_L);
+    }
+}
+
+constexpr int kHomesPerWave = 16;
+constexpr int kWaveSize = 64;
+constexpr int kQueueCap = 2 * kWaveSize;
+constexpr uint32_t kQMask = (1u << 27) - 1;
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ Seg load_sorted(const double* __restrict__ s, int64_t stride, int q)
+{
+    Seg c;
+    c.r[0] = s[q]; c.r[1] = s[stride + q]; c.r[2] = s[2 * stride + q];
+    c.n[0] = s[3 * stride + q]; c.n[1] = s[4 * stride + q]; c.n[2] = s[5 * stride + q];
+    c.hl = s[6 * stride + q];
+    return c;
+}
+
+// Candidate screening and segment tests. Each wavefront owns 16 consecutive
+// bin-ordered home cells. For home cell i it walks the stencil rows (bins
+// contiguous in x are contiguous in sorted order), lane by lane, and keeps
+// candidates j with j < i (original indices, the reference's j < i loop),
+// |r_i - r_j| <= thr + hl_i + hl_j (+ margin; below this the centerline
+// distance cannot reach thr, so dropping them cannot change the list) and the
+// reference's prefilter |r_i - r_j| < thr + Ldiv + 2R. Survivors go to a
+// per-wave LDS queue; every time it holds 64 entries the wavefront runs 64
+// segment-segment tests at once, so the expensive path never runs on a
+// mostly idle wavefront.
+template <int WAVES>
+__global__ void __launch_bounds__(WAVES * 64)
+k_candidates(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
+             const int* __restrict__ sid, const int* __restrict__ skey,
+             const int* __restrict__ bin_off, int slot_cap, int* __restrict__ count,
+             int* __restrict__ slot_j, double* __restrict__ slot_g,
+             unsigned long long* __restrict__ counters, int* __restrict__ status)
+{
+    __shared__ uint32_t queue[WAVES][kQueueCap];
+    __shared__ int hcnt[WAVES][kHomesPerWave];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int h0 = (blockIdx.x * WAVES + wave) * kHomesPerWave;
+    if (h0 >= n) return;
+    const int hN = min(kHomesPerWave, n - h0);
+    const Plan P = *plan;
+    const double thr = P.thr, pre = P.pre;
+    if (lane < kHomesPerWave) hcnt[wave][lane] = 0;
+    wave_sync();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    int qn = 0;
+    unsigned long long screened = 0, tested = 0;
+
+    auto process = [&](int cnt, int start) {
+        bool act = lane < cnt;
+        uint32_t e = act ? queue[wave][start + lane] : 0u;
+        wave_sync();
+        if (act) {
+            int hh = (int)(e >> 27), q = (int)(e & kQMask);
+            int hs = h0 + hh;
+            Seg ci = load_sorted(sorted, stride, hs);
+            Seg cj = load_sorted(sorted, stride, q);
+            double d[3], s, t;
+            dist_between_cells(ci, cj, d, s, t);
+            if (norm3(d[0], d[1], d[2]) < thr) {
+                int slot = atomicAdd(&hcnt[wave][hh], 1);
+                if (slot < slot_cap) {
+                    int64_t base = (int64_t)sid[hs] * slot_cap + slot;
+                    slot_j[base] = sid[q];
+                    double* g = slot_g + base * 5;
+                    g[0] = d[0]; g[1] = d[1]; g[2] = d[2]; g[3] = s; g[4] = t;
+                }
+            }
+        }
+        tested += (unsigned long long)cnt;
+    };
+
+    for (int hh = 0; hh < hN; ++hh) {
+        const int hs = h0 + hh;
+        const double xi = sorted[hs], yi = sorted[stride + hs], zi = sorted[2 * stride + hs];
+        const double hli = sorted[6 * stride + hs];
+        const int idi = sid[hs];
+        const int key = skey[hs];
+        const int bx = key % P.dims[0];
+        const int by = (key / P.dims[0]) % P.dims[1];
+        const int bz = key / (P.dims[0] * P.dims[1]);
+        const int x0 = max(0, bx - P.reach), x1 = min(P.dims[0] - 1, bx + P.reach);
+        for (int z = max(0, bz - P.reach); z <= min(P.dims[2] - 1, bz + P.reach); ++z) {
+            for (int y = max(0, by - P.reach); y <= min(P.dims[1] - 1, by + P.reach); ++y) {
+                const int row = (z * P.dims[1] + y) * P.dims[0];
+                const int lo = bin_off[row + x0], hi = bin_off[row + x1 + 1];
+                for (int base = lo; base < hi; base += 64) {
+                    const int q = base + lane;
+                    bool pass = false;
+                    if (q < hi) {
+                        const int idj = sid[q];
+                        if (idj < idi) {
+                            const double dx = xi - sorted[q];
+                            const double dy = yi - sorted[stride + q];
+                            const double dz = zi - sorted[2 * stride + q];
+                            const double r2 = (dx * dx + dy * dy) + dz * dz;
+                            const double bound = (thr + hli + sorted[6 * stride + q]) * (1 + 1e-9) + 1e-9;
+                            if (r2 <= bound * bound) pass = sqrt(r2) < pre;
+                        }
+                    }
+                    const uint64_t mask = __ballot(pass);
+                    screened += (unsigned long long)min(64, hi - base);
+                    if (pass) queue[wave][qn + __popcll(mask & lt_mask)] = ((uint32_t)hh << 27) | (uint32_t)q;
+                    qn += __popcll(mask);
+                    wave_sync();
+                    if (qn >= 64) {
+                        process(64, qn - 64);
+                        qn -= 64;
+                        wave_sync();
+                    }
+                }
+            }
+        }
+    }
+    if (qn > 0) process(qn, 0);
+    wave_sync();
+    if (lane < hN) {
+        int c = hcnt[wave][lane];
+        if (c > slot_cap) { atomicOr(status, ST_SLOT_OVERFLOW); c = slot_cap; }
+        count[sid[h0 + lane]] = c;
+    }
+    if (lane == 0) {
+        atomicAdd(&counters[0], screened);
+        atomicAdd(&counters[1], tested);
+    }
+}
+
+// Packs each cell's slots into the global row order: rows of cell i start at
+// offsets[i]; within the cell, ascending partner index (rank by comparison).
+__global__ void k_compact(int n, int slot_cap, const int* __restrict__ count,
+                          const int* __restrict__ offsets, const int* __restrict__ slot_j,
+                          const double* __restrict__ slot_g, int64_t max_pairs,
+                          int* __restrict__ pi, int* __restrict__ pj, double* __restrict__ pdx,
+                          double* __restrict__ pdy, double* __restrict__ pdz,
+                          double* __restrict__ ps, double* __restrict__ pt,
+                          int* __restrict__ d_m, int* __restrict__ status)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int c = count[i];
+        int64_t base = (int64_t)i * slot_cap;
+        int off = offsets[i];
+        for (int e = 0; e < c; ++e) {
+            int je = slot_j[base + e];
+            int rank = 0;
+            for (int f = 0; f < c; ++f) rank += slot_j[base + f] < je;
+            int64_t row = (int64_t)off + rank;
+            if (row < max_pairs) {
+                const double* g = slot_g + (base + e) * 5;
+                pi[row] = i; pj[row] = je;
+                pdx[row] = g[0]; pdy[row] = g[1]; pdz[row] = g[2];
+                ps[row] = g[3]; pt[row] = g[4];
+            }
+        }
+        if (i == n - 1) {
+            int64_t m = (int64_t)offsets[n];
+            if (m > max_pairs) { atomicOr(status, ST_PAIR_OVERFLOW); m = max_pairs; }
+            *d_m = (int)m;
+        }
+    }
+}
+
+// m x 7 column-major doubles <-> SoA pair list.
+__global__ void k_import_nb(const double* __restrict__ nb, int64_t m, int64_t ld,
+                            int* __restrict__ pi, int* __restrict__ pj, double* __restrict__ pdx,
+                            double* __restrict__ pdy, double* __restrict__ pdz,
+                            double* __restrict__ ps, double* __restrict__ pt)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        pi[k] = (int)nb[k]; pj[k] = (int)nb[ld + k];
+        pdx[k] = nb[2 * ld + k]; pdy[k] = nb[3 * ld + k]; pdz[k] = nb[4 * ld + k];
+        ps[k] = nb[5 * ld + k]; pt[k] = nb[6 * ld + k];
+    }
+}
+
+__global__ void k_export_nb(const int* __restrict__ d_m, double* __restrict__ nb, int64_t ld,
+                            const int* __restrict__ pi, const int* __restrict__ pj,
+                            const double* __restrict__ pdx, const double* __restrict__ pdy,
+                            const double* __restrict__ pdz, const double* __restrict__ ps,
+                            const double* __restrict__ pt)
+{
+    const int64_t m = *d_m;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        nb[k] = pi[k]; nb[ld + k] = pj[k];
+        nb[2 * ld + k] = pdx[k]; nb[3 * ld + k] = pdy[k]; nb[4 * ld + k] = pdz[k];
+        nb[5 * ld + k] = ps[k]; nb[6 * ld + k] = pt[k];
+    }
+}
+
+__device__ __forceinline__ Seg load_cell_seg(const double* __restrict__ cells, int ld, int i)
+{
+    Seg c;
+    c.r[0] = cellv(cells, ld, i, C_RX); c.r[1] = cellv(cells, ld, i, C_RY); c.r[2] = cellv(cells, ld, i, C_RZ);
+    c.n[0] = cellv(cells, ld, i, C_NX); c.n[1] = cellv(cells, ld, i, C_NY); c.n[2] = cellv(cells, ld, i, C_NZ);
+    c.hl = cellv(cells, ld, i, C_HALF_L);
+    return c;
+}
+
+// updateNeighborDistances, mechanics.hpp:604-648.
+__global__ void k_update_distances(const int* __restrict__ d_m, const double* __restrict__ cells, int ld,
+                                   const int* __restrict__ pi, const int* __restrict__ pj,
+                                   double* __restrict__ pdx, double* __restrict__ pdy,
+                                   double* __restrict__ pdz, double* __restrict__ ps,
+                                   double* __restrict__ pt)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        Seg ci = load_cell_seg(cells, ld, pi[k]);
+        Seg cj = load_cell_seg(cells, ld, pj[k]);
+        double d[3], s, t;
+        dist_between_cells(ci, cj, d, s, t);
+        pdx[k] = d[0]; pdy[k] = d[1]; pdz[k] = d[2]; ps[k] = s; pt[k] = t;
+    }
+}
+
+// Counts rows per cell of one index column (for transposes).
+__global__ void k_count_key(const int* __restrict__ d_m, const int* __restrict__ key, int* __restrict__ cnt)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+        atomicAdd(&cnt[key[k]], 1);
+}
+
+__global__ void k_fill_perm(const int* __restrict__ d_m, const int* __restrict__ key,
+                            const int* __restrict__ off, int* __restrict__ cursor, int* __restrict__ perm)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        int c = key[k];
+        int pos = off[c] + atomicAdd(&cursor[c], 1);
+        perm[pos] = k;
+    }
+}
+
+// Sorts each cell's segment of a permutation ascending (row order).
+__global__ void k_sort_segments(int n, const int* __restrict__ off, int* __restrict__ perm)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        int a = off[c], b = off[c + 1];
+        for (int x = a + 1; x < b; ++x) {
+            int v = perm[x];
+            int y = x - 1;
+            while (y >= a && perm[y] > v) { perm[y + 1] = perm[y]; --y; }
+            perm[y + 1] = v;
+        }
+    }
+}
+
+struct ForceParams {
+    double R, Rcell, E0, Req, max_overlap;
+    double pref0, pref1, pref2;
+    int adhesion;
+    int jkr_mode;
+    int colidx_gamma;
+    int pad;
+};
+
+// Per-pair force magnitudes (the scalar factors of cellCellInteractionForces,
+// mechanics.hpp:1209-1350 and :1390-1659): mag[0][k] multiplies the unit
+// distance vector of the main term (Hertz, EPS-capped Hertz or JKR
+// term1 - term2), mag[1][k] that of the separately accumulated hard-core term
+// (adhesive overload only; 0 otherwise). Pairs without overlap get 0.
+__global__ void __launch_bounds__(256)
+k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int ld,
+              const int* __restrict__ pi, const int* __restrict__ pj,
+              const double* __restrict__ pdx, const double* __restrict__ pdy,
+              const double* __restrict__ pdz, ForceParams fp,
+              const double* __restrict__ tab_ov, const double* __restrict__ tab_g,
+              const double* __restrict__ tab_r, int tab_nd, int tab_ng,
+              const double* __restrict__ sur_radius,
+              double* __restrict__ mag_main, double* __restrict__ mag_hard,
+              int* __restrict__ jcnt, int* __restrict__ status)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const int i = pi[k], j = pj[k];
+        if (jcnt) atomicAdd(&jcnt[j], 1);
+        const double dist = norm3(pdx[k], pdy[k], pdz[k]);
+        const double overlap = 2 * fp.R - dist;
+        double mm = 0, mh = 0;
+        if (overlap > 0) {
+            if (!fp.adhesion) {
+                if (overlap <= fp.max_overlap) {
+                    mm = fp.pref0 * pow_half<3>(overlap);
+                } else {
+                    mm = fp.pref2;
+                    mm += fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
+                }
+            } else {
+                const double gamma = dmin(cellv(cells, ld, i, fp.colidx_gamma),
+                                          cellv(cells, ld, j, fp.colidx_gamma));
+                if (gamma > 0) {
+                    double delta = overlap;
+                    if (fp.max_overlap >= 0 && delta > fp.max_overlap) delta = fp.max_overlap;
+                    double radius;
+                    if (fp.jkr_mode == BFM_JKR_TABLE_1D)
+                        radius = table_radius_1d(tab_ov, tab_r, tab_nd, delta);
+                    else if (fp.jkr_mode == BFM_JKR_TABLE_2D)
+                        radius = table_radius_2d(tab_ov, tab_nd, tab_g, tab_ng, tab_r, delta, gamma);
+                    else if (fp.jkr_mode == BFM_JKR_SURROGATE && sur_radius)
+                        radius = sur_radius[k];
+                    else
+                        radius = jkr_radius_newton(delta, fp.Req, fp.E0, gamma, 0.0, 200, nullptr);
+                    if (!(radius > 0)) atomicOr(status, ST_JKR_ROOT);
+                    mm = jkr_magnitude(radius, fp.Req, fp.E0, gamma);
+                } else {
+                    if (overlap <= fp.max_overlap) mm = fp.pref0 * pow_half<3>(overlap);
+                    else mm = fp.pref2;
+                }
+                if (overlap > fp.max_overlap)
+                    mh = fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
+            }
+        }
+        mag_main[k] = mm;
+        mag_hard[k] = mh;
+    }
+}
+
+struct VelParams {
+    SurfaceConsts sc;
+    double nz_threshold;
+    double coeff;
+    int no_surface;
+    int has_noise;
+};
+
+// dE/dq of the cell-cell interaction for cell c: the rows where c is i
+// (ranges ioff, optionally through perm_i) and where c is j (perm_j, sorted)
+// are merged in row order, so each component is accumulated in the order the
+// reference's sequential loop adds it (mechanics.hpp:1438-1647).
+__device__ __forceinline__ void gather_cc(int c, const int* __restrict__ ioff, const int* __restrict__ perm_i,
+                                          const int* __restrict__ joff, const int* __restrict__ perm_j,
+                                          const double* __restrict__ pdx, const double* __restrict__ pdy,
+                                          const double* __restrict__ pdz, const double* __restrict__ ps,
+                                          const double* __restrict__ pt, const double* __restrict__ mag_main,
+                                          const double* __restrict__ mag_hard, double R, double acc[6])
+{
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] = 0;
+    int a = ioff[c], ae = ioff[c + 1], b = joff[c], be = joff[c + 1];
+    while (a < ae || b < be) {
+        int ka = (a < ae) ? (perm_i ? perm_i[a] : a) : 0x7fffffff;
+        int kb = (b < be) ? perm_j[b] : 0x7fffffff;
+        const bool role_i = ka < kb;
+        const int k = role_i ? ka : kb;
+        if (role_i) ++a; else ++b;
+        const double dx = pdx[k], dy = pdy[k], dz = pdz[k];
+        const double dist = norm3(dx, dy, dz);
+        const double overlap = 2 * R - dist;
+        if (!(overlap > 0)) continue;
+        const double dn[3] = {dx / dist, dy / dist, dz / dist};
+        const double mm = mag_main[k];
+        const double st = role_i ? ps[k] : -pt[k];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double w = mm * dn[q];
+            acc[q] += role_i ? w : -w;
+            acc[q + 3] += st * w;
+        }
+        const double mh = mag_hard[k];
+        if (mh != 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double w = mh * dn[q];
+                acc[q] += role_i ? w : -w;
+                acc[q + 3] += st * w;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ CellVel load_cellvel(const double* __restrict__ cells, int ld, int i)
+{
+    CellVel v;
+    v.rz = cellv(cells, ld, i, C_RZ);
+    v.nx = cellv(cells, ld, i, C_NX); v.ny = cellv(cells, ld, i, C_NY); v.nz = cellv(cells, ld, i, C_NZ);
+    v.l = cellv(cells, ld, i, C_L); v.hl = cellv(cells, ld, i, C_HALF_L);
+    v.eta0 = cellv(cells, ld, i, C_ETA0); v.eta1 = cellv(cells, ld, i, C_ETA1);
+    v.drx = cellv(cells, ld, i, C_DRX); v.dry = cellv(cells, ld, i, C_DRY);
+    v.dnx = cellv(cells, ld, i, C_DNX); v.dny = cellv(cells, ld, i, C_DNY);
+    return v;
+}
+
+// Conservative forces (-F_cc - F_rep - F_adh, mechanics.hpp:1768) + noise and
+// the velocity solve for one cell.
+__device__ __forceinline__ void cell_velocity(const VelParams& vp, const CellVel& cv, double sigma0,
+                                              bool a2d, const double cc[6], const double* noise6,
+                                              double v[6])
+{
+    const double ss = a2d ? NAN : (vp.sc.R - cv.rz) / cv.nz;
+    double rep2 = 0, rep5 = 0, adh2 = 0, adh5 = 0;
+    if (!vp.no_surface)
+        surface_forces(vp.sc, a2d, cv.rz, cv.nz, cv.hl, cv.l, sigma0, ss, rep2, rep5, adh2, adh5);
+    double f[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) f[q] = -cc[q];
+    f[2] = (f[2] - rep2) - adh2;
+    f[5] = (f[5] - rep5) - adh5;
+    if (noise6) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[q] += noise6[q];
+    }
+    velocity_solve(cv, a2d, ss, vp.sc, vp.coeff, vp.no_surface, f, v);
+}
+
+// getVelocities for every cell (mechanics.hpp:2011-2347).
+__global__ void __launch_bounds__(256)
+k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restrict__ assume_2d,
+             const double* __restrict__ noise, VelParams vp,
+             const int* __restrict__ ioff, const int* __restrict__ perm_i,
+             const int* __restrict__ joff, const int* __restrict__ perm_j,
+             const double* __restrict__ pdx, const double* __restrict__ pdy,
+             const double* __restrict__ pdz, const double* __restrict__ ps,
+             const double* __restrict__ pt, const double* __restrict__ mag_main,
+             const double* __restrict__ mag_hard, double* __restrict__ vel,
+             double* __restrict__ cc_out, int* __restrict__ status)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double cc[6];
+        gather_cc(c, ioff, perm_i, joff, perm_j, pdx, pdy, pdz, ps, pt, mag_main, mag_hard, vp.sc.R, cc);
+        if (cc_out) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cc_out[(int64_t)q * n + c] = cc[q];
+        }
+        if (!vel) continue;
+        const CellVel cv = load_cellvel(cells, ld, c);
+        const bool a2d = assume_2d ? (assume_2d[c] != 0) : (cv.nz < vp.nz_threshold);
+        double nz6[6];
+        if (noise) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) nz6[q] = noise[(int64_t)q * n + c];
+        }
+        double v[6];
+        cell_velocity(vp, cv, cellv(cells, ld, c, C_SIGMA0), a2d, cc, noise ? nz6 : nullptr, v);
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            vel[(int64_t)q * n + c] = v[q];
+            bad |= !isfinite(v[q]);
+        }
+        if (bad) atomicOr(status, ST_NAN);
+    }
+}
+
+// Conservative forces only (getConservativeForces, include_constraint = false).
+__global__ void k_conservative(int n, const double* __restrict__ cells, int ld,
+                               const int* __restrict__ assume_2d, VelParams vp,
+                               const double* __restrict__ cc, double* __restrict__ out)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const CellVel cv = load_cellvel(cells, ld, c);
+        const bool a2d = assume_2d[c] != 0;
+        const double ss = a2d ? NAN : (vp.sc.R - cv.rz) / cv.nz;
+        double rep2 = 0, rep5 = 0, adh2 = 0, adh5 = 0;
+        if (!vp.no_surface)
+            surface_forces(vp.sc, a2d, cv.rz, cv.nz, cv.hl, cv.l, cellv(cells, ld, c, C_SIGMA0), ss,
+                           rep2, rep5, adh2, adh5);
+        for (int q = 0; q < 6; ++q) {
+            double r = (q == 2) ? rep2 : (q == 5 ? rep5 : 0.0);
+            double a = (q == 2) ? adh2 : (q == 5 ? adh5 : 0.0);
+            out[(int64_t)q * n + c] = (-cc[(int64_t)q * n + c] - r) - a;
+        }
+    }
+}
+
+__global__ void k_assume2d(int n, const double* __restrict__ cells, int ld, double thr, int* __restrict__ a2d)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        a2d[c] = cellv(cells, ld, c, C_NZ) < thr;
+}
+
+// One RK stage st >= 0 of stepRungeKuttaAdaptive (mechanics.hpp:2534-2557):
+// cells_i = cells + dt * sum_j A(st, j) k_j, normalizeOrientations, then
+// getVelocities with the step's frozen neighbor geometry (cell-cell forces cc
+// are identical in every stage, because they only read the neighbor rows).
+__global__ void __launch_bounds__(256)
+k_rk_stage(int n, const double* __restrict__ cells, int ld, const int* __restrict__ assume_2d,
+           const double* __restrict__ noise, VelParams vp, Tableau tb, int st, double dt,
+           const double* __restrict__ cc, double* __restrict__ kbuf, int* __restrict__ status)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        CellVel cv = load_cellvel(cells, ld, c);
+        if (st > 0) {
+            double y[6] = {cellv(cells, ld, c, C_RX), cellv(cells, ld, c, C_RY), cv.rz,
+                           cv.nx, cv.ny, cv.nz};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                double mult = 0;
+                for (int j = 0; j < st; ++j) mult += kbuf[((int64_t)j * 6 + q) * n + c] * tb.A[st * tb.s + j];
+                y[q] += mult * dt;
+            }
+            double nrm = norm3(y[3], y[4], y[5]);
+            y[3] /= nrm; y[4] /= nrm; y[5] /= nrm;
+            if (y[5] < 0) { y[3] *= -1; y[4] *= -1; y[5] *= -1; }
+            cv.rz = y[2]; cv.nx = y[3]; cv.ny = y[4]; cv.nz = y[5];
+        }
+        const bool a2d = assume_2d[c] != 0;
+        double ccl[6], nz6[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
+        if (noise) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) nz6[q] = noise[(int64_t)q * n + c];
+        }
+        double v[6];
+        cell_velocity(vp, cv, cellv(cells, ld, c, C_SIGMA0), a2d, ccl, noise ? nz6 : nullptr, v);
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            kbuf[((int64_t)st * 6 + q) * n + c] = v[q];
+            bad |= !isfinite(v[q]);
+        }
+        if (bad) atomicOr(status, ST_NAN);
+    }
+}
+
+__device__ __forceinline__ unsigned long long order_key(double x)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double order_val(unsigned long long k)
+{
+    unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+__global__ void k_err_init(unsigned long long* e) { *e = order_key(-INFINITY); }
+__global__ void k_err_final(const unsigned long long* e, double* out) { *out = order_val(*e); }
+
+// RK update (mechanics.hpp:2559-2611): weighted sums, position update,
+// embedded error, stored velocities, normalizeOrientations; plus the
+// controller's scaled-error maximum (simulation.hpp:1688-1695).
+__global__ void __launch_bounds__(256)
+k_rk_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau tb, double dt,
+           const double* __restrict__ kbuf, double* __restrict__ cells_new, int ld_new,
+           double* __restrict__ errors, double tol, unsigned long long* __restrict__ errmax)
+{
+    double local = -INFINITY;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        for (int col = 0; col < ncols; ++col)
+            if (col < 1 || col > 12) cells_new[(int64_t)col * ld_new + c] = cellv(cells, ld, c, col);
+        double y[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            double v1 = 0, v2 = 0;
+            for (int st = 0; st < tb.s; ++st) {
+                const double kv = kbuf[((int64_t)st * 6 + q) * n + c];
+                v1 += kv * tb.b[st];
+                v2 += kv * tb.bs[st];
+            }
+            const double d1 = v1 * dt, d2 = v2 * dt;
+            const double y0 = cellv(cells, ld, c, C_RX + q);
+            y[q] = y0 + d1;
+            const double e = d1 - d2;
+            errors[(int64_t)q * n + c] = e;
+            cells_new[(int64_t)(C_DRX + q) * ld_new + c] = v1;
+            if (tol > 0) local = fmax(local, e / (tol * (1.0 + fabs(y0))));
+        }
+        double nrm = norm3(y[3], y[4], y[5]);
+        y[3] /= nrm; y[4] /= nrm; y[5] /= nrm;
+        if (y[5] < 0) { y[3] *= -1; y[4] *= -1; y[5] *= -1; }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) cells_new[(int64_t)(C_RX + q) * ld_new + c] = y[q];
+    }
+    if (tol > 0) {
+        __shared__ double red[256];
+        red[threadIdx.x] = local;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicMax(errmax, order_key(red[0]));
+    }
+}
+
+}  // namespace bfm
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kMaxCols = 32;
+constexpr int kBBoxParts = 256;
+
+template <typename T>
+void dalloc(T** p, size_t count)
+{
+    HIPCHK(hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1)));
+}
+
+int nbits(int64_t v)
+{
+    int b = 1;
+    while ((int64_t(1) << b) <= v) ++b;
+    return b;
+}
+
+void check_status(bfm_ctx* c)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int st = 0;
+    HIPCHK(hipMemcpy(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) {
+        int zero = 0;
+        HIPCHK(hipMemcpy(c->d_status, &zero, sizeof(int), hipMemcpyHostToDevice));
+        if (st & ST_SLOT_OVERFLOW)
+            throw ArgError{BFM_ERR_CAPACITY, "a cell has more neighbors than the per-cell slot capacity"};
+        if (st & ST_PAIR_OVERFLOW)
+            throw ArgError{BFM_ERR_CAPACITY, "neighbor pairs exceed max_pairs of the context"};
+        if (st & ST_JKR_ROOT)
+            throw ArgError{BFM_ERR_JKR_ROOT,
+                           "JKR contact radius is undefined; invalid number of real roots found "
+                           "from radius-overlap polynomial"};
+        if (st & ST_NAN) throw ArgError{BFM_ERR_NAN, "Found nan in velocities"};
+        throw ArgError{BFM_ERR_HIP, "internal device status error"};
+    }
+}
+
+void need_params(bfm_ctx* c)
+{
+    require(c->params_set, BFM_ERR_ARG, "bfm_set_params has not been called");
+}
+
+void check_cells(bfm_ctx* c, int n, int ld)
+{
+    require(n >= 0 && n <= c->max_cells, BFM_ERR_CAPACITY, "n exceeds max_cells of the context");
+    require(ld >= n, BFM_ERR_ARG, "ld < n");
+}
+
+ForceParams force_params(const bfm_ctx* c)
+{
+    ForceParams fp;
+    fp.R = c->p.R; fp.Rcell = c->p.Rcell; fp.E0 = c->p.E0;
+    fp.Req = c->p.R / 2;
+    fp.max_overlap = 2 * (c->p.R - c->p.Rcell);
+    fp.pref0 = c->p.repulsion_prefactors[0];
+    fp.pref1 = c->p.repulsion_prefactors[1];
+    fp.pref2 = c->p.repulsion_prefactors[2];
+    fp.adhesion = c->p.adhesion_mode != BFM_ADHESION_NONE;
+    fp.jkr_mode = c->p.jkr_mode;
+    fp.colidx_gamma = c->p.colidx_gamma;
+    fp.pad = 0;
+    return fp;
+}
+
+VelParams vel_params(const bfm_ctx* c)
+{
+    VelParams vp;
+    const double R = c->p.R, E0 = c->p.E0;
+    vp.sc.R = R; vp.sc.E0 = E0; vp.sc.sqR = std::sqrt(R);
+    vp.sc.rep0 = 2 * E0;
+    vp.sc.rep1 = (8. / 3.) * E0 * std::pow(R, 0.5);
+    vp.sc.rep2 = 2 * E0 * std::pow(R, 0.5);
+    vp.sc.adh0 = std::pow(R, 0.5) / 2;
+    vp.sc.adh1 = 2 * bfm::kPi * R;
+    vp.sc.adh2 = 2 * std::pow(R, 0.5);
+    vp.nz_threshold = c->p.nz_threshold;
+    vp.coeff = c->p.cell_surface_coulomb_coeff;
+    vp.no_surface = c->p.no_surface;
+    vp.has_noise = 0;
+    return vp;
+}
+
+void scan(bfm_ctx* c, const int* in, int* out, int64_t items, const char* name)
+{
+    LaunchScope ls(c, name);
+    size_t tb = c->temp_bytes;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, in, out, (int)items, c->stream));
+}
+
+// Device neighbor build into the context's internal list.
+void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv)
+{
+    c->transposed = false;
+    c->list_n = n;
+    c->list_sorted_i = true;
+    c->use_perm_i = false;
+    c->ioff_valid = true;
+    if (n <= 1) {
+        int zero = 0;
+        HIPCHK(hipMemcpyAsync(c->d_m, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        KLAUNCH(c, "fill", k_fill_int, 1, 256, 0, c->d_offsets, 0, (int64_t)(n + 1));
+        c->h_m = 0;
+        return;
+    }
+    const double pre = thr + Ldiv + 2 * R;
+    const int nparts = std::min(kBBoxParts, grid_for(n, 256));
+    KLAUNCH(c, "bbox", k_bbox_partial, nparts, 256, 0, d_cells, n, ld, c->d_partials);
+    KLAUNCH(c, "plan", k_plan, 1, 64, 0, c->d_partials, nparts, thr, pre, c->bin_scale, c->max_bins, c->d_plan);
+    KLAUNCH(c, "fill", k_fill_int, grid_for(c->max_bins + 1, 256, 4096), 256, 0, c->d_bin_count, 0,
+            (int64_t)c->max_bins + 1);
+    KLAUNCH(c, "keys", k_keys, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_keys, c->d_vals,
+            c->d_bin_count);
+    {
+        LaunchScope ls(c, "radix_sort");
+        size_t tb = c->temp_bytes;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->d_temp, tb, c->d_keys, c->d_keys_alt, c->d_vals,
+                                                  c->d_vals_alt, n, 0, nbits(c->max_bins), c->stream));
+    }
+    scan(c, c->d_bin_count, c->d_bin_off, (int64_t)c->max_bins + 1, "scan_bins");
+    const int64_t stride = c->max_cells;
+    KLAUNCH(c, "reorder", k_reorder, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_vals_alt, c->d_sorted,
+            c->d_sid, stride);
+    HIPCHK(hipMemcpyAsync(c->d_skey, c->d_keys_alt, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+    const int waves = std::max(1, c->block_threads / 64);
+    const int homes_per_wg = waves * kHomesPerWave;
+    const int g = (n + homes_per_wg - 1) / homes_per_wg;
+    if (waves == 1)
+        KLAUNCH(c, "candidates", k_candidates<1>, g, 64, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                c->d_status);
+    else if (waves == 2)
+        KLAUNCH(c, "candidates", k_candidates<2>, g, 128, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                c->d_status);
+    else if (waves == 4)
+        KLAUNCH(c, "candidates", k_candidates<4>, g, 256, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                c->d_status);
+    else
+        KLAUNCH(c, "candidates", k_candidates<8>, g, 512, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                c->d_status);
+    KLAUNCH(c, "fill", k_fill_int, 1, 64, 0, c->d_count + n, 0, (int64_t)1);
+    scan(c, c->d_count, c->d_offsets, (int64_t)n + 1, "scan_counts");
+    KLAUNCH(c, "compact", k_compact, grid_for(n, 256), 256, 0, n, c->slot_cap, c->d_count, c->d_offsets,
+            c->d_slot_j, c->d_slot_g, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_m,
+            c->d_status);
+    c->h_m = -1;
+}
+
+// Transposes of the current list: rows per cell as j (sorted), and as i when
+// the list is not known to be sorted by i. k_pair_forces counts the j rows.
+void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
+                               const double* sur_radius)
+{
+    ForceParams fp = force_params(c);
+    if (!c->transposed)
+        KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
+    KLAUNCH(c, "pair_forces", k_pair_forces, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj, c->pdx,
+            c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng, sur_radius,
+            c->d_mag, c->d_mag + c->max_pairs, c->transposed ? (int*)nullptr : c->d_jcnt, c->d_status);
+    if (c->transposed) return;
+    scan(c, c->d_jcnt, c->d_joff, (int64_t)n + 1, "scan_j");
+    KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
+    KLAUNCH(c, "fill_perm", k_fill_perm, kPairGrid, 256, 0, c->d_m, c->pj, c->d_joff, c->d_jcnt, c->d_perm_j);
+    KLAUNCH(c, "sort_segments", k_sort_segments, grid_for(n, 256), 256, 0, n, c->d_joff, c->d_perm_j);
+    if (!c->ioff_valid) {
+        KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_icnt, 0, (int64_t)n + 1);
+        KLAUNCH(c, "count_i", k_count_key, kPairGrid, 256, 0, c->d_m, c->pi, c->d_icnt);
+        scan(c, c->d_icnt, c->d_offsets, (int64_t)n + 1, "scan_i");
+        c->ioff_valid = true;
+        if (!c->list_sorted_i) {
+            KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_icnt, 0, (int64_t)n + 1);
+            KLAUNCH(c, "fill_perm", k_fill_perm, kPairGrid, 256, 0, c->d_m, c->pi, c->d_offsets, c->d_icnt,
+                    c->d_perm_i);
+            KLAUNCH(c, "sort_segments", k_sort_segments, grid_for(n, 256), 256, 0, n, c->d_offsets,
+                    c->d_perm_i);
+        }
+    }
+    c->use_perm_i = !c->list_sorted_i;
+    c->transposed = true;
+}
+
+const int* ioff_of(bfm_ctx* c) { return c->d_offsets; }
+
+void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld)
+{
+    const double* sur = nullptr;
+    if (c->p.adhesion_mode != BFM_ADHESION_NONE && c->p.jkr_mode == BFM_JKR_SURROGATE) {
+        require(c->sur != nullptr, BFM_ERR_ARG, "surrogate weights not set (bfm_set_surrogate)");
+        sur = bfm::surrogate_pair_radii(c, d_cells, ld, n);
+    }
+    if ((c->p.jkr_mode == BFM_JKR_TABLE_1D || c->p.jkr_mode == BFM_JKR_TABLE_2D) &&
+        c->p.adhesion_mode != BFM_ADHESION_NONE)
+        require(c->d_tab_ov != nullptr, BFM_ERR_ARG, "JKR table not set (bfm_set_jkr_table)");
+    pair_forces_and_transpose(c, d_cells, ld, n, sur);
+}
+
+void launch_velocities(bfm_ctx* c, const double* d_cells, int n, int ld, const int* d_a2d,
+                       const double* d_noise, double* d_vel, double* d_cc)
+{
+    VelParams vp = vel_params(c);
+    KLAUNCH(c, "velocities", k_velocities, grid_for(n, 256), 256, 0, n, d_cells, ld, d_a2d, d_noise, vp,
+            ioff_of(c), c->use_perm_i ? c->d_perm_i : (const int*)nullptr, c->d_joff, c->d_perm_j, c->pdx,
+            c->pdy, c->pdz, c->ps, c->pt, c->d_mag, c->d_mag + c->max_pairs, d_vel, d_cc, c->d_status);
+}
+
+void import_host_neighbors(bfm_ctx* c, const double* nb, int64_t m, int64_t ld_nb, int n)
+{
+    require(m >= 0 && m <= c->max_pairs, BFM_ERR_CAPACITY, "m exceeds max_pairs of the context");
+    require(m == 0 || ld_nb >= m, BFM_ERR_ARG, "ld of neighbors < m");
+    bool sorted = true;
+    for (int64_t k = 0; k < m; ++k) {
+        double vi = nb[k], vj = nb[ld_nb + k];
+        require(vi >= 0 && vi < n && vj >= 0 && vj < n && vi == (int)vi && vj == (int)vj, BFM_ERR_ARG,
+                "neighbor indices out of range");
+        if (k > 0 && nb[k] < nb[k - 1]) sorted = false;
+    }
+    if (m > 0) {
+        HIPCHK(hipMemcpy2DAsync(c->d_nb, sizeof(double) * m, nb, sizeof(double) * ld_nb, sizeof(double) * m, 7,
+                                hipMemcpyHostToDevice, c->stream));
+        KLAUNCH(c, "import_nb", k_import_nb, grid_for(m, 256, 4096), 256, 0, c->d_nb, m, m, c->pi, c->pj, c->pdx,
+                c->pdy, c->pdz, c->ps, c->pt);
+    }
+    int mi = (int)m;
+    HIPCHK(hipMemcpyAsync(c->d_m, &mi, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    c->h_m = m;
+    c->list_n = n;
+    c->list_sorted_i = sorted;
+    c->transposed = false;
+    c->ioff_valid = false;
+}
+
+void upload_cells(bfm_ctx* c, const double* cells, int n, int ld)
+{
+    require(c->p.ncols > 0 && c->p.ncols <= kMaxCols, BFM_ERR_ARG, "params.ncols must be in 1..32");
+    if (n > 0)
+        HIPCHK(hipMemcpy2DAsync(c->d_cells, sizeof(double) * c->max_cells, cells, sizeof(double) * ld,
+                                sizeof(double) * n, c->p.ncols, hipMemcpyHostToDevice, c->stream));
+}
+
+void download6(bfm_ctx* c, const double* d, double* h, int n)
+{
+    if (n > 0)
+        HIPCHK(hipMemcpy2DAsync(h, sizeof(double) * n, d, sizeof(double) * n, sizeof(double) * n, 6,
+                                hipMemcpyDeviceToHost, c->stream));
+}
+
+Tableau make_tableau(const double* A, const double* b, const double* bs, int s)
+{
+    require(s >= 1 && s <= 7, BFM_ERR_ARG, "RK stages must be 1..7");
+    Tableau tb;
+    std::memset(&tb, 0, sizeof(tb));
+    tb.s = s;
+    for (int i = 0; i < s * s; ++i) tb.A[i] = A[i];
+    for (int i = 0; i < s; ++i) { tb.b[i] = b[i]; tb.bs[i] = bs[i]; }
+    return tb;
+}
+
+// RK step on device data with the internal list (stepRungeKuttaAdaptive).
+void step_rk_device(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld,
+                    const double* d_noise, double* d_cells_new, int ld_new, double* d_errors, double tol,
+                    double* d_err_scaled)
+{
+    need_params(c);
+    require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
+    require(c->p.friction_mode == 0, BFM_ERR_UNSUPPORTED, "cell-cell friction is not supported");
+    compute_forces(c, d_cells, n, ld);
+    KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
+            c->d_assume2d);
+    launch_velocities(c, d_cells, n, ld, c->d_assume2d, d_noise, nullptr, c->d_cc);
+    VelParams vp = vel_params(c);
+    for (int st = 0; st < tb.s; ++st)
+        KLAUNCH(c, "rk_stage", k_rk_stage, grid_for(n, 256), 256, 0, n, d_cells, ld, c->d_assume2d, d_noise, vp,
+                tb, st, dt, c->d_cc, c->d_k, c->d_status);
+    KLAUNCH(c, "err_init", k_err_init, 1, 1, 0, c->d_errmax);
+    KLAUNCH(c, "rk_final", k_rk_final, grid_for(n, 256), 256, 0, n, c->p.ncols, d_cells, ld, tb, dt, c->d_k,
+            d_cells_new, ld_new, d_errors, tol, c->d_errmax);
+    if (tol > 0 && d_err_scaled) KLAUNCH(c, "err_final", k_err_final, 1, 1, 0, c->d_errmax, d_err_scaled);
+}
+
+template <typename F>
+int guarded(bfm_ctx* c, F&& f)
+{
+    if (!c) return BFM_ERR_ARG;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        f();
+        return BFM_OK;
+    } catch (const ArgError& e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const HipError& e) {
+        c->err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        return BFM_ERR_HIP;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return BFM_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
+{
+    if (!out || max_cells < 1 || max_pairs < 1 || max_cells >= (1 << 27) || max_pairs >= (int64_t(1) << 31))
+        return BFM_ERR_ARG;
+    bfm_ctx* c = new bfm_ctx();
+    c->device = device;
+    c->max_cells = max_cells;
+    c->max_pairs = max_pairs;
+    c->max_bins = std::max(2 * max_cells, 4096);
+    int rc = guarded(c, [&] {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+        const size_t N = max_cells, P = max_pairs;
+        dalloc(&c->d_status, 1);
+        dalloc(&c->d_counters, 4);
+        dalloc(&c->d_cells, N * kMaxCols);
+        dalloc(&c->d_cells_new, N * kMaxCols);
+        dalloc(&c->d_nb, P * 7);
+        dalloc(&c->d_noise, N * 6);
+        dalloc(&c->d_out6, N * 6);
+        dalloc(&c->d_err6, N * 6);
+        dalloc(&c->d_assume2d, N);
+        dalloc(&c->d_partials, kBBoxParts * 7);
+        dalloc(&c->d_plan, 1);
+        dalloc(&c->d_keys, N); dalloc(&c->d_keys_alt, N);
+        dalloc(&c->d_vals, N); dalloc(&c->d_vals_alt, N);
+        dalloc(&c->d_bin_count, (size_t)c->max_bins + 1);
+        dalloc(&c->d_bin_off, (size_t)c->max_bins + 1);
+        dalloc(&c->d_sorted, N * 7);
+        dalloc(&c->d_sid, N);
+        dalloc(&c->d_skey, N);
+        dalloc(&c->d_count, N + 1);
+        dalloc(&c->d_offsets, N + 1);
+        dalloc(&c->d_slot_j, N * c->slot_cap);
+        dalloc(&c->d_slot_g, N * c->slot_cap * 5);
+        dalloc(&c->pi, P); dalloc(&c->pj, P);
+        dalloc(&c->pdx, P); dalloc(&c->pdy, P); dalloc(&c->pdz, P);
+        dalloc(&c->ps, P); dalloc(&c->pt, P);
+        dalloc(&c->d_m, 1);
+        dalloc(&c->d_joff, N + 1);
+        dalloc(&c->d_icnt, N + 1);
+        dalloc(&c->d_jcnt, N + 1);
+        dalloc(&c->d_perm_i, P);
+        dalloc(&c->d_perm_j, P);
+        dalloc(&c->d_mag, 2 * P);
+        dalloc(&c->d_cc, 6 * N);
+        dalloc(&c->d_k, 7 * 6 * N);
+        dalloc(&c->d_errmax, 1);
+        HIPCHK(hipMemset(c->d_status, 0, sizeof(int)));
+        HIPCHK(hipMemset(c->d_counters, 0, 4 * sizeof(unsigned long long)));
+        HIPCHK(hipMemset(c->d_m, 0, sizeof(int)));
+        // temporary storage: max over the sort and the scans
+        size_t t1 = 0, t2 = 0, t3 = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, c->d_keys, c->d_keys_alt, c->d_vals,
+                                                  c->d_vals_alt, max_cells, 0, nbits(c->max_bins)));
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, c->d_bin_count, c->d_bin_off, c->max_bins + 1));
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, c->d_count, c->d_offsets, max_cells + 1));
+        c->temp_bytes = std::max(t1, std::max(t2, t3));
+        HIPCHK(hipMalloc(&c->d_temp, c->temp_bytes));
+        HIPCHK(hipDeviceSynchronize());
+    });
+    if (rc != BFM_OK) {
+        *out = c;   // keep the context so the caller can read bfm_last_error
+        return rc;
+    }
+    *out = c;
+    return BFM_OK;
+}
+
+int bfm_destroy(bfm_ctx* c)
+{
+    if (!c) return BFM_ERR_ARG;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->d_status, c->d_counters, c->d_cells, c->d_cells_new, c->d_nb, c->d_noise, c->d_out6,
+                    c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
+                    c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
+                    c->d_offsets, c->d_slot_j, c->d_slot_g, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
+                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_mag, c->d_cc,
+                    c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    bfm::surrogate_free(c);
+    for (auto& t : c->pending) { c->event_pool.push_back(t.a); c->event_pool.push_back(t.b); }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return BFM_OK;
+}
+
+const char* bfm_last_error(bfm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int bfm_set_params(bfm_ctx* c, const bfm_params* p)
+{
+    return guarded(c, [&] {
+        require(p != nullptr, BFM_ERR_ARG, "params is NULL");
+        require(p->R > 0 && p->Rcell > 0 && p->Rcell <= p->R && p->E0 > 0, BFM_ERR_ARG,
+                "invalid radii or modulus");
+        require(p->friction_mode == 0, BFM_ERR_UNSUPPORTED, "cell-cell friction is not supported");
+        require(p->adhesion_mode == BFM_ADHESION_NONE || p->adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC,
+                BFM_ERR_UNSUPPORTED, "only NONE and JKR_ISOTROPIC adhesion are supported");
+        require(p->jkr_mode >= 0 && p->jkr_mode <= 3, BFM_ERR_ARG, "invalid jkr_mode");
+        require(p->ncols >= 21 && p->ncols <= kMaxCols, BFM_ERR_ARG, "ncols must be in 21..32");
+        if (p->adhesion_mode != BFM_ADHESION_NONE)
+            require(p->colidx_gamma >= 21 && p->colidx_gamma < p->ncols, BFM_ERR_ARG, "invalid colidx_gamma");
+        c->p = *p;
+        c->params_set = true;
+    });
+}
+
+int bfm_set_stream(bfm_ctx* c, void* stream)
+{
+    return guarded(c, [&] {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->own_stream) HIPCHK(hipStreamDestroy(c->stream));
+        if (stream) {
+            c->stream = (hipStream_t)stream;
+            c->own_stream = false;
+        } else {
+            HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+    });
+}
+
+int bfm_sync(bfm_ctx* c)
+{
+    return guarded(c, [&] { check_status(c); });
+}
+
+int bfm_set_jkr_table(bfm_ctx* c, const double* ov, int nd, const double* gm, int ng, const double* rad)
+{
+    return guarded(c, [&] {
+        require(ov && rad && nd >= 2, BFM_ERR_ARG, "invalid JKR table");
+        require(ng == 0 || (gm && ng >= 2), BFM_ERR_ARG, "invalid gamma mesh");
+        for (void* q : {(void*)c->d_tab_ov, (void*)c->d_tab_g, (void*)c->d_tab_r})
+            if (q) HIPCHK(hipFree(q));
+        c->d_tab_ov = c->d_tab_g = c->d_tab_r = nullptr;
+        dalloc(&c->d_tab_ov, nd);
+        HIPCHK(hipMemcpy(c->d_tab_ov, ov, sizeof(double) * nd, hipMemcpyHostToDevice));
+        if (ng > 0) {
+            dalloc(&c->d_tab_g, ng);
+            HIPCHK(hipMemcpy(c->d_tab_g, gm, sizeof(double) * ng, hipMemcpyHostToDevice));
+        }
+        size_t nr = (size_t)nd * std::max(ng, 1);
+        dalloc(&c->d_tab_r, nr);
+        HIPCHK(hipMemcpy(c->d_tab_r, rad, sizeof(double) * nr, hipMemcpyHostToDevice));
+        c->tab_nd = nd;
+        c->tab_ng = ng;
+    });
+}
+
+int bfm_get_cell_neighbors(bfm_ctx* c, const double* cells, int n, int ld, double thr, double R, double Ldiv,
+                           double* out, int64_t ld_out, int64_t* m_out)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        require(m_out != nullptr, BFM_ERR_ARG, "m_out is NULL");
+        upload_cells(c, cells, n, ld);
+        build_neighbors(c, c->d_cells, n, c->max_cells, thr, R, Ldiv);
+        check_status(c);
+        int m = 0;
+        HIPCHK(hipMemcpy(&m, c->d_m, sizeof(int), hipMemcpyDeviceToHost));
+        c->h_m = m;
+        *m_out = m;
+        require(m <= ld_out, BFM_ERR_CAPACITY, "neighbors_out is too small (see m_out)");
+        if (m > 0) {
+            KLAUNCH(c, "export_nb", k_export_nb, grid_for(m, 256, 4096), 256, 0, c->d_m, c->d_nb, (int64_t)m,
+                    c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+            HIPCHK(hipMemcpy2DAsync(out, sizeof(double) * ld_out, c->d_nb, sizeof(double) * m,
+                                    sizeof(double) * m, 7, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bfm_update_neighbor_distances(bfm_ctx* c, const double* cells, int n, int ld, double* nb, int64_t m,
+                                  int64_t ld_nb)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        upload_cells(c, cells, n, ld);
+        import_host_neighbors(c, nb, m, ld_nb, n);
+        if (m > 0) {
+            KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, c->d_cells,
+                    c->max_cells, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+            KLAUNCH(c, "export_nb", k_export_nb, grid_for(m, 256, 4096), 256, 0, c->d_m, c->d_nb, (int64_t)m,
+                    c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+            HIPCHK(hipMemcpy2DAsync(nb + 2 * ld_nb, sizeof(double) * ld_nb, c->d_nb + 2 * m, sizeof(double) * m,
+                                    sizeof(double) * m, 5, hipMemcpyDeviceToHost, c->stream));
+        }
+        check_status(c);
+    });
+}
+
+int bfm_get_conservative_forces(bfm_ctx* c, const double* cells, int n, int ld, const double* nb, int64_t m,
+                                int64_t ld_nb, const int* assume_2d, double* forces)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        require(assume_2d && forces, BFM_ERR_ARG, "NULL argument");
+        upload_cells(c, cells, n, ld);
+        import_host_neighbors(c, nb, m, ld_nb, n);
+        if (n > 0)
+            HIPCHK(hipMemcpyAsync(c->d_assume2d, assume_2d, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+        compute_forces(c, c->d_cells, n, c->max_cells);
+        launch_velocities(c, c->d_cells, n, c->max_cells, c->d_assume2d, nullptr, nullptr, c->d_cc);
+        VelParams vp = vel_params(c);
+        KLAUNCH(c, "conservative", k_conservative, grid_for(n, 256), 256, 0, n, c->d_cells, c->max_cells,
+                c->d_assume2d, vp, c->d_cc, c->d_out6);
+        download6(c, c->d_out6, forces, n);
+        check_status(c);
+    });
+}
+
+int bfm_get_velocities(bfm_ctx* c, const double* cells, int n, int ld, const double* nb, int64_t m,
+                       int64_t ld_nb, const int* assume_2d, const double* noise, double* vel)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        require(assume_2d && vel, BFM_ERR_ARG, "NULL argument");
+        upload_cells(c, cells, n, ld);
+        import_host_neighbors(c, nb, m, ld_nb, n);
+        if (n > 0) {
+            HIPCHK(hipMemcpyAsync(c->d_assume2d, assume_2d, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+            if (noise)
+                HIPCHK(hipMemcpyAsync(c->d_noise, noise, sizeof(double) * 6 * n, hipMemcpyHostToDevice, c->stream));
+        }
+        compute_forces(c, c->d_cells, n, c->max_cells);
+        launch_velocities(c, c->d_cells, n, c->max_cells, c->d_assume2d, noise ? c->d_noise : nullptr,
+                          c->d_out6, nullptr);
+        download6(c, c->d_out6, vel, n);
+        check_status(c);
+    });
+}
+
+int bfm_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs, int s, double dt,
+                const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                const double* noise, double* cells_new, double* errors)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        require(A && b && bs && cells_new && errors, BFM_ERR_ARG, "NULL argument");
+        Tableau tb = make_tableau(A, b, bs, s);
+        upload_cells(c, cells, n, ld);
+        import_host_neighbors(c, nb, m, ld_nb, n);
+        if (noise && n > 0)
+            HIPCHK(hipMemcpyAsync(c->d_noise, noise, sizeof(double) * 6 * n, hipMemcpyHostToDevice, c->stream));
+        step_rk_device(c, tb, dt, c->d_cells, n, c->max_cells, noise ? c->d_noise : nullptr, c->d_cells_new,
+                       c->max_cells, c->d_err6, 0.0, nullptr);
+        if (n > 0) {
+            HIPCHK(hipMemcpy2DAsync(cells_new, sizeof(double) * n, c->d_cells_new, sizeof(double) * c->max_cells,
+                                    sizeof(double) * n, c->p.ncols, hipMemcpyDeviceToHost, c->stream));
+            download6(c, c->d_err6, errors, n);
+        }
+        check_status(c);
+    });
+}
+
+int bfm_dev_build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv,
+                            int64_t* m_out)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        build_neighbors(c, d_cells, n, ld, thr, R, Ldiv);
+        if (m_out) {
+            check_status(c);
+            int m = 0;
+            HIPCHK(hipMemcpy(&m, c->d_m, sizeof(int), hipMemcpyDeviceToHost));
+            c->h_m = m;
+            *m_out = m;
+        }
+    });
+}
+
+int bfm_dev_export_neighbors(bfm_ctx* c, double* d_nb, int64_t ld_out)
+{
+    return guarded(c, [&] {
+        int m = 0;
+        HIPCHK(hipMemcpyAsync(&m, c->d_m, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        require(m <= ld_out, BFM_ERR_CAPACITY, "output too small for the neighbor list");
+        if (m > 0)
+            KLAUNCH(c, "export_nb", k_export_nb, grid_for(m, 256, 4096), 256, 0, c->d_m, d_nb, ld_out, c->pi,
+                    c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+    });
+}
+
+int bfm_dev_import_neighbors(bfm_ctx* c, const double* d_nb, int64_t m, int64_t ld_nb, int n)
+{
+    return guarded(c, [&] {
+        require(m >= 0 && m <= c->max_pairs, BFM_ERR_CAPACITY, "m exceeds max_pairs");
+        require(n >= 0 && n <= c->max_cells, BFM_ERR_CAPACITY, "n exceeds max_cells");
+        if (m > 0)
+            KLAUNCH(c, "import_nb", k_import_nb, grid_for(m, 256, 4096), 256, 0, d_nb, m, ld_nb, c->pi, c->pj,
+                    c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+        int mi = (int)m;
+        HIPCHK(hipMemcpyAsync(c->d_m, &mi, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->h_m = m;
+        c->list_n = n;
+        c->list_sorted_i = false;   // unknown order: use the general transpose
+        c->transposed = false;
+        c->ioff_valid = false;
+    });
+}
+
+int bfm_dev_get_velocities(bfm_ctx* c, const double* d_cells, int n, int ld, const int* d_a2d,
+                           const double* d_noise, double* d_vel)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
+        compute_forces(c, d_cells, n, ld);
+        launch_velocities(c, d_cells, n, ld, d_a2d, d_noise, d_vel, nullptr);
+    });
+}
+
+int bfm_dev_stage(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv,
+                  double* d_vel)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        build_neighbors(c, d_cells, n, ld, thr, R, Ldiv);
+        compute_forces(c, d_cells, n, ld);
+        launch_velocities(c, d_cells, n, ld, nullptr, nullptr, d_vel, nullptr);
+    });
+}
+
+int bfm_dev_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs, int s, double dt,
+                    const double* d_cells, int n, int ld, const double* d_noise, double* d_cells_new,
+                    double* d_errors, double tol, double* d_err_scaled)
+{
+    return guarded(c, [&] {
+        check_cells(c, n, ld);
+        Tableau tb = make_tableau(A, b, bs, s);
+        step_rk_device(c, tb, dt, d_cells, n, ld, d_noise, d_cells_new, n, d_errors, tol, d_err_scaled);
+    });
+}
+
+int bfm_enable_timing(bfm_ctx* c, int enable)
+{
+    return guarded(c, [&] { c->timing = enable != 0; });
+}
+
+int bfm_reset_timing(bfm_ctx* c)
+{
+    return guarded(c, [&] {
+        resolve_timing(c);
+        c->ktimes.clear();
+        c->korder.clear();
+    });
+}
+
+int bfm_kernel_times(bfm_ctx* c, char* names, int names_cap, double* ms, int64_t* counts, int cap, int* nk)
+{
+    return guarded(c, [&] {
+        resolve_timing(c);
+        std::string all;
+        int k = 0;
+        for (auto& name : c->korder) {
+            if (k >= cap) break;
+            ms[k] = c->ktimes[name].first;
+            counts[k] = c->ktimes[name].second;
+            all += name;
+            all += '\n';
+            ++k;
+        }
+        *nk = k;
+        if (names && names_cap > 0) {
+            std::strncpy(names, all.c_str(), names_cap - 1);
+            names[names_cap - 1] = 0;
+        }
+    });
+}
+
+int bfm_stats(bfm_ctx* c, int64_t* m, int64_t* tests, double* edge, int* nbins)
+{
+    return guarded(c, [&] {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int mm = 0;
+        HIPCHK(hipMemcpy(&mm, c->d_m, sizeof(int), hipMemcpyDeviceToHost));
+        unsigned long long ctr[2];
+        HIPCHK(hipMemcpy(ctr, c->d_counters, sizeof(ctr), hipMemcpyDeviceToHost));
+        Plan P;
+        HIPCHK(hipMemcpy(&P, c->d_plan, sizeof(Plan), hipMemcpyDeviceToHost));
+        if (m) *m = mm;
+        if (tests) { tests[0] = (int64_t)ctr[1]; }
+        if (edge) *edge = P.edge;
+        if (nbins) *nbins = P.nbins;
+        HIPCHK(hipMemset(c->d_counters, 0, sizeof(ctr)));
+    });
+}
+
+int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads)
+{
+    return guarded(c, [&] {
+        if (bin_scale > 0) {
+            require(bin_scale >= 0.2 && bin_scale <= 4.0, BFM_ERR_ARG, "bin_scale out of range");
+            c->bin_scale = bin_scale;
+        }
+        if (block_threads > 0) {
+            require(block_threads == 64 || block_threads == 128 || block_threads == 256 || block_threads == 512,
+                    BFM_ERR_ARG, "block_threads must be 64, 128, 256 or 512");
+            c->block_threads = block_threads;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+/*
+ * bfm.h -- C ABI of the MI355X mechanical step for spherocylinder biofilm ABMs.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never throws,
+ * and returns an int status (BFM_OK = 0). bfm_last_error(ctx) gives the message
+ * of the last failure on that context.
+ *
+ * Array conventions (identical to the reference's Eigen arrays):
+ *   cells      column-major n x ncols doubles with leading dimension ld >= n;
+ *              element (i, c) is cells[c * ld + i]. Column indices follow
+ *              ABM-3d/ABM-3d-cpp/include/indices.hpp:17-47 (id 0, r 1-3, n 4-6,
+ *              dr 7-9, dn 10-12, l 13, half_l 14, t0 15, growth 16, eta0 17,
+ *              eta1 18, sigma0 19, group 20) plus the optional gamma column
+ *              (params.colidx_gamma, 21 in the reference: simulation.hpp:1182).
+ *   neighbors  column-major m x 7 doubles with leading dimension ld_nb >= m:
+ *              (i, j, dx, dy, dz, s_i, s_j), rows sorted (i asc, j asc), j < i,
+ *              exactly as getCellNeighbors writes them (mechanics.hpp:536-586).
+ *   forces / velocities / noise / errors
+ *              column-major n x 6 doubles with leading dimension n.
+ *
+ * Entry points named bfm_* take HOST pointers and return once the results are
+ * in host memory (the reference's synchronous call semantics). Entry points
+ * named bfm_dev_* take DEVICE pointers, are enqueued on the context's stream
+ * and return after enqueue; bfm_sync waits for them.
+ *
+ * Threading: one context per host thread. Calls on one context are
+ * stream-ordered.
+ */
+#ifndef BFM_H
+#define BFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bfm_status {
+    BFM_OK = 0,
+    BFM_ERR_ARG = 1,          /* invalid argument */
+    BFM_ERR_CAPACITY = 2,     /* output or internal capacity exceeded */
+    BFM_ERR_NAN = 3,          /* NaN / Inf detected in forces or velocities */
+    BFM_ERR_JKR_ROOT = 4,     /* JKR contact-radius quartic has no admissible root */
+    BFM_ERR_HIP = 5,          /* HIP runtime error */
+    BFM_ERR_UNSUPPORTED = 6   /* mode not implemented behind this ABI */
+};
+
+/* Adhesion modes: AdhesionMode, mechanics.hpp:55-60. */
+enum bfm_adhesion_mode { BFM_ADHESION_NONE = 0, BFM_ADHESION_JKR_ISOTROPIC = 1 };
+
+/* How the isotropic JKR contact radius is obtained for a contact pair. */
+enum bfm_jkr_mode {
+    BFM_JKR_EXACT = 0,      /* per-pair FP64 solve of the quartic (adhesion.hpp:171-240) */
+    BFM_JKR_TABLE_1D = 1,   /* overlap table, fixed gamma (adhesion.hpp:282-370) */
+    BFM_JKR_TABLE_2D = 2,   /* (overlap, gamma) table (adhesion.hpp:412-526) */
+    BFM_JKR_SURROGATE = 3   /* BF16 MLP guess + FP64 polish, exact fallback outside envelope */
+};
+
+/* Physical and numerical parameters of getVelocities / stepRungeKuttaAdaptive
+ * (mechanics.hpp:2011-2028, 2406-2427). */
+typedef struct bfm_params {
+    double R;                          /* cell radius incl. EPS */
+    double Rcell;                      /* cell radius excl. EPS */
+    double E0;                         /* EPS modulus */
+    double repulsion_prefactors[3];    /* simulation.hpp:1271-1274 */
+    double nz_threshold;
+    double cell_cell_coulomb_coeff;    /* unused: friction mode NONE only */
+    double cell_surface_coulomb_coeff;
+    int no_surface;
+    int adhesion_mode;                 /* enum bfm_adhesion_mode */
+    int jkr_mode;                      /* enum bfm_jkr_mode */
+    int colidx_gamma;                  /* gamma column, -1 when adhesion is off */
+    int friction_mode;                 /* must be 0 (FrictionMode::NONE) */
+    int ncols;                         /* number of columns of the cells array */
+} bfm_params;
+
+typedef struct bfm_ctx bfm_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+int bfm_create(bfm_ctx** ctx, int device, int max_cells, int64_t max_pairs);
+int bfm_destroy(bfm_ctx* ctx);
+const char* bfm_last_error(bfm_ctx* ctx);
+int bfm_set_params(bfm_ctx* ctx, const bfm_params* params);
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL = own stream. */
+int bfm_set_stream(bfm_ctx* ctx, void* stream);
+int bfm_sync(bfm_ctx* ctx);
+
+/* JKR contact-radius table (IsotropicJKRDataFixedGamma, mechanics.hpp:113-153,
+ * or IsotropicJKRDataVariableGamma, mechanics.hpp:155-197). n_gamma == 0 gives
+ * the 1-D table; radii is n_overlap (1-D) or n_overlap x n_gamma row-major
+ * radii[i * n_gamma + j] (2-D). */
+int bfm_set_jkr_table(bfm_ctx* ctx, const double* overlaps, int n_overlap,
+                      const double* gammas, int n_gamma, const double* radii);
+
+/* JKR surrogate network: 4 -> 64 -> 64 -> 64 -> 1 with SiLU, weights as BF16
+ * bit patterns (uint16) row-major [out][in], biases FP32. envelope =
+ * {delta_lo, delta_hi, gamma_lo, gamma_hi} in absolute units (um, Pa um). */
+int bfm_set_surrogate(bfm_ctx* ctx, const uint16_t* w1, const float* b1,
+                      const uint16_t* w2, const float* b2,
+                      const uint16_t* w3, const float* b3,
+                      const uint16_t* w4, const float* b4,
+                      const double* envelope);
+
+/* ---- neighbor search -------------------------------------------------- */
+/* getCellNeighbors (mechanics.hpp:520-593). Writes m rows into neighbors_out
+ * (leading dimension ld_out). If m > ld_out, returns BFM_ERR_CAPACITY and sets
+ * *m_out to the required m. */
+int bfm_get_cell_neighbors(bfm_ctx* ctx, const double* cells, int n, int ld,
+                           double neighbor_threshold, double R, double Ldiv,
+                           double* neighbors_out, int64_t ld_out, int64_t* m_out);
+
+/* updateNeighborDistances (mechanics.hpp:604-648): recomputes columns 2-6. */
+int bfm_update_neighbor_distances(bfm_ctx* ctx, const double* cells, int n, int ld,
+                                  double* neighbors, int64_t m, int64_t ld_nb);
+
+/* ---- forces and velocities -------------------------------------------- */
+/* getConservativeForces (mechanics.hpp:1702-1769), include_constraint = false. */
+int bfm_get_conservative_forces(bfm_ctx* ctx, const double* cells, int n, int ld,
+                                const double* neighbors, int64_t m, int64_t ld_nb,
+                                const int* assume_2d, double* forces_out);
+
+/* getVelocities (mechanics.hpp:2011-2347). noise may be NULL (zero noise). */
+int bfm_get_velocities(bfm_ctx* ctx, const double* cells, int n, int ld,
+                       const double* neighbors, int64_t m, int64_t ld_nb,
+                       const int* assume_2d, const double* noise, double* vel_out);
+
+/* stepRungeKuttaAdaptive (mechanics.hpp:2406-2612). A is s x s row-major,
+ * b and bs have s entries. cells_new has the layout of cells with leading
+ * dimension n. noise (n x 6) may be NULL; the caller draws it from its own RNG
+ * stream, once per call, as the reference does (mechanics.hpp:2492-2510). */
+int bfm_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* bs,
+                int s, double dt, const double* cells, int n, int ld,
+                const double* neighbors, int64_t m, int64_t ld_nb,
+                const double* noise, double* cells_new, double* errors);
+
+/* ---- device-resident pipeline (device pointers, stream-ordered) -------- */
+/* Builds the context's internal neighbor list from device cells. *m_out (host
+ * pointer, may be NULL) receives m after a stream synchronisation; pass NULL
+ * to stay asynchronous. */
+int bfm_dev_build_neighbors(bfm_ctx* ctx, const double* d_cells, int n, int ld,
+                            double neighbor_threshold, double R, double Ldiv,
+                            int64_t* m_out);
+
+/* Copies the internal neighbor list into a device m x 7 column-major array. */
+int bfm_dev_export_neighbors(bfm_ctx* ctx, double* d_neighbors, int64_t ld_out);
+
+/* Loads an m x 7 device neighbor array as the internal list. */
+int bfm_dev_import_neighbors(bfm_ctx* ctx, const double* d_neighbors, int64_t m,
+                             int64_t ld_nb, int n);
+
+/* getVelocities on device data with the internal neighbor list. assume_2d is
+ * derived from d_cells when d_assume_2d is NULL. d_noise may be NULL. */
+int bfm_dev_get_velocities(bfm_ctx* ctx, const double* d_cells, int n, int ld,
+                           const int* d_assume_2d, const double* d_noise,
+                           double* d_vel);
+
+/* One config-1 RK stage: binning, sort, neighbor list, Hertz+JKR force and
+ * torque, surface forces and velocities, all on device. */
+int bfm_dev_stage(bfm_ctx* ctx, const double* d_cells, int n, int ld,
+                  double neighbor_threshold, double R, double Ldiv, double* d_vel);
+
+/* stepRungeKuttaAdaptive on device data with the internal neighbor list.
+ * d_err_scaled receives max over entries of errors / (tol * (1 + |y|)) (the
+ * reference controller's quantity, simulation.hpp:1688-1695) when tol > 0. */
+int bfm_dev_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* bs,
+                    int s, double dt, const double* d_cells, int n, int ld,
+                    const double* d_noise, double* d_cells_new, double* d_errors,
+                    double tol, double* d_err_scaled);
+
+/* ---- instrumentation -------------------------------------------------- */
+/* When enabled, every kernel launch is bracketed by HIP events on the launch
+ * stream; bfm_kernel_times returns per-kernel accumulated milliseconds and
+ * launch counts (names separated by '\n' in names_out). */
+int bfm_enable_timing(bfm_ctx* ctx, int enable);
+int bfm_kernel_times(bfm_ctx* ctx, char* names_out, int names_cap, double* ms_out,
+                     int64_t* counts_out, int cap, int* n_kernels);
+int bfm_reset_timing(bfm_ctx* ctx);
+
+/* Number of pairs in the internal list, candidate segment tests performed by
+ * the last neighbor build, and the last build's bin geometry. */
+int bfm_stats(bfm_ctx* ctx, int64_t* m, int64_t* candidate_tests, double* bin_edge,
+              int* nbins);
+
+/* Tuning knobs of the neighbor build: bin edge as a multiple of the search
+ * cutoff (>= 1 keeps the 27-bin stencil exact; 0.5 uses a 125-bin stencil),
+ * and threads per workgroup of the candidate kernel. 0 = defaults. */
+int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BFM_H */

@@ -1,0 +1,926 @@
+/*
+ * oracle.cpp -- CPU oracle of the mechanical step (TEST INFRASTRUCTURE ONLY).
+ *
+ * Restates the reference algorithms of ABM-3d/ABM-3d-cpp/include (paths below
+ * are relative to that directory) without Eigen, Boost or CGAL. It is the
+ * checker of the HIP path and the CPU baseline of bench.py; the product never
+ * links it. Build: oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
+ *
+ * Floating-point conventions: every expression is written with the
+ * evaluation order of the reference expression it restates (C++ evaluates
+ * a * b * c as (a * b) * c), and the build disables FMA contraction, so the
+ * rounding of each step is that of the plain IEEE operation.
+ *
+ * Deliberate differences from the reference (all documented in DESIGN.md):
+ *  - distBetweenCells, non-parallel branch: CGAL's segment-segment routine is
+ *    replaced by the clamped closed-form minimiser below (the minimiser of a
+ *    strictly convex quadratic is unique, so results agree to rounding).
+ *  - jkrContactRadius: the 30/100-digit Aberth solve is replaced by a
+ *    monotone FP64 Newton iteration on u = sqrt(a) (pinned against mpmath).
+ *  - Eigen's norm summation order is taken as ((x*x + y*y) + z*z).
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+// Column indices, include/indices.hpp:17-47.
+enum {
+    C_ID = 0, C_RX = 1, C_RY = 2, C_RZ = 3, C_NX = 4, C_NY = 5, C_NZ = 6,
+    C_DRX = 7, C_DRY = 8, C_DRZ = 9, C_DNX = 10, C_DNY = 11, C_DNZ = 12,
+    C_L = 13, C_HALF_L = 14, C_T0 = 15, C_GROWTH = 16, C_ETA0 = 17,
+    C_ETA1 = 18, C_SIGMA0 = 19, C_GROUP = 20
+};
+
+inline double cell(const double* cells, int ld, int i, int c) { return cells[(int64_t)c * ld + i]; }
+
+inline double norm3(double x, double y, double z) { return std::sqrt((x * x + y * y) + z * z); }
+
+// JKR table registered by oracle_set_jkr_table (the JKRData of the reference).
+std::vector<double> g_tab_overlaps, g_tab_gammas, g_tab_radii;
+
+}  // namespace
+
+extern "C" {
+
+/* nearestCellBodyCoordToPoint, distances.hpp:92-105. */
+double oracle_nearest_cell_body_coord(const double* r, const double* n, double half_l,
+                                      const double* q)
+{
+    double s = ((q[0] - r[0]) * n[0] + (q[1] - r[1]) * n[1]) + (q[2] - r[2]) * n[2];
+    if (std::fabs(s) <= half_l) return s;
+    else if (s > half_l) return half_l;
+    else return -half_l;
+}
+
+/* distBetweenCells, distances.hpp:206-357. d runs from cell 1 to cell 2. */
+void oracle_dist_between_cells(const double* r1, const double* n1, double half_l1,
+                               const double* r2, const double* n2, double half_l2,
+                               double* d, double* s_out, double* t_out)
+{
+    double s = 0, t = 0;
+    double cos_theta = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
+    if (cos_theta >= 0.9999 || cos_theta <= -0.9999) {
+        // Near-parallel branch, distances.hpp:228-337.
+        double p1[3], q1[3], p2[3], q2[3];
+        for (int k = 0; k < 3; ++k) {
+            p1[k] = r1[k] - half_l1 * n1[k];
+            q1[k] = r1[k] + half_l1 * n1[k];
+            p2[k] = r2[k] - half_l2 * n2[k];
+            q2[k] = r2[k] + half_l2 * n2[k];
+        }
+        double s_p1 = oracle_nearest_cell_body_coord(r2, n2, half_l2, p1);
+        double s_q1 = oracle_nearest_cell_body_coord(r2, n2, half_l2, q1);
+        double dp1[3], dq1[3];
+        for (int k = 0; k < 3; ++k) {
+            dp1[k] = (r2[k] + s_p1 * n2[k]) - p1[k];
+            dq1[k] = (r2[k] + s_q1 * n2[k]) - q1[k];
+        }
+        double dist_p1 = norm3(dp1[0], dp1[1], dp1[2]);
+        double dist_q1 = norm3(dq1[0], dq1[1], dq1[2]);
+        if (s_p1 == s_q1) {
+            if (dist_p1 < dist_q1) {
+                s = -half_l1; t = s_p1;
+                d[0] = dp1[0]; d[1] = dp1[1]; d[2] = dp1[2];
+            } else {
+                s = half_l1; t = s_q1;
+                d[0] = dq1[0]; d[1] = dq1[1]; d[2] = dq1[2];
+            }
+        } else {
+            double s_p2 = oracle_nearest_cell_body_coord(r1, n1, half_l1, p2);
+            double s_q2 = oracle_nearest_cell_body_coord(r1, n1, half_l1, q2);
+            double dp2[3], dq2[3];
+            for (int k = 0; k < 3; ++k) {
+                dp2[k] = (r1[k] + s_p2 * n1[k]) - p2[k];
+                dq2[k] = (r1[k] + s_q2 * n1[k]) - q2[k];
+            }
+            double dists[4] = {dist_p1, dist_q1, norm3(dp2[0], dp2[1], dp2[2]),
+                               norm3(dq2[0], dq2[1], dq2[2])};
+            // std::sort of 4 (distance, index) pairs keyed on distance
+            // (distances.hpp:274-286); libstdc++ sorts n < 16 by insertion
+            // sort, which is stable, so ties keep index order.
+            int idx[4] = {0, 1, 2, 3};
+            for (int a = 1; a < 4; ++a) {
+                int v = idx[a];
+                int b = a - 1;
+                while (b >= 0 && dists[v] < dists[idx[b]]) { idx[b + 1] = idx[b]; --b; }
+                idx[b + 1] = v;
+            }
+            bool has[4] = {false, false, false, false};
+            has[idx[0]] = true; has[idx[1]] = true;
+            if (has[0]) {
+                if (has[1]) {
+                    s = 0.0;
+                    t = oracle_nearest_cell_body_coord(r2, n2, half_l2, r1);
+                    for (int k = 0; k < 3; ++k) d[k] = (r2[k] + t * n2[k]) - r1[k];
+                } else {
+                    s = has[2] ? (-half_l1 + s_p2) / 2 : (-half_l1 + s_q2) / 2;
+                    double c[3] = {r1[0] + s * n1[0], r1[1] + s * n1[1], r1[2] + s * n1[2]};
+                    t = oracle_nearest_cell_body_coord(r2, n2, half_l2, c);
+                    for (int k = 0; k < 3; ++k) d[k] = ((r2[k] + t * n2[k]) - r1[k]) - s * n1[k];
+                }
+            } else if (has[1]) {
+                s = has[2] ? (half_l1 + s_p2) / 2 : (half_l1 + s_q2) / 2;
+                double c[3] = {r1[0] + s * n1[0], r1[1] + s * n1[1], r1[2] + s * n1[2]};
+                t = oracle_nearest_cell_body_coord(r2, n2, half_l2, c);
+                for (int k = 0; k < 3; ++k) d[k] = ((r2[k] + t * n2[k]) - r1[k]) - s * n1[k];
+            } else {
+                t = 0.0;
+                s = oracle_nearest_cell_body_coord(r1, n1, half_l1, r2);
+                for (int k = 0; k < 3; ++k) d[k] = (r2[k] - r1[k]) - s * n1[k];
+            }
+        }
+    } else {
+        // Skew branch (distances.hpp:338-345): closest points of the two
+        // centerline segments. Minimise |r12 + t n2 - s n1|^2 over the box
+        // [-hl1, hl1] x [-hl2, hl2]: solve the 2x2 normal equations, clamp s,
+        // take the optimal t for that s, and if t had to be clamped take the
+        // optimal s for the clamped t.
+        double r12[3] = {r2[0] - r1[0], r2[1] - r1[1], r2[2] - r1[2]};
+        double a = (n1[0] * n1[0] + n1[1] * n1[1]) + n1[2] * n1[2];
+        double b = (n1[0] * n2[0] + n1[1] * n2[1]) + n1[2] * n2[2];
+        double c = (n2[0] * n2[0] + n2[1] * n2[1]) + n2[2] * n2[2];
+        double e = (n1[0] * r12[0] + n1[1] * r12[1]) + n1[2] * r12[2];
+        double f = (n2[0] * r12[0] + n2[1] * r12[1]) + n2[2] * r12[2];
+        double det = a * c - b * b;
+        s = (e * c - b * f) / det;
+        s = std::min(std::max(s, -half_l1), half_l1);
+        t = (b * s - f) / c;
+        if (t < -half_l2 || t > half_l2) {
+            t = std::min(std::max(t, -half_l2), half_l2);
+            s = (e + b * t) / a;
+            s = std::min(std::max(s, -half_l1), half_l1);
+        }
+        for (int k = 0; k < 3; ++k) d[k] = ((r2[k] + t * n2[k]) - r1[k]) - s * n1[k];
+    }
+    *s_out = s;
+    *t_out = t;
+}
+
+static inline void cell_geom(const double* cells, int ld, int i, double* r, double* n, double* hl)
+{
+    r[0] = cell(cells, ld, i, C_RX); r[1] = cell(cells, ld, i, C_RY); r[2] = cell(cells, ld, i, C_RZ);
+    n[0] = cell(cells, ld, i, C_NX); n[1] = cell(cells, ld, i, C_NY); n[2] = cell(cells, ld, i, C_NZ);
+    *hl = cell(cells, ld, i, C_HALF_L);
+}
+
+/* getCellNeighbors, mechanics.hpp:520-593: serial all-pairs loop over j < i
+ * with the center-distance prefilter (:559) and |d| < threshold (:580). */
+int64_t oracle_get_cell_neighbors(const double* cells, int n, int ld, double thr,
+                                  double R, double Ldiv, double* out, int64_t ld_out)
+{
+    if (n <= 1) return 0;
+    const double pre = thr + Ldiv + 2 * R;
+    int64_t idx = 0;
+    for (int i = 0; i < n; ++i) {
+        double ri[3], ni[3], hli;
+        cell_geom(cells, ld, i, ri, ni, &hli);
+        for (int j = 0; j < i; ++j) {
+            double rj[3], nj[3], hlj;
+            cell_geom(cells, ld, j, rj, nj, &hlj);
+            double dist_rij = norm3(ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]);
+            if (dist_rij < pre) {
+                double d[3], si, sj;
+                oracle_dist_between_cells(ri, ni, hli, rj, nj, hlj, d, &si, &sj);
+                if (norm3(d[0], d[1], d[2]) < thr) {
+                    if (idx < ld_out) {
+                        out[0 * ld_out + idx] = i; out[1 * ld_out + idx] = j;
+                        out[2 * ld_out + idx] = d[0]; out[3 * ld_out + idx] = d[1];
+                        out[4 * ld_out + idx] = d[2]; out[5 * ld_out + idx] = si;
+                        out[6 * ld_out + idx] = sj;
+                    }
+                    idx++;
+                }
+            }
+        }
+    }
+    return idx;
+}
+
+/* Same list as oracle_get_cell_neighbors, found with a uniform grid and
+ * OpenMP (context timing only; not the reference's algorithm). A candidate is
+ * dropped without the segment test only when |r_i - r_j| exceeds
+ * thr + hl_i + hl_j by a margin, which bounds |d| >= thr for any pair of
+ * centerline points, so the list is identical. */
+int64_t oracle_get_cell_neighbors_binned(const double* cells, int n, int ld, double thr,
+                                         double R, double Ldiv, double* out,
+                                         int64_t ld_out, int nthreads)
+{
+    if (n <= 1) return 0;
+    const double pre = thr + Ldiv + 2 * R;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, hlmax = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            double v = cell(cells, ld, i, C_RX + k);
+            lo[k] = std::min(lo[k], v); hi[k] = std::max(hi[k], v);
+        }
+        hlmax = std::max(hlmax, cell(cells, ld, i, C_HALF_L));
+    }
+    double cut = std::min(pre, thr + 2 * hlmax) * (1 + 1e-9) + 1e-9;
+    int dims[3];
+    double edge = cut;
+    for (;;) {
+        int64_t tot = 1;
+        for (int k = 0; k < 3; ++k) { dims[k] = (int)((hi[k] - lo[k]) / edge) + 1; tot *= dims[k]; }
+        if (tot <= 4 * (int64_t)n + 64) break;
+        edge *= 1.25;
+    }
+    int64_t nb = (int64_t)dims[0] * dims[1] * dims[2];
+    std::vector<int> key(n), start(nb + 1, 0), order(n);
+    for (int i = 0; i < n; ++i) {
+        int b[3];
+        for (int k = 0; k < 3; ++k)
+            b[k] = std::min(dims[k] - 1, (int)((cell(cells, ld, i, C_RX + k) - lo[k]) / edge));
+        key[i] = (b[2] * dims[1] + b[1]) * dims[0] + b[0];
+        start[key[i] + 1]++;
+    }
+    for (int64_t b = 0; b < nb; ++b) start[b + 1] += start[b];
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < n; ++i) order[fill[key[i]]++] = i;
+    }
+    std::vector<std::vector<double>> rows(n);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+        double ri[3], ni[3], hli;
+        cell_geom(cells, ld, i, ri, ni, &hli);
+        int b0 = key[i] % dims[0], b1 = (key[i] / dims[0]) % dims[1], b2 = key[i] / (dims[0] * dims[1]);
+        std::vector<std::pair<int, int>> cand;
+        for (int z = std::max(0, b2 - 1); z <= std::min(dims[2] - 1, b2 + 1); ++z)
+            for (int y = std::max(0, b1 - 1); y <= std::min(dims[1] - 1, b1 + 1); ++y)
+                for (int x = std::max(0, b0 - 1); x <= std::min(dims[0] - 1, b0 + 1); ++x) {
+                    int bb = (z * dims[1] + y) * dims[0] + x;
+                    for (int q = start[bb]; q < start[bb + 1]; ++q) {
+                        int j = order[q];
+                        if (j < i) cand.push_back({j, 0});
+                    }
+                }
+        std::sort(cand.begin(), cand.end());
+        std::vector<double>& mine = rows[i];
+        for (auto& cj : cand) {
+            int j = cj.first;
+            double rj[3], nj[3], hlj;
+            cell_geom(cells, ld, j, rj, nj, &hlj);
+            double dx = ri[0] - rj[0], dy = ri[1] - rj[1], dz = ri[2] - rj[2];
+            double r2 = (dx * dx + dy * dy) + dz * dz;
+            double bound = (thr + hli + hlj) * (1 + 1e-9) + 1e-9;
+            if (r2 > bound * bound) continue;
+            if (!(std::sqrt(r2) < pre)) continue;
+            double d[3], si, sj;
+            oracle_dist_between_cells(ri, ni, hli, rj, nj, hlj, d, &si, &sj);
+            if (norm3(d[0], d[1], d[2]) < thr) {
+                double row[6] = {(double)j, d[0], d[1], d[2], si, sj};
+                mine.insert(mine.end(), row, row + 6);
+            }
+        }
+    }
+    int64_t idx = 0;
+    for (int i = 0; i < n; ++i) {
+        for (size_t q = 0; q < rows[i].size(); q += 6) {
+            if (idx < ld_out) {
+                out[idx] = i;
+                for (int c = 0; c < 6; ++c) out[(c + 1) * ld_out + idx] = rows[i][q + c];
+            }
+            idx++;
+        }
+    }
+    return idx;
+}
+
+/* updateNeighborDistances, mechanics.hpp:604-648. */
+void oracle_update_neighbor_distances(const double* cells, int n, int ld,
+                                      double* nb, int64_t m, int64_t ld_nb)
+{
+    (void)n;
+    for (int64_t k = 0; k < m; ++k) {
+        int i = (int)nb[k], j = (int)nb[ld_nb + k];
+        double ri[3], ni[3], hli, rj[3], nj[3], hlj, d[3], si, sj;
+        cell_geom(cells, ld, i, ri, ni, &hli);
+        cell_geom(cells, ld, j, rj, nj, &hlj);
+        oracle_dist_between_cells(ri, ni, hli, rj, nj, hlj, d, &si, &sj);
+        nb[2 * ld_nb + k] = d[0]; nb[3 * ld_nb + k] = d[1]; nb[4 * ld_nb + k] = d[2];
+        nb[5 * ld_nb + k] = si; nb[6 * ld_nb + k] = sj;
+    }
+}
+
+/* jkrContactRadius (jkr.hpp:67-144) returns the larger real root of
+ * a^4 - 2 R delta a^2 - (4 pi R^2 gamma / E) a + R^2 delta^2 = 0. Writing the
+ * quartic as (a^2 - R delta)^2 = (4 pi R^2 gamma / E) a, the larger root lies
+ * on the branch a^2 - R delta = +R sqrt(4 pi gamma a / E), i.e. u = sqrt(a) is
+ * the largest positive root of F(u) = u^4 - p u - q with p = R sqrt(4 pi
+ * gamma / E), q = R delta. F is convex for u > 0, so Newton started above the
+ * root decreases monotonically onto it; the loop stops when an iterate no
+ * longer decreases. Returns NaN when no root exists (delta below pull-off). */
+double oracle_jkr_contact_radius(double delta, double Req, double E, double gamma, int* iters)
+{
+    double p = Req * std::sqrt(4 * kPi * gamma / E);
+    double q = Req * delta;
+    double u = std::max(std::cbrt(2 * p), std::sqrt(std::sqrt(std::max(2 * q, 0.0))));
+    int it = 0;
+    if (u > 0) {
+        for (; it < 200; ++it) {
+            double u2 = u * u;
+            double F = (u2 * u2 - p * u) - q;
+            double dF = (4.0 * u2) * u - p;
+            if (!(dF > 0)) break;
+            double un = u - F / dF;
+            if (!(un < u)) break;
+            u = un;
+        }
+    }
+    if (iters) *iters = it;
+    double u2 = u * u;
+    if ((u2 * u2 - p * u) - q > 1e-9 * std::max(1.0, std::fabs(q))) return NAN;
+    return u2;
+}
+
+/* Force magnitude of forcesIsotropicJKRLagrange (adhesion.hpp:206-212):
+ * returns term1 - term2 = 4 E0 a^3 / (3 Req) - 4 sqrt(pi a^3 gamma E0). */
+double oracle_jkr_force_magnitude(double radius, double Req, double E0, double gamma)
+{
+    double a3 = radius * radius * radius;
+    double term1 = 4 * E0 * a3 / (3 * Req);
+    double term2 = 4 * std::sqrt(kPi * a3 * gamma * E0);
+    return term1 - term2;
+}
+
+/* Eigen's LinSpaced(n, lo, hi) for floating point: low half from lo upward,
+ * high half from hi downward, so both ends are exact. */
+void oracle_linspace(double lo, double hi, int n, double* out)
+{
+    if (n == 1) { out[0] = hi; return; }
+    double step = (hi - lo) / (n - 1);
+    int flip = (n - 1) / 2;
+    for (int i = 0; i < n; ++i)
+        out[i] = (i < flip) ? lo + i * step : hi - (n - 1 - i) * step;
+}
+
+/* nearestValue, utils.hpp:1169-1215. */
+int oracle_nearest_value(const double* values, int n, double x)
+{
+    if (x <= values[0]) return 0;
+    else if (x >= values[n - 1]) return n - 1;
+    int low = 0, high = n - 1, nearest = 0;
+    while (low <= high) {
+        int mid = (low + high) / 2;
+        if (values[mid] <= x && x < values[mid + 1]) { nearest = mid; break; }
+        else if (x >= values[mid + 1]) low = mid + 1;
+        else high = mid - 1;
+    }
+    if (x - values[nearest] < values[nearest + 1] - x) return nearest;
+    return nearest + 1;
+}
+
+/* Interpolated table radius, adhesion.hpp:312-328. */
+double oracle_jkr_table_radius_1d(const double* overlaps, const double* radii, int n, double delta)
+{
+    int i1 = oracle_nearest_value(overlaps, n, delta);
+    int i2 = (overlaps[i1] < delta ? i1 + 1 : i1 - 1);
+    if (i1 > i2) std::swap(i1, i2);
+    double d1 = overlaps[i1], d2 = overlaps[i2], r1 = radii[i1], r2 = radii[i2];
+    return (r1 * (d2 - delta) + r2 * (delta - d1)) / (d2 - d1);
+}
+
+/* Bilinear table radius, adhesion.hpp:452-490. radii row-major [nd][ng]. */
+double oracle_jkr_table_radius_2d(const double* overlaps, int nd, const double* gammas, int ng,
+                                  const double* radii, double delta, double gamma)
+{
+    int d1i = oracle_nearest_value(overlaps, nd, delta);
+    int g1i = oracle_nearest_value(gammas, ng, gamma);
+    int d2i = (overlaps[d1i] < delta ? d1i + 1 : d1i - 1);
+    if (d1i > d2i) std::swap(d1i, d2i);
+    int g2i = (gammas[g1i] < gamma ? g1i + 1 : g1i - 1);
+    if (g1i > g2i) std::swap(g1i, g2i);
+    double d1 = overlaps[d1i], d2 = overlaps[d2i], g1 = gammas[g1i], g2 = gammas[g2i];
+    double denom = (d2 - d1) * (g2 - g1);
+    double w11 = (d2 - delta) * (g2 - gamma) / denom;
+    double w12 = (d2 - delta) * (gamma - g1) / denom;
+    double w21 = (delta - d1) * (g2 - gamma) / denom;
+    double w22 = (delta - d1) * (gamma - g1) / denom;
+    double radius = 0;
+    radius += w11 * radii[d1i * ng + g1i];
+    radius += w12 * radii[d1i * ng + g2i];
+    radius += w21 * radii[d2i * ng + g1i];
+    radius += w22 * radii[d2i * ng + g2i];
+    return radius;
+}
+
+void oracle_set_jkr_table(const double* overlaps, int n_overlap, const double* gammas,
+                          int n_gamma, const double* radii)
+{
+    g_tab_overlaps.assign(overlaps, overlaps + n_overlap);
+    g_tab_gammas.assign(gammas ? gammas : overlaps, gammas ? gammas + n_gamma : overlaps);
+    g_tab_radii.assign(radii, radii + (int64_t)n_overlap * std::max(n_gamma, 1));
+}
+
+/* integrals.hpp:56-64 (phi). */
+static inline double phi_(double rz, double nz, double R, double s) { return R - rz - s * nz; }
+
+/* integral1, integrals.hpp:116-144. */
+double oracle_integral1(double rz, double nz, double R, double half_l, double g, double ss)
+{
+    if (ss > half_l) {
+        double o1 = std::pow(phi_(rz, nz, R, -half_l), g + 1);
+        double o2 = std::pow(phi_(rz, nz, R, half_l), g + 1);
+        return (o1 - o2) / (nz * (g + 1));
+    } else if (ss > -half_l) {
+        double o1 = std::pow(phi_(rz, nz, R, -half_l), g + 1);
+        return o1 / (nz * (g + 1));
+    }
+    return 0;
+}
+
+/* integral2, integrals.hpp:158-184. */
+double oracle_integral2(double rz, double nz, double R, double half_l, double g, double ss)
+{
+    double term1 = oracle_integral1(rz, nz, R, half_l, g + 1, ss);
+    double term2 = 0;
+    if (ss > half_l) {
+        double o1 = std::pow(phi_(rz, nz, R, -half_l), g + 1);
+        double o2 = std::pow(phi_(rz, nz, R, half_l), g + 1);
+        term2 = -half_l * (o1 + o2);
+    } else if (ss > -half_l) {
+        double o1 = std::pow(phi_(rz, nz, R, -half_l), g + 1);
+        term2 = -half_l * o1;
+    }
+    return (term1 + term2) / (nz * (g + 1));
+}
+
+/* integral3, integrals.hpp:198-224. */
+double oracle_integral3(double rz, double nz, double R, double half_l, double g, double ss)
+{
+    double term1 = 2 * oracle_integral2(rz, nz, R, half_l, g + 1, ss);
+    double term2 = 0;
+    if (ss > half_l) {
+        double o1 = std::pow(phi_(rz, nz, R, -half_l), g + 1);
+        double o2 = std::pow(phi_(rz, nz, R, half_l), g + 1);
+        term2 = half_l * half_l * (o1 - o2);
+    } else if (ss > -half_l) {
+        double o1 = std::pow(phi_(rz, nz, R, -half_l), g + 1);
+        term2 = half_l * half_l * o1;
+    }
+    return (term1 + term2) / (nz * (g + 1));
+}
+
+/* integral4..6, integrals.hpp:240-318. */
+double oracle_integral4(double nz, double half_l, double ss)
+{
+    (void)nz;
+    if (ss > half_l) return 2 * half_l;
+    else if (ss > -half_l) return ss + half_l;
+    return 0;
+}
+double oracle_integral5(double nz, double half_l, double ss)
+{
+    (void)nz;
+    if (-half_l < ss && ss <= half_l) return (ss * ss - half_l * half_l) / 2.0;
+    return 0.0;
+}
+double oracle_integral6(double nz, double half_l, double ss)
+{
+    (void)nz;
+    if (ss > half_l) return 2 * half_l * half_l * half_l / 3.0;
+    else if (ss > -half_l) return (ss * ss * ss + half_l * half_l * half_l) / 3.0;
+    return 0.0;
+}
+
+/* areaIntegrals, integrals.hpp:400-463. */
+void oracle_area_integrals(double rz, double nz, double R, double half_l, double ss, double* out)
+{
+    double int4 = oracle_integral4(nz, half_l, ss);
+    double int5 = oracle_integral5(nz, half_l, ss);
+    double int6 = oracle_integral6(nz, half_l, ss);
+    double int1a = oracle_integral1(rz, nz, R, half_l, 0.5, ss);
+    double int1b = oracle_integral1(rz, nz, R, half_l, 1.5, ss);
+    double int1c = oracle_integral1(rz, nz, R, half_l, 2.5, ss);
+    double sqR = std::sqrt(R);
+    double term1 = sqR * (1 - nz * nz) * int1a;
+    double term2 = kPi * R * nz * nz * int4;
+    double area1 = term1 + term2;
+    double overlap1 = 0, overlap2 = 0;
+    if (ss > half_l) {
+        overlap1 = phi_(rz, nz, R, -half_l);
+        overlap2 = phi_(rz, nz, R, half_l);
+    } else if (ss > -half_l) {
+        overlap1 = phi_(rz, nz, R, -half_l);
+    }
+    double o1p1 = std::pow(overlap1, 1.5), o2p1 = std::pow(overlap2, 1.5);
+    term2 = -half_l * (o1p1 + o2p1);
+    double int2a = (int1b + term2) / (1.5 * nz);
+    term1 = sqR * (1 - nz * nz) * int2a;
+    term2 = kPi * R * nz * nz * int5;
+    double area2 = term1 + term2;
+    double o1p2 = o1p1 * overlap1, o2p2 = o2p1 * overlap2;
+    term2 = -half_l * (o1p2 + o2p2);
+    double int2b = (int1c + term2) / (2.5 * nz);
+    term1 = 2 * int2b;
+    term2 = half_l * half_l * (o1p1 - o2p1);
+    double int3 = (term1 + term2) / (1.5 * nz);
+    term1 = sqR * (1 - nz * nz) * int3;
+    term2 = kPi * R * nz * nz * int6;
+    double area3 = term1 + term2;
+    out[0] = area1; out[1] = area2; out[2] = area3;
+}
+
+/* cellCellInteractionForces: without adhesion mechanics.hpp:1209-1350, with
+ * adhesion :1390-1659 (include_constraint = false, friction NONE). dEdq is
+ * n x 6 column-major, ld n, and is overwritten. Rows are accumulated in the
+ * order of the neighbor rows, as the reference does. */
+void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* nb,
+                             int64_t m, int64_t ld_nb, const bfm_params* p, double* dEdq)
+{
+    std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
+    const double R = p->R, Rcell = p->Rcell, E0 = p->E0;
+    const double Req = R / 2;
+    const double max_overlap = 2 * (R - Rcell);
+    const bool adhesion = p->adhesion_mode != BFM_ADHESION_NONE;
+    for (int64_t k = 0; k < m; ++k) {
+        int i = (int)nb[k], j = (int)nb[ld_nb + k];
+        double dij[3] = {nb[2 * ld_nb + k], nb[3 * ld_nb + k], nb[4 * ld_nb + k]};
+        double dist = norm3(dij[0], dij[1], dij[2]);
+        double overlap = 2 * R - dist;
+        if (!(overlap > 0)) continue;
+        double si = nb[5 * ld_nb + k], sj = nb[6 * ld_nb + k];
+        double dijn[3] = {dij[0] / dist, dij[1] / dist, dij[2] / dist};
+        double w[3];            // dE/dr_i of the main term; dE/dr_j = -w
+        bool hard = false;
+        double wh[3] = {0, 0, 0};
+        if (!adhesion) {
+            double mag;
+            if (overlap <= max_overlap) {
+                mag = p->repulsion_prefactors[0] * std::pow(overlap, 1.5);
+            } else {
+                mag = p->repulsion_prefactors[2];
+                mag += p->repulsion_prefactors[1] * std::pow(overlap - max_overlap, 1.5);
+            }
+            for (int c = 0; c < 3; ++c) w[c] = mag * dijn[c];
+        } else {
+            double gamma = std::min(cell(cells, ld, i, p->colidx_gamma),
+                                    cell(cells, ld, j, p->colidx_gamma));
+            if (gamma > 0) {
+                double delta = overlap;
+                if (max_overlap >= 0 && delta > max_overlap) delta = max_overlap;
+                double radius;
+                if (p->jkr_mode == BFM_JKR_TABLE_1D)
+                    radius = oracle_jkr_table_radius_1d(g_tab_overlaps.data(), g_tab_radii.data(),
+                                                        (int)g_tab_overlaps.size(), delta);
+                else if (p->jkr_mode == BFM_JKR_TABLE_2D)
+                    radius = oracle_jkr_table_radius_2d(
+                        g_tab_overlaps.data(), (int)g_tab_overlaps.size(), g_tab_gammas.data(),
+                        (int)g_tab_gammas.size(), g_tab_radii.data(), delta, gamma);
+                else
+                    radius = oracle_jkr_contact_radius(delta, Req, E0, gamma, nullptr);
+                double a3 = radius * radius * radius;
+                double term1 = 4 * E0 * a3 / (3 * Req);
+                double term2 = 4 * std::sqrt(kPi * a3 * gamma * E0);
+                // v = (term2 - term1) * d12n; dEdq(i) gets -v and -s v, dEdq(j)
+                // gets v and t v (adhesion.hpp:212-229). -v == (term1 - term2) * d12n.
+                for (int c = 0; c < 3; ++c) w[c] = (term1 - term2) * dijn[c];
+            } else {
+                double mag;
+                if (overlap <= 2 * (R - Rcell)) mag = p->repulsion_prefactors[0] * std::pow(overlap, 1.5);
+                else mag = p->repulsion_prefactors[2];
+                for (int c = 0; c < 3; ++c) w[c] = mag * dijn[c];
+            }
+            if (overlap > 2 * (R - Rcell)) {
+                double mag = p->repulsion_prefactors[1] * std::pow(overlap - 2 * (R - Rcell), 1.5);
+                for (int c = 0; c < 3; ++c) wh[c] = mag * dijn[c];
+                hard = true;
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            dEdq[c * n + i] += w[c];
+            dEdq[(c + 3) * n + i] += si * w[c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            dEdq[c * n + j] += -w[c];
+            dEdq[(c + 3) * n + j] += -sj * w[c];
+        }
+        if (hard) {
+            for (int c = 0; c < 3; ++c) {
+                dEdq[c * n + i] += wh[c];
+                dEdq[(c + 3) * n + i] += si * wh[c];
+            }
+            for (int c = 0; c < 3; ++c) {
+                dEdq[c * n + j] += -wh[c];
+                dEdq[(c + 3) * n + j] += -sj * wh[c];
+            }
+        }
+    }
+}
+
+/* cellSurfaceRepulsionForces, mechanics.hpp:673-780 (include_constraint =
+ * false). Writes columns 2 and 5 of dEdq (n x 6, zeroed here). */
+void oracle_surface_repulsion_forces(const double* cells, int n, int ld, const double* ss,
+                                     double R, double E0, const int* assume_2d, double* dEdq)
+{
+    std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
+    const double prefactor0 = 2 * E0;
+    const double prefactor1 = (8. / 3.) * E0 * std::pow(R, 0.5);
+    const double prefactor2 = 2 * E0 * std::pow(R, 0.5);
+    #pragma omp parallel for if(n >= 50)
+    for (int i = 0; i < n; ++i) {
+        double rz = cell(cells, ld, i, C_RZ), nz = cell(cells, ld, i, C_NZ);
+        double hl = cell(cells, ld, i, C_HALF_L);
+        if (assume_2d[i]) {
+            double ph = R - rz;
+            if (ph > 0) dEdq[2 * n + i] = -prefactor0 * ph * cell(cells, ld, i, C_L);
+        } else {
+            double nz2 = nz * nz;
+            double int1 = oracle_integral1(rz, nz, R, hl, 1.0, ss[i]);
+            double int2 = oracle_integral1(rz, nz, R, hl, 0.5, ss[i]);
+            dEdq[2 * n + i] = -prefactor0 * ((1.0 - nz2) * int1 + std::sqrt(R) * nz2 * int2);
+            double int3 = oracle_integral1(rz, nz, R, hl, 2.0, ss[i]);
+            double int4 = oracle_integral2(rz, nz, R, hl, 1.0, ss[i]);
+            double int5 = oracle_integral1(rz, nz, R, hl, 1.5, ss[i]);
+            double int6 = oracle_integral2(rz, nz, R, hl, 0.5, ss[i]);
+            double f5 = 0;
+            f5 -= prefactor0 * nz * int3;
+            f5 -= prefactor0 * (1 - nz2) * int4;
+            f5 += prefactor1 * nz * int5;
+            f5 -= prefactor2 * nz2 * int6;
+            dEdq[5 * n + i] = f5;
+        }
+    }
+}
+
+/* cellSurfaceAdhesionForces, mechanics.hpp:800-910 (include_constraint = false). */
+void oracle_surface_adhesion_forces(const double* cells, int n, int ld, const double* ss,
+                                    double R, const int* assume_2d, double* dEdq)
+{
+    std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
+    const double prefactor0 = std::pow(R, 0.5) / 2;
+    const double prefactor1 = 2 * kPi * R;
+    const double prefactor2 = 2 * std::pow(R, 0.5);
+    #pragma omp parallel for if(n >= 50)
+    for (int i = 0; i < n; ++i) {
+        double rz = cell(cells, ld, i, C_RZ), nz = cell(cells, ld, i, C_NZ);
+        double hl = cell(cells, ld, i, C_HALF_L), sigma0 = cell(cells, ld, i, C_SIGMA0);
+        if (assume_2d[i]) {
+            double ph = R - rz;
+            if (ph > 0) dEdq[2 * n + i] = sigma0 * prefactor0 * cell(cells, ld, i, C_L) / std::pow(ph, 0.5);
+        } else {
+            double nz2 = nz * nz;
+            double int1 = oracle_integral1(rz, nz, R, hl, -0.5, ss[i]);
+            double term2 = 0.0;
+            if (std::fabs(ss[i]) < hl) term2 = (prefactor1 / 2) * nz;
+            double f2 = prefactor0 * (1 - nz2) * int1 + term2;
+            f2 *= sigma0;
+            dEdq[2 * n + i] = f2;
+            double int2 = oracle_integral1(rz, nz, R, hl, 0.5, ss[i]);
+            double int3 = oracle_integral2(rz, nz, R, hl, -0.5, ss[i]);
+            double int4 = oracle_integral4(nz, hl, ss[i]);
+            double term4 = 0.0;
+            if (std::fabs(ss[i]) < hl) term4 = (prefactor1 / 2) * (R - rz);
+            double f5 = 0;
+            f5 += prefactor2 * nz * int2;
+            f5 += prefactor0 * (1 - nz2) * int3;
+            f5 -= prefactor1 * nz * int4;
+            f5 += term4;
+            f5 *= sigma0;
+            dEdq[5 * n + i] = f5;
+        }
+    }
+}
+
+static void sstar_all(const double* cells, int n, int ld, double R, const int* assume_2d, double* ss)
+{
+    for (int i = 0; i < n; ++i)
+        ss[i] = assume_2d[i] ? NAN : (R - cell(cells, ld, i, C_RZ)) / cell(cells, ld, i, C_NZ);
+}
+
+/* getConservativeForces, mechanics.hpp:1702-1769 (include_constraint = false):
+ * -F_cc - F_rep - F_adh, left to right. */
+void oracle_conservative_forces(const double* cells, int n, int ld, const double* nb,
+                                int64_t m, int64_t ld_nb, const int* assume_2d,
+                                const bfm_params* p, double* forces)
+{
+    std::vector<double> ss(n), cc(6 * (size_t)n), rep(6 * (size_t)n, 0.0), adh(6 * (size_t)n, 0.0);
+    sstar_all(cells, n, ld, p->R, assume_2d, ss.data());
+    oracle_cell_cell_forces(cells, n, ld, nb, m, ld_nb, p, cc.data());
+    if (!p->no_surface) {
+        oracle_surface_repulsion_forces(cells, n, ld, ss.data(), p->R, p->E0, assume_2d, rep.data());
+        oracle_surface_adhesion_forces(cells, n, ld, ss.data(), p->R, assume_2d, adh.data());
+    }
+    for (size_t q = 0; q < 6 * (size_t)n; ++q) forces[q] = (-cc[q] - rep[q]) - adh[q];
+}
+
+/* getVelocities, mechanics.hpp:2011-2347 (friction NONE). */
+void oracle_get_velocities(const double* cells, int n, int ld, const double* nb, int64_t m,
+                           int64_t ld_nb, const int* assume_2d, const double* noise,
+                           const bfm_params* p, double* vel)
+{
+    const double R = p->R, E0 = p->E0;
+    const double coeff = p->cell_surface_coulomb_coeff;
+    std::vector<double> ss(n), F(6 * (size_t)n);
+    sstar_all(cells, n, ld, R, assume_2d, ss.data());
+    oracle_conservative_forces(cells, n, ld, nb, m, ld_nb, assume_2d, p, F.data());
+    if (noise)
+        for (size_t q = 0; q < 6 * (size_t)n; ++q) F[q] += noise[q];
+    #pragma omp parallel for if(n >= 50)
+    for (int i = 0; i < n; ++i) {
+        double f[6];
+        for (int c = 0; c < 6; ++c) f[c] = F[(size_t)c * n + i];
+        double v[6];
+        const double rz = cell(cells, ld, i, C_RZ), nx = cell(cells, ld, i, C_NX);
+        const double ny = cell(cells, ld, i, C_NY), nz = cell(cells, ld, i, C_NZ);
+        const double l = cell(cells, ld, i, C_L), hl = cell(cells, ld, i, C_HALF_L);
+        const double eta0 = cell(cells, ld, i, C_ETA0);
+        bool contacting;
+        if (!assume_2d[i]) {
+            double z1 = rz - hl * nz, z2 = rz + hl * nz;
+            contacting = (z1 < R || z2 < R);
+        } else {
+            contacting = (rz < R);
+        }
+        if (p->no_surface || !contacting) {
+            double K = eta0 * l;
+            double L = K * l * l / 12;
+            double mult = -(nx * f[3] + ny * f[4] + nz * f[5]);
+            v[0] = f[0] / K; v[1] = f[1] / K; v[2] = f[2] / K;
+            v[3] = (f[3] + mult * nx) / L;
+            v[4] = (f[4] + mult * ny) / L;
+            v[5] = (f[5] + mult * nz) / L;
+        } else if (assume_2d[i]) {
+            double K, L;
+            double eta1 = cell(cells, ld, i, C_ETA1);
+            if (coeff == 0) {
+                double a = std::sqrt(R * (R - rz));
+                double eta_combined = eta0 + eta1 * a / R;
+                K = eta_combined * l;
+                L = K * l * l / 12;
+            } else {
+                double K_vis = eta0 * l;
+                double L_vis = K_vis * l * l / 12;
+                double a = std::sqrt(R * (R - rz));
+                double K_fric = eta1 * l * a / R;
+                double drx = cell(cells, ld, i, C_DRX), dry = cell(cells, ld, i, C_DRY);
+                double drxy = std::sqrt(drx * drx + dry * dry);
+                double friction_norm = K_fric * drxy;
+                double max_norm = coeff * (2 * E0 * (R - rz) * l);
+                if (friction_norm > max_norm) {
+                    double eta1_truncated = max_norm * R / (drxy * l * a);
+                    K_fric = eta1_truncated * l * a / R;
+                }
+                double L_fric = K_fric * l * l / 12;
+                K = K_vis + K_fric;
+                L = L_vis + L_fric;
+            }
+            double mult = -(nx * f[3] + ny * f[4]);
+            v[0] = f[0] / K;
+            v[1] = f[1] / K;
+            v[3] = (f[3] + mult * nx) / L;
+            v[4] = (f[4] + mult * ny) / L;
+            v[2] = f[2] / (eta0 * l);
+            v[5] = 12 * f[5] / (eta0 * l * l * l);
+        } else {
+            double eta1 = cell(cells, ld, i, C_ETA1);
+            double ai[3];
+            oracle_area_integrals(rz, nz, R, hl, ss[i], ai);
+            if (coeff > 0) {
+                double drx = cell(cells, ld, i, C_DRX), dry = cell(cells, ld, i, C_DRY);
+                double dnx = cell(cells, ld, i, C_DNX), dny = cell(cells, ld, i, C_DNY);
+                double drxy = std::sqrt(drx * drx + dry * dry);
+                double dnxy = std::sqrt(dnx * dnx + dny * dny);
+                double friction_norm = ai[0] * drxy;
+                friction_norm += ai[1] * dnxy;
+                friction_norm *= (eta1 / R);
+                double max_norm = (1 - nz * nz) * oracle_integral1(rz, nz, R, hl, 1, ss[i]);
+                max_norm += std::sqrt(R) * nz * nz * oracle_integral1(rz, nz, R, hl, 0.5, ss[i]);
+                max_norm *= (2 * E0 * coeff);
+                if (friction_norm > max_norm)
+                    eta1 = max_norm * R / (ai[0] * drxy + ai[1] * dnxy);
+            }
+            const double c1 = eta0 * l + (eta1 / R) * ai[0];
+            const double c2 = eta0 * l;
+            const double c3 = c2 * l * l / 12 + (eta1 / R) * ai[2];
+            const double c4 = c2 * l * l / 12;
+            const double c5 = (eta1 / R) * ai[1];
+            const double term1 = c3 - (c5 * c5 / c1);
+            const double w4 = f[3] - f[0] * (c5 / c1);
+            const double w5 = f[4] - f[1] * (c5 / c1);
+            const double nx2 = nx / term1, ny2 = ny / term1, nz2 = nz / c4;
+            const double alpha = 2 * (nx * nx2 + ny * ny2 + nz * nz2);
+            const double beta = -(w4 * nx2 + w5 * ny2 + f[5] * nz2);
+            const double lambda = beta / alpha;
+            v[5] = (f[5] + 2 * nz * lambda) / c4;
+            v[4] = (w5 + 2 * ny * lambda) / term1;
+            v[3] = (w4 + 2 * nx * lambda) / term1;
+            v[2] = f[2] / c2;
+            v[1] = (f[1] - c5 * v[4]) / c1;
+            v[0] = (f[0] - c5 * v[3]) / c1;
+        }
+        for (int c = 0; c < 6; ++c) vel[(size_t)c * n + i] = v[c];
+    }
+}
+
+/* normalizeOrientations, mechanics.hpp:1918-1959. */
+void oracle_normalize_orientations(double* cells, int n, int ld)
+{
+    for (int i = 0; i < n; ++i) {
+        double* nx = &cells[(int64_t)C_NX * ld + i];
+        double* ny = &cells[(int64_t)C_NY * ld + i];
+        double* nz = &cells[(int64_t)C_NZ * ld + i];
+        double nrm = norm3(*nx, *ny, *nz);
+        *nx /= nrm; *ny /= nrm; *nz /= nrm;
+        if (*nz < 0) { *nx *= -1; *ny *= -1; *nz *= -1; }
+    }
+}
+
+/* stepRungeKuttaAdaptive, mechanics.hpp:2406-2612. A is s x s row-major.
+ * cells_new has leading dimension n and the column count of params.ncols. */
+void oracle_step_rk(const double* A, const double* b, const double* bs, int s, double dt,
+                    const double* cells, int n, int ld, const double* nb, int64_t m,
+                    int64_t ld_nb, const double* noise, const bfm_params* p,
+                    double* cells_new, double* errors)
+{
+    const int ncols = p->ncols;
+    std::vector<int> assume_2d(n);
+    for (int i = 0; i < n; ++i) assume_2d[i] = (cell(cells, ld, i, C_NZ) < p->nz_threshold);
+    std::vector<std::vector<double>> k(s, std::vector<double>(6 * (size_t)n));
+    std::vector<double> ci((size_t)ncols * n);
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < n; ++i) ci[(size_t)c * n + i] = cell(cells, ld, i, c);
+    oracle_get_velocities(ci.data(), n, n, nb, m, ld_nb, assume_2d.data(), noise, p, k[0].data());
+    for (int st = 1; st < s; ++st) {
+        std::vector<double> mult(6 * (size_t)n, 0.0);
+        for (int j = 0; j < st; ++j)
+            for (size_t q = 0; q < 6 * (size_t)n; ++q) mult[q] += k[j][q] * A[st * s + j];
+        for (int c = 0; c < ncols; ++c)
+            for (int i = 0; i < n; ++i) ci[(size_t)c * n + i] = cell(cells, ld, i, c);
+        for (int c = 0; c < 6; ++c)
+            for (int i = 0; i < n; ++i) ci[(size_t)(c + 1) * n + i] += mult[(size_t)c * n + i] * dt;
+        oracle_normalize_orientations(ci.data(), n, n);
+        oracle_get_velocities(ci.data(), n, n, nb, m, ld_nb, assume_2d.data(), noise, p, k[st].data());
+    }
+    std::vector<double> v1(6 * (size_t)n, 0.0), v2(6 * (size_t)n, 0.0);
+    for (int st = 0; st < s; ++st)
+        for (size_t q = 0; q < 6 * (size_t)n; ++q) {
+            v1[q] += k[st][q] * b[st];
+            v2[q] += k[st][q] * bs[st];
+        }
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < n; ++i) cells_new[(size_t)c * n + i] = cell(cells, ld, i, c);
+    for (size_t q = 0; q < 6 * (size_t)n; ++q) {
+        double d1 = v1[q] * dt, d2 = v2[q] * dt;
+        cells_new[(size_t)n + q] += d1;
+        errors[q] = d1 - d2;
+        cells_new[(size_t)7 * n + q] = v1[q];
+    }
+    oracle_normalize_orientations(cells_new, n, n);
+}
+
+/* The controller's error measure, simulation.hpp:1688-1695:
+ * max(max_k errors_k / (tol * (1 + |y_k|)), min_error), a signed maximum. */
+double oracle_max_scaled_error(const double* cells, int n, int ld, const double* errors,
+                               double tol, double min_error)
+{
+    double mx = -INFINITY;
+    for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < n; ++i) {
+            double z = 1.0 + std::fabs(cell(cells, ld, i, c + 1));
+            double v = errors[(size_t)c * n + i] / (tol * z);
+            if (v > mx) mx = v;
+        }
+    return std::max(mx, min_error);
+}
+
+/* Step-size controller, simulation.hpp:1688-1769, driven by a script of
+ * errors instead of real steps. */
+int oracle_controller_trace(double dt, double error_order, int max_tries,
+                            double min_stepsize, double max_stepsize,
+                            const double* err_seq, int n_err, double* dts)
+{
+    int k = 0, nd = 0;
+    dts[nd++] = dt;
+    double max_error = err_seq[std::min(k++, n_err - 1)];
+    bool exceeded = max_error > 1.0;
+    double factor = 0.9 * std::pow(1.0 / max_error, 1.0 / (error_order + 1));
+    if (factor >= 10) factor = 10;
+    else if (factor < 0.2) factor = 0.2;
+    int j = 0;
+    while (exceeded && j < max_tries) {
+        dts[nd++] = dt * factor;
+        max_error = err_seq[std::min(k++, n_err - 1)];
+        exceeded = max_error > 1.0;
+        factor *= 0.9 * std::pow(1.0 / max_error, 1.0 / (error_order + 1));
+        if (factor >= 10) { factor = 10; break; }
+        else if (factor < 0.2) { factor = 0.2; break; }
+        j++;
+    }
+    if (dt * factor < min_stepsize) factor = min_stepsize / dt;
+    else if (dt * factor > max_stepsize) factor = max_stepsize / dt;
+    dt *= factor;
+    dts[nd++] = dt;
+    return nd;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+/*
+ * oracle.h -- CPU oracle for the mechanical step (TEST INFRASTRUCTURE ONLY).
+ *
+ * A from-scratch C++ restatement of the reference's hot-path algorithms, used
+ * by tests/ as the checker of the HIP path, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg. Nothing in the product links or calls it.
+ *
+ * Each function cites the reference file:line it follows (paths relative to
+ * ABM-3d/ABM-3d-cpp/). Arrays use the layouts documented in include/bfm.h.
+ *
+ * Parity pins (tests/): the 16 distBetweenCells known answers
+ * (test/src/testDistBetweenCells.cpp:143-506), brute-force minimisation of the
+ * segment distance, JKR roots against mpmath 50-digit fixtures and the
+ * reference's closed forms (test/src/testJKR.cpp, testContactArea.cpp:102-162),
+ * surface integrals against quadrature, surface forces against finite
+ * differences of the surface energy, the velocity solve against the residual
+ * of the 7x7 system, the RK step against tableau order tests, and the
+ * controller against a scripted trace.
+ */
+#ifndef BFM_ORACLE_H
+#define BFM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/bfm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+double oracle_nearest_cell_body_coord(const double* r, const double* n, double half_l,
+                                      const double* q);
+void oracle_dist_between_cells(const double* r1, const double* n1, double half_l1,
+                               const double* r2, const double* n2, double half_l2,
+                               double* d, double* s, double* t);
+int64_t oracle_get_cell_neighbors(const double* cells, int n, int ld, double thr,
+                                  double R, double Ldiv, double* out, int64_t ld_out);
+int64_t oracle_get_cell_neighbors_binned(const double* cells, int n, int ld, double thr,
+                                         double R, double Ldiv, double* out,
+                                         int64_t ld_out, int nthreads);
+void oracle_update_neighbor_distances(const double* cells, int n, int ld,
+                                      double* neighbors, int64_t m, int64_t ld_nb);
+
+/* JKR: larger real root of a^4 - 2 R d a^2 - (4 pi R^2 g / E) a + R^2 d^2. */
+double oracle_jkr_contact_radius(double delta, double Req, double E, double gamma,
+                                 int* iters);
+double oracle_jkr_force_magnitude(double radius, double Req, double E0, double gamma);
+void oracle_linspace(double lo, double hi, int n, double* out);
+int oracle_nearest_value(const double* values, int n, double x);
+double oracle_jkr_table_radius_1d(const double* overlaps, const double* radii, int n,
+                                  double delta);
+double oracle_jkr_table_radius_2d(const double* overlaps, int nd, const double* gammas,
+                                  int ng, const double* radii, double delta, double gamma);
+void oracle_set_jkr_table(const double* overlaps, int n_overlap, const double* gammas,
+                          int n_gamma, const double* radii);
+
+/* Surface integrals, integrals.hpp:35-463. */
+double oracle_integral1(double rz, double nz, double R, double half_l, double g, double ss);
+double oracle_integral2(double rz, double nz, double R, double half_l, double g, double ss);
+double oracle_integral3(double rz, double nz, double R, double half_l, double g, double ss);
+double oracle_integral4(double nz, double half_l, double ss);
+double oracle_integral5(double nz, double half_l, double ss);
+double oracle_integral6(double nz, double half_l, double ss);
+void oracle_area_integrals(double rz, double nz, double R, double half_l, double ss,
+                           double* out3);
+
+void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* neighbors,
+                             int64_t m, int64_t ld_nb, const bfm_params* p, double* dEdq);
+void oracle_surface_repulsion_forces(const double* cells, int n, int ld, const double* ss,
+                                     double R, double E0, const int* assume_2d, double* dEdq);
+void oracle_surface_adhesion_forces(const double* cells, int n, int ld, const double* ss,
+                                    double R, const int* assume_2d, double* dEdq);
+void oracle_conservative_forces(const double* cells, int n, int ld, const double* neighbors,
+                                int64_t m, int64_t ld_nb, const int* assume_2d,
+                                const bfm_params* p, double* forces);
+void oracle_get_velocities(const double* cells, int n, int ld, const double* neighbors,
+                           int64_t m, int64_t ld_nb, const int* assume_2d,
+                           const double* noise, const bfm_params* p, double* vel);
+void oracle_normalize_orientations(double* cells, int n, int ld);
+void oracle_step_rk(const double* A, const double* b, const double* bs, int s, double dt,
+                    const double* cells, int n, int ld, const double* neighbors, int64_t m,
+                    int64_t ld_nb, const double* noise, const bfm_params* p,
+                    double* cells_new, double* errors);
+double oracle_max_scaled_error(const double* cells, int n, int ld, const double* errors,
+                               double tol, double min_error);
+/* One pass of the step-size controller (simulation.hpp:1688-1769) on a scripted
+ * sequence of errors: err_seq[k] is the error the k-th attempted step returns.
+ * Writes the dt of every attempted step (initial, retries, final redo) into
+ * dts and returns their number. */
+int oracle_controller_trace(double dt, double error_order, int max_tries,
+                            double min_stepsize, double max_stepsize,
+                            const double* err_seq, int n_err, double* dts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,265 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker of the HIP path. Array layouts follow include/bfm.h
+(column-major numpy arrays).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run make -C oracle")
+        L = C.CDLL(path)
+        d, i64 = C.c_double, C.c_int64
+        L.oracle_nearest_cell_body_coord.argtypes = [_dp, _dp, d, _dp]
+        L.oracle_nearest_cell_body_coord.restype = d
+        L.oracle_dist_between_cells.argtypes = [_dp, _dp, d, _dp, _dp, d, _dp, _dp, _dp]
+        L.oracle_dist_between_cells.restype = None
+        for nm in ("oracle_get_cell_neighbors",):
+            getattr(L, nm).argtypes = [_dp, C.c_int, C.c_int, d, d, d, _dp, i64]
+            getattr(L, nm).restype = i64
+        L.oracle_get_cell_neighbors_binned.argtypes = [_dp, C.c_int, C.c_int, d, d, d, _dp, i64,
+                                                       C.c_int]
+        L.oracle_get_cell_neighbors_binned.restype = i64
+        L.oracle_update_neighbor_distances.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64]
+        L.oracle_update_neighbor_distances.restype = None
+        L.oracle_jkr_contact_radius.argtypes = [d, d, d, d, _ip]
+        L.oracle_jkr_contact_radius.restype = d
+        L.oracle_jkr_force_magnitude.argtypes = [d, d, d, d]
+        L.oracle_jkr_force_magnitude.restype = d
+        L.oracle_linspace.argtypes = [d, d, C.c_int, _dp]
+        L.oracle_linspace.restype = None
+        L.oracle_nearest_value.argtypes = [_dp, C.c_int, d]
+        L.oracle_nearest_value.restype = C.c_int
+        L.oracle_jkr_table_radius_1d.argtypes = [_dp, _dp, C.c_int, d]
+        L.oracle_jkr_table_radius_1d.restype = d
+        L.oracle_jkr_table_radius_2d.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, d, d]
+        L.oracle_jkr_table_radius_2d.restype = d
+        L.oracle_set_jkr_table.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp]
+        L.oracle_set_jkr_table.restype = None
+        for nm in ("oracle_integral1", "oracle_integral2", "oracle_integral3"):
+            getattr(L, nm).argtypes = [d] * 6
+            getattr(L, nm).restype = d
+        for nm in ("oracle_integral4", "oracle_integral5", "oracle_integral6"):
+            getattr(L, nm).argtypes = [d] * 3
+            getattr(L, nm).restype = d
+        L.oracle_area_integrals.argtypes = [d, d, d, d, d, _dp]
+        L.oracle_area_integrals.restype = None
+        vp = C.c_void_p
+        L.oracle_cell_cell_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, vp, _dp]
+        L.oracle_surface_repulsion_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, d, d, _ip, _dp]
+        L.oracle_surface_adhesion_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, d, _ip, _dp]
+        L.oracle_conservative_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, _ip, vp,
+                                                 _dp]
+        L.oracle_get_velocities.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, _ip, _dp, vp,
+                                            _dp]
+        L.oracle_normalize_orientations.argtypes = [_dp, C.c_int, C.c_int]
+        L.oracle_step_rk.argtypes = [_dp, _dp, _dp, C.c_int, d, _dp, C.c_int, C.c_int, _dp, i64,
+                                     i64, _dp, vp, _dp, _dp]
+        L.oracle_max_scaled_error.argtypes = [_dp, C.c_int, C.c_int, _dp, d, d]
+        L.oracle_max_scaled_error.restype = d
+        L.oracle_controller_trace.argtypes = [d, d, C.c_int, d, d, _dp, C.c_int, _dp]
+        L.oracle_controller_trace.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _f(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _v3(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def nearest_cell_body_coord(r, n, half_l, q):
+    return lib().oracle_nearest_cell_body_coord(_p(_v3(r)), _p(_v3(n)), half_l, _p(_v3(q)))
+
+
+def dist_between_cells(r1, n1, hl1, r2, n2, hl2):
+    d = np.zeros(3)
+    s, t = C.c_double(), C.c_double()
+    lib().oracle_dist_between_cells(_p(_v3(r1)), _p(_v3(n1)), hl1, _p(_v3(r2)), _p(_v3(n2)), hl2,
+                                    _p(d), C.byref(s), C.byref(t))
+    return d, s.value, t.value
+
+
+def get_cell_neighbors(cells, thr, R, Ldiv, binned=False, nthreads=0):
+    cells = _f(cells)
+    n = cells.shape[0]
+    cap = max(16 * n, 16)
+    while True:
+        out = np.zeros((cap, 7), order="F")
+        if binned:
+            m = lib().oracle_get_cell_neighbors_binned(_p(cells), n, n, thr, R, Ldiv, _p(out), cap,
+                                                       nthreads)
+        else:
+            m = lib().oracle_get_cell_neighbors(_p(cells), n, n, thr, R, Ldiv, _p(out), cap)
+        if m <= cap:
+            return np.asfortranarray(out[:m])
+        cap = m
+
+
+def update_neighbor_distances(cells, nb):
+    cells = _f(cells)
+    lib().oracle_update_neighbor_distances(_p(cells), cells.shape[0], cells.shape[0], _p(nb),
+                                           nb.shape[0], max(nb.shape[0], 1))
+
+
+def jkr_contact_radius(delta, Req, E, gamma):
+    it = C.c_int()
+    a = lib().oracle_jkr_contact_radius(delta, Req, E, gamma, C.byref(it))
+    return a, it.value
+
+
+def jkr_force_magnitude(radius, Req, E0, gamma):
+    return lib().oracle_jkr_force_magnitude(radius, Req, E0, gamma)
+
+
+def linspace(lo, hi, n):
+    out = np.zeros(n)
+    lib().oracle_linspace(lo, hi, n, _p(out))
+    return out
+
+
+def nearest_value(values, x):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    return lib().oracle_nearest_value(_p(v), len(v), x)
+
+
+def set_jkr_table(overlaps, radii, gammas=None):
+    ov = np.ascontiguousarray(overlaps, dtype=np.float64)
+    rad = np.ascontiguousarray(radii, dtype=np.float64)
+    g = None if gammas is None else np.ascontiguousarray(gammas, dtype=np.float64)
+    lib().oracle_set_jkr_table(_p(ov), len(ov), _p(g), 0 if g is None else len(g), _p(rad))
+
+
+def table_radius_1d(overlaps, radii, delta):
+    ov = np.ascontiguousarray(overlaps, dtype=np.float64)
+    rad = np.ascontiguousarray(radii, dtype=np.float64)
+    return lib().oracle_jkr_table_radius_1d(_p(ov), _p(rad), len(ov), delta)
+
+
+def table_radius_2d(overlaps, gammas, radii, delta, gamma):
+    ov = np.ascontiguousarray(overlaps, dtype=np.float64)
+    g = np.ascontiguousarray(gammas, dtype=np.float64)
+    rad = np.ascontiguousarray(radii, dtype=np.float64)
+    return lib().oracle_jkr_table_radius_2d(_p(ov), len(ov), _p(g), len(g), _p(rad), delta, gamma)
+
+
+def integral(k, *args):
+    return getattr(lib(), f"oracle_integral{k}")(*args)
+
+
+def area_integrals(rz, nz, R, hl, ss):
+    out = np.zeros(3)
+    lib().oracle_area_integrals(rz, nz, R, hl, ss, _p(out))
+    return out
+
+
+def cell_cell_forces(cells, nb, params):
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_cell_cell_forces(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                                  C.byref(params), _p(out))
+    return out
+
+
+def _ss(cells, R, a2d):
+    rz, nz = cells[:, 3], cells[:, 6]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ss = (R - rz) / nz
+    ss[np.asarray(a2d) != 0] = np.nan
+    return np.ascontiguousarray(ss)
+
+
+def surface_repulsion_forces(cells, R, E0, assume_2d):
+    cells = _f(cells)
+    n = cells.shape[0]
+    a2d = np.ascontiguousarray(assume_2d, dtype=np.int32)
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_surface_repulsion_forces(_p(cells), n, n, _p(_ss(cells, R, a2d)), R, E0,
+                                          a2d.ctypes.data_as(_ip), _p(out))
+    return out
+
+
+def surface_adhesion_forces(cells, R, assume_2d):
+    cells = _f(cells)
+    n = cells.shape[0]
+    a2d = np.ascontiguousarray(assume_2d, dtype=np.int32)
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_surface_adhesion_forces(_p(cells), n, n, _p(_ss(cells, R, a2d)), R,
+                                         a2d.ctypes.data_as(_ip), _p(out))
+    return out
+
+
+def conservative_forces(cells, nb, assume_2d, params):
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    a2d = np.ascontiguousarray(assume_2d, dtype=np.int32)
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_conservative_forces(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                                     a2d.ctypes.data_as(_ip), C.byref(params), _p(out))
+    return out
+
+
+def get_velocities(cells, nb, assume_2d, noise, params):
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    a2d = np.ascontiguousarray(assume_2d, dtype=np.int32)
+    nz = None if noise is None else _f(noise)
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_get_velocities(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                                a2d.ctypes.data_as(_ip), _p(nz), C.byref(params), _p(out))
+    return out
+
+
+def normalize_orientations(cells):
+    cells = _f(cells).copy(order="F")
+    lib().oracle_normalize_orientations(_p(cells), cells.shape[0], cells.shape[0])
+    return cells
+
+
+def step_rk(A, b, bs, dt, cells, nb, params, noise=None):
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    bs = np.ascontiguousarray(bs, dtype=np.float64)
+    nz = None if noise is None else _f(noise)
+    out = np.zeros((n, cells.shape[1]), order="F")
+    err = np.zeros((n, 6), order="F")
+    lib().oracle_step_rk(_p(A), _p(b), _p(bs), len(b), dt, _p(cells), n, n, _p(nb), nb.shape[0],
+                         max(nb.shape[0], 1), _p(nz), C.byref(params), _p(out), _p(err))
+    return out, err
+
+
+def max_scaled_error(cells, errors, tol, min_error=1e-30):
+    cells, errors = _f(cells), _f(errors)
+    return lib().oracle_max_scaled_error(_p(cells), cells.shape[0], cells.shape[0], _p(errors),
+                                         tol, min_error)
+
+
+def controller_trace(dt, error_order, max_tries, min_step, max_step, errs):
+    e = np.ascontiguousarray(errs, dtype=np.float64)
+    out = np.zeros(max_tries + 4)
+    k = lib().oracle_controller_trace(dt, error_order, max_tries, min_step, max_step, _p(e), len(e),
+                                      _p(out))
+    return out[:k]
