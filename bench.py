@@ -1,0 +1,197 @@
+"""Benchmark: RK-stage cell-updates/s on the 200,000-cell 3-D dome snapshot (FP64).
+
+One step = one full RK stage of the mechanical step on device-resident input:
+binning, sort, neighbor list (segment-segment tests), Hertz + JKR force and
+torque, surface forces and the velocity solve (bfm_dev_stage). value =
+cells / stage time over the timed steps.
+
+roofline: the dominant kernel of the stage (per-kernel HIP events on the
+launch stream, measured in this run); cpu_baseline: the CPU oracle (the
+reference algorithms restated: serial O(N^2) neighbor search, serial pair
+loop, OpenMP at the reference's per-cell sites) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "biofilm-abms_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations per unit of work of the candidate kernel, counted from
+# bfm_device.hpp / k_candidates (adds, muls, divides and square roots each 1):
+SCREEN_FLOPS = 14      # r_ij^2, bound, comparison per screened candidate with j < i
+TEST_FLOPS = 80        # segment-segment closest points + |d| per queued pair
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=200000)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--jkr", default="exact", choices=["exact", "table", "surrogate"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU work for the sampled O(N^2) neighbor search")
+    ap.add_argument("--bin-scale", type=float, default=0.0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=-1)
+    return ap.parse_args()
+
+
+def cpu_baseline(cells, thr, R, Ldiv, params, budget):
+    """Times the oracle on the same snapshot: C1 = getVelocities on a prebuilt
+    list (all N cells, OpenMP at the reference sites), C2 = the serial O(N^2)
+    getCellNeighbors, sampled on a block of rows centred on N/2 (row i costs
+    O(i), so the block has the mean row cost) and scaled to N rows."""
+    import numpy as np
+    from oracle import oracle as orc
+
+    n = cells.shape[0]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    nb = orc.get_cell_neighbors(cells, thr, R, Ldiv, binned=True, nthreads=threads)
+    a2d = (cells[:, 6] < params.nz_threshold).astype(np.int32)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.get_velocities(cells, nb, a2d, None, params)
+        times.append(time.perf_counter() - t0)
+    c1 = float(np.median(times))
+    rows = 200
+    t0 = time.perf_counter()
+    orc.count_neighbor_rows(cells, thr, R, Ldiv, n // 2 - rows // 2, n // 2 + rows // 2)
+    t_probe = time.perf_counter() - t0
+    rows = int(min(n, max(200, rows * budget / max(t_probe, 1e-6))))
+    t0 = time.perf_counter()
+    orc.count_neighbor_rows(cells, thr, R, Ldiv, n // 2 - rows // 2, n // 2 - rows // 2 + rows)
+    t_rows = time.perf_counter() - t0
+    c2 = t_rows * n / rows
+    return dict(value=n / (c1 + c2), unit="cell-updates/s", cores=threads, kind="port",
+                sample=(f"oracle on the same {n}-cell snapshot: getVelocities on a prebuilt list "
+                        f"(all cells, {threads} OpenMP threads, median of 3: {c1*1e3:.1f} ms) + "
+                        f"serial O(N^2) getCellNeighbors timed on {rows} rows centred on N/2 "
+                        f"({t_rows:.1f} s) and scaled to N rows ({c2:.1f} s)"),
+                stage_only_value=n / c1, velocities_s=c1, neighbors_s=c2)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
+              "design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import bfm
+
+    n = args.cells
+    R, Rcell, Ldiv = 0.8, 0.5, 2.4
+    thr = 0.5 * (2 * R + Ldiv)
+    cells, dome_r = bfm.make_dome(n, seed=args.seed)
+    jkr_mode = {"exact": bfm.core.JKR_EXACT, "table": bfm.core.JKR_TABLE_1D,
+                "surrogate": bfm.core.JKR_SURROGATE}[args.jkr]
+    params = bfm.core.make_params(jkr_mode=jkr_mode)
+    ctx = bfm.Context(n, max_pairs=16 * n, params=params)
+    if args.bin_scale or args.block or args.variant >= 0:
+        ctx.set_tuning(args.bin_scale, args.block, args.variant)
+    if args.jkr == "table":
+        from bfm.jkr import reference_table
+        ov, rad = reference_table(R, Rcell, params.E0, 100.0)
+        ctx.set_jkr_table(ov, rad)
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream.cuda_stream)
+    with torch.cuda.stream(stream):
+        d_cells = torch.from_numpy(np.ascontiguousarray(cells.T)).to("cuda")
+        d_vel = torch.zeros((6, n), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def stage():
+        ctx.dev_stage(d_cells.data_ptr(), n, n, thr, R, Ldiv, d_vel.data_ptr())
+
+    for _ in range(args.warmup):
+        stage()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for _ in range(args.steps):
+        stage()
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ctx.sync()
+    ms = t_start.elapsed_time(t_end) / args.steps
+    value = n / (ms * 1e-3)
+
+    # per-kernel times (HIP events around each launch on the launch stream)
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    ctx.stats()  # resets the candidate counters
+    for _ in range(args.steps):
+        stage()
+    kt = ctx.kernel_times()
+    st = ctx.stats()
+    ctx.enable_timing(False)
+    m = st["m"]
+    per_kernel = {k: v[0] / v[1] for k, v in kt.items()}
+    dom = max(kt, key=lambda k: kt[k][0])
+    tests_per_launch = st["segment_tests"] / args.steps
+    # screened candidates with j < i per launch: counters[0] is not exported;
+    # estimate from the grid cutoff (documented in DESIGN.md) is not used here.
+    if dom == "candidates":
+        flops = tests_per_launch * TEST_FLOPS
+        achieved = flops / (per_kernel[dom] * 1e-3) / 1e12
+        roof = dict(kernel=dom, bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS,
+                    unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
+                    avg_us=per_kernel[dom] * 1e3, algorithmic_flops=flops)
+    else:
+        # HBM-bound per-cell / per-pair kernels: algorithmic bytes
+        bytes_ = n * 176 + m * 48
+        achieved = bytes_ / (per_kernel[dom] * 1e-3) / 1e9
+        roof = dict(kernel=dom, bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=None, avg_us=per_kernel[dom] * 1e3,
+                    algorithmic_bytes=bytes_)
+    stage_bytes = n * 328 + m * 96
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cells, thr, R, Ldiv, params, args.cpu_budget)
+    out = {
+        "metric": "RK-stage cell-updates/s, 200k-cell 3D FP64",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "config-1 static 3-D dome snapshot: one RK stage (binning, sort, "
+                               "neighbor list, Hertz+JKR force/torque, velocities)",
+                   "cells": n, "pairs": m, "pairs_per_cell": m / n, "dome_radius_um": dome_r,
+                   "seed": args.seed, "jkr": args.jkr, "neighbor_threshold_um": thr,
+                   "bin_edge_um": st["bin_edge"], "nbins": st["nbins"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "stage_hbm_frac": (stage_bytes / (ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
+        "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(per_kernel.items(),
+                                                              key=lambda x: -x[1])},
+        "wall_s_timed": wall,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -142,7 +142,7 @@ def load_library():
         "bfm_kernel_times": [vp, C.c_char_p, C.c_int, _dp, _i64p, C.c_int, _ip],
         "bfm_reset_timing": [vp],
         "bfm_stats": [vp, _i64p, _i64p, _dp, _ip],
-        "bfm_set_tuning": [vp, C.c_double, C.c_int],
+        "bfm_set_tuning": [vp, C.c_double, C.c_int, C.c_int],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -209,8 +209,9 @@ class Context:
         self._check(self.lib.bfm_set_jkr_table(self.h, _ptr(ov), len(ov), _ptr(g),
                                                0 if g is None else len(g), _ptr(rad)))
 
-    def set_tuning(self, bin_scale=0.0, block_threads=0):
-        self._check(self.lib.bfm_set_tuning(self.h, float(bin_scale), int(block_threads)))
+    def set_tuning(self, bin_scale=0.0, block_threads=0, kernel_variant=-1):
+        self._check(self.lib.bfm_set_tuning(self.h, float(bin_scale), int(block_threads),
+                                            int(kernel_variant)))
 
     def get_cell_neighbors(self, cells, thr, R, Ldiv):
         cells = _f64(cells)

@@ -59,6 +59,7 @@ struct bfm_ctx {
     int slot_cap = 64;
     double bin_scale = 1.0;
     int block_threads = 256;
+    int cand_variant = 1;           // 0: wave-queue kernel, 1: LDS-staged kernel
 
     bfm_params p{};
     bool params_set = false;

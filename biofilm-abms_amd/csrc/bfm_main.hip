@@ -196,19 +196,33 @@ __global__ void __launch_bounds__(256) k_bbox_partial(const double* __restrict__
 // Final reduction and grid plan. The search cutoff covers every pair the
 // reference can keep: |r_i - r_j| < thr + Ldiv + 2R (its prefilter) and,
 // since |d| >= |r_i - r_j| - hl_i - hl_j, |r_i - r_j| < thr + 2 max(hl).
-__global__ void k_plan(const double* __restrict__ partials, int nparts, double thr, double pre,
-                       double bin_scale, int max_bins, Plan* __restrict__ plan)
+__global__ void __launch_bounds__(256) k_plan(const double* __restrict__ partials, int nparts,
+                                              double thr, double pre, double bin_scale, int max_bins,
+                                              Plan* __restrict__ plan)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    double hl = 0;
-    for (int b = 0; b < nparts; ++b) {
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fmin(lo[k], partials[b * 7 + k]);
-            hi[k] = fmax(hi[k], partials[b * 7 + 3 + k]);
-        }
-        hl = fmax(hl, partials[b * 7 + 6]);
+    __shared__ double red[7][256];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        red[k][tid] = tid < nparts ? partials[tid * 7 + k] : INFINITY;
+        red[3 + k][tid] = tid < nparts ? partials[tid * 7 + 3 + k] : -INFINITY;
     }
+    red[6][tid] = tid < nparts ? partials[tid * 7 + 6] : 0.0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                red[k][tid] = fmin(red[k][tid], red[k][tid + w]);
+                red[3 + k][tid] = fmax(red[3 + k][tid], red[3 + k][tid + w]);
+            }
+            red[6][tid] = fmax(red[6][tid], red[6][tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    double lo[3] = {red[0][0], red[1][0], red[2][0]}, hi[3] = {red[3][0], red[4][0], red[5][0]};
+    double hl = red[6][0];
     double cut = fmin(pre, thr + 2 * hl) * (1 + 1e-9) + 1e-9;
     double edge = cut * bin_scale;
     Plan P;
@@ -406,6 +420,161 @@ k_candidates(int n, const Plan* __restrict__ plan, const double* __restrict__ so
     }
     if (lane == 0) {
         atomicAdd(&counters[0], screened);
+        atomicAdd(&counters[1], tested);
+    }
+}
+
+// LDS-staged variant of the candidate search. A workgroup owns HOMES
+// consecutive bin-ordered home cells; homes that share a grid row (same y, z
+// bin) form a group, and the union of the group's stencil rows (bins
+// [xmin - reach, xmax + reach] of each neighboring row, contiguous in sorted
+// order) is staged into LDS in chunks of CAP cells. Each thread owns staged
+// candidates and screens them against a batch of BATCH homes read from LDS
+// (wavefront-uniform addresses, i.e. broadcasts); the survivors (at most
+// BATCH * CAP, so the queue cannot overflow) are tested with the segment
+// kernel by all threads. The predicate and the arithmetic are those of
+// k_candidates, so the list is identical.
+template <int HOMES, int CAP, int BATCH>
+__global__ void __launch_bounds__(256)
+k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
+                 const int* __restrict__ sid, const int* __restrict__ skey,
+                 const int* __restrict__ bin_off, int slot_cap, int* __restrict__ count,
+                 int* __restrict__ slot_j, double* __restrict__ slot_g,
+                 unsigned long long* __restrict__ counters, int* __restrict__ status)
+{
+    static_assert(BATCH * CAP <= 65536 && CAP <= 4096 && BATCH <= 16, "queue entry packing");
+    __shared__ double c_x[CAP], c_y[CAP], c_z[CAP], c_hl[CAP];
+    __shared__ int c_id[CAP], c_q[CAP], c_bx[CAP];
+    __shared__ uint16_t queue[BATCH * CAP];
+    __shared__ double h_x[HOMES], h_y[HOMES], h_z[HOMES], h_nx[HOMES], h_ny[HOMES], h_nz[HOMES], h_hl[HOMES];
+    __shared__ int h_id[HOMES], h_bx[HOMES], h_row[HOMES], h_cnt[HOMES];
+    __shared__ int seg_lo[49], seg_pre[50];
+    __shared__ int s_qn, s_nseg;
+    __shared__ unsigned long long s_screened, s_tested;
+
+    const int tid = threadIdx.x;
+    const int h0 = blockIdx.x * HOMES;
+    if (h0 >= n) return;
+    const int H = min(HOMES, n - h0);
+    const Plan P = *plan;
+    const double thr = P.thr, pre = P.pre;
+    const int reach = P.reach;
+    if (tid < H) {
+        const int hs = h0 + tid;
+        h_x[tid] = sorted[hs]; h_y[tid] = sorted[stride + hs]; h_z[tid] = sorted[2 * stride + hs];
+        h_nx[tid] = sorted[3 * stride + hs]; h_ny[tid] = sorted[4 * stride + hs];
+        h_nz[tid] = sorted[5 * stride + hs]; h_hl[tid] = sorted[6 * stride + hs];
+        h_id[tid] = sid[hs];
+        const int key = skey[hs];
+        h_bx[tid] = key % P.dims[0];
+        h_row[tid] = key / P.dims[0];
+        h_cnt[tid] = 0;
+    }
+    if (tid == 0) { s_screened = 0; s_tested = 0; }
+    __syncthreads();
+    unsigned long long screened = 0, tested = 0;
+
+    for (int g0 = 0; g0 < H;) {
+        const int row = h_row[g0];
+        int g1 = g0 + 1;
+        while (g1 < H && h_row[g1] == row) ++g1;
+        const int xmin = h_bx[g0], xmax = h_bx[g1 - 1];
+        const int by = row % P.dims[1], bz = row / P.dims[1];
+        const int side = 2 * reach + 1;
+        if (tid < side * side) {
+            const int dz = tid / side - reach, dy = tid % side - reach;
+            const int zz = bz + dz, yy = by + dy;
+            int lo = 0, len = 0;
+            if (zz >= 0 && zz < P.dims[2] && yy >= 0 && yy < P.dims[1]) {
+                const int rb = (zz * P.dims[1] + yy) * P.dims[0];
+                const int xlo = max(0, xmin - reach), xhi = min(P.dims[0] - 1, xmax + reach);
+                lo = bin_off[rb + xlo];
+                len = bin_off[rb + xhi + 1] - lo;
+            }
+            seg_lo[tid] = lo;
+            seg_pre[tid + 1] = len;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            seg_pre[0] = 0;
+            for (int k = 0; k < side * side; ++k) seg_pre[k + 1] += seg_pre[k];
+            s_nseg = side * side;
+        }
+        __syncthreads();
+        const int S = seg_pre[s_nseg];
+        for (int c0 = 0; c0 < S; c0 += CAP) {
+            const int C = min(CAP, S - c0);
+            for (int t = tid; t < C; t += 256) {
+                const int gidx = c0 + t;
+                int sg = 0;
+                while (seg_pre[sg + 1] <= gidx) ++sg;
+                const int q = seg_lo[sg] + (gidx - seg_pre[sg]);
+                c_x[t] = sorted[q]; c_y[t] = sorted[stride + q]; c_z[t] = sorted[2 * stride + q];
+                c_hl[t] = sorted[6 * stride + q];
+                c_id[t] = sid[q];
+                c_q[t] = q;
+                c_bx[t] = skey[q] % P.dims[0];
+            }
+            __syncthreads();
+            for (int b0 = g0; b0 < g1; b0 += BATCH) {
+                const int G = min(BATCH, g1 - b0);
+                if (tid == 0) s_qn = 0;
+                __syncthreads();
+                for (int t = tid; t < C; t += 256) {
+                    const double x = c_x[t], y = c_y[t], z = c_z[t], hl = c_hl[t];
+                    const int id = c_id[t], bx = c_bx[t];
+                    for (int hh = 0; hh < G; ++hh) {
+                        const int h = b0 + hh;
+                        if (abs(h_bx[h] - bx) > reach || !(id < h_id[h])) continue;
+                        ++screened;
+                        const double dx = h_x[h] - x, dy = h_y[h] - y, dz = h_z[h] - z;
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        const double bound = (thr + h_hl[h] + hl) * (1 + 1e-9) + 1e-9;
+                        if (r2 <= bound * bound && sqrt(r2) < pre) {
+                            const int pos = atomicAdd(&s_qn, 1);
+                            queue[pos] = (uint16_t)((hh << 12) | t);
+                        }
+                    }
+                }
+                __syncthreads();
+                const int qn = s_qn;
+                for (int e = tid; e < qn; e += 256) {
+                    const int ent = queue[e];
+                    const int h = b0 + (ent >> 12), t = ent & 4095;
+                    Seg ci, cj;
+                    ci.r[0] = h_x[h]; ci.r[1] = h_y[h]; ci.r[2] = h_z[h];
+                    ci.n[0] = h_nx[h]; ci.n[1] = h_ny[h]; ci.n[2] = h_nz[h]; ci.hl = h_hl[h];
+                    const int q = c_q[t];
+                    cj.r[0] = c_x[t]; cj.r[1] = c_y[t]; cj.r[2] = c_z[t];
+                    cj.n[0] = sorted[3 * stride + q]; cj.n[1] = sorted[4 * stride + q];
+                    cj.n[2] = sorted[5 * stride + q]; cj.hl = c_hl[t];
+                    double d[3], sv, tv;
+                    dist_between_cells(ci, cj, d, sv, tv);
+                    if (norm3(d[0], d[1], d[2]) < thr) {
+                        const int slot = atomicAdd(&h_cnt[h], 1);
+                        if (slot < slot_cap) {
+                            const int64_t base = (int64_t)h_id[h] * slot_cap + slot;
+                            slot_j[base] = c_id[t];
+                            double* gp = slot_g + base * 5;
+                            gp[0] = d[0]; gp[1] = d[1]; gp[2] = d[2]; gp[3] = sv; gp[4] = tv;
+                        }
+                    }
+                }
+                if (tid == 0) tested += (unsigned long long)qn;
+                __syncthreads();
+            }
+        }
+        g0 = g1;
+    }
+    atomicAdd(&s_screened, screened);
+    __syncthreads();
+    if (tid < H) {
+        int c = h_cnt[tid];
+        if (c > slot_cap) { atomicOr(status, ST_SLOT_OVERFLOW); c = slot_cap; }
+        count[h_id[tid]] = c;
+    }
+    if (tid == 0) {
+        atomicAdd(&counters[0], s_screened);
         atomicAdd(&counters[1], tested);
     }
 }
@@ -975,7 +1144,7 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     const double pre = thr + Ldiv + 2 * R;
     const int nparts = std::min(kBBoxParts, grid_for(n, 256));
     KLAUNCH(c, "bbox", k_bbox_partial, nparts, 256, 0, d_cells, n, ld, c->d_partials);
-    KLAUNCH(c, "plan", k_plan, 1, 64, 0, c->d_partials, nparts, thr, pre, c->bin_scale, c->max_bins, c->d_plan);
+    KLAUNCH(c, "plan", k_plan, 1, 256, 0, c->d_partials, nparts, thr, pre, c->bin_scale, c->max_bins, c->d_plan);
     KLAUNCH(c, "fill", k_fill_int, grid_for(c->max_bins + 1, 256, 4096), 256, 0, c->d_bin_count, 0,
             (int64_t)c->max_bins + 1);
     KLAUNCH(c, "keys", k_keys, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_keys, c->d_vals,
@@ -991,25 +1160,33 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     KLAUNCH(c, "reorder", k_reorder, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_vals_alt, c->d_sorted,
             c->d_sid, stride);
     HIPCHK(hipMemcpyAsync(c->d_skey, c->d_keys_alt, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
-    const int waves = std::max(1, c->block_threads / 64);
-    const int homes_per_wg = waves * kHomesPerWave;
-    const int g = (n + homes_per_wg - 1) / homes_per_wg;
-    if (waves == 1)
-        KLAUNCH(c, "candidates", k_candidates<1>, g, 64, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                c->d_status);
-    else if (waves == 2)
-        KLAUNCH(c, "candidates", k_candidates<2>, g, 128, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                c->d_status);
-    else if (waves == 4)
-        KLAUNCH(c, "candidates", k_candidates<4>, g, 256, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                c->d_status);
-    else
-        KLAUNCH(c, "candidates", k_candidates<8>, g, 512, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                c->d_status);
+    if (c->cand_variant == 1) {
+        constexpr int HOMES = 32;
+        const int g = (n + HOMES - 1) / HOMES;
+        KLAUNCH(c, "candidates", (k_candidates_lds<HOMES, 640, 8>), g, 256, 0, n, c->d_plan, c->d_sorted,
+                stride, c->d_sid, c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j,
+                c->d_slot_g, c->d_counters, c->d_status);
+    } else {
+        const int waves = std::max(1, c->block_threads / 64);
+        const int homes_per_wg = waves * kHomesPerWave;
+        const int g = (n + homes_per_wg - 1) / homes_per_wg;
+        if (waves == 1)
+            KLAUNCH(c, "candidates", k_candidates<1>, g, 64, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                    c->d_status);
+        else if (waves == 2)
+            KLAUNCH(c, "candidates", k_candidates<2>, g, 128, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                    c->d_status);
+        else if (waves == 4)
+            KLAUNCH(c, "candidates", k_candidates<4>, g, 256, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                    c->d_status);
+        else
+            KLAUNCH(c, "candidates", k_candidates<8>, g, 512, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
+                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
+                    c->d_status);
+    }
     KLAUNCH(c, "fill", k_fill_int, 1, 64, 0, c->d_count + n, 0, (int64_t)1);
     scan(c, c->d_count, c->d_offsets, (int64_t)n + 1, "scan_counts");
     KLAUNCH(c, "compact", k_compact, grid_for(n, 256), 256, 0, n, c->slot_cap, c->d_count, c->d_offsets,
@@ -1248,8 +1425,8 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
 int bfm_destroy(bfm_ctx* c)
 {
     if (!c) return BFM_ERR_ARG;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_status, c->d_counters, c->d_cells, c->d_cells_new, c->d_nb, c->d_noise, c->d_out6,
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
@@ -1257,11 +1434,11 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_mag, c->d_cc,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp};
     for (void* q : ptrs)
-        if (q) hipFree(q);
+        if (q) (void)hipFree(q);
     bfm::surrogate_free(c);
     for (auto& t : c->pending) { c->event_pool.push_back(t.a); c->event_pool.push_back(t.b); }
-    for (auto e : c->event_pool) hipEventDestroy(e);
-    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BFM_OK;
 }
@@ -1578,11 +1755,15 @@ int bfm_stats(bfm_ctx* c, int64_t* m, int64_t* tests, double* edge, int* nbins)
     });
 }
 
-int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads)
+int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads, int kernel_variant)
 {
     return guarded(c, [&] {
+        if (kernel_variant >= 0) {
+            require(kernel_variant <= 1, BFM_ERR_ARG, "kernel_variant must be 0 or 1");
+            c->cand_variant = kernel_variant;
+        }
         if (bin_scale > 0) {
-            require(bin_scale >= 0.2 && bin_scale <= 4.0, BFM_ERR_ARG, "bin_scale out of range");
+            require(bin_scale >= 1.0 / 3.0 && bin_scale <= 4.0, BFM_ERR_ARG, "bin_scale out of range");
             c->bin_scale = bin_scale;
         }
         if (block_threads > 0) {

@@ -182,9 +182,11 @@ int bfm_stats(bfm_ctx* ctx, int64_t* m, int64_t* candidate_tests, double* bin_ed
               int* nbins);
 
 /* Tuning knobs of the neighbor build: bin edge as a multiple of the search
- * cutoff (>= 1 keeps the 27-bin stencil exact; 0.5 uses a 125-bin stencil),
- * and threads per workgroup of the candidate kernel. 0 = defaults. */
-int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads);
+ * cutoff (1/3 .. 4; 1 gives the 27-bin stencil, 0.5 the 125-bin stencil),
+ * threads per workgroup of the wave-queue candidate kernel, and the candidate
+ * kernel (0 = wave-queue, 1 = LDS-staged). 0 / 0 / -1 keep the current value.
+ * No knob changes the neighbor list, only the time to build it. */
+int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads, int kernel_variant);
 
 #ifdef __cplusplus
 }

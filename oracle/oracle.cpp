@@ -204,6 +204,31 @@ int64_t oracle_get_cell_neighbors(const double* cells, int n, int ld, double thr
     return idx;
 }
 
+/* Rows [row_lo, row_hi) of oracle_get_cell_neighbors (same loop body),
+ * counting kept pairs only: a bounded sample of the O(N^2) search for CPU
+ * timing. Row i costs O(i), so a block centred on N/2 has the mean cost. */
+int64_t oracle_count_neighbor_rows(const double* cells, int n, int ld, double thr, double R,
+                                   double Ldiv, int row_lo, int row_hi)
+{
+    const double pre = thr + Ldiv + 2 * R;
+    int64_t idx = 0;
+    for (int i = std::max(row_lo, 0); i < std::min(row_hi, n); ++i) {
+        double ri[3], ni[3], hli;
+        cell_geom(cells, ld, i, ri, ni, &hli);
+        for (int j = 0; j < i; ++j) {
+            double rj[3], nj[3], hlj;
+            cell_geom(cells, ld, j, rj, nj, &hlj);
+            double dist_rij = norm3(ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]);
+            if (dist_rij < pre) {
+                double d[3], si, sj;
+                oracle_dist_between_cells(ri, ni, hli, rj, nj, hlj, d, &si, &sj);
+                if (norm3(d[0], d[1], d[2]) < thr) idx++;
+            }
+        }
+    }
+    return idx;
+}
+
 /* Same list as oracle_get_cell_neighbors, found with a uniform grid and
  * OpenMP (context timing only; not the reference's algorithm). A candidate is
  * dropped without the segment test only when |r_i - r_j| exceeds

@@ -37,6 +37,8 @@ int64_t oracle_get_cell_neighbors(const double* cells, int n, int ld, double thr
 int64_t oracle_get_cell_neighbors_binned(const double* cells, int n, int ld, double thr,
                                          double R, double Ldiv, double* out,
                                          int64_t ld_out, int nthreads);
+int64_t oracle_count_neighbor_rows(const double* cells, int n, int ld, double thr, double R,
+                                   double Ldiv, int row_lo, int row_hi);
 void oracle_update_neighbor_distances(const double* cells, int n, int ld,
                                       double* neighbors, int64_t m, int64_t ld_nb);
 

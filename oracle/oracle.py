@@ -33,6 +33,9 @@ def lib():
         L.oracle_get_cell_neighbors_binned.argtypes = [_dp, C.c_int, C.c_int, d, d, d, _dp, i64,
                                                        C.c_int]
         L.oracle_get_cell_neighbors_binned.restype = i64
+        L.oracle_count_neighbor_rows.argtypes = [_dp, C.c_int, C.c_int, d, d, d, C.c_int,
+                                                 C.c_int]
+        L.oracle_count_neighbor_rows.restype = i64
         L.oracle_update_neighbor_distances.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64]
         L.oracle_update_neighbor_distances.restype = None
         L.oracle_jkr_contact_radius.argtypes = [d, d, d, d, _ip]
@@ -114,6 +117,12 @@ def get_cell_neighbors(cells, thr, R, Ldiv, binned=False, nthreads=0):
         if m <= cap:
             return np.asfortranarray(out[:m])
         cap = m
+
+
+def count_neighbor_rows(cells, thr, R, Ldiv, row_lo, row_hi):
+    cells = _f(cells)
+    return lib().oracle_count_neighbor_rows(_p(cells), cells.shape[0], cells.shape[0], thr, R,
+                                            Ldiv, row_lo, row_hi)
 
 
 def update_neighbor_distances(cells, nb):
