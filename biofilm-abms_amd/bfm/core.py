@@ -111,6 +111,13 @@ def load_library():
     path = library_path()
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    # torch ships its own HIP runtime under the same soname as /opt/rocm's; the
+    # first one loaded serves the whole process. Load torch's first so torch and
+    # libbfm share one runtime (device pointers and streams pass between them).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     vp = C.c_void_p
     sig = {

@@ -1053,7 +1053,7 @@ int nbits(int64_t v)
     return b;
 }
 
-void check_status(bfm_ctx* c)
+int take_status(bfm_ctx* c)
 {
     HIPCHK(hipStreamSynchronize(c->stream));
     int st = 0;
@@ -1061,18 +1061,45 @@ void check_status(bfm_ctx* c)
     if (st) {
         int zero = 0;
         HIPCHK(hipMemcpy(c->d_status, &zero, sizeof(int), hipMemcpyHostToDevice));
-        if (st & ST_SLOT_OVERFLOW)
-            throw ArgError{BFM_ERR_CAPACITY, "a cell has more neighbors than the per-cell slot capacity"};
-        if (st & ST_PAIR_OVERFLOW)
-            throw ArgError{BFM_ERR_CAPACITY, "neighbor pairs exceed max_pairs of the context"};
-        if (st & ST_JKR_ROOT)
-            throw ArgError{BFM_ERR_JKR_ROOT,
-                           "JKR contact radius is undefined; invalid number of real roots found "
-                           "from radius-overlap polynomial"};
-        if (st & ST_NAN) throw ArgError{BFM_ERR_NAN, "Found nan in velocities"};
-        throw ArgError{BFM_ERR_HIP, "internal device status error"};
     }
+    return st;
 }
+
+// Doubles the per-cell slot capacity of the neighbor build.
+void grow_slots(bfm_ctx* c)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int cap = c->slot_cap * 2;
+    int* sj = nullptr;
+    double* sg = nullptr;
+    HIPCHK(hipMalloc((void**)&sj, sizeof(int) * (size_t)c->max_cells * cap));
+    HIPCHK(hipMalloc((void**)&sg, sizeof(double) * (size_t)c->max_cells * cap * 5));
+    HIPCHK(hipFree(c->d_slot_j));
+    HIPCHK(hipFree(c->d_slot_g));
+    c->d_slot_j = sj;
+    c->d_slot_g = sg;
+    c->slot_cap = cap;
+}
+
+void raise_status(bfm_ctx* c, int st)
+{
+    if (!st) return;
+    if (st & ST_SLOT_OVERFLOW) {
+        grow_slots(c);
+        throw ArgError{BFM_ERR_CAPACITY, "a cell had more neighbors than the per-cell slot capacity; "
+                                         "the capacity has been doubled, run the call again"};
+    }
+    if (st & ST_PAIR_OVERFLOW)
+        throw ArgError{BFM_ERR_CAPACITY, "neighbor pairs exceed max_pairs of the context"};
+    if (st & ST_JKR_ROOT)
+        throw ArgError{BFM_ERR_JKR_ROOT,
+                       "JKR contact radius is undefined; invalid number of real roots found "
+                       "from radius-overlap polynomial"};
+    if (st & ST_NAN) throw ArgError{BFM_ERR_NAN, "Found nan in velocities"};
+    throw ArgError{BFM_ERR_HIP, "internal device status error"};
+}
+
+void check_status(bfm_ctx* c) { raise_status(c, take_status(c)); }
 
 void need_params(bfm_ctx* c)
 {
@@ -1193,6 +1220,22 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
             c->d_slot_j, c->d_slot_g, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_m,
             c->d_status);
     c->h_m = -1;
+}
+
+// Synchronous build: grows the per-cell slot capacity until the list fits.
+void build_neighbors_sync(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R,
+                          double Ldiv)
+{
+    for (;;) {
+        build_neighbors(c, d_cells, n, ld, thr, R, Ldiv);
+        int st = take_status(c);
+        if ((st & ST_SLOT_OVERFLOW) && !(st & ~ST_SLOT_OVERFLOW) && c->slot_cap < 4096) {
+            grow_slots(c);
+            continue;
+        }
+        raise_status(c, st);
+        return;
+    }
 }
 
 // Transposes of the current list: rows per cell as j (sorted), and as i when
@@ -1513,8 +1556,7 @@ int bfm_get_cell_neighbors(bfm_ctx* c, const double* cells, int n, int ld, doubl
         check_cells(c, n, ld);
         require(m_out != nullptr, BFM_ERR_ARG, "m_out is NULL");
         upload_cells(c, cells, n, ld);
-        build_neighbors(c, c->d_cells, n, c->max_cells, thr, R, Ldiv);
-        check_status(c);
+        build_neighbors_sync(c, c->d_cells, n, c->max_cells, thr, R, Ldiv);
         int m = 0;
         HIPCHK(hipMemcpy(&m, c->d_m, sizeof(int), hipMemcpyDeviceToHost));
         c->h_m = m;
@@ -1623,13 +1665,14 @@ int bfm_dev_build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, do
     return guarded(c, [&] {
         need_params(c);
         check_cells(c, n, ld);
-        build_neighbors(c, d_cells, n, ld, thr, R, Ldiv);
         if (m_out) {
-            check_status(c);
+            build_neighbors_sync(c, d_cells, n, ld, thr, R, Ldiv);
             int m = 0;
             HIPCHK(hipMemcpy(&m, c->d_m, sizeof(int), hipMemcpyDeviceToHost));
             c->h_m = m;
             *m_out = m;
+        } else {
+            build_neighbors(c, d_cells, n, ld, thr, R, Ldiv);
         }
     });
 }
