@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # FP64 operations per unit of work of the candidate kernel, counted from
 # bfm_device.hpp / k_candidates (adds, muls, divides and square roots each 1):
-SCREEN_FLOPS = 14      # r_ij^2, bound, comparison per screened candidate with j < i
+SCREEN_FLOPS = 12      # r_ij^2 (8), bound and its square (4) per screened pair
 TEST_FLOPS = 80        # segment-segment closest points + |d| per queued pair
 
 
@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (calibrates FETCH/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -91,6 +93,15 @@ def main():
 
     import bfm
 
+    if args.copy_calibration:
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8,
+                          "launches": 3}))
+        return
     n = args.cells
     R, Rcell, Ldiv = 0.8, 0.5, 2.4
     thr = 0.5 * (2 * R + Ldiv)
@@ -145,14 +156,14 @@ def main():
     per_kernel = {k: v[0] / v[1] for k, v in kt.items()}
     dom = max(kt, key=lambda k: kt[k][0])
     tests_per_launch = st["segment_tests"] / args.steps
-    # screened candidates with j < i per launch: counters[0] is not exported;
-    # estimate from the grid cutoff (documented in DESIGN.md) is not used here.
+    screened_per_launch = st["screened"] / args.steps
     if dom == "candidates":
-        flops = tests_per_launch * TEST_FLOPS
+        flops = tests_per_launch * TEST_FLOPS + screened_per_launch * SCREEN_FLOPS
         achieved = flops / (per_kernel[dom] * 1e-3) / 1e12
         roof = dict(kernel=dom, bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS,
                     unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
-                    avg_us=per_kernel[dom] * 1e3, algorithmic_flops=flops)
+                    avg_us=per_kernel[dom] * 1e3, algorithmic_flops=flops,
+                    screened_per_launch=screened_per_launch, tests_per_launch=tests_per_launch)
     else:
         # HBM-bound per-cell / per-pair kernels: algorithmic bytes
         bytes_ = n * 176 + m * 48

@@ -148,7 +148,7 @@ def load_library():
         "bfm_enable_timing": [vp, C.c_int],
         "bfm_kernel_times": [vp, C.c_char_p, C.c_int, _dp, _i64p, C.c_int, _ip],
         "bfm_reset_timing": [vp],
-        "bfm_stats": [vp, _i64p, _i64p, _dp, _ip],
+        "bfm_stats": [vp, _i64p, _i64p, _i64p, _dp, _ip],
         "bfm_set_tuning": [vp, C.c_double, C.c_int, C.c_int],
     }
     for name, args in sig.items():
@@ -328,10 +328,11 @@ class Context:
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
 
     def stats(self):
-        m, tests, edge, nb = C.c_int64(), C.c_int64(), C.c_double(), C.c_int()
-        self._check(self.lib.bfm_stats(self.h, C.byref(m), C.byref(tests), C.byref(edge),
-                                       C.byref(nb)))
-        return dict(m=m.value, segment_tests=tests.value, bin_edge=edge.value, nbins=nb.value)
+        m, scr, tests, edge, nb = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double(), C.c_int()
+        self._check(self.lib.bfm_stats(self.h, C.byref(m), C.byref(scr), C.byref(tests),
+                                       C.byref(edge), C.byref(nb)))
+        return dict(m=m.value, screened=scr.value, segment_tests=tests.value, bin_edge=edge.value,
+                    nbins=nb.value)
 
 
 # ---- reference-named API (mechanics.hpp) -----------------------------------

@@ -18,6 +18,7 @@ struct Plan {
     double inv_edge;
     double pre;       // reference prefilter radius thr + Ldiv + 2R
     double thr;
+    double fmargin;   // FP32 pre-screen margin: 1e-3 um + 1e-6 * max |coordinate|
     int dims[3];
     int reach;        // stencil half-width in bins
     int nbins;
@@ -56,10 +57,10 @@ struct bfm_ctx {
     int max_cells = 0;
     int64_t max_pairs = 0;
     int max_bins = 0;
-    int slot_cap = 64;
+    int slot_cap = 32;
     double bin_scale = 1.0;
     int block_threads = 256;
-    int cand_variant = 1;           // 0: wave-queue kernel, 1: LDS-staged kernel
+    int cand_variant = 3;           // LDS candidate kernel configuration (HOMES, CAP, BATCH)
 
     bfm_params p{};
     bool params_set = false;
@@ -91,8 +92,8 @@ struct bfm_ctx {
     int* d_skey = nullptr;          // sorted keys
     int* d_count = nullptr;         // per-cell neighbor counts (max_cells + 1)
     int* d_offsets = nullptr;       // exclusive scan of counts (max_cells + 1)
-    int* d_slot_j = nullptr;        // max_cells x slot_cap
-    double* d_slot_g = nullptr;     // max_cells x slot_cap x 5
+    int* d_slot_j = nullptr;        // slot_cap x max_cells (entry-major)
+    double* d_slot_g = nullptr;     // 5 x slot_cap x max_cells (component, entry, cell)
 
     // internal neighbor list (SoA)
     int* pi = nullptr; int* pj = nullptr;
@@ -104,15 +105,22 @@ struct bfm_ctx {
     bool list_sorted_i = false;     // rows sorted by i (i-role ranges contiguous)
     bool transposed = false;        // ioff/joff/perm valid for the current list
 
-    // transposes (i-role offsets live in d_offsets)
-    bool ioff_valid = false;
+    bool list_csr = false;          // per-cell CSR (device-built lists) instead of transposes
+    unsigned long long* d_cnt64 = nullptr;  // per cell: (rows as i) << 32 | (rows as j)
+    unsigned long long* d_off64 = nullptr;  // exclusive scan of d_cnt64
+    int* d_jslot = nullptr;         // max_cells x slot_cap: partners with larger index
+    int* d_jrow = nullptr;          // max_cells x slot_cap: row index of each jslot pair
+    int* d_slot_s2 = nullptr;       // max_cells x slot_cap: position of the pair in j's jslot
+    int* d_csr = nullptr;           // 2 x max_pairs row indices (bit 31: j role)
+    double* d_rec = nullptr;        // 8 x max_pairs: per-row {d, s, t, main, hard, 0}
+
+    // transposes of imported lists (i-role offsets live in d_offsets)
     int* d_joff = nullptr;                          // max_cells + 1
     int* d_icnt = nullptr; int* d_jcnt = nullptr;   // max_cells + 1
     int* d_perm_i = nullptr; int* d_perm_j = nullptr;  // max_pairs
     bool use_perm_i = false;
 
-    // per-pair force magnitudes and per-cell forces
-    double* d_mag = nullptr;        // 2 x max_pairs (main, hard)
+    // per-cell forces
     double* d_cc = nullptr;         // 6 x max_cells
     double* d_k = nullptr;          // 7 x 6 x max_cells stage velocities
     unsigned long long* d_errmax = nullptr;

@@ -6,20 +6,13 @@
 //
 // Pipeline of one RK stage (bfm_dev_stage):
 //   k_bbox_partial, k_plan        bounding box and uniform-grid plan (device side)
-//   k_keys                        bin id per cell + bin histogram
-//   hipcub radix sort             (bin id, cell index): deterministic bin order
-//   hipcub scan                   bin offsets
-//   k_reorder                     bin-ordered SoA copy of the geometry
-//   k_candidates                  one wavefront per 16 home cells; lanes screen
-//                                 the stencil bins, survivors are compacted into
-//                                 an LDS queue (ballot + popcount) and the
-//                                 segment-segment test runs on full wavefronts
-//   hipcub scan, k_compact        per-cell counts -> sorted (i, j) rows
-//   k_pair_forces                 Hertz / JKR magnitude per pair (+ j counts)
-//   k_fill_perm, k_sort_segments  transpose (rows where a cell is j), ascending
-//   k_velocities                  per cell: forces summed in ascending partner
-//                                 order (the reference's row order), surface
-//                                 forces, velocity solve
+//   k_keys + scan + k_scatter     counting sort of the cells into bin order
+//   k_candidates_lds              half-stencil candidate screening from LDS-staged
+//                                 bins, LDS queue, segment-segment tests
+//   scan (64-bit), k_compact      sorted (i asc, j asc) rows + per-cell row lists
+//   k_pair_forces                 Hertz / JKR magnitude per row (64-byte records)
+//   k_velocities                  per cell: forces summed in the reference's row
+//                                 order, surface forces, velocity solve
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -247,6 +240,11 @@ __global__ void __launch_bounds__(256) k_plan(const double* __restrict__ partial
     }
     P.edge = edge;
     P.inv_edge = 1.0 / edge;
+    // FP32 copies of coordinates carry errors <= 2^-24 |x|; a difference of two
+    // of them <= 2^-23 max|x| ~ 1.2e-7 max|x|, squared distances add a few ulps.
+    double amax = 0;
+    for (int k = 0; k < 3; ++k) amax = fmax(amax, fmax(fabs(lo[k]), fabs(hi[k])));
+    P.fmargin = 1e-3 + 1e-6 * amax;
     P.pre = pre;
     P.thr = thr;
     P.ok = 1;
@@ -274,14 +272,20 @@ __global__ void k_keys(const double* __restrict__ cells, int n, int ld, const Pl
     }
 }
 
-// Bin-ordered SoA copy of the geometry the candidate kernel reads.
-__global__ void k_reorder(const double* __restrict__ cells, int n, int ld,
-                          const int* __restrict__ sorted_vals, double* __restrict__ sorted,
-                          int* __restrict__ sid, int64_t stride)
+// Counting-sort scatter: each cell takes the next free position of its bin
+// (cursor starts at the bin offsets) and its geometry is copied there. The
+// order inside a bin depends on atomic timing, but nothing downstream does:
+// every pair is tested from the larger original index, and each cell's list
+// is sorted by partner index during compaction.
+__global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const int* __restrict__ keys,
+                          int* __restrict__ cursor, double* __restrict__ sorted, int* __restrict__ sid,
+                          int* __restrict__ skey, int64_t stride)
 {
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        int i = sorted_vals[q];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int key = keys[i];
+        const int q = atomicAdd(&cursor[key], 1);
         sid[q] = i;
+        skey[q] = key;
         sorted[0 * stride + q] = cellv(cells, ld, i, C_RX);
         sorted[1 * stride + q] = cellv(cells, ld, i, C_RY);
         sorted[2 * stride + q] = cellv(cells, ld, i, C_RZ);
@@ -292,16 +296,11 @@ __global__ void k_reorder(const double* __restrict__ cells, int n, int ld,
     }
 }
 
-constexpr int kHomesPerWave = 16;
-constexpr int kWaveSize = 64;
-constexpr int kQueueCap = 2 * kWaveSize;
-constexpr uint32_t kQMask = (1u << 27) - 1;
-
-__device__ __forceinline__ void wave_sync()
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x)
+        p[q] = 0ull;
 }
 
 __device__ __forceinline__ Seg load_sorted(const double* __restrict__ s, int64_t stride, int q)
@@ -313,144 +312,40 @@ __device__ __forceinline__ Seg load_sorted(const double* __restrict__ s, int64_t
     return c;
 }
 
-// Candidate screening and segment tests. Each wavefront owns 16 consecutive
-// bin-ordered home cells. For home cell i it walks the stencil rows (bins
-// contiguous in x are contiguous in sorted order), lane by lane, and keeps
-// candidates j with j < i (original indices, the reference's j < i loop),
-// |r_i - r_j| <= thr + hl_i + hl_j (+ margin; below this the centerline
-// distance cannot reach thr, so dropping them cannot change the list) and the
-// reference's prefilter |r_i - r_j| < thr + Ldiv + 2R. Survivors go to a
-// per-wave LDS queue; every time it holds 64 entries the wavefront runs 64
-// segment-segment tests at once, so the expensive path never runs on a
-// mostly idle wavefront.
-template <int WAVES>
-__global__ void __launch_bounds__(WAVES * 64)
-k_candidates(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
-             const int* __restrict__ sid, const int* __restrict__ skey,
-             const int* __restrict__ bin_off, int slot_cap, int* __restrict__ count,
-             int* __restrict__ slot_j, double* __restrict__ slot_g,
-             unsigned long long* __restrict__ counters, int* __restrict__ status)
-{
-    __shared__ uint32_t queue[WAVES][kQueueCap];
-    __shared__ int hcnt[WAVES][kHomesPerWave];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int h0 = (blockIdx.x * WAVES + wave) * kHomesPerWave;
-    if (h0 >= n) return;
-    const int hN = min(kHomesPerWave, n - h0);
-    const Plan P = *plan;
-    const double thr = P.thr, pre = P.pre;
-    if (lane < kHomesPerWave) hcnt[wave][lane] = 0;
-    wave_sync();
-    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    int qn = 0;
-    unsigned long long screened = 0, tested = 0;
-
-    auto process = [&](int cnt, int start) {
-        bool act = lane < cnt;
-        uint32_t e = act ? queue[wave][start + lane] : 0u;
-        wave_sync();
-        if (act) {
-            int hh = (int)(e >> 27), q = (int)(e & kQMask);
-            int hs = h0 + hh;
-            Seg ci = load_sorted(sorted, stride, hs);
-            Seg cj = load_sorted(sorted, stride, q);
-            double d[3], s, t;
-            dist_between_cells(ci, cj, d, s, t);
-            if (norm3(d[0], d[1], d[2]) < thr) {
-                int slot = atomicAdd(&hcnt[wave][hh], 1);
-                if (slot < slot_cap) {
-                    int64_t base = (int64_t)sid[hs] * slot_cap + slot;
-                    slot_j[base] = sid[q];
-                    double* g = slot_g + base * 5;
-                    g[0] = d[0]; g[1] = d[1]; g[2] = d[2]; g[3] = s; g[4] = t;
-                }
-            }
-        }
-        tested += (unsigned long long)cnt;
-    };
-
-    for (int hh = 0; hh < hN; ++hh) {
-        const int hs = h0 + hh;
-        const double xi = sorted[hs], yi = sorted[stride + hs], zi = sorted[2 * stride + hs];
-        const double hli = sorted[6 * stride + hs];
-        const int idi = sid[hs];
-        const int key = skey[hs];
-        const int bx = key % P.dims[0];
-        const int by = (key / P.dims[0]) % P.dims[1];
-        const int bz = key / (P.dims[0] * P.dims[1]);
-        const int x0 = max(0, bx - P.reach), x1 = min(P.dims[0] - 1, bx + P.reach);
-        for (int z = max(0, bz - P.reach); z <= min(P.dims[2] - 1, bz + P.reach); ++z) {
-            for (int y = max(0, by - P.reach); y <= min(P.dims[1] - 1, by + P.reach); ++y) {
-                const int row = (z * P.dims[1] + y) * P.dims[0];
-                const int lo = bin_off[row + x0], hi = bin_off[row + x1 + 1];
-                for (int base = lo; base < hi; base += 64) {
-                    const int q = base + lane;
-                    bool pass = false;
-                    if (q < hi) {
-                        const int idj = sid[q];
-                        if (idj < idi) {
-                            const double dx = xi - sorted[q];
-                            const double dy = yi - sorted[stride + q];
-                            const double dz = zi - sorted[2 * stride + q];
-                            const double r2 = (dx * dx + dy * dy) + dz * dz;
-                            const double bound = (thr + hli + sorted[6 * stride + q]) * (1 + 1e-9) + 1e-9;
-                            if (r2 <= bound * bound) pass = sqrt(r2) < pre;
-                        }
-                    }
-                    const uint64_t mask = __ballot(pass);
-                    screened += (unsigned long long)min(64, hi - base);
-                    if (pass) queue[wave][qn + __popcll(mask & lt_mask)] = ((uint32_t)hh << 27) | (uint32_t)q;
-                    qn += __popcll(mask);
-                    wave_sync();
-                    if (qn >= 64) {
-                        process(64, qn - 64);
-                        qn -= 64;
-                        wave_sync();
-                    }
-                }
-            }
-        }
-    }
-    if (qn > 0) process(qn, 0);
-    wave_sync();
-    if (lane < hN) {
-        int c = hcnt[wave][lane];
-        if (c > slot_cap) { atomicOr(status, ST_SLOT_OVERFLOW); c = slot_cap; }
-        count[sid[h0 + lane]] = c;
-    }
-    if (lane == 0) {
-        atomicAdd(&counters[0], screened);
-        atomicAdd(&counters[1], tested);
-    }
-}
-
-// LDS-staged variant of the candidate search. A workgroup owns HOMES
-// consecutive bin-ordered home cells; homes that share a grid row (same y, z
-// bin) form a group, and the union of the group's stencil rows (bins
-// [xmin - reach, xmax + reach] of each neighboring row, contiguous in sorted
-// order) is staged into LDS in chunks of CAP cells. Each thread owns staged
-// candidates and screens them against a batch of BATCH homes read from LDS
-// (wavefront-uniform addresses, i.e. broadcasts); the survivors (at most
-// BATCH * CAP, so the queue cannot overflow) are tested with the segment
-// kernel by all threads. The predicate and the arithmetic are those of
-// k_candidates, so the list is identical.
-template <int HOMES, int CAP, int BATCH>
+// LDS-staged candidate search over a half stencil. A workgroup owns HOMES
+// consecutive bin-ordered cells; homes sharing a grid row (same y, z bin)
+// form a group. Every unordered pair is screened once, by the cell that
+// comes later in bin order: for a group only bins ordered before the home
+// cells are candidates -- the stencil rows below the home row (full x window
+// [xmin - reach, xmax + reach]) and, in the home row, bins xmin - reach .. xmax
+// up to the last home. These ranges are contiguous in sorted order and are
+// staged into LDS in chunks of CAP cells. Each thread owns staged candidates
+// and screens them against a batch of BATCH homes (LDS broadcasts); survivors
+// (at most BATCH * CAP: the queue cannot overflow) are tested with the
+// segment kernel by all threads. A kept pair becomes row (i, j) with
+// i = max(original index), j = min, computed as distBetweenCells(cell i,
+// cell j) exactly as the reference's j < i loop does; its slot in i's list is
+// taken with a global atomic, and i is also appended to j's list of
+// partners with larger index (the compaction sorts both lists).
+template <int HOMES, int CAP, int BATCH, int QCAP>
 __global__ void __launch_bounds__(256)
 k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
                  const int* __restrict__ sid, const int* __restrict__ skey,
-                 const int* __restrict__ bin_off, int slot_cap, int* __restrict__ count,
-                 int* __restrict__ slot_j, double* __restrict__ slot_g,
-                 unsigned long long* __restrict__ counters, int* __restrict__ status)
+                 const int* __restrict__ bin_off, int slot_cap, unsigned long long* __restrict__ cnt64,
+                 int* __restrict__ slot_j, int* __restrict__ slot_s2, double* __restrict__ slot_g,
+                 int* __restrict__ jslot, unsigned long long* __restrict__ counters)
 {
-    static_assert(BATCH * CAP <= 65536 && CAP <= 4096 && BATCH <= 16, "queue entry packing");
-    __shared__ double c_x[CAP], c_y[CAP], c_z[CAP], c_hl[CAP];
+    static_assert(HOMES <= 64 && CAP <= 1024 && BATCH * CAP <= QCAP, "queue entry packing / bound");
+    __shared__ float4 c_f[CAP];             // FP32 copy (x, y, z, hl) for the pre-screen
     __shared__ int c_id[CAP], c_q[CAP], c_bx[CAP];
-    __shared__ uint16_t queue[BATCH * CAP];
+    __shared__ uint16_t queue[QCAP];
+    __shared__ int s_ovf;
     __shared__ double h_x[HOMES], h_y[HOMES], h_z[HOMES], h_nx[HOMES], h_ny[HOMES], h_nz[HOMES], h_hl[HOMES];
-    __shared__ int h_id[HOMES], h_bx[HOMES], h_row[HOMES], h_cnt[HOMES];
+    __shared__ int h_id[HOMES], h_bx[HOMES], h_row[HOMES];
+    __shared__ float4 h_f[HOMES];           // FP32 (x, y, z, thr + hl + 1e-3)
     __shared__ int seg_lo[49], seg_pre[50];
-    __shared__ int s_qn, s_nseg;
-    __shared__ unsigned long long s_screened, s_tested;
+    __shared__ int s_qn;
+    __shared__ unsigned long long s_screened;
 
     const int tid = threadIdx.x;
     const int h0 = blockIdx.x * HOMES;
@@ -465,14 +360,17 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
         h_nx[tid] = sorted[3 * stride + hs]; h_ny[tid] = sorted[4 * stride + hs];
         h_nz[tid] = sorted[5 * stride + hs]; h_hl[tid] = sorted[6 * stride + hs];
         h_id[tid] = sid[hs];
+        h_f[tid] = make_float4((float)h_x[tid], (float)h_y[tid], (float)h_z[tid],
+                               (float)(thr + h_hl[tid] + P.fmargin));
         const int key = skey[hs];
         h_bx[tid] = key % P.dims[0];
         h_row[tid] = key / P.dims[0];
-        h_cnt[tid] = 0;
     }
-    if (tid == 0) { s_screened = 0; s_tested = 0; }
+    if (tid == 0) s_screened = 0;
     __syncthreads();
     unsigned long long screened = 0, tested = 0;
+    const int side = 2 * reach + 1;
+    const int nseg = reach * side + reach + 1;   // rows before the home row, then the home row
 
     for (int g0 = 0; g0 < H;) {
         const int row = h_row[g0];
@@ -480,16 +378,20 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
         while (g1 < H && h_row[g1] == row) ++g1;
         const int xmin = h_bx[g0], xmax = h_bx[g1 - 1];
         const int by = row % P.dims[1], bz = row / P.dims[1];
-        const int side = 2 * reach + 1;
-        if (tid < side * side) {
+        const int last = h0 + g1 - 1;                  // sorted position of the group's last home
+        if (tid < nseg) {
+            // rows (dz, dy) lexicographically before (0, 0), then (0, 0) itself
             const int dz = tid / side - reach, dy = tid % side - reach;
             const int zz = bz + dz, yy = by + dy;
             int lo = 0, len = 0;
             if (zz >= 0 && zz < P.dims[2] && yy >= 0 && yy < P.dims[1]) {
                 const int rb = (zz * P.dims[1] + yy) * P.dims[0];
-                const int xlo = max(0, xmin - reach), xhi = min(P.dims[0] - 1, xmax + reach);
+                const int xlo = max(0, xmin - reach);
+                const int xhi = (dz == 0 && dy == 0) ? xmax : min(P.dims[0] - 1, xmax + reach);
                 lo = bin_off[rb + xlo];
-                len = bin_off[rb + xhi + 1] - lo;
+                int hi = bin_off[rb + xhi + 1];
+                if (dz == 0 && dy == 0) hi = min(hi, last);
+                len = max(0, hi - lo);
             }
             seg_lo[tid] = lo;
             seg_pre[tid + 1] = len;
@@ -497,11 +399,10 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
         __syncthreads();
         if (tid == 0) {
             seg_pre[0] = 0;
-            for (int k = 0; k < side * side; ++k) seg_pre[k + 1] += seg_pre[k];
-            s_nseg = side * side;
+            for (int k = 0; k < nseg; ++k) seg_pre[k + 1] += seg_pre[k];
         }
         __syncthreads();
-        const int S = seg_pre[s_nseg];
+        const int S = seg_pre[nseg];
         for (int c0 = 0; c0 < S; c0 += CAP) {
             const int C = min(CAP, S - c0);
             for (int t = tid; t < C; t += 256) {
@@ -509,58 +410,99 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
                 int sg = 0;
                 while (seg_pre[sg + 1] <= gidx) ++sg;
                 const int q = seg_lo[sg] + (gidx - seg_pre[sg]);
-                c_x[t] = sorted[q]; c_y[t] = sorted[stride + q]; c_z[t] = sorted[2 * stride + q];
-                c_hl[t] = sorted[6 * stride + q];
+                c_f[t] = make_float4((float)sorted[q], (float)sorted[stride + q], (float)sorted[2 * stride + q],
+                                     (float)sorted[6 * stride + q]);
                 c_id[t] = sid[q];
                 c_q[t] = q;
                 c_bx[t] = skey[q] % P.dims[0];
             }
             __syncthreads();
-            for (int b0 = g0; b0 < g1; b0 += BATCH) {
-                const int G = min(BATCH, g1 - b0);
-                if (tid == 0) s_qn = 0;
-                __syncthreads();
-                for (int t = tid; t < C; t += 256) {
-                    const double x = c_x[t], y = c_y[t], z = c_z[t], hl = c_hl[t];
-                    const int id = c_id[t], bx = c_bx[t];
-                    for (int hh = 0; hh < G; ++hh) {
-                        const int h = b0 + hh;
-                        if (abs(h_bx[h] - bx) > reach || !(id < h_id[h])) continue;
-                        ++screened;
-                        const double dx = h_x[h] - x, dy = h_y[h] - y, dz = h_z[h] - z;
+            // First try the whole group as one batch; if the queue would
+            // overflow (only in extremely dense packings), discard it and
+            // redo the chunk in batches of BATCH homes, whose queue is bounded
+            // by BATCH * CAP <= QCAP. No test has run before the check.
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                const int BT = attempt == 0 ? (g1 - g0) : BATCH;
+                bool overflow = false;
+                for (int b0 = g0; b0 < g1; b0 += BT) {
+                    const int G = min(BT, g1 - b0);
+                    if (tid == 0) { s_qn = 0; s_ovf = 0; }
+                    __syncthreads();
+                    for (int t = tid; t < C; t += 256) {
+                        const float4 cf = c_f[t];
+                        const int bx = c_bx[t];
+                        // homes after the candidate in sorted order, with |bx_h - bx| <= reach
+                        // homes of the group are sorted by bx: binary searches for the
+                        // first home with bx >= bx - reach and the first with bx > bx + reach
+                        int hl0 = g0, hi0 = g1;
+                        while (hl0 < hi0) {
+                            const int mid = (hl0 + hi0) >> 1;
+                            if (h_bx[mid] < bx - reach) hl0 = mid + 1; else hi0 = mid;
+                        }
+                        int hr0 = hl0;
+                        hi0 = g1;
+                        while (hr0 < hi0) {
+                            const int mid = (hr0 + hi0) >> 1;
+                            if (h_bx[mid] <= bx + reach) hr0 = mid + 1; else hi0 = mid;
+                        }
+                        const int first = max(max(b0, hl0), c_q[t] - h0 + 1);
+                        const int stop = min(b0 + G, hr0);
+                        for (int h = first; h < stop; ++h) {
+                            ++screened;
+                            // FP32 pre-screen: the rounding error of the FP32 distance is far
+                            // below the margin folded into h_f.w (1e-3 um + 1e-6 max|x|), so no
+                            // pair the FP64 test below would keep is dropped.
+                            const float4 hf = h_f[h];
+                            const float fx = hf.x - cf.x, fy = hf.y - cf.y, fz = hf.z - cf.z;
+                            const float fb = hf.w + cf.w;
+                            if (fx * fx + fy * fy + fz * fz <= fb * fb) {
+                                const int pos = atomicAdd(&s_qn, 1);
+                                if (pos < QCAP) queue[pos] = (uint16_t)(((h - b0) << 10) | t);
+                                else s_ovf = 1;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    if (s_ovf) { overflow = true; break; }
+                    const int qn = s_qn;
+                    for (int e = tid; e < qn; e += 256) {
+                        const int ent = queue[e];
+                        const int h = b0 + (ent >> 10), t = ent & 1023;
+                        Seg ch, cc;
+                        ch.r[0] = h_x[h]; ch.r[1] = h_y[h]; ch.r[2] = h_z[h];
+                        ch.n[0] = h_nx[h]; ch.n[1] = h_ny[h]; ch.n[2] = h_nz[h]; ch.hl = h_hl[h];
+                        cc = load_sorted(sorted, stride, c_q[t]);
+                        const bool home_is_i = h_id[h] > c_id[t];
+                        const int i = home_is_i ? h_id[h] : c_id[t];
+                        const int j = home_is_i ? c_id[t] : h_id[h];
+                        // |r_i - r_j| bound (below it |d| cannot reach thr) and the
+                        // reference prefilter |r_i - r_j| < thr + Ldiv + 2R
+                        const double dx = ch.r[0] - cc.r[0], dy = ch.r[1] - cc.r[1], dz = ch.r[2] - cc.r[2];
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
-                        const double bound = (thr + h_hl[h] + hl) * (1 + 1e-9) + 1e-9;
-                        if (r2 <= bound * bound && sqrt(r2) < pre) {
-                            const int pos = atomicAdd(&s_qn, 1);
-                            queue[pos] = (uint16_t)((hh << 12) | t);
+                        const double bound = (thr + ch.hl + cc.hl) * (1 + 1e-9) + 1e-9;
+                        if (!(r2 <= bound * bound && sqrt(r2) < pre)) continue;
+                        double d[3], sv, tv;
+                        if (home_is_i) dist_between_cells(ch, cc, d, sv, tv);
+                        else dist_between_cells(cc, ch, d, sv, tv);
+                        if (norm3(d[0], d[1], d[2]) < thr) {
+                            // high half: rows of i (i-role); low half: rows where j is the partner
+                            const int slot = (int)(atomicAdd(&cnt64[i], 1ull << 32) >> 32);
+                            const int slot2 = (int)(atomicAdd(&cnt64[j], 1ull) & 0xffffffffull);
+                            if (slot < slot_cap) {
+                                const int64_t e = (int64_t)slot * stride + i;
+                                const int64_t plane = (int64_t)slot_cap * stride;
+                                slot_j[e] = j;
+                                slot_s2[e] = slot2;
+                                slot_g[e] = d[0]; slot_g[plane + e] = d[1]; slot_g[2 * plane + e] = d[2];
+                                slot_g[3 * plane + e] = sv; slot_g[4 * plane + e] = tv;
+                            }
+                            if (slot2 < slot_cap) jslot[(int64_t)slot2 * stride + j] = i;
                         }
                     }
+                    if (tid == 0) tested += (unsigned long long)qn;
+                    __syncthreads();
                 }
-                __syncthreads();
-                const int qn = s_qn;
-                for (int e = tid; e < qn; e += 256) {
-                    const int ent = queue[e];
-                    const int h = b0 + (ent >> 12), t = ent & 4095;
-                    Seg ci, cj;
-                    ci.r[0] = h_x[h]; ci.r[1] = h_y[h]; ci.r[2] = h_z[h];
-                    ci.n[0] = h_nx[h]; ci.n[1] = h_ny[h]; ci.n[2] = h_nz[h]; ci.hl = h_hl[h];
-                    const int q = c_q[t];
-                    cj.r[0] = c_x[t]; cj.r[1] = c_y[t]; cj.r[2] = c_z[t];
-                    cj.n[0] = sorted[3 * stride + q]; cj.n[1] = sorted[4 * stride + q];
-                    cj.n[2] = sorted[5 * stride + q]; cj.hl = c_hl[t];
-                    double d[3], sv, tv;
-                    dist_between_cells(ci, cj, d, sv, tv);
-                    if (norm3(d[0], d[1], d[2]) < thr) {
-                        const int slot = atomicAdd(&h_cnt[h], 1);
-                        if (slot < slot_cap) {
-                            const int64_t base = (int64_t)h_id[h] * slot_cap + slot;
-                            slot_j[base] = c_id[t];
-                            double* gp = slot_g + base * 5;
-                            gp[0] = d[0]; gp[1] = d[1]; gp[2] = d[2]; gp[3] = sv; gp[4] = tv;
-                        }
-                    }
-                }
-                if (tid == 0) tested += (unsigned long long)qn;
+                if (!overflow) break;
                 __syncthreads();
             }
         }
@@ -568,47 +510,77 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
     }
     atomicAdd(&s_screened, screened);
     __syncthreads();
-    if (tid < H) {
-        int c = h_cnt[tid];
-        if (c > slot_cap) { atomicOr(status, ST_SLOT_OVERFLOW); c = slot_cap; }
-        count[h_id[tid]] = c;
-    }
     if (tid == 0) {
         atomicAdd(&counters[0], s_screened);
         atomicAdd(&counters[1], tested);
     }
 }
 
-// Packs each cell's slots into the global row order: rows of cell i start at
-// offsets[i]; within the cell, ascending partner index (rank by comparison).
-__global__ void k_compact(int n, int slot_cap, const int* __restrict__ count,
-                          const int* __restrict__ offsets, const int* __restrict__ slot_j,
-                          const double* __restrict__ slot_g, int64_t max_pairs,
-                          int* __restrict__ pi, int* __restrict__ pj, double* __restrict__ pdx,
-                          double* __restrict__ pdy, double* __restrict__ pdz,
-                          double* __restrict__ ps, double* __restrict__ pt,
-                          int* __restrict__ d_m, int* __restrict__ status)
+// Packs the slots into the reference's row order and builds the per-cell
+// force lists. Rows of cell i start at hi(off64[i]) and are sorted by
+// partner j (rank by comparison). The force list of cell c (starting at
+// hi(off64[c]) + lo(off64[c])) holds the rows where c is i (ascending j, i.e.
+// ascending row) followed by the rows where c is j (ascending i), i.e. all
+// rows touching c in row order -- the order the reference's sequential pair
+// loop adds them (mechanics.hpp:1438-1647); bit 31 marks the j role.
+// Pass 1 (per cell i): rows, i-role entries, and the row index of each pair
+// stored at the pair's slot in j's partner list (jrow).
+__global__ void k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+                               const unsigned long long* __restrict__ off64, const int* __restrict__ slot_j,
+                               const int* __restrict__ slot_s2, const double* __restrict__ slot_g,
+                               int64_t max_pairs, int* __restrict__ pi, int* __restrict__ pj,
+                               double* __restrict__ pdx, double* __restrict__ pdy, double* __restrict__ pdz,
+                               double* __restrict__ ps, double* __restrict__ pt, int* __restrict__ csr,
+                               int* __restrict__ jrow, int* __restrict__ d_m, int* __restrict__ status)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int c = count[i];
-        int64_t base = (int64_t)i * slot_cap;
-        int off = offsets[i];
-        for (int e = 0; e < c; ++e) {
-            int je = slot_j[base + e];
-            int rank = 0;
-            for (int f = 0; f < c; ++f) rank += slot_j[base + f] < je;
-            int64_t row = (int64_t)off + rank;
-            if (row < max_pairs) {
-                const double* g = slot_g + (base + e) * 5;
-                pi[row] = i; pj[row] = je;
-                pdx[row] = g[0]; pdy[row] = g[1]; pdz[row] = g[2];
-                ps[row] = g[3]; pt[row] = g[4];
-            }
+    const int64_t plane = (int64_t)slot_cap * stride;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const unsigned long long cv = cnt64[c], ov = off64[c];
+        int ci = (int)(cv >> 32);
+        const int cj = (int)(cv & 0xffffffffull);
+        if (ci > slot_cap || cj > slot_cap) {
+            atomicOr(status, ST_SLOT_OVERFLOW);
+            ci = min(ci, slot_cap);
         }
-        if (i == n - 1) {
-            int64_t m = (int64_t)offsets[n];
+        const int64_t oi = (int64_t)(ov >> 32), base2 = oi + (int64_t)(ov & 0xffffffffull);
+        const bool fits = oi + ci <= max_pairs;
+        for (int e = 0; e < ci; ++e) {
+            const int64_t se = (int64_t)e * stride + c;
+            const int je = slot_j[se];
+            int rank = 0;
+            for (int f = 0; f < ci; ++f) rank += slot_j[(int64_t)f * stride + c] < je;
+            const int64_t row = oi + rank;
+            if (!fits) continue;
+            pi[row] = c; pj[row] = je;
+            pdx[row] = slot_g[se]; pdy[row] = slot_g[plane + se]; pdz[row] = slot_g[2 * plane + se];
+            ps[row] = slot_g[3 * plane + se]; pt[row] = slot_g[4 * plane + se];
+            csr[base2 + rank] = (int)row;
+            const int s2 = slot_s2[se];
+            if (s2 < slot_cap) jrow[(int64_t)s2 * stride + je] = (int)row;
+        }
+        if (c == n - 1) {
+            int64_t m = (int64_t)(off64[n] >> 32);
             if (m > max_pairs) { atomicOr(status, ST_PAIR_OVERFLOW); m = max_pairs; }
             *d_m = (int)m;
+        }
+    }
+}
+
+// Pass 2 (per cell c): j-role entries sorted by partner i.
+__global__ void k_compact_jrole(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+                                const unsigned long long* __restrict__ off64, const int* __restrict__ jslot,
+                                const int* __restrict__ jrow, int64_t max_pairs, int* __restrict__ csr)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const unsigned long long cv = cnt64[c], ov = off64[c];
+        const int ci = min((int)(cv >> 32), slot_cap), cj = min((int)(cv & 0xffffffffull), slot_cap);
+        const int64_t base2 = (int64_t)(ov >> 32) + (int64_t)(ov & 0xffffffffull);
+        if ((int64_t)(off64[n] >> 32) > max_pairs) continue;
+        for (int e = 0; e < cj; ++e) {
+            const int ie = jslot[(int64_t)e * stride + c];
+            int rank = 0;
+            for (int f = 0; f < cj; ++f) rank += jslot[(int64_t)f * stride + c] < ie;
+            csr[base2 + ci + rank] = jrow[(int64_t)e * stride + c] | (int)0x80000000u;
         }
     }
 }
@@ -711,10 +683,12 @@ struct ForceParams {
 };
 
 // Per-pair force magnitudes (the scalar factors of cellCellInteractionForces,
-// mechanics.hpp:1209-1350 and :1390-1659): mag[0][k] multiplies the unit
-// distance vector of the main term (Hertz, EPS-capped Hertz or JKR
-// term1 - term2), mag[1][k] that of the separately accumulated hard-core term
-// (adhesive overload only; 0 otherwise). Pairs without overlap get 0.
+// mechanics.hpp:1209-1350 and :1390-1659), written with the pair geometry as
+// one 64-byte record per row: {dx, dy, dz, s_i, s_j, main, hard, 0}. main
+// multiplies the unit distance vector of the main term (Hertz, EPS-capped
+// Hertz or JKR term1 - term2), hard that of the separately accumulated
+// hard-core term (adhesive overload only; 0 otherwise). Rows without overlap
+// get 0 and are skipped by the gather.
 __global__ void __launch_bounds__(256)
 k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int ld,
               const int* __restrict__ pi, const int* __restrict__ pj,
@@ -722,15 +696,16 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
               const double* __restrict__ pdz, ForceParams fp,
               const double* __restrict__ tab_ov, const double* __restrict__ tab_g,
               const double* __restrict__ tab_r, int tab_nd, int tab_ng,
-              const double* __restrict__ sur_radius,
-              double* __restrict__ mag_main, double* __restrict__ mag_hard,
+              const double* __restrict__ ps, const double* __restrict__ pt,
+              const double* __restrict__ sur_radius, double* __restrict__ rec,
               int* __restrict__ jcnt, int* __restrict__ status)
 {
     const int m = *d_m;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
         const int i = pi[k], j = pj[k];
         if (jcnt) atomicAdd(&jcnt[j], 1);
-        const double dist = norm3(pdx[k], pdy[k], pdz[k]);
+        const double dx = pdx[k], dy = pdy[k], dz = pdz[k];
+        const double dist = norm3(dx, dy, dz);
         const double overlap = 2 * fp.R - dist;
         double mm = 0, mh = 0;
         if (overlap > 0) {
@@ -766,8 +741,11 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
                     mh = fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
             }
         }
-        mag_main[k] = mm;
-        mag_hard[k] = mh;
+        double2* r = reinterpret_cast<double2*>(rec + (int64_t)k * 8);
+        r[0] = make_double2(dx, dy);
+        r[1] = make_double2(dz, ps[k]);
+        r[2] = make_double2(pt[k], mm);
+        r[3] = make_double2(mh, 0.0);
     }
 }
 
@@ -779,47 +757,77 @@ struct VelParams {
     int has_noise;
 };
 
-// dE/dq of the cell-cell interaction for cell c: the rows where c is i
-// (ranges ioff, optionally through perm_i) and where c is j (perm_j, sorted)
-// are merged in row order, so each component is accumulated in the order the
-// reference's sequential loop adds it (mechanics.hpp:1438-1647).
-__device__ __forceinline__ void gather_cc(int c, const int* __restrict__ ioff, const int* __restrict__ perm_i,
-                                          const int* __restrict__ joff, const int* __restrict__ perm_j,
-                                          const double* __restrict__ pdx, const double* __restrict__ pdy,
-                                          const double* __restrict__ pdz, const double* __restrict__ ps,
-                                          const double* __restrict__ pt, const double* __restrict__ mag_main,
-                                          const double* __restrict__ mag_hard, double R, double acc[6])
+// Adds one row's contribution to dE/dq of one of its cells (row i: +w,
+// s_i w; row j: -w, -s_j w; then the hard-core term the same way), exactly as
+// the reference's sequential loop adds forces.row(0/1) and forces_hard
+// (mechanics.hpp:1589-1591, 1643-1644).
+__device__ __forceinline__ void add_row(const double* __restrict__ rec, int k, bool role_i, double R,
+                                        double acc[6])
 {
+    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)k * 8);
+    const double2 a = r[0], b = r[1], c = r[2], d = r[3];
+    const double dx = a.x, dy = a.y, dz = b.x;
+    const double dist = norm3(dx, dy, dz);
+    const double overlap = 2 * R - dist;
+    if (!(overlap > 0)) return;
+    const double dn[3] = {dx / dist, dy / dist, dz / dist};
+    const double mm = c.y;
+    const double st = role_i ? b.y : -c.x;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) acc[q] = 0;
-    int a = ioff[c], ae = ioff[c + 1], b = joff[c], be = joff[c + 1];
-    while (a < ae || b < be) {
-        int ka = (a < ae) ? (perm_i ? perm_i[a] : a) : 0x7fffffff;
-        int kb = (b < be) ? perm_j[b] : 0x7fffffff;
-        const bool role_i = ka < kb;
-        const int k = role_i ? ka : kb;
-        if (role_i) ++a; else ++b;
-        const double dx = pdx[k], dy = pdy[k], dz = pdz[k];
-        const double dist = norm3(dx, dy, dz);
-        const double overlap = 2 * R - dist;
-        if (!(overlap > 0)) continue;
-        const double dn[3] = {dx / dist, dy / dist, dz / dist};
-        const double mm = mag_main[k];
-        const double st = role_i ? ps[k] : -pt[k];
+    for (int q = 0; q < 3; ++q) {
+        const double w = mm * dn[q];
+        acc[q] += role_i ? w : -w;
+        acc[q + 3] += st * w;
+    }
+    const double mh = d.x;
+    if (mh != 0) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const double w = mm * dn[q];
+            const double w = mh * dn[q];
             acc[q] += role_i ? w : -w;
             acc[q + 3] += st * w;
         }
-        const double mh = mag_hard[k];
-        if (mh != 0) {
+    }
+}
+
+// Row lists of a cell. CSR form (lists built by the device neighbor search):
+// one run of row indices in row order, bit 31 = j role. Merge form (lists
+// imported from the caller, in any row order): rows where c is i (ioff,
+// optionally through perm_i) and where c is j (joff / perm_j, ascending),
+// merged by row index. Either way each component is accumulated in the
+// order of the rows, as the reference's sequential loop does
+// (mechanics.hpp:1438-1647).
+struct RowLists {
+    const unsigned long long* off64;
+    const int* csr;
+    const int* ioff;
+    const int* perm_i;
+    const int* joff;
+    const int* perm_j;
+};
+
+template <bool CSR>
+__device__ __forceinline__ void gather_cc(int c, const RowLists& L, const double* __restrict__ rec, double R,
+                                          double acc[6])
+{
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const double w = mh * dn[q];
-                acc[q] += role_i ? w : -w;
-                acc[q + 3] += st * w;
-            }
+    for (int q = 0; q < 6; ++q) acc[q] = 0;
+    if constexpr (CSR) {
+        const unsigned long long o0 = L.off64[c], o1 = L.off64[c + 1];
+        const int64_t e0 = (int64_t)(o0 >> 32) + (int64_t)(o0 & 0xffffffffull);
+        const int64_t e1 = (int64_t)(o1 >> 32) + (int64_t)(o1 & 0xffffffffull);
+        for (int64_t e = e0; e < e1; ++e) {
+            const int ent = L.csr[e];
+            add_row(rec, ent & 0x7fffffff, ent >= 0, R, acc);
+        }
+    } else {
+        int a = L.ioff[c], ae = L.ioff[c + 1], b = L.joff[c], be = L.joff[c + 1];
+        while (a < ae || b < be) {
+            const int ka = (a < ae) ? (L.perm_i ? L.perm_i[a] : a) : 0x7fffffff;
+            const int kb = (b < be) ? L.perm_j[b] : 0x7fffffff;
+            const bool role_i = ka < kb;
+            if (role_i) ++a; else ++b;
+            add_row(rec, role_i ? ka : kb, role_i, R, acc);
         }
     }
 }
@@ -859,20 +867,15 @@ __device__ __forceinline__ void cell_velocity(const VelParams& vp, const CellVel
 }
 
 // getVelocities for every cell (mechanics.hpp:2011-2347).
+template <bool CSR>
 __global__ void __launch_bounds__(256)
 k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restrict__ assume_2d,
-             const double* __restrict__ noise, VelParams vp,
-             const int* __restrict__ ioff, const int* __restrict__ perm_i,
-             const int* __restrict__ joff, const int* __restrict__ perm_j,
-             const double* __restrict__ pdx, const double* __restrict__ pdy,
-             const double* __restrict__ pdz, const double* __restrict__ ps,
-             const double* __restrict__ pt, const double* __restrict__ mag_main,
-             const double* __restrict__ mag_hard, double* __restrict__ vel,
-             double* __restrict__ cc_out, int* __restrict__ status)
+             const double* __restrict__ noise, VelParams vp, RowLists L, const double* __restrict__ rec,
+             double* __restrict__ vel, double* __restrict__ cc_out, int* __restrict__ status)
 {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         double cc[6];
-        gather_cc(c, ioff, perm_i, joff, perm_j, pdx, pdy, pdz, ps, pt, mag_main, mag_hard, vp.sc.R, cc);
+        gather_cc<CSR>(c, L, rec, vp.sc.R, cc);
         if (cc_out) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) cc_out[(int64_t)q * n + c] = cc[q];
@@ -1074,10 +1077,18 @@ void grow_slots(bfm_ctx* c)
     double* sg = nullptr;
     HIPCHK(hipMalloc((void**)&sj, sizeof(int) * (size_t)c->max_cells * cap));
     HIPCHK(hipMalloc((void**)&sg, sizeof(double) * (size_t)c->max_cells * cap * 5));
-    HIPCHK(hipFree(c->d_slot_j));
-    HIPCHK(hipFree(c->d_slot_g));
+    int *js = nullptr, *jr = nullptr, *s2 = nullptr;
+    HIPCHK(hipMalloc((void**)&js, sizeof(int) * (size_t)c->max_cells * cap));
+    HIPCHK(hipMalloc((void**)&jr, sizeof(int) * (size_t)c->max_cells * cap));
+    HIPCHK(hipMalloc((void**)&s2, sizeof(int) * (size_t)c->max_cells * cap));
+    for (void* q : {(void*)c->d_slot_j, (void*)c->d_slot_g, (void*)c->d_jslot, (void*)c->d_jrow,
+                    (void*)c->d_slot_s2})
+        HIPCHK(hipFree(q));
     c->d_slot_j = sj;
     c->d_slot_g = sg;
+    c->d_jslot = js;
+    c->d_jrow = jr;
+    c->d_slot_s2 = s2;
     c->slot_cap = cap;
 }
 
@@ -1153,6 +1164,21 @@ void scan(bfm_ctx* c, const int* in, int* out, int64_t items, const char* name)
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, in, out, (int)items, c->stream));
 }
 
+void launch_candidates(bfm_ctx* c, int n, int64_t stride)
+{
+#define BFM_CAND(HOMES, CAP, BATCH, QCAP)                                                              \
+    KLAUNCH(c, "candidates", (k_candidates_lds<HOMES, CAP, BATCH, QCAP>), (n + HOMES - 1) / HOMES, 256, 0, n, \
+            c->d_plan, c->d_sorted, stride, c->d_sid, c->d_skey, c->d_bin_off, c->slot_cap, c->d_cnt64,   \
+            c->d_slot_j, c->d_slot_s2, c->d_slot_g, c->d_jslot, c->d_counters)
+    switch (c->cand_variant) {
+        case 0: BFM_CAND(32, 640, 8, 8192); break;
+        case 2: BFM_CAND(64, 1024, 8, 8192); break;
+        case 3: BFM_CAND(64, 512, 8, 4096); break;
+        default: BFM_CAND(64, 640, 8, 8192); break;
+    }
+#undef BFM_CAND
+}
+
 // Device neighbor build into the context's internal list.
 void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv)
 {
@@ -1160,12 +1186,12 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     c->list_n = n;
     c->list_sorted_i = true;
     c->use_perm_i = false;
-    c->ioff_valid = true;
     if (n <= 1) {
         int zero = 0;
         HIPCHK(hipMemcpyAsync(c->d_m, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
-        KLAUNCH(c, "fill", k_fill_int, 1, 256, 0, c->d_offsets, 0, (int64_t)(n + 1));
+        KLAUNCH(c, "fill", k_fill_u64, 1, 64, 0, c->d_off64, (int64_t)(n + 1));
         c->h_m = 0;
+        c->list_csr = true;
         return;
     }
     const double pre = thr + Ldiv + 2 * R;
@@ -1176,50 +1202,26 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
             (int64_t)c->max_bins + 1);
     KLAUNCH(c, "keys", k_keys, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_keys, c->d_vals,
             c->d_bin_count);
-    {
-        LaunchScope ls(c, "radix_sort");
-        size_t tb = c->temp_bytes;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->d_temp, tb, c->d_keys, c->d_keys_alt, c->d_vals,
-                                                  c->d_vals_alt, n, 0, nbits(c->max_bins), c->stream));
-    }
     scan(c, c->d_bin_count, c->d_bin_off, (int64_t)c->max_bins + 1, "scan_bins");
+    HIPCHK(hipMemcpyAsync(c->d_bin_count, c->d_bin_off, sizeof(int) * ((size_t)c->max_bins + 1),
+                          hipMemcpyDeviceToDevice, c->stream));
     const int64_t stride = c->max_cells;
-    KLAUNCH(c, "reorder", k_reorder, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_vals_alt, c->d_sorted,
-            c->d_sid, stride);
-    HIPCHK(hipMemcpyAsync(c->d_skey, c->d_keys_alt, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
-    if (c->cand_variant == 1) {
-        constexpr int HOMES = 32;
-        const int g = (n + HOMES - 1) / HOMES;
-        KLAUNCH(c, "candidates", (k_candidates_lds<HOMES, 640, 8>), g, 256, 0, n, c->d_plan, c->d_sorted,
-                stride, c->d_sid, c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j,
-                c->d_slot_g, c->d_counters, c->d_status);
-    } else {
-        const int waves = std::max(1, c->block_threads / 64);
-        const int homes_per_wg = waves * kHomesPerWave;
-        const int g = (n + homes_per_wg - 1) / homes_per_wg;
-        if (waves == 1)
-            KLAUNCH(c, "candidates", k_candidates<1>, g, 64, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                    c->d_status);
-        else if (waves == 2)
-            KLAUNCH(c, "candidates", k_candidates<2>, g, 128, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                    c->d_status);
-        else if (waves == 4)
-            KLAUNCH(c, "candidates", k_candidates<4>, g, 256, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                    c->d_status);
-        else
-            KLAUNCH(c, "candidates", k_candidates<8>, g, 512, 0, n, c->d_plan, c->d_sorted, stride, c->d_sid,
-                    c->d_skey, c->d_bin_off, c->slot_cap, c->d_count, c->d_slot_j, c->d_slot_g, c->d_counters,
-                    c->d_status);
+    KLAUNCH(c, "scatter", k_scatter, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_bin_count,
+            c->d_sorted, c->d_sid, c->d_skey, stride);
+    KLAUNCH(c, "fill", k_fill_u64, grid_for(n + 1, 256, 4096), 256, 0, c->d_cnt64, (int64_t)n + 1);
+    launch_candidates(c, n, stride);
+    {
+        LaunchScope ls(c, "scan_counts");
+        size_t tb = c->temp_bytes;
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, c->d_cnt64, c->d_off64, n + 1, c->stream));
     }
-    KLAUNCH(c, "fill", k_fill_int, 1, 64, 0, c->d_count + n, 0, (int64_t)1);
-    scan(c, c->d_count, c->d_offsets, (int64_t)n + 1, "scan_counts");
-    KLAUNCH(c, "compact", k_compact, grid_for(n, 256), 256, 0, n, c->slot_cap, c->d_count, c->d_offsets,
-            c->d_slot_j, c->d_slot_g, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_m,
-            c->d_status);
+    KLAUNCH(c, "compact_rows", k_compact_rows, grid_for(n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64, c->d_off64,
+            c->d_slot_j, c->d_slot_s2, c->d_slot_g, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps,
+            c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status);
+    KLAUNCH(c, "compact_jrole", k_compact_jrole, grid_for(n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64,
+            c->d_off64, c->d_jslot, c->d_jrow, c->max_pairs, c->d_csr);
     c->h_m = -1;
+    c->list_csr = true;
 }
 
 // Synchronous build: grows the per-cell slot capacity until the list fits.
@@ -1244,34 +1246,44 @@ void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
                                const double* sur_radius)
 {
     ForceParams fp = force_params(c);
-    if (!c->transposed)
+    const bool need_t = !c->list_csr && !c->transposed;
+    if (need_t)
         KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
     KLAUNCH(c, "pair_forces", k_pair_forces, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj, c->pdx,
-            c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng, sur_radius,
-            c->d_mag, c->d_mag + c->max_pairs, c->transposed ? (int*)nullptr : c->d_jcnt, c->d_status);
-    if (c->transposed) return;
+            c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng, c->ps, c->pt,
+            sur_radius, c->d_rec, need_t ? c->d_jcnt : (int*)nullptr, c->d_status);
+    if (!need_t) return;
+    // lists imported from the caller: transposes for the merge-form gather
     scan(c, c->d_jcnt, c->d_joff, (int64_t)n + 1, "scan_j");
     KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
     KLAUNCH(c, "fill_perm", k_fill_perm, kPairGrid, 256, 0, c->d_m, c->pj, c->d_joff, c->d_jcnt, c->d_perm_j);
     KLAUNCH(c, "sort_segments", k_sort_segments, grid_for(n, 256), 256, 0, n, c->d_joff, c->d_perm_j);
-    if (!c->ioff_valid) {
+    KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_icnt, 0, (int64_t)n + 1);
+    KLAUNCH(c, "count_i", k_count_key, kPairGrid, 256, 0, c->d_m, c->pi, c->d_icnt);
+    scan(c, c->d_icnt, c->d_offsets, (int64_t)n + 1, "scan_i");
+    if (!c->list_sorted_i) {
         KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_icnt, 0, (int64_t)n + 1);
-        KLAUNCH(c, "count_i", k_count_key, kPairGrid, 256, 0, c->d_m, c->pi, c->d_icnt);
-        scan(c, c->d_icnt, c->d_offsets, (int64_t)n + 1, "scan_i");
-        c->ioff_valid = true;
-        if (!c->list_sorted_i) {
-            KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_icnt, 0, (int64_t)n + 1);
-            KLAUNCH(c, "fill_perm", k_fill_perm, kPairGrid, 256, 0, c->d_m, c->pi, c->d_offsets, c->d_icnt,
-                    c->d_perm_i);
-            KLAUNCH(c, "sort_segments", k_sort_segments, grid_for(n, 256), 256, 0, n, c->d_offsets,
-                    c->d_perm_i);
-        }
+        KLAUNCH(c, "fill_perm", k_fill_perm, kPairGrid, 256, 0, c->d_m, c->pi, c->d_offsets, c->d_icnt,
+                c->d_perm_i);
+        KLAUNCH(c, "sort_segments", k_sort_segments, grid_for(n, 256), 256, 0, n, c->d_offsets, c->d_perm_i);
     }
     c->use_perm_i = !c->list_sorted_i;
     c->transposed = true;
 }
 
-const int* ioff_of(bfm_ctx* c) { return c->d_offsets; }
+RowLists row_lists(bfm_ctx* c)
+{
+    RowLists L;
+    L.off64 = c->d_off64;
+    L.csr = c->d_csr;
+    L.ioff = c->d_offsets;
+    L.perm_i = c->use_perm_i ? c->d_perm_i : nullptr;
+    L.joff = c->d_joff;
+    L.perm_j = c->d_perm_j;
+    return L;
+}
+
+
 
 void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld)
 {
@@ -1290,9 +1302,13 @@ void launch_velocities(bfm_ctx* c, const double* d_cells, int n, int ld, const i
                        const double* d_noise, double* d_vel, double* d_cc)
 {
     VelParams vp = vel_params(c);
-    KLAUNCH(c, "velocities", k_velocities, grid_for(n, 256), 256, 0, n, d_cells, ld, d_a2d, d_noise, vp,
-            ioff_of(c), c->use_perm_i ? c->d_perm_i : (const int*)nullptr, c->d_joff, c->d_perm_j, c->pdx,
-            c->pdy, c->pdz, c->ps, c->pt, c->d_mag, c->d_mag + c->max_pairs, d_vel, d_cc, c->d_status);
+    RowLists L = row_lists(c);
+    if (c->list_csr)
+        KLAUNCH(c, "velocities", k_velocities<true>, grid_for(n, 256), 256, 0, n, d_cells, ld, d_a2d, d_noise,
+                vp, L, c->d_rec, d_vel, d_cc, c->d_status);
+    else
+        KLAUNCH(c, "velocities", k_velocities<false>, grid_for(n, 256), 256, 0, n, d_cells, ld, d_a2d, d_noise,
+                vp, L, c->d_rec, d_vel, d_cc, c->d_status);
 }
 
 void import_host_neighbors(bfm_ctx* c, const double* nb, int64_t m, int64_t ld_nb, int n)
@@ -1318,7 +1334,7 @@ void import_host_neighbors(bfm_ctx* c, const double* nb, int64_t m, int64_t ld_n
     c->list_n = n;
     c->list_sorted_i = sorted;
     c->transposed = false;
-    c->ioff_valid = false;
+    c->list_csr = false;
 }
 
 void upload_cells(bfm_ctx* c, const double* cells, int n, int ld)
@@ -1440,7 +1456,13 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_jcnt, N + 1);
         dalloc(&c->d_perm_i, P);
         dalloc(&c->d_perm_j, P);
-        dalloc(&c->d_mag, 2 * P);
+        dalloc(&c->d_rec, 8 * P);
+        dalloc(&c->d_cnt64, N + 1);
+        dalloc(&c->d_off64, N + 1);
+        dalloc(&c->d_jslot, N * c->slot_cap);
+        dalloc(&c->d_jrow, N * c->slot_cap);
+        dalloc(&c->d_slot_s2, N * c->slot_cap);
+        dalloc(&c->d_csr, 2 * P);
         dalloc(&c->d_cc, 6 * N);
         dalloc(&c->d_k, 7 * 6 * N);
         dalloc(&c->d_errmax, 1);
@@ -1453,7 +1475,9 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
                                                   c->d_vals_alt, max_cells, 0, nbits(c->max_bins)));
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, c->d_bin_count, c->d_bin_off, c->max_bins + 1));
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, c->d_count, c->d_offsets, max_cells + 1));
-        c->temp_bytes = std::max(t1, std::max(t2, t3));
+        size_t t4 = 0;
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, c->d_cnt64, c->d_off64, max_cells + 1));
+        c->temp_bytes = std::max(std::max(t1, t4), std::max(t2, t3));
         HIPCHK(hipMalloc(&c->d_temp, c->temp_bytes));
         HIPCHK(hipDeviceSynchronize());
     });
@@ -1474,7 +1498,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_g, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
-                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_mag, c->d_cc,
+                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_slot_s2, c->d_csr, c->d_cc,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -1705,7 +1729,7 @@ int bfm_dev_import_neighbors(bfm_ctx* c, const double* d_nb, int64_t m, int64_t 
         c->list_n = n;
         c->list_sorted_i = false;   // unknown order: use the general transpose
         c->transposed = false;
-        c->ioff_valid = false;
+        c->list_csr = false;
     });
 }
 
@@ -1780,7 +1804,7 @@ int bfm_kernel_times(bfm_ctx* c, char* names, int names_cap, double* ms, int64_t
     });
 }
 
-int bfm_stats(bfm_ctx* c, int64_t* m, int64_t* tests, double* edge, int* nbins)
+int bfm_stats(bfm_ctx* c, int64_t* m, int64_t* screened, int64_t* tests, double* edge, int* nbins)
 {
     return guarded(c, [&] {
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1791,7 +1815,8 @@ int bfm_stats(bfm_ctx* c, int64_t* m, int64_t* tests, double* edge, int* nbins)
         Plan P;
         HIPCHK(hipMemcpy(&P, c->d_plan, sizeof(Plan), hipMemcpyDeviceToHost));
         if (m) *m = mm;
-        if (tests) { tests[0] = (int64_t)ctr[1]; }
+        if (screened) *screened = (int64_t)ctr[0];
+        if (tests) *tests = (int64_t)ctr[1];
         if (edge) *edge = P.edge;
         if (nbins) *nbins = P.nbins;
         HIPCHK(hipMemset(c->d_counters, 0, sizeof(ctr)));
@@ -1802,7 +1827,7 @@ int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads, int kernel_v
 {
     return guarded(c, [&] {
         if (kernel_variant >= 0) {
-            require(kernel_variant <= 1, BFM_ERR_ARG, "kernel_variant must be 0 or 1");
+            require(kernel_variant <= 3, BFM_ERR_ARG, "kernel_variant must be 0..3");
             c->cand_variant = kernel_variant;
         }
         if (bin_scale > 0) {

@@ -176,15 +176,17 @@ int bfm_kernel_times(bfm_ctx* ctx, char* names_out, int names_cap, double* ms_ou
                      int64_t* counts_out, int cap, int* n_kernels);
 int bfm_reset_timing(bfm_ctx* ctx);
 
-/* Number of pairs in the internal list, candidate segment tests performed by
- * the last neighbor build, and the last build's bin geometry. */
-int bfm_stats(bfm_ctx* ctx, int64_t* m, int64_t* candidate_tests, double* bin_edge,
+/* Number of pairs in the internal list; candidate pairs screened (center
+ * distance bound) and segment-segment tests run by the neighbor builds since
+ * the previous call (the counters are reset); the last build's bin geometry. */
+int bfm_stats(bfm_ctx* ctx, int64_t* m, int64_t* screened, int64_t* tested, double* bin_edge,
               int* nbins);
 
 /* Tuning knobs of the neighbor build: bin edge as a multiple of the search
  * cutoff (1/3 .. 4; 1 gives the 27-bin stencil, 0.5 the 125-bin stencil),
- * threads per workgroup of the wave-queue candidate kernel, and the candidate
- * kernel (0 = wave-queue, 1 = LDS-staged). 0 / 0 / -1 keep the current value.
+ * threads per workgroup (reserved, 0), and the candidate-kernel configuration
+ * 0..3 (homes per workgroup, LDS chunk, batch). 0 / 0 / -1 keep the current
+ * values.
  * No knob changes the neighbor list, only the time to build it. */
 int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads, int kernel_variant);
 
