@@ -116,6 +116,9 @@ def main():
         from bfm.jkr import reference_table
         ov, rad = reference_table(R, Rcell, params.E0, 100.0)
         ctx.set_jkr_table(ov, rad)
+    if args.jkr == "surrogate":
+        from bfm import surrogate as sur
+        sur.install(ctx, sur.load())
     stream = torch.cuda.Stream()
     ctx.set_stream(stream.cuda_stream)
     with torch.cuda.stream(stream):
@@ -197,6 +200,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "stage_hbm_frac": (stage_bytes / (ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
+        "jkr_path_us": round(1e3 * (per_kernel.get("pair_forces", 0) + per_kernel.get("sur_radius", 0)), 2),
         "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(per_kernel.items(),
                                                               key=lambda x: -x[1])},
         "wall_s_timed": wall,

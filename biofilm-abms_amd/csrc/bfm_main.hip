@@ -129,6 +129,17 @@ constexpr int kPairGrid = 2048;   // grid-stride launches over device-sized pair
 
 }  // namespace
 
+namespace bfm {
+
+void launch_timed(bfm_ctx* c, const char* name, const std::function<void()>& launch)
+{
+    LaunchScope ls(c, name);
+    launch();
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace bfm
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------

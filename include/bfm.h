@@ -101,6 +101,27 @@ int bfm_set_surrogate(bfm_ctx* ctx, const uint16_t* w1, const float* b1,
                       const uint16_t* w4, const float* b4,
                       const double* envelope);
 
+/* Surrogate training and evaluation (no counterpart in the reference: the
+ * learned JKR radius is an addition of this implementation). Parameters are
+ * FP32 masters in the order W1[64][4] b1[64] W2[64][64] b2[64] W3[64][64]
+ * b3[64] W4[64] b4[1] (8705 floats). */
+int bfm_surrogate_set_polish(bfm_ctx* ctx, int polish_steps);
+/* Trains on n_samples seeded samples of the exact FP64 solve (overlap
+ * U[-0.2 R, 0.3 R], gamma U[10, 1000] Pa um, samples without a JKR root
+ * excluded), Adam with cosine-decayed lr; installs the weights. */
+int bfm_surrogate_train(bfm_ctx* ctx, int n_samples, int epochs, int batch, double lr,
+                        uint64_t seed, float* params_out, float* loss_per_epoch_out);
+/* Network output t (= sqrt(a) / p^(1/3)) and the radius after `polish` FP64
+ * Newton steps, for (delta, gamma) samples. */
+int bfm_surrogate_eval(bfm_ctx* ctx, const double* delta, const double* gamma, int n,
+                       int polish, float* t_out, double* radius_out);
+/* Loss, outputs and parameter gradient of one forward/backward pass on the GPU
+ * (features X[B][4], targets T[B]); for checking the training kernels. */
+int bfm_surrogate_grad(bfm_ctx* ctx, const float* params, const float* X, const float* T,
+                       int B, float* loss_out, float* y_out, float* grad_out);
+/* The seeded training samples (features X[n][4], targets T[n], valid flags). */
+int bfm_surrogate_generate(bfm_ctx* ctx, int n, uint64_t seed, float* X, float* T, int* ok);
+
 /* ---- neighbor search -------------------------------------------------- */
 /* getCellNeighbors (mechanics.hpp:520-593). Writes m rows into neighbors_out
  * (leading dimension ld_out). If m > ld_out, returns BFM_ERR_CAPACITY and sets

@@ -1,0 +1,126 @@
+"""JKR surrogate on the GPU: MFMA forward and backward kernels against the CPU
+reference network (oracle/mlp_ref.py), training, and the accuracy of the
+surrogate radius and forces against the exact FP64 solve of the oracle.
+
+Force error metric: |F_sur - F_exact| / (|term1| + |term2|) per pair, where
+F = term1 - term2 = 4 E0 a^3 / (3 Req) - 4 sqrt(pi a^3 gamma E0)
+(adhesion.hpp:206-212); the bar is 1e-4 inside the training envelope."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+E0, R, REQ = 3900.0, 0.8, 0.4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import bfm
+    return bfm.Context(20000, params=bfm.core.make_params(jkr_mode=bfm.core.JKR_SURROGATE))
+
+
+def _force(a, g):
+    a3 = a ** 3
+    t1 = 4 * E0 * a3 / (3 * REQ)
+    t2 = 4 * np.sqrt(np.pi * a3 * g * E0)
+    return t1 - t2, np.abs(t1) + np.abs(t2)
+
+
+def test_gradients_match_cpu_oracle(ctx):
+    from bfm import surrogate as sur
+    from oracle import mlp_ref
+    rng = np.random.default_rng(1)
+    params = rng.normal(scale=0.25, size=mlp_ref.NPARAMS).astype(np.float32)
+    X = rng.uniform(-1, 1, (512, 4)).astype(np.float32)
+    T = rng.uniform(0.6, 1.4, 512).astype(np.float32)
+    loss, y, g = sur.grad(ctx, params, X, T)
+    loss_r, y_r, g_r = mlp_ref.loss_and_grad(params.astype(np.float64), X.astype(np.float64), T)
+    assert abs(loss - loss_r) <= 1e-2 * loss_r
+    assert np.max(np.abs(y - y_r)) <= 2e-2 * np.max(np.abs(y_r))
+    off = 0
+    for name, shape in mlp_ref.SIZES:
+        k = int(np.prod(shape))
+        a, b = g[off:off + k], g_r[off:off + k]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        print(f"{name}: rel grad err {err:.2e}", flush=True)
+        assert err < 3e-2, name
+        off += k
+
+
+def test_training_reduces_loss(ctx):
+    from bfm import surrogate as sur
+    params, losses = sur.train(ctx, n_samples=200_000, epochs=8, batch=2048, lr=1e-3, seed=3)
+    print("losses", losses, flush=True)
+    # targets t lie in about [0.6, 1.5] (variance ~0.04); a trained network is far below
+    assert np.all(np.isfinite(losses)) and losses[-1] < 0.25 * losses[0] and losses[-1] < 5e-5
+
+
+def _trained_params(ctx):
+    from bfm import surrogate as sur
+    if os.path.exists(sur.DEFAULT_WEIGHTS):
+        return sur.load()
+    params, _ = sur.train(ctx, n_samples=1_000_000, epochs=60, batch=4096, lr=1e-3, seed=42)
+    return params
+
+
+def test_surrogate_accuracy_and_gain(ctx):
+    """Inside the envelope (contacts: 0 < delta <= 0.3 R, 10 <= gamma <= 1000):
+    network + one FP64 Newton step meets 1e-4; without the network (the same
+    Newton step from the closed-form start, or the reference's 100-point table)
+    the error is far larger."""
+    from bfm import surrogate as sur
+    from bfm.jkr import contact_radius, reference_table, reference_table_2d
+    from oracle import oracle as orc
+    sur.install(ctx, _trained_params(ctx))
+    rng = np.random.default_rng(5)
+    d = rng.uniform(1e-4, 0.3 * R, 20000)
+    g = np.exp(rng.uniform(np.log(10.0), np.log(1000.0), 20000))
+    exact = np.array([orc.jkr_contact_radius(a, REQ, E0, b)[0] for a, b in zip(d, g)])
+    f_ex, scale = _force(exact, g)
+    t0, r0 = sur.evaluate(ctx, d, g, polish=0)
+    t1, r1 = sur.evaluate(ctx, d, g, polish=1)
+    err0 = np.max(np.abs(_force(r0, g)[0] - f_ex) / scale)
+    err1 = np.max(np.abs(_force(r1, g)[0] - f_ex) / scale)
+    # without the network: one Newton step from the closed-form upper bound
+    p = REQ * np.sqrt(4 * np.pi * g / E0)
+    q = REQ * d
+    u = np.maximum(np.cbrt(2 * p), np.sqrt(np.sqrt(2 * q)))
+    u = u - ((u ** 4 - p * u) - q) / (4 * u ** 3 - p)
+    err_nn = np.max(np.abs(_force(u * u, g)[0] - f_ex) / scale)
+    # the reference's production tables: 100-point overlap table at fixed gamma = 100
+    # (simulation.hpp:1396-1404) and the 100 x 100 (overlap, gamma) table for variable gamma
+    # (:1383-1395), here over gamma in [0, 1000]
+    ov, rad = reference_table(R, 0.5, E0, 100.0)
+    tab = np.array([orc.table_radius_1d(ov, rad, x) for x in d])
+    ex100 = contact_radius(d, REQ, E0, 100.0)
+    err_tab1 = np.max(np.abs(_force(tab, 100.0)[0] - _force(ex100, 100.0)[0]) / _force(ex100, 100.0)[1])
+    ov2, gm2, rad2 = reference_table_2d(R, 0.5, E0, 1000.0)
+    flat = np.ascontiguousarray(rad2.ravel())
+    tab2 = np.array([orc.table_radius_2d(ov2, gm2, flat, x, y) for x, y in zip(d, g)])
+    err_tab2 = np.max(np.abs(_force(tab2, g)[0] - f_ex) / scale)
+    print(f"force err: network+1 Newton {err1:.2e}, network only {err0:.2e}, "
+          f"1 Newton from bound (no network) {err_nn:.2e}, 100-pt table at fixed gamma "
+          f"{err_tab1:.2e}, 100x100 (overlap, gamma) table {err_tab2:.2e}", flush=True)
+    assert err1 <= 1e-4
+    assert err_nn > 10 * err1        # what the network gains at equal Newton work
+    assert err_tab2 > err1           # variable gamma: better than the reference's 2-D table
+
+
+def test_velocities_with_surrogate(ctx, orc):
+    import bfm
+    from bfm import surrogate as sur
+    from gpu_common import LDIV, THR, rel_err
+    cells, _ = bfm.make_dome(12000, seed=41)
+    nb = orc.get_cell_neighbors(cells, THR, R, LDIV, binned=True)
+    p_sur = bfm.core.make_params(jkr_mode=bfm.core.JKR_SURROGATE)
+    c = bfm.Context(12000, params=p_sur)
+    sur.install(c, _trained_params(ctx))
+    a2d = (cells[:, 6] < p_sur.nz_threshold).astype(np.int32)
+    v = c.get_velocities(cells, nb, a2d)
+    v_ref = orc.get_velocities(cells, nb, a2d, None, bfm.core.make_params())
+    err = rel_err(v, v_ref)
+    print(f"velocity rel err with surrogate: {err:.2e}", flush=True)
+    assert err <= 1e-4
