@@ -42,6 +42,13 @@ def parse():
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--workload", default="stage", choices=["stage", "growth"],
+                    help="stage: the RK-stage bench (default); growth: the founder-to-N growth run")
+    ap.add_argument("--growth-cells", type=int, default=100000)
+    ap.add_argument("--growth-tableau", default="bogacki_shampine")
+    ap.add_argument("--growth-max-wall", type=float, default=900.0)
+    ap.add_argument("--growth-cpu-cells", type=int, default=64,
+                    help="target of the oracle growth run timed beside the GPU one (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -88,6 +95,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
               "design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "growth":
+        return growth(args)
     import numpy as np
     import torch
 
@@ -205,6 +214,64 @@ def main():
                                                               key=lambda x: -x[1])},
         "wall_s_timed": wall,
     }
+    print(json.dumps(out))
+
+
+def growth(args):
+    """Founder cell grown to --growth-cells on the GPU (bfm_simulate): wall time,
+    iterations, RK step evaluations and the distribution of accepted dt; the
+    oracle's loop timed on the host to a small target with the same stream."""
+    import numpy as np
+
+    import bfm
+    from bfm import sim
+    from bfm.core import make_params
+
+    target = args.growth_cells
+    cap = int(target * 1.1) + 1024
+    cfg = sim.sim_config(n_cells=target, tableau=args.growth_tableau, seed=args.seed,
+                         report_seconds=20.0, max_wall_seconds=args.growth_max_wall)
+    params = make_params()
+    ctx = bfm.Context(cap, max_pairs=16 * cap, params=params)
+    c0 = sim.founder_cell()
+    cells, parents, trace, st = sim.simulate(ctx, cfg, c0, cap=cap, trace_cap=50_000_000)
+    n = cells.shape[0]
+    reached = n >= target
+    lt = np.log10(trace)
+    hist, edges = np.histogram(lt, bins=np.arange(-9.5, -2.4, 0.5))
+    gpu = dict(n_final=n, reached_target=bool(reached), wall_s=st["wall_seconds"],
+               iterations=st["iterations"], rk_steps=st["rk_steps"], divisions=st["divisions"],
+               neighbor_builds=st["neighbor_builds"], t_end_h=st["t_end"],
+               host_division_s=st["host_division_seconds"],
+               us_per_iteration=1e6 * st["wall_seconds"] / max(st["iterations"], 1),
+               dt_percentiles_h={str(q): float(np.percentile(trace, q)) for q in (1, 10, 50, 90, 99)},
+               dt_log10_histogram={f"{edges[i]:.1f}": int(hist[i]) for i in range(len(hist))})
+    cpu = None
+    if args.growth_cpu_cells > 0 and not args.no_cpu:
+        from oracle import oracle as orc
+        cfg_c = sim.sim_config(n_cells=args.growth_cpu_cells, tableau=args.growth_tableau,
+                               seed=args.seed)
+        cap_c = 2 * args.growth_cpu_cells + 64
+        t0 = time.perf_counter()
+        cc, tr_c, it_c, steps_c = orc.simulate(cfg_c, params, c0, cap=cap_c)
+        t_cpu = time.perf_counter() - t0
+        ctx_s = bfm.Context(cap_c, max_pairs=32 * cap_c + 1024, params=params)
+        cg, _, tr_g, st_g = sim.simulate(ctx_s, cfg_c, c0, cap=cap_c)
+        cpu = dict(cells=args.growth_cpu_cells, oracle_wall_s=t_cpu, gpu_wall_s=st_g["wall_seconds"],
+                   iterations=it_c, same_iterations=bool(it_c == st_g["iterations"]),
+                   max_abs_state_diff=float(np.max(np.abs(cc[:, 1:7] - cg[:, 1:7])))
+                   if cc.shape == cg.shape else None, kind="port",
+                   cores=int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1))
+    out = {"metric": f"founder-to-{target} growth wall time (3D, FP64, tol 1e-8)",
+           "value": st["wall_seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"growth from one founder cell to {target} cells",
+                      "tableau": args.growth_tableau, "seed": args.seed, "tol": 1e-8,
+                      "max_stepsize_h": cfg.max_stepsize, "min_stepsize_h": cfg.min_stepsize,
+                      "iter_update_neighbors": cfg.iter_update_neighbors,
+                      "daughter_angle_xy_bound": cfg.daughter_angle_xy_bound,
+                      "daughter_angle_z_bound": cfg.daughter_angle_z_bound},
+           "gpu": gpu, "cpu_baseline": cpu}
     print(json.dumps(out))
 
 

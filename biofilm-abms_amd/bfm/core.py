@@ -150,6 +150,12 @@ def load_library():
         "bfm_reset_timing": [vp],
         "bfm_stats": [vp, _i64p, _i64p, _i64p, _dp, _ip],
         "bfm_set_tuning": [vp, C.c_double, C.c_int, C.c_int],
+        "bfm_dev_update_neighbor_distances": [vp, vp, C.c_int, C.c_int],
+        "bfm_simulate": [vp, vp, _dp, C.c_int, C.c_int, _ip, C.c_int, _dp, C.c_int, _ip, _ip, _ip,
+                         _dp, C.c_int, vp],
+        "bfm_divide_cells": [vp, _dp, _ip, C.c_int, C.c_int, C.c_double, C.c_uint64, _ip, _ip,
+                             C.c_int, _ip],
+        "bfm_rng_uniform": [C.c_uint64, C.c_int, _dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -27,7 +27,7 @@ enum {
 __host__ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __host__ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 
-__device__ __forceinline__ double norm3(double x, double y, double z)
+__host__ __device__ __forceinline__ double norm3(double x, double y, double z)
 {
     return sqrt((x * x + y * y) + z * z);
 }
@@ -55,7 +55,7 @@ struct Seg {
 };
 
 // nearestCellBodyCoordToPoint, distances.hpp:92-105.
-__device__ __forceinline__ double nearest_coord(const Seg& c, double qx, double qy, double qz)
+__host__ __device__ __forceinline__ double nearest_coord(const Seg& c, double qx, double qy, double qz)
 {
     double s = ((qx - c.r[0]) * c.n[0] + (qy - c.r[1]) * c.n[1]) + (qz - c.r[2]) * c.n[2];
     if (fabs(s) <= c.hl) return s;
@@ -66,7 +66,7 @@ __device__ __forceinline__ double nearest_coord(const Seg& c, double qx, double 
 // distBetweenCells, distances.hpp:206-357. d runs from cell 1 to cell 2.
 // The skew branch is the clamped minimiser of |r12 + t n2 - s n1|^2 (the
 // reference calls CGAL's segment-segment distance there).
-__device__ __forceinline__ void dist_between_cells(const Seg& c1, const Seg& c2, double d[3],
+__host__ __device__ __forceinline__ void dist_between_cells(const Seg& c1, const Seg& c2, double d[3],
                                                    double& s_out, double& t_out)
 {
     double s = 0, t = 0;

@@ -1022,7 +1022,7 @@ k_rk_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau t
             const double y0 = cellv(cells, ld, c, C_RX + q);
             y[q] = y0 + d1;
             const double e = d1 - d2;
-            errors[(int64_t)q * n + c] = e;
+            if (errors) errors[(int64_t)q * n + c] = e;
             cells_new[(int64_t)(C_DRX + q) * ld_new + c] = v1;
             if (tol > 0) local = fmax(local, e / (tol * (1.0 + fabs(y0))));
         }
@@ -1712,6 +1712,16 @@ int bfm_dev_build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, do
     });
 }
 
+int bfm_dev_update_neighbor_distances(bfm_ctx* c, const double* d_cells, int n, int ld)
+{
+    return guarded(c, [&] {
+        check_cells(c, n, ld);
+        require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
+        KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj,
+                c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+    });
+}
+
 int bfm_dev_export_neighbors(bfm_ctx* c, double* d_nb, int64_t ld_out)
 {
     return guarded(c, [&] {
@@ -1775,7 +1785,7 @@ int bfm_dev_step_rk(bfm_ctx* c, const double* A, const double* b, const double* 
     return guarded(c, [&] {
         check_cells(c, n, ld);
         Tableau tb = make_tableau(A, b, bs, s);
-        step_rk_device(c, tb, dt, d_cells, n, ld, d_noise, d_cells_new, n, d_errors, tol, d_err_scaled);
+        step_rk_device(c, tb, dt, d_cells, n, ld, d_noise, d_cells_new, ld, d_errors, tol, d_err_scaled);
     });
 }
 

@@ -1,0 +1,860 @@
+// bfm_sim.hip -- the growth / division loop of runSimulation
+// (ABM-3d/ABM-3d-cpp/include/simulation.hpp:1106-2081) over the device
+// mechanical step.
+//
+// State lives on the device in the reference's column-major cell layout with
+// a fixed leading dimension (the capacity), so rows appended by division do
+// not move the existing ones. Per iteration the host reads back a handful of
+// scalars: the number of cells past the division length and the controller's
+// scaled error. Division itself follows divideCells (growth.hpp:456-720) on
+// the host, with the reference's random draws in the reference's order; the
+// minimum-distance checks of a division are split into the part against the
+// unchanged population (a device kernel, exact-safe screening) and the part
+// among the rewritten rows (host, the same segment arithmetic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "bfm_ctx.hpp"
+#include "bfm_device.hpp"
+
+using namespace bfm;
+
+namespace {
+
+struct SimError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void sim_fail(int code, const std::string& msg) { throw SimError{code, msg}; }
+
+void hchk(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) sim_fail(BFM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define SCHK(x) hchk((x), #x)
+
+void abi(bfm_ctx* c, int rc)
+{
+    if (rc != BFM_OK) sim_fail(rc, bfm_last_error(c));
+}
+
+// ---------------------------------------------------------------------------
+// random stream: MT19937 (Matsumoto & Nishimura 1998; boost::random::mt19937
+// is this generator), uniform_01 on a 32-bit engine = x * 2^-32, and the
+// Box-Muller draw of standardNormal (utils.hpp:648-655).
+// ---------------------------------------------------------------------------
+struct Mt19937 {
+    uint32_t mt[624];
+    int idx;
+    explicit Mt19937(uint32_t seed)
+    {
+        mt[0] = seed;
+        for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+        idx = 624;
+    }
+    void twist()
+    {
+        for (int i = 0; i < 624; ++i) {
+            uint32_t y = (mt[i] & 0x80000000u) | (mt[(i + 1) % 624] & 0x7fffffffu);
+            uint32_t v = mt[(i + 397) % 624] ^ (y >> 1);
+            if (y & 1u) v ^= 0x9908b0dfu;
+            mt[i] = v;
+        }
+        idx = 0;
+    }
+    uint32_t next()
+    {
+        if (idx >= 624) twist();
+        uint32_t y = mt[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+};
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+double uniform01(Mt19937& g) { return (double)g.next() * (1.0 / 4294967296.0); }
+
+double standard_normal(Mt19937& g)
+{
+    double u = uniform01(g);
+    double v = uniform01(g);
+    double c = sqrt(-2 * log(u));
+    return c * cos(kTwoPi * v);
+}
+
+// The distributions runSimulation builds (simulation.hpp:1294-1339).
+struct Draws {
+    const bfm_sim_config* cfg;
+    double growth(Mt19937& g, int group) const
+    {
+        require_group(group);
+        return cfg->growth_mean[group - 1] + cfg->growth_std[group - 1] * standard_normal(g);
+    }
+    double daughter_length(Mt19937& g) const
+    {
+        double r = 0.5 + cfg->daughter_length_std * standard_normal(g);
+        if (r < 0) return 0.0;
+        else if (r > 1) return 1.0;
+        else return r;
+    }
+    double angle_xy(Mt19937& g) const
+    {
+        double r = uniform01(g);
+        return -cfg->daughter_angle_xy_bound + 2 * cfg->daughter_angle_xy_bound * r;
+    }
+    double angle_z(Mt19937& g) const
+    {
+        double r = uniform01(g);
+        return -cfg->daughter_angle_z_bound + 2 * cfg->daughter_angle_z_bound * r;
+    }
+    static void require_group(int group)
+    {
+        if (group < 1 || group > 2) sim_fail(BFM_ERR_ARG, "cell group must be 1 or 2");
+    }
+};
+
+// rotateXY and rotateOutOfXY (utils.hpp:791-829).
+void rotate_xy(const double n[3], double theta, double out[3])
+{
+    double s = sin(theta), c = cos(theta);
+    out[0] = (c * n[0] + (-s) * n[1]) + 0.0 * n[2];
+    out[1] = (s * n[0] + c * n[1]) + 0.0 * n[2];
+    out[2] = (0.0 * n[0] + 0.0 * n[1]) + 1.0 * n[2];
+}
+
+void rotate_out_of_xy(const double n[3], double theta, double out[3])
+{
+    double v0 = n[1] * 1.0 - n[2] * 0.0;       // v = n x e_z
+    double v1 = n[2] * 0.0 - n[0] * 1.0;
+    double v2 = n[0] * 0.0 - n[1] * 0.0;
+    double nrm = sqrt((v0 * v0 + v1 * v1) + v2 * v2);
+    v0 /= nrm; v1 /= nrm; v2 /= nrm;
+    double s = sin(theta), c = cos(theta);
+    double w0 = v1 * n[2] - v2 * n[1];         // w = v x n
+    double w1 = v2 * n[0] - v0 * n[2];
+    double w2 = v0 * n[1] - v1 * n[0];
+    out[0] = n[0] * c + w0 * s;
+    out[1] = n[1] * c + w1 * s;
+    out[2] = n[2] * c + w2 * s;
+}
+
+// Query segment for the minimum-distance checks.
+struct Query {
+    double r[3], n[3], hl;
+    int self;       // index in the population to skip (-1: none)
+};
+
+Seg seg_of_row(const double* row)
+{
+    Seg s;
+    s.r[0] = row[C_RX]; s.r[1] = row[C_RY]; s.r[2] = row[C_RZ];
+    s.n[0] = row[C_NX]; s.n[1] = row[C_NY]; s.n[2] = row[C_NZ];
+    s.hl = row[C_HALF_L];
+    return s;
+}
+
+// |d| of distBetweenCells(query, other), as minDistToCell (growth.hpp:128-162).
+double seg_dist(const Seg& a, const Seg& b)
+{
+    double d[3], s, t;
+    dist_between_cells(a, b, d, s, t);
+    return norm3(d[0], d[1], d[2]);
+}
+
+// A pair whose centers are farther apart than cut + both half lengths (with a
+// relative and absolute margin far above the rounding of |d|) has |d| >= cut,
+// so it cannot change a comparison of the minimum with cut.
+__host__ __device__ inline bool beyond_cut(const Seg& a, const Seg& b, double cut)
+{
+    double dx = b.r[0] - a.r[0], dy = b.r[1] - a.r[1], dz = b.r[2] - a.r[2];
+    double bound = (cut + a.hl + b.hl) * (1.0 + 1e-9) + 1e-9;
+    return (dx * dx + dy * dy) + dz * dz > bound * bound;
+}
+
+// Minimum distances from queries to the unchanged part of the population.
+// Only comparisons of the results with `cut` are meaningful (screened pairs
+// are left out).
+using BaseMinFn = std::function<void(const std::vector<Query>&, bool skip_dividing, double cut,
+                                     std::vector<double>&)>;
+
+struct DivisionResult {
+    std::vector<double> mothers;     // n_div x ncols, row-major: rewritten dividing rows
+    std::vector<double> daughters;   // n_div x ncols, row-major: appended rows
+    int tries = 0;
+    bool satisfied = true;
+};
+
+// divideCells (growth.hpp:456-720) for the dividing rows `mrows` (n_div x
+// ncols, row-major, original values) at indices idx (ascending) of an
+// n-cell population.
+DivisionResult divide_rows(const bfm_sim_config* cfg, Mt19937& rng, double t, int n, int ncols,
+                           const std::vector<int>& idx, const std::vector<double>& mrows,
+                           const BaseMinFn& base_min, std::vector<int>& parents)
+{
+    const Draws dr{cfg};
+    const int nd = (int)idx.size();
+    const double R = cfg->R;
+    const double mindist_default = 2 * cfg->Rcell;
+    DivisionResult out;
+
+    // check_distance: is there room to re-orient the daughters?
+    std::vector<int> check(nd, 1), satisfies(nd, 0);
+    {
+        std::vector<Query> q(nd);
+        for (int i = 0; i < nd; ++i) {
+            Seg s = seg_of_row(&mrows[(size_t)i * ncols]);
+            std::memcpy(q[i].r, s.r, sizeof(s.r));
+            std::memcpy(q[i].n, s.n, sizeof(s.n));
+            q[i].hl = s.hl;
+            q[i].self = idx[i];
+        }
+        std::vector<double> md;
+        base_min(q, false, mindist_default, md);
+        for (int i = 0; i < nd; ++i)
+            if (md[i] < mindist_default) check[i] = 0;
+    }
+    for (int i = 0; i < nd; ++i)
+        if (check[i] == 0) satisfies[i] = 1;
+
+    auto n_satisfied = [&] {
+        int s = 0;
+        for (int v : satisfies) s += v;
+        return s;
+    };
+
+    const int ntries_total = 10;
+    int ntries = 0;
+    std::vector<double> mo, da;
+    while (ntries == 0 || (ntries < ntries_total && n_satisfied() < nd)) {
+        mo = mrows;
+        da = mrows;
+        std::vector<double> txy1(nd, 0.0), tz1(nd, 0.0), txy2(nd, 0.0), tz2(nd, 0.0);
+        for (int i = 0; i < nd; ++i) {
+            if (ntries < ntries_total - 1 && check[i] == 1) {
+                txy1[i] = dr.angle_xy(rng);
+                tz1[i] = dr.angle_z(rng);
+                txy2[i] = dr.angle_xy(rng);
+                tz2[i] = dr.angle_z(rng);
+            }
+        }
+        for (int i = 0; i < nd; ++i) {
+            const double* row = &mrows[(size_t)i * ncols];
+            double u[3] = {row[C_NX], row[C_NY], row[C_NZ]}, w[3], v[3];
+            rotate_xy(u, txy1[i], w);
+            rotate_out_of_xy(w, tz1[i], v);
+            double* m = &mo[(size_t)i * ncols];
+            m[C_NX] = v[0]; m[C_NY] = v[1]; m[C_NZ] = v[2];
+            rotate_xy(u, txy2[i], w);
+            rotate_out_of_xy(w, tz2[i], v);
+            double* d = &da[(size_t)i * ncols];
+            d[C_NX] = v[0]; d[C_NY] = v[1]; d[C_NZ] = v[2];
+        }
+        std::vector<double> M(nd);
+        for (int i = 0; i < nd; ++i) M[i] = dr.daughter_length(rng);
+        for (int i = 0; i < nd; ++i) {
+            double* m = &mo[(size_t)i * ncols];
+            double* d = &da[(size_t)i * ncols];
+            double Ld = m[C_L] - 2 * R;
+            double L1 = M[i] * Ld;
+            double L2 = (1 - M[i]) * Ld;
+            double div = -m[C_HALF_L] + L1 + R;
+            double delta1 = L1 / 2 + R;
+            double delta2 = L2 / 2 + R;
+            m[C_L] = L1;
+            m[C_HALF_L] = L1 / 2;
+            d[C_L] = L2;
+            d[C_HALF_L] = L2 / 2;
+            m[C_RX] = m[C_RX] + (div - delta1) * m[C_NX];
+            d[C_RX] = d[C_RX] + (div + delta2) * d[C_NX];
+            m[C_RY] = m[C_RY] + (div - delta1) * m[C_NY];
+            d[C_RY] = d[C_RY] + (div + delta2) * d[C_NY];
+            m[C_RZ] = m[C_RZ] + (div - delta1) * m[C_NZ];
+            d[C_RZ] = d[C_RZ] + (div + delta2) * d[C_NZ];
+            m[C_T0] = t;
+            d[C_T0] = t;
+        }
+        for (int i = 0; i < nd; ++i) {
+            double* m = &mo[(size_t)i * ncols];
+            m[C_GROWTH] = dr.growth(rng, (int)m[C_GROUP]);
+        }
+        for (int i = 0; i < nd; ++i) {
+            double* d = &da[(size_t)i * ncols];
+            d[C_GROWTH] = dr.growth(rng, (int)d[C_GROUP]);
+        }
+
+        // minimum distance from each re-oriented daughter to the new population
+        std::vector<Query> q;
+        std::vector<int> qcell;   // position among the 2 nd new rows (mothers first)
+        for (int i = 0; i < nd; ++i) {
+            if (check[i] != 1) continue;
+            for (int which = 0; which < 2; ++which) {
+                const double* row = which == 0 ? &mo[(size_t)i * ncols] : &da[(size_t)i * ncols];
+                Query qq;
+                Seg s = seg_of_row(row);
+                std::memcpy(qq.r, s.r, sizeof(s.r));
+                std::memcpy(qq.n, s.n, sizeof(s.n));
+                qq.hl = s.hl;
+                qq.self = -1;
+                q.push_back(qq);
+                qcell.push_back(which * nd + i);
+            }
+        }
+        if (!q.empty()) {
+            std::vector<double> md;
+            base_min(q, true, mindist_default, md);
+            for (size_t k = 0; k < q.size(); ++k) {
+                Seg a;
+                std::memcpy(a.r, q[k].r, sizeof(a.r));
+                std::memcpy(a.n, q[k].n, sizeof(a.n));
+                a.hl = q[k].hl;
+                for (int r = 0; r < 2 * nd; ++r) {
+                    if (r == qcell[k]) continue;
+                    const double* row = r < nd ? &mo[(size_t)r * ncols] : &da[(size_t)(r - nd) * ncols];
+                    Seg b = seg_of_row(row);
+                    if (beyond_cut(a, b, mindist_default)) continue;
+                    double dist = seg_dist(a, b);
+                    if (dist < md[k]) md[k] = dist;
+                }
+            }
+            for (size_t k = 0; k + 1 < q.size(); k += 2) {
+                int i = qcell[k];
+                if (md[k] >= mindist_default && md[k + 1] >= mindist_default) satisfies[i] = 1;
+            }
+        }
+        ntries++;
+    }
+    out.tries = ntries;
+    out.satisfied = n_satisfied() == nd;
+
+    // cell ids and lineage
+    int max_id = (int)parents.size() - 1;
+    for (int i = 0; i < nd; ++i) {
+        int id1 = max_id + (2 * i + 1);
+        int id2 = max_id + (2 * i + 2);
+        int parent_id = (int)mo[(size_t)i * ncols + C_ID];
+        mo[(size_t)i * ncols + C_ID] = id1;
+        da[(size_t)i * ncols + C_ID] = id2;
+        parents.push_back(parent_id);
+        parents.push_back(parent_id);
+    }
+    (void)n;
+    out.mothers = std::move(mo);
+    out.daughters = std::move(da);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// device helpers of the loop
+// ---------------------------------------------------------------------------
+
+// growCells (growth.hpp:51-58).
+__global__ void k_grow(double* __restrict__ cells, int ld, int n, double R, double dt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double l = cells[(size_t)C_L * ld + i];
+    double g = cells[(size_t)C_GROWTH * ld + i];
+    l = l + (g * (4 * R / 3 + l)) * dt;
+    cells[(size_t)C_L * ld + i] = l;
+    cells[(size_t)C_HALF_L * ld + i] = l / 2;
+}
+
+// divideMaxLength (growth.hpp:68-73): indices of cells with l > Ldiv, in
+// arbitrary order (the host sorts them).
+__global__ void k_find_dividing(const double* __restrict__ cells, int ld, int n, double Ldiv,
+                                int* __restrict__ count, int* __restrict__ idx)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (cells[(size_t)C_L * ld + i] > Ldiv) idx[atomicAdd(count, 1)] = i;
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_set_flags(int* __restrict__ flags, const int* __restrict__ idx, int k, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) flags[idx[i]] = v;
+}
+
+// rows (k x ncols, column-major, ld k) <- cells[idx[r], :]
+__global__ void k_gather_rows(const double* __restrict__ cells, int ld, int ncols, const int* __restrict__ idx,
+                              int k, double* __restrict__ rows)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k * ncols) return;
+    int r = t % k, c = t / k;
+    rows[(size_t)c * k + r] = cells[(size_t)c * ld + idx[r]];
+}
+
+// cells[idx[r], :] <- rows (k x ncols, column-major, ld k)
+__global__ void k_scatter_rows(double* __restrict__ cells, int ld, int ncols, const int* __restrict__ idx, int k,
+                               const double* __restrict__ rows)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k * ncols) return;
+    int r = t % k, c = t / k;
+    cells[(size_t)c * ld + idx[r]] = rows[(size_t)c * k + r];
+}
+
+// Minimum |d| from each query segment (7 doubles: r, n, half_l) to cells j in
+// [0, n) with j != qself[q] and (skip == nullptr or skip[j] == 0). Pairs
+// beyond `cut` are screened out, so only comparisons with cut are exact. Out
+// holds the bit patterns of non-negative doubles (ordered like the values).
+__global__ void __launch_bounds__(256)
+k_min_dist(const double* __restrict__ cells, int ld, int n, const double* __restrict__ q,
+           const int* __restrict__ qself, const int* __restrict__ skip, double cut,
+           unsigned long long* __restrict__ out)
+{
+    const int qi = blockIdx.y;
+    Seg a;
+    a.r[0] = q[qi * 7 + 0]; a.r[1] = q[qi * 7 + 1]; a.r[2] = q[qi * 7 + 2];
+    a.n[0] = q[qi * 7 + 3]; a.n[1] = q[qi * 7 + 4]; a.n[2] = q[qi * 7 + 5];
+    a.hl = q[qi * 7 + 6];
+    const int self = qself[qi];
+    double best = INFINITY;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        if (j == self || (skip && skip[j])) continue;
+        Seg b;
+        b.r[0] = cells[(size_t)C_RX * ld + j]; b.r[1] = cells[(size_t)C_RY * ld + j];
+        b.r[2] = cells[(size_t)C_RZ * ld + j];
+        b.n[0] = cells[(size_t)C_NX * ld + j]; b.n[1] = cells[(size_t)C_NY * ld + j];
+        b.n[2] = cells[(size_t)C_NZ * ld + j];
+        b.hl = cells[(size_t)C_HALF_L * ld + j];
+        if (beyond_cut(a, b, cut)) continue;
+        double d[3], s, t;
+        dist_between_cells(a, b, d, s, t);
+        double dist = norm3(d[0], d[1], d[2]);
+        if (dist < best) best = dist;
+    }
+    for (int off = 32; off > 0; off >>= 1) best = fmin(best, __shfl_xor(best, off));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+        atomicMin(&out[qi], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+int blocks_for(int64_t n, int b) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// the loop
+// ---------------------------------------------------------------------------
+struct TableauDef {
+    int s;
+    double A[49], b[7], bs[7];
+    double order;
+};
+
+// Butcher tableaux of runSimulation (simulation.hpp:1207-1268).
+TableauDef tableau(int which)
+{
+    TableauDef t;
+    std::memset(&t, 0, sizeof(t));
+    auto set = [&](int s, std::initializer_list<double> A, std::initializer_list<double> b,
+                   std::initializer_list<double> bs, double order) {
+        t.s = s;
+        int k = 0;
+        for (double v : A) t.A[k++] = v;
+        k = 0;
+        for (double v : b) t.b[k++] = v;
+        k = 0;
+        for (double v : bs) t.bs[k++] = v;
+        t.order = order;
+    };
+    switch (which) {
+    case 0:
+        set(2, {0, 0, 1, 0}, {0.5, 0.5}, {1, 0}, 1);
+        break;
+    case 1:
+        set(4, {0, 0, 0, 0, 1. / 2., 0, 0, 0, 0, 3. / 4., 0, 0, 2. / 9., 1. / 3., 4. / 9., 0},
+            {2. / 9., 1. / 3., 4. / 9., 0}, {7. / 24., 1. / 4., 1. / 3., 1. / 8.}, 2);
+        break;
+    case 2:
+        set(6,
+            {0, 0, 0, 0, 0, 0,
+             1. / 4., 0, 0, 0, 0, 0,
+             3. / 32., 9. / 32., 0, 0, 0, 0,
+             1932. / 2197., -7200. / 2197., 7296. / 2197., 0, 0, 0,
+             439. / 216., -8.0, 3680. / 513., -845. / 4104., 0, 0,
+             -8. / 27., 2.0, -3544. / 2565., 1859. / 4104., -11. / 40., 0},
+            {16. / 135., 0, 6656. / 12825., 28561. / 56430., -9. / 50., 2. / 55.},
+            {25. / 216., 0, 1408. / 2565., 2197. / 4104., -1. / 5., 0}, 4);
+        break;
+    case 3:
+        set(7,
+            {0, 0, 0, 0, 0, 0, 0,
+             1. / 5., 0, 0, 0, 0, 0, 0,
+             3. / 40., 9. / 40., 0, 0, 0, 0, 0,
+             44. / 45., -56. / 15., 32. / 9., 0, 0, 0, 0,
+             19372. / 6561., -25360. / 2187., 64448. / 6561., -212. / 729, 0, 0, 0,
+             9017. / 3168., -355. / 33., 46732. / 5247., 49. / 176., -5103. / 18656., 0, 0,
+             35. / 384., 0, 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0},
+            {35. / 384., 0, 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0},
+            {5179. / 57600., 0, 7571. / 16695., 393. / 640., -92097. / 339200., 187. / 2100., 1. / 40.}, 4);
+        break;
+    default:
+        sim_fail(BFM_ERR_ARG, "tableau must be 0..3");
+    }
+    return t;
+}
+
+// Termination rule of runSimulation (simulation.hpp:1574-1614): any of the
+// positive criteria; none given terminates at once.
+bool terminate_now(const bfm_sim_config* cfg, int n, int64_t iter, double t)
+{
+    bool has_n = cfg->n_cells > 0, has_i = cfg->max_iter > 0, has_t = cfg->max_time > 0;
+    if (!has_n && !has_i && !has_t) return true;
+    return (has_n && n >= cfg->n_cells) || (has_i && iter >= cfg->max_iter) || (has_t && t >= cfg->max_time);
+}
+
+void check_config(const bfm_sim_config* cfg, int ncols)
+{
+    if (!cfg) sim_fail(BFM_ERR_ARG, "NULL config");
+    if (cfg->iter_update_neighbors < 1 || cfg->iter_update_stepsize < 1)
+        sim_fail(BFM_ERR_ARG, "iter_update_neighbors and iter_update_stepsize must be >= 1");
+    if (cfg->max_tries_update_stepsize < 0) sim_fail(BFM_ERR_ARG, "max_tries_update_stepsize < 0");
+    if (!(cfg->max_stepsize > 0) || !(cfg->min_stepsize > 0) || cfg->min_stepsize > cfg->max_stepsize)
+        sim_fail(BFM_ERR_ARG, "need 0 < min_stepsize <= max_stepsize");
+    if (!(cfg->max_error_allowed > 0)) sim_fail(BFM_ERR_ARG, "max_error_allowed must be > 0");
+    if (ncols < 21 || ncols > 32) sim_fail(BFM_ERR_ARG, "ncols must be in [21, 32]");
+}
+
+struct PinnedBuf {
+    void* p = nullptr;
+    explicit PinnedBuf(size_t bytes) { SCHK(hipHostMalloc(&p, bytes, hipHostMallocDefault)); }
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    explicit DevBuf(size_t count) { SCHK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T))); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
+              const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
+              int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats)
+{
+    check_config(cfg, ncols);
+    if (!c->params_set) sim_fail(BFM_ERR_ARG, "bfm_set_params has not been called");
+    if (c->p.ncols != ncols) sim_fail(BFM_ERR_ARG, "ncols differs from the parameters' ncols");
+    if (!cells_in || !cells_out || !n_out || n_in < 1 || cap < n_in || cap > c->max_cells)
+        sim_fail(BFM_ERR_ARG, "need 1 <= n_in <= cap <= max_cells and non-NULL arrays");
+    const auto wall0 = std::chrono::steady_clock::now();
+    double div_seconds = 0;
+    const TableauDef tb = tableau(cfg->tableau);
+    hipStream_t st = c->stream;
+    const int ld = cap;
+
+    DevBuf<double> bufA((size_t)ld * ncols), bufB((size_t)ld * ncols);
+    DevBuf<int> d_count(1), d_idx(cap), d_flags(cap), d_rowidx(2 * (size_t)cap), d_qself(2 * (size_t)cap);
+    DevBuf<double> d_err(1), d_rows((size_t)2 * cap * ncols), d_q((size_t)2 * cap * 7);
+    DevBuf<unsigned long long> d_qmin(2 * (size_t)cap);
+    PinnedBuf h_small(64);
+    double* d_cur = bufA.p;
+    double* d_new = bufB.p;
+    SCHK(hipMemsetAsync(d_flags.p, 0, sizeof(int) * cap, st));
+    SCHK(hipMemcpy2DAsync(d_cur, sizeof(double) * ld, cells_in, sizeof(double) * n_in, sizeof(double) * n_in,
+                          ncols, hipMemcpyHostToDevice, st));
+
+    std::vector<int> parents;
+    if (parents_in && n_parents_in > 0) {
+        parents.assign(parents_in, parents_in + n_parents_in);
+    } else {
+        int max_id = 0;
+        for (int i = 0; i < n_in; ++i) max_id = std::max(max_id, (int)cells_in[(size_t)C_ID * n_in + i]);
+        parents.assign(max_id + 1, -1);
+    }
+
+    Mt19937 rng((uint32_t)cfg->seed);
+    const double thr = cfg->neighbor_threshold, R = cfg->R, Ldiv = cfg->Ldiv;
+    int n = n_in;
+    int64_t iter = 0, rk_steps = 0, divisions = 0, builds = 0;
+    double t = 0, dt = cfg->max_stepsize;
+
+    auto build = [&] {
+        int64_t m = 0;   // synchronous build: grows the per-cell slot capacity when needed
+        abi(c, bfm_dev_build_neighbors(c, d_cur, n, ld, thr, R, Ldiv, &m));
+        ++builds;
+    };
+    auto read_int = [&](const int* d) {
+        SCHK(hipMemcpyAsync(h_small.p, d, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        return *(int*)h_small.p;
+    };
+    auto rk = [&](double h) {
+        abi(c, bfm_dev_step_rk(c, tb.A, tb.b, tb.bs, tb.s, h, d_cur, n, ld, nullptr, d_new, nullptr,
+                               cfg->max_error_allowed, d_err.p));
+        ++rk_steps;
+    };
+    auto step_error = [&] {
+        SCHK(hipMemcpyAsync(h_small.p, d_err.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        return std::max(*(double*)h_small.p, cfg->min_error);
+    };
+    // Minimum distances against the unchanged population, on the device.
+    BaseMinFn base_min = [&](const std::vector<Query>& q, bool skip_dividing, double cut,
+                             std::vector<double>& out) {
+        const int nq = (int)q.size();
+        std::vector<double> hq((size_t)nq * 7);
+        std::vector<int> hs(nq);
+        for (int k = 0; k < nq; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                hq[(size_t)k * 7 + a] = q[k].r[a];
+                hq[(size_t)k * 7 + 3 + a] = q[k].n[a];
+            }
+            hq[(size_t)k * 7 + 6] = q[k].hl;
+            hs[k] = q[k].self;
+        }
+        SCHK(hipMemcpyAsync(d_q.p, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_qself.p, hs.data(), sizeof(int) * nq, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(nq, 256)), dim3(256), 0, st, d_qmin.p, nq,
+                           (unsigned long long)0x7FF0000000000000ull);
+        int gx = std::min(blocks_for(n, 256), 64);
+        hipLaunchKernelGGL(k_min_dist, dim3(gx, nq), dim3(256), 0, st, d_cur, ld, n, d_q.p, d_qself.p,
+                           skip_dividing ? d_flags.p : nullptr, cut, d_qmin.p);
+        SCHK(hipGetLastError());
+        std::vector<unsigned long long> bits(nq);
+        SCHK(hipMemcpyAsync(bits.data(), d_qmin.p, sizeof(unsigned long long) * nq, hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        out.resize(nq);
+        for (int k = 0; k < nq; ++k) std::memcpy(&out[k], &bits[k], sizeof(double));
+    };
+
+    build();
+    std::vector<int> idx;
+    auto wall = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count(); };
+    double next_report = cfg->report_seconds;
+    while (!terminate_now(cfg, n, iter, t)) {
+        if (cfg->max_wall_seconds > 0 && wall() >= cfg->max_wall_seconds) break;
+        if (cfg->report_seconds > 0 && wall() >= next_report) {
+            std::fprintf(stderr, "[bfm_simulate] wall %.1f s  iter %lld  n %d  t %.6f h  dt %.3e h  rk steps %lld\n",
+                         wall(), (long long)iter, n, t, dt, (long long)rk_steps);
+            std::fflush(stderr);
+            next_report += cfg->report_seconds;
+        }
+        // divideMaxLength + divideCells
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
+        hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
+                           d_count.p, d_idx.p);
+        SCHK(hipGetLastError());
+        int nd = read_int(d_count.p);
+        if (nd > 0) {
+            auto td = std::chrono::steady_clock::now();
+            if (n + nd > cap) sim_fail(BFM_ERR_CAPACITY, "division would exceed the cell capacity");
+            idx.resize(nd);
+            SCHK(hipMemcpyAsync(idx.data(), d_idx.p, sizeof(int) * nd, hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+            std::sort(idx.begin(), idx.end());
+            SCHK(hipMemcpyAsync(d_idx.p, idx.data(), sizeof(int) * nd, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for((int64_t)nd * ncols, 256)), dim3(256), 0, st, d_cur,
+                               ld, ncols, d_idx.p, nd, d_rows.p);
+            std::vector<double> colm((size_t)nd * ncols), mrows((size_t)nd * ncols);
+            SCHK(hipMemcpyAsync(colm.data(), d_rows.p, sizeof(double) * colm.size(), hipMemcpyDeviceToHost, st));
+            hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p, nd,
+                               1);
+            SCHK(hipStreamSynchronize(st));
+            for (int r = 0; r < nd; ++r)
+                for (int k = 0; k < ncols; ++k) mrows[(size_t)r * ncols + k] = colm[(size_t)k * nd + r];
+            DivisionResult dv = divide_rows(cfg, rng, t, n, ncols, idx, mrows, base_min, parents);
+            // write back: mothers in place, daughters appended at n, n+1, ...
+            std::vector<int> dst(2 * (size_t)nd);
+            std::vector<double> rows((size_t)2 * nd * ncols);
+            for (int r = 0; r < nd; ++r) {
+                dst[r] = idx[r];
+                dst[nd + r] = n + r;
+                for (int k = 0; k < ncols; ++k) {
+                    rows[(size_t)k * 2 * nd + r] = dv.mothers[(size_t)r * ncols + k];
+                    rows[(size_t)k * 2 * nd + nd + r] = dv.daughters[(size_t)r * ncols + k];
+                }
+            }
+            SCHK(hipMemcpyAsync(d_rowidx.p, dst.data(), sizeof(int) * dst.size(), hipMemcpyHostToDevice, st));
+            SCHK(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_scatter_rows, dim3(blocks_for((int64_t)2 * nd * ncols, 256)), dim3(256), 0, st,
+                               d_cur, ld, ncols, d_rowidx.p, 2 * nd, d_rows.p);
+            hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p, nd,
+                               0);
+            SCHK(hipGetLastError());
+            SCHK(hipStreamSynchronize(st));
+            n += nd;
+            divisions += nd;
+            div_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
+            build();
+        }
+
+        // stepRungeKuttaAdaptive with the step-size controller (simulation.hpp:1677-1769)
+        rk(dt);
+        if (iter % cfg->iter_update_stepsize == 0) {
+            const double p1 = 1.0 / (tb.order + 1);
+            double max_error = step_error();
+            bool exceeded = max_error > 1.0;
+            double factor = 0.9 * pow(1.0 / max_error, p1);
+            if (factor >= 10) factor = 10;
+            else if (factor < 0.2) factor = 0.2;
+            int j = 0;
+            while (exceeded && j < cfg->max_tries_update_stepsize) {
+                double dt_new = dt * factor;
+                rk(dt_new);
+                max_error = step_error();
+                exceeded = max_error > 1.0;
+                factor *= 0.9 * pow(1.0 / max_error, p1);
+                if (factor >= 10) {
+                    factor = 10;
+                    break;
+                } else if (factor < 0.2) {
+                    factor = 0.2;
+                    break;
+                }
+                j++;
+            }
+            if (dt * factor < cfg->min_stepsize) factor = cfg->min_stepsize / dt;
+            else if (dt * factor > cfg->max_stepsize) factor = cfg->max_stepsize / dt;
+            dt *= factor;
+            rk(dt);
+        }
+        std::swap(d_cur, d_new);
+        hipLaunchKernelGGL(k_grow, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, R, dt);
+        SCHK(hipGetLastError());
+        abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
+        abi(c, bfm_sync(c));   // device status: non-finite velocities, JKR roots
+        if (dt_trace && iter < cap_trace) dt_trace[iter] = dt;
+        t += dt;
+        iter++;
+        if (iter % cfg->iter_update_neighbors == 0) build();
+    }
+    abi(c, bfm_sync(c));
+    SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, d_cur, sizeof(double) * ld, sizeof(double) * n, ncols,
+                          hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    *n_out = n;
+    if (parents_out && n_parents_out) {
+        std::copy(parents.begin(), parents.end(), parents_out);
+        *n_parents_out = (int)parents.size();
+    }
+    if (stats) {
+        stats->iterations = iter;
+        stats->rk_steps = rk_steps;
+        stats->divisions = divisions;
+        stats->neighbor_builds = builds;
+        stats->t_end = t;
+        stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+        stats->host_division_seconds = div_seconds;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bfm_simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
+                 const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
+                 int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats)
+{
+    if (!c) return BFM_ERR_ARG;
+    try {
+        SCHK(hipSetDevice(c->device));
+        simulate(c, cfg, cells_in, n_in, ncols, parents_in, n_parents_in, cells_out, cap, n_out, parents_out,
+                 n_parents_out, dt_trace, cap_trace, stats);
+        return BFM_OK;
+    } catch (const SimError& e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return BFM_ERR_ARG;
+    }
+}
+
+int bfm_divide_cells(const bfm_sim_config* cfg, double* cells, int* n_io, int cap, int ncols, double t,
+                     uint64_t seed, int* parents, int* n_parents, int parents_cap, int* tries_out)
+{
+    try {
+        if (!cfg || !cells || !n_io || !parents || !n_parents) return BFM_ERR_ARG;
+        const int n = *n_io;
+        if (n < 0 || n > cap || ncols < 21 || ncols > 32) return BFM_ERR_ARG;
+        std::vector<int> idx;
+        for (int i = 0; i < n; ++i)
+            if (cells[(size_t)C_L * cap + i] > cfg->Ldiv) idx.push_back(i);
+        const int nd = (int)idx.size();
+        if (n + nd > cap || *n_parents + 2 * nd > parents_cap) return BFM_ERR_CAPACITY;
+        if (tries_out) *tries_out = 0;
+        if (nd == 0) return BFM_OK;
+        std::vector<double> mrows((size_t)nd * ncols);
+        for (int r = 0; r < nd; ++r)
+            for (int k = 0; k < ncols; ++k) mrows[(size_t)r * ncols + k] = cells[(size_t)k * cap + idx[r]];
+        std::vector<char> dividing(n, 0);
+        for (int i : idx) dividing[i] = 1;
+        BaseMinFn base_min = [&](const std::vector<Query>& q, bool skip_dividing, double cut,
+                                 std::vector<double>& out) {
+            out.assign(q.size(), INFINITY);
+            for (size_t k = 0; k < q.size(); ++k) {
+                Seg a;
+                std::memcpy(a.r, q[k].r, sizeof(a.r));
+                std::memcpy(a.n, q[k].n, sizeof(a.n));
+                a.hl = q[k].hl;
+                for (int j = 0; j < n; ++j) {
+                    if (j == q[k].self || (skip_dividing && dividing[j])) continue;
+                    Seg b;
+                    b.r[0] = cells[(size_t)C_RX * cap + j]; b.r[1] = cells[(size_t)C_RY * cap + j];
+                    b.r[2] = cells[(size_t)C_RZ * cap + j];
+                    b.n[0] = cells[(size_t)C_NX * cap + j]; b.n[1] = cells[(size_t)C_NY * cap + j];
+                    b.n[2] = cells[(size_t)C_NZ * cap + j];
+                    b.hl = cells[(size_t)C_HALF_L * cap + j];
+                    if (beyond_cut(a, b, cut)) continue;
+                    double dist = seg_dist(a, b);
+                    if (dist < out[k]) out[k] = dist;
+                }
+            }
+        };
+        std::vector<int> par(parents, parents + *n_parents);
+        Mt19937 rng((uint32_t)seed);
+        DivisionResult dv = divide_rows(cfg, rng, t, n, ncols, idx, mrows, base_min, par);
+        for (int r = 0; r < nd; ++r)
+            for (int k = 0; k < ncols; ++k) {
+                cells[(size_t)k * cap + idx[r]] = dv.mothers[(size_t)r * ncols + k];
+                cells[(size_t)k * cap + n + r] = dv.daughters[(size_t)r * ncols + k];
+            }
+        *n_io = n + nd;
+        std::copy(par.begin(), par.end(), parents);
+        *n_parents = (int)par.size();
+        if (tries_out) *tries_out = dv.tries;
+        return BFM_OK;
+    } catch (const SimError& e) {
+        return e.code;
+    } catch (const std::exception&) {
+        return BFM_ERR_ARG;
+    }
+}
+
+int bfm_rng_uniform(uint64_t seed, int count, double* out)
+{
+    if (count < 0 || (count > 0 && !out)) return BFM_ERR_ARG;
+    Mt19937 g((uint32_t)seed);
+    for (int i = 0; i < count; ++i) out[i] = uniform01(g);
+    return BFM_OK;
+}
+
+}  // extern "C"

@@ -162,6 +162,9 @@ int bfm_dev_build_neighbors(bfm_ctx* ctx, const double* d_cells, int n, int ld,
                             double neighbor_threshold, double R, double Ldiv,
                             int64_t* m_out);
 
+/* updateNeighborDistances (mechanics.hpp:604-648) on the internal list. */
+int bfm_dev_update_neighbor_distances(bfm_ctx* ctx, const double* d_cells, int n, int ld);
+
 /* Copies the internal neighbor list into a device m x 7 column-major array. */
 int bfm_dev_export_neighbors(bfm_ctx* ctx, double* d_neighbors, int64_t ld_out);
 
@@ -181,12 +184,66 @@ int bfm_dev_stage(bfm_ctx* ctx, const double* d_cells, int n, int ld,
                   double neighbor_threshold, double R, double Ldiv, double* d_vel);
 
 /* stepRungeKuttaAdaptive on device data with the internal neighbor list.
- * d_err_scaled receives max over entries of errors / (tol * (1 + |y|)) (the
+ * d_cells_new has the layout of d_cells (leading dimension ld); d_errors
+ * (n x 6, leading dimension n) may be NULL. d_err_scaled receives max over entries of errors / (tol * (1 + |y|)) (the
  * reference controller's quantity, simulation.hpp:1688-1695) when tol > 0. */
 int bfm_dev_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* bs,
                     int s, double dt, const double* d_cells, int n, int ld,
                     const double* d_noise, double* d_cells_new, double* d_errors,
                     double tol, double* d_err_scaled);
+
+/* ---- simulation driver (runSimulation, simulation.hpp:1106-2081) -------- */
+/* Parameters of the growth/division loop; the mechanics parameters come from
+ * bfm_set_params. tableau: 0 Heun-Euler, 1 Bogacki-Shampine, 2 RKF45,
+ * 3 Dormand-Prince (simulation.hpp:1207-1268). The random stream is a 32-bit
+ * Mersenne twister seeded with `seed`; uniform draws are x * 2^-32 and normal
+ * draws use Box-Muller as utils.hpp:648-655; draws follow the reference's
+ * order (divideCells, growth.hpp:456-720). Noise is not supported (zero). */
+typedef struct bfm_sim_config {
+    double R, Rcell, Ldiv;
+    double max_stepsize, min_stepsize, max_error_allowed, min_error;
+    double neighbor_threshold;
+    double growth_mean[2], growth_std[2];    /* per group (groups 1 and 2) */
+    double daughter_length_std, daughter_angle_xy_bound, daughter_angle_z_bound;
+    double max_time;
+    int iter_update_neighbors, iter_update_stepsize, max_tries_update_stepsize;
+    int n_cells, max_iter;
+    int tableau;
+    uint64_t seed;
+    double report_seconds;     /* > 0: a progress line on stderr at this wall interval
+                                  (the reference prints per dt_write, simulation.hpp:2026-2046) */
+    double max_wall_seconds;   /* > 0: also stop after this much wall time */
+} bfm_sim_config;
+
+typedef struct bfm_sim_stats {
+    int64_t iterations, rk_steps, divisions, neighbor_builds;
+    double t_end, wall_seconds, host_division_seconds;
+} bfm_sim_stats;
+
+/* Runs the loop from cells_in (n_in x ncols, leading dimension n_in) until
+ * the reference's terminate rule holds: n >= n_cells, iter >= max_iter or
+ * t >= max_time, over the criteria that are > 0. Cells and neighbor list stay
+ * on the device; divisions are decided on the host with the reference's
+ * random draws. parents_in (NULL: one -1 entry per id up to the largest id in
+ * cells_in) is the lineage vector of runSimulation; parents_out must hold
+ * n_parents_in + 2 (cap - n_in) ints. Writes the final cells (leading
+ * dimension cap) and the accepted dt of each iteration (up to cap_trace).
+ * BFM_ERR_CAPACITY when a division would exceed cap (cap <= max_cells). */
+int bfm_simulate(bfm_ctx* ctx, const bfm_sim_config* cfg, const double* cells_in, int n_in,
+                 int ncols, const int* parents_in, int n_parents_in, double* cells_out, int cap,
+                 int* n_out, int* parents_out, int* n_parents_out, double* dt_trace,
+                 int cap_trace, bfm_sim_stats* stats);
+
+/* divideMaxLength + divideCells (growth.hpp:68-73, 456-720) on host data:
+ * cells is n x ncols with leading dimension cap, daughters are appended. The
+ * random stream starts fresh from seed. tries_out (may be NULL) receives the
+ * number of placement attempts. Needs no device. */
+int bfm_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, int ncols,
+                     double t, uint64_t seed, int* parents, int* n_parents, int parents_cap,
+                     int* tries_out);
+
+/* The first `count` uniform_01 draws of the random stream seeded with seed. */
+int bfm_rng_uniform(uint64_t seed, int count, double* out);
 
 /* ---- instrumentation -------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the launch

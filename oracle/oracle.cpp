@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <random>
 #include <vector>
 #include <omp.h>
 
@@ -946,6 +947,309 @@ int oracle_controller_trace(double dt, double error_order, int max_tries,
     dt *= factor;
     dts[nd++] = dt;
     return nd;
+}
+
+
+}  // extern "C"
+
+/* ---- growth, division and the simulation loop ------------------------- */
+
+namespace {
+
+/* boost::random::uniform_01<> on mt19937: x * 2^-32 (growth.hpp callers,
+ * simulation.hpp:1294). std::mt19937 is the same generator as boost's. */
+double orc_uniform(std::mt19937& g) { return (double)g() * (1.0 / 4294967296.0); }
+
+/* standardNormal, utils.hpp:648-655. */
+double orc_normal(std::mt19937& g)
+{
+    double u = orc_uniform(g);
+    double v = orc_uniform(g);
+    double c = std::sqrt(-2 * std::log(u));
+    return c * std::cos(2 * kPi * v);
+}
+
+/* rotateXY, utils.hpp:791-801: the 3x3 rotation about z applied to n. */
+void orc_rotate_xy(const double* n, double theta, double* out)
+{
+    double rot[3][3] = {{std::cos(theta), -std::sin(theta), 0},
+                        {std::sin(theta), std::cos(theta), 0},
+                        {0, 0, 1}};
+    for (int r = 0; r < 3; ++r) out[r] = (rot[r][0] * n[0] + rot[r][1] * n[1]) + rot[r][2] * n[2];
+}
+
+/* rotateOutOfXY, utils.hpp:814-829: Rodrigues rotation about n x e_z. */
+void orc_rotate_out_of_xy(const double* n, double theta, double* out)
+{
+    const double z[3] = {0, 0, 1};
+    double v[3] = {n[1] * z[2] - n[2] * z[1], n[2] * z[0] - n[0] * z[2], n[0] * z[1] - n[1] * z[0]};
+    double nrm = norm3(v[0], v[1], v[2]);
+    for (double& x : v) x /= nrm;
+    double w[3] = {v[1] * n[2] - v[2] * n[1], v[2] * n[0] - v[0] * n[2], v[0] * n[1] - v[1] * n[0]};
+    for (int k = 0; k < 3; ++k) out[k] = n[k] * std::cos(theta) + w[k] * std::sin(theta);
+}
+
+/* minDistToCell, growth.hpp:128-162: brute force over all other cells. */
+double orc_min_dist_to_cell(const double* cells, int n, int ld, int i)
+{
+    double ri[3], ni[3], hli;
+    cell_geom(cells, ld, i, ri, ni, &hli);
+    double mindist = INFINITY;
+    for (int j = 0; j < n; ++j) {
+        if (i == j) continue;
+        double rj[3], nj[3], hlj, d[3], si, sj;
+        cell_geom(cells, ld, j, rj, nj, &hlj);
+        oracle_dist_between_cells(ri, ni, hli, rj, nj, hlj, d, &si, &sj);
+        double dist = norm3(d[0], d[1], d[2]);
+        if (dist < mindist) mindist = dist;
+    }
+    return mindist;
+}
+
+/* divideCells, growth.hpp:456-720, on a column-major population with leading
+ * dimension cap; daughters are appended. Returns the number of tries. */
+int orc_divide(const bfm_sim_config* cfg, std::mt19937& rng, double t, double* cells, int& n, int cap,
+               int ncols, std::vector<int>& parents)
+{
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i)
+        if (cells[(int64_t)C_L * cap + i] > cfg->Ldiv) idx.push_back(i);   // divideMaxLength
+    const int nd = (int)idx.size();
+    if (nd == 0) return 0;
+    const double R = cfg->R, mindist_default = 2 * cfg->Rcell;
+    auto growth = [&](int group) {
+        return cfg->growth_mean[group - 1] + cfg->growth_std[group - 1] * orc_normal(rng);
+    };
+    auto dlen = [&] {
+        double r = 0.5 + cfg->daughter_length_std * orc_normal(rng);
+        return r < 0 ? 0.0 : (r > 1 ? 1.0 : r);
+    };
+    auto axy = [&] {
+        double r = orc_uniform(rng);
+        return -cfg->daughter_angle_xy_bound + 2 * cfg->daughter_angle_xy_bound * r;
+    };
+    auto az = [&] {
+        double r = orc_uniform(rng);
+        return -cfg->daughter_angle_z_bound + 2 * cfg->daughter_angle_z_bound * r;
+    };
+    std::vector<int> check(nd, 1), satisfies(nd, 0);
+    for (int m = 0; m < nd; ++m)
+        if (orc_min_dist_to_cell(cells, n, cap, idx[m]) < mindist_default) check[m] = 0;
+    for (int m = 0; m < nd; ++m)
+        if (check[m] == 0) satisfies[m] = 1;
+    // cells_total: n + nd rows, leading dimension n + nd
+    const int nt = n + nd;
+    std::vector<double> total((size_t)nt * ncols);
+    auto T = [&](int i, int c) -> double& { return total[(size_t)c * nt + i]; };
+    int ntries = 0, sat = 0;
+    for (int v : satisfies) sat += v;
+    while (ntries == 0 || (ntries < 10 && sat < nd)) {
+        for (int c = 0; c < ncols; ++c)
+            for (int i = 0; i < n; ++i) T(i, c) = cells[(int64_t)c * cap + i];
+        for (int m = 0; m < nd; ++m)
+            for (int c = 0; c < ncols; ++c) T(n + m, c) = cells[(int64_t)c * cap + idx[m]];
+        std::vector<double> txy1(nd, 0), tz1(nd, 0), txy2(nd, 0), tz2(nd, 0);
+        for (int m = 0; m < nd; ++m)
+            if (ntries < 9 && check[m] == 1) {
+                txy1[m] = axy(); tz1[m] = az(); txy2[m] = axy(); tz2[m] = az();
+            }
+        for (int m = 0; m < nd; ++m) {
+            double u[3] = {T(idx[m], C_NX), T(idx[m], C_NY), T(idx[m], C_NZ)}, w[3], v[3];
+            orc_rotate_xy(u, txy1[m], w);
+            orc_rotate_out_of_xy(w, tz1[m], v);
+            for (int k = 0; k < 3; ++k) T(idx[m], C_NX + k) = v[k];
+            orc_rotate_xy(u, txy2[m], w);
+            orc_rotate_out_of_xy(w, tz2[m], v);
+            for (int k = 0; k < 3; ++k) T(n + m, C_NX + k) = v[k];
+        }
+        std::vector<double> M(nd);
+        for (int m = 0; m < nd; ++m) M[m] = dlen();
+        for (int m = 0; m < nd; ++m) {
+            const int i = idx[m], d = n + m;
+            double Ld = T(i, C_L) - 2 * R;
+            double L1 = M[m] * Ld, L2 = (1 - M[m]) * Ld;
+            double div = -T(i, C_HALF_L) + L1 + R;
+            double delta1 = L1 / 2 + R, delta2 = L2 / 2 + R;
+            T(i, C_L) = L1; T(i, C_HALF_L) = L1 / 2;
+            T(d, C_L) = L2; T(d, C_HALF_L) = L2 / 2;
+            for (int k = 0; k < 3; ++k) {
+                T(i, C_RX + k) = T(i, C_RX + k) + (div - delta1) * T(i, C_NX + k);
+                T(d, C_RX + k) = T(d, C_RX + k) + (div + delta2) * T(d, C_NX + k);
+            }
+            T(i, C_T0) = t;
+            T(d, C_T0) = t;
+        }
+        for (int m = 0; m < nd; ++m) T(idx[m], C_GROWTH) = growth((int)T(idx[m], C_GROUP));
+        for (int m = 0; m < nd; ++m) T(n + m, C_GROWTH) = growth((int)T(n + m, C_GROUP));
+        for (int m = 0; m < nd; ++m) {
+            if (check[m] != 1) continue;
+            double d1 = orc_min_dist_to_cell(total.data(), nt, nt, idx[m]);
+            double d2 = orc_min_dist_to_cell(total.data(), nt, nt, n + m);
+            if (d1 >= mindist_default && d2 >= mindist_default) satisfies[m] = 1;
+        }
+        sat = 0;
+        for (int v : satisfies) sat += v;
+        ntries++;
+    }
+    int max_id = (int)parents.size() - 1;
+    for (int m = 0; m < nd; ++m) {
+        int parent_id = (int)T(idx[m], C_ID);
+        T(idx[m], C_ID) = max_id + (2 * m + 1);
+        T(n + m, C_ID) = max_id + (2 * m + 2);
+        parents.push_back(parent_id);
+        parents.push_back(parent_id);
+    }
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < nt; ++i) cells[(int64_t)c * cap + i] = T(i, c);
+    n = nt;
+    return ntries;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, int ncols, double t,
+                        uint64_t seed, int* parents, int* n_parents, int parents_cap, int* tries_out)
+{
+    int nd = 0;
+    for (int i = 0; i < *n; ++i) nd += cells[(int64_t)C_L * cap + i] > cfg->Ldiv;
+    if (*n + nd > cap || *n_parents + 2 * nd > parents_cap) return BFM_ERR_CAPACITY;
+    std::mt19937 rng((uint32_t)seed);
+    std::vector<int> par(parents, parents + *n_parents);
+    int tries = orc_divide(cfg, rng, t, cells, *n, cap, ncols, par);
+    std::copy(par.begin(), par.end(), parents);
+    *n_parents = (int)par.size();
+    if (tries_out) *tries_out = tries;
+    return BFM_OK;
+}
+
+/* runSimulation's loop, simulation.hpp:1616-1864 (noise, switching, basal
+ * pruning and file output off): divide, rebuild the neighbor list after a
+ * division, adaptive RK step with the controller, grow, refresh the neighbor
+ * distances, advance t and iter, periodic rebuild. The neighbor list comes
+ * from the binned builder (the same list as getCellNeighbors). */
+int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double* cells_in, int n_in,
+                    double* cells_out, int cap, int* n_out, double* dt_trace, int cap_trace,
+                    int64_t* iters_out, int64_t* rk_steps_out, int nthreads)
+{
+    const int ncols = p->ncols;
+    static const double HE_A[4] = {0, 0, 1, 0}, HE_b[2] = {0.5, 0.5}, HE_bs[2] = {1, 0};
+    static const double BS_A[16] = {0, 0, 0, 0, 1. / 2., 0, 0, 0, 0, 3. / 4., 0, 0, 2. / 9., 1. / 3., 4. / 9., 0};
+    static const double BS_b[4] = {2. / 9., 1. / 3., 4. / 9., 0}, BS_bs[4] = {7. / 24., 1. / 4., 1. / 3., 1. / 8.};
+    static const double RKF_A[36] = {0, 0, 0, 0, 0, 0, 1. / 4., 0, 0, 0, 0, 0, 3. / 32., 9. / 32., 0, 0, 0, 0,
+                                     1932. / 2197., -7200. / 2197., 7296. / 2197., 0, 0, 0,
+                                     439. / 216., -8.0, 3680. / 513., -845. / 4104., 0, 0,
+                                     -8. / 27., 2.0, -3544. / 2565., 1859. / 4104., -11. / 40., 0};
+    static const double RKF_b[6] = {16. / 135., 0, 6656. / 12825., 28561. / 56430., -9. / 50., 2. / 55.};
+    static const double RKF_bs[6] = {25. / 216., 0, 1408. / 2565., 2197. / 4104., -1. / 5., 0};
+    static const double DP_A[49] = {0, 0, 0, 0, 0, 0, 0, 1. / 5., 0, 0, 0, 0, 0, 0, 3. / 40., 9. / 40., 0, 0, 0, 0, 0,
+                                    44. / 45., -56. / 15., 32. / 9., 0, 0, 0, 0,
+                                    19372. / 6561., -25360. / 2187., 64448. / 6561., -212. / 729, 0, 0, 0,
+                                    9017. / 3168., -355. / 33., 46732. / 5247., 49. / 176., -5103. / 18656., 0, 0,
+                                    35. / 384., 0, 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0};
+    static const double DP_b[7] = {35. / 384., 0, 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0};
+    static const double DP_bs[7] = {5179. / 57600., 0, 7571. / 16695., 393. / 640., -92097. / 339200.,
+                                    187. / 2100., 1. / 40.};
+    const double *A, *b, *bs;
+    int s;
+    double order;
+    switch (cfg->tableau) {
+    case 0: A = HE_A; b = HE_b; bs = HE_bs; s = 2; order = 1; break;
+    case 1: A = BS_A; b = BS_b; bs = BS_bs; s = 4; order = 2; break;
+    case 2: A = RKF_A; b = RKF_b; bs = RKF_bs; s = 6; order = 4; break;
+    case 3: A = DP_A; b = DP_b; bs = DP_bs; s = 7; order = 4; break;
+    default: return BFM_ERR_ARG;
+    }
+    if (n_in < 1 || n_in > cap || cfg->iter_update_neighbors < 1 || cfg->iter_update_stepsize < 1)
+        return BFM_ERR_ARG;
+    std::vector<double> cells((size_t)cap * ncols, 0.0), cells_new((size_t)cap * ncols), errors;
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < n_in; ++i) cells[(size_t)c * cap + i] = cells_in[(size_t)c * n_in + i];
+    int max_id = 0;
+    for (int i = 0; i < n_in; ++i) max_id = std::max(max_id, (int)cells_in[(size_t)C_ID * n_in + i]);
+    std::vector<int> parents(max_id + 1, -1);
+    std::mt19937 rng((uint32_t)cfg->seed);
+    int n = n_in;
+    int64_t iter = 0, rk_steps = 0;
+    double t = 0, dt = cfg->max_stepsize;
+    std::vector<double> nb;
+    int64_t m = 0, ld_nb = 0;
+    auto build = [&] {
+        // compact copy with ld = n for the builder
+        int64_t need = oracle_get_cell_neighbors_binned(cells.data(), n, cap, cfg->neighbor_threshold, cfg->R,
+                                                        cfg->Ldiv, nullptr, 0, nthreads);
+        ld_nb = std::max<int64_t>(need, 1);
+        nb.assign((size_t)ld_nb * 7, 0.0);
+        m = oracle_get_cell_neighbors_binned(cells.data(), n, cap, cfg->neighbor_threshold, cfg->R, cfg->Ldiv,
+                                             nb.data(), ld_nb, nthreads);
+    };
+    std::vector<double> packed, packed_new;
+    auto rk = [&](double h) {
+        packed.assign((size_t)n * ncols, 0.0);
+        for (int c = 0; c < ncols; ++c)
+            std::memcpy(&packed[(size_t)c * n], &cells[(size_t)c * cap], sizeof(double) * n);
+        packed_new.assign((size_t)n * ncols, 0.0);
+        errors.assign((size_t)n * 6, 0.0);
+        oracle_step_rk(A, b, bs, s, h, packed.data(), n, n, nb.data(), m, ld_nb, nullptr, p, packed_new.data(),
+                       errors.data());
+        ++rk_steps;
+        return std::max(oracle_max_scaled_error(packed.data(), n, n, errors.data(), cfg->max_error_allowed, 0.0),
+                        cfg->min_error);
+    };
+    auto terminate = [&] {
+        bool hn = cfg->n_cells > 0, hi = cfg->max_iter > 0, ht = cfg->max_time > 0;
+        if (!hn && !hi && !ht) return true;
+        return (hn && n >= cfg->n_cells) || (hi && iter >= cfg->max_iter) || (ht && t >= cfg->max_time);
+    };
+    build();
+    while (!terminate()) {
+        int nd = 0;
+        for (int i = 0; i < n; ++i) nd += cells[(size_t)C_L * cap + i] > cfg->Ldiv;
+        if (nd > 0) {
+            if (n + nd > cap) return BFM_ERR_CAPACITY;
+            orc_divide(cfg, rng, t, cells.data(), n, cap, ncols, parents);
+            build();
+        }
+        double max_error = rk(dt);
+        if (iter % cfg->iter_update_stepsize == 0) {
+            bool exceeded = max_error > 1.0;
+            double factor = 0.9 * std::pow(1.0 / max_error, 1.0 / (order + 1));
+            if (factor >= 10) factor = 10;
+            else if (factor < 0.2) factor = 0.2;
+            int j = 0;
+            while (exceeded && j < cfg->max_tries_update_stepsize) {
+                max_error = rk(dt * factor);
+                exceeded = max_error > 1.0;
+                factor *= 0.9 * std::pow(1.0 / max_error, 1.0 / (order + 1));
+                if (factor >= 10) { factor = 10; break; }
+                else if (factor < 0.2) { factor = 0.2; break; }
+                j++;
+            }
+            if (dt * factor < cfg->min_stepsize) factor = cfg->min_stepsize / dt;
+            else if (dt * factor > cfg->max_stepsize) factor = cfg->max_stepsize / dt;
+            dt *= factor;
+            rk(dt);
+        }
+        for (int c = 0; c < ncols; ++c)
+            std::memcpy(&cells[(size_t)c * cap], &packed_new[(size_t)c * n], sizeof(double) * n);
+        for (int i = 0; i < n; ++i) {   // growCells, growth.hpp:51-58
+            double& l = cells[(size_t)C_L * cap + i];
+            l += (cells[(size_t)C_GROWTH * cap + i] * (4 * cfg->R / 3 + l)) * dt;
+            cells[(size_t)C_HALF_L * cap + i] = l / 2;
+        }
+        oracle_update_neighbor_distances(cells.data(), n, cap, nb.data(), m, ld_nb);
+        if (dt_trace && iter < cap_trace) dt_trace[iter] = dt;
+        t += dt;
+        iter++;
+        if (iter % cfg->iter_update_neighbors == 0) build();
+    }
+    for (int c = 0; c < ncols; ++c)
+        std::memcpy(&cells_out[(size_t)c * cap], &cells[(size_t)c * cap], sizeof(double) * n);
+    *n_out = n;
+    if (iters_out) *iters_out = iter;
+    if (rk_steps_out) *rk_steps_out = rk_steps;
+    return BFM_OK;
 }
 
 }  // extern "C"

@@ -92,6 +92,19 @@ int oracle_controller_trace(double dt, double error_order, int max_tries,
                             double min_stepsize, double max_stepsize,
                             const double* err_seq, int n_err, double* dts);
 
+/* divideMaxLength + divideCells (growth.hpp:68-73, 456-720) with a fresh
+ * std::mt19937(seed): cells n x ncols, leading dimension cap, daughters
+ * appended; brute-force minDistToCell. */
+int oracle_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, int ncols,
+                        double t, uint64_t seed, int* parents, int* n_parents, int parents_cap,
+                        int* tries_out);
+/* runSimulation's loop (simulation.hpp:1616-1864) with zero noise: writes the
+ * final cells (leading dimension cap), n, the accepted dt per iteration, the
+ * iteration count and the number of RK step evaluations. */
+int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double* cells_in, int n_in,
+                    double* cells_out, int cap, int* n_out, double* dt_trace, int cap_trace,
+                    int64_t* iters_out, int64_t* rk_steps_out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,12 @@ def lib():
         L.oracle_max_scaled_error.restype = d
         L.oracle_controller_trace.argtypes = [d, d, C.c_int, d, d, _dp, C.c_int, _dp]
         L.oracle_controller_trace.restype = C.c_int
+        L.oracle_divide_cells.argtypes = [vp, _dp, _ip, C.c_int, C.c_int, d, C.c_uint64, _ip, _ip,
+                                          C.c_int, _ip]
+        L.oracle_divide_cells.restype = C.c_int
+        L.oracle_simulate.argtypes = [vp, vp, _dp, C.c_int, _dp, C.c_int, _ip, _dp, C.c_int,
+                                      C.POINTER(i64), C.POINTER(i64), C.c_int]
+        L.oracle_simulate.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -272,3 +278,40 @@ def controller_trace(dt, error_order, max_tries, min_step, max_step, errs):
     k = lib().oracle_controller_trace(dt, error_order, max_tries, min_step, max_step, _p(e), len(e),
                                       _p(out))
     return out[:k]
+
+
+def divide_cells(cfg, cells, t, seed, parents=None):
+    """oracle_divide_cells on a copy; returns (cells, parents, tries)."""
+    cells = np.asarray(cells, dtype=np.float64)
+    n, ncols = cells.shape
+    cap = 2 * n + 1
+    buf = np.zeros((cap, ncols), order="F")
+    buf[:n] = cells
+    if parents is None:
+        parents = -np.ones(int(cells[:, 0].max()) + 1, dtype=np.int32)
+    pcap = len(parents) + 2 * n + 2
+    par = np.zeros(pcap, dtype=np.int32)
+    par[:len(parents)] = parents
+    n_io, n_par, tries = C.c_int(n), C.c_int(len(parents)), C.c_int()
+    rc = lib().oracle_divide_cells(C.byref(cfg), _p(buf), C.byref(n_io), cap, ncols, t, seed,
+                                   par.ctypes.data_as(_ip), C.byref(n_par), pcap, C.byref(tries))
+    if rc != 0:
+        raise RuntimeError(f"oracle_divide_cells: status {rc}")
+    return np.asfortranarray(buf[:n_io.value]), par[:n_par.value].copy(), tries.value
+
+
+def simulate(cfg, params, cells, cap, trace_cap=1_000_000, nthreads=0):
+    """oracle_simulate; returns (cells, dt_trace, iterations, rk_steps)."""
+    cells = _f(cells)
+    n_in = cells.shape[0]
+    out = np.zeros((cap, params.ncols), order="F")
+    n_out = C.c_int()
+    trace = np.zeros(trace_cap)
+    iters, steps = C.c_int64(), C.c_int64()
+    rc = lib().oracle_simulate(C.byref(cfg), C.byref(params), _p(cells), n_in, _p(out), cap,
+                               C.byref(n_out), _p(trace), trace_cap, C.byref(iters),
+                               C.byref(steps), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_simulate: status {rc}")
+    it = min(iters.value, trace_cap)
+    return np.asfortranarray(out[:n_out.value]), trace[:it].copy(), iters.value, steps.value
