@@ -28,6 +28,30 @@ SCREEN_FLOPS = 12      # r_ij^2 (8), bound and its square (4) per screened pair
 TEST_FLOPS = 80        # segment-segment closest points + |d| per queued pair
 
 
+# HBM traffic per launch (FETCH_SIZE and WRITE_SIZE, copy-calibrated) of the
+# stage kernels, from the committed rocprofv3 summary of the same workload.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stage_200k_dome_kernels.md")
+KERNEL_SYMBOL = {"candidates": "k_candidates_lds", "compact_rows": "k_compact_rows",
+                 "velocities": "k_velocities", "pair_forces": "k_pair_forces",
+                 "scatter": "k_scatter", "compact_jrole": "k_compact_jrole"}
+
+
+def profiled_traffic(kernel, cells):
+    """Read + write HBM bytes per launch of `kernel` measured by rocprofv3 on
+    the 200k-cell dome; None for other workloads or kernels."""
+    sym = KERNEL_SYMBOL.get(kernel)
+    if cells != 200000 or sym is None or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cols = [c.strip() for c in line.split("|")]
+        if len(cols) > 7 and sym in cols[1]:
+            try:
+                return (float(cols[6]) + float(cols[7])) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,7 +197,8 @@ def main():
         flops = tests_per_launch * TEST_FLOPS + screened_per_launch * SCREEN_FLOPS
         achieved = flops / (per_kernel[dom] * 1e-3) / 1e12
         roof = dict(kernel=dom, bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS,
-                    unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
+                    unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
+                    traffic=profiled_traffic(dom, n),
                     avg_us=per_kernel[dom] * 1e3, algorithmic_flops=flops,
                     screened_per_launch=screened_per_launch, tests_per_launch=tests_per_launch)
     else:
@@ -181,7 +206,8 @@ def main():
         bytes_ = n * 176 + m * 48
         achieved = bytes_ / (per_kernel[dom] * 1e-3) / 1e9
         roof = dict(kernel=dom, bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None, avg_us=per_kernel[dom] * 1e3,
+                    frac=achieved / HBM_PEAK_GBS, traffic=profiled_traffic(dom, n),
+                    avg_us=per_kernel[dom] * 1e3,
                     algorithmic_bytes=bytes_)
     stage_bytes = n * 328 + m * 96
     cpu = None
