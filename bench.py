@@ -66,8 +66,16 @@ def parse():
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
-    ap.add_argument("--workload", default="stage", choices=["stage", "growth"],
-                    help="stage: the RK-stage bench (default); growth: the founder-to-N growth run")
+    ap.add_argument("--workload", default="stage", choices=["stage", "growth", "ensemble"],
+                    help="stage: the RK-stage bench (default); growth: the founder-to-N growth run; "
+                         "ensemble: batched replicates")
+    ap.add_argument("--ens-reps", type=int, default=256)
+    ap.add_argument("--ens-lo", type=int, default=500)
+    ap.add_argument("--ens-hi", type=int, default=20000)
+    ap.add_argument("--ens-minutes", type=float, default=10.0, help="simulated minutes per replicate")
+    ap.add_argument("--ens-max-wall", type=float, default=240.0)
+    ap.add_argument("--ens-tableau", default="bogacki_shampine")
+    ap.add_argument("--ens-jkr", default="surrogate", choices=["exact", "surrogate"])
     ap.add_argument("--growth-cells", type=int, default=100000)
     ap.add_argument("--growth-tableau", default="bogacki_shampine")
     ap.add_argument("--growth-max-wall", type=float, default=900.0)
@@ -121,6 +129,8 @@ def main():
         sys.exit(2)
     if args.workload == "growth":
         return growth(args)
+    if args.workload == "ensemble":
+        return ensemble(args)
     import numpy as np
     import torch
 
@@ -299,6 +309,74 @@ def growth(args):
                       "daughter_angle_z_bound": cfg.daughter_angle_z_bound},
            "gpu": gpu, "cpu_baseline": cpu}
     print(json.dumps(out))
+
+
+def ensemble(args):
+    """Batched replicates: --ens-reps domes with log-uniform cell counts in
+    [--ens-lo, --ens-hi] (seeds 0..R-1), two phenotypes (gamma 50 / 500 Pa um)
+    assigned 50/50 by a seeded draw, advanced --ens-minutes simulated minutes
+    (or until --ens-max-wall seconds) by bfm_ensemble_simulate with the
+    autotuner and the learned JKR radius. value = RK stage cell-updates per
+    second over the whole run (every attempt of every replicate)."""
+    import numpy as np
+
+    import bfm
+    from bfm import sim
+    from bfm.core import make_params, JKR_SURROGATE, JKR_EXACT
+
+    sizes = bfm.ensemble_sizes(args.ens_reps, args.ens_lo, args.ens_hi, seed=0)
+    # half-lengths below Ldiv / 2 - R: no cell starts past the division length
+    reps, pheno = bfm.make_ensemble(sizes, seeds=range(args.ens_reps), hl_hi=1.2)
+    total = int(sizes.sum())
+    cap = int(total * 1.3) + 1024
+    params = make_params(jkr_mode=JKR_SURROGATE if args.ens_jkr == "surrogate" else JKR_EXACT)
+    ctx = bfm.Context(cap, max_pairs=16 * cap, params=params)
+    if args.ens_jkr == "surrogate":
+        from bfm import surrogate as sur
+        sur.install(ctx, sur.load())
+    cfg = sim.sim_config(max_time=args.ens_minutes / 60.0, tableau=args.ens_tableau, seed=0,
+                         report_seconds=20.0, max_wall_seconds=args.ens_max_wall)
+    out, t_out, it_out, st = sim.ensemble_simulate(ctx, cfg, reps, list(range(args.ens_reps)),
+                                                   autotune=True, cap=cap)
+    value = st["stage_cell_updates"] / st["wall_seconds"]
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        A, b, bs, _ = bfm.tableau(args.ens_tableau)
+        p_exact = make_params()
+        t_cpu, upd, used = 0.0, 0, []
+        for r in np.argsort(sizes)[len(sizes) // 2:len(sizes) // 2 + 4]:
+            cells = reps[r]
+            nb = orc.get_cell_neighbors(cells, cfg.neighbor_threshold, cfg.R, cfg.Ldiv, binned=True)
+            t0 = time.perf_counter()
+            orc.step_rk(A, b, bs, 1e-6, cells, nb, p_exact)
+            t_cpu += time.perf_counter() - t0
+            upd += cells.shape[0] * len(b)
+            used.append(int(cells.shape[0]))
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        cpu = dict(value=upd / t_cpu, unit="stage cell-updates/s", cores=threads, kind="port",
+                   sample=f"oracle stepRungeKuttaAdaptive ({args.ens_tableau}, exact JKR, prebuilt lists) "
+                          f"on 4 median-size replicates {used}, one step each")
+    out_j = {"metric": "batched-ensemble RK stage cell-updates/s (3D, FP64)",
+             "value": value, "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True,
+             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+             "config": {"workload": f"{args.ens_reps} replicate domes, {args.ens_lo}-{args.ens_hi} cells "
+                                    f"log-uniform, gamma 50/500 Pa um 50/50, {args.ens_minutes} simulated min",
+                        "replicates": args.ens_reps, "cells_start": total, "cells_end": st["n_final"],
+                        "tableau": args.ens_tableau, "jkr": args.ens_jkr,
+                        "phenotype_counts": [int((pheno == 0).sum()), int((pheno == 1).sum())]},
+             "ensemble": {"wall_s": st["wall_seconds"], "iterations": st["iterations"],
+                          "attempts": st["attempts"], "divisions": st["divisions"],
+                          "neighbor_builds": st["neighbor_builds"],
+                          "host_division_s": st["host_division_seconds"],
+                          "autotune_ms_per_build": st["autotune_ms"],
+                          "tuned_bin_scale": st["tuned_bin_scale"], "tuned_variant": st["tuned_variant"],
+                          "failed_replicates": st["failed_replicates"],
+                          "t_reached_min_h": float(t_out.min()), "t_reached_median_h": float(np.median(t_out)),
+                          "t_target_h": args.ens_minutes / 60.0,
+                          "replicates_finished": int(np.sum(t_out >= args.ens_minutes / 60.0))},
+             "cpu_baseline": cpu}
+    print(json.dumps(out_j))
 
 
 if __name__ == "__main__":

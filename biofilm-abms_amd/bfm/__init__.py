@@ -12,4 +12,4 @@ from .core import (  # noqa: F401
     getCellNeighbors, updateNeighborDistances, getConservativeForces, getVelocities,
     stepRungeKuttaAdaptive, repulsion_prefactors, COL,
 )
-from .workloads import make_dome  # noqa: F401
+from .workloads import make_dome, make_ensemble, ensemble_sizes  # noqa: F401

@@ -156,6 +156,10 @@ def load_library():
         "bfm_divide_cells": [vp, _dp, _ip, C.c_int, C.c_int, C.c_double, C.c_uint64, _ip, _ip,
                              C.c_int, _ip],
         "bfm_rng_uniform": [C.c_uint64, C.c_int, _dp],
+        "bfm_ensemble_simulate": [vp, vp, C.c_int, _ip, _dp, C.c_int, vp, C.c_int, _dp, C.c_int, _ip,
+                                  _dp, vp, _ip, vp],
+        "bfm_autotune_neighbors": [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                                   _ip, _dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -37,6 +37,15 @@ class SimStats(C.Structure):
                 ("wall_seconds", C.c_double), ("host_division_seconds", C.c_double)]
 
 
+class EnsembleStats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("attempts", C.c_int64), ("divisions", C.c_int64),
+                ("neighbor_builds", C.c_int64), ("stage_cell_updates", C.c_int64),
+                ("wall_seconds", C.c_double), ("host_division_seconds", C.c_double),
+                ("autotune_ms", C.c_double), ("tuned_bin_scale", C.c_double),
+                ("tuned_variant", C.c_int), ("n_final", C.c_int), ("failed", C.c_int),
+                ("pad", C.c_int)]
+
+
 def sim_config(n_cells=0, max_iter=0, max_time=0.0, R=0.8, Rcell=0.5, L0=0.4,
                max_stepsize=1e-4, min_stepsize=1e-9, max_error_allowed=1e-8, min_error=1e-30,
                growth_mean=(1.5, 1.5), growth_std=(0.15, 0.15), daughter_length_std=0.125,
@@ -143,3 +152,40 @@ def rng_uniform(seed, count):
     if rc != 0:
         raise BFMError(rc, "bfm_rng_uniform failed")
     return out
+
+
+def ensemble_simulate(ctx: Context, cfg: SimConfig, replicates, seeds, autotune=True, cap=None):
+    """Runs every replicate's loop on the GPU in lockstep (bfm_ensemble_simulate).
+    replicates: list of (n_r x ncols) cell arrays; seeds: one division stream
+    seed per replicate. Returns (list of final cell arrays, t per replicate,
+    iterations per replicate, stats dict with the failed replicates' indices)."""
+    lib = load_library()
+    n_rep = len(replicates)
+    ncols = replicates[0].shape[1]
+    sizes = np.array([r.shape[0] for r in replicates], dtype=np.int32)
+    cells = np.asfortranarray(np.vstack(replicates), dtype=np.float64)
+    cap = int(cap or ctx.max_cells)
+    out = np.zeros((cap, ncols), order="F")
+    sizes_out = np.zeros(n_rep, dtype=np.int32)
+    t_out = np.zeros(n_rep)
+    it_out = np.zeros(n_rep, dtype=np.int64)
+    failed = np.zeros(n_rep, dtype=np.int32)
+    seeds_a = np.ascontiguousarray(seeds, dtype=np.uint64)
+    st = EnsembleStats()
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    rc = lib.bfm_ensemble_simulate(ctx.h, C.byref(cfg), n_rep, sizes.ctypes.data_as(ip),
+                                   cells.ctypes.data_as(dp), ncols,
+                                   seeds_a.ctypes.data_as(C.POINTER(C.c_uint64)), int(bool(autotune)),
+                                   out.ctypes.data_as(dp), cap, sizes_out.ctypes.data_as(ip),
+                                   t_out.ctypes.data_as(dp),
+                                   it_out.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   failed.ctypes.data_as(ip), C.byref(st))
+    ctx._check(rc)
+    stats = {k: getattr(st, k) for k, _ in EnsembleStats._fields_}
+    stats["failed_replicates"] = np.nonzero(failed)[0].tolist()
+    res, o = [], 0
+    for r in range(n_rep):
+        res.append(np.asfortranarray(out[o:o + sizes_out[r]]))
+        o += sizes_out[r]
+    return res, t_out, it_out, stats

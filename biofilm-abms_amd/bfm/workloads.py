@@ -36,3 +36,24 @@ def make_dome(n, seed=42, R=0.8, Rcell=0.5, phi=0.5, hl_lo=0.4, hl_hi=1.6, gamma
     if got != n:
         raise RuntimeError(f"dome generator placed only {got} of {n} cells")
     return cells, rd.value
+
+
+def ensemble_sizes(n_rep=256, lo=500, hi=20000, seed=0):
+    """Replicate cell counts round(exp(U[ln lo, ln hi])) from one seeded stream."""
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n_rep))).astype(np.int64)
+
+
+def make_ensemble(sizes, seeds=None, gammas=(50.0, 500.0), phenotype_seed=1234, **dome_kw):
+    """One dome snapshot per replicate (replicate r: make_dome(sizes[r], seed=seeds[r])),
+    each replicate assigned one of two phenotypes (surface energy gammas[0] or
+    gammas[1], Pa um) by a seeded 50/50 draw. Returns (list of cell arrays,
+    phenotype per replicate)."""
+    n_rep = len(sizes)
+    seeds = list(range(n_rep)) if seeds is None else list(seeds)
+    pheno = np.random.default_rng(phenotype_seed).permutation(np.arange(n_rep) % 2)
+    reps = []
+    for r in range(n_rep):
+        cells, _ = make_dome(int(sizes[r]), seed=seeds[r], gamma=gammas[int(pheno[r])], **dome_kw)
+        reps.append(cells)
+    return reps, pheno

@@ -25,6 +25,15 @@ struct Plan {
     int ok;
 };
 
+// Grid of one replicate in a batched neighbor build: its own origin and
+// dimensions, the shared edge of Plan, and the offset of its bins in the
+// global bin numbering. A single colony is replicate 0 with base 0.
+struct RepGrid {
+    double lo[3];
+    int dims[3];
+    int base;
+};
+
 // Device status bits, checked at synchronisation points.
 enum StatusBits {
     ST_SLOT_OVERFLOW = 1,   // a cell has more neighbors than the per-cell slot capacity
@@ -92,8 +101,8 @@ struct bfm_ctx {
     int* d_skey = nullptr;          // sorted keys
     int* d_count = nullptr;         // per-cell neighbor counts (max_cells + 1)
     int* d_offsets = nullptr;       // exclusive scan of counts (max_cells + 1)
-    int* d_slot_j = nullptr;        // slot_cap x max_cells (entry-major)
-    double* d_slot_g = nullptr;     // 5 x slot_cap x max_cells (component, entry, cell)
+    int* d_slot_j = nullptr;        // slot_cap x max_cells (entry-major): partner j
+    double* d_slot_rec = nullptr;   // slot_cap x max_cells records of 8 doubles: d, s, t, (partner slot), pad
 
     // internal neighbor list (SoA)
     int* pi = nullptr; int* pj = nullptr;
@@ -110,7 +119,6 @@ struct bfm_ctx {
     unsigned long long* d_off64 = nullptr;  // exclusive scan of d_cnt64
     int* d_jslot = nullptr;         // max_cells x slot_cap: partners with larger index
     int* d_jrow = nullptr;          // max_cells x slot_cap: row index of each jslot pair
-    int* d_slot_s2 = nullptr;       // max_cells x slot_cap: position of the pair in j's jslot
     int* d_csr = nullptr;           // 2 x max_pairs row indices (bit 31: j role)
     double* d_rec = nullptr;        // 8 x max_pairs: per-row {d, s, t, main, hard, 0}
 
@@ -144,6 +152,13 @@ struct bfm_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, int64_t>> ktimes;
     std::vector<std::string> korder;
+
+    // batched replicates (0 = a single colony)
+    int n_rep = 0;
+    int max_reps = 0;
+    int* d_rep = nullptr;                   // max_cells: replicate of each cell
+    bfm::RepGrid* d_grid = nullptr;         // max(1, max_reps) replicate grids
+    unsigned long long* d_rep_box = nullptr;  // max_reps x 7 ordered keys: lo[3], hi[3], max half_l
 
     // last build stats
     double last_edge = 0;

@@ -1,0 +1,39 @@
+// bfm_internal.hpp -- library-internal entry points shared by the simulation
+// drivers (bfm_sim.hip) and the mechanical-step implementation (bfm_main.hip).
+// Each returns a bfm_status and sets the context's error message, like the
+// C ABI functions.
+#pragma once
+
+#include "bfm_ctx.hpp"
+
+namespace bfm {
+
+// Frozen-geometry part of a step (mechanics.hpp:2483-2556): assume_2d from the
+// step-start orientations, pair forces and the per-cell cell-cell forces.
+// Every RK attempt of one iteration reuses them.
+int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld);
+
+// One RK attempt of a batch of replicates after step_prepare: the stages and
+// the update, with the cell count read from d_n and each replicate's dt and
+// active flag from d_dt / d_active (inactive rows are copied unchanged);
+// d_errmax[r] receives replicate r's scaled-error maximum as an ordered key
+// (reset inside); d_bad[r] is or-ed with 1 when replicate r produced a
+// non-finite position or orientation (the caller clears it). Enqueues only
+// kernels, so a stream capture records it.
+int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld, const int* d_n,
+                         const int* d_rep, const double* d_dt, const int* d_active, int n_rep,
+                         double* d_cells_new, double tol, unsigned long long* d_errmax, int* d_bad);
+
+// growCells with per-replicate dt and active flags, cell count from d_n.
+int grow_batched(bfm_ctx* c, double* d_cells, int ld, const int* d_n, double R, const int* d_rep,
+                 const double* d_dt, const int* d_active);
+
+// Switches the context to n_rep batched replicates (0: a single colony);
+// allocates the replicate arrays on first use. d_rep must then hold each
+// cell's replicate before a neighbor build.
+int set_replicate_count(bfm_ctx* c, int n_rep);
+
+// Ordered-key <-> double for the scaled-error maxima.
+double order_key_value(unsigned long long k);
+
+}  // namespace bfm

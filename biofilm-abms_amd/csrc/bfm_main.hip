@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "bfm_ctx.hpp"
+#include "bfm_internal.hpp"
 #include "bfm_device.hpp"
 #include "bfm_surrogate.hpp"
 
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(256) k_bbox_partial(const double* __restrict__
 // since |d| >= |r_i - r_j| - hl_i - hl_j, |r_i - r_j| < thr + 2 max(hl).
 __global__ void __launch_bounds__(256) k_plan(const double* __restrict__ partials, int nparts,
                                               double thr, double pre, double bin_scale, int max_bins,
-                                              Plan* __restrict__ plan)
+                                              Plan* __restrict__ plan, RepGrid* __restrict__ grid)
 {
     __shared__ double red[7][256];
     const int tid = threadIdx.x;
@@ -260,6 +261,10 @@ __global__ void __launch_bounds__(256) k_plan(const double* __restrict__ partial
     P.thr = thr;
     P.ok = 1;
     *plan = P;
+    RepGrid G;
+    for (int k = 0; k < 3; ++k) { G.lo[k] = P.lo[k]; G.dims[k] = P.dims[k]; }
+    G.base = 0;
+    *grid = G;
 }
 
 __device__ __forceinline__ int bin_coord(double x, double lo, double edge, int dim)
@@ -323,6 +328,18 @@ __device__ __forceinline__ Seg load_sorted(const double* __restrict__ s, int64_t
     return c;
 }
 
+// MI355X dispatches work-groups to its 8 XCDs round-robin (work-group b runs
+// on XCD b % 8), and each XCD has its own L2. This bijection gives every XCD
+// one contiguous range of logical blocks, so the bins a range stages (a slab
+// of the grid plus its halo) stay in that XCD's L2 instead of being fetched
+// by all eight.
+__device__ __forceinline__ int xcd_block(int b, int nblocks)
+{
+    const int q = nblocks >> 3, r = nblocks & 7;
+    const int xcd = b & 7, k = b >> 3;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
+}
+
 // LDS-staged candidate search over a half stencil. A workgroup owns HOMES
 // consecutive bin-ordered cells; homes sharing a grid row (same y, z bin)
 // form a group. Every unordered pair is screened once, by the cell that
@@ -342,8 +359,9 @@ template <int HOMES, int CAP, int BATCH, int QCAP>
 __global__ void __launch_bounds__(256)
 k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
                  const int* __restrict__ sid, const int* __restrict__ skey,
+                 const int* __restrict__ rep, const RepGrid* __restrict__ grid,
                  const int* __restrict__ bin_off, int slot_cap, unsigned long long* __restrict__ cnt64,
-                 int* __restrict__ slot_j, int* __restrict__ slot_s2, double* __restrict__ slot_g,
+                 int* __restrict__ slot_j, double* __restrict__ slot_rec,
                  int* __restrict__ jslot, unsigned long long* __restrict__ counters)
 {
     static_assert(HOMES <= 64 && CAP <= 1024 && BATCH * CAP <= QCAP, "queue entry packing / bound");
@@ -352,14 +370,14 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
     __shared__ uint16_t queue[QCAP];
     __shared__ int s_ovf;
     __shared__ double h_x[HOMES], h_y[HOMES], h_z[HOMES], h_nx[HOMES], h_ny[HOMES], h_nz[HOMES], h_hl[HOMES];
-    __shared__ int h_id[HOMES], h_bx[HOMES], h_row[HOMES];
+    __shared__ int h_id[HOMES], h_bx[HOMES], h_row[HOMES], h_rep[HOMES];
     __shared__ float4 h_f[HOMES];           // FP32 (x, y, z, thr + hl + 1e-3)
     __shared__ int seg_lo[49], seg_pre[50];
     __shared__ int s_qn;
     __shared__ unsigned long long s_screened;
 
     const int tid = threadIdx.x;
-    const int h0 = blockIdx.x * HOMES;
+    const int h0 = xcd_block(blockIdx.x, gridDim.x) * HOMES;
     if (h0 >= n) return;
     const int H = min(HOMES, n - h0);
     const Plan P = *plan;
@@ -373,9 +391,14 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
         h_id[tid] = sid[hs];
         h_f[tid] = make_float4((float)h_x[tid], (float)h_y[tid], (float)h_z[tid],
                                (float)(thr + h_hl[tid] + P.fmargin));
+        // bins of replicate r are numbered grid[r].base + (z dims_y + y) dims_x + x;
+        // key - x identifies the (replicate, y, z) row
         const int key = skey[hs];
-        h_bx[tid] = key % P.dims[0];
-        h_row[tid] = key / P.dims[0];
+        const int r = rep ? rep[h_id[tid]] : 0;
+        const int bx = (key - grid[r].base) % grid[r].dims[0];
+        h_bx[tid] = bx;
+        h_row[tid] = key - bx;
+        h_rep[tid] = r;
     }
     if (tid == 0) s_screened = 0;
     __syncthreads();
@@ -388,17 +411,19 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
         int g1 = g0 + 1;
         while (g1 < H && h_row[g1] == row) ++g1;
         const int xmin = h_bx[g0], xmax = h_bx[g1 - 1];
-        const int by = row % P.dims[1], bz = row / P.dims[1];
+        const RepGrid Gr = grid[h_rep[g0]];
+        const int rrow = (row - Gr.base) / Gr.dims[0];
+        const int by = rrow % Gr.dims[1], bz = rrow / Gr.dims[1];
         const int last = h0 + g1 - 1;                  // sorted position of the group's last home
         if (tid < nseg) {
             // rows (dz, dy) lexicographically before (0, 0), then (0, 0) itself
             const int dz = tid / side - reach, dy = tid % side - reach;
             const int zz = bz + dz, yy = by + dy;
             int lo = 0, len = 0;
-            if (zz >= 0 && zz < P.dims[2] && yy >= 0 && yy < P.dims[1]) {
-                const int rb = (zz * P.dims[1] + yy) * P.dims[0];
+            if (zz >= 0 && zz < Gr.dims[2] && yy >= 0 && yy < Gr.dims[1]) {
+                const int rb = Gr.base + (zz * Gr.dims[1] + yy) * Gr.dims[0];
                 const int xlo = max(0, xmin - reach);
-                const int xhi = (dz == 0 && dy == 0) ? xmax : min(P.dims[0] - 1, xmax + reach);
+                const int xhi = (dz == 0 && dy == 0) ? xmax : min(Gr.dims[0] - 1, xmax + reach);
                 lo = bin_off[rb + xlo];
                 int hi = bin_off[rb + xhi + 1];
                 if (dz == 0 && dy == 0) hi = min(hi, last);
@@ -425,7 +450,7 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
                                      (float)sorted[6 * stride + q]);
                 c_id[t] = sid[q];
                 c_q[t] = q;
-                c_bx[t] = skey[q] % P.dims[0];
+                c_bx[t] = (skey[q] - Gr.base) % Gr.dims[0];
             }
             __syncthreads();
             // First try the whole group as one batch; if the queue would
@@ -501,11 +526,11 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
                             const int slot2 = (int)(atomicAdd(&cnt64[j], 1ull) & 0xffffffffull);
                             if (slot < slot_cap) {
                                 const int64_t e = (int64_t)slot * stride + i;
-                                const int64_t plane = (int64_t)slot_cap * stride;
                                 slot_j[e] = j;
-                                slot_s2[e] = slot2;
-                                slot_g[e] = d[0]; slot_g[plane + e] = d[1]; slot_g[2 * plane + e] = d[2];
-                                slot_g[3 * plane + e] = sv; slot_g[4 * plane + e] = tv;
+                                double2* r = reinterpret_cast<double2*>(slot_rec + 8 * e);
+                                r[0] = make_double2(d[0], d[1]);
+                                r[1] = make_double2(d[2], sv);
+                                r[2] = make_double2(tv, __longlong_as_double((long long)slot2));
                             }
                             if (slot2 < slot_cap) jslot[(int64_t)slot2 * stride + j] = i;
                         }
@@ -538,13 +563,12 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
 // stored at the pair's slot in j's partner list (jrow).
 __global__ void k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
                                const unsigned long long* __restrict__ off64, const int* __restrict__ slot_j,
-                               const int* __restrict__ slot_s2, const double* __restrict__ slot_g,
+                               const double* __restrict__ slot_rec,
                                int64_t max_pairs, int* __restrict__ pi, int* __restrict__ pj,
                                double* __restrict__ pdx, double* __restrict__ pdy, double* __restrict__ pdz,
                                double* __restrict__ ps, double* __restrict__ pt, int* __restrict__ csr,
                                int* __restrict__ jrow, int* __restrict__ d_m, int* __restrict__ status)
 {
-    const int64_t plane = (int64_t)slot_cap * stride;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         const unsigned long long cv = cnt64[c], ov = off64[c];
         int ci = (int)(cv >> 32);
@@ -562,11 +586,13 @@ __global__ void k_compact_rows(int n, int64_t stride, int slot_cap, const unsign
             for (int f = 0; f < ci; ++f) rank += slot_j[(int64_t)f * stride + c] < je;
             const int64_t row = oi + rank;
             if (!fits) continue;
+            const double2* r = reinterpret_cast<const double2*>(slot_rec + 8 * se);
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2];
             pi[row] = c; pj[row] = je;
-            pdx[row] = slot_g[se]; pdy[row] = slot_g[plane + se]; pdz[row] = slot_g[2 * plane + se];
-            ps[row] = slot_g[3 * plane + se]; pt[row] = slot_g[4 * plane + se];
+            pdx[row] = r0.x; pdy[row] = r0.y; pdz[row] = r1.x;
+            ps[row] = r1.y; pt[row] = r2.x;
             csr[base2 + rank] = (int)row;
-            const int s2 = slot_s2[se];
+            const int s2 = (int)__double_as_longlong(r2.y);
             if (s2 < slot_cap) jrow[(int64_t)s2 * stride + je] = (int)row;
         }
         if (c == n - 1) {
@@ -942,12 +968,23 @@ __global__ void k_assume2d(int n, const double* __restrict__ cells, int ld, doub
 // cells_i = cells + dt * sum_j A(st, j) k_j, normalizeOrientations, then
 // getVelocities with the step's frozen neighbor geometry (cell-cell forces cc
 // are identical in every stage, because they only read the neighbor rows).
+// Batched replicates: dn (device cell count) replaces n, and rep / dt_rep /
+// active give each cell its replicate's step size; cells of inactive
+// replicates are skipped.
 __global__ void __launch_bounds__(256)
-k_rk_stage(int n, const double* __restrict__ cells, int ld, const int* __restrict__ assume_2d,
-           const double* __restrict__ noise, VelParams vp, Tableau tb, int st, double dt,
-           const double* __restrict__ cc, double* __restrict__ kbuf, int* __restrict__ status)
+k_rk_stage(int n, const int* __restrict__ dn, const double* __restrict__ cells, int ld,
+           const int* __restrict__ assume_2d, const double* __restrict__ noise, VelParams vp, Tableau tb, int st,
+           double dt, const int* __restrict__ rep, const double* __restrict__ dt_rep,
+           const int* __restrict__ active, const double* __restrict__ cc, double* __restrict__ kbuf,
+           int* __restrict__ status)
 {
+    if (dn) n = *dn;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        if (rep) {
+            const int r = rep[c];
+            if (!active[r]) continue;
+            dt = dt_rep[r];
+        }
         CellVel cv = load_cellvel(cells, ld, c);
         if (st > 0) {
             double y[6] = {cellv(cells, ld, c, C_RX), cellv(cells, ld, c, C_RY), cv.rz,
@@ -979,7 +1016,8 @@ k_rk_stage(int n, const double* __restrict__ cells, int ld, const int* __restric
             kbuf[((int64_t)st * 6 + q) * n + c] = v[q];
             bad |= !isfinite(v[q]);
         }
-        if (bad) atomicOr(status, ST_NAN);
+        // batched replicates report non-finite results per replicate (k_rk_final_b)
+        if (bad && !rep) atomicOr(status, ST_NAN);
     }
 }
 
@@ -995,6 +1033,14 @@ __device__ __forceinline__ double order_val(unsigned long long k)
 }
 
 __global__ void k_err_init(unsigned long long* e) { *e = order_key(-INFINITY); }
+
+inline double order_val_host(unsigned long long k)
+{
+    unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    double d;
+    std::memcpy(&d, &u, sizeof(d));
+    return d;
+}
 __global__ void k_err_final(const unsigned long long* e, double* out) { *out = order_val(*e); }
 
 // RK update (mechanics.hpp:2559-2611): weighted sums, position update,
@@ -1044,6 +1090,247 @@ k_rk_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau t
     }
 }
 
+// k_rk_final for batched replicates: device cell count, per-replicate dt and
+// active mask (inactive replicates' rows are copied unchanged), and the
+// scaled-error maximum per replicate (errmax[r]). Replicates occupy runs of
+// consecutive rows, so each block reduces its 256 values per run before one
+// atomic per run.
+__global__ void __launch_bounds__(256)
+k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld, Tableau tb,
+             const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
+             const double* __restrict__ kbuf, double* __restrict__ cells_new, double tol,
+             unsigned long long* __restrict__ errmax, int* __restrict__ bad_rep)
+{
+    __shared__ double s_val[256];
+    __shared__ int s_rep[256];
+    const int n = *dn;
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const int c = base + threadIdx.x;
+        double local = -INFINITY;
+        int r = -1;
+        if (c < n) {
+            r = rep[c];
+            if (!active[r]) {
+                for (int col = 0; col < ncols; ++col) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
+                r = -1;
+            } else {
+                const double dt = dt_rep[r];
+                for (int col = 0; col < ncols; ++col)
+                    if (col < 1 || col > 12) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
+                double y[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    double v1 = 0, v2 = 0;
+                    for (int st = 0; st < tb.s; ++st) {
+                        const double kv = kbuf[((int64_t)st * 6 + q) * n + c];
+                        v1 += kv * tb.b[st];
+                        v2 += kv * tb.bs[st];
+                    }
+                    const double d1 = v1 * dt, d2 = v2 * dt;
+                    const double y0 = cellv(cells, ld, c, C_RX + q);
+                    y[q] = y0 + d1;
+                    cells_new[(int64_t)(C_DRX + q) * ld + c] = v1;
+                    local = fmax(local, (d1 - d2) / (tol * (1.0 + fabs(y0))));
+                }
+                double nrm = norm3(y[3], y[4], y[5]);
+                y[3] /= nrm; y[4] /= nrm; y[5] /= nrm;
+                if (y[5] < 0) { y[3] *= -1; y[4] *= -1; y[5] *= -1; }
+                bool bad = false;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    cells_new[(int64_t)(C_RX + q) * ld + c] = y[q];
+                    bad |= !isfinite(y[q]);
+                }
+                if (bad) atomicOr(&bad_rep[r], 1);
+            }
+        }
+        s_val[threadIdx.x] = local;
+        s_rep[threadIdx.x] = r;
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (r >= 0 && (t == 0 || s_rep[t - 1] != r)) {
+            double m = local;
+            for (int u = t + 1; u < 256 && s_rep[u] == r; ++u) m = fmax(m, s_val[u]);
+            atomicMax(&errmax[r], order_key(m));
+        }
+        __syncthreads();
+    }
+}
+
+// growCells (growth.hpp:51-58) with each replicate's dt (inactive: unchanged).
+__global__ void k_grow_b(const int* __restrict__ dn, double* __restrict__ cells, int ld, double R,
+                         const int* __restrict__ rep, const double* __restrict__ dt_rep,
+                         const int* __restrict__ active)
+{
+    const int n = *dn;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int r = rep[i];
+        if (!active[r]) continue;
+        double l = cells[(int64_t)C_L * ld + i];
+        const double g = cells[(int64_t)C_GROWTH * ld + i];
+        l = l + (g * (4 * R / 3 + l)) * dt_rep[r];
+        cells[(int64_t)C_L * ld + i] = l;
+        cells[(int64_t)C_HALF_L * ld + i] = l / 2;
+    }
+}
+
+__global__ void k_fill_key(unsigned long long* __restrict__ p, int n, double v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = order_key(v);
+}
+
+__global__ void k_box_init(unsigned long long* __restrict__ box, int n_rep)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rep) return;
+    for (int k = 0; k < 7; ++k) box[r * 7 + k] = order_key(k < 3 ? INFINITY : -INFINITY);
+}
+
+// Per-replicate bounding boxes and max half-length (ordered-key atomics; the
+// block first reduces each run of equal replicate ids).
+__global__ void __launch_bounds__(256)
+k_bbox_rep(const double* __restrict__ cells, int n, int ld, const int* __restrict__ rep,
+           unsigned long long* __restrict__ box)
+{
+    __shared__ double s_v[7][256];
+    __shared__ int s_rep[256];
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const int i = base + threadIdx.x;
+        const int r = i < n ? rep[i] : -1;
+        if (r >= 0) {
+            for (int k = 0; k < 3; ++k) {
+                s_v[k][threadIdx.x] = cellv(cells, ld, i, C_RX + k);
+                s_v[3 + k][threadIdx.x] = s_v[k][threadIdx.x];
+            }
+            s_v[6][threadIdx.x] = cellv(cells, ld, i, C_HALF_L);
+        }
+        s_rep[threadIdx.x] = r;
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (r >= 0 && (t == 0 || s_rep[t - 1] != r)) {
+            double v[7];
+            for (int k = 0; k < 7; ++k) v[k] = s_v[k][t];
+            for (int u = t + 1; u < 256 && s_rep[u] == r; ++u) {
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = fmin(v[k], s_v[k][u]);
+                    v[3 + k] = fmax(v[3 + k], s_v[3 + k][u]);
+                }
+                v[6] = fmax(v[6], s_v[6][u]);
+            }
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&box[r * 7 + k], order_key(v[k]));
+                atomicMax(&box[r * 7 + 3 + k], order_key(v[3 + k]));
+            }
+            atomicMax(&box[r * 7 + 6], order_key(v[6]));
+        }
+        __syncthreads();
+    }
+}
+
+// Grid plans of all replicates with one shared edge (so one stencil reach):
+// the edge grows by 1.25 until the replicates' bins fit in max_bins; bases are
+// the exclusive prefix sums of the replicates' bin counts. One block.
+__global__ void __launch_bounds__(256)
+k_plan_rep(const unsigned long long* __restrict__ box, int n_rep, double thr, double pre, double bin_scale,
+           int max_bins, Plan* __restrict__ plan, RepGrid* __restrict__ grid)
+{
+    __shared__ double s_red[256];
+    __shared__ long long s_tot[256];
+    __shared__ double s_edge;
+    const int t = threadIdx.x;
+    double hl = 0, amax = 0;
+    for (int r = t; r < n_rep; r += 256) {
+        const double lo0 = order_val(box[r * 7]);
+        if (!(lo0 <= order_val(box[r * 7 + 3]))) continue;   // empty replicate
+        hl = fmax(hl, order_val(box[r * 7 + 6]));
+        for (int k = 0; k < 3; ++k)
+            amax = fmax(amax, fmax(fabs(order_val(box[r * 7 + k])), fabs(order_val(box[r * 7 + 3 + k]))));
+    }
+    s_red[t] = hl;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) { if (t < w) s_red[t] = fmax(s_red[t], s_red[t + w]); __syncthreads(); }
+    hl = s_red[0];
+    __syncthreads();
+    s_red[t] = amax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) { if (t < w) s_red[t] = fmax(s_red[t], s_red[t + w]); __syncthreads(); }
+    amax = s_red[0];
+    const double cut = fmin(pre, thr + 2 * hl) * (1 + 1e-9) + 1e-9;
+    if (t == 0) s_edge = cut * bin_scale;
+    __syncthreads();
+    for (;;) {
+        const double edge = s_edge;
+        long long tot = 0;
+        for (int r = t; r < n_rep; r += 256) {
+            long long b = 1;
+            const double lo0 = order_val(box[r * 7]);
+            if (lo0 <= order_val(box[r * 7 + 3]))
+                for (int k = 0; k < 3; ++k)
+                    b *= (long long)floor((order_val(box[r * 7 + 3 + k]) - order_val(box[r * 7 + k])) / edge) + 1;
+            tot += b;
+        }
+        s_tot[t] = tot;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) { if (t < w) s_tot[t] += s_tot[t + w]; __syncthreads(); }
+        const bool fits = s_tot[0] <= max_bins;
+        __syncthreads();
+        if (fits) break;
+        if (t == 0) s_edge = edge * 1.25;
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const double edge = s_edge;
+    int reach = (int)ceil(cut / edge - 1e-12);
+    if (reach < 1) reach = 1;
+    if (reach * edge < cut) reach++;
+    int base = 0;
+    for (int r = 0; r < n_rep; ++r) {
+        RepGrid G;
+        const double lo0 = order_val(box[r * 7]);
+        if (lo0 <= order_val(box[r * 7 + 3])) {
+            for (int k = 0; k < 3; ++k) {
+                G.lo[k] = order_val(box[r * 7 + k]);
+                G.dims[k] = (int)floor((order_val(box[r * 7 + 3 + k]) - G.lo[k]) / edge) + 1;
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) { G.lo[k] = 0; G.dims[k] = 1; }
+        }
+        G.base = base;
+        base += G.dims[0] * G.dims[1] * G.dims[2];
+        grid[r] = G;
+    }
+    Plan P;
+    for (int k = 0; k < 3; ++k) { P.lo[k] = 0; P.dims[k] = 1; }
+    P.edge = edge;
+    P.inv_edge = 1.0 / edge;
+    P.reach = reach;
+    P.nbins = base;
+    P.fmargin = 1e-3 + 1e-6 * amax;
+    P.pre = pre;
+    P.thr = thr;
+    P.ok = 1;
+    *plan = P;
+}
+
+// Bin keys of batched replicates: grid[rep].base + local bin.
+__global__ void k_keys_rep(const double* __restrict__ cells, int n, int ld, const Plan* __restrict__ plan,
+                           const int* __restrict__ rep, const RepGrid* __restrict__ grid,
+                           int* __restrict__ keys, int* __restrict__ vals, int* __restrict__ hist)
+{
+    const double edge = plan->edge;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const RepGrid G = grid[rep[i]];
+        int bx = bin_coord(cellv(cells, ld, i, C_RX), G.lo[0], edge, G.dims[0]);
+        int by = bin_coord(cellv(cells, ld, i, C_RY), G.lo[1], edge, G.dims[1]);
+        int bz = bin_coord(cellv(cells, ld, i, C_RZ), G.lo[2], edge, G.dims[2]);
+        int key = G.base + (bz * G.dims[1] + by) * G.dims[0] + bx;
+        keys[i] = key;
+        vals[i] = i;
+        atomicAdd(&hist[key], 1);
+    }
+}
+
 }  // namespace bfm
 
 // ---------------------------------------------------------------------------
@@ -1085,21 +1372,18 @@ void grow_slots(bfm_ctx* c)
     HIPCHK(hipStreamSynchronize(c->stream));
     const int cap = c->slot_cap * 2;
     int* sj = nullptr;
-    double* sg = nullptr;
+    double* sr = nullptr;
     HIPCHK(hipMalloc((void**)&sj, sizeof(int) * (size_t)c->max_cells * cap));
-    HIPCHK(hipMalloc((void**)&sg, sizeof(double) * (size_t)c->max_cells * cap * 5));
-    int *js = nullptr, *jr = nullptr, *s2 = nullptr;
+    HIPCHK(hipMalloc((void**)&sr, sizeof(double) * (size_t)c->max_cells * cap * 8));
+    int *js = nullptr, *jr = nullptr;
     HIPCHK(hipMalloc((void**)&js, sizeof(int) * (size_t)c->max_cells * cap));
     HIPCHK(hipMalloc((void**)&jr, sizeof(int) * (size_t)c->max_cells * cap));
-    HIPCHK(hipMalloc((void**)&s2, sizeof(int) * (size_t)c->max_cells * cap));
-    for (void* q : {(void*)c->d_slot_j, (void*)c->d_slot_g, (void*)c->d_jslot, (void*)c->d_jrow,
-                    (void*)c->d_slot_s2})
+    for (void* q : {(void*)c->d_slot_j, (void*)c->d_slot_rec, (void*)c->d_jslot, (void*)c->d_jrow})
         HIPCHK(hipFree(q));
     c->d_slot_j = sj;
-    c->d_slot_g = sg;
+    c->d_slot_rec = sr;
     c->d_jslot = js;
     c->d_jrow = jr;
-    c->d_slot_s2 = s2;
     c->slot_cap = cap;
 }
 
@@ -1179,8 +1463,9 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
 {
 #define BFM_CAND(HOMES, CAP, BATCH, QCAP)                                                              \
     KLAUNCH(c, "candidates", (k_candidates_lds<HOMES, CAP, BATCH, QCAP>), (n + HOMES - 1) / HOMES, 256, 0, n, \
-            c->d_plan, c->d_sorted, stride, c->d_sid, c->d_skey, c->d_bin_off, c->slot_cap, c->d_cnt64,   \
-            c->d_slot_j, c->d_slot_s2, c->d_slot_g, c->d_jslot, c->d_counters)
+            c->d_plan, c->d_sorted, stride, c->d_sid, c->d_skey, c->n_rep > 0 ? c->d_rep : (const int*)nullptr, \
+            c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64,                                             \
+            c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_counters)
     switch (c->cand_variant) {
         case 0: BFM_CAND(32, 640, 8, 8192); break;
         case 2: BFM_CAND(64, 1024, 8, 8192); break;
@@ -1206,13 +1491,26 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
         return;
     }
     const double pre = thr + Ldiv + 2 * R;
-    const int nparts = std::min(kBBoxParts, grid_for(n, 256));
-    KLAUNCH(c, "bbox", k_bbox_partial, nparts, 256, 0, d_cells, n, ld, c->d_partials);
-    KLAUNCH(c, "plan", k_plan, 1, 256, 0, c->d_partials, nparts, thr, pre, c->bin_scale, c->max_bins, c->d_plan);
+    if (c->n_rep > 0) {
+        KLAUNCH(c, "bbox", k_box_init, grid_for(c->n_rep, 256), 256, 0, c->d_rep_box, c->n_rep);
+        KLAUNCH(c, "bbox", k_bbox_rep, std::min(grid_for(n, 256), 2048), 256, 0, d_cells, n, ld, c->d_rep,
+                c->d_rep_box);
+        KLAUNCH(c, "plan", k_plan_rep, 1, 256, 0, c->d_rep_box, c->n_rep, thr, pre, c->bin_scale, c->max_bins,
+                c->d_plan, c->d_grid);
+    } else {
+        const int nparts = std::min(kBBoxParts, grid_for(n, 256));
+        KLAUNCH(c, "bbox", k_bbox_partial, nparts, 256, 0, d_cells, n, ld, c->d_partials);
+        KLAUNCH(c, "plan", k_plan, 1, 256, 0, c->d_partials, nparts, thr, pre, c->bin_scale, c->max_bins,
+                c->d_plan, c->d_grid);
+    }
     KLAUNCH(c, "fill", k_fill_int, grid_for(c->max_bins + 1, 256, 4096), 256, 0, c->d_bin_count, 0,
             (int64_t)c->max_bins + 1);
-    KLAUNCH(c, "keys", k_keys, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_keys, c->d_vals,
-            c->d_bin_count);
+    if (c->n_rep > 0)
+        KLAUNCH(c, "keys", k_keys_rep, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_rep, c->d_grid,
+                c->d_keys, c->d_vals, c->d_bin_count);
+    else
+        KLAUNCH(c, "keys", k_keys, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_keys, c->d_vals,
+                c->d_bin_count);
     scan(c, c->d_bin_count, c->d_bin_off, (int64_t)c->max_bins + 1, "scan_bins");
     HIPCHK(hipMemcpyAsync(c->d_bin_count, c->d_bin_off, sizeof(int) * ((size_t)c->max_bins + 1),
                           hipMemcpyDeviceToDevice, c->stream));
@@ -1227,7 +1525,7 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, c->d_cnt64, c->d_off64, n + 1, c->stream));
     }
     KLAUNCH(c, "compact_rows", k_compact_rows, grid_for(n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64, c->d_off64,
-            c->d_slot_j, c->d_slot_s2, c->d_slot_g, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps,
+            c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps,
             c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status);
     KLAUNCH(c, "compact_jrole", k_compact_jrole, grid_for(n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64,
             c->d_off64, c->d_jslot, c->d_jrow, c->max_pairs, c->d_csr);
@@ -1388,8 +1686,9 @@ void step_rk_device(bfm_ctx* c, const Tableau& tb, double dt, const double* d_ce
     launch_velocities(c, d_cells, n, ld, c->d_assume2d, d_noise, nullptr, c->d_cc);
     VelParams vp = vel_params(c);
     for (int st = 0; st < tb.s; ++st)
-        KLAUNCH(c, "rk_stage", k_rk_stage, grid_for(n, 256), 256, 0, n, d_cells, ld, c->d_assume2d, d_noise, vp,
-                tb, st, dt, c->d_cc, c->d_k, c->d_status);
+        KLAUNCH(c, "rk_stage", k_rk_stage, grid_for(n, 256), 256, 0, n, (const int*)nullptr, d_cells, ld,
+                c->d_assume2d, d_noise, vp, tb, st, dt, (const int*)nullptr, (const double*)nullptr,
+                (const int*)nullptr, c->d_cc, c->d_k, c->d_status);
     KLAUNCH(c, "err_init", k_err_init, 1, 1, 0, c->d_errmax);
     KLAUNCH(c, "rk_final", k_rk_final, grid_for(n, 256), 256, 0, n, c->p.ncols, d_cells, ld, tb, dt, c->d_k,
             d_cells_new, ld_new, d_errors, tol, c->d_errmax);
@@ -1417,6 +1716,79 @@ int guarded(bfm_ctx* c, F&& f)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// library-internal entry points (bfm_internal.hpp)
+// ---------------------------------------------------------------------------
+namespace bfm {
+
+constexpr int kBatchGrid = 2048;   // fixed grid of the graph-captured batched kernels
+
+int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
+        require(c->p.friction_mode == 0, BFM_ERR_UNSUPPORTED, "cell-cell friction is not supported");
+        compute_forces(c, d_cells, n, ld);
+        KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
+                c->d_assume2d);
+        launch_velocities(c, d_cells, n, ld, c->d_assume2d, nullptr, nullptr, c->d_cc);
+    });
+}
+
+int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld, const int* d_n,
+                         const int* d_rep, const double* d_dt, const int* d_active, int n_rep,
+                         double* d_cells_new, double tol, unsigned long long* d_errmax, int* d_bad)
+{
+    return guarded(c, [&] {
+        require(tol > 0, BFM_ERR_ARG, "tol must be > 0");
+        VelParams vp = vel_params(c);
+        hipLaunchKernelGGL(k_fill_key, dim3(grid_for(n_rep, 256)), dim3(256), 0, c->stream, d_errmax, n_rep,
+                           -INFINITY);
+        for (int st = 0; st < tb.s; ++st)
+            hipLaunchKernelGGL(k_rk_stage, dim3(kBatchGrid), dim3(256), 0, c->stream, 0, d_n, d_cells, ld,
+                               c->d_assume2d, (const double*)nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active,
+                               c->d_cc, c->d_k, c->d_status);
+        hipLaunchKernelGGL(k_rk_final_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld,
+                           tb, d_rep, d_dt, d_active, c->d_k, d_cells_new, tol, d_errmax, d_bad);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int grow_batched(bfm_ctx* c, double* d_cells, int ld, const int* d_n, double R, const int* d_rep,
+                 const double* d_dt, const int* d_active)
+{
+    return guarded(c, [&] {
+        hipLaunchKernelGGL(k_grow_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, d_cells, ld, R, d_rep, d_dt,
+                           d_active);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int set_replicate_count(bfm_ctx* c, int n_rep)
+{
+    return guarded(c, [&] {
+        require(n_rep >= 0, BFM_ERR_ARG, "n_rep < 0");
+        if (n_rep > c->max_reps) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (void* q : {(void*)c->d_grid, (void*)c->d_rep_box})
+                if (q) HIPCHK(hipFree(q));
+            c->d_grid = nullptr;
+            c->d_rep_box = nullptr;
+            dalloc(&c->d_grid, (size_t)n_rep);
+            dalloc(&c->d_rep_box, (size_t)n_rep * 7);
+            if (!c->d_rep) dalloc(&c->d_rep, (size_t)c->max_cells);
+            c->max_reps = n_rep;
+        }
+        c->n_rep = n_rep;
+    });
+}
+
+double order_key_value(unsigned long long k) { return order_val_host(k); }
+
+}  // namespace bfm
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1447,6 +1819,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_assume2d, N);
         dalloc(&c->d_partials, kBBoxParts * 7);
         dalloc(&c->d_plan, 1);
+        dalloc(&c->d_grid, 1);
         dalloc(&c->d_keys, N); dalloc(&c->d_keys_alt, N);
         dalloc(&c->d_vals, N); dalloc(&c->d_vals_alt, N);
         dalloc(&c->d_bin_count, (size_t)c->max_bins + 1);
@@ -1457,7 +1830,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_count, N + 1);
         dalloc(&c->d_offsets, N + 1);
         dalloc(&c->d_slot_j, N * c->slot_cap);
-        dalloc(&c->d_slot_g, N * c->slot_cap * 5);
+        dalloc(&c->d_slot_rec, N * c->slot_cap * 8);
         dalloc(&c->pi, P); dalloc(&c->pj, P);
         dalloc(&c->pdx, P); dalloc(&c->pdy, P); dalloc(&c->pdz, P);
         dalloc(&c->ps, P); dalloc(&c->pt, P);
@@ -1472,7 +1845,6 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_off64, N + 1);
         dalloc(&c->d_jslot, N * c->slot_cap);
         dalloc(&c->d_jrow, N * c->slot_cap);
-        dalloc(&c->d_slot_s2, N * c->slot_cap);
         dalloc(&c->d_csr, 2 * P);
         dalloc(&c->d_cc, 6 * N);
         dalloc(&c->d_k, 7 * 6 * N);
@@ -1508,8 +1880,8 @@ int bfm_destroy(bfm_ctx* c)
     void* ptrs[] = {c->d_status, c->d_counters, c->d_cells, c->d_cells_new, c->d_nb, c->d_noise, c->d_out6,
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
-                    c->d_offsets, c->d_slot_j, c->d_slot_g, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
-                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_slot_s2, c->d_csr, c->d_cc,
+                    c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
+                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -1786,6 +2158,47 @@ int bfm_dev_step_rk(bfm_ctx* c, const double* A, const double* b, const double* 
         check_cells(c, n, ld);
         Tableau tb = make_tableau(A, b, bs, s);
         step_rk_device(c, tb, dt, d_cells, n, ld, d_noise, d_cells_new, ld, d_errors, tol, d_err_scaled);
+    });
+}
+
+int bfm_autotune_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv,
+                           double* best_scale, int* best_variant, double* best_ms)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        static const double scales[] = {0.5, 0.75, 1.0, 1.25};
+        static const int variants[] = {0, 1, 2, 3};
+        double best = INFINITY, bs = c->bin_scale;
+        int bv = c->cand_variant;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const bool was_timing = c->timing;
+        c->timing = false;
+        for (double sc : scales)
+            for (int v : variants) {
+                c->bin_scale = sc;
+                c->cand_variant = v;
+                build_neighbors_sync(c, d_cells, n, ld, thr, R, Ldiv);   // warm-up, grows slots if needed
+                HIPCHK(hipEventRecord(e0, c->stream));
+                for (int k = 0; k < 3; ++k) build_neighbors(c, d_cells, n, ld, thr, R, Ldiv);
+                HIPCHK(hipEventRecord(e1, c->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                raise_status(c, take_status(c));
+                if (ms / 3 < best) { best = ms / 3; bs = sc; bv = v; }
+            }
+        c->timing = was_timing;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->bin_scale = bs;
+        c->cand_variant = bv;
+        build_neighbors_sync(c, d_cells, n, ld, thr, R, Ldiv);
+        if (best_scale) *best_scale = bs;
+        if (best_variant) *best_variant = bv;
+        if (best_ms) *best_ms = best;
     });
 }
 

@@ -24,6 +24,7 @@
 
 #include "bfm_ctx.hpp"
 #include "bfm_device.hpp"
+#include "bfm_internal.hpp"
 
 using namespace bfm;
 
@@ -151,10 +152,14 @@ void rotate_out_of_xy(const double n[3], double theta, double out[3])
     out[2] = n[2] * c + w2 * s;
 }
 
-// Query segment for the minimum-distance checks.
+// Query segment for the minimum-distance checks: distances to the cells in
+// [lo, hi) (one replicate's rows), skipping `self` and, when skip_dividing,
+// the cells being divided (their rows are rewritten).
 struct Query {
     double r[3], n[3], hl;
     int self;       // index in the population to skip (-1: none)
+    int lo, hi;
+    int skip_dividing;
 };
 
 Seg seg_of_row(const double* row)
@@ -164,6 +169,20 @@ Seg seg_of_row(const double* row)
     s.n[0] = row[C_NX]; s.n[1] = row[C_NY]; s.n[2] = row[C_NZ];
     s.hl = row[C_HALF_L];
     return s;
+}
+
+Query query_of_row(const double* row, int self, int lo, int hi, int skip_dividing)
+{
+    Query q;
+    Seg s = seg_of_row(row);
+    std::memcpy(q.r, s.r, sizeof(s.r));
+    std::memcpy(q.n, s.n, sizeof(s.n));
+    q.hl = s.hl;
+    q.self = self;
+    q.lo = lo;
+    q.hi = hi;
+    q.skip_dividing = skip_dividing;
+    return q;
 }
 
 // |d| of distBetweenCells(query, other), as minDistToCell (growth.hpp:128-162).
@@ -187,173 +206,182 @@ __host__ __device__ inline bool beyond_cut(const Seg& a, const Seg& b, double cu
 // Minimum distances from queries to the unchanged part of the population.
 // Only comparisons of the results with `cut` are meaningful (screened pairs
 // are left out).
-using BaseMinFn = std::function<void(const std::vector<Query>&, bool skip_dividing, double cut,
-                                     std::vector<double>&)>;
+using BaseMinFn = std::function<void(const std::vector<Query>&, double cut, std::vector<double>&)>;
 
-struct DivisionResult {
-    std::vector<double> mothers;     // n_div x ncols, row-major: rewritten dividing rows
-    std::vector<double> daughters;   // n_div x ncols, row-major: appended rows
-    int tries = 0;
-    bool satisfied = true;
+// One replicate's divideCells call (growth.hpp:456-720) as a state machine, so
+// that the placement rounds of many replicates share one device query.
+struct DivisionJob {
+    // inputs
+    Mt19937* rng = nullptr;
+    std::vector<int>* parents = nullptr;
+    double t = 0;
+    int lo = 0, hi = 0;                 // the replicate's rows [lo, hi) before division
+    std::vector<int> idx;               // rows of the dividing cells, ascending
+    std::vector<double> mrows;          // their original values, nd x ncols row-major
+    // state
+    int nd = 0, ntries = 0;
+    std::vector<int> check, satisfies;
+    std::vector<double> mo, da;         // proposal: rewritten mothers and appended daughters
+    std::vector<int> qcell;             // this round's queries: position among the 2 nd new rows
+    size_t q0 = 0;                      // first query of this job in the round
+    bool active() const
+    {
+        int sat = 0;
+        for (int v : satisfies) sat += v;
+        return ntries == 0 || (ntries < 10 && sat < nd);
+    }
 };
 
-// divideCells (growth.hpp:456-720) for the dividing rows `mrows` (n_div x
-// ncols, row-major, original values) at indices idx (ascending) of an
-// n-cell population.
-DivisionResult divide_rows(const bfm_sim_config* cfg, Mt19937& rng, double t, int n, int ncols,
-                           const std::vector<int>& idx, const std::vector<double>& mrows,
-                           const BaseMinFn& base_min, std::vector<int>& parents)
+// One placement attempt of divideCells: re-orientation angles (four draws per
+// cell with room, none on the last try), length ratios, the mothers' then the
+// daughters' growth rates, in the reference's order.
+void propose(const bfm_sim_config* cfg, int ncols, DivisionJob& jb)
 {
     const Draws dr{cfg};
-    const int nd = (int)idx.size();
+    const int nd = jb.nd;
     const double R = cfg->R;
-    const double mindist_default = 2 * cfg->Rcell;
-    DivisionResult out;
+    Mt19937& rng = *jb.rng;
+    jb.mo = jb.mrows;
+    jb.da = jb.mrows;
+    std::vector<double> txy1(nd, 0.0), tz1(nd, 0.0), txy2(nd, 0.0), tz2(nd, 0.0);
+    for (int i = 0; i < nd; ++i) {
+        if (jb.ntries < 10 - 1 && jb.check[i] == 1) {
+            txy1[i] = dr.angle_xy(rng);
+            tz1[i] = dr.angle_z(rng);
+            txy2[i] = dr.angle_xy(rng);
+            tz2[i] = dr.angle_z(rng);
+        }
+    }
+    for (int i = 0; i < nd; ++i) {
+        const double* row = &jb.mrows[(size_t)i * ncols];
+        double u[3] = {row[C_NX], row[C_NY], row[C_NZ]}, w[3], v[3];
+        rotate_xy(u, txy1[i], w);
+        rotate_out_of_xy(w, tz1[i], v);
+        double* m = &jb.mo[(size_t)i * ncols];
+        m[C_NX] = v[0]; m[C_NY] = v[1]; m[C_NZ] = v[2];
+        rotate_xy(u, txy2[i], w);
+        rotate_out_of_xy(w, tz2[i], v);
+        double* d = &jb.da[(size_t)i * ncols];
+        d[C_NX] = v[0]; d[C_NY] = v[1]; d[C_NZ] = v[2];
+    }
+    std::vector<double> M(nd);
+    for (int i = 0; i < nd; ++i) M[i] = dr.daughter_length(rng);
+    for (int i = 0; i < nd; ++i) {
+        double* m = &jb.mo[(size_t)i * ncols];
+        double* d = &jb.da[(size_t)i * ncols];
+        double Ld = m[C_L] - 2 * R;
+        double L1 = M[i] * Ld;
+        double L2 = (1 - M[i]) * Ld;
+        double div = -m[C_HALF_L] + L1 + R;
+        double delta1 = L1 / 2 + R;
+        double delta2 = L2 / 2 + R;
+        m[C_L] = L1;
+        m[C_HALF_L] = L1 / 2;
+        d[C_L] = L2;
+        d[C_HALF_L] = L2 / 2;
+        m[C_RX] = m[C_RX] + (div - delta1) * m[C_NX];
+        d[C_RX] = d[C_RX] + (div + delta2) * d[C_NX];
+        m[C_RY] = m[C_RY] + (div - delta1) * m[C_NY];
+        d[C_RY] = d[C_RY] + (div + delta2) * d[C_NY];
+        m[C_RZ] = m[C_RZ] + (div - delta1) * m[C_NZ];
+        d[C_RZ] = d[C_RZ] + (div + delta2) * d[C_NZ];
+        m[C_T0] = jb.t;
+        d[C_T0] = jb.t;
+    }
+    for (int i = 0; i < nd; ++i) {
+        double* m = &jb.mo[(size_t)i * ncols];
+        m[C_GROWTH] = dr.growth(rng, (int)m[C_GROUP]);
+    }
+    for (int i = 0; i < nd; ++i) {
+        double* d = &jb.da[(size_t)i * ncols];
+        d[C_GROWTH] = dr.growth(rng, (int)d[C_GROUP]);
+    }
+}
 
-    // check_distance: is there room to re-orient the daughters?
-    std::vector<int> check(nd, 1), satisfies(nd, 0);
+// divideCells for every job: the no-room check, then placement rounds until
+// each job has its criterion met or ten tries, then ids and lineage
+// (growth.hpp:700-712). Each round issues one base_min call for all jobs.
+void divide_batch(const bfm_sim_config* cfg, int ncols, std::vector<DivisionJob>& jobs, const BaseMinFn& base_min)
+{
+    const double cut = 2 * cfg->Rcell;
     {
-        std::vector<Query> q(nd);
-        for (int i = 0; i < nd; ++i) {
-            Seg s = seg_of_row(&mrows[(size_t)i * ncols]);
-            std::memcpy(q[i].r, s.r, sizeof(s.r));
-            std::memcpy(q[i].n, s.n, sizeof(s.n));
-            q[i].hl = s.hl;
-            q[i].self = idx[i];
+        std::vector<Query> q;
+        for (auto& jb : jobs) {
+            jb.nd = (int)jb.idx.size();
+            jb.check.assign(jb.nd, 1);
+            jb.satisfies.assign(jb.nd, 0);
+            jb.ntries = 0;
+            for (int i = 0; i < jb.nd; ++i)
+                q.push_back(query_of_row(&jb.mrows[(size_t)i * ncols], jb.idx[i], jb.lo, jb.hi, 0));
         }
         std::vector<double> md;
-        base_min(q, false, mindist_default, md);
-        for (int i = 0; i < nd; ++i)
-            if (md[i] < mindist_default) check[i] = 0;
+        if (!q.empty()) base_min(q, cut, md);
+        size_t k = 0;
+        for (auto& jb : jobs)
+            for (int i = 0; i < jb.nd; ++i, ++k) {
+                if (md[k] < cut) jb.check[i] = 0;
+                if (jb.check[i] == 0) jb.satisfies[i] = 1;
+            }
     }
-    for (int i = 0; i < nd; ++i)
-        if (check[i] == 0) satisfies[i] = 1;
-
-    auto n_satisfied = [&] {
-        int s = 0;
-        for (int v : satisfies) s += v;
-        return s;
-    };
-
-    const int ntries_total = 10;
-    int ntries = 0;
-    std::vector<double> mo, da;
-    while (ntries == 0 || (ntries < ntries_total && n_satisfied() < nd)) {
-        mo = mrows;
-        da = mrows;
-        std::vector<double> txy1(nd, 0.0), tz1(nd, 0.0), txy2(nd, 0.0), tz2(nd, 0.0);
-        for (int i = 0; i < nd; ++i) {
-            if (ntries < ntries_total - 1 && check[i] == 1) {
-                txy1[i] = dr.angle_xy(rng);
-                tz1[i] = dr.angle_z(rng);
-                txy2[i] = dr.angle_xy(rng);
-                tz2[i] = dr.angle_z(rng);
-            }
-        }
-        for (int i = 0; i < nd; ++i) {
-            const double* row = &mrows[(size_t)i * ncols];
-            double u[3] = {row[C_NX], row[C_NY], row[C_NZ]}, w[3], v[3];
-            rotate_xy(u, txy1[i], w);
-            rotate_out_of_xy(w, tz1[i], v);
-            double* m = &mo[(size_t)i * ncols];
-            m[C_NX] = v[0]; m[C_NY] = v[1]; m[C_NZ] = v[2];
-            rotate_xy(u, txy2[i], w);
-            rotate_out_of_xy(w, tz2[i], v);
-            double* d = &da[(size_t)i * ncols];
-            d[C_NX] = v[0]; d[C_NY] = v[1]; d[C_NZ] = v[2];
-        }
-        std::vector<double> M(nd);
-        for (int i = 0; i < nd; ++i) M[i] = dr.daughter_length(rng);
-        for (int i = 0; i < nd; ++i) {
-            double* m = &mo[(size_t)i * ncols];
-            double* d = &da[(size_t)i * ncols];
-            double Ld = m[C_L] - 2 * R;
-            double L1 = M[i] * Ld;
-            double L2 = (1 - M[i]) * Ld;
-            double div = -m[C_HALF_L] + L1 + R;
-            double delta1 = L1 / 2 + R;
-            double delta2 = L2 / 2 + R;
-            m[C_L] = L1;
-            m[C_HALF_L] = L1 / 2;
-            d[C_L] = L2;
-            d[C_HALF_L] = L2 / 2;
-            m[C_RX] = m[C_RX] + (div - delta1) * m[C_NX];
-            d[C_RX] = d[C_RX] + (div + delta2) * d[C_NX];
-            m[C_RY] = m[C_RY] + (div - delta1) * m[C_NY];
-            d[C_RY] = d[C_RY] + (div + delta2) * d[C_NY];
-            m[C_RZ] = m[C_RZ] + (div - delta1) * m[C_NZ];
-            d[C_RZ] = d[C_RZ] + (div + delta2) * d[C_NZ];
-            m[C_T0] = t;
-            d[C_T0] = t;
-        }
-        for (int i = 0; i < nd; ++i) {
-            double* m = &mo[(size_t)i * ncols];
-            m[C_GROWTH] = dr.growth(rng, (int)m[C_GROUP]);
-        }
-        for (int i = 0; i < nd; ++i) {
-            double* d = &da[(size_t)i * ncols];
-            d[C_GROWTH] = dr.growth(rng, (int)d[C_GROUP]);
-        }
-
-        // minimum distance from each re-oriented daughter to the new population
+    for (;;) {
         std::vector<Query> q;
-        std::vector<int> qcell;   // position among the 2 nd new rows (mothers first)
-        for (int i = 0; i < nd; ++i) {
-            if (check[i] != 1) continue;
-            for (int which = 0; which < 2; ++which) {
-                const double* row = which == 0 ? &mo[(size_t)i * ncols] : &da[(size_t)i * ncols];
-                Query qq;
-                Seg s = seg_of_row(row);
-                std::memcpy(qq.r, s.r, sizeof(s.r));
-                std::memcpy(qq.n, s.n, sizeof(s.n));
-                qq.hl = s.hl;
-                qq.self = -1;
-                q.push_back(qq);
-                qcell.push_back(which * nd + i);
+        std::vector<DivisionJob*> round;
+        for (auto& jb : jobs) {
+            if (!jb.active()) continue;
+            round.push_back(&jb);
+            propose(cfg, ncols, jb);
+            jb.qcell.clear();
+            jb.q0 = q.size();
+            for (int i = 0; i < jb.nd; ++i) {
+                if (jb.check[i] != 1) continue;
+                q.push_back(query_of_row(&jb.mo[(size_t)i * ncols], -1, jb.lo, jb.hi, 1));
+                jb.qcell.push_back(i);
+                q.push_back(query_of_row(&jb.da[(size_t)i * ncols], -1, jb.lo, jb.hi, 1));
+                jb.qcell.push_back(jb.nd + i);
             }
         }
-        if (!q.empty()) {
-            std::vector<double> md;
-            base_min(q, true, mindist_default, md);
-            for (size_t k = 0; k < q.size(); ++k) {
+        if (round.empty()) break;
+        std::vector<double> md;
+        if (!q.empty()) base_min(q, cut, md);
+        for (DivisionJob* jp : round) {
+            DivisionJob& jb = *jp;
+            const int nd = jb.nd;
+            for (size_t k = 0; k < jb.qcell.size(); ++k) {
+                const Query& qq = q[jb.q0 + k];
                 Seg a;
-                std::memcpy(a.r, q[k].r, sizeof(a.r));
-                std::memcpy(a.n, q[k].n, sizeof(a.n));
-                a.hl = q[k].hl;
+                std::memcpy(a.r, qq.r, sizeof(a.r));
+                std::memcpy(a.n, qq.n, sizeof(a.n));
+                a.hl = qq.hl;
+                double& mk = md[jb.q0 + k];
                 for (int r = 0; r < 2 * nd; ++r) {
-                    if (r == qcell[k]) continue;
-                    const double* row = r < nd ? &mo[(size_t)r * ncols] : &da[(size_t)(r - nd) * ncols];
+                    if (r == jb.qcell[k]) continue;
+                    const double* row = r < nd ? &jb.mo[(size_t)r * ncols] : &jb.da[(size_t)(r - nd) * ncols];
                     Seg b = seg_of_row(row);
-                    if (beyond_cut(a, b, mindist_default)) continue;
+                    if (beyond_cut(a, b, cut)) continue;
                     double dist = seg_dist(a, b);
-                    if (dist < md[k]) md[k] = dist;
+                    if (dist < mk) mk = dist;
                 }
             }
-            for (size_t k = 0; k + 1 < q.size(); k += 2) {
-                int i = qcell[k];
-                if (md[k] >= mindist_default && md[k + 1] >= mindist_default) satisfies[i] = 1;
+            for (size_t k = 0; k + 1 < jb.qcell.size(); k += 2) {
+                const int i = jb.qcell[k];
+                if (md[jb.q0 + k] >= cut && md[jb.q0 + k + 1] >= cut) jb.satisfies[i] = 1;
             }
+            jb.ntries++;
         }
-        ntries++;
     }
-    out.tries = ntries;
-    out.satisfied = n_satisfied() == nd;
-
-    // cell ids and lineage
-    int max_id = (int)parents.size() - 1;
-    for (int i = 0; i < nd; ++i) {
-        int id1 = max_id + (2 * i + 1);
-        int id2 = max_id + (2 * i + 2);
-        int parent_id = (int)mo[(size_t)i * ncols + C_ID];
-        mo[(size_t)i * ncols + C_ID] = id1;
-        da[(size_t)i * ncols + C_ID] = id2;
-        parents.push_back(parent_id);
-        parents.push_back(parent_id);
+    for (auto& jb : jobs) {
+        std::vector<int>& parents = *jb.parents;
+        int max_id = (int)parents.size() - 1;
+        for (int i = 0; i < jb.nd; ++i) {
+            int id1 = max_id + (2 * i + 1);
+            int id2 = max_id + (2 * i + 2);
+            int parent_id = (int)jb.mo[(size_t)i * ncols + C_ID];
+            jb.mo[(size_t)i * ncols + C_ID] = id1;
+            jb.da[(size_t)i * ncols + C_ID] = id2;
+            parents.push_back(parent_id);
+            parents.push_back(parent_id);
+        }
     }
-    (void)n;
-    out.mothers = std::move(mo);
-    out.daughters = std::move(da);
-    return out;
 }
 
 // ---------------------------------------------------------------------------
@@ -416,24 +444,26 @@ __global__ void k_scatter_rows(double* __restrict__ cells, int ld, int ncols, co
     cells[(size_t)c * ld + idx[r]] = rows[(size_t)c * k + r];
 }
 
-// Minimum |d| from each query segment (7 doubles: r, n, half_l) to cells j in
-// [0, n) with j != qself[q] and (skip == nullptr or skip[j] == 0). Pairs
-// beyond `cut` are screened out, so only comparisons with cut are exact. Out
-// holds the bit patterns of non-negative doubles (ordered like the values).
+// Minimum |d| from each query segment (7 doubles: r, n, half_l) to the cells
+// j in [lo, hi) of its replicate with j != self and, when the query asks for
+// it, skip[j] == 0 (qi4 = {self, lo, hi, skip?}). Pairs beyond `cut` are
+// screened out, so only comparisons with cut are exact. Out holds the bit
+// patterns of non-negative doubles (ordered like the values).
 __global__ void __launch_bounds__(256)
-k_min_dist(const double* __restrict__ cells, int ld, int n, const double* __restrict__ q,
-           const int* __restrict__ qself, const int* __restrict__ skip, double cut,
-           unsigned long long* __restrict__ out)
+k_min_dist(const double* __restrict__ cells, int ld, const double* __restrict__ q, const int4* __restrict__ qi4,
+           const int* __restrict__ skip, double cut, unsigned long long* __restrict__ out)
 {
     const int qi = blockIdx.y;
     Seg a;
     a.r[0] = q[qi * 7 + 0]; a.r[1] = q[qi * 7 + 1]; a.r[2] = q[qi * 7 + 2];
     a.n[0] = q[qi * 7 + 3]; a.n[1] = q[qi * 7 + 4]; a.n[2] = q[qi * 7 + 5];
     a.hl = q[qi * 7 + 6];
-    const int self = qself[qi];
+    const int4 meta = qi4[qi];
+    const int self = meta.x;
+    const bool use_skip = meta.w != 0;
     double best = INFINITY;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        if (j == self || (skip && skip[j])) continue;
+    for (int j = meta.y + blockIdx.x * blockDim.x + threadIdx.x; j < meta.z; j += gridDim.x * blockDim.x) {
+        if (j == self || (use_skip && skip[j])) continue;
         Seg b;
         b.r[0] = cells[(size_t)C_RX * ld + j]; b.r[1] = cells[(size_t)C_RY * ld + j];
         b.r[2] = cells[(size_t)C_RZ * ld + j];
@@ -456,11 +486,85 @@ k_min_dist(const double* __restrict__ cells, int ld, int n, const double* __rest
     }
 }
 
+// d_rep[i] = r for the rows [starts[r], starts[r + 1]) of replicate r.
+__global__ void k_fill_rep(int* __restrict__ rep, const int* __restrict__ starts)
+{
+    const int r = blockIdx.x;
+    for (int i = starts[r] + threadIdx.x; i < starts[r + 1]; i += blockDim.x) rep[i] = r;
+}
+
+// Moves every row i (replicate rep[i]) to i + shift[rep[i]]: replicates stay
+// contiguous and in order after each gains its daughters at its end.
+__global__ void k_repack(const double* __restrict__ src, double* __restrict__ dst, int ld, int ncols, int n,
+                         const int* __restrict__ rep, const int* __restrict__ shift)
+{
+    const int64_t total = (int64_t)n * ncols;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n), col = (int)(t / n);
+        dst[(int64_t)col * ld + i + shift[rep[i]]] = src[(int64_t)col * ld + i];
+    }
+}
+
 int blocks_for(int64_t n, int b) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
 // the loop
 // ---------------------------------------------------------------------------
+
+// Device side of the division placement checks: one k_min_dist launch per
+// batch of queries (every replicate's queries of one placement round).
+struct DeviceMinDist {
+    hipStream_t st;
+    int cap;
+    double* d_q = nullptr;
+    int4* d_meta = nullptr;
+    unsigned long long* d_out = nullptr;
+    DeviceMinDist(int capacity, hipStream_t stream) : st(stream), cap(std::max(capacity, 16))
+    {
+        SCHK(hipMalloc((void**)&d_q, sizeof(double) * 7 * (size_t)cap));
+        SCHK(hipMalloc((void**)&d_meta, sizeof(int4) * (size_t)cap));
+        SCHK(hipMalloc((void**)&d_out, sizeof(unsigned long long) * (size_t)cap));
+    }
+    ~DeviceMinDist()
+    {
+        (void)hipFree(d_q);
+        (void)hipFree(d_meta);
+        (void)hipFree(d_out);
+    }
+    void run(const std::vector<Query>& q, const double* cells, int ld, const int* skip, double cut,
+             std::vector<double>& out)
+    {
+        out.resize(q.size());
+        for (size_t k0 = 0; k0 < q.size(); k0 += (size_t)cap) {
+            const int nq = (int)std::min(q.size() - k0, (size_t)cap);
+            std::vector<double> hq((size_t)nq * 7);
+            std::vector<int4> hm(nq);
+            int span = 1;
+            for (int k = 0; k < nq; ++k) {
+                const Query& qq = q[k0 + k];
+                for (int a = 0; a < 3; ++a) {
+                    hq[(size_t)k * 7 + a] = qq.r[a];
+                    hq[(size_t)k * 7 + 3 + a] = qq.n[a];
+                }
+                hq[(size_t)k * 7 + 6] = qq.hl;
+                hm[k] = make_int4(qq.self, qq.lo, qq.hi, qq.skip_dividing);
+                span = std::max(span, qq.hi - qq.lo);
+            }
+            SCHK(hipMemcpyAsync(d_q, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice, st));
+            SCHK(hipMemcpyAsync(d_meta, hm.data(), sizeof(int4) * nq, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(nq, 256)), dim3(256), 0, st, d_out, nq,
+                               (unsigned long long)0x7FF0000000000000ull);
+            const int gx = std::min(blocks_for(span, 256), 64);
+            hipLaunchKernelGGL(k_min_dist, dim3(gx, nq), dim3(256), 0, st, cells, ld, d_q, d_meta, skip, cut, d_out);
+            SCHK(hipGetLastError());
+            std::vector<unsigned long long> bits(nq);
+            SCHK(hipMemcpyAsync(bits.data(), d_out, sizeof(unsigned long long) * nq, hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+            for (int k = 0; k < nq; ++k) std::memcpy(&out[k0 + k], &bits[k], sizeof(double));
+        }
+    }
+};
 struct TableauDef {
     int s;
     double A[49], b[7], bs[7];
@@ -570,9 +674,8 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
     const int ld = cap;
 
     DevBuf<double> bufA((size_t)ld * ncols), bufB((size_t)ld * ncols);
-    DevBuf<int> d_count(1), d_idx(cap), d_flags(cap), d_rowidx(2 * (size_t)cap), d_qself(2 * (size_t)cap);
-    DevBuf<double> d_err(1), d_rows((size_t)2 * cap * ncols), d_q((size_t)2 * cap * 7);
-    DevBuf<unsigned long long> d_qmin(2 * (size_t)cap);
+    DevBuf<int> d_count(1), d_idx(cap), d_flags(cap), d_rowidx(2 * (size_t)cap);
+    DevBuf<double> d_err(1), d_rows((size_t)2 * cap * ncols);
     PinnedBuf h_small(64);
     double* d_cur = bufA.p;
     double* d_new = bufB.p;
@@ -615,33 +718,9 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
         SCHK(hipStreamSynchronize(st));
         return std::max(*(double*)h_small.p, cfg->min_error);
     };
-    // Minimum distances against the unchanged population, on the device.
-    BaseMinFn base_min = [&](const std::vector<Query>& q, bool skip_dividing, double cut,
-                             std::vector<double>& out) {
-        const int nq = (int)q.size();
-        std::vector<double> hq((size_t)nq * 7);
-        std::vector<int> hs(nq);
-        for (int k = 0; k < nq; ++k) {
-            for (int a = 0; a < 3; ++a) {
-                hq[(size_t)k * 7 + a] = q[k].r[a];
-                hq[(size_t)k * 7 + 3 + a] = q[k].n[a];
-            }
-            hq[(size_t)k * 7 + 6] = q[k].hl;
-            hs[k] = q[k].self;
-        }
-        SCHK(hipMemcpyAsync(d_q.p, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice, st));
-        SCHK(hipMemcpyAsync(d_qself.p, hs.data(), sizeof(int) * nq, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(nq, 256)), dim3(256), 0, st, d_qmin.p, nq,
-                           (unsigned long long)0x7FF0000000000000ull);
-        int gx = std::min(blocks_for(n, 256), 64);
-        hipLaunchKernelGGL(k_min_dist, dim3(gx, nq), dim3(256), 0, st, d_cur, ld, n, d_q.p, d_qself.p,
-                           skip_dividing ? d_flags.p : nullptr, cut, d_qmin.p);
-        SCHK(hipGetLastError());
-        std::vector<unsigned long long> bits(nq);
-        SCHK(hipMemcpyAsync(bits.data(), d_qmin.p, sizeof(unsigned long long) * nq, hipMemcpyDeviceToHost, st));
-        SCHK(hipStreamSynchronize(st));
-        out.resize(nq);
-        for (int k = 0; k < nq; ++k) std::memcpy(&out[k], &bits[k], sizeof(double));
+    DeviceMinDist dmin(cap, st);
+    BaseMinFn base_min = [&](const std::vector<Query>& q, double cut, std::vector<double>& out) {
+        dmin.run(q, d_cur, ld, d_flags.p, cut, out);
     };
 
     build();
@@ -679,7 +758,16 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
             SCHK(hipStreamSynchronize(st));
             for (int r = 0; r < nd; ++r)
                 for (int k = 0; k < ncols; ++k) mrows[(size_t)r * ncols + k] = colm[(size_t)k * nd + r];
-            DivisionResult dv = divide_rows(cfg, rng, t, n, ncols, idx, mrows, base_min, parents);
+            std::vector<DivisionJob> jobs(1);
+            DivisionJob& jb = jobs[0];
+            jb.rng = &rng;
+            jb.parents = &parents;
+            jb.t = t;
+            jb.lo = 0;
+            jb.hi = n;
+            jb.idx = idx;
+            jb.mrows = std::move(mrows);
+            divide_batch(cfg, ncols, jobs, base_min);
             // write back: mothers in place, daughters appended at n, n+1, ...
             std::vector<int> dst(2 * (size_t)nd);
             std::vector<double> rows((size_t)2 * nd * ncols);
@@ -687,8 +775,8 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
                 dst[r] = idx[r];
                 dst[nd + r] = n + r;
                 for (int k = 0; k < ncols; ++k) {
-                    rows[(size_t)k * 2 * nd + r] = dv.mothers[(size_t)r * ncols + k];
-                    rows[(size_t)k * 2 * nd + nd + r] = dv.daughters[(size_t)r * ncols + k];
+                    rows[(size_t)k * 2 * nd + r] = jb.mo[(size_t)r * ncols + k];
+                    rows[(size_t)k * 2 * nd + nd + r] = jb.da[(size_t)r * ncols + k];
                 }
             }
             SCHK(hipMemcpyAsync(d_rowidx.p, dst.data(), sizeof(int) * dst.size(), hipMemcpyHostToDevice, st));
@@ -765,9 +853,402 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
     }
 }
 
+// runSimulation for a batch of independent replicates in lockstep: every
+// iteration advances each unfinished replicate by its own dt with its own
+// controller, division stream and lineage. Replicates occupy contiguous row
+// ranges (re-packed after divisions, which keeps each replicate's row order),
+// the neighbor build keeps pairs inside a replicate, and one captured graph
+// (stages + update for all replicates) serves every RK attempt.
+void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, const double* cells_in, int ncols,
+              const uint64_t* seeds, int autotune, double* cells_out, int cap, int* sizes_out, double* t_out,
+              int64_t* iters_out, int* failed_out, bfm_ensemble_stats* stats)
+{
+    check_config(cfg, ncols);
+    if (!c->params_set) sim_fail(BFM_ERR_ARG, "bfm_set_params has not been called");
+    if (c->p.ncols != ncols) sim_fail(BFM_ERR_ARG, "ncols differs from the parameters' ncols");
+    if (R < 1 || !sizes || !cells_in || !seeds || !cells_out || !sizes_out)
+        sim_fail(BFM_ERR_ARG, "need n_rep >= 1 and non-NULL arrays");
+    int64_t total64 = 0;
+    for (int r = 0; r < R; ++r) {
+        if (sizes[r] < 1) sim_fail(BFM_ERR_ARG, "every replicate needs at least one cell");
+        total64 += sizes[r];
+    }
+    if (total64 > cap || cap > c->max_cells) sim_fail(BFM_ERR_ARG, "need sum(sizes) <= cap <= max_cells");
+    const auto wall0 = std::chrono::steady_clock::now();
+    auto wall = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count(); };
+    const TableauDef tdef = tableau(cfg->tableau);
+    Tableau tb;
+    std::memset(&tb, 0, sizeof(tb));
+    tb.s = tdef.s;
+    for (int i = 0; i < 49; ++i) tb.A[i] = tdef.A[i];
+    for (int i = 0; i < 7; ++i) { tb.b[i] = tdef.b[i]; tb.bs[i] = tdef.bs[i]; }
+    hipStream_t st = c->stream;
+    const int ld = cap;
+    int n = (int)total64;
+
+    DevBuf<double> bufA((size_t)ld * ncols), bufB((size_t)ld * ncols), d_dt(R), d_rows((size_t)2 * cap * ncols);
+    DevBuf<int> d_n(1), d_active(R), d_flags(cap), d_idx(cap), d_count(1), d_rowidx(2 * (size_t)cap),
+        d_starts(R + 1), d_shift(R);
+    DevBuf<unsigned long long> d_errmax(R);
+    DevBuf<int> d_bad(R);
+    PinnedBuf h_dt(sizeof(double) * R), h_act(sizeof(int) * R), h_err(sizeof(unsigned long long) * R),
+        h_bad(sizeof(int) * R), h_small(64);
+    int* p_bad = (int*)h_bad.p;
+    SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
+    double* p_dt = (double*)h_dt.p;
+    int* p_act = (int*)h_act.p;
+    unsigned long long* p_err = (unsigned long long*)h_err.p;
+    double* d_cur = bufA.p;
+    double* d_new = bufB.p;
+    SCHK(hipMemsetAsync(d_flags.p, 0, sizeof(int) * cap, st));
+    SCHK(hipMemcpy2DAsync(d_cur, sizeof(double) * ld, cells_in, sizeof(double) * n, sizeof(double) * n, ncols,
+                          hipMemcpyHostToDevice, st));
+
+    std::vector<int> start(R + 1, 0), nr(sizes, sizes + R);
+    for (int r = 0; r < R; ++r) start[r + 1] = start[r] + nr[r];
+    std::vector<double> t(R, 0.0), dt(R, cfg->max_stepsize);
+    std::vector<int64_t> iters(R, 0);
+    std::vector<char> done(R, 0);
+    std::vector<Mt19937> rng;
+    rng.reserve(R);
+    std::vector<std::vector<int>> parents(R);
+    for (int r = 0; r < R; ++r) {
+        rng.emplace_back((uint32_t)seeds[r]);
+        int max_id = 0;
+        for (int i = start[r]; i < start[r + 1]; ++i) max_id = std::max(max_id, (int)cells_in[(size_t)C_ID * n + i]);
+        parents[r].assign(max_id + 1, -1);
+    }
+    abi(c, set_replicate_count(c, R));
+    auto upload_layout = [&] {
+        SCHK(hipMemcpyAsync(d_starts.p, start.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_n.p, &n, sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_fill_rep, dim3(R), dim3(256), 0, st, c->d_rep, d_starts.p);
+        SCHK(hipGetLastError());
+        SCHK(hipStreamSynchronize(st));
+    };
+    upload_layout();
+
+    const double thr = cfg->neighbor_threshold, Rr = cfg->R, Ldiv = cfg->Ldiv;
+    int64_t builds = 0, attempts = 0, divisions = 0, iter = 0, stage_updates = 0;
+    double div_seconds = 0;
+    auto build = [&] {
+        int64_t m = 0;
+        abi(c, bfm_dev_build_neighbors(c, d_cur, n, ld, thr, Rr, Ldiv, &m));
+        ++builds;
+    };
+    build();
+    double tuned_scale = c->bin_scale, tuned_ms = 0;
+    int tuned_variant = c->cand_variant;
+    if (autotune) {
+        abi(c, bfm_autotune_neighbors(c, d_cur, n, ld, thr, Rr, Ldiv, &tuned_scale, &tuned_variant, &tuned_ms));
+        ++builds;
+    }
+
+    // one captured RK attempt per buffer orientation
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    struct GraphGuard {
+        hipGraphExec_t* e;
+        hipGraph_t* g;
+        ~GraphGuard()
+        {
+            for (int k = 0; k < 2; ++k) {
+                if (e[k]) (void)hipGraphExecDestroy(e[k]);
+                if (g[k]) (void)hipGraphDestroy(g[k]);
+            }
+        }
+    } guard{exec, graph};
+    const bool was_timing = c->timing;
+    c->timing = false;
+    for (int k = 0; k < 2; ++k) {
+        const double* src = k == 0 ? bufA.p : bufB.p;
+        double* dst = k == 0 ? bufB.p : bufA.p;
+        SCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = step_attempt_batched(c, tb, src, ld, d_n.p, c->d_rep, d_dt.p, d_active.p, R, dst,
+                                      cfg->max_error_allowed, d_errmax.p, d_bad.p);
+        hipError_t ec = hipStreamEndCapture(st, &graph[k]);
+        abi(c, rc);
+        SCHK(ec);
+        SCHK(hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
+    }
+    c->timing = was_timing;
+
+    DeviceMinDist dmin(2 * cap, st);
+    BaseMinFn base_min = [&](const std::vector<Query>& q, double cut, std::vector<double>& out) {
+        dmin.run(q, d_cur, ld, d_flags.p, cut, out);
+    };
+    std::vector<char> act(R, 0);
+    auto replay = [&](const std::vector<char>& mask, const std::vector<double>& h) {
+        int64_t cells_on = 0;
+        for (int r = 0; r < R; ++r) {
+            p_act[r] = mask[r] ? 1 : 0;
+            p_dt[r] = h[r];
+            if (mask[r]) cells_on += nr[r];
+        }
+        SCHK(hipMemcpyAsync(d_dt.p, p_dt, sizeof(double) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_active.p, p_act, sizeof(int) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
+        ++attempts;
+        stage_updates += cells_on * tb.s;
+    };
+    std::vector<char> failed(R, 0);
+    int n_failed = 0;
+    // Non-finite results freeze their replicate at the state the iteration
+    // started from (the reference stops such a run, simulation.hpp:1803-1822);
+    // the others go on. Returns whether a replicate failed in the last replay.
+    auto check_failures = [&](std::vector<char>& mask) {
+        SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        bool any_new = false;
+        for (int r = 0; r < R; ++r)
+            if (p_bad[r] && !failed[r]) {
+                failed[r] = 1;
+                done[r] = 1;
+                act[r] = 0;
+                mask[r] = 0;
+                ++n_failed;
+                any_new = true;
+            }
+        if (any_new) SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
+        return any_new;
+    };
+    auto read_errors = [&](std::vector<double>& e) {
+        SCHK(hipMemcpyAsync(p_err, d_errmax.p, sizeof(unsigned long long) * R, hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        for (int r = 0; r < R; ++r) e[r] = std::max(order_key_value(p_err[r]), cfg->min_error);
+    };
+    double next_report = cfg->report_seconds;
+    std::vector<int> idx;
+    const double p1 = 1.0 / (tdef.order + 1);
+    for (;;) {
+        bool any = false;
+        for (int r = 0; r < R; ++r) {
+            if (!done[r] && terminate_now(cfg, nr[r], iter, t[r])) done[r] = 1;
+            act[r] = !done[r];
+            any |= act[r] != 0;
+        }
+        if (!any) break;
+        if (cfg->max_wall_seconds > 0 && wall() >= cfg->max_wall_seconds) break;
+        if (cfg->report_seconds > 0 && wall() >= next_report) {
+            int n_act = 0;
+            double tmin = 1e300;
+            for (int r = 0; r < R; ++r)
+                if (act[r]) { ++n_act; tmin = std::min(tmin, t[r]); }
+            std::fprintf(stderr, "[bfm_ensemble] wall %.1f s  iter %lld  cells %d  active %d  min t %.6f h  attempts %lld\n",
+                         wall(), (long long)iter, n, n_act, tmin, (long long)attempts);
+            std::fflush(stderr);
+            next_report += cfg->report_seconds;
+        }
+
+        // divideMaxLength + divideCells per active replicate
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
+        hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
+                           d_count.p, d_idx.p);
+        SCHK(hipMemcpyAsync(h_small.p, d_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        const int nd_all = *(int*)h_small.p;
+        if (nd_all > 0) {
+            auto td = std::chrono::steady_clock::now();
+            idx.resize(nd_all);
+            SCHK(hipMemcpyAsync(idx.data(), d_idx.p, sizeof(int) * nd_all, hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+            std::sort(idx.begin(), idx.end());
+            std::vector<int> keep, keep_rep;
+            for (int i : idx) {
+                const int r = (int)(std::upper_bound(start.begin(), start.end(), i) - start.begin()) - 1;
+                if (act[r]) { keep.push_back(i); keep_rep.push_back(r); }
+            }
+            const int nd = (int)keep.size();
+            if (nd > 0) {
+                if (n + nd > cap) sim_fail(BFM_ERR_CAPACITY, "division would exceed the cell capacity");
+                SCHK(hipMemcpyAsync(d_idx.p, keep.data(), sizeof(int) * nd, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for((int64_t)nd * ncols, 256)), dim3(256), 0, st,
+                                   d_cur, ld, ncols, d_idx.p, nd, d_rows.p);
+                hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p,
+                                   nd, 1);
+                std::vector<double> colm((size_t)nd * ncols);
+                SCHK(hipMemcpyAsync(colm.data(), d_rows.p, sizeof(double) * colm.size(), hipMemcpyDeviceToHost,
+                                    st));
+                SCHK(hipStreamSynchronize(st));
+                std::vector<DivisionJob> jobs;
+                std::vector<int> job_of(R, -1);
+                for (int k = 0; k < nd; ++k) {
+                    const int r = keep_rep[k];
+                    if (job_of[r] < 0) {
+                        job_of[r] = (int)jobs.size();
+                        jobs.emplace_back();
+                        DivisionJob& jb = jobs.back();
+                        jb.rng = &rng[r];
+                        jb.parents = &parents[r];
+                        jb.t = t[r];
+                        jb.lo = start[r];
+                        jb.hi = start[r + 1];
+                    }
+                    DivisionJob& jb = jobs[job_of[r]];
+                    jb.idx.push_back(keep[k]);
+                    for (int col = 0; col < ncols; ++col) jb.mrows.push_back(colm[(size_t)col * nd + k]);
+                }
+                divide_batch(cfg, ncols, jobs, base_min);
+                // re-pack so that every replicate gains its daughters at its end
+                std::vector<int> nstart(R + 1, 0), shift(R), ndr(R, 0);
+                for (auto& jb : jobs) ndr[(int)(std::upper_bound(start.begin(), start.end(), jb.lo) - start.begin()) - 1] = jb.nd;
+                for (int r = 0; r < R; ++r) {
+                    nstart[r + 1] = nstart[r] + nr[r] + ndr[r];
+                    shift[r] = nstart[r] - start[r];
+                }
+                SCHK(hipMemcpyAsync(d_shift.p, shift.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_repack, dim3(std::min(blocks_for((int64_t)n * ncols, 256), 8192)), dim3(256), 0,
+                                   st, d_cur, d_new, ld, ncols, n, c->d_rep, d_shift.p);
+                std::vector<int> dst;
+                std::vector<double> rows((size_t)2 * nd * ncols);
+                dst.reserve(2 * nd);
+                int k = 0;
+                for (auto& jb : jobs) {
+                    const int r = (int)(std::upper_bound(start.begin(), start.end(), jb.lo) - start.begin()) - 1;
+                    for (int i = 0; i < jb.nd; ++i, ++k) dst.push_back(jb.idx[i] + shift[r]);
+                }
+                for (auto& jb : jobs) {
+                    const int r = (int)(std::upper_bound(start.begin(), start.end(), jb.lo) - start.begin()) - 1;
+                    for (int i = 0; i < jb.nd; ++i) dst.push_back(nstart[r] + nr[r] + i);
+                }
+                k = 0;
+                for (auto& jb : jobs)
+                    for (int i = 0; i < jb.nd; ++i, ++k)
+                        for (int col = 0; col < ncols; ++col)
+                            rows[(size_t)col * 2 * nd + k] = jb.mo[(size_t)i * ncols + col];
+                for (auto& jb : jobs)
+                    for (int i = 0; i < jb.nd; ++i, ++k)
+                        for (int col = 0; col < ncols; ++col)
+                            rows[(size_t)col * 2 * nd + k] = jb.da[(size_t)i * ncols + col];
+                SCHK(hipMemcpyAsync(d_rowidx.p, dst.data(), sizeof(int) * dst.size(), hipMemcpyHostToDevice, st));
+                SCHK(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice,
+                                    st));
+                hipLaunchKernelGGL(k_scatter_rows, dim3(blocks_for((int64_t)2 * nd * ncols, 256)), dim3(256), 0, st,
+                                   d_new, ld, ncols, d_rowidx.p, 2 * nd, d_rows.p);
+                hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p,
+                                   nd, 0);
+                SCHK(hipGetLastError());
+                std::swap(d_cur, d_new);
+                for (int r = 0; r < R; ++r) nr[r] += ndr[r];
+                start = nstart;
+                n += nd;
+                divisions += nd;
+                upload_layout();
+                div_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
+                build();
+            }
+        }
+
+        // adaptive RK step of every active replicate (simulation.hpp:1677-1769)
+        abi(c, step_prepare(c, d_cur, n, ld));
+        replay(act, dt);
+        if (iter % cfg->iter_update_stepsize == 0) {
+            std::vector<double> err(R), factor(R, 1.0), trial(R);
+            std::vector<char> exceeded(R, 0), retry(R, 0);
+            std::vector<int> j(R, 0);
+            read_errors(err);
+            check_failures(act);
+            bool any_retry = false;
+            for (int r = 0; r < R; ++r) {
+                if (!act[r]) continue;
+                exceeded[r] = err[r] > 1.0;
+                double f = 0.9 * pow(1.0 / err[r], p1);
+                if (f >= 10) f = 10;
+                else if (f < 0.2) f = 0.2;
+                factor[r] = f;
+                retry[r] = exceeded[r] && j[r] < cfg->max_tries_update_stepsize;
+                any_retry |= retry[r] != 0;
+            }
+            while (any_retry) {
+                for (int r = 0; r < R; ++r) trial[r] = retry[r] ? dt[r] * factor[r] : dt[r];
+                replay(retry, trial);
+                read_errors(err);
+                check_failures(retry);
+                any_retry = false;
+                for (int r = 0; r < R; ++r) {
+                    if (!retry[r]) continue;
+                    exceeded[r] = err[r] > 1.0;
+                    factor[r] *= 0.9 * pow(1.0 / err[r], p1);
+                    bool stop = false;
+                    if (factor[r] >= 10) { factor[r] = 10; stop = true; }
+                    else if (factor[r] < 0.2) { factor[r] = 0.2; stop = true; }
+                    if (!stop) j[r]++;
+                    retry[r] = !stop && exceeded[r] && j[r] < cfg->max_tries_update_stepsize;
+                    any_retry |= retry[r] != 0;
+                }
+            }
+            for (int r = 0; r < R; ++r) {
+                if (!act[r]) continue;
+                if (dt[r] * factor[r] < cfg->min_stepsize) factor[r] = cfg->min_stepsize / dt[r];
+                else if (dt[r] * factor[r] > cfg->max_stepsize) factor[r] = cfg->max_stepsize / dt[r];
+                dt[r] *= factor[r];
+            }
+            replay(act, dt);
+        }
+        // a replicate that failed in the accepted attempt: redo the attempt
+        // without it, so its rows are carried over unchanged
+        while (check_failures(act)) replay(act, dt);
+        std::swap(d_cur, d_new);
+        abi(c, grow_batched(c, d_cur, ld, d_n.p, Rr, c->d_rep, d_dt.p, d_active.p));
+        abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
+        abi(c, bfm_sync(c));
+        for (int r = 0; r < R; ++r)
+            if (act[r]) { t[r] += dt[r]; iters[r]++; }
+        iter++;
+        if (iter % cfg->iter_update_neighbors == 0) build();
+    }
+    abi(c, bfm_sync(c));
+    SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, d_cur, sizeof(double) * ld, sizeof(double) * n, ncols,
+                          hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    for (int r = 0; r < R; ++r) {
+        sizes_out[r] = nr[r];
+        if (t_out) t_out[r] = t[r];
+        if (iters_out) iters_out[r] = iters[r];
+        if (failed_out) failed_out[r] = failed[r];
+    }
+    abi(c, set_replicate_count(c, 0));
+    if (stats) {
+        stats->iterations = iter;
+        stats->attempts = attempts;
+        stats->divisions = divisions;
+        stats->neighbor_builds = builds;
+        stats->stage_cell_updates = stage_updates;
+        stats->wall_seconds = wall();
+        stats->host_division_seconds = div_seconds;
+        stats->autotune_ms = tuned_ms;
+        stats->tuned_bin_scale = tuned_scale;
+        stats->tuned_variant = tuned_variant;
+        stats->n_final = n;
+        stats->failed = n_failed;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int bfm_ensemble_simulate(bfm_ctx* c, const bfm_sim_config* cfg, int n_rep, const int* sizes,
+                          const double* cells_in, int ncols, const uint64_t* seeds, int autotune,
+                          double* cells_out, int cap, int* sizes_out, double* t_out, int64_t* iters_out,
+                          int* failed_out, bfm_ensemble_stats* stats)
+{
+    if (!c) return BFM_ERR_ARG;
+    try {
+        SCHK(hipSetDevice(c->device));
+        ensemble(c, cfg, n_rep, sizes, cells_in, ncols, seeds, autotune, cells_out, cap, sizes_out, t_out,
+                 iters_out, failed_out, stats);
+        return BFM_OK;
+    } catch (const SimError& e) {
+        (void)set_replicate_count(c, 0);
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        (void)set_replicate_count(c, 0);
+        c->err = e.what();
+        return BFM_ERR_ARG;
+    }
+}
 
 int bfm_simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
                  const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
@@ -807,16 +1288,15 @@ int bfm_divide_cells(const bfm_sim_config* cfg, double* cells, int* n_io, int ca
             for (int k = 0; k < ncols; ++k) mrows[(size_t)r * ncols + k] = cells[(size_t)k * cap + idx[r]];
         std::vector<char> dividing(n, 0);
         for (int i : idx) dividing[i] = 1;
-        BaseMinFn base_min = [&](const std::vector<Query>& q, bool skip_dividing, double cut,
-                                 std::vector<double>& out) {
+        BaseMinFn base_min = [&](const std::vector<Query>& q, double cut, std::vector<double>& out) {
             out.assign(q.size(), INFINITY);
             for (size_t k = 0; k < q.size(); ++k) {
                 Seg a;
                 std::memcpy(a.r, q[k].r, sizeof(a.r));
                 std::memcpy(a.n, q[k].n, sizeof(a.n));
                 a.hl = q[k].hl;
-                for (int j = 0; j < n; ++j) {
-                    if (j == q[k].self || (skip_dividing && dividing[j])) continue;
+                for (int j = q[k].lo; j < q[k].hi; ++j) {
+                    if (j == q[k].self || (q[k].skip_dividing && dividing[j])) continue;
                     Seg b;
                     b.r[0] = cells[(size_t)C_RX * cap + j]; b.r[1] = cells[(size_t)C_RY * cap + j];
                     b.r[2] = cells[(size_t)C_RZ * cap + j];
@@ -831,16 +1311,25 @@ int bfm_divide_cells(const bfm_sim_config* cfg, double* cells, int* n_io, int ca
         };
         std::vector<int> par(parents, parents + *n_parents);
         Mt19937 rng((uint32_t)seed);
-        DivisionResult dv = divide_rows(cfg, rng, t, n, ncols, idx, mrows, base_min, par);
+        std::vector<DivisionJob> jobs(1);
+        DivisionJob& jb = jobs[0];
+        jb.rng = &rng;
+        jb.parents = &par;
+        jb.t = t;
+        jb.lo = 0;
+        jb.hi = n;
+        jb.idx = idx;
+        jb.mrows = std::move(mrows);
+        divide_batch(cfg, ncols, jobs, base_min);
         for (int r = 0; r < nd; ++r)
             for (int k = 0; k < ncols; ++k) {
-                cells[(size_t)k * cap + idx[r]] = dv.mothers[(size_t)r * ncols + k];
-                cells[(size_t)k * cap + n + r] = dv.daughters[(size_t)r * ncols + k];
+                cells[(size_t)k * cap + idx[r]] = jb.mo[(size_t)r * ncols + k];
+                cells[(size_t)k * cap + n + r] = jb.da[(size_t)r * ncols + k];
             }
         *n_io = n + nd;
         std::copy(par.begin(), par.end(), parents);
         *n_parents = (int)par.size();
-        if (tries_out) *tries_out = dv.tries;
+        if (tries_out) *tries_out = jb.ntries;
         return BFM_OK;
     } catch (const SimError& e) {
         return e.code;

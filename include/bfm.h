@@ -245,6 +245,44 @@ int bfm_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, 
 /* The first `count` uniform_01 draws of the random stream seeded with seed. */
 int bfm_rng_uniform(uint64_t seed, int count, double* out);
 
+/* ---- batched replicates (no counterpart in the reference: many independent
+ * runSimulation loops advanced together) ---------------------------------- */
+typedef struct bfm_ensemble_stats {
+    int64_t iterations;          /* lockstep iterations */
+    int64_t attempts;            /* replays of the captured RK attempt (all replicates) */
+    int64_t divisions, neighbor_builds;
+    int64_t stage_cell_updates;  /* sum over attempts of (cells of the attempting replicates) x stages */
+    double wall_seconds, host_division_seconds, autotune_ms;
+    double tuned_bin_scale;
+    int tuned_variant;
+    int n_final;
+    int failed;                  /* replicates stopped by non-finite results */
+    int pad;
+} bfm_ensemble_stats;
+
+/* Runs n_rep independent replicates of the bfm_simulate loop (same config and
+ * parameters; replicate r has its own cells, random stream seeds[r], dt,
+ * controller and lineage) until each meets the terminate rule. cells_in holds
+ * the replicates back to back (sum(sizes) rows, leading dimension
+ * sum(sizes)); cells_out receives them back to back (leading dimension cap),
+ * sizes_out the final sizes, t_out / iters_out / failed_out (may be NULL) each
+ * replicate's time, iteration count and failure flag. A replicate whose step
+ * produces non-finite positions or orientations is frozen at the state its
+ * iteration started from and flagged; the others continue. Every RK attempt of all replicates is one
+ * replay of a captured graph (one launch per stage covering all replicates);
+ * autotune != 0 first times the neighbor build over bin scales and candidate
+ * kernel variants (bfm_autotune_neighbors) and keeps the fastest. */
+int bfm_ensemble_simulate(bfm_ctx* ctx, const bfm_sim_config* cfg, int n_rep, const int* sizes,
+                          const double* cells_in, int ncols, const uint64_t* seeds, int autotune,
+                          double* cells_out, int cap, int* sizes_out, double* t_out, int64_t* iters_out,
+                          int* failed_out, bfm_ensemble_stats* stats);
+
+/* Times the device neighbor build on these cells for bin scales {0.5, 0.75,
+ * 1, 1.25} x candidate-kernel variants 0..3 and keeps the fastest (every
+ * choice gives the same list). Rebuilds the list with the choice. */
+int bfm_autotune_neighbors(bfm_ctx* ctx, const double* d_cells, int n, int ld, double neighbor_threshold,
+                           double R, double Ldiv, double* best_scale, int* best_variant, double* best_ms);
+
 /* ---- instrumentation -------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the launch
  * stream; bfm_kernel_times returns per-kernel accumulated milliseconds and
