@@ -48,9 +48,13 @@ def test_ensemble_equals_independent_runs(tableau):
 
 
 def test_ensemble_many_replicates_autotuned():
-    """64 uneven replicates with the autotuner on and the default rebuild
-    period: every replicate advances, the tuner picks a valid configuration,
-    and two replicates agree with independent runs to 1e-9. Replicate 47
+    """64 uneven replicates with the autotuner on: every replicate advances,
+    the tuner picks a valid configuration, and replicates agree bit for bit
+    with independent runs under the same rebuild schedule (the list rebuilt
+    every iteration; with a longer period the batch rebuilds all replicates
+    whenever one divides, so a replicate can see a fresher list than alone,
+    which matters only when cells cross the 0.4 um skin between rebuilds, as
+    in replicate 37's unrelaxed start). Replicate 47
     draws a daughter length ratio of 4e-5 at its first division (a 4 sigma
     draw of the clipped N(0.5, 0.125)); its rotational drag ~ l^3 makes the
     next step non-finite, as it does for the independent run (and the
@@ -60,7 +64,7 @@ def test_ensemble_many_replicates_autotuned():
     sizes = np.rint(np.exp(rng.uniform(np.log(30), np.log(300), 64))).astype(int)
     reps = [make_dome(int(n), seed=r, gamma=(50.0, 500.0)[r % 2])[0] for r, n in enumerate(sizes)]
     seeds = list(range(64))
-    cfg = sim.sim_config(max_iter=12, tableau="bogacki_shampine", seed=0)
+    cfg = sim.sim_config(max_iter=12, tableau="bogacki_shampine", seed=0, iter_update_neighbors=1)
     cap = 2 * int(sizes.sum()) + 64
     print(f"ensemble of 64 replicates, {int(sizes.sum())} cells", flush=True)
     out, t_out, it_out, st = sim.ensemble_simulate(_ctx(cap), cfg, reps, seeds, autotune=True, cap=cap)
@@ -68,14 +72,13 @@ def test_ensemble_many_replicates_autotuned():
     assert st["failed_replicates"] == [47]
     ok = np.ones(64, bool)
     ok[47] = False
-    assert np.all(it_out[ok] == 12) and np.all(t_out > 0)
+    assert np.all(it_out[ok] == 12) and np.all(t_out[ok] > 0) and it_out[47] < 12
     assert st["n_final"] == sum(o.shape[0] for o in out) >= sizes.sum()
     for r in (0, 37):
         cfg.seed = seeds[r]
         capr = 2 * reps[r].shape[0] + 64
         want, _, _, _ = sim.simulate(_ctx(capr), cfg, reps[r], cap=capr)
-        assert out[r].shape == want.shape
-        assert np.max(np.abs(out[r][:, 1:15] - want[:, 1:15])) < 1e-9
+        assert np.array_equal(out[r], want)
 
 
 def test_failed_replicate_matches_independent_failure():
