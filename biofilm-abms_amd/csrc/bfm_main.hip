@@ -9,7 +9,8 @@
 //   k_keys + scan + k_scatter     counting sort of the cells into bin order
 //   k_candidates_lds              half-stencil candidate screening from LDS-staged
 //                                 bins, LDS queue, segment-segment tests
-//   scan (64-bit), k_compact      sorted (i asc, j asc) rows + per-cell row lists
+//   scan (64-bit), k_compact_rows, k_compact_jrole
+//                                 sorted (i asc, j asc) rows + per-cell row lists
 //   k_pair_forces                 Hertz / JKR magnitude per row (64-byte records)
 //   k_velocities                  per cell: forces summed in the reference's row
 //                                 order, surface forces, velocity solve
@@ -552,6 +553,201 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
     }
 }
 
+// The candidate search of k_candidates_lds with the screen turned around: a
+// work-group owns 64 consecutive bin-ordered homes, lane l of every wave
+// holds home l (FP32 position and bound in registers), and each wave takes
+// staged candidates one at a time (broadcast from LDS) and screens it against
+// all 64 homes at once. Lanes whose home is not in the current group, not
+// within `reach` bins in x, or not after the candidate in bin order are idle.
+// Hits are appended with one LDS atomic per wave (ballot + mbcnt). The test
+// of the survivors and everything written are identical to k_candidates_lds.
+template <int CAP, int BATCH, int QCAP>
+__global__ void __launch_bounds__(256)
+k_candidates_wave(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
+                  const int* __restrict__ sid, const int* __restrict__ skey,
+                  const int* __restrict__ rep, const RepGrid* __restrict__ grid,
+                  const int* __restrict__ bin_off, int slot_cap, unsigned long long* __restrict__ cnt64,
+                  int* __restrict__ slot_j, double* __restrict__ slot_rec,
+                  int* __restrict__ jslot, unsigned long long* __restrict__ counters)
+{
+    constexpr int HOMES = 64;
+    static_assert(CAP <= 1024 && BATCH * CAP <= QCAP, "queue entry packing / bound");
+    __shared__ float4 c_f[CAP];
+    __shared__ int c_q[CAP], c_bx[CAP];
+    __shared__ uint16_t queue[QCAP];
+    __shared__ int s_ovf;
+    __shared__ double h_x[HOMES], h_y[HOMES], h_z[HOMES], h_nx[HOMES], h_ny[HOMES], h_nz[HOMES], h_hl[HOMES];
+    __shared__ int h_id[HOMES], h_bx[HOMES], h_row[HOMES], h_rep[HOMES];
+    __shared__ int seg_lo[49], seg_pre[50];
+    __shared__ int s_qn;
+    __shared__ unsigned long long s_screened;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int h0 = xcd_block(blockIdx.x, gridDim.x) * HOMES;
+    if (h0 >= n) return;
+    const int H = min(HOMES, n - h0);
+    const Plan P = *plan;
+    const double thr = P.thr, pre = P.pre;
+    const int reach = P.reach;
+    if (tid < H) {
+        const int hs = h0 + tid;
+        h_x[tid] = sorted[hs]; h_y[tid] = sorted[stride + hs]; h_z[tid] = sorted[2 * stride + hs];
+        h_nx[tid] = sorted[3 * stride + hs]; h_ny[tid] = sorted[4 * stride + hs];
+        h_nz[tid] = sorted[5 * stride + hs]; h_hl[tid] = sorted[6 * stride + hs];
+        h_id[tid] = sid[hs];
+        const int key = skey[hs];
+        const int r = rep ? rep[h_id[tid]] : 0;
+        const int bx = (key - grid[r].base) % grid[r].dims[0];
+        h_bx[tid] = bx;
+        h_row[tid] = key - bx;
+        h_rep[tid] = r;
+    }
+    if (tid == 0) s_screened = 0;
+    __syncthreads();
+    // this lane's home, in registers
+    float4 hf = make_float4(0.f, 0.f, 0.f, 0.f);
+    int my_bx = 0;
+    if (lane < H) {
+        hf = make_float4((float)h_x[lane], (float)h_y[lane], (float)h_z[lane],
+                         (float)(thr + h_hl[lane] + P.fmargin));
+        my_bx = h_bx[lane];
+    }
+    unsigned long long screened = 0, tested = 0;
+    const int side = 2 * reach + 1;
+    const int nseg = reach * side + reach + 1;
+
+    for (int g0 = 0; g0 < H;) {
+        const int row = h_row[g0];
+        int g1 = g0 + 1;
+        while (g1 < H && h_row[g1] == row) ++g1;
+        const int xmin = h_bx[g0], xmax = h_bx[g1 - 1];
+        const RepGrid Gr = grid[h_rep[g0]];
+        const int rrow = (row - Gr.base) / Gr.dims[0];
+        const int by = rrow % Gr.dims[1], bz = rrow / Gr.dims[1];
+        const int last = h0 + g1 - 1;
+        if (tid < nseg) {
+            const int dz = tid / side - reach, dy = tid % side - reach;
+            const int zz = bz + dz, yy = by + dy;
+            int lo = 0, len = 0;
+            if (zz >= 0 && zz < Gr.dims[2] && yy >= 0 && yy < Gr.dims[1]) {
+                const int rb = Gr.base + (zz * Gr.dims[1] + yy) * Gr.dims[0];
+                const int xlo = max(0, xmin - reach);
+                const int xhi = (dz == 0 && dy == 0) ? xmax : min(Gr.dims[0] - 1, xmax + reach);
+                lo = bin_off[rb + xlo];
+                int hi = bin_off[rb + xhi + 1];
+                if (dz == 0 && dy == 0) hi = min(hi, last);
+                len = max(0, hi - lo);
+            }
+            seg_lo[tid] = lo;
+            seg_pre[tid + 1] = len;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            seg_pre[0] = 0;
+            for (int k = 0; k < nseg; ++k) seg_pre[k + 1] += seg_pre[k];
+        }
+        __syncthreads();
+        const int S = seg_pre[nseg];
+        for (int c0 = 0; c0 < S; c0 += CAP) {
+            const int C = min(CAP, S - c0);
+            for (int t = tid; t < C; t += 256) {
+                const int gidx = c0 + t;
+                int sg = 0;
+                while (seg_pre[sg + 1] <= gidx) ++sg;
+                const int q = seg_lo[sg] + (gidx - seg_pre[sg]);
+                c_f[t] = make_float4((float)sorted[q], (float)sorted[stride + q], (float)sorted[2 * stride + q],
+                                     (float)sorted[6 * stride + q]);
+                c_q[t] = q;
+                c_bx[t] = (skey[q] - Gr.base) % Gr.dims[0];
+            }
+            __syncthreads();
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                const int BT = attempt == 0 ? (g1 - g0) : BATCH;
+                bool overflow = false;
+                for (int b0 = g0; b0 < g1; b0 += BT) {
+                    const int b1 = min(b0 + BT, g1);
+                    if (tid == 0) { s_qn = 0; s_ovf = 0; }
+                    __syncthreads();
+                    const bool in_batch = lane >= b0 && lane < b1;
+                    for (int t = wave; t < C; t += 4) {
+                        const float4 cf = c_f[t];
+                        const int cbx = c_bx[t];
+                        const int cq = c_q[t];
+                        const bool valid = in_batch && abs(my_bx - cbx) <= reach && cq < h0 + lane;
+                        // FP32 pre-screen; its rounding error is far below the margin in hf.w
+                        const float fx = hf.x - cf.x, fy = hf.y - cf.y, fz = hf.z - cf.z;
+                        const float fb = hf.w + cf.w;
+                        const bool pass = valid && (fx * fx + fy * fy + fz * fz <= fb * fb);
+                        screened += valid ? 1 : 0;
+                        const unsigned long long mask = __ballot(pass);
+                        if (mask) {
+                            const int cnt = __popcll(mask);
+                            int base = 0;
+                            if (lane == 0) base = atomicAdd(&s_qn, cnt);
+                            base = __shfl(base, 0);
+                            if (pass) {
+                                const int below = __builtin_amdgcn_mbcnt_hi(
+                                    (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
+                                const int pos = base + below;
+                                if (pos < QCAP) queue[pos] = (uint16_t)(((lane - b0) << 10) | t);
+                                else s_ovf = 1;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    if (s_ovf) { overflow = true; break; }
+                    const int qn = s_qn;
+                    for (int e = tid; e < qn; e += 256) {
+                        const int ent = queue[e];
+                        const int h = b0 + (ent >> 10), t = ent & 1023;
+                        Seg ch, cc;
+                        ch.r[0] = h_x[h]; ch.r[1] = h_y[h]; ch.r[2] = h_z[h];
+                        ch.n[0] = h_nx[h]; ch.n[1] = h_ny[h]; ch.n[2] = h_nz[h]; ch.hl = h_hl[h];
+                        const int cq = c_q[t];
+                        cc = load_sorted(sorted, stride, cq);
+                        const int cid = sid[cq];
+                        const bool home_is_i = h_id[h] > cid;
+                        const int i = home_is_i ? h_id[h] : cid;
+                        const int j = home_is_i ? cid : h_id[h];
+                        const double dx = ch.r[0] - cc.r[0], dy = ch.r[1] - cc.r[1], dz = ch.r[2] - cc.r[2];
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        const double bound = (thr + ch.hl + cc.hl) * (1 + 1e-9) + 1e-9;
+                        if (!(r2 <= bound * bound && sqrt(r2) < pre)) continue;
+                        double d[3], sv, tv;
+                        if (home_is_i) dist_between_cells(ch, cc, d, sv, tv);
+                        else dist_between_cells(cc, ch, d, sv, tv);
+                        if (norm3(d[0], d[1], d[2]) < thr) {
+                            const int slot = (int)(atomicAdd(&cnt64[i], 1ull << 32) >> 32);
+                            const int slot2 = (int)(atomicAdd(&cnt64[j], 1ull) & 0xffffffffull);
+                            if (slot < slot_cap) {
+                                const int64_t e2 = (int64_t)slot * stride + i;
+                                slot_j[e2] = j;
+                                double2* r = reinterpret_cast<double2*>(slot_rec + 8 * e2);
+                                r[0] = make_double2(d[0], d[1]);
+                                r[1] = make_double2(d[2], sv);
+                                r[2] = make_double2(tv, __longlong_as_double((long long)slot2));
+                            }
+                            if (slot2 < slot_cap) jslot[(int64_t)slot2 * stride + j] = i;
+                        }
+                    }
+                    if (tid == 0) tested += (unsigned long long)qn;
+                    __syncthreads();
+                }
+                if (!overflow) break;
+                __syncthreads();
+            }
+        }
+        g0 = g1;
+    }
+    atomicAdd(&s_screened, screened);
+    __syncthreads();
+    if (tid == 0) {
+        atomicAdd(&counters[0], s_screened);
+        atomicAdd(&counters[1], tested);
+    }
+}
+
 // Packs the slots into the reference's row order and builds the per-cell
 // force lists. Rows of cell i start at hi(off64[i]) and are sorted by
 // partner j (rank by comparison). The force list of cell c (starting at
@@ -559,33 +755,50 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
 // ascending row) followed by the rows where c is j (ascending i), i.e. all
 // rows touching c in row order -- the order the reference's sequential pair
 // loop adds them (mechanics.hpp:1438-1647); bit 31 marks the j role.
+// Eight lanes per cell: lane k of a cell's group holds the cell's entries
+// k, k + 8, ...; an entry's rank among the cell's partners comes from
+// width-8 shuffles, so the work per cell is O(entries) per lane and no wave
+// waits for its most connected cell's O(entries^2) loop.
 // Pass 1 (per cell i): rows, i-role entries, and the row index of each pair
 // stored at the pair's slot in j's partner list (jrow).
-__global__ void k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
-                               const unsigned long long* __restrict__ off64, const int* __restrict__ slot_j,
-                               const double* __restrict__ slot_rec,
-                               int64_t max_pairs, int* __restrict__ pi, int* __restrict__ pj,
-                               double* __restrict__ pdx, double* __restrict__ pdy, double* __restrict__ pdz,
-                               double* __restrict__ ps, double* __restrict__ pt, int* __restrict__ csr,
-                               int* __restrict__ jrow, int* __restrict__ d_m, int* __restrict__ status)
+__global__ void __launch_bounds__(256)
+k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+               const unsigned long long* __restrict__ off64, const int* __restrict__ slot_j,
+               const double* __restrict__ slot_rec, int64_t max_pairs, int* __restrict__ pi, int* __restrict__ pj,
+               double* __restrict__ pdx, double* __restrict__ pdy, double* __restrict__ pdz,
+               double* __restrict__ ps, double* __restrict__ pt, int* __restrict__ csr,
+               int* __restrict__ jrow, int* __restrict__ d_m, int* __restrict__ status)
 {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const bool fits = (int64_t)(off64[n] >> 32) <= max_pairs;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(g >> 3), k = (int)(g & 7);
+    int ci = 0;
+    int64_t oi = 0, base2 = 0;
+    if (c < n) {
         const unsigned long long cv = cnt64[c], ov = off64[c];
-        int ci = (int)(cv >> 32);
+        ci = (int)(cv >> 32);
         const int cj = (int)(cv & 0xffffffffull);
         if (ci > slot_cap || cj > slot_cap) {
-            atomicOr(status, ST_SLOT_OVERFLOW);
+            if (k == 0) atomicOr(status, ST_SLOT_OVERFLOW);
             ci = min(ci, slot_cap);
         }
-        const int64_t oi = (int64_t)(ov >> 32), base2 = oi + (int64_t)(ov & 0xffffffffull);
-        const bool fits = oi + ci <= max_pairs;
-        for (int e = 0; e < ci; ++e) {
+        oi = (int64_t)(ov >> 32);
+        base2 = oi + (int64_t)(ov & 0xffffffffull);
+    }
+    for (int a = 0; a * 8 < ci; ++a) {
+        const int e = a * 8 + k;
+        const int je = e < ci ? slot_j[(int64_t)e * stride + c] : INT_MAX;
+        int rank = 0;
+        for (int b = 0; b * 8 < ci; ++b) {
+            const int eb = b * 8 + k;
+            const int jb = b == a ? je : (eb < ci ? slot_j[(int64_t)eb * stride + c] : INT_MAX);
+#pragma unroll
+            for (int f = 0; f < 8; ++f) rank += __shfl(jb, f, 8) < je;
+        }
+        if (e < ci && fits) {
             const int64_t se = (int64_t)e * stride + c;
-            const int je = slot_j[se];
-            int rank = 0;
-            for (int f = 0; f < ci; ++f) rank += slot_j[(int64_t)f * stride + c] < je;
             const int64_t row = oi + rank;
-            if (!fits) continue;
             const double2* r = reinterpret_cast<const double2*>(slot_rec + 8 * se);
             const double2 r0 = r[0], r1 = r[1], r2 = r[2];
             pi[row] = c; pj[row] = je;
@@ -595,30 +808,45 @@ __global__ void k_compact_rows(int n, int64_t stride, int slot_cap, const unsign
             const int s2 = (int)__double_as_longlong(r2.y);
             if (s2 < slot_cap) jrow[(int64_t)s2 * stride + je] = (int)row;
         }
-        if (c == n - 1) {
-            int64_t m = (int64_t)(off64[n] >> 32);
-            if (m > max_pairs) { atomicOr(status, ST_PAIR_OVERFLOW); m = max_pairs; }
-            *d_m = (int)m;
-        }
+    }
+    if (c == n - 1 && k == 0) {
+        int64_t m = (int64_t)(off64[n] >> 32);
+        if (m > max_pairs) { atomicOr(status, ST_PAIR_OVERFLOW); m = max_pairs; }
+        *d_m = (int)m;
+    }
     }
 }
 
-// Pass 2 (per cell c): j-role entries sorted by partner i.
-__global__ void k_compact_jrole(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
-                                const unsigned long long* __restrict__ off64, const int* __restrict__ jslot,
-                                const int* __restrict__ jrow, int64_t max_pairs, int* __restrict__ csr)
+// Pass 2 (per cell c, eight lanes): j-role entries sorted by partner i.
+__global__ void __launch_bounds__(256)
+k_compact_jrole(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+                const unsigned long long* __restrict__ off64, const int* __restrict__ jslot,
+                const int* __restrict__ jrow, int64_t max_pairs, int* __restrict__ csr)
 {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    if ((int64_t)(off64[n] >> 32) > max_pairs) return;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(g >> 3), k = (int)(g & 7);
+    int ci = 0, cj = 0;
+    int64_t base2 = 0;
+    if (c < n) {
         const unsigned long long cv = cnt64[c], ov = off64[c];
-        const int ci = min((int)(cv >> 32), slot_cap), cj = min((int)(cv & 0xffffffffull), slot_cap);
-        const int64_t base2 = (int64_t)(ov >> 32) + (int64_t)(ov & 0xffffffffull);
-        if ((int64_t)(off64[n] >> 32) > max_pairs) continue;
-        for (int e = 0; e < cj; ++e) {
-            const int ie = jslot[(int64_t)e * stride + c];
-            int rank = 0;
-            for (int f = 0; f < cj; ++f) rank += jslot[(int64_t)f * stride + c] < ie;
-            csr[base2 + ci + rank] = jrow[(int64_t)e * stride + c] | (int)0x80000000u;
+        ci = min((int)(cv >> 32), slot_cap);
+        cj = min((int)(cv & 0xffffffffull), slot_cap);
+        base2 = (int64_t)(ov >> 32) + (int64_t)(ov & 0xffffffffull);
+    }
+    for (int a = 0; a * 8 < cj; ++a) {
+        const int e = a * 8 + k;
+        const int ie = e < cj ? jslot[(int64_t)e * stride + c] : INT_MAX;
+        int rank = 0;
+        for (int b = 0; b * 8 < cj; ++b) {
+            const int eb = b * 8 + k;
+            const int ib = b == a ? ie : (eb < cj ? jslot[(int64_t)eb * stride + c] : INT_MAX);
+#pragma unroll
+            for (int f = 0; f < 8; ++f) rank += __shfl(ib, f, 8) < ie;
         }
+        if (e < cj) csr[base2 + ci + rank] = jrow[(int64_t)e * stride + c] | (int)0x80000000u;
+    }
     }
 }
 
@@ -1466,13 +1694,21 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
             c->d_plan, c->d_sorted, stride, c->d_sid, c->d_skey, c->n_rep > 0 ? c->d_rep : (const int*)nullptr, \
             c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64,                                             \
             c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_counters)
+#define BFM_CANDW(CAP, BATCH, QCAP)                                                                      \
+    KLAUNCH(c, "candidates", (k_candidates_wave<CAP, BATCH, QCAP>), (n + 63) / 64, 256, 0, n, c->d_plan,     \
+            c->d_sorted, stride, c->d_sid, c->d_skey, c->n_rep > 0 ? c->d_rep : (const int*)nullptr,         \
+            c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_jslot,        \
+            c->d_counters)
     switch (c->cand_variant) {
+        case 4: BFM_CANDW(512, 8, 4096); break;
+        case 5: BFM_CANDW(1024, 8, 8192); break;
         case 0: BFM_CAND(32, 640, 8, 8192); break;
         case 2: BFM_CAND(64, 1024, 8, 8192); break;
         case 3: BFM_CAND(64, 512, 8, 4096); break;
         default: BFM_CAND(64, 640, 8, 8192); break;
     }
 #undef BFM_CAND
+#undef BFM_CANDW
 }
 
 // Device neighbor build into the context's internal list.
@@ -1524,10 +1760,10 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
         size_t tb = c->temp_bytes;
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, c->d_cnt64, c->d_off64, n + 1, c->stream));
     }
-    KLAUNCH(c, "compact_rows", k_compact_rows, grid_for(n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64, c->d_off64,
+    KLAUNCH(c, "compact_rows", k_compact_rows, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64, c->d_off64,
             c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps,
             c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status);
-    KLAUNCH(c, "compact_jrole", k_compact_jrole, grid_for(n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64,
+    KLAUNCH(c, "compact_jrole", k_compact_jrole, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64,
             c->d_off64, c->d_jslot, c->d_jrow, c->max_pairs, c->d_csr);
     c->h_m = -1;
     c->list_csr = true;
@@ -2168,7 +2404,7 @@ int bfm_autotune_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, dou
         need_params(c);
         check_cells(c, n, ld);
         static const double scales[] = {0.5, 0.75, 1.0, 1.25};
-        static const int variants[] = {0, 1, 2, 3};
+        static const int variants[] = {0, 1, 2, 3, 4, 5};
         double best = INFINITY, bs = c->bin_scale;
         int bv = c->cand_variant;
         hipEvent_t e0, e1;
@@ -2261,7 +2497,7 @@ int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads, int kernel_v
 {
     return guarded(c, [&] {
         if (kernel_variant >= 0) {
-            require(kernel_variant <= 3, BFM_ERR_ARG, "kernel_variant must be 0..3");
+            require(kernel_variant <= 5, BFM_ERR_ARG, "kernel_variant must be 0..5");
             c->cand_variant = kernel_variant;
         }
         if (bin_scale > 0) {

@@ -73,7 +73,9 @@ def test_tiny_inputs(ctx, orc, n):
 
 @pytest.mark.parametrize("bin_scale,block,variant", [
     (0.5, 256, 0), (2.0, 64, 0), (1.0, 512, 0), (0.34, 128, 0),
-    (0.5, 0, 1), (2.0, 0, 1), (1.0, 0, 1), (0.34, 0, 1)])
+    (0.5, 0, 1), (2.0, 0, 1), (1.0, 0, 1), (0.34, 0, 1),
+    (1.0, 0, 2), (0.5, 0, 3), (1.25, 0, 3),
+    (1.0, 0, 4), (0.5, 0, 4), (0.34, 0, 4), (2.0, 0, 4), (1.25, 0, 5), (0.75, 0, 5)])
 def test_tuning_knobs_do_not_change_the_list(orc, bin_scale, block, variant):
     import bfm
     cells, _ = bfm.make_dome(8000, seed=12)
@@ -81,6 +83,21 @@ def test_tuning_knobs_do_not_change_the_list(orc, bin_scale, block, variant):
     c.set_tuning(bin_scale, block, variant)
     got = c.get_cell_neighbors(cells, THR, R, LDIV)
     ref = orc.get_cell_neighbors(cells, THR, R, LDIV)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("variant", [3, 4, 5])
+def test_dense_packing_queue_overflow_path(orc, variant):
+    """A packing dense enough that a group's survivors overflow the LDS queue
+    in one pass, so the candidate kernels redo the chunk in home batches."""
+    import bfm
+    from gpu_common import random_box
+    cells = random_box(6000, 9.0, 21)
+    c = bfm.Context(6000, max_pairs=500 * 6000, params=bfm.core.make_params())
+    c.set_tuning(0, 0, variant)
+    got = c.get_cell_neighbors(cells, THR, R, LDIV)
+    ref = orc.get_cell_neighbors(cells, THR, R, LDIV, binned=True)
+    assert got.shape[0] > 50 * 6000
     assert np.array_equal(got, ref)
 
 
