@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=200000)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--jkr", default="exact", choices=["exact", "table", "surrogate"])
+    ap.add_argument("--jkr", default="exact", choices=["exact", "table", "surrogate", "none"],
+                    help="JKR radius: exact FP64 solve, 1-D table, learned radius, or no adhesion (Hertz only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU work for the sampled O(N^2) neighbor search")
@@ -150,8 +151,8 @@ def main():
     thr = 0.5 * (2 * R + Ldiv)
     cells, dome_r = bfm.make_dome(n, seed=args.seed)
     jkr_mode = {"exact": bfm.core.JKR_EXACT, "table": bfm.core.JKR_TABLE_1D,
-                "surrogate": bfm.core.JKR_SURROGATE}[args.jkr]
-    params = bfm.core.make_params(jkr_mode=jkr_mode)
+                "surrogate": bfm.core.JKR_SURROGATE, "none": bfm.core.JKR_EXACT}[args.jkr]
+    params = bfm.core.make_params(jkr_mode=jkr_mode, adhesion=args.jkr != "none")
     ctx = bfm.Context(n, max_pairs=16 * n, params=params)
     if args.bin_scale or args.block or args.variant >= 0:
         ctx.set_tuning(args.bin_scale, args.block, args.variant)

@@ -284,23 +284,22 @@ __global__ void k_keys(const double* __restrict__ cells, int n, int ld, const Pl
         int bz = bin_coord(cellv(cells, ld, i, C_RZ), P.lo[2], P.edge, P.dims[2]);
         int key = (bz * P.dims[1] + by) * P.dims[0] + bx;
         keys[i] = key;
-        vals[i] = i;
-        atomicAdd(&hist[key], 1);
+        vals[i] = atomicAdd(&hist[key], 1);   // rank of the cell within its bin
     }
 }
 
-// Counting-sort scatter: each cell takes the next free position of its bin
-// (cursor starts at the bin offsets) and its geometry is copied there. The
+// Counting-sort scatter: each cell goes to its bin's offset plus its rank
+// within the bin (taken by the histogram atomic of k_keys), with its geometry. The
 // order inside a bin depends on atomic timing, but nothing downstream does:
 // every pair is tested from the larger original index, and each cell's list
 // is sorted by partner index during compaction.
 __global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const int* __restrict__ keys,
-                          int* __restrict__ cursor, double* __restrict__ sorted, int* __restrict__ sid,
-                          int* __restrict__ skey, int64_t stride)
+                          const int* __restrict__ rank, const int* __restrict__ bin_off,
+                          double* __restrict__ sorted, int* __restrict__ sid, int* __restrict__ skey, int64_t stride)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int key = keys[i];
-        const int q = atomicAdd(&cursor[key], 1);
+        const int q = bin_off[key] + rank[i];
         sid[q] = i;
         skey[q] = key;
         sorted[0 * stride + q] = cellv(cells, ld, i, C_RX);
@@ -1554,8 +1553,7 @@ __global__ void k_keys_rep(const double* __restrict__ cells, int n, int ld, cons
         int bz = bin_coord(cellv(cells, ld, i, C_RZ), G.lo[2], edge, G.dims[2]);
         int key = G.base + (bz * G.dims[1] + by) * G.dims[0] + bx;
         keys[i] = key;
-        vals[i] = i;
-        atomicAdd(&hist[key], 1);
+        vals[i] = atomicAdd(&hist[key], 1);   // rank of the cell within its bin
     }
 }
 
@@ -1748,10 +1746,8 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
         KLAUNCH(c, "keys", k_keys, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_keys, c->d_vals,
                 c->d_bin_count);
     scan(c, c->d_bin_count, c->d_bin_off, (int64_t)c->max_bins + 1, "scan_bins");
-    HIPCHK(hipMemcpyAsync(c->d_bin_count, c->d_bin_off, sizeof(int) * ((size_t)c->max_bins + 1),
-                          hipMemcpyDeviceToDevice, c->stream));
     const int64_t stride = c->max_cells;
-    KLAUNCH(c, "scatter", k_scatter, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_bin_count,
+    KLAUNCH(c, "scatter", k_scatter, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals, c->d_bin_off,
             c->d_sorted, c->d_sid, c->d_skey, stride);
     KLAUNCH(c, "fill", k_fill_u64, grid_for(n + 1, 256, 4096), 256, 0, c->d_cnt64, (int64_t)n + 1);
     launch_candidates(c, n, stride);
