@@ -69,7 +69,7 @@ struct bfm_ctx {
     int slot_cap = 32;
     double bin_scale = 1.0;
     int block_threads = 256;
-    int cand_variant = 3;           // LDS candidate kernel configuration (HOMES, CAP, BATCH)
+    int cand_variant = 6;           // LDS candidate kernel configuration (HOMES, CAP, BATCH)
 
     bfm_params p{};
     bool params_set = false;

@@ -364,7 +364,10 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
                  int* __restrict__ slot_j, double* __restrict__ slot_rec,
                  int* __restrict__ jslot, unsigned long long* __restrict__ counters)
 {
-    static_assert(HOMES <= 64 && CAP <= 1024 && BATCH * CAP <= QCAP, "queue entry packing / bound");
+    // queue entry = (home - batch start) << TB | candidate, 16 bits
+    constexpr int TB = CAP <= 512 ? 9 : 10;
+    static_assert(HOMES <= (1 << (16 - TB)) && CAP <= (1 << TB) && BATCH * CAP <= QCAP,
+                  "queue entry packing / bound");
     __shared__ float4 c_f[CAP];             // FP32 copy (x, y, z, hl) for the pre-screen
     __shared__ int c_id[CAP], c_q[CAP], c_bx[CAP];
     __shared__ uint16_t queue[QCAP];
@@ -493,7 +496,7 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
                             const float fb = hf.w + cf.w;
                             if (fx * fx + fy * fy + fz * fz <= fb * fb) {
                                 const int pos = atomicAdd(&s_qn, 1);
-                                if (pos < QCAP) queue[pos] = (uint16_t)(((h - b0) << 10) | t);
+                                if (pos < QCAP) queue[pos] = (uint16_t)(((h - b0) << TB) | t);
                                 else s_ovf = 1;
                             }
                         }
@@ -503,7 +506,7 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
                     const int qn = s_qn;
                     for (int e = tid; e < qn; e += 256) {
                         const int ent = queue[e];
-                        const int h = b0 + (ent >> 10), t = ent & 1023;
+                        const int h = b0 + (ent >> TB), t = ent & ((1 << TB) - 1);
                         Seg ch, cc;
                         ch.r[0] = h_x[h]; ch.r[1] = h_y[h]; ch.r[2] = h_z[h];
                         ch.n[0] = h_nx[h]; ch.n[1] = h_ny[h]; ch.n[2] = h_nz[h]; ch.hl = h_hl[h];
@@ -1700,6 +1703,7 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
     switch (c->cand_variant) {
         case 4: BFM_CANDW(512, 8, 4096); break;
         case 5: BFM_CANDW(1024, 8, 8192); break;
+        case 6: BFM_CAND(128, 512, 8, 4096); break;
         case 0: BFM_CAND(32, 640, 8, 8192); break;
         case 2: BFM_CAND(64, 1024, 8, 8192); break;
         case 3: BFM_CAND(64, 512, 8, 4096); break;
@@ -2400,7 +2404,7 @@ int bfm_autotune_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, dou
         need_params(c);
         check_cells(c, n, ld);
         static const double scales[] = {0.5, 0.75, 1.0, 1.25};
-        static const int variants[] = {0, 1, 2, 3, 4, 5};
+        static const int variants[] = {0, 1, 2, 3, 4, 5, 6};
         double best = INFINITY, bs = c->bin_scale;
         int bv = c->cand_variant;
         hipEvent_t e0, e1;
@@ -2493,7 +2497,7 @@ int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads, int kernel_v
 {
     return guarded(c, [&] {
         if (kernel_variant >= 0) {
-            require(kernel_variant <= 5, BFM_ERR_ARG, "kernel_variant must be 0..5");
+            require(kernel_variant <= 6, BFM_ERR_ARG, "kernel_variant must be 0..6");
             c->cand_variant = kernel_variant;
         }
         if (bin_scale > 0) {
