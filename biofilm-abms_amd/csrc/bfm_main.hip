@@ -899,6 +899,8 @@ __global__ void k_update_distances(const int* __restrict__ d_m, const double* __
 {
     const int m = *d_m;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        // rows of a build that overflowed its capacity (reported by status) may be stale
+        if ((unsigned)pi[k] >= (unsigned)ld || (unsigned)pj[k] >= (unsigned)ld) continue;
         Seg ci = load_cell_seg(cells, ld, pi[k]);
         Seg cj = load_cell_seg(cells, ld, pj[k]);
         double d[3], s, t;
@@ -970,6 +972,12 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
     const int m = *d_m;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
         const int i = pi[k], j = pj[k];
+        if ((unsigned)i >= (unsigned)ld || (unsigned)j >= (unsigned)ld) {
+            // a row of a build that overflowed its capacity (reported by status)
+            double2* r = reinterpret_cast<double2*>(rec + (int64_t)k * 8);
+            r[0] = r[1] = r[2] = r[3] = make_double2(0.0, 0.0);
+            continue;
+        }
         if (jcnt) atomicAdd(&jcnt[j], 1);
         const double dx = pdx[k], dy = pdy[k], dz = pdz[k];
         const double dist = norm3(dx, dy, dz);
@@ -1067,6 +1075,7 @@ __device__ __forceinline__ void add_row(const double* __restrict__ rec, int k, b
 struct RowLists {
     const unsigned long long* off64;
     const int* csr;
+    int64_t cap;            // max_pairs: bounds for a list that overflowed (reported by status)
     const int* ioff;
     const int* perm_i;
     const int* joff;
@@ -1082,10 +1091,13 @@ __device__ __forceinline__ void gather_cc(int c, const RowLists& L, const double
     if constexpr (CSR) {
         const unsigned long long o0 = L.off64[c], o1 = L.off64[c + 1];
         const int64_t e0 = (int64_t)(o0 >> 32) + (int64_t)(o0 & 0xffffffffull);
-        const int64_t e1 = (int64_t)(o1 >> 32) + (int64_t)(o1 & 0xffffffffull);
+        const int64_t e1 = min((int64_t)(o1 >> 32) + (int64_t)(o1 & 0xffffffffull), 2 * L.cap);
         for (int64_t e = e0; e < e1; ++e) {
+            // a build whose slots or pair capacity overflowed is redone or
+            // reported; until then its entries may be stale: stay in bounds
             const int ent = L.csr[e];
-            add_row(rec, ent & 0x7fffffff, ent >= 0, R, acc);
+            const int row = ent & 0x7fffffff;
+            if (row < L.cap) add_row(rec, row, ent >= 0, R, acc);
         }
     } else {
         int a = L.ioff[c], ae = L.ioff[c + 1], b = L.joff[c], be = L.joff[c + 1];
@@ -1821,6 +1833,7 @@ RowLists row_lists(bfm_ctx* c)
     RowLists L;
     L.off64 = c->d_off64;
     L.csr = c->d_csr;
+    L.cap = c->max_pairs;
     L.ioff = c->d_offsets;
     L.perm_i = c->use_perm_i ? c->d_perm_i : nullptr;
     L.joff = c->d_joff;

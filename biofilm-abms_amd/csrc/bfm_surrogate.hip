@@ -263,7 +263,8 @@ k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int 
     const int waves_total = gridDim.x * 4;
     for (int base = (blockIdx.x * 4 + wave) * 64; base < m; base += waves_total * 64) {
         const int k = base + lane;
-        bool valid = k < m, inside = false;
+        // rows of a build that overflowed its capacity (reported by status) may be stale
+        bool valid = k < m && (unsigned)pi[k] < (unsigned)ld && (unsigned)pj[k] < (unsigned)ld, inside = false;
         double delta = 0, gamma = 0;
         if (valid) {
             const double dist = norm3(pdx[k], pdy[k], pdz[k]);
