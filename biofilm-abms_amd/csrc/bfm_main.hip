@@ -374,6 +374,7 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
     __shared__ int s_ovf;
     __shared__ double h_x[HOMES], h_y[HOMES], h_z[HOMES], h_nx[HOMES], h_ny[HOMES], h_nz[HOMES], h_hl[HOMES];
     __shared__ int h_id[HOMES], h_bx[HOMES], h_row[HOMES], h_rep[HOMES];
+    __shared__ int h_gbase[HOMES], h_gd0[HOMES], h_gd1[HOMES], h_gd2[HOMES];   // the home's replicate grid
     __shared__ float4 h_f[HOMES];           // FP32 (x, y, z, thr + hl + 1e-3)
     __shared__ int seg_lo[49], seg_pre[50];
     __shared__ int s_qn;
@@ -398,10 +399,15 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
         // key - x identifies the (replicate, y, z) row
         const int key = skey[hs];
         const int r = rep ? rep[h_id[tid]] : 0;
-        const int bx = (key - grid[r].base) % grid[r].dims[0];
+        const int gbase = grid[r].base, gd0 = grid[r].dims[0];
+        const int bx = (key - gbase) % gd0;
         h_bx[tid] = bx;
         h_row[tid] = key - bx;
         h_rep[tid] = r;
+        h_gbase[tid] = gbase;
+        h_gd0[tid] = gd0;
+        h_gd1[tid] = grid[r].dims[1];
+        h_gd2[tid] = grid[r].dims[2];
     }
     if (tid == 0) s_screened = 0;
     __syncthreads();
@@ -414,31 +420,37 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
         int g1 = g0 + 1;
         while (g1 < H && h_row[g1] == row) ++g1;
         const int xmin = h_bx[g0], xmax = h_bx[g1 - 1];
-        const RepGrid Gr = grid[h_rep[g0]];
-        const int rrow = (row - Gr.base) / Gr.dims[0];
-        const int by = rrow % Gr.dims[1], bz = rrow / Gr.dims[1];
+        const int gbase = h_gbase[g0], gd0 = h_gd0[g0], gd1 = h_gd1[g0], gd2 = h_gd2[g0];
+        const int rrow = (row - gbase) / gd0;
+        const int by = rrow % gd1, bz = rrow / gd1;
         const int last = h0 + g1 - 1;                  // sorted position of the group's last home
-        if (tid < nseg) {
-            // rows (dz, dy) lexicographically before (0, 0), then (0, 0) itself
-            const int dz = tid / side - reach, dy = tid % side - reach;
-            const int zz = bz + dz, yy = by + dy;
+        if (tid < 64) {   // the first wave: segment bounds and their prefix sums (nseg <= 49)
             int lo = 0, len = 0;
-            if (zz >= 0 && zz < Gr.dims[2] && yy >= 0 && yy < Gr.dims[1]) {
-                const int rb = Gr.base + (zz * Gr.dims[1] + yy) * Gr.dims[0];
-                const int xlo = max(0, xmin - reach);
-                const int xhi = (dz == 0 && dy == 0) ? xmax : min(Gr.dims[0] - 1, xmax + reach);
-                lo = bin_off[rb + xlo];
-                int hi = bin_off[rb + xhi + 1];
-                if (dz == 0 && dy == 0) hi = min(hi, last);
-                len = max(0, hi - lo);
+            if (tid < nseg) {
+                // rows (dz, dy) lexicographically before (0, 0), then (0, 0) itself
+                const int dz = tid / side - reach, dy = tid % side - reach;
+                const int zz = bz + dz, yy = by + dy;
+                if (zz >= 0 && zz < gd2 && yy >= 0 && yy < gd1) {
+                    const int rb = gbase + (zz * gd1 + yy) * gd0;
+                    const int xlo = max(0, xmin - reach);
+                    const int xhi = (dz == 0 && dy == 0) ? xmax : min(gd0 - 1, xmax + reach);
+                    lo = bin_off[rb + xlo];
+                    int hi = bin_off[rb + xhi + 1];
+                    if (dz == 0 && dy == 0) hi = min(hi, last);
+                    len = max(0, hi - lo);
+                }
             }
-            seg_lo[tid] = lo;
-            seg_pre[tid + 1] = len;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            seg_pre[0] = 0;
-            for (int k = 0; k < nseg; ++k) seg_pre[k + 1] += seg_pre[k];
+            int incl = len;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off);
+                if (tid >= off) incl += v;
+            }
+            if (tid < nseg) {
+                seg_lo[tid] = lo;
+                seg_pre[tid + 1] = incl;
+            }
+            if (tid == 0) seg_pre[0] = 0;
         }
         __syncthreads();
         const int S = seg_pre[nseg];
@@ -453,7 +465,7 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
                                      (float)sorted[6 * stride + q]);
                 c_id[t] = sid[q];
                 c_q[t] = q;
-                c_bx[t] = (skey[q] - Gr.base) % Gr.dims[0];
+                c_bx[t] = (skey[q] - gbase) % gd0;
             }
             __syncthreads();
             // First try the whole group as one batch; if the queue would
