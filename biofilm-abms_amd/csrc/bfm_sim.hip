@@ -658,210 +658,25 @@ struct DevBuf {
     ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
-void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
-              const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
-              int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats)
-{
-    check_config(cfg, ncols);
-    if (!c->params_set) sim_fail(BFM_ERR_ARG, "bfm_set_params has not been called");
-    if (c->p.ncols != ncols) sim_fail(BFM_ERR_ARG, "ncols differs from the parameters' ncols");
-    if (!cells_in || !cells_out || !n_out || n_in < 1 || cap < n_in || cap > c->max_cells)
-        sim_fail(BFM_ERR_ARG, "need 1 <= n_in <= cap <= max_cells and non-NULL arrays");
-    const auto wall0 = std::chrono::steady_clock::now();
-    double div_seconds = 0;
-    const TableauDef tb = tableau(cfg->tableau);
-    hipStream_t st = c->stream;
-    const int ld = cap;
-
-    DevBuf<double> bufA((size_t)ld * ncols), bufB((size_t)ld * ncols);
-    DevBuf<int> d_count(1), d_idx(cap), d_flags(cap), d_rowidx(2 * (size_t)cap);
-    DevBuf<double> d_err(1), d_rows((size_t)2 * cap * ncols);
-    PinnedBuf h_small(64);
-    double* d_cur = bufA.p;
-    double* d_new = bufB.p;
-    SCHK(hipMemsetAsync(d_flags.p, 0, sizeof(int) * cap, st));
-    SCHK(hipMemcpy2DAsync(d_cur, sizeof(double) * ld, cells_in, sizeof(double) * n_in, sizeof(double) * n_in,
-                          ncols, hipMemcpyHostToDevice, st));
-
-    std::vector<int> parents;
-    if (parents_in && n_parents_in > 0) {
-        parents.assign(parents_in, parents_in + n_parents_in);
-    } else {
-        int max_id = 0;
-        for (int i = 0; i < n_in; ++i) max_id = std::max(max_id, (int)cells_in[(size_t)C_ID * n_in + i]);
-        parents.assign(max_id + 1, -1);
-    }
-
-    Mt19937 rng((uint32_t)cfg->seed);
-    const double thr = cfg->neighbor_threshold, R = cfg->R, Ldiv = cfg->Ldiv;
-    int n = n_in;
-    int64_t iter = 0, rk_steps = 0, divisions = 0, builds = 0;
-    double t = 0, dt = cfg->max_stepsize;
-
-    auto build = [&] {
-        int64_t m = 0;   // synchronous build: grows the per-cell slot capacity when needed
-        abi(c, bfm_dev_build_neighbors(c, d_cur, n, ld, thr, R, Ldiv, &m));
-        ++builds;
-    };
-    auto read_int = [&](const int* d) {
-        SCHK(hipMemcpyAsync(h_small.p, d, sizeof(int), hipMemcpyDeviceToHost, st));
-        SCHK(hipStreamSynchronize(st));
-        return *(int*)h_small.p;
-    };
-    auto rk = [&](double h) {
-        abi(c, bfm_dev_step_rk(c, tb.A, tb.b, tb.bs, tb.s, h, d_cur, n, ld, nullptr, d_new, nullptr,
-                               cfg->max_error_allowed, d_err.p));
-        ++rk_steps;
-    };
-    auto step_error = [&] {
-        SCHK(hipMemcpyAsync(h_small.p, d_err.p, sizeof(double), hipMemcpyDeviceToHost, st));
-        SCHK(hipStreamSynchronize(st));
-        return std::max(*(double*)h_small.p, cfg->min_error);
-    };
-    DeviceMinDist dmin(cap, st);
-    BaseMinFn base_min = [&](const std::vector<Query>& q, double cut, std::vector<double>& out) {
-        dmin.run(q, d_cur, ld, d_flags.p, cut, out);
-    };
-
-    build();
-    std::vector<int> idx;
-    auto wall = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count(); };
-    double next_report = cfg->report_seconds;
-    while (!terminate_now(cfg, n, iter, t)) {
-        if (cfg->max_wall_seconds > 0 && wall() >= cfg->max_wall_seconds) break;
-        if (cfg->report_seconds > 0 && wall() >= next_report) {
-            std::fprintf(stderr, "[bfm_simulate] wall %.1f s  iter %lld  n %d  t %.6f h  dt %.3e h  rk steps %lld\n",
-                         wall(), (long long)iter, n, t, dt, (long long)rk_steps);
-            std::fflush(stderr);
-            next_report += cfg->report_seconds;
-        }
-        // divideMaxLength + divideCells
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
-        hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
-                           d_count.p, d_idx.p);
-        SCHK(hipGetLastError());
-        int nd = read_int(d_count.p);
-        if (nd > 0) {
-            auto td = std::chrono::steady_clock::now();
-            if (n + nd > cap) sim_fail(BFM_ERR_CAPACITY, "division would exceed the cell capacity");
-            idx.resize(nd);
-            SCHK(hipMemcpyAsync(idx.data(), d_idx.p, sizeof(int) * nd, hipMemcpyDeviceToHost, st));
-            SCHK(hipStreamSynchronize(st));
-            std::sort(idx.begin(), idx.end());
-            SCHK(hipMemcpyAsync(d_idx.p, idx.data(), sizeof(int) * nd, hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for((int64_t)nd * ncols, 256)), dim3(256), 0, st, d_cur,
-                               ld, ncols, d_idx.p, nd, d_rows.p);
-            std::vector<double> colm((size_t)nd * ncols), mrows((size_t)nd * ncols);
-            SCHK(hipMemcpyAsync(colm.data(), d_rows.p, sizeof(double) * colm.size(), hipMemcpyDeviceToHost, st));
-            hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p, nd,
-                               1);
-            SCHK(hipStreamSynchronize(st));
-            for (int r = 0; r < nd; ++r)
-                for (int k = 0; k < ncols; ++k) mrows[(size_t)r * ncols + k] = colm[(size_t)k * nd + r];
-            std::vector<DivisionJob> jobs(1);
-            DivisionJob& jb = jobs[0];
-            jb.rng = &rng;
-            jb.parents = &parents;
-            jb.t = t;
-            jb.lo = 0;
-            jb.hi = n;
-            jb.idx = idx;
-            jb.mrows = std::move(mrows);
-            divide_batch(cfg, ncols, jobs, base_min);
-            // write back: mothers in place, daughters appended at n, n+1, ...
-            std::vector<int> dst(2 * (size_t)nd);
-            std::vector<double> rows((size_t)2 * nd * ncols);
-            for (int r = 0; r < nd; ++r) {
-                dst[r] = idx[r];
-                dst[nd + r] = n + r;
-                for (int k = 0; k < ncols; ++k) {
-                    rows[(size_t)k * 2 * nd + r] = jb.mo[(size_t)r * ncols + k];
-                    rows[(size_t)k * 2 * nd + nd + r] = jb.da[(size_t)r * ncols + k];
-                }
-            }
-            SCHK(hipMemcpyAsync(d_rowidx.p, dst.data(), sizeof(int) * dst.size(), hipMemcpyHostToDevice, st));
-            SCHK(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_scatter_rows, dim3(blocks_for((int64_t)2 * nd * ncols, 256)), dim3(256), 0, st,
-                               d_cur, ld, ncols, d_rowidx.p, 2 * nd, d_rows.p);
-            hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p, nd,
-                               0);
-            SCHK(hipGetLastError());
-            SCHK(hipStreamSynchronize(st));
-            n += nd;
-            divisions += nd;
-            div_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
-            build();
-        }
-
-        // stepRungeKuttaAdaptive with the step-size controller (simulation.hpp:1677-1769)
-        rk(dt);
-        if (iter % cfg->iter_update_stepsize == 0) {
-            const double p1 = 1.0 / (tb.order + 1);
-            double max_error = step_error();
-            bool exceeded = max_error > 1.0;
-            double factor = 0.9 * pow(1.0 / max_error, p1);
-            if (factor >= 10) factor = 10;
-            else if (factor < 0.2) factor = 0.2;
-            int j = 0;
-            while (exceeded && j < cfg->max_tries_update_stepsize) {
-                double dt_new = dt * factor;
-                rk(dt_new);
-                max_error = step_error();
-                exceeded = max_error > 1.0;
-                factor *= 0.9 * pow(1.0 / max_error, p1);
-                if (factor >= 10) {
-                    factor = 10;
-                    break;
-                } else if (factor < 0.2) {
-                    factor = 0.2;
-                    break;
-                }
-                j++;
-            }
-            if (dt * factor < cfg->min_stepsize) factor = cfg->min_stepsize / dt;
-            else if (dt * factor > cfg->max_stepsize) factor = cfg->max_stepsize / dt;
-            dt *= factor;
-            rk(dt);
-        }
-        std::swap(d_cur, d_new);
-        hipLaunchKernelGGL(k_grow, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, R, dt);
-        SCHK(hipGetLastError());
-        abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
-        abi(c, bfm_sync(c));   // device status: non-finite velocities, JKR roots
-        if (dt_trace && iter < cap_trace) dt_trace[iter] = dt;
-        t += dt;
-        iter++;
-        if (iter % cfg->iter_update_neighbors == 0) build();
-    }
-    abi(c, bfm_sync(c));
-    SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, d_cur, sizeof(double) * ld, sizeof(double) * n, ncols,
-                          hipMemcpyDeviceToHost, st));
-    SCHK(hipStreamSynchronize(st));
-    *n_out = n;
-    if (parents_out && n_parents_out) {
-        std::copy(parents.begin(), parents.end(), parents_out);
-        *n_parents_out = (int)parents.size();
-    }
-    if (stats) {
-        stats->iterations = iter;
-        stats->rk_steps = rk_steps;
-        stats->divisions = divisions;
-        stats->neighbor_builds = builds;
-        stats->t_end = t;
-        stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
-        stats->host_division_seconds = div_seconds;
-    }
-}
-
 // runSimulation for a batch of independent replicates in lockstep: every
 // iteration advances each unfinished replicate by its own dt with its own
 // controller, division stream and lineage. Replicates occupy contiguous row
 // ranges (re-packed after divisions, which keeps each replicate's row order),
 // the neighbor build keeps pairs inside a replicate, and one captured graph
 // (stages + update for all replicates) serves every RK attempt.
+// Extra inputs and outputs of a one-replicate run (bfm_simulate).
+struct SingleRunIO {
+    const int* parents_in = nullptr;
+    int n_parents_in = 0;
+    int* parents_out = nullptr;
+    int* n_parents_out = nullptr;
+    double* dt_trace = nullptr;     // accepted dt of replicate 0 per iteration
+    int cap_trace = 0;
+};
+
 void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, const double* cells_in, int ncols,
               const uint64_t* seeds, int autotune, double* cells_out, int cap, int* sizes_out, double* t_out,
-              int64_t* iters_out, int* failed_out, bfm_ensemble_stats* stats)
+              int64_t* iters_out, int* failed_out, bfm_ensemble_stats* stats, const SingleRunIO* io = nullptr)
 {
     check_config(cfg, ncols);
     if (!c->params_set) sim_fail(BFM_ERR_ARG, "bfm_set_params has not been called");
@@ -914,6 +729,10 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     std::vector<std::vector<int>> parents(R);
     for (int r = 0; r < R; ++r) {
         rng.emplace_back((uint32_t)seeds[r]);
+        if (io && r == 0 && io->parents_in && io->n_parents_in > 0) {
+            parents[r].assign(io->parents_in, io->parents_in + io->n_parents_in);
+            continue;
+        }
         int max_id = 0;
         for (int i = start[r]; i < start[r + 1]; ++i) max_id = std::max(max_id, (int)cells_in[(size_t)C_ID * n + i]);
         parents[r].assign(max_id + 1, -1);
@@ -996,9 +815,11 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     // Non-finite results freeze their replicate at the state the iteration
     // started from (the reference stops such a run, simulation.hpp:1803-1822);
     // the others go on. Returns whether a replicate failed in the last replay.
-    auto check_failures = [&](std::vector<char>& mask) {
-        SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
-        SCHK(hipStreamSynchronize(st));
+    auto check_failures = [&](std::vector<char>& mask, bool flags_read) {
+        if (!flags_read) {
+            SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+        }
         bool any_new = false;
         for (int r = 0; r < R; ++r)
             if (p_bad[r] && !failed[r]) {
@@ -1012,8 +833,10 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         if (any_new) SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
         return any_new;
     };
+    // error maxima and failure flags of the last replay (one synchronisation)
     auto read_errors = [&](std::vector<double>& e) {
         SCHK(hipMemcpyAsync(p_err, d_errmax.p, sizeof(unsigned long long) * R, hipMemcpyDeviceToHost, st));
+        SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
         SCHK(hipStreamSynchronize(st));
         for (int r = 0; r < R; ++r) e[r] = std::max(order_key_value(p_err[r]), cfg->min_error);
     };
@@ -1034,8 +857,13 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             double tmin = 1e300;
             for (int r = 0; r < R; ++r)
                 if (act[r]) { ++n_act; tmin = std::min(tmin, t[r]); }
-            std::fprintf(stderr, "[bfm_ensemble] wall %.1f s  iter %lld  cells %d  active %d  min t %.6f h  attempts %lld\n",
-                         wall(), (long long)iter, n, n_act, tmin, (long long)attempts);
+            if (R == 1)
+                std::fprintf(stderr, "[bfm_simulate] wall %.1f s  iter %lld  n %d  t %.6f h  dt %.3e h  rk attempts %lld\n",
+                             wall(), (long long)iter, n, t[0], dt[0], (long long)attempts);
+            else
+                std::fprintf(stderr,
+                             "[bfm_ensemble] wall %.1f s  iter %lld  cells %d  active %d  min t %.6f h  attempts %lld\n",
+                             wall(), (long long)iter, n, n_act, tmin, (long long)attempts);
             std::fflush(stderr);
             next_report += cfg->report_seconds;
         }
@@ -1147,7 +975,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             std::vector<char> exceeded(R, 0), retry(R, 0);
             std::vector<int> j(R, 0);
             read_errors(err);
-            check_failures(act);
+            check_failures(act, true);
             bool any_retry = false;
             for (int r = 0; r < R; ++r) {
                 if (!act[r]) continue;
@@ -1163,7 +991,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
                 for (int r = 0; r < R; ++r) trial[r] = retry[r] ? dt[r] * factor[r] : dt[r];
                 replay(retry, trial);
                 read_errors(err);
-                check_failures(retry);
+                check_failures(retry, true);
                 any_retry = false;
                 for (int r = 0; r < R; ++r) {
                     if (!retry[r]) continue;
@@ -1187,11 +1015,12 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         }
         // a replicate that failed in the accepted attempt: redo the attempt
         // without it, so its rows are carried over unchanged
-        while (check_failures(act)) replay(act, dt);
+        while (check_failures(act, false)) replay(act, dt);
         std::swap(d_cur, d_new);
         abi(c, grow_batched(c, d_cur, ld, d_n.p, Rr, c->d_rep, d_dt.p, d_active.p));
         abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
         abi(c, bfm_sync(c));
+        if (io && io->dt_trace && act[0] && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = dt[0];
         for (int r = 0; r < R; ++r)
             if (act[r]) { t[r] += dt[r]; iters[r]++; }
         iter++;
@@ -1208,6 +1037,10 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         if (failed_out) failed_out[r] = failed[r];
     }
     abi(c, set_replicate_count(c, 0));
+    if (io && io->parents_out && io->n_parents_out) {
+        std::copy(parents[0].begin(), parents[0].end(), io->parents_out);
+        *io->n_parents_out = (int)parents[0].size();
+    }
     if (stats) {
         stats->iterations = iter;
         stats->attempts = attempts;
@@ -1222,6 +1055,41 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         stats->n_final = n;
         stats->failed = n_failed;
     }
+}
+
+// runSimulation for one colony: the batched loop with one replicate (one
+// captured graph per RK attempt, forces computed once per iteration).
+void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
+              const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
+              int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats)
+{
+    if (!cells_in || !cells_out || !n_out || n_in < 1 || cap < n_in || cap > c->max_cells)
+        sim_fail(BFM_ERR_ARG, "need 1 <= n_in <= cap <= max_cells and non-NULL arrays");
+    SingleRunIO io;
+    io.parents_in = parents_in;
+    io.n_parents_in = n_parents_in;
+    io.parents_out = parents_out;
+    io.n_parents_out = n_parents_out;
+    io.dt_trace = dt_trace;
+    io.cap_trace = cap_trace;
+    const uint64_t seed = cfg ? cfg->seed : 0;
+    int n = 0, failed = 0;
+    double t = 0;
+    int64_t iters = 0;
+    bfm_ensemble_stats es;
+    std::memset(&es, 0, sizeof(es));
+    ensemble(c, cfg, 1, &n_in, cells_in, ncols, &seed, 0, cells_out, cap, &n, &t, &iters, &failed, &es, &io);
+    *n_out = n;
+    if (stats) {
+        stats->iterations = iters;
+        stats->rk_steps = es.attempts;
+        stats->divisions = es.divisions;
+        stats->neighbor_builds = es.neighbor_builds;
+        stats->t_end = t;
+        stats->wall_seconds = es.wall_seconds;
+        stats->host_division_seconds = es.host_division_seconds;
+    }
+    if (failed) sim_fail(BFM_ERR_NAN, "non-finite positions or orientations (the run stopped at the last finite state)");
 }
 
 }  // namespace
