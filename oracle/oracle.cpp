@@ -11,7 +11,7 @@
  * a * b * c as (a * b) * c), and the build disables FMA contraction, so the
  * rounding of each step is that of the plain IEEE operation.
  *
- * Deliberate differences from the reference (all documented in DESIGN.md):
+ * Deliberate differences from the reference:
  *  - distBetweenCells, non-parallel branch: CGAL's segment-segment routine is
  *    replaced by the clamped closed-form minimiser below (the minimiser of a
  *    strictly convex quadratic is unique, so results agree to rounding).
