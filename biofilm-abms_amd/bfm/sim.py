@@ -154,6 +154,49 @@ def rng_uniform(seed, count):
     return out
 
 
+def jkr_equilibrium_distance(R, E0, gamma, dinit, dmin, increment_overlap=1e-8, newton_tol=1e-8,
+                            max_iter=1000):
+    """jkrEquilibriumDistance (jkr.hpp:1310-1340) with its default arguments:
+    the cell-cell distance at which the JKR force vanishes."""
+    lib = load_library()
+    deq, res = C.c_double(), C.c_double()
+    rc = lib.bfm_jkr_equilibrium_distance(R, E0, gamma, dinit, dmin, increment_overlap,
+                                          newton_tol, max_iter, C.byref(deq), C.byref(res))
+    if rc != 0:
+        raise BFMError(rc, "bfm_jkr_equilibrium_distance failed")
+    return deq.value
+
+
+def jkr_optimal_surface_energy_density(R, Rcell, E0, deq_target, init_gamma=10.0,
+                                       log_increment_gamma=1e-6, increment_overlap=1e-8,
+                                       outer_newton_tol=1e-8, inner_newton_tol=1e-8,
+                                       outer_max_iter=1000, inner_max_iter=1000):
+    """jkrOptimalSurfaceEnergyDensity (jkr.hpp:1369-1408) with its default
+    arguments: the surface energy density whose JKR equilibrium distance is
+    deq_target."""
+    lib = load_library()
+    g, res = C.c_double(), C.c_double()
+    rc = lib.bfm_jkr_optimal_surface_energy_density(
+        R, Rcell, E0, deq_target, init_gamma, log_increment_gamma, increment_overlap,
+        outer_newton_tol, inner_newton_tol, outer_max_iter, inner_max_iter, C.byref(g),
+        C.byref(res))
+    if rc != 0:
+        raise BFMError(rc, "bfm_jkr_optimal_surface_energy_density failed")
+    return g.value
+
+
+def init_surface_energy(cells, eqdist, R=0.8, Rcell=0.5, E0=3900.0):
+    """The adhesion setup of runSimulation (simulation.hpp:1352-1374): the
+    surface energy density for the equilibrium distance `eqdist` (solved from
+    init_gamma 100), written to the gamma column for group-1 cells and zero for
+    the others. Returns the density."""
+    gamma = jkr_optimal_surface_energy_density(R, Rcell, E0, eqdist, 100.0, 1e-6, 1e-8, 1e-8,
+                                               1e-8, 1000, 1000)
+    group1 = cells[:, COL["group"]] == 1
+    cells[:, COL["gamma"]] = np.where(group1, gamma, 0.0)
+    return gamma
+
+
 def ensemble_simulate(ctx: Context, cfg: SimConfig, replicates, seeds, autotune=True, cap=None):
     """Runs every replicate's loop on the GPU in lockstep (bfm_ensemble_simulate).
     replicates: list of (n_r x ncols) cell arrays; seeds: one division stream

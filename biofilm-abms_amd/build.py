@@ -14,7 +14,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["csrc/bfm_main.hip", "csrc/bfm_surrogate.hip", "csrc/bfm_sim.hip"]
+HIP_SOURCES = ["csrc/bfm_main.hip", "csrc/bfm_surrogate.hip", "csrc/bfm_sim.hip",
+               "csrc/bfm_setup.hip"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]

@@ -175,7 +175,7 @@ __host__ __device__ __forceinline__ void dist_between_cells(const Seg& c1, const
 // Larger root of the JKR quartic (jkr.hpp:67-144), as sqrt(a) = u, the largest
 // positive root of u^4 - p u - q (p = Req sqrt(4 pi gamma / E), q = Req delta).
 // Newton from an upper bound decreases monotonically onto the root.
-__device__ __forceinline__ double jkr_radius_newton(double delta, double Req, double E,
+__host__ __device__ __forceinline__ double jkr_radius_newton(double delta, double Req, double E,
                                                     double gamma, double u0, int max_iter,
                                                     int* iters)
 {

@@ -245,6 +245,33 @@ int bfm_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, 
 /* The first `count` uniform_01 draws of the random stream seeded with seed. */
 int bfm_rng_uniform(uint64_t seed, int count, double* out);
 
+/* ---- JKR setup (host only, no device) ------------------------------------ */
+/* jkrEquilibriumDistance (jkr.hpp:1310-1340): the centerline distance 2R - delta
+ * at which the JKR force of two cells of radius R (equivalent radius R / 2)
+ * vanishes. newtonRaphson (rootFinding.hpp:42-75) on delta from 2R - d_init,
+ * central differences of step increment_overlap, stop at |force| <= newton_tol
+ * or max_iter steps; delta capped at 2R - d_min. residual_out (may be NULL)
+ * receives the force at the last iterate. BFM_ERR_JKR_ROOT when an iterate
+ * falls below the pull-off overlap (the reference throws there, jkr.hpp:99-105),
+ * BFM_ERR_NAN when the iteration diverges. */
+int bfm_jkr_equilibrium_distance(double R, double E0, double gamma, double d_init, double d_min,
+                                 double increment_overlap, double newton_tol, int max_iter,
+                                 double* deq_out, double* residual_out);
+
+/* jkrOptimalSurfaceEnergyDensity (jkr.hpp:1369-1408): the surface energy density
+ * gamma whose equilibrium distance (inner solves from 0.75 * 2R, capped at
+ * 2 Rcell) is deq_target, by newtonRaphson on log10(gamma) from
+ * log10(init_gamma). runSimulation calls it with init_gamma 100,
+ * log_increment_gamma 1e-6, increment_overlap 1e-8, tolerances 1e-8 and 1000
+ * iterations each (simulation.hpp:1354-1358). residual_out (may be NULL)
+ * receives distance minus target at the last iterate. */
+int bfm_jkr_optimal_surface_energy_density(double R, double Rcell, double E0, double deq_target,
+                                           double init_gamma, double log_increment_gamma,
+                                           double increment_overlap, double outer_newton_tol,
+                                           double inner_newton_tol, int outer_max_iter,
+                                           int inner_max_iter, double* gamma_out,
+                                           double* residual_out);
+
 /* ---- batched replicates (no counterpart in the reference: many independent
  * runSimulation loops advanced together) ---------------------------------- */
 typedef struct bfm_ensemble_stats {

@@ -378,6 +378,71 @@ double oracle_jkr_force_magnitude(double radius, double Req, double E0, double g
     return term1 - term2;
 }
 
+/* newtonRaphson (rootFinding.hpp:42-75): Newton with the derivative from
+ * central differences of step dx while |f| > tol and iter < max_iter. */
+extern "C++" template <class F>
+double newton_raphson(F&& f, double x0, double dx, double tol, int max_iter, double* f_out)
+{
+    double x = x0;
+    double fx = f(x);
+    int iter = 0;
+    while (std::fabs(fx) > tol && iter < max_iter) {
+        double deriv = (f(x + dx) - f(x - dx)) / (2 * dx);
+        double update = -fx / deriv;
+        x += update;
+        fx = f(x);
+        iter++;
+    }
+    if (f_out) *f_out = fx;
+    return x;
+}
+
+/* jkrEquilibriumDistance (jkr.hpp:1310-1340): Newton on the overlap x of
+ * f(x) = 4/3 E0/Req a^3 - 4 sqrt(pi gamma E0) a^1.5, a = JKR radius at x with
+ * Req = R/2, from 2R - dinit; overlap capped at 2R - dmin (std::min order).
+ * NaN when an iterate has no JKR root (the reference throws there). */
+double oracle_jkr_equilibrium_distance(double R, double E0, double gamma, double dinit,
+                                       double dmin, double increment_overlap, double newton_tol,
+                                       int max_iter, double* residual)
+{
+    double overlap = 2 * R - dinit;
+    double max_overlap = 2 * R - dmin;
+    auto func = [&](double x) {
+        double Req = R / 2;
+        double radius = oracle_jkr_contact_radius(x, Req, E0, gamma, nullptr);
+        double prefactor1 = 4.0 / 3.0 * E0 / Req;
+        double prefactor2 = 4 * std::sqrt(kPi * gamma * E0);
+        double f_hertz = prefactor1 * radius * radius * radius;
+        double f_jkr = prefactor2 * std::pow(radius, 1.5);
+        return f_hertz - f_jkr;
+    };
+    double root = newton_raphson(func, overlap, increment_overlap, newton_tol, max_iter, residual);
+    double delta = std::min(root, max_overlap);
+    return 2 * R - delta;
+}
+
+/* jkrOptimalSurfaceEnergyDensity (jkr.hpp:1369-1408): Newton on log10(gamma)
+ * of deq(gamma) - deq_target, inner solves from 0.75 * 2R capped at 2 Rcell. */
+double oracle_jkr_optimal_surface_energy_density(double R, double Rcell, double E0,
+                                                 double deq_target, double init_gamma,
+                                                 double log_increment_gamma,
+                                                 double increment_overlap, double outer_tol,
+                                                 double inner_tol, int outer_max_iter,
+                                                 int inner_max_iter, double* residual)
+{
+    auto func = [&](double x) {
+        double dinit = 0.75 * 2 * R;
+        double dmin = 2 * Rcell;
+        double gamma = std::pow(10.0, x);
+        double deq = oracle_jkr_equilibrium_distance(R, E0, gamma, dinit, dmin, increment_overlap,
+                                                     inner_tol, inner_max_iter, nullptr);
+        return deq - deq_target;
+    };
+    double x = newton_raphson(func, std::log10(init_gamma), log_increment_gamma, outer_tol,
+                              outer_max_iter, residual);
+    return std::pow(10.0, x);
+}
+
 /* Eigen's LinSpaced(n, lo, hi) for floating point: low half from lo upward,
  * high half from hi downward, so both ends are exact. */
 void oracle_linspace(double lo, double hi, int n, double* out)

@@ -46,6 +46,15 @@ void oracle_update_neighbor_distances(const double* cells, int n, int ld,
 double oracle_jkr_contact_radius(double delta, double Req, double E, double gamma,
                                  int* iters);
 double oracle_jkr_force_magnitude(double radius, double Req, double E0, double gamma);
+double oracle_jkr_equilibrium_distance(double R, double E0, double gamma, double dinit,
+                                       double dmin, double increment_overlap, double newton_tol,
+                                       int max_iter, double* residual);
+double oracle_jkr_optimal_surface_energy_density(double R, double Rcell, double E0,
+                                                 double deq_target, double init_gamma,
+                                                 double log_increment_gamma,
+                                                 double increment_overlap, double outer_tol,
+                                                 double inner_tol, int outer_max_iter,
+                                                 int inner_max_iter, double* residual);
 void oracle_linspace(double lo, double hi, int n, double* out);
 int oracle_nearest_value(const double* values, int n, double x);
 double oracle_jkr_table_radius_1d(const double* overlaps, const double* radii, int n,

@@ -42,6 +42,10 @@ def lib():
         L.oracle_jkr_contact_radius.restype = d
         L.oracle_jkr_force_magnitude.argtypes = [d, d, d, d]
         L.oracle_jkr_force_magnitude.restype = d
+        L.oracle_jkr_equilibrium_distance.argtypes = [d] * 7 + [C.c_int, _dp]
+        L.oracle_jkr_equilibrium_distance.restype = d
+        L.oracle_jkr_optimal_surface_energy_density.argtypes = [d] * 9 + [C.c_int, C.c_int, _dp]
+        L.oracle_jkr_optimal_surface_energy_density.restype = d
         L.oracle_linspace.argtypes = [d, d, C.c_int, _dp]
         L.oracle_linspace.restype = None
         L.oracle_nearest_value.argtypes = [_dp, C.c_int, d]
@@ -145,6 +149,22 @@ def jkr_contact_radius(delta, Req, E, gamma):
 
 def jkr_force_magnitude(radius, Req, E0, gamma):
     return lib().oracle_jkr_force_magnitude(radius, Req, E0, gamma)
+
+
+def jkr_equilibrium_distance(R, E0, gamma, dinit, dmin, increment_overlap=1e-8,
+                            newton_tol=1e-8, max_iter=1000):
+    """oracle_jkr_equilibrium_distance; NaN when an iterate has no JKR root."""
+    return lib().oracle_jkr_equilibrium_distance(R, E0, gamma, dinit, dmin, increment_overlap,
+                                                 newton_tol, max_iter, None)
+
+
+def jkr_optimal_surface_energy_density(R, Rcell, E0, deq_target, init_gamma=10.0,
+                                       log_increment_gamma=1e-6, increment_overlap=1e-8,
+                                       outer_tol=1e-8, inner_tol=1e-8, outer_max_iter=1000,
+                                       inner_max_iter=1000):
+    return lib().oracle_jkr_optimal_surface_energy_density(
+        R, Rcell, E0, deq_target, init_gamma, log_increment_gamma, increment_overlap,
+        outer_tol, inner_tol, outer_max_iter, inner_max_iter, None)
 
 
 def linspace(lo, hi, n):

@@ -17,14 +17,14 @@ TOL_STATE = 4e-12 * 10000
 TOL_CONTACT = 5e-7
 
 
-def _ctx(cap):
+def _ctx(cap, adhesion=False):
     from bfm import Context
-    return Context(cap, max_pairs=32 * cap + 1024, params=tc.params())
+    return Context(cap, max_pairs=32 * cap + 1024, params=tc.params(adhesion))
 
 
-def _check(orc, got, name, d, t_end=None, iters=None):
-    c0 = tc.cells(name, d)
-    want, trace_o, iters_o, _ = orc.simulate(tc.config(), tc.params(), c0, cap=8)
+def _check(orc, got, name, d, t_end=None, iters=None, gamma=0.0, deq=2 * tc.R):
+    c0 = tc.cells(name, d, gamma=gamma)
+    want, trace_o, iters_o, _ = orc.simulate(tc.config(), tc.params(gamma > 0), c0, cap=8)
     assert got.shape == (2, 22)
     if iters is not None:
         assert iters == iters_o
@@ -33,7 +33,7 @@ def _check(orc, got, name, d, t_end=None, iters=None):
     assert np.max(np.abs(got[:, 1:7] - want[:2, 1:7])) < TOL_STATE, (name, d)
     final = tc.distance(orc, got)
     if d < 2 * tc.R:
-        assert abs(final - 2 * tc.R) < TOL_CONTACT, (name, d, final)
+        assert abs(final - deq) < TOL_CONTACT, (name, d, final)
     else:
         assert np.array_equal(got, c0)
 
@@ -58,3 +58,20 @@ def test_two_cell_simulate(orc, name):
     got, parents, trace, stats = sim.simulate(_ctx(8), tc.config(), tc.cells(name, 1.0), cap=8)
     assert stats["divisions"] == 0
     _check(orc, got, name, 1.0, t_end=stats["t_end"], iters=stats["iterations"])
+
+
+def test_two_cell_adhesion_ensemble(orc):
+    """The 27 cases with JKR adhesion (exact contact radius) at the surface
+    energy density of a 1.5 um equilibrium distance, solved by the product's
+    host setup (bfm_jkr_optimal_surface_energy_density)."""
+    from bfm import sim
+    gamma = sim.jkr_optimal_surface_energy_density(tc.R, 0.5, 3900.0, 1.5, 100.0)
+    deq = sim.jkr_equilibrium_distance(tc.R, 3900.0, gamma, 1.5 * tc.R, 1.0)
+    reps = [tc.cells(name, d, gamma=gamma) for name, d in tc.CASES]
+    cap = 2 * len(reps) + 64
+    print(f"two-cell adhesion ensemble of {len(reps)} replicates, gamma {gamma}", flush=True)
+    out, t_out, it_out, st = sim.ensemble_simulate(_ctx(cap, adhesion=True), tc.config(), reps,
+                                                   list(range(len(reps))), autotune=False, cap=cap)
+    assert st["failed_replicates"] == [] and np.all(it_out == 10000)
+    for r, (name, d) in enumerate(tc.CASES):
+        _check(orc, out[r], name, d, t_end=t_out[r], gamma=gamma, deq=deq)

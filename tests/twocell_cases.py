@@ -8,6 +8,9 @@ Hertz repulsion acts only at overlap (distance < 2R = 1.6), so the pairs
 started at 1.0 and 1.4 are pushed apart to 2R and the pair at 1.8 never
 moves. The reference test prints the equilibrium distance next to this
 expectation (testTwoCellSimulations.cpp:114-133); the tests here assert it.
+With JKR adhesion at the surface energy density of a target equilibrium
+distance (jkrOptimalSurfaceEnergyDensity), the overlapping pairs end at that
+distance instead.
 
 Cell columns: the reference's comma initializer (testTwoCellSimulations.cpp:
 89-96) lists 22 values per row for its 21-column array, an older layout with
@@ -38,7 +41,8 @@ CONFIGS = {
 CASES = [(name, d) for name in CONFIGS for d in DISTS]
 
 
-def cells(name, d, ncols=22):
+def cells(name, d, gamma=0.0, ncols=22):
+    """The two cells of a case; gamma fills the surface energy column (21)."""
     r2, n2 = CONFIGS[name](d)
     c = np.zeros((2, ncols), order="F")
     for k, (r, n) in enumerate((((0.0, 0.0, Z), (1.0, 0.0, 0.0)), (r2, n2))):
@@ -47,6 +51,8 @@ def cells(name, d, ncols=22):
         c[k, 4:7] = n
         c[k, 13], c[k, 14] = 2 * HALF_L, HALF_L
         c[k, 17], c[k, 18], c[k, 19], c[k, 20] = 1e-3, 720.0, 100.0, 1
+        if ncols > 21:
+            c[k, 21] = gamma
     return c
 
 
@@ -64,10 +70,12 @@ def config():
     return c
 
 
-def params():
+def params(adhesion=False):
+    """No surface, Hertz only (the reference scenario), or with isotropic JKR
+    adhesion between the cells (exact contact radius)."""
     from bfm.core import make_params
     return make_params(R=R, nz_threshold=1e-20, cell_surface_coulomb_coeff=0.0,
-                       no_surface=True, adhesion=False)
+                       no_surface=True, adhesion=adhesion)
 
 
 def distance(orc, c):
