@@ -162,6 +162,14 @@ def load_library():
                                   _dp, vp, _ip, vp],
         "bfm_autotune_neighbors": [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
                                    _ip, _dp],
+        "bfm_write_cells": [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_char_p),
+                            C.POINTER(C.c_char_p), C.c_int, _ip, _ip, C.c_int],
+        "bfm_read_cells": [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int, _ip, _ip, C.c_char_p, C.c_int,
+                           _ip],
+        "bfm_write_lineage": [C.c_char_p, _ip, C.c_int],
+        "bfm_read_lineage": [C.c_char_p, _ip, C.c_int, _ip],
+        "bfm_load_config": [C.c_char_p, vp],
+        "bfm_run": [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -169,6 +177,8 @@ def load_library():
         f.restype = C.c_int
     lib.bfm_last_error.argtypes = [vp]
     lib.bfm_last_error.restype = C.c_char_p
+    lib.bfm_io_last_error.argtypes = []
+    lib.bfm_io_last_error.restype = C.c_char_p
     _LIB = lib
     return lib
 

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(PKG)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["csrc/bfm_main.hip", "csrc/bfm_surrogate.hip", "csrc/bfm_sim.hip",
-               "csrc/bfm_setup.hip"]
+               "csrc/bfm_setup.hip", "csrc/bfm_io.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]

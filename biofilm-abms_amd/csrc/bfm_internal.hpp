@@ -4,6 +4,10 @@
 // C ABI functions.
 #pragma once
 
+#include <cstdint>
+#include <functional>
+#include <vector>
+
 #include "bfm_ctx.hpp"
 
 namespace bfm {
@@ -35,5 +39,24 @@ int set_replicate_count(bfm_ctx* c, int n_rep);
 
 // Ordered-key <-> double for the scaled-error maxima.
 double order_key_value(unsigned long long k);
+
+// Frame output of a one-colony run: after every iteration in which the
+// simulated time passes a multiple of dt_write (simulation.hpp:2023-2063),
+// fn receives the cells on the host (n rows, leading dimension ld), the
+// iteration count and time after the update, the accepted dt, the last
+// scaled-error maximum, the wall seconds since the previous frame and the
+// lineage vector.
+struct FrameHook {
+    double dt_write = 0;
+    std::function<void(const double* cells, int n, int ld, int64_t iter, double t, double dt, double max_err,
+                       double wall_elapsed, const std::vector<int>& parents)>
+        fn;
+};
+
+// bfm_simulate with an optional frame hook (NULL: none). Returns a
+// bfm_status and sets the context's error message.
+int simulate_with_frames(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
+                         const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out,
+                         int* parents_out, int* n_parents_out, bfm_sim_stats* stats, const FrameHook* hook);
 
 }  // namespace bfm

@@ -672,6 +672,7 @@ struct SingleRunIO {
     int* n_parents_out = nullptr;
     double* dt_trace = nullptr;     // accepted dt of replicate 0 per iteration
     int cap_trace = 0;
+    const FrameHook* hook = nullptr;
 };
 
 void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, const double* cells_in, int ncols,
@@ -833,12 +834,16 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         if (any_new) SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
         return any_new;
     };
+    double last_err0 = 0;
+    double last_frame_wall = 0;
+    std::vector<double> frame_buf;
     // error maxima and failure flags of the last replay (one synchronisation)
     auto read_errors = [&](std::vector<double>& e) {
         SCHK(hipMemcpyAsync(p_err, d_errmax.p, sizeof(unsigned long long) * R, hipMemcpyDeviceToHost, st));
         SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
         SCHK(hipStreamSynchronize(st));
         for (int r = 0; r < R; ++r) e[r] = std::max(order_key_value(p_err[r]), cfg->min_error);
+        last_err0 = e[0];
     };
     double next_report = cfg->report_seconds;
     std::vector<int> idx;
@@ -1025,6 +1030,21 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             if (act[r]) { t[r] += dt[r]; iters[r]++; }
         iter++;
         if (iter % cfg->iter_update_neighbors == 0) build();
+        // a frame when t has just passed a multiple of dt_write (simulation.hpp:2023-2063)
+        if (io && io->hook && io->hook->fn && io->hook->dt_write > 0 && act[0]) {
+            const double dw = io->hook->dt_write;
+            const double told = std::fmod(t[0] - dt[0] + 1e-12, dw), tnew = std::fmod(t[0] + 1e-12, dw);
+            if (told > tnew) {
+                frame_buf.resize((size_t)n * ncols);
+                SCHK(hipMemcpy2DAsync(frame_buf.data(), sizeof(double) * n, d_cur, sizeof(double) * ld,
+                                      sizeof(double) * n, ncols, hipMemcpyDeviceToHost, st));
+                SCHK(hipStreamSynchronize(st));
+                const double now = wall();
+                io->hook->fn(frame_buf.data(), n, n, iters[0], t[0], dt[0], last_err0, now - last_frame_wall,
+                             parents[0]);
+                last_frame_wall = now;
+            }
+        }
     }
     abi(c, bfm_sync(c));
     SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, d_cur, sizeof(double) * ld, sizeof(double) * n, ncols,
@@ -1061,7 +1081,8 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
 // captured graph per RK attempt, forces computed once per iteration).
 void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
               const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
-              int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats)
+              int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats,
+              const FrameHook* hook = nullptr)
 {
     if (!cells_in || !cells_out || !n_out || n_in < 1 || cap < n_in || cap > c->max_cells)
         sim_fail(BFM_ERR_ARG, "need 1 <= n_in <= cap <= max_cells and non-NULL arrays");
@@ -1072,6 +1093,7 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
     io.n_parents_out = n_parents_out;
     io.dt_trace = dt_trace;
     io.cap_trace = cap_trace;
+    io.hook = hook;
     const uint64_t seed = cfg ? cfg->seed : 0;
     int n = 0, failed = 0;
     double t = 0;
@@ -1093,6 +1115,29 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
 }
 
 }  // namespace
+
+namespace bfm {
+
+int simulate_with_frames(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int n_in, int ncols,
+                         const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out,
+                         int* parents_out, int* n_parents_out, bfm_sim_stats* stats, const FrameHook* hook)
+{
+    if (!c) return BFM_ERR_ARG;
+    try {
+        SCHK(hipSetDevice(c->device));
+        simulate(c, cfg, cells_in, n_in, ncols, parents_in, n_parents_in, cells_out, cap, n_out, parents_out,
+                 n_parents_out, nullptr, 0, stats, hook);
+        return BFM_OK;
+    } catch (const SimError& e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return BFM_ERR_ARG;
+    }
+}
+
+}  // namespace bfm
 
 extern "C" {
 

@@ -220,6 +220,35 @@ __device__ inline void mlp_tile(const WFrags& F, const float* __restrict__ b, co
     __builtin_amdgcn_wave_barrier();
 }
 
+// The features in FP32 for inference: they are rounded to BF16 before the
+// first layer, so the FP32 logarithm and quotients change nothing there.
+__device__ inline void features_f32(double delta, double r, double gamma, const Envelope& e, float x[4])
+{
+    const float dn = (float)(delta / e.R);
+    x[0] = (2.0f * dn - (float)(e.dhi + e.dlo)) / (float)(e.dhi - e.dlo);
+    x[1] = x[2] = (float)(r / e.R) - 1.0f;
+    const float l0 = logf((float)e.glo), l1 = logf((float)e.ghi);
+    x[3] = (2.0f * logf((float)gamma) - (l1 + l0)) / (l1 - l0);
+}
+
+// Inference form of polish_radius: the starting guess u = t p^(1/3) carries
+// the network's error (about 1e-3), so p^(1/3) is taken in FP32; p, q and the
+// Newton steps stay FP64.
+__device__ inline double polish_radius_fast(float t, double delta, double Req, double E, double gamma, int polish)
+{
+    const double p = Req * sqrt(4 * kPi * gamma / E);
+    const double q = Req * delta;
+    double u = (double)(t * cbrtf((float)p));
+    for (int it = 0; it < polish; ++it) {
+        const double u2 = u * u;
+        const double F = (u2 * u2 - p * u) - q;
+        const double dF = (4.0 * u2) * u - p;
+        if (!(dF > 0)) break;
+        u = u - F / dF;
+    }
+    return u * u;
+}
+
 // Radius from the network output t: u = t p^(1/3), then `polish` Newton steps.
 __device__ inline double polish_radius(double t, double delta, double Req, double E, double gamma, int polish)
 {
@@ -242,10 +271,117 @@ struct RadiusArgs {
     int polish;
 };
 
+// Inference forward of a wavefront's 64 samples (four 16-sample tiles) in the
+// transposed form D = W H^T: neurons are the MFMA M index and samples the N
+// index. A layer's output fragment (lane l: neurons 16 t + 4 (l / 16) + i of
+// sample l % 16, i = 0..3) is then, after bias and SiLU, exactly the B
+// fragment of k-step t of the next layer, so activations stay in registers
+// (no LDS round trip between layers), and the weight A fragments are the
+// WFrags of the training forward. x: the four features of sample `lane`
+// (zero where unused); returns y of sample `lane`.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// Two FP32 values to packed BF16 (round to nearest even, one v_cvt_pk_bf16_f32).
+__device__ inline uint32_t pack_bf16x2(float a, float b)
+{
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2v));
+}
+
+// SiLU with the hardware exp2 and reciprocal (1 ulp) instead of an IEEE divide.
+__device__ inline float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+__device__ inline float mlp_forward64(const bf16x4 (*wl)[64], const float* bs, const float x[4], int lane)
+{
+    const int col = lane & 15, g = lane >> 4;
+    float y_mine = 0.f;
+#pragma unroll 1
+    for (int T = 0; T < 4; ++T) {
+        // the weight fragments are re-read from LDS for every tile rather than
+        // held in 72 registers across the loop
+        asm volatile("" ::: "memory");
+        // layer 1: B = X^T (K = 4 features padded to 16; lanes 0..15 hold sample 16 T + lane)
+        bf16x4 xb;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float v = __shfl(x[i], 16 * T + col);
+            xb[i] = lane < 16 ? (short)f2bf(v) : (short)0;
+        }
+        f32x4 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[t][lane], xb, *(const f32x4*)(bs + 16 * t + 4 * g),
+                                                              0, 0, 0);
+        bf16x4 h[4];
+        u32x2 hw[4];
+#pragma unroll
+        for (int layer = 0; layer < 3; ++layer) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                hw[t][0] = pack_bf16x2(silu_fast(acc[t][0]), silu_fast(acc[t][1]));
+                hw[t][1] = pack_bf16x2(silu_fast(acc[t][2]), silu_fast(acc[t][3]));
+                h[t] = __builtin_bit_cast(bf16x4, hw[t]);
+            }
+            if (layer == 2) break;
+            // the accumulator starts at the next layer's bias
+            const int fb = layer == 0 ? 4 : 20;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                f32x4 z = *(const f32x4*)(bs + 64 * (layer + 1) + 16 * t + 4 * g);
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[fb + 4 * s + t][lane], h[s], z, 0, 0, 0);
+                acc[t] = z;
+            }
+        }
+        // layer 4: y = b4 + sum_n W4[n] H3[n]; this lane holds 16 of the 64 neurons of sample col
+        float part = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const f32x4 w = *(const f32x4*)(bs + 196 + 16 * t + 4 * g);
+            part += w[0] * __builtin_bit_cast(float, hw[t][0] << 16);
+            part += w[1] * __builtin_bit_cast(float, hw[t][0] & 0xffff0000u);
+            part += w[2] * __builtin_bit_cast(float, hw[t][1] << 16);
+            part += w[3] * __builtin_bit_cast(float, hw[t][1] & 0xffff0000u);
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (T == g) y_mine = part + bs[192];
+    }
+    return y_mine;
+}
+
+// LDS copy of the weight A fragments in fragment-major order (conflict-free
+// 8-byte reads): [0, 4) W1 n-tiles, [4, 20) W2 (k-step s, n-tile t) at
+// 4 + 4 s + t, [20, 36) W3 likewise.
+__device__ inline void stage_frags(bf16x4 (*wl)[64], const uint16_t* __restrict__ w1, const uint16_t* __restrict__ w2,
+                                   const uint16_t* __restrict__ w3)
+{
+    for (int e = threadIdx.x; e < 36 * 64; e += blockDim.x) {
+        const int f = e >> 6, l = e & 63;
+        bf16x4 v;
+        if (f < 4) v = load_bfrag(w1, 16, 0, f, l);
+        else if (f < 20) v = load_bfrag(w2, 64, (f - 4) >> 2, (f - 4) & 3, l);
+        else v = load_bfrag(w3, 64, (f - 20) >> 2, (f - 20) & 3, l);
+        wl[f][l] = v;
+    }
+}
+
+// LDS copy of the biases and of W4 (as FP32) for mlp_forward64:
+// [0, 192) b1 b2 b3, [192] b4, [196, 260) W4.
+__device__ inline void stage_bias(float* bs, const float* __restrict__ b, const uint16_t* __restrict__ w4)
+{
+    for (int i = threadIdx.x; i < 260; i += blockDim.x)
+        bs[i] = i < 193 ? b[i] : (i >= 196 ? bf2f(w4[i - 196]) : 0.f);
+    __syncthreads();
+}
+
 // Per-row contact radius for the JKR pairs of the neighbor list: rows with
 // overlap > 0 and gamma > 0 inside the envelope take the network's guess and
-// the polish; other JKR rows take the exact solve. 64 rows per wavefront
-// iteration = four 16-row MFMA tiles.
+// the polish; other JKR rows take the exact solve. Each wavefront takes 64
+// rows per iteration (four 16-row MFMA tiles).
 __global__ void __launch_bounds__(256)
 k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int ld, const int* __restrict__ pi,
              const int* __restrict__ pj, const double* __restrict__ pdx, const double* __restrict__ pdy,
@@ -253,13 +389,12 @@ k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int 
              const uint16_t* __restrict__ w2, const uint16_t* __restrict__ w3, const uint16_t* __restrict__ w4,
              const float* __restrict__ bias, double* __restrict__ radius)
 {
-    __shared__ float feat[4][64 * 4];
-    __shared__ uint16_t hbuf[4][16 * 68];
-    __shared__ float yout[4][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ __align__(16) float bs[260];
+    __shared__ bf16x4 wl[36][64];
+    stage_frags(wl, w1, w2, w3);
+    stage_bias(bs, bias, w4);
     const int m = *d_m;
-    WFrags F;
-    load_frags(SurrogateWeights{}, w1, w2, w3, lane, F);
     const int waves_total = gridDim.x * 4;
     for (int base = (blockIdx.x * 4 + wave) * 64; base < m; base += waves_total * 64) {
         const int k = base + lane;
@@ -277,28 +412,15 @@ k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int 
                      gamma <= env.ghi;
         }
         float x[4] = {0.f, 0.f, 0.f, 0.f};
-        if (inside) features(delta, ra.R, ra.R, gamma, env, x);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) feat[wave][lane * 4 + c] = x[c];
-        const uint64_t any = __ballot(inside);
-        if (any) {
-            __builtin_amdgcn_wave_barrier();
-            for (int tile = 0; tile < 4; ++tile) {
-                if (!((any >> (16 * tile)) & 0xffffull)) continue;
-                float y4[4];
-                mlp_tile(F, bias, w4, &feat[wave][tile * 64], hbuf[wave], lane, y4, nullptr);
-                if ((lane & 15) == 0)
-                    for (int i = 0; i < 4; ++i) yout[wave][tile * 16 + 4 * (lane >> 4) + i] = y4[i];
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
+        if (inside) features_f32(delta, ra.R, gamma, env, x);
+        float y = 0.f;
+        if (__ballot(inside)) y = mlp_forward64(wl, bs, x, lane);
         if (valid) {
             double r;
-            if (inside) r = polish_radius((double)yout[wave][lane], delta, ra.Req, ra.E0, gamma, ra.polish);
+            if (inside) r = polish_radius_fast(y, delta, ra.Req, ra.E0, gamma, ra.polish);
             else r = jkr_radius_newton(delta, ra.Req, ra.E0, gamma, 0.0, 200, nullptr);
             radius[k] = r;
         }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -310,32 +432,21 @@ k_sur_eval(int n, const double* __restrict__ delta, const double* __restrict__ g
            const uint16_t* __restrict__ w3, const uint16_t* __restrict__ w4, const float* __restrict__ bias,
            float* __restrict__ t_out, double* __restrict__ r_out)
 {
-    __shared__ float feat[4][64 * 4];
-    __shared__ uint16_t hbuf[4][16 * 68];
-    __shared__ float yout[4][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    WFrags F;
-    load_frags(SurrogateWeights{}, w1, w2, w3, lane, F);
+    __shared__ __align__(16) float bs[260];
+    __shared__ bf16x4 wl[36][64];
+    stage_frags(wl, w1, w2, w3);
+    stage_bias(bs, bias, w4);
     const int waves_total = gridDim.x * 4;
     for (int base = (blockIdx.x * 4 + wave) * 64; base < n; base += waves_total * 64) {
         const int k = base + lane;
         float x[4] = {0.f, 0.f, 0.f, 0.f};
         if (k < n) features(delta[k], R, R, gamma[k], env, x);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) feat[wave][lane * 4 + c] = x[c];
-        __builtin_amdgcn_wave_barrier();
-        for (int tile = 0; tile < 4; ++tile) {
-            float y4[4];
-            mlp_tile(F, bias, w4, &feat[wave][tile * 64], hbuf[wave], lane, y4, nullptr);
-            if ((lane & 15) == 0)
-                for (int i = 0; i < 4; ++i) yout[wave][tile * 16 + 4 * (lane >> 4) + i] = y4[i];
-        }
-        __builtin_amdgcn_wave_barrier();
+        const float y = mlp_forward64(wl, bs, x, lane);
         if (k < n) {
-            t_out[k] = yout[wave][lane];
-            r_out[k] = polish_radius((double)yout[wave][lane], delta[k], Req, E, gamma[k], polish);
+            t_out[k] = y;
+            r_out[k] = polish_radius((double)y, delta[k], Req, E, gamma[k], polish);
         }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -601,7 +712,7 @@ const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, in
     ra.colidx_gamma = c->p.colidx_gamma;
     ra.polish = s->polish;
     bfm::launch_timed(c, "sur_radius", [&] {
-        hipLaunchKernelGGL(k_sur_radius, dim3(1024), dim3(256), 0, c->stream, c->d_m, d_cells, ld, c->pi, c->pj,
+        hipLaunchKernelGGL(k_sur_radius, dim3(1280), dim3(256), 0, c->stream, c->d_m, d_cells, ld, c->pi, c->pj,
                            c->pdx, c->pdy, c->pdz, ra, s->env, s->w1, s->w2, s->w3, s->w4, s->b, s->radius);
     });
     return s->radius;

@@ -335,6 +335,70 @@ int bfm_stats(bfm_ctx* ctx, int64_t* m, int64_t* screened, int64_t* tested, doub
  * No knob changes the neighbor list, only the time to build it. */
 int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads, int kernel_variant);
 
+/* ---- frames, lineage, configuration and the driver (host only) ---------- */
+/* Message of the last failed call below (per thread). */
+const char* bfm_io_last_error(void);
+
+/* writeCells (utils.hpp:146-198, the std::map overload): "# key = value" lines
+ * sorted by key (a repeated key keeps its last value), then one row per cell,
+ * tab-separated, 10 significant digits; columns 0 (id) and 20 (group) as
+ * integers, 1..19 as doubles, a column >= 21 only if listed in extra_cols
+ * (extra_types 0: integer, else double). cells: n x ncols, leading dim ld. */
+int bfm_write_cells(const char* path, const double* cells, int n, int ld, int ncols,
+                    const char* const* keys, const char* const* values, int n_params,
+                    const int* extra_cols, const int* extra_types, int n_extra);
+
+/* readCells (utils.hpp:95-143): rows and "# key = value" lines. n_out and
+ * ncols_out (at least 21, the widest row otherwise; short rows are padded with
+ * 0) are always set; params_out (may be NULL) receives "key = value\n" lines
+ * in file order, params_len_out their length. With cells == NULL only the
+ * sizes are returned; else cells (leading dimension ld >= n, room for
+ * max_cols columns and cap rows) receives the rows. Empty lines are skipped. */
+int bfm_read_cells(const char* path, double* cells, int cap, int ld, int max_cols, int* n_out,
+                   int* ncols_out, char* params_out, int params_cap, int* params_len_out);
+
+/* writeLineage / readLineage (utils.hpp:207-258): "i<TAB>parents[i]" lines.
+ * bfm_read_lineage returns parents[child] = parent for child ids 0..n-1
+ * (the vector abmConstAdhesion.cpp:386-393 builds); parents == NULL: count only. */
+int bfm_write_lineage(const char* path, const int* parents, int n);
+int bfm_read_lineage(const char* path, int* parents, int cap, int* n_out);
+
+/* The JSON configuration of the 3-D driver (abmConstAdhesion.cpp:31-333):
+ * keys and value kinds as there (floating-point keys need a number with a
+ * fraction or exponent, integer keys an integer literal), optional keys with
+ * the driver's defaults; Ldiv = 2 L0 + 2 R, neighbor_threshold = R + Ldiv / 2. */
+typedef struct bfm_run_config {
+    double R, Rcell, L0, Ldiv, E0, Ecell, sigma0, eta_ambient, eta_surface;
+    double max_stepsize, min_stepsize, dt_write, max_time, max_error_allowed;
+    double growth_mean, growth_std, daughter_length_std, daughter_angle_xy_bound,
+        daughter_angle_z_bound;
+    double nz_threshold, max_rxy_noise, max_rz_noise, max_nxy_noise, max_nz_noise;
+    double basal_min_overlap, adhesion_eqdist, jkr_imag_tol, jkr_aberth_tol;
+    double eta_cell_cell, cell_cell_coulomb_coeff, cell_surface_coulomb_coeff, M0;
+    double neighbor_threshold;
+    int iter_update_stepsize, iter_update_neighbors, max_iter, n_cells;
+    int basal_only, adhesion_mode, friction_mode, integration_mode, no_surface;
+    int n_mesh_overlap, n_mesh_gamma;
+    char init_filename[1024], lineage_filename[1024];
+} bfm_run_config;
+int bfm_load_config(const char* json_path, bfm_run_config* out);
+
+/* The driver main() of abmConstAdhesion.cpp:31-457 with runSimulation's output
+ * (simulation.hpp:1444-1572, 2023-2079): reads the config, starts from the
+ * founder cell or from init_filename + lineage_filename, sets the surface
+ * energy density of group 1 from adhesion_eqdist, runs the growth loop on
+ * `device` (rng_seed seeds the division stream) and writes
+ * <outprefix>_init.txt, <outprefix>_iter<k>.txt whenever t passes a multiple
+ * of dt_write, <outprefix>_final.txt and <outprefix>_lineage.txt. jkr_mode:
+ * BFM_JKR_TABLE_1D (runSimulation's table of n_mesh_overlap exact radii) or
+ * BFM_JKR_EXACT. max_cells: capacity (0: 2 max(n_cells, initial) + 16).
+ * verbose prints the reference's per-frame progress line on stdout (and a wall-clock
+ * progress line on stderr every 30 s). Modes the
+ * device loop lacks (anisotropic JKR, friction, Verlet, basal_only, noise)
+ * return BFM_ERR_UNSUPPORTED. stats_out may be NULL. */
+int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int device, int jkr_mode,
+            int max_cells, int verbose, bfm_sim_stats* stats_out);
+
 #ifdef __cplusplus
 }
 #endif
