@@ -34,7 +34,8 @@ class Params(C.Structure):
                 ("cell_cell_coulomb_coeff", C.c_double),
                 ("cell_surface_coulomb_coeff", C.c_double), ("no_surface", C.c_int),
                 ("adhesion_mode", C.c_int), ("jkr_mode", C.c_int), ("colidx_gamma", C.c_int),
-                ("friction_mode", C.c_int), ("ncols", C.c_int)]
+                ("friction_mode", C.c_int), ("ncols", C.c_int),
+                ("colidx_eta_cell_cell", C.c_int)]
 
 
 def repulsion_prefactors(R, Rcell, E0, Ecell):
@@ -46,20 +47,29 @@ def repulsion_prefactors(R, Rcell, E0, Ecell):
 
 def make_params(R=0.8, Rcell=0.5, E0=3900.0, Ecell=3.9e6, nz_threshold=1e-8,
                 cell_surface_coulomb_coeff=1.0, no_surface=False, adhesion=True,
-                jkr_mode=JKR_EXACT, ncols=22):
+                jkr_mode=JKR_EXACT, ncols=None, friction=False, cell_cell_coulomb_coeff=1.0,
+                colidx_eta_cell_cell=None):
+    """bfm_params. ncols defaults to 22 (the gamma column 21 is present) plus
+    one friction-coefficient column when friction is on, which is then the
+    last column (22 with adhesion, as simulation.hpp:1176-1188 places it)."""
+    if ncols is None:
+        ncols = 22 + int(bool(friction))
+    if colidx_eta_cell_cell is None:
+        colidx_eta_cell_cell = ncols - 1 if friction else -1
     p = Params()
     p.R, p.Rcell, p.E0 = R, Rcell, E0
     for k, v in enumerate(repulsion_prefactors(R, Rcell, E0, Ecell)):
         p.repulsion_prefactors[k] = v
     p.nz_threshold = nz_threshold
-    p.cell_cell_coulomb_coeff = 1.0
+    p.cell_cell_coulomb_coeff = cell_cell_coulomb_coeff
     p.cell_surface_coulomb_coeff = cell_surface_coulomb_coeff
     p.no_surface = int(bool(no_surface))
     p.adhesion_mode = ADHESION_JKR_ISOTROPIC if adhesion else ADHESION_NONE
     p.jkr_mode = jkr_mode
     p.colidx_gamma = COL["gamma"] if adhesion else -1
-    p.friction_mode = 0
+    p.friction_mode = 1 if friction else 0
     p.ncols = ncols
+    p.colidx_eta_cell_cell = colidx_eta_cell_cell
     return p
 
 

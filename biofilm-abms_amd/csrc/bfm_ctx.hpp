@@ -130,6 +130,8 @@ struct bfm_ctx {
 
     // per-cell forces
     double* d_cc = nullptr;         // 6 x max_cells
+    double* d_frec = nullptr;       // 4 x max_pairs friction rows {f, contact} (friction on)
+    double* d_zst = nullptr;        // 2 x max_cells stage rz, nz for the friction rows (friction on)
     double* d_k = nullptr;          // 7 x 6 x max_cells stage velocities
     unsigned long long* d_errmax = nullptr;
 

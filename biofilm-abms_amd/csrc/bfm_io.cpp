@@ -677,15 +677,18 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
         bfm_run_config o;
         load_config(json_path, &o);
         if (o.adhesion_mode == 2) io_fail(BFM_ERR_UNSUPPORTED, "anisotropic JKR adhesion is not supported");
-        if (o.friction_mode != 0) io_fail(BFM_ERR_UNSUPPORTED, "cell-cell friction is not supported");
         if (o.integration_mode == 0) io_fail(BFM_ERR_UNSUPPORTED, "velocity Verlet is not supported");
         if (o.basal_only) io_fail(BFM_ERR_UNSUPPORTED, "basal_only is not supported");
         if (o.max_rxy_noise > 0 || o.max_rz_noise > 0 || o.max_nxy_noise > 0 || o.max_nz_noise > 0)
             io_fail(BFM_ERR_UNSUPPORTED, "noise in the growth loop is not supported");
         if (jkr_mode != BFM_JKR_EXACT && jkr_mode != BFM_JKR_TABLE_1D)
             io_fail(BFM_ERR_ARG, "bfm_run: jkr_mode must be BFM_JKR_EXACT or BFM_JKR_TABLE_1D");
-        const bool adhesion = o.adhesion_mode != 0;
-        const int ncols = kNcolsRequired + (adhesion ? 1 : 0);
+        const bool adhesion = o.adhesion_mode != 0, friction = o.friction_mode != 0;
+        // extra columns: gamma (21 with adhesion), then the friction
+        // coefficient (simulation.hpp:1176-1188)
+        const int ncols = kNcolsRequired + (adhesion ? 1 : 0) + (friction ? 1 : 0);
+        const int col_gamma = adhesion ? 21 : -1;
+        const int col_eta = friction ? (adhesion ? 22 : 21) : -1;
 
         // initial population (abmConstAdhesion.cpp:333-436)
         std::vector<double> cells;
@@ -717,6 +720,7 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
             const double row[21] = {0, 0, 0, rz, 1, 0, 0, 0, 0, 0, 0, 0, 0, o.L0, o.L0 / 2, 0, o.growth_mean,
                                     o.eta_ambient, o.eta_surface, o.sigma0, 1};
             for (int j = 0; j < 21; ++j) cells[j] = row[j];
+            if (friction) cells[col_eta] = o.eta_cell_cell;
             parents.push_back(-1);
         }
 
@@ -730,7 +734,7 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
                                                                    1e-8, 1e-8, 1e-8, 1000, 1000, &max_gamma, &resid);
             if (grc != BFM_OK) io_fail(grc, "surface energy density for adhesion_eqdist could not be found");
             for (int i = 0; i < n0; ++i)
-                cells[(size_t)21 * n0 + i] = cells[(size_t)kColGroup * n0 + i] == 1 ? max_gamma : 0.0;
+                cells[(size_t)col_gamma * n0 + i] = cells[(size_t)kColGroup * n0 + i] == 1 ? max_gamma : 0.0;
             const int nm = o.n_mesh_overlap;
             if (nm < 2) io_fail(BFM_ERR_ARG, "adhesion_n_mesh_overlap must be >= 2");
             overlaps.resize(nm);
@@ -768,9 +772,10 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
         prm.no_surface = o.no_surface;
         prm.adhesion_mode = adhesion ? BFM_ADHESION_JKR_ISOTROPIC : BFM_ADHESION_NONE;
         prm.jkr_mode = jkr_mode;
-        prm.colidx_gamma = adhesion ? 21 : -1;
-        prm.friction_mode = 0;
+        prm.colidx_gamma = col_gamma;
+        prm.friction_mode = friction ? 1 : 0;
         prm.ncols = ncols;
+        prm.colidx_eta_cell_cell = col_eta;
         if ((rc2 = bfm_set_params(ctx, &prm)) != BFM_OK) io_fail(rc2, bfm_last_error(ctx));
         if (adhesion && jkr_mode == BFM_JKR_TABLE_1D &&
             (rc2 = bfm_set_jkr_table(ctx, overlaps.data(), (int)overlaps.size(), nullptr, 0, radii.data())) != BFM_OK)
@@ -803,7 +808,8 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
 
         std::map<std::string, std::string> params = frame_params(o, rng_seed, max_gamma);
         std::unordered_map<int, int> extra;
-        if (adhesion) extra[21] = 1;
+        if (adhesion) extra[col_gamma] = 1;
+        if (friction) extra[col_eta] = 1;
         const std::string prefix = outprefix;
         params["t_curr"] = fmt_g(0.0);
         write_frame(prefix + "_init.txt", cells.data(), n0, n0, ncols, params, extra);

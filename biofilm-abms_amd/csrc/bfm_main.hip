@@ -1123,6 +1123,146 @@ __device__ __forceinline__ void gather_cc(int c, const RowLists& L, const double
     }
 }
 
+// Friction of one contact row (cellCellFrictionForces, mechanics.hpp:1031-1175,
+// include_constraint = false): the tangential relative velocity at the
+// contact from the stored velocities (columns 7-12), in 2-D when the distance
+// vector is nearly horizontal and both contact points are within 1.01 R of
+// the surface (positions of the stage being evaluated), scaled by
+// -eta sqrt(Req overlap) and truncated by Coulomb's law.
+struct FricParams {
+    double R, Req, E0, coulomb;
+    int no_surface;
+    int colidx_eta;
+};
+
+// zst: stage (rz, nz) per cell, 2 x n (nullptr: the cells' own columns).
+__global__ void __launch_bounds__(256)
+k_friction_rows(const int* __restrict__ d_m, const double* __restrict__ cells, int ld, int n_zst,
+                const double* __restrict__ zst, const int* __restrict__ pi, const int* __restrict__ pj,
+                const double* __restrict__ pdx, const double* __restrict__ pdy, const double* __restrict__ pdz,
+                const double* __restrict__ ps, const double* __restrict__ pt, FricParams fq,
+                double* __restrict__ frec)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        double2* out = reinterpret_cast<double2*>(frec + (int64_t)k * 4);
+        const int i = pi[k], j = pj[k];
+        double d[3] = {pdx[k], pdy[k], pdz[k]};
+        double dist = norm3(d[0], d[1], d[2]);
+        double overlap = 2 * fq.R - dist;
+        if ((unsigned)i >= (unsigned)ld || (unsigned)j >= (unsigned)ld || !(overlap > 0)) {
+            out[0] = out[1] = make_double2(0.0, 0.0);
+            continue;
+        }
+        const double si = ps[k], sj = pt[k];
+        double dn[3] = {d[0] / dist, d[1] / dist, d[2] / dist};
+        const double eta = dmin(cellv(cells, ld, i, fq.colidx_eta), cellv(cells, ld, j, fq.colidx_eta));
+        bool a2d = false;
+        if (!fq.no_surface && fabs(dn[2]) < 1e-3) {
+            const double zi = zst ? zst[i] : cellv(cells, ld, i, C_RZ);
+            const double nzi = zst ? zst[n_zst + i] : cellv(cells, ld, i, C_NZ);
+            const double zj = zst ? zst[j] : cellv(cells, ld, j, C_RZ);
+            const double nzj = zst ? zst[n_zst + j] : cellv(cells, ld, j, C_NZ);
+            if (zi + si * nzi < 1.01 * fq.R && zj + sj * nzj < 1.01 * fq.R) a2d = true;
+        }
+        double rv[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double vi = cellv(cells, ld, i, C_DRX + q) + si * cellv(cells, ld, i, C_DNX + q);
+            const double vj = cellv(cells, ld, j, C_DRX + q) + sj * cellv(cells, ld, j, C_DNX + q);
+            rv[q] = vi - vj;
+        }
+        if (a2d) {
+            d[2] = 0;
+            dist = norm3(d[0], d[1], d[2]);
+            dn[0] = d[0] / dist; dn[1] = d[1] / dist; dn[2] = d[2] / dist;
+            overlap = 2 * fq.R - dist;
+            rv[2] = 0;
+        }
+        const double dot = (rv[0] * dn[0] + rv[1] * dn[1]) + rv[2] * dn[2];
+        const double pre = -eta * sqrt(fq.Req * overlap);
+        double f[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f[q] = pre * (rv[q] - dot * dn[q]);
+        if (fq.coulomb > 0) {
+            const double fn = norm3(f[0], f[1], f[2]);
+            const double mx = fq.coulomb * (4. / 3.) * fq.E0 * sqrt(fq.Req) * pow_half<3>(overlap);
+            if (fn > mx) {
+                const double ratio = mx / fn;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) f[q] *= ratio;
+            }
+        }
+        out[0] = make_double2(f[0], f[1]);
+        out[1] = make_double2(f[2], 1.0);   // .y: contact flag
+    }
+}
+
+// Visits a cell's rows in row order (CSR or merge form, see RowLists) as
+// fn(row, role_i).
+template <bool CSR, class F>
+__device__ __forceinline__ void for_rows(int c, const RowLists& L, F&& fn)
+{
+    if constexpr (CSR) {
+        const unsigned long long o0 = L.off64[c], o1 = L.off64[c + 1];
+        const int64_t e0 = (int64_t)(o0 >> 32) + (int64_t)(o0 & 0xffffffffull);
+        const int64_t e1 = min((int64_t)(o1 >> 32) + (int64_t)(o1 & 0xffffffffull), 2 * L.cap);
+        for (int64_t e = e0; e < e1; ++e) {
+            const int ent = L.csr[e];
+            const int row = ent & 0x7fffffff;
+            if (row < L.cap) fn(row, ent >= 0);
+        }
+    } else {
+        int a = L.ioff[c], ae = L.ioff[c + 1], b = L.joff[c], be = L.joff[c + 1];
+        while (a < ae || b < be) {
+            const int ka = (a < ae) ? (L.perm_i ? L.perm_i[a] : a) : 0x7fffffff;
+            const int kb = (b < be) ? L.perm_j[b] : 0x7fffffff;
+            const bool role_i = ka < kb;
+            if (role_i) ++a; else ++b;
+            fn(role_i ? ka : kb, role_i);
+        }
+    }
+}
+
+// The friction part of a cell's dE/dq, accumulated in row order as the
+// reference's loop does: row i: -f and -s_i f; row j: +f and -(-s_j f).
+template <bool CSR>
+__device__ __forceinline__ void gather_fric(int c, const RowLists& L, const double* __restrict__ frec,
+                                            const double* __restrict__ ps, const double* __restrict__ pt,
+                                            double acc[6])
+{
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] = 0;
+    for_rows<CSR>(c, L, [&](int row, bool role_i) {
+        const double2* r = reinterpret_cast<const double2*>(frec + (int64_t)row * 4);
+        const double2 a = r[0], b = r[1];
+        if (b.y == 0) return;
+        const double f[3] = {a.x, a.y, b.x};
+        if (role_i) {
+            const double si = ps[row];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                acc[q] -= f[q];
+                acc[q + 3] -= si * f[q];
+            }
+        } else {
+            const double msj = -pt[row];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                acc[q] += f[q];
+                acc[q + 3] -= msj * f[q];
+            }
+        }
+    });
+}
+
+// Friction inputs of the per-cell kernels (frec == nullptr: friction off).
+struct FricGather {
+    const double* frec;
+    const double* ps;
+    const double* pt;
+};
+
 __device__ __forceinline__ CellVel load_cellvel(const double* __restrict__ cells, int ld, int i)
 {
     CellVel v;
@@ -1158,11 +1298,13 @@ __device__ __forceinline__ void cell_velocity(const VelParams& vp, const CellVel
 }
 
 // getVelocities for every cell (mechanics.hpp:2011-2347).
+// cc_out receives the conservative cell-cell part only (the RK stages reuse
+// it); with friction on, the velocities use cc + friction (mechanics.hpp:1650-1656).
 template <bool CSR>
 __global__ void __launch_bounds__(256)
 k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restrict__ assume_2d,
              const double* __restrict__ noise, VelParams vp, RowLists L, const double* __restrict__ rec,
-             double* __restrict__ vel, double* __restrict__ cc_out, int* __restrict__ status)
+             FricGather fg, double* __restrict__ vel, double* __restrict__ cc_out, int* __restrict__ status)
 {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         double cc[6];
@@ -1172,6 +1314,12 @@ k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restr
             for (int q = 0; q < 6; ++q) cc_out[(int64_t)q * n + c] = cc[q];
         }
         if (!vel) continue;
+        if (fg.frec) {
+            double fr[6];
+            gather_fric<CSR>(c, L, fg.frec, fg.ps, fg.pt, fr);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cc[q] += fr[q];
+        }
         const CellVel cv = load_cellvel(cells, ld, c);
         const bool a2d = assume_2d ? (assume_2d[c] != 0) : (cv.nz < vp.nz_threshold);
         double nz6[6];
@@ -1192,11 +1340,14 @@ k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restr
 }
 
 // Conservative forces only (getConservativeForces, include_constraint = false).
+template <bool CSR>
 __global__ void k_conservative(int n, const double* __restrict__ cells, int ld,
                                const int* __restrict__ assume_2d, VelParams vp,
-                               const double* __restrict__ cc, double* __restrict__ out)
+                               const double* __restrict__ cc, RowLists L, FricGather fg, double* __restrict__ out)
 {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double fr[6] = {0, 0, 0, 0, 0, 0};
+        if (fg.frec) gather_fric<CSR>(c, L, fg.frec, fg.ps, fg.pt, fr);
         const CellVel cv = load_cellvel(cells, ld, c);
         const bool a2d = assume_2d[c] != 0;
         const double ss = a2d ? NAN : (vp.sc.R - cv.rz) / cv.nz;
@@ -1207,7 +1358,8 @@ __global__ void k_conservative(int n, const double* __restrict__ cells, int ld,
         for (int q = 0; q < 6; ++q) {
             double r = (q == 2) ? rep2 : (q == 5 ? rep5 : 0.0);
             double a = (q == 2) ? adh2 : (q == 5 ? adh5 : 0.0);
-            out[(int64_t)q * n + c] = (-cc[(int64_t)q * n + c] - r) - a;
+            const double cct = fg.frec ? cc[(int64_t)q * n + c] + fr[q] : cc[(int64_t)q * n + c];
+            out[(int64_t)q * n + c] = (-cct - r) - a;
         }
     }
 }
@@ -1218,19 +1370,60 @@ __global__ void k_assume2d(int n, const double* __restrict__ cells, int ld, doub
         a2d[c] = cellv(cells, ld, c, C_NZ) < thr;
 }
 
+// Coordinates of cell c at stage st > 0 (mechanics.hpp:2544-2548): cells +
+// dt * sum_j A(st, j) k_j, then normalizeOrientations (mechanics.hpp:1918-1959).
+__device__ __forceinline__ void stage_coords(const double* __restrict__ cells, int ld, int n, int c,
+                                             const double* __restrict__ kbuf, const Tableau& tb, int st, double dt,
+                                             double y[6])
+{
+    y[0] = cellv(cells, ld, c, C_RX); y[1] = cellv(cells, ld, c, C_RY); y[2] = cellv(cells, ld, c, C_RZ);
+    y[3] = cellv(cells, ld, c, C_NX); y[4] = cellv(cells, ld, c, C_NY); y[5] = cellv(cells, ld, c, C_NZ);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double mult = 0;
+        for (int j = 0; j < st; ++j) mult += kbuf[((int64_t)j * 6 + q) * n + c] * tb.A[st * tb.s + j];
+        y[q] += mult * dt;
+    }
+    double nrm = norm3(y[3], y[4], y[5]);
+    y[3] /= nrm; y[4] /= nrm; y[5] /= nrm;
+    if (y[5] < 0) { y[3] *= -1; y[4] *= -1; y[5] *= -1; }
+}
+
+// Stage heights and z-orientations (2 x n) for the friction rows of stage st > 0.
+__global__ void __launch_bounds__(256)
+k_stage_z(int n, const int* __restrict__ dn, const double* __restrict__ cells, int ld, const Tableau tb, int st,
+          double dt, const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
+          const double* __restrict__ kbuf, double* __restrict__ zst, int n_zst)
+{
+    if (dn) n = *dn;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        if (rep) {
+            const int r = rep[c];
+            if (!active[r]) continue;
+            dt = dt_rep[r];
+        }
+        double y[6];
+        stage_coords(cells, ld, n, c, kbuf, tb, st, dt, y);
+        zst[c] = y[2];
+        zst[n_zst + c] = y[5];
+    }
+}
+
 // One RK stage st >= 0 of stepRungeKuttaAdaptive (mechanics.hpp:2534-2557):
 // cells_i = cells + dt * sum_j A(st, j) k_j, normalizeOrientations, then
-// getVelocities with the step's frozen neighbor geometry (cell-cell forces cc
-// are identical in every stage, because they only read the neighbor rows).
+// getVelocities with the step's frozen neighbor geometry (the conservative
+// cell-cell forces cc are identical in every stage, because they only read
+// the neighbor rows; friction rows, when on, are recomputed per stage).
 // Batched replicates: dn (device cell count) replaces n, and rep / dt_rep /
 // active give each cell its replicate's step size; cells of inactive
 // replicates are skipped.
+template <bool CSR>
 __global__ void __launch_bounds__(256)
 k_rk_stage(int n, const int* __restrict__ dn, const double* __restrict__ cells, int ld,
            const int* __restrict__ assume_2d, const double* __restrict__ noise, VelParams vp, Tableau tb, int st,
            double dt, const int* __restrict__ rep, const double* __restrict__ dt_rep,
-           const int* __restrict__ active, const double* __restrict__ cc, double* __restrict__ kbuf,
-           int* __restrict__ status)
+           const int* __restrict__ active, const double* __restrict__ cc, RowLists L, FricGather fg,
+           double* __restrict__ kbuf, int* __restrict__ status)
 {
     if (dn) n = *dn;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
@@ -1241,23 +1434,20 @@ k_rk_stage(int n, const int* __restrict__ dn, const double* __restrict__ cells, 
         }
         CellVel cv = load_cellvel(cells, ld, c);
         if (st > 0) {
-            double y[6] = {cellv(cells, ld, c, C_RX), cellv(cells, ld, c, C_RY), cv.rz,
-                           cv.nx, cv.ny, cv.nz};
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                double mult = 0;
-                for (int j = 0; j < st; ++j) mult += kbuf[((int64_t)j * 6 + q) * n + c] * tb.A[st * tb.s + j];
-                y[q] += mult * dt;
-            }
-            double nrm = norm3(y[3], y[4], y[5]);
-            y[3] /= nrm; y[4] /= nrm; y[5] /= nrm;
-            if (y[5] < 0) { y[3] *= -1; y[4] *= -1; y[5] *= -1; }
+            double y[6];
+            stage_coords(cells, ld, n, c, kbuf, tb, st, dt, y);
             cv.rz = y[2]; cv.nx = y[3]; cv.ny = y[4]; cv.nz = y[5];
         }
         const bool a2d = assume_2d[c] != 0;
         double ccl[6], nz6[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
+        if (fg.frec) {
+            double fr[6];
+            gather_fric<CSR>(c, L, fg.frec, fg.ps, fg.pt, fr);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ccl[q] += fr[q];
+        }
         if (noise) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) nz6[q] = noise[(int64_t)q * n + c];
@@ -1853,6 +2043,32 @@ RowLists row_lists(bfm_ctx* c)
     return L;
 }
 
+bool friction_on(const bfm_ctx* c) { return c->p.friction_mode == 1; }
+
+FricGather fric_gather(const bfm_ctx* c)
+{
+    FricGather fg;
+    fg.frec = friction_on(c) ? c->d_frec : nullptr;
+    fg.ps = c->ps;
+    fg.pt = c->pt;
+    return fg;
+}
+
+// Friction rows of the current list at the cells' own positions (zst ==
+// nullptr) or at stage positions (zst: 2 x n_zst).
+void launch_friction(bfm_ctx* c, const double* d_cells, int ld, const double* zst, int n_zst)
+{
+    FricParams fq;
+    fq.R = c->p.R;
+    fq.Req = c->p.R / 2;
+    fq.E0 = c->p.E0;
+    fq.coulomb = c->p.cell_cell_coulomb_coeff;
+    fq.no_surface = c->p.no_surface;
+    fq.colidx_eta = c->p.colidx_eta_cell_cell;
+    KLAUNCH(c, "friction_rows", k_friction_rows, grid_for(c->max_pairs, 256, 4096), 256, 0, c->d_m, d_cells, ld,
+            n_zst, zst, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt, fq, c->d_frec);
+}
+
 
 
 void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld)
@@ -1873,12 +2089,61 @@ void launch_velocities(bfm_ctx* c, const double* d_cells, int n, int ld, const i
 {
     VelParams vp = vel_params(c);
     RowLists L = row_lists(c);
+    FricGather fg = fric_gather(c);
+    if (fg.frec && d_vel) launch_friction(c, d_cells, ld, nullptr, 0);
     if (c->list_csr)
         KLAUNCH(c, "velocities", k_velocities<true>, grid_for(n, 256), 256, 0, n, d_cells, ld, d_a2d, d_noise,
-                vp, L, c->d_rec, d_vel, d_cc, c->d_status);
+                vp, L, c->d_rec, fg, d_vel, d_cc, c->d_status);
     else
         KLAUNCH(c, "velocities", k_velocities<false>, grid_for(n, 256), 256, 0, n, d_cells, ld, d_a2d, d_noise,
-                vp, L, c->d_rec, d_vel, d_cc, c->d_status);
+                vp, L, c->d_rec, fg, d_vel, d_cc, c->d_status);
+}
+
+// One RK stage's friction rows and velocities (single colony or batched).
+void launch_stage(bfm_ctx* c, int n, const int* d_n, const double* d_cells, int ld, const double* d_noise,
+                  const VelParams& vp, const Tableau& tb, int st, double dt, const int* d_rep, const double* d_dt,
+                  const int* d_active, int grid, bool timed)
+{
+    RowLists L = row_lists(c);
+    FricGather fg = fric_gather(c);
+    if (fg.frec) {
+        const double* zst = nullptr;
+        if (st > 0) {
+            hipLaunchKernelGGL(k_stage_z, dim3(grid), dim3(256), 0, c->stream, n, d_n, d_cells, ld, tb, st, dt,
+                               d_rep, d_dt, d_active, c->d_k, c->d_zst, c->max_cells);
+            HIPCHK(hipGetLastError());
+            zst = c->d_zst;
+        }
+        FricParams fq;
+        fq.R = c->p.R;
+        fq.Req = c->p.R / 2;
+        fq.E0 = c->p.E0;
+        fq.coulomb = c->p.cell_cell_coulomb_coeff;
+        fq.no_surface = c->p.no_surface;
+        fq.colidx_eta = c->p.colidx_eta_cell_cell;
+        hipLaunchKernelGGL(k_friction_rows, dim3(grid_for(c->max_pairs, 256, 4096)), dim3(256), 0, c->stream, c->d_m,
+                           d_cells, ld, c->max_cells, zst, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt, fq,
+                           c->d_frec);
+        HIPCHK(hipGetLastError());
+    }
+    if (timed) {
+        if (c->list_csr)
+            KLAUNCH(c, "rk_stage", k_rk_stage<true>, grid, 256, 0, n, d_n, d_cells, ld, c->d_assume2d, d_noise, vp,
+                    tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg, c->d_k, c->d_status);
+        else
+            KLAUNCH(c, "rk_stage", k_rk_stage<false>, grid, 256, 0, n, d_n, d_cells, ld, c->d_assume2d, d_noise, vp,
+                    tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg, c->d_k, c->d_status);
+    } else {
+        if (c->list_csr)
+            hipLaunchKernelGGL(k_rk_stage<true>, dim3(grid), dim3(256), 0, c->stream, n, d_n, d_cells, ld,
+                               c->d_assume2d, d_noise, vp, tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg,
+                               c->d_k, c->d_status);
+        else
+            hipLaunchKernelGGL(k_rk_stage<false>, dim3(grid), dim3(256), 0, c->stream, n, d_n, d_cells, ld,
+                               c->d_assume2d, d_noise, vp, tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg,
+                               c->d_k, c->d_status);
+        HIPCHK(hipGetLastError());
+    }
 }
 
 void import_host_neighbors(bfm_ctx* c, const double* nb, int64_t m, int64_t ld_nb, int n)
@@ -1940,16 +2205,14 @@ void step_rk_device(bfm_ctx* c, const Tableau& tb, double dt, const double* d_ce
 {
     need_params(c);
     require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
-    require(c->p.friction_mode == 0, BFM_ERR_UNSUPPORTED, "cell-cell friction is not supported");
     compute_forces(c, d_cells, n, ld);
     KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
             c->d_assume2d);
     launch_velocities(c, d_cells, n, ld, c->d_assume2d, d_noise, nullptr, c->d_cc);
     VelParams vp = vel_params(c);
     for (int st = 0; st < tb.s; ++st)
-        KLAUNCH(c, "rk_stage", k_rk_stage, grid_for(n, 256), 256, 0, n, (const int*)nullptr, d_cells, ld,
-                c->d_assume2d, d_noise, vp, tb, st, dt, (const int*)nullptr, (const double*)nullptr,
-                (const int*)nullptr, c->d_cc, c->d_k, c->d_status);
+        launch_stage(c, n, nullptr, d_cells, ld, d_noise, vp, tb, st, dt, nullptr, nullptr, nullptr,
+                     grid_for(n, 256), true);
     KLAUNCH(c, "err_init", k_err_init, 1, 1, 0, c->d_errmax);
     KLAUNCH(c, "rk_final", k_rk_final, grid_for(n, 256), 256, 0, n, c->p.ncols, d_cells, ld, tb, dt, c->d_k,
             d_cells_new, ld_new, d_errors, tol, c->d_errmax);
@@ -1991,7 +2254,6 @@ int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld)
         need_params(c);
         check_cells(c, n, ld);
         require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
-        require(c->p.friction_mode == 0, BFM_ERR_UNSUPPORTED, "cell-cell friction is not supported");
         compute_forces(c, d_cells, n, ld);
         KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
                 c->d_assume2d);
@@ -2009,9 +2271,7 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
         hipLaunchKernelGGL(k_fill_key, dim3(grid_for(n_rep, 256)), dim3(256), 0, c->stream, d_errmax, n_rep,
                            -INFINITY);
         for (int st = 0; st < tb.s; ++st)
-            hipLaunchKernelGGL(k_rk_stage, dim3(kBatchGrid), dim3(256), 0, c->stream, 0, d_n, d_cells, ld,
-                               c->d_assume2d, (const double*)nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active,
-                               c->d_cc, c->d_k, c->d_status);
+            launch_stage(c, 0, d_n, d_cells, ld, nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active, kBatchGrid, false);
         hipLaunchKernelGGL(k_rk_final_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld,
                            tb, d_rep, d_dt, d_active, c->d_k, d_cells_new, tol, d_errmax, d_bad);
         HIPCHK(hipGetLastError());
@@ -2143,7 +2403,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
-                    c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp};
+                    c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     bfm::surrogate_free(c);
@@ -2162,13 +2422,24 @@ int bfm_set_params(bfm_ctx* c, const bfm_params* p)
         require(p != nullptr, BFM_ERR_ARG, "params is NULL");
         require(p->R > 0 && p->Rcell > 0 && p->Rcell <= p->R && p->E0 > 0, BFM_ERR_ARG,
                 "invalid radii or modulus");
-        require(p->friction_mode == 0, BFM_ERR_UNSUPPORTED, "cell-cell friction is not supported");
+        require(p->friction_mode == 0 || p->friction_mode == 1, BFM_ERR_UNSUPPORTED,
+                "friction_mode must be 0 (none) or 1 (kinetic)");
         require(p->adhesion_mode == BFM_ADHESION_NONE || p->adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC,
                 BFM_ERR_UNSUPPORTED, "only NONE and JKR_ISOTROPIC adhesion are supported");
         require(p->jkr_mode >= 0 && p->jkr_mode <= 3, BFM_ERR_ARG, "invalid jkr_mode");
         require(p->ncols >= 21 && p->ncols <= kMaxCols, BFM_ERR_ARG, "ncols must be in 21..32");
         if (p->adhesion_mode != BFM_ADHESION_NONE)
             require(p->colidx_gamma >= 21 && p->colidx_gamma < p->ncols, BFM_ERR_ARG, "invalid colidx_gamma");
+        if (p->friction_mode == 1) {
+            require(p->colidx_eta_cell_cell >= 21 && p->colidx_eta_cell_cell < p->ncols &&
+                        p->colidx_eta_cell_cell != p->colidx_gamma,
+                    BFM_ERR_ARG, "invalid colidx_eta_cell_cell");
+            if (!c->d_frec) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                dalloc(&c->d_frec, (size_t)c->max_pairs * 4);
+                dalloc(&c->d_zst, (size_t)c->max_cells * 2);
+            }
+        }
         c->p = *p;
         c->params_set = true;
     });
@@ -2274,8 +2545,15 @@ int bfm_get_conservative_forces(bfm_ctx* c, const double* cells, int n, int ld, 
         compute_forces(c, c->d_cells, n, c->max_cells);
         launch_velocities(c, c->d_cells, n, c->max_cells, c->d_assume2d, nullptr, nullptr, c->d_cc);
         VelParams vp = vel_params(c);
-        KLAUNCH(c, "conservative", k_conservative, grid_for(n, 256), 256, 0, n, c->d_cells, c->max_cells,
-                c->d_assume2d, vp, c->d_cc, c->d_out6);
+        FricGather fg = fric_gather(c);
+        RowLists L = row_lists(c);
+        if (fg.frec) launch_friction(c, c->d_cells, c->max_cells, nullptr, 0);
+        if (c->list_csr)
+            KLAUNCH(c, "conservative", k_conservative<true>, grid_for(n, 256), 256, 0, n, c->d_cells, c->max_cells,
+                    c->d_assume2d, vp, c->d_cc, L, fg, c->d_out6);
+        else
+            KLAUNCH(c, "conservative", k_conservative<false>, grid_for(n, 256), 256, 0, n, c->d_cells,
+                    c->max_cells, c->d_assume2d, vp, c->d_cc, L, fg, c->d_out6);
         download6(c, c->d_out6, forces, n);
         check_status(c);
     });

@@ -64,14 +64,17 @@ typedef struct bfm_params {
     double E0;                         /* EPS modulus */
     double repulsion_prefactors[3];    /* simulation.hpp:1271-1274 */
     double nz_threshold;
-    double cell_cell_coulomb_coeff;    /* unused: friction mode NONE only */
+    double cell_cell_coulomb_coeff;    /* Coulomb cap of cell-cell friction (<= 0: none) */
     double cell_surface_coulomb_coeff;
     int no_surface;
     int adhesion_mode;                 /* enum bfm_adhesion_mode */
     int jkr_mode;                      /* enum bfm_jkr_mode */
     int colidx_gamma;                  /* gamma column, -1 when adhesion is off */
-    int friction_mode;                 /* must be 0 (FrictionMode::NONE) */
+    int friction_mode;                 /* FrictionMode (mechanics.hpp:65-69): 0 NONE, 1 KINETIC
+                                          (cellCellFrictionForces, mechanics.hpp:1031-1175) */
     int ncols;                         /* number of columns of the cells array */
+    int colidx_eta_cell_cell;          /* friction coefficient column (friction on: 22 with
+                                          adhesion, else 21; simulation.hpp:1185-1188) */
 } bfm_params;
 
 typedef struct bfm_ctx bfm_ctx;
@@ -394,7 +397,7 @@ int bfm_load_config(const char* json_path, bfm_run_config* out);
  * BFM_JKR_EXACT. max_cells: capacity (0: 2 max(n_cells, initial) + 16).
  * verbose prints the reference's per-frame progress line on stdout (and a wall-clock
  * progress line on stderr every 30 s). Modes the
- * device loop lacks (anisotropic JKR, friction, Verlet, basal_only, noise)
+ * device loop lacks (anisotropic JKR, Verlet, basal_only, noise)
  * return BFM_ERR_UNSUPPORTED. stats_out may be NULL. */
 int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int device, int jkr_mode,
             int max_cells, int verbose, bfm_sim_stats* stats_out);

@@ -621,10 +621,68 @@ void oracle_area_integrals(double rz, double nz, double R, double half_l, double
     out[0] = area1; out[1] = area2; out[2] = area3;
 }
 
+/* cellCellFrictionForces, mechanics.hpp:1031-1175 (include_constraint =
+ * false, dz_threshold 1e-3 as passed at :1343-1346 and :1652-1655). dEdq is
+ * n x 6 column-major, ld n, overwritten; rows accumulated in row order. */
+void oracle_friction_forces(const double* cells, int n, int ld, const double* nb, int64_t m,
+                            int64_t ld_nb, const bfm_params* p, double* dEdq)
+{
+    std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
+    const double R = p->R, E0 = p->E0, Req = R / 2;
+    const double coulomb = p->cell_cell_coulomb_coeff;
+    const int ce = p->colidx_eta_cell_cell;
+    for (int64_t k = 0; k < m; ++k) {
+        int i = (int)nb[k], j = (int)nb[ld_nb + k];
+        double dij[3] = {nb[2 * ld_nb + k], nb[3 * ld_nb + k], nb[4 * ld_nb + k]};
+        double dist = norm3(dij[0], dij[1], dij[2]);
+        double overlap = 2 * R - dist;
+        if (!(overlap > 0)) continue;
+        double si = nb[5 * ld_nb + k], sj = nb[6 * ld_nb + k];
+        double dijn[3] = {dij[0] / dist, dij[1] / dist, dij[2] / dist};
+        double eta = std::min(cell(cells, ld, i, ce), cell(cells, ld, j, ce));
+        bool assume_2d = false;
+        if (!p->no_surface && std::fabs(dijn[2]) < 1e-3) {
+            double piz = cell(cells, ld, i, C_RZ) + si * cell(cells, ld, i, C_NZ);
+            double pjz = cell(cells, ld, j, C_RZ) + sj * cell(cells, ld, j, C_NZ);
+            if (piz < 1.01 * R && pjz < 1.01 * R) assume_2d = true;
+        }
+        double rvij[3];
+        for (int c = 0; c < 3; ++c) {
+            double vi = cell(cells, ld, i, C_DRX + c) + si * cell(cells, ld, i, C_DNX + c);
+            double vj = cell(cells, ld, j, C_DRX + c) + sj * cell(cells, ld, j, C_DNX + c);
+            rvij[c] = vi - vj;
+        }
+        if (assume_2d) {
+            dij[2] = 0;
+            dist = norm3(dij[0], dij[1], dij[2]);
+            for (int c = 0; c < 3; ++c) dijn[c] = dij[c] / dist;
+            overlap = 2 * R - dist;
+            rvij[2] = 0;
+        }
+        double dot = (rvij[0] * dijn[0] + rvij[1] * dijn[1]) + rvij[2] * dijn[2];
+        double force[3];
+        double pre = -eta * std::sqrt(Req * overlap);
+        for (int c = 0; c < 3; ++c) force[c] = pre * (rvij[c] - dot * dijn[c]);
+        if (coulomb > 0) {
+            double fn = norm3(force[0], force[1], force[2]);
+            double max_norm = coulomb * (4. / 3.) * E0 * std::sqrt(Req) * std::pow(overlap, 1.5);
+            if (fn > max_norm)
+                for (int c = 0; c < 3; ++c) force[c] *= (max_norm / fn);
+        }
+        for (int c = 0; c < 3; ++c) {
+            dEdq[c * n + i] -= force[c];
+            dEdq[(c + 3) * n + i] -= si * force[c];
+            dEdq[c * n + j] += force[c];
+            dEdq[(c + 3) * n + j] -= (-sj) * force[c];
+        }
+    }
+}
+
 /* cellCellInteractionForces: without adhesion mechanics.hpp:1209-1350, with
- * adhesion :1390-1659 (include_constraint = false, friction NONE). dEdq is
- * n x 6 column-major, ld n, and is overwritten. Rows are accumulated in the
- * order of the neighbor rows, as the reference does. */
+ * adhesion :1390-1659 (include_constraint = false), plus the friction array
+ * when friction_mode is KINETIC (:1341-1347, :1650-1656). dEdq is n x 6
+ * column-major, ld n, and is overwritten. Rows are accumulated in the order
+ * of the neighbor rows, as the reference does. */
 void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* nb,
                              int64_t m, int64_t ld_nb, const bfm_params* p, double* dEdq)
 {
@@ -705,6 +763,11 @@ void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* n
                 dEdq[(c + 3) * n + j] += -sj * wh[c];
             }
         }
+    }
+    if (p->friction_mode == 1) {
+        std::vector<double> fr(6 * (size_t)n);
+        oracle_friction_forces(cells, n, ld, nb, m, ld_nb, p, fr.data());
+        for (size_t q = 0; q < 6 * (size_t)n; ++q) dEdq[q] += fr[q];
     }
 }
 
@@ -804,7 +867,7 @@ void oracle_conservative_forces(const double* cells, int n, int ld, const double
     for (size_t q = 0; q < 6 * (size_t)n; ++q) forces[q] = (-cc[q] - rep[q]) - adh[q];
 }
 
-/* getVelocities, mechanics.hpp:2011-2347 (friction NONE). */
+/* getVelocities, mechanics.hpp:2011-2347. */
 void oracle_get_velocities(const double* cells, int n, int ld, const double* nb, int64_t m,
                            int64_t ld_nb, const int* assume_2d, const double* noise,
                            const bfm_params* p, double* vel)

@@ -32,6 +32,8 @@ double oracle_nearest_cell_body_coord(const double* r, const double* n, double h
 void oracle_dist_between_cells(const double* r1, const double* n1, double half_l1,
                                const double* r2, const double* n2, double half_l2,
                                double* d, double* s, double* t);
+void oracle_friction_forces(const double* cells, int n, int ld, const double* nb, int64_t m,
+                            int64_t ld_nb, const bfm_params* p, double* dEdq);
 int64_t oracle_get_cell_neighbors(const double* cells, int n, int ld, double thr,
                                   double R, double Ldiv, double* out, int64_t ld_out);
 int64_t oracle_get_cell_neighbors_binned(const double* cells, int n, int ld, double thr,

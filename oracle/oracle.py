@@ -66,6 +66,7 @@ def lib():
         L.oracle_area_integrals.restype = None
         vp = C.c_void_p
         L.oracle_cell_cell_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, vp, _dp]
+        L.oracle_friction_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, vp, _dp]
         L.oracle_surface_repulsion_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, d, d, _ip, _dp]
         L.oracle_surface_adhesion_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, d, _ip, _dp]
         L.oracle_conservative_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, _ip, vp,
@@ -214,6 +215,16 @@ def cell_cell_forces(cells, nb, params):
     out = np.zeros((n, 6), order="F")
     lib().oracle_cell_cell_forces(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
                                   C.byref(params), _p(out))
+    return out
+
+
+def friction_forces(cells, nb, params):
+    """cellCellFrictionForces (mechanics.hpp:1031-1175): dE/dq n x 6."""
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_friction_forces(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                                 C.byref(params), _p(out))
     return out
 
 
