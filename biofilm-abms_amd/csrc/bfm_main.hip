@@ -567,6 +567,209 @@ k_candidates_lds(int n, const Plan* __restrict__ plan, const double* __restrict_
     }
 }
 
+// Wavefront-per-bin candidate search (no work-group barriers; the four
+// wavefronts of a work-group are independent). A wavefront takes one home bin
+// at a time: it stages up to 64 of the bin's cells (one per lane) in its own
+// LDS area and sweeps the bin's half stencil 64 candidates at a time, one per
+// lane. Every lane screens its candidate against all homes of the chunk (FP32,
+// broadcast LDS reads, a trip count uniform across the wavefront); survivors
+// go to a wavefront-private LDS queue whose slices of 64 are tested with the
+// FP64 segment kernel, one pair per lane. The pair rule is that of
+// k_candidates_lds: a pair is screened once, by the cell later in bin order
+// (candidates in the stencil rows before the home row, then in the home row
+// up to the home bin, at sorted positions before the home). Bins are split
+// into eight contiguous ranges, one per XCD (work-group b runs on XCD b % 8),
+// and the wavefronts of an XCD stride through its range.
+template <int QCAP>
+__global__ void __launch_bounds__(256)
+k_candidates_bins(const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
+                  const int* __restrict__ sid, const RepGrid* __restrict__ grid, int n_grids,
+                  const int* __restrict__ bin_off, int slot_cap, unsigned long long* __restrict__ cnt64,
+                  int* __restrict__ slot_j, double* __restrict__ slot_rec, int* __restrict__ jslot,
+                  unsigned long long* __restrict__ counters)
+{
+    constexpr int WAVES = 4;
+    static_assert(QCAP >= 128 && QCAP % 64 == 0, "queue holds at least two slices");
+    __shared__ float4 s_hf[WAVES][64];          // FP32 (x, y, z, thr + hl + margin) of the homes
+    __shared__ double s_h[WAVES][7][64];        // FP64 geometry of the homes
+    __shared__ int s_hid[WAVES][64];
+    __shared__ int s_queue[WAVES][QCAP];        // (candidate sorted position << 6) | home
+    __shared__ int s_seg[WAVES][2][64];         // segment start, inclusive prefix of lengths
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const Plan P = *plan;
+    const double thr = P.thr, pre = P.pre;
+    const float fmargin = (float)P.fmargin;
+    const int reach = P.reach, side = 2 * reach + 1, nseg = reach * side + reach + 1;
+    const int nb = P.nbins;
+    const int xcd = blockIdx.x & 7, kx = blockIdx.x >> 3;
+    const int blocks_x = (gridDim.x - xcd + 7) >> 3;          // blocks that run on this XCD
+    const int waves_x = blocks_x * WAVES;
+    const int r_lo = (int)((int64_t)nb * xcd / 8), r_hi = (int)((int64_t)nb * (xcd + 1) / 8);
+    float4* hf = s_hf[wave];
+    int* queue = s_queue[wave];
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long screened = 0, tested = 0;
+    int qn = 0;   // queue fill, uniform across the wavefront
+
+    // tests entries [0, qn) of the queue, 64 per round, then empties it
+    auto drain = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int e0 = 0; e0 < qn; e0 += 64) {
+            const int e = e0 + lane;
+            if (e < qn) {
+                const int ent = queue[e];
+                const int h = ent & 63, q = ent >> 6;
+                Seg ch, cc;
+                ch.r[0] = s_h[wave][0][h]; ch.r[1] = s_h[wave][1][h]; ch.r[2] = s_h[wave][2][h];
+                ch.n[0] = s_h[wave][3][h]; ch.n[1] = s_h[wave][4][h]; ch.n[2] = s_h[wave][5][h];
+                ch.hl = s_h[wave][6][h];
+                cc = load_sorted(sorted, stride, q);
+                const int hid = s_hid[wave][h], cid = sid[q];
+                const bool home_is_i = hid > cid;
+                const int i = home_is_i ? hid : cid;
+                const int j = home_is_i ? cid : hid;
+                const double dx = ch.r[0] - cc.r[0], dy = ch.r[1] - cc.r[1], dz = ch.r[2] - cc.r[2];
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                const double bound = (thr + ch.hl + cc.hl) * (1 + 1e-9) + 1e-9;
+                if (r2 <= bound * bound && sqrt(r2) < pre) {
+                    double d[3], sv, tv;
+                    if (home_is_i) dist_between_cells(ch, cc, d, sv, tv);
+                    else dist_between_cells(cc, ch, d, sv, tv);
+                    if (norm3(d[0], d[1], d[2]) < thr) {
+                        const int slot = (int)(atomicAdd(&cnt64[i], 1ull << 32) >> 32);
+                        const int slot2 = (int)(atomicAdd(&cnt64[j], 1ull) & 0xffffffffull);
+                        if (slot < slot_cap) {
+                            const int64_t es = (int64_t)slot * stride + i;
+                            slot_j[es] = j;
+                            double2* rr = reinterpret_cast<double2*>(slot_rec + 8 * es);
+                            rr[0] = make_double2(d[0], d[1]);
+                            rr[1] = make_double2(d[2], sv);
+                            rr[2] = make_double2(tv, __longlong_as_double((long long)slot2));
+                        }
+                        if (slot2 < slot_cap) jslot[(int64_t)slot2 * stride + j] = i;
+                    }
+                }
+            }
+        }
+        tested += (unsigned long long)qn;
+        qn = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    for (int b = r_lo + kx * WAVES + wave; b < r_hi; b += waves_x) {
+        const int hb0 = bin_off[b], hb1 = bin_off[b + 1];
+        if (hb0 >= hb1) continue;
+        // the previous bin's LDS reads are done before this bin's writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int r = 0;
+        if (n_grids > 1) {   // the replicate whose bins contain b
+            int lo = 0, hi = n_grids - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (grid[mid].base <= b) lo = mid; else hi = mid - 1;
+            }
+            r = lo;
+        }
+        const int gbase = grid[r].base, gd0 = grid[r].dims[0], gd1 = grid[r].dims[1], gd2 = grid[r].dims[2];
+        const int local = b - gbase;
+        const int bx = local % gd0, rrow = local / gd0, by = rrow % gd1, bz = rrow / gd1;
+        // the half stencil as contiguous runs of sorted positions: rows (dz, dy)
+        // before (0, 0) over [bx - reach, bx + reach], then the home row up to bx
+        int lo = 0, len = 0;
+        if (lane < nseg) {
+            const int dz = lane / side - reach, dy = lane % side - reach;
+            const int zz = bz + dz, yy = by + dy;
+            if (zz >= 0 && zz < gd2 && yy >= 0 && yy < gd1) {
+                const int rb = gbase + (zz * gd1 + yy) * gd0;
+                const int xlo = max(0, bx - reach);
+                const int xhi = (dz == 0 && dy == 0) ? bx : min(gd0 - 1, bx + reach);
+                lo = bin_off[rb + xlo];
+                len = bin_off[rb + xhi + 1] - lo;
+            }
+        }
+        int incl = len;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        s_seg[wave][0][lane] = lo;
+        s_seg[wave][1][lane] = incl;
+        const int S = __shfl(incl, nseg - 1);
+        for (int h0 = hb0; h0 < hb1; h0 += 64) {
+            const int H = min(64, hb1 - h0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < H) {
+                const int hs = h0 + lane;
+                double g7[7];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+                    g7[k] = sorted[k * stride + hs];
+                    s_h[wave][k][lane] = g7[k];
+                }
+                s_hid[wave][lane] = sid[hs];
+                hf[lane] = make_float4((float)g7[0], (float)g7[1], (float)g7[2], (float)(thr + g7[6] + P.fmargin));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int qlast = h0 + H - 1;   // a candidate pairs only with homes after it
+            for (int c0 = 0; c0 < S; c0 += 64) {
+                const int g = c0 + lane;
+                bool have = g < S;
+                int q = 0;
+                float4 cf = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (have) {
+                    int sg = 0;
+                    while (s_seg[wave][1][sg] <= g) ++sg;
+                    q = s_seg[wave][0][sg] + (g - (sg ? s_seg[wave][1][sg - 1] : 0));
+                    have = q < qlast;
+                    if (have)
+                        cf = make_float4((float)sorted[q], (float)sorted[stride + q], (float)sorted[2 * stride + q],
+                                         (float)sorted[6 * stride + q]);
+                }
+                if (!__ballot(have)) continue;
+                // homes at or before the candidate never pair with it: start at
+                // the first home after the lowest candidate of the wavefront
+                for (int h = 0; h < H; ++h) {
+                    bool pass = false;
+                    if (have && q < h0 + h) {
+                        ++screened;
+                        const float4 hh = hf[h];
+                        const float fx = hh.x - cf.x, fy = hh.y - cf.y, fz = hh.z - cf.z;
+                        const float fb = hh.w + cf.w;
+                        pass = fx * fx + fy * fy + fz * fz <= fb * fb;
+                    }
+                    const uint64_t m = __ballot(pass);
+                    if (m) {
+                        if (pass) queue[qn + __popcll(m & below)] = (q << 6) | h;
+                        qn += __popcll(m);
+                        if (qn > QCAP - 64) drain();
+                    }
+                }
+            }
+            if (qn) drain();
+        }
+    }
+    (void)fmargin;
+    // one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) {
+        screened += __shfl_xor(screened, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&counters[0], screened);
+        atomicAdd(&counters[1], tested);
+    }
+}
+
 // The candidate search of k_candidates_lds with the screen turned around: a
 // work-group owns 64 consecutive bin-ordered homes, lane l of every wave
 // holds home l (FP32 position and bound in registers), and each wave takes
@@ -1915,6 +2118,11 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
             c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_jslot,        \
             c->d_counters)
     switch (c->cand_variant) {
+        case 7:
+            KLAUNCH(c, "candidates", k_candidates_bins<256>, 2048, 256, 0, c->d_plan, c->d_sorted, stride, c->d_sid,
+                    c->d_grid, std::max(c->n_rep, 1), c->d_bin_off, c->slot_cap, c->d_cnt64, c->d_slot_j,
+                    c->d_slot_rec, c->d_jslot, c->d_counters);
+            break;
         case 4: BFM_CANDW(512, 8, 4096); break;
         case 5: BFM_CANDW(1024, 8, 8192); break;
         case 6: BFM_CAND(128, 512, 8, 4096); break;
@@ -2707,7 +2915,7 @@ int bfm_autotune_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, dou
         need_params(c);
         check_cells(c, n, ld);
         static const double scales[] = {0.5, 0.75, 1.0, 1.25};
-        static const int variants[] = {0, 1, 2, 3, 4, 5, 6};
+        static const int variants[] = {0, 1, 2, 3, 4, 5, 6, 7};
         double best = INFINITY, bs = c->bin_scale;
         int bv = c->cand_variant;
         hipEvent_t e0, e1;
@@ -2800,7 +3008,9 @@ int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads, int kernel_v
 {
     return guarded(c, [&] {
         if (kernel_variant >= 0) {
-            require(kernel_variant <= 6, BFM_ERR_ARG, "kernel_variant must be 0..6");
+            require(kernel_variant <= 7, BFM_ERR_ARG, "kernel_variant must be 0..7");
+            require(kernel_variant != 7 || c->max_cells < (1 << 25), BFM_ERR_ARG,
+                    "kernel_variant 7 needs max_cells < 2^25");
             c->cand_variant = kernel_variant;
         }
         if (bin_scale > 0) {
