@@ -95,6 +95,7 @@ struct bfm_ctx {
     int* d_keys = nullptr; int* d_keys_alt = nullptr;
     int* d_vals = nullptr; int* d_vals_alt = nullptr;
     int* d_bin_count = nullptr;     // max_bins + 1
+    bool bins_dirty = true;         // histogram not known to be zero (see build_neighbors)
     int* d_bin_off = nullptr;       // max_bins + 1
     double* d_sorted = nullptr;     // 7 x max_cells SoA: x y z nx ny nz hl
     int* d_sid = nullptr;           // sorted -> original index

@@ -293,12 +293,19 @@ __global__ void k_keys(const double* __restrict__ cells, int n, int ld, const Pl
 // order inside a bin depends on atomic timing, but nothing downstream does:
 // every pair is tested from the larger original index, and each cell's list
 // is sorted by partner index during compaction.
+// It also clears the per-cell pair counters (n + 1) that the candidate kernel
+// increments next, and the histogram entries of the occupied bins (every
+// nonzero entry has a cell), which leaves the histogram zero for the next build.
 __global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const int* __restrict__ keys,
                           const int* __restrict__ rank, const int* __restrict__ bin_off,
-                          double* __restrict__ sorted, int* __restrict__ sid, int* __restrict__ skey, int64_t stride)
+                          double* __restrict__ sorted, int* __restrict__ sid, int* __restrict__ skey, int64_t stride,
+                          unsigned long long* __restrict__ cnt64, int* __restrict__ hist)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt64[n] = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        cnt64[i] = 0;
         const int key = keys[i];
+        hist[key] = 0;
         const int q = bin_off[key] + rank[i];
         sid[q] = i;
         skey[q] = key;
@@ -2163,8 +2170,13 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
         KLAUNCH(c, "plan", k_plan, 1, 256, 0, c->d_partials, nparts, thr, pre, c->bin_scale, c->max_bins,
                 c->d_plan, c->d_grid);
     }
-    KLAUNCH(c, "fill", k_fill_int, grid_for(c->max_bins + 1, 256, 4096), 256, 0, c->d_bin_count, 0,
-            (int64_t)c->max_bins + 1);
+    // k_scatter clears the histogram entries it used, so the histogram is
+    // zero for the next build; a fill is needed only for the first build or
+    // after one that did not reach the scatter
+    if (c->bins_dirty)
+        KLAUNCH(c, "fill", k_fill_int, grid_for(c->max_bins + 1, 256, 4096), 256, 0, c->d_bin_count, 0,
+                (int64_t)c->max_bins + 1);
+    c->bins_dirty = true;
     if (c->n_rep > 0)
         KLAUNCH(c, "keys", k_keys_rep, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_rep, c->d_grid,
                 c->d_keys, c->d_vals, c->d_bin_count);
@@ -2174,8 +2186,8 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     scan(c, c->d_bin_count, c->d_bin_off, (int64_t)c->max_bins + 1, "scan_bins");
     const int64_t stride = c->max_cells;
     KLAUNCH(c, "scatter", k_scatter, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals, c->d_bin_off,
-            c->d_sorted, c->d_sid, c->d_skey, stride);
-    KLAUNCH(c, "fill", k_fill_u64, grid_for(n + 1, 256, 4096), 256, 0, c->d_cnt64, (int64_t)n + 1);
+            c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count);
+    c->bins_dirty = false;
     launch_candidates(c, n, stride);
     {
         LaunchScope ls(c, "scan_counts");
