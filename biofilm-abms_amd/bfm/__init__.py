@@ -10,6 +10,6 @@ fallback, and importing the library fails loudly when it is missing.
 from .core import (  # noqa: F401
     BFMError, Context, Params, load_library, library_path, tableau, TABLEAUX,
     getCellNeighbors, updateNeighborDistances, getConservativeForces, getVelocities,
-    stepRungeKuttaAdaptive, repulsion_prefactors, COL,
+    stepRungeKuttaAdaptive, stepVelocityVerlet, repulsion_prefactors, COL,
 )
 from .workloads import make_dome, make_ensemble, ensemble_sizes  # noqa: F401

@@ -172,6 +172,9 @@ def load_library():
                                   _dp, vp, _ip, vp],
         "bfm_autotune_neighbors": [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
                                    _ip, _dp],
+        "bfm_step_verlet": [vp, C.c_double, C.c_double, _dp, C.c_int, C.c_int, _dp, C.c_int64, C.c_int64,
+                            _dp, _dp],
+        "bfm_dev_step_verlet": [vp, C.c_double, C.c_double, vp, C.c_int, C.c_int, vp, vp, C.c_int],
         "bfm_write_cells": [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_char_p),
                             C.POINTER(C.c_char_p), C.c_int, _ip, _ip, C.c_int],
         "bfm_read_cells": [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int, _ip, _ip, C.c_char_p, C.c_int,
@@ -309,6 +312,21 @@ class Context:
             nb.shape[0], max(nb.shape[0], 1), _ptr(nz), _ptr(out), _ptr(err)))
         return out, err
 
+    def step_verlet(self, dt, rho, cells, nb, noise=None):
+        """stepVelocityVerlet (mechanics.hpp:2668-2996); returns the new cells."""
+        cells, nb = _f64(cells), _f64(nb)
+        n = cells.shape[0]
+        nz = None if noise is None else _f64(noise)
+        out = np.zeros((n, cells.shape[1]), order="F")
+        self._check(self.lib.bfm_step_verlet(self.h, dt, rho, _ptr(cells), n, n, _ptr(nb), nb.shape[0],
+                                             max(nb.shape[0], 1), _ptr(nz), _ptr(out)))
+        return out
+
+    def dev_step_verlet(self, dt, rho, d_cells, n, ld, d_new, ld_new, d_noise=None):
+        self._check(self.lib.bfm_dev_step_verlet(self.h, dt, rho, C.c_void_p(d_cells), n, ld,
+                                                 C.c_void_p(d_noise) if d_noise else None,
+                                                 C.c_void_p(d_new), ld_new))
+
     # ---- device-resident entry points (device pointers as ints) ----------
     def dev_build_neighbors(self, d_cells, n, ld, thr, R, Ldiv, want_m=True):
         m = C.c_int64()
@@ -403,6 +421,11 @@ def getConservativeForces(cells, neighbors, assume_2d, params):
 def getVelocities(cells, neighbors, assume_2d, noise, params):
     """mechanics.hpp:2011-2347."""
     return _ctx_for(np.shape(cells)[0], params).get_velocities(cells, neighbors, assume_2d, noise)
+
+
+def stepVelocityVerlet(cells, neighbors, dt, rho, params, noise=None):
+    """mechanics.hpp:2668-2996; returns cells_new. noise is drawn by the caller."""
+    return _ctx_for(np.shape(cells)[0], params).step_verlet(dt, rho, cells, neighbors, noise)
 
 
 def stepRungeKuttaAdaptive(A, b, bs, cells, neighbors, dt, params, noise=None):

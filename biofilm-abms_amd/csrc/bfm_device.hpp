@@ -416,6 +416,51 @@ __device__ __forceinline__ void surface_forces(const SurfaceConsts& k, bool assu
     adh5 = g5;
 }
 
+// cellSurfaceRepulsionForces / cellSurfaceAdhesionForces with
+// include_constraint = true (mechanics.hpp:741-754, 868-884): dE/drz and the
+// constrained torques about (n_y, -n_x, 0), entries 2, 3, 4 of rep[] / adh[].
+__device__ __forceinline__ void surface_forces_ic(const SurfaceConsts& k, bool assume_2d, double rz, double nx,
+                                                  double ny, double nz, double hl, double l, double sigma0,
+                                                  double ss, double rep[6], double adh[6])
+{
+    const double R = k.R;
+    if (assume_2d) {
+        double ph = R - rz;
+        if (ph > 0) {
+            rep[2] = -k.rep0 * ph * l;
+            adh[2] = sigma0 * k.adh0 * l / sqrt(ph);
+        }
+        return;
+    }
+    const double nz2 = nz * nz;
+    const double cx = ny, cy = -nx;
+    const double int1 = integral1<4>(rz, nz, R, hl, ss);
+    const double int2 = integral1<3>(rz, nz, R, hl, ss);
+    rep[2] = -k.rep0 * ((1.0 - nz2) * int1 + k.sqR * nz2 * int2);
+    const double i3 = integral2<4>(rz, nz, R, hl, ss);
+    const double i4 = integral2<3>(rz, nz, R, hl, ss);
+    rep[3] = -k.rep0 * cx * ((1.0 - nz2) * i3 + k.sqR * nz2 * i4);
+    rep[4] = -k.rep0 * cy * ((1.0 - nz2) * i3 + k.sqR * nz2 * i4);
+    const double a1 = integral1<1>(rz, nz, R, hl, ss);
+    double term2 = 0.0;
+    if (fabs(ss) < hl) term2 = (k.adh1 / 2) * nz;
+    double g2 = k.adh0 * (1 - nz2) * a1 + term2;
+    g2 *= sigma0;
+    adh[2] = g2;
+    const double a2 = integral2<1>(rz, nz, R, hl, ss);
+    double w1 = 0, w2 = 0;
+    if (fabs(ss) < hl) {
+        w1 = (k.adh1 / 2) * ss * nz * cx;
+        w2 = (k.adh1 / 2) * ss * nz * cy;
+    }
+    double g3 = k.adh0 * (1 - nz2) * a2 * cx - w1;
+    g3 *= sigma0;
+    double g4 = k.adh0 * (1 - nz2) * a2 * cy - w2;
+    g4 *= sigma0;
+    adh[3] = g3;
+    adh[4] = g4;
+}
+
 struct CellVel {
     double rz, nx, ny, nz, l, hl, eta0, eta1;
     double drx, dry, dnx, dny;   // stored velocities (Coulomb truncation)

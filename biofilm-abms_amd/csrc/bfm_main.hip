@@ -1204,7 +1204,7 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
         const double dx = pdx[k], dy = pdy[k], dz = pdz[k];
         const double dist = norm3(dx, dy, dz);
         const double overlap = 2 * fp.R - dist;
-        double mm = 0, mh = 0;
+        double mm = 0, mh = 0, jkr = 0;
         if (overlap > 0) {
             if (!fp.adhesion) {
                 if (overlap <= fp.max_overlap) {
@@ -1230,6 +1230,7 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
                         radius = jkr_radius_newton(delta, fp.Req, fp.E0, gamma, 0.0, 200, nullptr);
                     if (!(radius > 0)) atomicOr(status, ST_JKR_ROOT);
                     mm = jkr_magnitude(radius, fp.Req, fp.E0, gamma);
+                    jkr = 1.0;
                 } else {
                     if (overlap <= fp.max_overlap) mm = fp.pref0 * pow_half<3>(overlap);
                     else mm = fp.pref2;
@@ -1242,7 +1243,7 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
         r[0] = make_double2(dx, dy);
         r[1] = make_double2(dz, ps[k]);
         r[2] = make_double2(pt[k], mm);
-        r[3] = make_double2(mh, 0.0);
+        r[3] = make_double2(mh, jkr);
     }
 }
 
@@ -1472,6 +1473,309 @@ struct FricGather {
     const double* ps;
     const double* pt;
 };
+
+// ---- include_constraint = true (velocity Verlet) ---------------------------
+
+// (s n) x v, the torque form of the constrained forces (mechanics.hpp:1300-1305).
+__device__ __forceinline__ void cross_sn(double s, const double nc[3], const double v[3], double out[3])
+{
+    const double a[3] = {s * nc[0], s * nc[1], s * nc[2]};
+    out[0] = a[1] * v[2] - a[2] * v[1];
+    out[1] = a[2] * v[0] - a[0] * v[2];
+    out[2] = a[0] * v[1] - a[1] * v[0];
+}
+
+// One row's constrained contribution to dE/dq of the cell with orientation
+// nc: the JKR term (record flag) as s (-(n . w) n + w) / t ((n . w) n - w)
+// (adhesion.hpp:225-231, v = -w), the Hertz, EPS-cap and hard-core terms as
+// (s n) x (+-w) (mechanics.hpp:1298-1306, 1554-1562, 1608-1616).
+__device__ __forceinline__ void add_row_ic(const double* __restrict__ rec, int k, bool role_i, double R,
+                                           const double nc[3], double acc[6])
+{
+    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)k * 8);
+    const double2 a = r[0], b = r[1], c = r[2], d = r[3];
+    const double dx = a.x, dy = a.y, dz = b.x;
+    const double dist = norm3(dx, dy, dz);
+    const double overlap = 2 * R - dist;
+    if (!(overlap > 0)) return;
+    const double dn[3] = {dx / dist, dy / dist, dz / dist};
+    const double s = role_i ? b.y : c.x;
+    double w[3], mw[3], t[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        w[q] = c.y * dn[q];
+        mw[q] = -w[q];
+        acc[q] += role_i ? w[q] : mw[q];
+    }
+    if (d.y != 0) {
+        const double wd = (nc[0] * w[0] + nc[1] * w[1]) + nc[2] * w[2];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) t[q] = role_i ? s * ((-wd) * nc[q] + w[q]) : s * (wd * nc[q] - w[q]);
+    } else {
+        cross_sn(s, nc, role_i ? w : mw, t);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q + 3] += t[q];
+    if (d.x != 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            w[q] = d.x * dn[q];
+            mw[q] = -w[q];
+            acc[q] += role_i ? w[q] : mw[q];
+        }
+        cross_sn(s, nc, role_i ? w : mw, t);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q + 3] += t[q];
+    }
+}
+
+// Constrained friction of a cell (cellCellFrictionForces with
+// include_constraint = true, mechanics.hpp:1152-1160): row i: -f, -(s_i n_i) x f;
+// row j: +f, -(s_j n_j) x (-f).
+template <bool CSR>
+__device__ __forceinline__ void gather_fric_ic(int c, const RowLists& L, const double* __restrict__ frec,
+                                               const double* __restrict__ ps, const double* __restrict__ pt,
+                                               const double nc[3], double acc[6])
+{
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] = 0;
+    for_rows<CSR>(c, L, [&](int row, bool role_i) {
+        const double2* r = reinterpret_cast<const double2*>(frec + (int64_t)row * 4);
+        const double2 a = r[0], b = r[1];
+        if (b.y == 0) return;
+        const double f[3] = {a.x, a.y, b.x}, mf[3] = {-a.x, -a.y, -b.x};
+        double t[3];
+        cross_sn(role_i ? ps[row] : pt[row], nc, role_i ? f : mf, t);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            acc[q] += role_i ? mf[q] : f[q];
+            acc[q + 3] -= t[q];
+        }
+    });
+}
+
+struct VerletParams {
+    VelParams vp;
+    double rho, dt;
+    double R3;          // R^3 as the host's pow(R, 3)
+    int ncols;
+};
+
+// The generalized forces of one cell in stepVelocityVerlet (mechanics.hpp:
+// 2845-2872): getConservativeForces with include_constraint = true, plus
+// noise, minus ambient viscosity (:1785-1806) and minus the cell-surface
+// friction (:1822-1900) truncated to cs_coulomb times the norm of the
+// constrained surface repulsion.
+template <bool CSR>
+__global__ void __launch_bounds__(256)
+k_verlet_forces(int n, const double* __restrict__ cells, int ld, const double* __restrict__ noise, VerletParams P,
+                RowLists L, const double* __restrict__ rec, FricGather fg, double* __restrict__ F)
+{
+    const VelParams& vp = P.vp;
+    const double R = vp.sc.R;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const double nc[3] = {cellv(cells, ld, c, C_NX), cellv(cells, ld, c, C_NY), cellv(cells, ld, c, C_NZ)};
+        const double rz = cellv(cells, ld, c, C_RZ), hl = cellv(cells, ld, c, C_HALF_L), l = cellv(cells, ld, c, C_L);
+        double cc[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) cc[q] = 0;
+        for_rows<CSR>(c, L, [&](int row, bool role_i) { add_row_ic(rec, row, role_i, R, nc, cc); });
+        if (fg.frec) {
+            double fr[6];
+            gather_fric_ic<CSR>(c, L, fg.frec, fg.ps, fg.pt, nc, fr);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cc[q] += fr[q];
+        }
+        const bool a2d = nc[2] < vp.nz_threshold;
+        const double ss = a2d ? NAN : (R - rz) / nc[2];
+        double rep[6] = {0, 0, 0, 0, 0, 0}, adh[6] = {0, 0, 0, 0, 0, 0};
+        if (!vp.no_surface)
+            surface_forces_ic(vp.sc, a2d, rz, nc[0], nc[1], nc[2], hl, l, cellv(cells, ld, c, C_SIGMA0), ss, rep,
+                              adh);
+        double f[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[q] = (-cc[q] - rep[q]) - adh[q];
+        if (noise) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) f[q] += noise[(int64_t)q * n + c];
+        }
+        const double dr[3] = {cellv(cells, ld, c, C_DRX), cellv(cells, ld, c, C_DRY), cellv(cells, ld, c, C_DRZ)};
+        const double dnv[3] = {cellv(cells, ld, c, C_DNX), cellv(cells, ld, c, C_DNY), cellv(cells, ld, c, C_DNZ)};
+        const double w[3] = {nc[1] * dnv[2] - nc[2] * dnv[1], nc[2] * dnv[0] - nc[0] * dnv[2],
+                             nc[0] * dnv[1] - nc[1] * dnv[0]};
+        {   // ambient viscosity, as written in the reference (l^3 and l^9)
+            const double eta = cellv(cells, ld, c, C_ETA0);
+            const double li = l * l * l;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                f[q] -= eta * li * dr[q];
+                f[q + 3] -= eta * li * li * li * w[q] / 12;
+            }
+        }
+        if (!vp.no_surface) {
+            double sf[6] = {0, 0, 0, 0, 0, 0};
+            const double eta1 = cellv(cells, ld, c, C_ETA1);
+            const double qv[3] = {dr[0], dr[1], 0.0};
+            const double mv[3] = {w[1] * nc[2] - w[2] * nc[1], w[2] * nc[0] - w[0] * nc[2], 0.0};
+            if (a2d) {
+                const double phi = R - rz;
+                if (phi > 0) {
+                    const double area = vp.sc.sqR * sqrt(phi) * l;
+                    sf[0] = eta1 * qv[0] * area / R;
+                    sf[1] = eta1 * qv[1] * area / R;
+                    const double cross_z = nc[0] * mv[1] - nc[1] * mv[0];
+                    const double l2 = l * l;
+                    sf[5] = eta1 * cross_z * area * l2 / (12 * R);
+                }
+            } else {
+                double ai[3];
+                area_integrals(rz, nc[2], R, hl, ss, vp.sc.sqR, ai);
+                sf[0] = eta1 * (qv[0] * ai[0] + mv[0] * ai[1]) / R;
+                sf[1] = eta1 * (qv[1] * ai[0] + mv[1] * ai[1]) / R;
+                const double c1[3] = {nc[1] * qv[2] - nc[2] * qv[1], nc[2] * qv[0] - nc[0] * qv[2],
+                                      nc[0] * qv[1] - nc[1] * qv[0]};
+                const double c2[3] = {nc[1] * mv[2] - nc[2] * mv[1], nc[2] * mv[0] - nc[0] * mv[2],
+                                      nc[0] * mv[1] - nc[1] * mv[0]};
+#pragma unroll
+                for (int q = 0; q < 3; ++q) sf[3 + q] = eta1 * (c1[q] * ai[1] + c2[q] * ai[2]) / R;
+            }
+            if (vp.coeff > 0) {
+                double fn2 = 0, rn2 = 0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    fn2 += sf[q] * sf[q];
+                    rn2 += rep[q] * rep[q];
+                }
+                const double fn = sqrt(fn2), mx = vp.coeff * sqrt(rn2);
+                if (fn > mx) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) sf[q] *= (mx / fn);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) f[q] -= sf[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) F[(int64_t)q * n + c] = f[q];
+    }
+}
+
+// Mass and the two moments of inertia of a spherocylinder of length l and
+// density rho (mechanics.hpp:2800-2822).
+__device__ __forceinline__ void verlet_inertia(double R, double R3, double rho, double l, double& mass,
+                                               double& I0, double& I1)
+{
+    const double volume = kPi * ((4. / 3.) * R3 + R * R * l);
+    mass = rho * volume;
+    const double diam = 2 * R, d2 = diam * diam, d3 = d2 * diam, d4 = d3 * diam;
+    const double l2 = l * l, l3 = l2 * l;
+    I0 = d4 * (l / 32 + diam / 60);
+    I1 = d2 * (4 * d3 + 15 * l * d2 + 20 * l2 * diam + 10 * l3) / 480;
+    I0 *= (rho * kPi);
+    I1 *= (rho * kPi);
+}
+
+// delta1 = F / m, delta2 = (t . n) n / I0 + (t - (t . n) n) / I1.
+__device__ __forceinline__ void verlet_accel(const double* __restrict__ F, int n, int c, const double nc[3],
+                                             double mass, double I0, double I1, double d1[3], double d2[3])
+{
+    const double t[3] = {F[(int64_t)3 * n + c], F[(int64_t)4 * n + c], F[(int64_t)5 * n + c]};
+    const double dot = (t[0] * nc[0] + t[1] * nc[1]) + t[2] * nc[2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double par = dot * nc[q];
+        const double perp = t[q] - par;
+        d1[q] = F[(int64_t)q * n + c] / mass;
+        d2[q] = par / I0;
+        d2[q] += perp / I1;
+    }
+}
+
+__device__ __forceinline__ void normalize_orientation(double nv[3])
+{
+    const double nrm = norm3(nv[0], nv[1], nv[2]);
+    nv[0] /= nrm; nv[1] /= nrm; nv[2] /= nrm;
+    if (nv[2] < 0) { nv[0] *= -1; nv[1] *= -1; nv[2] *= -1; }
+}
+
+// First half of stepVelocityVerlet (mechanics.hpp:2892-2916): half-step
+// velocities and angular velocities, end points advanced, new position,
+// orientation (normalised) and stored velocities; the half-step angular
+// velocities go to wh (3 x n).
+__global__ void __launch_bounds__(256)
+k_verlet_half(int n, const double* __restrict__ cells, int ld, const double* __restrict__ F, VerletParams P,
+              double R, double* __restrict__ out, int ld_out, double* __restrict__ wh)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        for (int col = 0; col < P.ncols; ++col) out[(int64_t)col * ld_out + c] = cellv(cells, ld, c, col);
+        const double nc[3] = {cellv(cells, ld, c, C_NX), cellv(cells, ld, c, C_NY), cellv(cells, ld, c, C_NZ)};
+        const double l = cellv(cells, ld, c, C_L), hl = cellv(cells, ld, c, C_HALF_L);
+        double mass, I0, I1, d1[3], d2[3];
+        verlet_inertia(R, P.R3, P.rho, l, mass, I0, I1);
+        verlet_accel(F, n, c, nc, mass, I0, I1, d1, d2);
+        const double dnv[3] = {cellv(cells, ld, c, C_DNX), cellv(cells, ld, c, C_DNY), cellv(cells, ld, c, C_DNZ)};
+        const double w[3] = {nc[1] * dnv[2] - nc[2] * dnv[1], nc[2] * dnv[0] - nc[0] * dnv[2],
+                             nc[0] * dnv[1] - nc[1] * dnv[0]};
+        const double h = 0.5 * P.dt;
+        double v[3], wi[3], pp[3], qq[3], pq[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v[q] = cellv(cells, ld, c, C_DRX + q) + h * d1[q];
+            wi[q] = w[q] + h * d2[q];
+            wh[(int64_t)q * n + c] = wi[q];
+            const double r = cellv(cells, ld, c, C_RX + q);
+            pp[q] = r - hl * nc[q];
+            qq[q] = r + hl * nc[q];
+            pq[q] = pp[q] - qq[q];
+        }
+        const double cr[3] = {(wi[1] * pq[2] - wi[2] * pq[1]) / 2, (wi[2] * pq[0] - wi[0] * pq[2]) / 2,
+                              (wi[0] * pq[1] - wi[1] * pq[0]) / 2};
+        double nn[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double pn = pp[q] + P.dt * (v[q] + cr[q]);
+            const double qn = qq[q] + P.dt * (v[q] - cr[q]);
+            out[(int64_t)(C_RX + q) * ld_out + c] = (pn + qn) / 2;
+            nn[q] = (qn - pn) / l;
+            out[(int64_t)(C_DRX + q) * ld_out + c] = v[q];
+        }
+        out[(int64_t)C_DNX * ld_out + c] = wi[1] * nc[2] - wi[2] * nc[1];
+        out[(int64_t)C_DNY * ld_out + c] = wi[2] * nc[0] - wi[0] * nc[2];
+        out[(int64_t)C_DNZ * ld_out + c] = wi[0] * nc[1] - wi[1] * nc[0];
+        normalize_orientation(nn);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) out[(int64_t)(C_NX + q) * ld_out + c] = nn[q];
+    }
+}
+
+// Second half (mechanics.hpp:2970-2992): velocities and angular velocities
+// completed with the forces at the new positions, dn = w x n, orientations
+// normalised again.
+__global__ void __launch_bounds__(256)
+k_verlet_final(int n, double* __restrict__ cells, int ld, const double* __restrict__ F, VerletParams P, double R,
+               const double* __restrict__ wh)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double nc[3] = {cells[(int64_t)C_NX * ld + c], cells[(int64_t)C_NY * ld + c], cells[(int64_t)C_NZ * ld + c]};
+        double mass, I0, I1, d1[3], d2[3];
+        verlet_inertia(R, P.R3, P.rho, cells[(int64_t)C_L * ld + c], mass, I0, I1);
+        verlet_accel(F, n, c, nc, mass, I0, I1, d1, d2);
+        const double h = 0.5 * P.dt;
+        double wf[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            cells[(int64_t)(C_DRX + q) * ld + c] += h * d1[q];
+            wf[q] = wh[(int64_t)q * n + c] + h * d2[q];
+        }
+        cells[(int64_t)C_DNX * ld + c] = wf[1] * nc[2] - wf[2] * nc[1];
+        cells[(int64_t)C_DNY * ld + c] = wf[2] * nc[0] - wf[0] * nc[2];
+        cells[(int64_t)C_DNZ * ld + c] = wf[0] * nc[1] - wf[1] * nc[0];
+        normalize_orientation(nc);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) cells[(int64_t)(C_NX + q) * ld + c] = nc[q];
+    }
+}
+
 
 __device__ __forceinline__ CellVel load_cellvel(const double* __restrict__ cells, int ld, int i)
 {
@@ -2439,6 +2743,46 @@ void step_rk_device(bfm_ctx* c, const Tableau& tb, double dt, const double* d_ce
     if (tol > 0 && d_err_scaled) KLAUNCH(c, "err_final", k_err_final, 1, 1, 0, c->d_errmax, d_err_scaled);
 }
 
+// stepVelocityVerlet on device data with the internal list
+// (mechanics.hpp:2668-2996). The list's distances are updated to the half
+// step (the reference's neighbors_new), so after the call the internal list
+// describes d_cells_new. Work arrays: the generalized forces (6 x n) and the
+// half-step angular velocities (3 x n) in the stage buffer.
+void step_verlet_device(bfm_ctx* c, double dt, double rho, const double* d_cells, int n, int ld,
+                        const double* d_noise, double* d_new, int ld_new)
+{
+    need_params(c);
+    require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
+    require(dt > 0 && rho > 0, BFM_ERR_ARG, "need dt > 0 and rho > 0");
+    VerletParams P;
+    P.vp = vel_params(c);
+    P.rho = rho;
+    P.dt = dt;
+    P.R3 = std::pow(c->p.R, 3);
+    P.ncols = c->p.ncols;
+    double* F = c->d_k;
+    double* wh = c->d_k + (int64_t)6 * n;
+    RowLists L = row_lists(c);
+    FricGather fg = fric_gather(c);
+    auto forces = [&](const double* cells, int ldc) {
+        compute_forces(c, cells, n, ldc);
+        if (fg.frec) launch_friction(c, cells, ldc, nullptr, 0);
+        if (c->list_csr)
+            KLAUNCH(c, "verlet_forces", k_verlet_forces<true>, grid_for(n, 256), 256, 0, n, cells, ldc, d_noise, P, L,
+                    c->d_rec, fg, F);
+        else
+            KLAUNCH(c, "verlet_forces", k_verlet_forces<false>, grid_for(n, 256), 256, 0, n, cells, ldc, d_noise, P,
+                    L, c->d_rec, fg, F);
+    };
+    forces(d_cells, ld);
+    KLAUNCH(c, "verlet_half", k_verlet_half, grid_for(n, 256), 256, 0, n, d_cells, ld, F, P, c->p.R, d_new, ld_new,
+            wh);
+    KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, d_new, ld_new, c->pi, c->pj,
+            c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+    forces(d_new, ld_new);
+    KLAUNCH(c, "verlet_final", k_verlet_final, grid_for(n, 256), 256, 0, n, d_new, ld_new, F, P, c->p.R, wh);
+}
+
 template <typename F>
 int guarded(bfm_ctx* c, F&& f)
 {
@@ -2822,6 +3166,36 @@ int bfm_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs, 
             download6(c, c->d_err6, errors, n);
         }
         check_status(c);
+    });
+}
+
+int bfm_step_verlet(bfm_ctx* c, double dt, double rho, const double* cells, int n, int ld, const double* nb,
+                    int64_t m, int64_t ld_nb, const double* noise, double* cells_new)
+{
+    return guarded(c, [&] {
+        need_params(c);
+        check_cells(c, n, ld);
+        require(cells_new != nullptr, BFM_ERR_ARG, "NULL argument");
+        upload_cells(c, cells, n, ld);
+        import_host_neighbors(c, nb, m, ld_nb, n);
+        if (noise && n > 0)
+            HIPCHK(hipMemcpyAsync(c->d_noise, noise, sizeof(double) * 6 * n, hipMemcpyHostToDevice, c->stream));
+        step_verlet_device(c, dt, rho, c->d_cells, n, c->max_cells, noise ? c->d_noise : nullptr, c->d_cells_new,
+                           c->max_cells);
+        if (n > 0)
+            HIPCHK(hipMemcpy2DAsync(cells_new, sizeof(double) * n, c->d_cells_new, sizeof(double) * c->max_cells,
+                                    sizeof(double) * n, c->p.ncols, hipMemcpyDeviceToHost, c->stream));
+        check_status(c);
+    });
+}
+
+int bfm_dev_step_verlet(bfm_ctx* c, double dt, double rho, const double* d_cells, int n, int ld,
+                        const double* d_noise, double* d_cells_new, int ld_new)
+{
+    return guarded(c, [&] {
+        check_cells(c, n, ld);
+        require(d_cells_new != nullptr && ld_new >= n, BFM_ERR_ARG, "need d_cells_new with ld_new >= n");
+        step_verlet_device(c, dt, rho, d_cells, n, ld, d_noise, d_cells_new, ld_new);
     });
 }
 

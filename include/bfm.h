@@ -157,6 +157,16 @@ int bfm_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* bs
                 const double* neighbors, int64_t m, int64_t ld_nb,
                 const double* noise, double* cells_new, double* errors);
 
+/* stepVelocityVerlet (mechanics.hpp:2668-2996): one velocity Verlet step of
+ * size dt for cells of density rho (runSimulation sets rho = M0 / volume of a
+ * cell of length L0, simulation.hpp:1280-1290), with the include_constraint
+ * forms of the forces, ambient viscosity and cell-surface friction; noise
+ * (n x 6, may be NULL) is added to both force evaluations as the reference
+ * draws it once per call. cells_new has leading dimension n. */
+int bfm_step_verlet(bfm_ctx* ctx, double dt, double rho, const double* cells, int n, int ld,
+                    const double* neighbors, int64_t m, int64_t ld_nb, const double* noise,
+                    double* cells_new);
+
 /* ---- device-resident pipeline (device pointers, stream-ordered) -------- */
 /* Builds the context's internal neighbor list from device cells. *m_out (host
  * pointer, may be NULL) receives m after a stream synchronisation; pass NULL
@@ -194,6 +204,11 @@ int bfm_dev_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double
                     int s, double dt, const double* d_cells, int n, int ld,
                     const double* d_noise, double* d_cells_new, double* d_errors,
                     double tol, double* d_err_scaled);
+
+/* stepVelocityVerlet on device arrays with the internal list, whose distances
+ * are updated to the new positions (the reference's neighbors_new). */
+int bfm_dev_step_verlet(bfm_ctx* ctx, double dt, double rho, const double* d_cells, int n, int ld,
+                        const double* d_noise, double* d_cells_new, int ld_new);
 
 /* ---- simulation driver (runSimulation, simulation.hpp:1106-2081) -------- */
 /* Parameters of the growth/division loop; the mechanics parameters come from

@@ -624,8 +624,31 @@ void oracle_area_integrals(double rz, double nz, double R, double half_l, double
 /* cellCellFrictionForces, mechanics.hpp:1031-1175 (include_constraint =
  * false, dz_threshold 1e-3 as passed at :1343-1346 and :1652-1655). dEdq is
  * n x 6 column-major, ld n, overwritten; rows accumulated in row order. */
+static void friction_forces_impl(const double* cells, int n, int ld, const double* nb, int64_t m,
+                                 int64_t ld_nb, const bfm_params* p, bool ic, double* dEdq);
+
 void oracle_friction_forces(const double* cells, int n, int ld, const double* nb, int64_t m,
                             int64_t ld_nb, const bfm_params* p, double* dEdq)
+{
+    friction_forces_impl(cells, n, ld, nb, m, ld_nb, p, false, dEdq);
+}
+
+/* (s n) x v, the torque form of include_constraint = true (mechanics.hpp:1300-1305). */
+static inline void cross_sn(double s, const double* nrm, const double* v, double* out)
+{
+    const double a[3] = {s * nrm[0], s * nrm[1], s * nrm[2]};
+    out[0] = a[1] * v[2] - a[2] * v[1];
+    out[1] = a[2] * v[0] - a[0] * v[2];
+    out[2] = a[0] * v[1] - a[1] * v[0];
+}
+
+static inline void cell_n(const double* cells, int ld, int i, double* nrm)
+{
+    nrm[0] = cell(cells, ld, i, C_NX); nrm[1] = cell(cells, ld, i, C_NY); nrm[2] = cell(cells, ld, i, C_NZ);
+}
+
+static void friction_forces_impl(const double* cells, int n, int ld, const double* nb, int64_t m,
+                                 int64_t ld_nb, const bfm_params* p, bool ic, double* dEdq)
 {
     std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
     const double R = p->R, E0 = p->E0, Req = R / 2;
@@ -669,11 +692,21 @@ void oracle_friction_forces(const double* cells, int n, int ld, const double* nb
             if (fn > max_norm)
                 for (int c = 0; c < 3; ++c) force[c] *= (max_norm / fn);
         }
+        double ti[3], tj[3];
+        if (!ic) {
+            for (int c = 0; c < 3; ++c) { ti[c] = si * force[c]; tj[c] = (-sj) * force[c]; }
+        } else {
+            double ni[3], nj[3], mf[3] = {-force[0], -force[1], -force[2]};
+            cell_n(cells, ld, i, ni);
+            cell_n(cells, ld, j, nj);
+            cross_sn(si, ni, force, ti);
+            cross_sn(sj, nj, mf, tj);
+        }
         for (int c = 0; c < 3; ++c) {
             dEdq[c * n + i] -= force[c];
-            dEdq[(c + 3) * n + i] -= si * force[c];
+            dEdq[(c + 3) * n + i] -= ti[c];
             dEdq[c * n + j] += force[c];
-            dEdq[(c + 3) * n + j] -= (-sj) * force[c];
+            dEdq[(c + 3) * n + j] -= tj[c];
         }
     }
 }
@@ -683,8 +716,26 @@ void oracle_friction_forces(const double* cells, int n, int ld, const double* nb
  * when friction_mode is KINETIC (:1341-1347, :1650-1656). dEdq is n x 6
  * column-major, ld n, and is overwritten. Rows are accumulated in the order
  * of the neighbor rows, as the reference does. */
+static void cell_cell_forces_impl(const double* cells, int n, int ld, const double* nb, int64_t m,
+                                  int64_t ld_nb, const bfm_params* p, bool ic, double* dEdq);
+
 void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* nb,
                              int64_t m, int64_t ld_nb, const bfm_params* p, double* dEdq)
+{
+    cell_cell_forces_impl(cells, n, ld, nb, m, ld_nb, p, false, dEdq);
+}
+
+/* include_constraint = true: the torques of the Hertz, EPS-cap and hard-core
+ * terms are (s n) x F (mechanics.hpp:1298-1306, 1554-1562, 1608-1616); those of
+ * the JKR term are s (-(n . F) n ... ) as adhesion.hpp:225-231 writes them. */
+void oracle_cell_cell_forces_ic(const double* cells, int n, int ld, const double* nb,
+                                int64_t m, int64_t ld_nb, const bfm_params* p, double* dEdq)
+{
+    cell_cell_forces_impl(cells, n, ld, nb, m, ld_nb, p, true, dEdq);
+}
+
+static void cell_cell_forces_impl(const double* cells, int n, int ld, const double* nb, int64_t m,
+                                  int64_t ld_nb, const bfm_params* p, bool ic, double* dEdq)
 {
     std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
     const double R = p->R, Rcell = p->Rcell, E0 = p->E0;
@@ -700,7 +751,7 @@ void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* n
         double si = nb[5 * ld_nb + k], sj = nb[6 * ld_nb + k];
         double dijn[3] = {dij[0] / dist, dij[1] / dist, dij[2] / dist};
         double w[3];            // dE/dr_i of the main term; dE/dr_j = -w
-        bool hard = false;
+        bool hard = false, jkr = false;
         double wh[3] = {0, 0, 0};
         if (!adhesion) {
             double mag;
@@ -733,6 +784,7 @@ void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* n
                 // v = (term2 - term1) * d12n; dEdq(i) gets -v and -s v, dEdq(j)
                 // gets v and t v (adhesion.hpp:212-229). -v == (term1 - term2) * d12n.
                 for (int c = 0; c < 3; ++c) w[c] = (term1 - term2) * dijn[c];
+                jkr = true;
             } else {
                 double mag;
                 if (overlap <= 2 * (R - Rcell)) mag = p->repulsion_prefactors[0] * std::pow(overlap, 1.5);
@@ -745,36 +797,71 @@ void oracle_cell_cell_forces(const double* cells, int n, int ld, const double* n
                 hard = true;
             }
         }
+        double ti[3], tj[3], ni[3], nj[3];
+        if (ic) { cell_n(cells, ld, i, ni); cell_n(cells, ld, j, nj); }
+        if (!ic) {
+            for (int c = 0; c < 3; ++c) { ti[c] = si * w[c]; tj[c] = -sj * w[c]; }
+        } else if (jkr) {
+            // v = -w: dEdq(0, 3:5) = s (-w1 n1 - v), dEdq(1, 3:5) = t (w2 n2 + v),
+            // w1 = n1 . (-v), w2 = n2 . (-v)   (adhesion.hpp:225-231)
+            double w1 = (ni[0] * w[0] + ni[1] * w[1]) + ni[2] * w[2];
+            double w2 = (nj[0] * w[0] + nj[1] * w[1]) + nj[2] * w[2];
+            for (int c = 0; c < 3; ++c) {
+                ti[c] = si * ((-w1) * ni[c] + w[c]);
+                tj[c] = sj * (w2 * nj[c] - w[c]);
+            }
+        } else {
+            double mw[3] = {-w[0], -w[1], -w[2]};
+            cross_sn(si, ni, w, ti);
+            cross_sn(sj, nj, mw, tj);
+        }
         for (int c = 0; c < 3; ++c) {
             dEdq[c * n + i] += w[c];
-            dEdq[(c + 3) * n + i] += si * w[c];
+            dEdq[(c + 3) * n + i] += ti[c];
         }
         for (int c = 0; c < 3; ++c) {
             dEdq[c * n + j] += -w[c];
-            dEdq[(c + 3) * n + j] += -sj * w[c];
+            dEdq[(c + 3) * n + j] += tj[c];
         }
         if (hard) {
+            double thi[3], thj[3];
+            if (!ic) {
+                for (int c = 0; c < 3; ++c) { thi[c] = si * wh[c]; thj[c] = -sj * wh[c]; }
+            } else {
+                double mwh[3] = {-wh[0], -wh[1], -wh[2]};
+                cross_sn(si, ni, wh, thi);
+                cross_sn(sj, nj, mwh, thj);
+            }
             for (int c = 0; c < 3; ++c) {
                 dEdq[c * n + i] += wh[c];
-                dEdq[(c + 3) * n + i] += si * wh[c];
+                dEdq[(c + 3) * n + i] += thi[c];
             }
             for (int c = 0; c < 3; ++c) {
                 dEdq[c * n + j] += -wh[c];
-                dEdq[(c + 3) * n + j] += -sj * wh[c];
+                dEdq[(c + 3) * n + j] += thj[c];
             }
         }
     }
     if (p->friction_mode == 1) {
         std::vector<double> fr(6 * (size_t)n);
-        oracle_friction_forces(cells, n, ld, nb, m, ld_nb, p, fr.data());
+        friction_forces_impl(cells, n, ld, nb, m, ld_nb, p, ic, fr.data());
         for (size_t q = 0; q < 6 * (size_t)n; ++q) dEdq[q] += fr[q];
     }
 }
 
 /* cellSurfaceRepulsionForces, mechanics.hpp:673-780 (include_constraint =
  * false). Writes columns 2 and 5 of dEdq (n x 6, zeroed here). */
+static void surface_repulsion_impl(const double* cells, int n, int ld, const double* ss, double R, double E0,
+                                   const int* assume_2d, bool ic, double* dEdq);
+
 void oracle_surface_repulsion_forces(const double* cells, int n, int ld, const double* ss,
                                      double R, double E0, const int* assume_2d, double* dEdq)
+{
+    surface_repulsion_impl(cells, n, ld, ss, R, E0, assume_2d, false, dEdq);
+}
+
+static void surface_repulsion_impl(const double* cells, int n, int ld, const double* ss, double R, double E0,
+                                   const int* assume_2d, bool ic, double* dEdq)
 {
     std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
     const double prefactor0 = 2 * E0;
@@ -792,6 +879,14 @@ void oracle_surface_repulsion_forces(const double* cells, int n, int ld, const d
             double int1 = oracle_integral1(rz, nz, R, hl, 1.0, ss[i]);
             double int2 = oracle_integral1(rz, nz, R, hl, 0.5, ss[i]);
             dEdq[2 * n + i] = -prefactor0 * ((1.0 - nz2) * int1 + std::sqrt(R) * nz2 * int2);
+            if (ic) {   // mechanics.hpp:741-754
+                double i3 = oracle_integral2(rz, nz, R, hl, 1.0, ss[i]);
+                double i4 = oracle_integral2(rz, nz, R, hl, 0.5, ss[i]);
+                double cx = cell(cells, ld, i, C_NY), cy = -cell(cells, ld, i, C_NX);
+                dEdq[3 * n + i] = -prefactor0 * cx * ((1.0 - nz2) * i3 + std::sqrt(R) * nz2 * i4);
+                dEdq[4 * n + i] = -prefactor0 * cy * ((1.0 - nz2) * i3 + std::sqrt(R) * nz2 * i4);
+                continue;
+            }
             double int3 = oracle_integral1(rz, nz, R, hl, 2.0, ss[i]);
             double int4 = oracle_integral2(rz, nz, R, hl, 1.0, ss[i]);
             double int5 = oracle_integral1(rz, nz, R, hl, 1.5, ss[i]);
@@ -807,8 +902,17 @@ void oracle_surface_repulsion_forces(const double* cells, int n, int ld, const d
 }
 
 /* cellSurfaceAdhesionForces, mechanics.hpp:800-910 (include_constraint = false). */
+static void surface_adhesion_impl(const double* cells, int n, int ld, const double* ss, double R,
+                                  const int* assume_2d, bool ic, double* dEdq);
+
 void oracle_surface_adhesion_forces(const double* cells, int n, int ld, const double* ss,
                                     double R, const int* assume_2d, double* dEdq)
+{
+    surface_adhesion_impl(cells, n, ld, ss, R, assume_2d, false, dEdq);
+}
+
+static void surface_adhesion_impl(const double* cells, int n, int ld, const double* ss, double R,
+                                  const int* assume_2d, bool ic, double* dEdq)
 {
     std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
     const double prefactor0 = std::pow(R, 0.5) / 2;
@@ -829,6 +933,22 @@ void oracle_surface_adhesion_forces(const double* cells, int n, int ld, const do
             double f2 = prefactor0 * (1 - nz2) * int1 + term2;
             f2 *= sigma0;
             dEdq[2 * n + i] = f2;
+            if (ic) {   // mechanics.hpp:868-884
+                double i2 = oracle_integral2(rz, nz, R, hl, -0.5, ss[i]);
+                double cx = cell(cells, ld, i, C_NY), cy = -cell(cells, ld, i, C_NX);
+                double w1 = 0, w2 = 0;
+                if (std::fabs(ss[i]) < hl) {
+                    w1 = (prefactor1 / 2) * ss[i] * nz * cx;
+                    w2 = (prefactor1 / 2) * ss[i] * nz * cy;
+                }
+                double f3 = prefactor0 * (1 - nz2) * i2 * cx - w1;
+                f3 *= sigma0;
+                double f4 = prefactor0 * (1 - nz2) * i2 * cy - w2;
+                f4 *= sigma0;
+                dEdq[3 * n + i] = f3;
+                dEdq[4 * n + i] = f4;
+                continue;
+            }
             double int2 = oracle_integral1(rz, nz, R, hl, 0.5, ss[i]);
             double int3 = oracle_integral2(rz, nz, R, hl, -0.5, ss[i]);
             double int4 = oracle_integral4(nz, hl, ss[i]);
@@ -1028,6 +1148,222 @@ void oracle_step_rk(const double* A, const double* b, const double* bs, int s, d
         cells_new[(size_t)n + q] += d1;
         errors[q] = d1 - d2;
         cells_new[(size_t)7 * n + q] = v1[q];
+    }
+    oracle_normalize_orientations(cells_new, n, n);
+}
+
+/* ---- velocity Verlet ------------------------------------------------------ */
+
+/* getConservativeForces with include_constraint = true (mechanics.hpp:1702-1769). */
+void oracle_conservative_forces_ic(const double* cells, int n, int ld, const double* nb, int64_t m,
+                                   int64_t ld_nb, const int* assume_2d, const bfm_params* p, double* forces)
+{
+    std::vector<double> ss(n), cc(6 * (size_t)n), rep(6 * (size_t)n, 0.0), adh(6 * (size_t)n, 0.0);
+    sstar_all(cells, n, ld, p->R, assume_2d, ss.data());
+    cell_cell_forces_impl(cells, n, ld, nb, m, ld_nb, p, true, cc.data());
+    if (!p->no_surface) {
+        surface_repulsion_impl(cells, n, ld, ss.data(), p->R, p->E0, assume_2d, true, rep.data());
+        surface_adhesion_impl(cells, n, ld, ss.data(), p->R, assume_2d, true, adh.data());
+    }
+    for (size_t q = 0; q < 6 * (size_t)n; ++q) forces[q] = (-cc[q] - rep[q]) - adh[q];
+}
+
+/* ambientViscosityForces, mechanics.hpp:1785-1806: eta0 l^3 dr and
+ * eta0 l^9 (n x dn) / 12, as written there. */
+void oracle_ambient_viscosity_forces(const double* cells, int n, int ld, double* dEdq)
+{
+    for (int i = 0; i < n; ++i) {
+        double nv[3], dn[3], w[3];
+        cell_n(cells, ld, i, nv);
+        dn[0] = cell(cells, ld, i, C_DNX); dn[1] = cell(cells, ld, i, C_DNY); dn[2] = cell(cells, ld, i, C_DNZ);
+        w[0] = nv[1] * dn[2] - nv[2] * dn[1];
+        w[1] = nv[2] * dn[0] - nv[0] * dn[2];
+        w[2] = nv[0] * dn[1] - nv[1] * dn[0];
+        const double eta = cell(cells, ld, i, C_ETA0);
+        const double li = std::pow(cell(cells, ld, i, C_L), 3);
+        for (int c = 0; c < 3; ++c) {
+            dEdq[c * n + i] = eta * li * cell(cells, ld, i, C_DRX + c);
+            dEdq[(c + 3) * n + i] = eta * li * li * li * w[c] / 12;
+        }
+    }
+}
+
+/* cellSurfaceFrictionForces, mechanics.hpp:1822-1900. */
+void oracle_surface_friction_forces(const double* cells, int n, int ld, const double* ss, double R,
+                                    const int* assume_2d, double* dEdq)
+{
+    std::memset(dEdq, 0, sizeof(double) * 6 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        double nv[3], dn[3], w[3];
+        cell_n(cells, ld, i, nv);
+        dn[0] = cell(cells, ld, i, C_DNX); dn[1] = cell(cells, ld, i, C_DNY); dn[2] = cell(cells, ld, i, C_DNZ);
+        w[0] = nv[1] * dn[2] - nv[2] * dn[1];
+        w[1] = nv[2] * dn[0] - nv[0] * dn[2];
+        w[2] = nv[0] * dn[1] - nv[1] * dn[0];
+        const double q[3] = {cell(cells, ld, i, C_DRX), cell(cells, ld, i, C_DRY), 0.0};
+        const double mv[3] = {w[1] * nv[2] - w[2] * nv[1], w[2] * nv[0] - w[0] * nv[2], 0.0};
+        const double eta = cell(cells, ld, i, C_ETA1);
+        const double l = cell(cells, ld, i, C_L);
+        if (assume_2d[i]) {
+            const double phi = R - cell(cells, ld, i, C_RZ);
+            if (phi > 0) {
+                const double area = std::sqrt(R) * std::sqrt(phi) * l;
+                dEdq[0 * n + i] = eta * q[0] * area / R;
+                dEdq[1 * n + i] = eta * q[1] * area / R;
+                const double cross_z = nv[0] * mv[1] - nv[1] * mv[0];
+                const double l2 = l * l;
+                dEdq[5 * n + i] = eta * cross_z * area * l2 / (12 * R);
+            }
+        } else {
+            double ai[3];
+            oracle_area_integrals(cell(cells, ld, i, C_RZ), nv[2], R, cell(cells, ld, i, C_HALF_L), ss[i], ai);
+            const double int1 = ai[0], int2 = ai[1], int3 = ai[2];
+            dEdq[0 * n + i] = eta * (q[0] * int1 + mv[0] * int2) / R;
+            dEdq[1 * n + i] = eta * (q[1] * int1 + mv[1] * int2) / R;
+            const double c1[3] = {nv[1] * q[2] - nv[2] * q[1], nv[2] * q[0] - nv[0] * q[2], nv[0] * q[1] - nv[1] * q[0]};
+            const double c2[3] = {nv[1] * mv[2] - nv[2] * mv[1], nv[2] * mv[0] - nv[0] * mv[2],
+                                  nv[0] * mv[1] - nv[1] * mv[0]};
+            for (int c = 0; c < 3; ++c) dEdq[(3 + c) * n + i] = eta * (c1[c] * int2 + c2[c] * int3) / R;
+        }
+    }
+}
+
+/* The generalized forces of stepVelocityVerlet (mechanics.hpp:2845-2872):
+ * getConservativeForces (include_constraint = true) + noise - ambient
+ * viscosity - cell-surface friction truncated to cs_coulomb |repulsion|. */
+static void verlet_forces(const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                          const int* a2d, const double* noise, const bfm_params* p, double* F)
+{
+    oracle_conservative_forces_ic(cells, n, ld, nb, m, ld_nb, a2d, p, F);
+    if (noise)
+        for (size_t q = 0; q < 6 * (size_t)n; ++q) F[q] += noise[q];
+    std::vector<double> amb(6 * (size_t)n);
+    oracle_ambient_viscosity_forces(cells, n, ld, amb.data());
+    for (size_t q = 0; q < 6 * (size_t)n; ++q) F[q] -= amb[q];
+    if (!p->no_surface) {
+        std::vector<double> ss(n), sf(6 * (size_t)n), rep(6 * (size_t)n);
+        sstar_all(cells, n, ld, p->R, a2d, ss.data());
+        oracle_surface_friction_forces(cells, n, ld, ss.data(), p->R, a2d, sf.data());
+        if (p->cell_surface_coulomb_coeff > 0) {
+            surface_repulsion_impl(cells, n, ld, ss.data(), p->R, p->E0, a2d, true, rep.data());
+            for (int i = 0; i < n; ++i) {
+                double fn2 = 0, rn2 = 0;
+                for (int c = 0; c < 6; ++c) {
+                    fn2 += sf[(size_t)c * n + i] * sf[(size_t)c * n + i];
+                    rn2 += rep[(size_t)c * n + i] * rep[(size_t)c * n + i];
+                }
+                const double fn = std::sqrt(fn2), mx = p->cell_surface_coulomb_coeff * std::sqrt(rn2);
+                if (fn > mx)
+                    for (int c = 0; c < 6; ++c) sf[(size_t)c * n + i] *= (mx / fn);
+            }
+        }
+        for (size_t q = 0; q < 6 * (size_t)n; ++q) F[q] -= sf[q];
+    }
+}
+
+/* Accelerations of stepVelocityVerlet (mechanics.hpp:2874-2891, 2975-2986):
+ * delta1 = F / mass, delta2 = torque_par / I0 + torque_perp / I1. */
+static void verlet_accel(const double* F, const double* cells, int n, int ld, int i, const double* mass,
+                         const double* I0, const double* I1, double d1[3], double d2[3])
+{
+    double nv[3], t[3], par[3], perp[3];
+    cell_n(cells, ld, i, nv);
+    for (int c = 0; c < 3; ++c) t[c] = F[(size_t)(3 + c) * n + i];
+    const double dot = (t[0] * nv[0] + t[1] * nv[1]) + t[2] * nv[2];
+    for (int c = 0; c < 3; ++c) {
+        par[c] = dot * nv[c];
+        perp[c] = t[c] - par[c];
+    }
+    for (int c = 0; c < 3; ++c) {
+        d1[c] = F[(size_t)c * n + i] / mass[i];
+        d2[c] = par[c] / I0[i];
+        d2[c] += perp[c] / I1[i];
+    }
+}
+
+/* stepVelocityVerlet, mechanics.hpp:2668-2996, with caller-drawn noise (n x 6,
+ * or NULL for none). cells_new has leading dimension n; neighbors are used
+ * as given for the first force evaluation and with distances updated to the
+ * half step for the second (a private copy, as the reference's neighbors_new). */
+void oracle_step_verlet(double dt, double rho, const double* cells, int n, int ld, const double* nb, int64_t m,
+                        int64_t ld_nb, const double* noise, const bfm_params* p, double* cells_new)
+{
+    const int ncols = p->ncols;
+    const double R = p->R;
+    std::vector<int> a2d(n);
+    for (int i = 0; i < n; ++i) a2d[i] = cell(cells, ld, i, C_NZ) < p->nz_threshold;
+    std::vector<double> mass(n), I0(n), I1(n);
+    for (int i = 0; i < n; ++i) {
+        const double length = cell(cells, ld, i, C_L);
+        const double volume = kPi * ((4. / 3.) * std::pow(R, 3) + R * R * length);
+        mass[i] = rho * volume;
+        const double diam = 2 * R, d2 = diam * diam, d3 = d2 * diam, d4 = d3 * diam;
+        const double l2 = length * length, l3 = l2 * length;
+        I0[i] = d4 * (length / 32 + diam / 60);
+        I1[i] = d2 * (4 * d3 + 15 * length * d2 + 20 * l2 * diam + 10 * l3) / 480;
+        I0[i] *= (rho * kPi);
+        I1[i] *= (rho * kPi);
+    }
+    std::vector<double> F(6 * (size_t)n);
+    verlet_forces(cells, n, ld, nb, m, ld_nb, a2d.data(), noise, p, F.data());
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < n; ++i) cells_new[(size_t)c * n + i] = cell(cells, ld, i, c);
+    std::vector<double> wh(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        double d1[3], d2[3], nv[3], dn[3], w[3], r[3], v[3], wi[3];
+        verlet_accel(F.data(), cells, n, ld, i, mass.data(), I0.data(), I1.data(), d1, d2);
+        cell_n(cells, ld, i, nv);
+        dn[0] = cell(cells, ld, i, C_DNX); dn[1] = cell(cells, ld, i, C_DNY); dn[2] = cell(cells, ld, i, C_DNZ);
+        w[0] = nv[1] * dn[2] - nv[2] * dn[1];
+        w[1] = nv[2] * dn[0] - nv[0] * dn[2];
+        w[2] = nv[0] * dn[1] - nv[1] * dn[0];
+        const double h = 0.5 * dt;
+        for (int c = 0; c < 3; ++c) {
+            r[c] = cell(cells, ld, i, C_RX + c);
+            v[c] = cell(cells, ld, i, C_DRX + c) + h * d1[c];
+            wi[c] = w[c] + h * d2[c];
+            wh[(size_t)c * n + i] = wi[c];
+        }
+        const double hl = cell(cells, ld, i, C_HALF_L), l = cell(cells, ld, i, C_L);
+        double pp[3], qq[3], pq[3], cr[3], pn[3], qn[3];
+        for (int c = 0; c < 3; ++c) {
+            pp[c] = r[c] - hl * nv[c];
+            qq[c] = r[c] + hl * nv[c];
+            pq[c] = pp[c] - qq[c];
+        }
+        cr[0] = (wi[1] * pq[2] - wi[2] * pq[1]) / 2;
+        cr[1] = (wi[2] * pq[0] - wi[0] * pq[2]) / 2;
+        cr[2] = (wi[0] * pq[1] - wi[1] * pq[0]) / 2;
+        for (int c = 0; c < 3; ++c) {
+            pn[c] = pp[c] + dt * (v[c] + cr[c]);
+            qn[c] = qq[c] + dt * (v[c] - cr[c]);
+            cells_new[(size_t)(C_RX + c) * n + i] = (pn[c] + qn[c]) / 2;
+            cells_new[(size_t)(C_NX + c) * n + i] = (qn[c] - pn[c]) / l;
+            cells_new[(size_t)(C_DRX + c) * n + i] = v[c];
+        }
+        cells_new[(size_t)C_DNX * n + i] = wi[1] * nv[2] - wi[2] * nv[1];
+        cells_new[(size_t)C_DNY * n + i] = wi[2] * nv[0] - wi[0] * nv[2];
+        cells_new[(size_t)C_DNZ * n + i] = wi[0] * nv[1] - wi[1] * nv[0];
+    }
+    oracle_normalize_orientations(cells_new, n, n);
+    std::vector<double> nb2((size_t)7 * (m > 0 ? m : 1));
+    for (int c = 0; c < 7; ++c)
+        for (int64_t k = 0; k < m; ++k) nb2[(size_t)c * m + k] = nb[(size_t)c * ld_nb + k];
+    oracle_update_neighbor_distances(cells_new, n, n, nb2.data(), m, m > 0 ? m : 1);
+    for (int i = 0; i < n; ++i) a2d[i] = cells_new[(size_t)C_NZ * n + i] < p->nz_threshold;
+    verlet_forces(cells_new, n, n, nb2.data(), m, m > 0 ? m : 1, a2d.data(), noise, p, F.data());
+    for (int i = 0; i < n; ++i) {
+        double d1[3], d2[3], nv[3], wf[3];
+        verlet_accel(F.data(), cells_new, n, n, i, mass.data(), I0.data(), I1.data(), d1, d2);
+        cell_n(cells_new, n, i, nv);
+        const double h = 0.5 * dt;
+        for (int c = 0; c < 3; ++c) {
+            cells_new[(size_t)(C_DRX + c) * n + i] += h * d1[c];
+            wf[c] = wh[(size_t)c * n + i] + h * d2[c];
+        }
+        cells_new[(size_t)C_DNX * n + i] = wf[1] * nv[2] - wf[2] * nv[1];
+        cells_new[(size_t)C_DNY * n + i] = wf[2] * nv[0] - wf[0] * nv[2];
+        cells_new[(size_t)C_DNZ * n + i] = wf[0] * nv[1] - wf[1] * nv[0];
     }
     oracle_normalize_orientations(cells_new, n, n);
 }

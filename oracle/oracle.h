@@ -34,6 +34,15 @@ void oracle_dist_between_cells(const double* r1, const double* n1, double half_l
                                double* d, double* s, double* t);
 void oracle_friction_forces(const double* cells, int n, int ld, const double* nb, int64_t m,
                             int64_t ld_nb, const bfm_params* p, double* dEdq);
+void oracle_cell_cell_forces_ic(const double* cells, int n, int ld, const double* nb,
+                                int64_t m, int64_t ld_nb, const bfm_params* p, double* dEdq);
+void oracle_conservative_forces_ic(const double* cells, int n, int ld, const double* nb, int64_t m,
+                                   int64_t ld_nb, const int* assume_2d, const bfm_params* p, double* forces);
+void oracle_ambient_viscosity_forces(const double* cells, int n, int ld, double* dEdq);
+void oracle_surface_friction_forces(const double* cells, int n, int ld, const double* ss, double R,
+                                    const int* assume_2d, double* dEdq);
+void oracle_step_verlet(double dt, double rho, const double* cells, int n, int ld, const double* nb, int64_t m,
+                        int64_t ld_nb, const double* noise, const bfm_params* p, double* cells_new);
 int64_t oracle_get_cell_neighbors(const double* cells, int n, int ld, double thr,
                                   double R, double Ldiv, double* out, int64_t ld_out);
 int64_t oracle_get_cell_neighbors_binned(const double* cells, int n, int ld, double thr,

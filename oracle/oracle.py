@@ -67,6 +67,11 @@ def lib():
         vp = C.c_void_p
         L.oracle_cell_cell_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, vp, _dp]
         L.oracle_friction_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, vp, _dp]
+        L.oracle_cell_cell_forces_ic.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, vp, _dp]
+        L.oracle_conservative_forces_ic.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, _ip, vp, _dp]
+        L.oracle_ambient_viscosity_forces.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        L.oracle_surface_friction_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, d, _ip, _dp]
+        L.oracle_step_verlet.argtypes = [d, d, _dp, C.c_int, C.c_int, _dp, i64, i64, _dp, vp, _dp]
         L.oracle_surface_repulsion_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, d, d, _ip, _dp]
         L.oracle_surface_adhesion_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, d, _ip, _dp]
         L.oracle_conservative_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, _ip, vp,
@@ -225,6 +230,37 @@ def friction_forces(cells, nb, params):
     out = np.zeros((n, 6), order="F")
     lib().oracle_friction_forces(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
                                  C.byref(params), _p(out))
+    return out
+
+
+def cell_cell_forces_ic(cells, nb, params):
+    """cellCellInteractionForces with include_constraint = true: dE/dq n x 6."""
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_cell_cell_forces_ic(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                                     C.byref(params), _p(out))
+    return out
+
+
+def conservative_forces_ic(cells, nb, assume_2d, params):
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    a2d = np.ascontiguousarray(assume_2d, dtype=np.int32)
+    out = np.zeros((n, 6), order="F")
+    lib().oracle_conservative_forces_ic(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                                        a2d.ctypes.data_as(_ip), C.byref(params), _p(out))
+    return out
+
+
+def step_verlet(dt, rho, cells, nb, params, noise=None):
+    """stepVelocityVerlet (mechanics.hpp:2668-2996); returns the new cells."""
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    out = np.zeros((n, params.ncols), order="F")
+    nz = _f(noise) if noise is not None else None
+    lib().oracle_step_verlet(dt, rho, _p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                             _p(nz) if nz is not None else None, C.byref(params), _p(out))
     return out
 
 
