@@ -14,7 +14,8 @@ import numpy as np
 
 from .core import COL, Context, load_library, BFMError
 
-TABLEAU_IDS = {"heun_euler": 0, "bogacki_shampine": 1, "rkf45": 2, "dormand_prince": 3}
+TABLEAU_IDS = {"heun_euler": 0, "bogacki_shampine": 1, "rkf45": 2, "dormand_prince": 3,
+               "velocity_verlet": -1}
 
 
 class SimConfig(C.Structure):
@@ -28,7 +29,8 @@ class SimConfig(C.Structure):
                 ("iter_update_neighbors", C.c_int), ("iter_update_stepsize", C.c_int),
                 ("max_tries_update_stepsize", C.c_int), ("n_cells", C.c_int),
                 ("max_iter", C.c_int), ("tableau", C.c_int), ("seed", C.c_uint64),
-                ("report_seconds", C.c_double), ("max_wall_seconds", C.c_double)]
+                ("report_seconds", C.c_double), ("max_wall_seconds", C.c_double),
+                ("rho", C.c_double)]
 
 
 class SimStats(C.Structure):
@@ -51,9 +53,10 @@ def sim_config(n_cells=0, max_iter=0, max_time=0.0, R=0.8, Rcell=0.5, L0=0.4,
                growth_mean=(1.5, 1.5), growth_std=(0.15, 0.15), daughter_length_std=0.125,
                daughter_angle_xy_bound=0.05, daughter_angle_z_bound=0.0,
                iter_update_neighbors=10, iter_update_stepsize=1, max_tries_update_stepsize=3,
-               tableau="heun_euler", seed=42, report_seconds=0.0, max_wall_seconds=0.0):
+               tableau="heun_euler", seed=42, report_seconds=0.0, max_wall_seconds=0.0, M0=0.0):
     """Loop parameters named as the JSON keys of abmConstAdhesion.cpp:38-66:
-    Ldiv = 2 L0 + 2 R and neighbor_threshold = (2 R + Ldiv) / 2."""
+    Ldiv = 2 L0 + 2 R and neighbor_threshold = (2 R + Ldiv) / 2; tableau
+    "velocity_verlet" with M0 > 0 runs stepVelocityVerlet at dt = max_stepsize."""
     c = SimConfig()
     c.R, c.Rcell = R, Rcell
     c.Ldiv = 2 * L0 + 2 * R
@@ -73,6 +76,8 @@ def sim_config(n_cells=0, max_iter=0, max_time=0.0, R=0.8, Rcell=0.5, L0=0.4,
     c.tableau = TABLEAU_IDS[tableau] if isinstance(tableau, str) else int(tableau)
     c.seed = seed
     c.report_seconds, c.max_wall_seconds = report_seconds, max_wall_seconds
+    # velocity Verlet: density from the mass M0 of a cell of length L0 (simulation.hpp:1280-1290)
+    c.rho = M0 / (np.pi * ((4. / 3.) * pow(R, 3) + R * R * L0)) if M0 > 0 else 0.0
     return c
 
 

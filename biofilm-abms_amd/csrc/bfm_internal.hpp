@@ -37,6 +37,12 @@ int grow_batched(bfm_ctx* c, double* d_cells, int ld, const int* d_n, double R, 
 // cell's replicate before a neighbor build.
 int set_replicate_count(bfm_ctx* c, int n_rep);
 
+// stepVelocityVerlet on device data with the internal list (whose distances
+// end at the new positions); d_bad[0] is set to 1 when a new position or
+// orientation is not finite.
+int step_verlet(bfm_ctx* c, double dt, double rho, const double* d_cells, int n, int ld, double* d_new, int ld_new,
+                int* d_bad);
+
 // Ordered-key <-> double for the scaled-error maxima.
 double order_key_value(unsigned long long k);
 

@@ -677,7 +677,8 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
         bfm_run_config o;
         load_config(json_path, &o);
         if (o.adhesion_mode == 2) io_fail(BFM_ERR_UNSUPPORTED, "anisotropic JKR adhesion is not supported");
-        if (o.integration_mode == 0) io_fail(BFM_ERR_UNSUPPORTED, "velocity Verlet is not supported");
+        if (o.integration_mode == 0 && !(o.M0 > 0))
+            io_fail(BFM_ERR_ARG, "Initial cell mass must be defined when using velocity Verlet");
         if (o.basal_only) io_fail(BFM_ERR_UNSUPPORTED, "basal_only is not supported");
         if (o.max_rxy_noise > 0 || o.max_rz_noise > 0 || o.max_nxy_noise > 0 || o.max_nz_noise > 0)
             io_fail(BFM_ERR_UNSUPPORTED, "noise in the growth loop is not supported");
@@ -802,7 +803,9 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
         sc.max_tries_update_stepsize = 3;
         sc.n_cells = o.n_cells;
         sc.max_iter = o.max_iter;
-        sc.tableau = o.integration_mode - 1;   // HEUN_EULER = 1 ... DORMAND_PRINCE = 4
+        sc.tableau = o.integration_mode - 1;   // VELOCITY_VERLET = 0 -> -1, HEUN_EULER = 1 ... DORMAND_PRINCE = 4
+        if (o.integration_mode == 0)
+            sc.rho = o.M0 / (bfm::kPi * ((4. / 3.) * std::pow(o.R, 3) + o.R * o.R * o.L0));
         sc.seed = (uint64_t)(uint32_t)rng_seed;
         sc.report_seconds = verbose ? 30.0 : 0.0;
 

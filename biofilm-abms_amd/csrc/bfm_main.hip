@@ -1660,6 +1660,17 @@ k_verlet_forces(int n, const double* __restrict__ cells, int ld, const double* _
     }
 }
 
+// Sets *bad when a position or orientation is not finite.
+__global__ void k_nonfinite(int n, const double* __restrict__ cells, int ld, int* __restrict__ bad)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ok &= isfinite(cells[(int64_t)(C_RX + q) * ld + c]);
+        if (!ok) atomicOr(bad, 1);
+    }
+}
+
 // Mass and the two moments of inertia of a spherocylinder of length l and
 // density rho (mechanics.hpp:2800-2822).
 __device__ __forceinline__ void verlet_inertia(double R, double R3, double rho, double l, double& mass,
@@ -2811,6 +2822,16 @@ int guarded(bfm_ctx* c, F&& f)
 namespace bfm {
 
 constexpr int kBatchGrid = 2048;   // fixed grid of the graph-captured batched kernels
+
+int step_verlet(bfm_ctx* c, double dt, double rho, const double* d_cells, int n, int ld, double* d_new, int ld_new,
+                int* d_bad)
+{
+    return guarded(c, [&] {
+        check_cells(c, n, ld);
+        step_verlet_device(c, dt, rho, d_cells, n, ld, nullptr, d_new, ld_new);
+        KLAUNCH(c, "nonfinite", k_nonfinite, grid_for(n, 256), 256, 0, n, d_new, ld_new, d_bad);
+    });
+}
 
 int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld)
 {

@@ -692,7 +692,11 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     if (total64 > cap || cap > c->max_cells) sim_fail(BFM_ERR_ARG, "need sum(sizes) <= cap <= max_cells");
     const auto wall0 = std::chrono::steady_clock::now();
     auto wall = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count(); };
-    const TableauDef tdef = tableau(cfg->tableau);
+    // tableau -1: velocity Verlet at the fixed dt = max_stepsize (simulation.hpp:1787-1798)
+    const bool verlet = cfg->tableau == -1;
+    if (verlet && (R != 1 || !(cfg->rho > 0)))
+        sim_fail(BFM_ERR_ARG, "velocity Verlet runs one colony and needs rho > 0");
+    const TableauDef tdef = tableau(verlet ? 0 : cfg->tableau);
     Tableau tb;
     std::memset(&tb, 0, sizeof(tb));
     tb.s = tdef.s;
@@ -780,7 +784,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     } guard{exec, graph};
     const bool was_timing = c->timing;
     c->timing = false;
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 2 && !verlet; ++k) {
         const double* src = k == 0 ? bufA.p : bufB.p;
         double* dst = k == 0 ? bufB.p : bufA.p;
         SCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -972,6 +976,24 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             }
         }
 
+        if (verlet) {
+            // stepVelocityVerlet (simulation.hpp:1787-1798); it leaves the list's
+            // distances at the new positions, refreshed again after growth below
+            p_dt[0] = dt[0];
+            p_act[0] = 1;
+            SCHK(hipMemcpyAsync(d_dt.p, p_dt, sizeof(double), hipMemcpyHostToDevice, st));
+            SCHK(hipMemcpyAsync(d_active.p, p_act, sizeof(int), hipMemcpyHostToDevice, st));
+            abi(c, step_verlet(c, dt[0], cfg->rho, d_cur, n, ld, d_new, ld, d_bad.p));
+            ++attempts;
+            stage_updates += 2 * (int64_t)n;
+            SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+            if (p_bad[0]) {   // the run stops at the last finite state (simulation.hpp:1803-1822)
+                failed[0] = 1;
+                ++n_failed;
+                break;
+            }
+        } else {
         // adaptive RK step of every active replicate (simulation.hpp:1677-1769)
         abi(c, step_prepare(c, d_cur, n, ld));
         replay(act, dt);
@@ -1021,6 +1043,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         // a replicate that failed in the accepted attempt: redo the attempt
         // without it, so its rows are carried over unchanged
         while (check_failures(act, false)) replay(act, dt);
+        }
         std::swap(d_cur, d_new);
         abi(c, grow_batched(c, d_cur, ld, d_n.p, Rr, c->d_rep, d_dt.p, d_active.p));
         abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));

@@ -213,7 +213,9 @@ int bfm_dev_step_verlet(bfm_ctx* ctx, double dt, double rho, const double* d_cel
 /* ---- simulation driver (runSimulation, simulation.hpp:1106-2081) -------- */
 /* Parameters of the growth/division loop; the mechanics parameters come from
  * bfm_set_params. tableau: 0 Heun-Euler, 1 Bogacki-Shampine, 2 RKF45,
- * 3 Dormand-Prince (simulation.hpp:1207-1268). The random stream is a 32-bit
+ * 3 Dormand-Prince (simulation.hpp:1207-1268), or -1 for velocity Verlet
+ * (stepVelocityVerlet at the fixed dt = max_stepsize, simulation.hpp:1787-1798;
+ * one colony only, rho > 0). The random stream is a 32-bit
  * Mersenne twister seeded with `seed`; uniform draws are x * 2^-32 and normal
  * draws use Box-Muller as utils.hpp:648-655; draws follow the reference's
  * order (divideCells, growth.hpp:456-720). Noise is not supported (zero). */
@@ -231,6 +233,8 @@ typedef struct bfm_sim_config {
     double report_seconds;     /* > 0: a progress line on stderr at this wall interval
                                   (the reference prints per dt_write, simulation.hpp:2026-2046) */
     double max_wall_seconds;   /* > 0: also stop after this much wall time */
+    double rho;                /* cell density for tableau -1 (velocity Verlet): M0 / volume of
+                                  a cell of length L0 (simulation.hpp:1280-1290) */
 } bfm_sim_config;
 
 typedef struct bfm_sim_stats {
@@ -413,7 +417,7 @@ int bfm_load_config(const char* json_path, bfm_run_config* out);
  * BFM_JKR_EXACT. max_cells: capacity (0: 2 max(n_cells, initial) + 16).
  * verbose prints the reference's per-frame progress line on stdout (and a wall-clock
  * progress line on stderr every 30 s). Modes the
- * device loop lacks (anisotropic JKR, Verlet, basal_only, noise)
+ * device loop lacks (anisotropic JKR, basal_only, noise)
  * return BFM_ERR_UNSUPPORTED. stats_out may be NULL. */
 int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int device, int jkr_mode,
             int max_cells, int verbose, bfm_sim_stats* stats_out);

@@ -1623,8 +1623,11 @@ int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double
     case 1: A = BS_A; b = BS_b; bs = BS_bs; s = 4; order = 2; break;
     case 2: A = RKF_A; b = RKF_b; bs = RKF_bs; s = 6; order = 4; break;
     case 3: A = DP_A; b = DP_b; bs = DP_bs; s = 7; order = 4; break;
+    case -1: A = HE_A; b = HE_b; bs = HE_bs; s = 2; order = 1; break;   // velocity Verlet (no tableau)
     default: return BFM_ERR_ARG;
     }
+    const bool verlet = cfg->tableau == -1;
+    if (verlet && !(cfg->rho > 0)) return BFM_ERR_ARG;
     if (n_in < 1 || n_in > cap || cfg->iter_update_neighbors < 1 || cfg->iter_update_stepsize < 1)
         return BFM_ERR_ARG;
     std::vector<double> cells((size_t)cap * ncols, 0.0), cells_new((size_t)cap * ncols), errors;
@@ -1675,8 +1678,17 @@ int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double
             orc_divide(cfg, rng, t, cells.data(), n, cap, ncols, parents);
             build();
         }
-        double max_error = rk(dt);
-        if (iter % cfg->iter_update_stepsize == 0) {
+        if (verlet) {   // stepVelocityVerlet at the fixed dt (simulation.hpp:1787-1798)
+            packed.assign((size_t)n * ncols, 0.0);
+            for (int c = 0; c < ncols; ++c)
+                std::memcpy(&packed[(size_t)c * n], &cells[(size_t)c * cap], sizeof(double) * n);
+            packed_new.assign((size_t)n * ncols, 0.0);
+            oracle_step_verlet(dt, cfg->rho, packed.data(), n, n, nb.data(), m, ld_nb, nullptr, p,
+                               packed_new.data());
+            ++rk_steps;
+        }
+        double max_error = verlet ? 0.0 : rk(dt);
+        if (!verlet && iter % cfg->iter_update_stepsize == 0) {
             bool exceeded = max_error > 1.0;
             double factor = 0.9 * std::pow(1.0 / max_error, 1.0 / (order + 1));
             if (factor >= 10) factor = 10;

@@ -118,3 +118,32 @@ def test_device_step_equals_host_step_and_trajectory(orc, box):
     err = rel_err(got[:, COLS], x_ref[:, COLS])
     print(f"20-step rel_err={err:.3e}", flush=True)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("max_iter,tol", [(5000, 4e-12 * 5000), (6000, 1e-4)])
+def test_growth_loop_with_verlet_matches_oracle(orc, max_iter, tol):
+    """bfm_simulate with tableau -1 (integration_mode 0 of the driver): the
+    founder grown with velocity Verlet at the fixed dt = max_stepsize
+    (simulation.hpp:1787-1798), M0 = 0.01 (rho from a cell of length L0),
+    against the oracle's loop. Before the first division (5,000 iterations)
+    the state agrees to rounding (4e-12 per iteration). After it the Verlet
+    dynamics amplify rounding differences: on the oracle alone a 1e-15
+    perturbation of the founder grows to 1.3e-6 by iteration 6,000 and to O(1)
+    by 10,000 (orientation flips at n_z ~ 0), so at 6,000 the bar is identical
+    division events and ids and the state within 1e-4."""
+    from bfm import sim, Context
+    from bfm.core import make_params
+    cfg = sim.sim_config(max_iter=max_iter, tableau="velocity_verlet", seed=42, M0=0.01)
+    p = make_params()
+    c0 = sim.founder_cell()
+    print(f"founder growth with velocity Verlet, {max_iter} iterations", flush=True)
+    ctx = Context(64, max_pairs=32 * 64 + 1024, params=p)
+    got, parents, trace, stats = sim.simulate(ctx, cfg, c0, cap=64)
+    want, trace_o, iters_o, steps_o = orc.simulate(cfg, p, c0, cap=64)
+    assert stats["iterations"] == iters_o == max_iter
+    assert stats["divisions"] == (0 if max_iter == 5000 else 1)
+    assert got.shape == want.shape and np.array_equal(got[:, [0, 20]], want[:, [0, 20]])
+    for cols in (slice(1, 7), slice(7, 13), slice(13, 15)):
+        scale = max(1.0, np.max(np.abs(want[:, cols])))
+        assert np.max(np.abs(got[:, cols] - want[:, cols])) < tol * scale
+    assert np.all(trace == cfg.max_stepsize)

@@ -173,8 +173,7 @@ def test_load_config_json_syntax(tmp_path):
 def test_run_rejects_unsupported_modes_before_the_device(tmp_path):
     from bfm import io, BFMError
     from bfm.core import BFM_ERR_UNSUPPORTED
-    for over in ({"adhesion_mode": 2}, {"max_rxy_noise": 0.1}, {"integration_mode": 0, "M0": 1.0},
-                 {"basal_only": 1, "basal_min_overlap": 0.0}):
+    for over in ({"adhesion_mode": 2}, {"max_rxy_noise": 0.1}, {"basal_only": 1, "basal_min_overlap": 0.0}):
         p = _write_json(tmp_path / "u.json", _config(**over))
         with pytest.raises(BFMError) as e:
             io.run(p, tmp_path / "out", 42)
