@@ -380,8 +380,12 @@ __device__ inline void stage_bias(float* bs, const float* __restrict__ b, const 
 
 // Per-row contact radius for the JKR pairs of the neighbor list: rows with
 // overlap > 0 and gamma > 0 inside the envelope take the network's guess and
-// the polish; other JKR rows take the exact solve. Each wavefront takes 64
-// rows per iteration (four 16-row MFMA tiles).
+// the polish; other JKR rows take the exact solve at once. Each wavefront
+// reads 64 rows per iteration and appends the rows inside the envelope to a
+// wavefront-private LDS queue; the network runs on full 64-row queue slices
+// (and once on the remainder at the end), so its work follows the rows inside
+// the envelope (about a quarter of the rows of a dense colony) rather than
+// every block of 64 rows that holds one.
 __global__ void __launch_bounds__(256)
 k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int ld, const int* __restrict__ pi,
              const int* __restrict__ pj, const double* __restrict__ pdx, const double* __restrict__ pdy,
@@ -392,10 +396,32 @@ k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __shared__ __align__(16) float bs[260];
     __shared__ bf16x4 wl[36][64];
+    __shared__ int q_row[4][128];
+    __shared__ double q_delta[4][128], q_gamma[4][128];
     stage_frags(wl, w1, w2, w3);
     stage_bias(bs, bias, w4);
     const int m = *d_m;
     const int waves_total = gridDim.x * 4;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int qn = 0;   // queue fill, uniform across the wavefront
+    // network + polish for queue entries [0, cnt), cnt <= 64
+    auto run_queue = [&](int cnt) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool on = lane < cnt;
+        int kq = 0;
+        double dq = 0, gq = 0;
+        float x[4] = {0.f, 0.f, 0.f, 0.f};
+        if (on) {
+            kq = q_row[wave][lane];
+            dq = q_delta[wave][lane];
+            gq = q_gamma[wave][lane];
+            features_f32(dq, ra.R, gq, env, x);
+        }
+        const float y = mlp_forward64(wl, bs, x, lane);
+        if (on) radius[kq] = polish_radius_fast(y, dq, ra.Req, ra.E0, gq, ra.polish);
+    };
     for (int base = (blockIdx.x * 4 + wave) * 64; base < m; base += waves_total * 64) {
         const int k = base + lane;
         // rows of a build that overflowed its capacity (reported by status) may be stale
@@ -411,17 +437,41 @@ k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int 
             inside = valid && delta >= env.dlo * env.R && delta <= env.dhi * env.R && gamma >= env.glo &&
                      gamma <= env.ghi;
         }
-        float x[4] = {0.f, 0.f, 0.f, 0.f};
-        if (inside) features_f32(delta, ra.R, gamma, env, x);
-        float y = 0.f;
-        if (__ballot(inside)) y = mlp_forward64(wl, bs, x, lane);
-        if (valid) {
-            double r;
-            if (inside) r = polish_radius_fast(y, delta, ra.Req, ra.E0, gamma, ra.polish);
-            else r = jkr_radius_newton(delta, ra.Req, ra.E0, gamma, 0.0, 200, nullptr);
-            radius[k] = r;
+        if (valid && !inside) radius[k] = jkr_radius_newton(delta, ra.Req, ra.E0, gamma, 0.0, 200, nullptr);
+        const uint64_t mk = __ballot(inside);
+        if (inside) {
+            const int pos = qn + __popcll(mk & below);
+            q_row[wave][pos] = k;
+            q_delta[wave][pos] = delta;
+            q_gamma[wave][pos] = gamma;
+        }
+        qn += __popcll(mk);
+        if (qn >= 64) {
+            run_queue(64);
+            // move the overflow (< 64 entries) to the front
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int rest = qn - 64;
+            int r0 = 0;
+            double r1 = 0, r2 = 0;
+            if (lane < rest) {
+                r0 = q_row[wave][64 + lane];
+                r1 = q_delta[wave][64 + lane];
+                r2 = q_gamma[wave][64 + lane];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < rest) {
+                q_row[wave][lane] = r0;
+                q_delta[wave][lane] = r1;
+                q_gamma[wave][lane] = r2;
+            }
+            qn = rest;
         }
     }
+    if (qn > 0) run_queue(qn);
 }
 
 // Network evaluation for given (delta, gamma) samples (tests and reports):
@@ -712,7 +762,7 @@ const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, in
     ra.colidx_gamma = c->p.colidx_gamma;
     ra.polish = s->polish;
     bfm::launch_timed(c, "sur_radius", [&] {
-        hipLaunchKernelGGL(k_sur_radius, dim3(1280), dim3(256), 0, c->stream, c->d_m, d_cells, ld, c->pi, c->pj,
+        hipLaunchKernelGGL(k_sur_radius, dim3(1024), dim3(256), 0, c->stream, c->d_m, d_cells, ld, c->pi, c->pj,
                            c->pdx, c->pdy, c->pdz, ra, s->env, s->w1, s->w2, s->w3, s->w4, s->b, s->radius);
     });
     return s->radius;
