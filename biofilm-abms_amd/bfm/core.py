@@ -16,7 +16,7 @@ BFM_OK, BFM_ERR_ARG, BFM_ERR_CAPACITY, BFM_ERR_NAN, BFM_ERR_JKR_ROOT, BFM_ERR_HI
     BFM_ERR_UNSUPPORTED = range(7)
 _STATUS = {1: "argument", 2: "capacity", 3: "nan", 4: "jkr_root", 5: "hip", 6: "unsupported"}
 
-ADHESION_NONE, ADHESION_JKR_ISOTROPIC = 0, 1
+ADHESION_NONE, ADHESION_JKR_ISOTROPIC, ADHESION_JKR_ANISOTROPIC = 0, 1, 2
 JKR_EXACT, JKR_TABLE_1D, JKR_TABLE_2D, JKR_SURROGATE = 0, 1, 2, 3
 
 
@@ -48,10 +48,12 @@ def repulsion_prefactors(R, Rcell, E0, Ecell):
 def make_params(R=0.8, Rcell=0.5, E0=3900.0, Ecell=3.9e6, nz_threshold=1e-8,
                 cell_surface_coulomb_coeff=1.0, no_surface=False, adhesion=True,
                 jkr_mode=JKR_EXACT, ncols=None, friction=False, cell_cell_coulomb_coeff=1.0,
-                colidx_eta_cell_cell=None):
+                colidx_eta_cell_cell=None, anisotropic=False):
     """bfm_params. ncols defaults to 22 (the gamma column 21 is present) plus
     one friction-coefficient column when friction is on, which is then the
-    last column (22 with adhesion, as simulation.hpp:1176-1188 places it)."""
+    last column (22 with adhesion, as simulation.hpp:1176-1188 places it).
+    anisotropic: adhesion mode JKR_ANISOTROPIC (the table path; set the tables
+    with Context.set_aniso_jkr_tables)."""
     if ncols is None:
         ncols = 22 + int(bool(friction))
     if colidx_eta_cell_cell is None:
@@ -64,13 +66,60 @@ def make_params(R=0.8, Rcell=0.5, E0=3900.0, Ecell=3.9e6, nz_threshold=1e-8,
     p.cell_cell_coulomb_coeff = cell_cell_coulomb_coeff
     p.cell_surface_coulomb_coeff = cell_surface_coulomb_coeff
     p.no_surface = int(bool(no_surface))
-    p.adhesion_mode = ADHESION_JKR_ISOTROPIC if adhesion else ADHESION_NONE
+    p.adhesion_mode = (ADHESION_JKR_ANISOTROPIC if anisotropic else ADHESION_JKR_ISOTROPIC) \
+        if adhesion else ADHESION_NONE
     p.jkr_mode = jkr_mode
     p.colidx_gamma = COL["gamma"] if adhesion else -1
     p.friction_mode = 1 if friction else 0
     p.ncols = ncols
     p.colidx_eta_cell_cell = colidx_eta_cell_cell
     return p
+
+
+_dpt = C.POINTER(C.c_double)
+
+
+class AnisoTablesC(C.Structure):
+    """bfm_aniso_jkr_tables."""
+    _fields_ = [("theta", _dpt), ("n_theta", C.c_int), ("half_l", _dpt), ("n_half_l", C.c_int),
+                ("coords", _dpt), ("n_coords", C.c_int), ("curvature_rx", _dpt),
+                ("rx", _dpt), ("n_rx", C.c_int), ("phi", _dpt), ("n_phi", C.c_int),
+                ("delta", _dpt), ("n_delta", C.c_int), ("gamma", _dpt), ("n_gamma", C.c_int),
+                ("force_radius", _dpt)]
+
+
+class AnisoJKRTables:
+    """The tables of AnisotropicJKRDataFixedGamma / VariableGamma
+    (mechanics.hpp:199-327): grids theta, half_l, coords (centerline
+    coordinate / half_l), rx, phi, delta and gamma (None: fixed gamma);
+    curvature_rx[i_theta, i_half_l, i_coord] = Rx at the contact point;
+    force_radius[i_rx1, i_rx2, i_phi, i_delta(, i_gamma), 0/1] = (force,
+    contact radius). Combinations absent from the precomputed files are 0."""
+
+    def __init__(self, theta, half_l, coords, curvature_rx, rx, phi, delta, force_radius,
+                 gamma=None, fixed_gamma=None):
+        a = lambda x: np.ascontiguousarray(x, dtype=np.float64)   # noqa: E731
+        self.theta, self.half_l, self.coords = a(theta), a(half_l), a(coords)
+        self.rx, self.phi, self.delta = a(rx), a(phi), a(delta)
+        self.gamma = None if gamma is None else a(gamma)
+        self.curvature_rx = a(curvature_rx).reshape(len(self.theta), len(self.half_l),
+                                                    len(self.coords))
+        shape = (len(self.rx), len(self.rx), len(self.phi), len(self.delta)) + \
+            ((len(self.gamma),) if self.gamma is not None else ()) + (2,)
+        self.force_radius = a(force_radius).reshape(shape)
+        self.fixed_gamma = fixed_gamma
+
+    def cstruct(self):
+        t = AnisoTablesC()
+        for name in ("theta", "half_l", "coords", "rx", "phi", "delta"):
+            arr = getattr(self, name)
+            setattr(t, name, _ptr(arr))
+            setattr(t, "n_" + name, len(arr))
+        t.gamma = _ptr(self.gamma) if self.gamma is not None else None
+        t.n_gamma = 0 if self.gamma is None else len(self.gamma)
+        t.curvature_rx = _ptr(self.curvature_rx)
+        t.force_radius = _ptr(self.force_radius)
+        return t
 
 
 # Butcher tableaux, simulation.hpp:1207-1268 (name -> (A, b, bs, error_order)).
@@ -138,6 +187,9 @@ def load_library():
         "bfm_sync": [vp],
         "bfm_set_jkr_table": [vp, _dp, C.c_int, _dp, C.c_int, _dp],
         "bfm_set_surrogate": [vp, vp, vp, vp, vp, vp, vp, vp, vp, _dp],
+        "bfm_set_aniso_jkr_tables": [vp, C.POINTER(AnisoTablesC)],
+        "bfm_read_aniso_jkr_tables": [C.c_char_p, C.c_char_p, C.c_int,
+                                      C.POINTER(C.POINTER(AnisoTablesC)), _dp],
         "bfm_get_cell_neighbors": [vp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    _dp, C.c_int64, _i64p],
         "bfm_update_neighbor_distances": [vp, _dp, C.c_int, C.c_int, _dp, C.c_int64, C.c_int64],
@@ -192,6 +244,8 @@ def load_library():
     lib.bfm_last_error.restype = C.c_char_p
     lib.bfm_io_last_error.argtypes = []
     lib.bfm_io_last_error.restype = C.c_char_p
+    lib.bfm_free_aniso_jkr_tables.argtypes = [C.POINTER(AnisoTablesC)]
+    lib.bfm_free_aniso_jkr_tables.restype = None
     _LIB = lib
     return lib
 
@@ -250,6 +304,11 @@ class Context:
         g = None if gammas is None else np.ascontiguousarray(gammas, dtype=np.float64)
         self._check(self.lib.bfm_set_jkr_table(self.h, _ptr(ov), len(ov), _ptr(g),
                                                0 if g is None else len(g), _ptr(rad)))
+
+    def set_aniso_jkr_tables(self, tables):
+        """bfm_set_aniso_jkr_tables from an AnisoJKRTables."""
+        t = tables.cstruct()
+        self._check(self.lib.bfm_set_aniso_jkr_tables(self.h, C.byref(t)))
 
     def set_tuning(self, bin_scale=0.0, block_threads=0, kernel_variant=-1):
         self._check(self.lib.bfm_set_tuning(self.h, float(bin_scale), int(block_threads),

@@ -32,7 +32,15 @@ class RunConfig(C.Structure):
             "iter_update_stepsize", "iter_update_neighbors", "max_iter", "n_cells", "basal_only",
             "adhesion_mode", "friction_mode", "integration_mode", "no_surface", "n_mesh_overlap",
             "n_mesh_gamma")] + \
-        [("init_filename", C.c_char * 1024), ("lineage_filename", C.c_char * 1024)]
+        [("init_filename", C.c_char * 1024), ("lineage_filename", C.c_char * 1024),
+         ("precompute_jkr_forces", C.c_int), ("curvature_filename", C.c_char * 1024),
+         ("jkr_forces_filename", C.c_char * 1024)] + \
+        [(k, C.c_double) for k in (
+            "calibrate_endpoint_radii", "min_aspect_ratio", "ellipsoid_project_tol", "brent_tol",
+            "init_bracket_dx")] + \
+        [(k, C.c_int) for k in (
+            "n_mesh_theta", "n_mesh_half_l", "n_mesh_centerline_coords", "n_mesh_curvature_radii",
+            "ellipsoid_project_max_iter", "brent_max_iter", "n_tries_bracket")]
 
     def as_dict(self):
         out = {}
@@ -100,6 +108,33 @@ def readLineage(filename):
     out = np.zeros(n.value, dtype=np.int32)
     _check(lib.bfm_read_lineage(fn, out.ctypes.data_as(_ip), n.value, C.byref(n)))
     return out
+
+
+def readAnisotropicJKRTables(curvature_filename, forces_filename, variable_gamma=False):
+    """parseCurvatureRadiiTable + parseReducedJKRForceTable (variable_gamma
+    False: the force file's first line is the fixed gamma) or
+    parseJKRForceTable (simulation.hpp:565-1008), as an AnisoJKRTables."""
+    from .core import AnisoJKRTables, AnisoTablesC
+    lib = load_library()
+    h = C.POINTER(AnisoTablesC)()
+    g = C.c_double()
+    _check(lib.bfm_read_aniso_jkr_tables(str(curvature_filename).encode(),
+                                         str(forces_filename).encode(), int(bool(variable_gamma)),
+                                         C.byref(h), C.byref(g)))
+    try:
+        t = h.contents
+        arr = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy() if n else np.zeros(0)  # noqa: E731
+        ng = t.n_gamma
+        tabs = AnisoJKRTables(
+            arr(t.theta, t.n_theta), arr(t.half_l, t.n_half_l), arr(t.coords, t.n_coords),
+            arr(t.curvature_rx, t.n_theta * t.n_half_l * t.n_coords),
+            arr(t.rx, t.n_rx), arr(t.phi, t.n_phi), arr(t.delta, t.n_delta),
+            arr(t.force_radius, t.n_rx * t.n_rx * t.n_phi * t.n_delta * max(ng, 1) * 2),
+            gamma=arr(t.gamma, ng) if ng else None,
+            fixed_gamma=None if variable_gamma else g.value)
+    finally:
+        lib.bfm_free_aniso_jkr_tables(h)
+    return tabs
 
 
 def load_config(json_filename):

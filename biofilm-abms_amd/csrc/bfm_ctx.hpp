@@ -140,6 +140,15 @@ struct bfm_ctx {
     double* d_tab_ov = nullptr; double* d_tab_g = nullptr; double* d_tab_r = nullptr;
     int tab_nd = 0, tab_ng = 0;
 
+    // anisotropic JKR tables: the seven grids back to back (theta, half_l,
+    // coords, Rx, phi, delta, gamma), Rx at the contact point per (theta,
+    // half_l, coord), and (force, radius) per (Rx1, Rx2, phi, delta[, gamma])
+    double* d_an_grid = nullptr;
+    double* d_an_curv = nullptr;
+    double* d_an_fr = nullptr;
+    int an_n[7] = {0, 0, 0, 0, 0, 0, 0};
+    double* d_nst = nullptr;        // 3 x max_cells stage orientations (anisotropic tables set)
+
     // surrogate
     bfm::SurrogateWeights* sur = nullptr;
     double sur_env[4] = {0, 0, 0, 0};

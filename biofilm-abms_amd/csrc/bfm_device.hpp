@@ -260,6 +260,98 @@ __device__ __forceinline__ double table_radius_2d(const double* ov, int nd, cons
     return radius;
 }
 
+// Anisotropic JKR tables on the device (AnisotropicJKRDataFixedGamma /
+// AnisotropicJKRDataVariableGamma, mechanics.hpp:199-327). The seven grids
+// lie back to back in `grid` (offsets off[], sizes n[]: theta, half_l,
+// centerline coordinate, Rx, phi, overlap, gamma; n[6] == 0 for the fixed-
+// gamma table); curv holds Rx at the contact point per (theta, half_l,
+// coordinate), fr the (force, radius) pair per (Rx1, Rx2, phi, overlap
+// [, gamma]), row-major in that order.
+constexpr int kAnisoGrids = 7;
+constexpr int kAnisoMaxGrid = 8192;   // grid points in total (staged in LDS)
+struct AnisoJKR {
+    const double* grid;
+    const double* curv;
+    const double2* fr;
+    int n[kAnisoGrids];
+    int off[kAnisoGrids];
+    int total;
+};
+
+// nearestValue (utils.hpp:1169-1215) on a non-decreasing grid. The
+// reference's bisection stops at the interval v[i] <= x < v[i+1], i.e. the
+// last i with v[i] <= x; this bisection keeps v[lo] <= x < v[hi] and ends at
+// the same i, then takes the nearer end (ties to the upper one). NaN gives 0
+// (the reference throws; the non-finite forces that follow are reported).
+__device__ __forceinline__ int nearest_sorted(const double* v, int n, double x)
+{
+    if (!(x > v[0])) return 0;
+    if (x >= v[n - 1]) return n - 1;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (v[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return (x - v[lo] < v[hi] - x) ? lo : hi;
+}
+
+// acosSafe, utils.hpp:629-638.
+__device__ __forceinline__ double acos_safe(double x)
+{
+    if (x >= 1) return 0.0;
+    if (x <= -1) return kPi;
+    return acos(x);
+}
+
+// Eigen's 3-vector dot, (a0 b0 + a1 b1) + a2 b2, without contraction into
+// FMAs: the angles index tables, so they are formed as the CPU forms them.
+__device__ __forceinline__ double dot3_rn(const double* a, const double* b)
+{
+    return __dadd_rn(__dadd_rn(__dmul_rn(a[0], b[0]), __dmul_rn(a[1], b[1])), __dmul_rn(a[2], b[2]));
+}
+
+// Table lookup of forcesAnisotropicJKRLagrange (adhesion.hpp:734-840 fixed
+// gamma, :842-1046 variable gamma): the angles between the overlap vector and
+// each orientation folded into [0, pi/2], the principal radius Rx at each
+// contact point, the pair ordered Rx1 <= Rx2, the angle between the cells
+// folded into [0, pi/2], then (force, radius) at the nearest grid points.
+// g: the grids (an LDS copy of AnisoJKR::grid); delta: the capped overlap.
+__device__ __forceinline__ double2 aniso_jkr_lookup(const AnisoJKR& an, const double* g, const double d[3],
+                                                    double dist, const double n1[3], double hl1,
+                                                    const double n2[3], double hl2, double s, double t,
+                                                    double delta, double gamma)
+{
+    constexpr double kHalfPi = 1.57079632679489661923;
+    const double dn[3] = {d[0] / dist, d[1] / dist, d[2] / dist};
+    // (-d12n).n2 and (-d12n).(-n2) are -c2 and c2 exactly (sign flips are exact)
+    const double c1 = dot3_rn(dn, n1), c2 = dot3_rn(dn, n2);
+    double th1 = acos_safe(c1);
+    if (th1 > kHalfPi) th1 = acos_safe(-c1);
+    double th2 = acos_safe(-c2);
+    if (th2 > kHalfPi) th2 = acos_safe(c2);
+    const double* g_th = g + an.off[0];
+    const double* g_hl = g + an.off[1];
+    const double* g_co = g + an.off[2];
+    const int it1 = nearest_sorted(g_th, an.n[0], th1), it2 = nearest_sorted(g_th, an.n[0], th2);
+    const int ih1 = nearest_sorted(g_hl, an.n[1], hl1), ih2 = nearest_sorted(g_hl, an.n[1], hl2);
+    const int ic1 = nearest_sorted(g_co, an.n[2], fabs(s) / hl1);
+    const int ic2 = nearest_sorted(g_co, an.n[2], fabs(t) / hl2);
+    double rx1 = an.curv[((int64_t)it1 * an.n[1] + ih1) * an.n[2] + ic1];
+    double rx2 = an.curv[((int64_t)it2 * an.n[1] + ih2) * an.n[2] + ic2];
+    if (rx1 > rx2) { const double tmp = rx1; rx1 = rx2; rx2 = tmp; }
+    double phi = acos_safe(dot3_rn(n1, n2));
+    if (phi > kHalfPi) phi = kPi - phi;
+    const int ir1 = nearest_sorted(g + an.off[3], an.n[3], rx1);
+    const int ir2 = nearest_sorted(g + an.off[3], an.n[3], rx2);
+    const int ip = nearest_sorted(g + an.off[4], an.n[4], phi);
+    const int id = nearest_sorted(g + an.off[5], an.n[5], delta);
+    const int ng = an.n[6] > 0 ? an.n[6] : 1;
+    const int ig = an.n[6] > 0 ? nearest_sorted(g + an.off[6], an.n[6], gamma) : 0;
+    const int64_t q = ((((int64_t)ir1 * an.n[3] + ir2) * an.n[4] + ip) * an.n[5] + id) * ng + ig;
+    return an.fr[q];
+}
+
 // ---- surface integrals, integrals.hpp -------------------------------------
 __device__ __forceinline__ double phi_(double rz, double nz, double R, double s)
 {

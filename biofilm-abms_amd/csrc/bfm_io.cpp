@@ -15,6 +15,7 @@
 //   table of runSimulation, the growth loop on the device with a frame at
 //   every dt_write crossing, the final frame and the lineage file.
 #include <algorithm>
+#include <array>
 #include <cerrno>
 #include <cmath>
 #include <cstdint>
@@ -24,6 +25,8 @@
 #include <fstream>
 #include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -195,6 +198,210 @@ std::vector<int> read_lineage_file(const std::string& path)
     }
     return parents;
 }
+
+// ---- anisotropic JKR tables (simulation.hpp:550-1008) ----------------------
+// Fields of a tab-separated line, as successive getline(ss, token, '\t')
+// calls see them (a missing field reads as the empty string).
+std::vector<std::string> tab_fields(const std::string& line)
+{
+    std::vector<std::string> f;
+    size_t pos = 0;
+    while (true) {
+        const size_t tab = line.find('\t', pos);
+        f.push_back(line.substr(pos, tab == std::string::npos ? std::string::npos : tab - pos));
+        if (tab == std::string::npos) break;
+        pos = tab + 1;
+    }
+    return f;
+}
+
+// std::stod: leading white space, the longest number prefix; no number or a
+// value out of range is an error.
+double stod_like(const std::string& tok, const std::string& path, int64_t lineno)
+{
+    char* end = nullptr;
+    errno = 0;
+    const double v = std::strtod(tok.c_str(), &end);
+    if (end == tok.c_str() || errno == ERANGE)
+        io_fail(BFM_ERR_ARG, "bad number '" + tok + "' in " + path + " line " + std::to_string(lineno));
+    return v;
+}
+
+double field(const std::vector<std::string>& f, size_t k, const std::string& path, int64_t lineno)
+{
+    return stod_like(k < f.size() ? f[k] : std::string(), path, lineno);
+}
+
+struct AnisoTables {
+    std::vector<double> theta, half_l, coords, curv, rx, phi, delta, gamma, fr;
+    double fixed_gamma = 0;
+    bfm_aniso_jkr_tables view;
+};
+
+// parseCurvatureRadiiTable (simulation.hpp:565-660): lines (theta, half_l,
+// coord, Rx, Ry) enumerated theta-major. A new theta value starts the next
+// theta index (half_l and coord indices back to 0), else a new half_l value
+// the next half_l index, else the line is the next coordinate; the grids are
+// the theta values, the half_l values under the first theta and the
+// coordinates under the first (theta, half_l). The reference keeps (Rx, Ry)
+// per index triple in a hash map; only Rx enters the forces.
+void parse_curvature_table(const std::string& path, AnisoTables& T)
+{
+    std::ifstream in(path);
+    if (!in) io_fail(BFM_ERR_ARG, "cannot open " + path);
+    std::string line;
+    if (!std::getline(in, line)) io_fail(BFM_ERR_ARG, path + " is empty");
+    int64_t lineno = 1;
+    auto f = tab_fields(line);
+    double th = field(f, 0, path, lineno), hl = field(f, 1, path, lineno), co = field(f, 2, path, lineno);
+    double rx = field(f, 3, path, lineno);
+    field(f, 4, path, lineno);
+    T.theta.assign(1, th);
+    T.half_l.assign(1, hl);
+    T.coords.assign(1, co);
+    std::vector<std::pair<std::array<int, 3>, double>> ent;
+    ent.push_back({{0, 0, 0}, rx});
+    int ti = 0, hi = 0, ci = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        f = tab_fields(line);
+        const double th2 = field(f, 0, path, lineno), hl2 = field(f, 1, path, lineno);
+        const double co2 = field(f, 2, path, lineno);
+        rx = field(f, 3, path, lineno);
+        field(f, 4, path, lineno);
+        if (th2 != th) {
+            T.theta.push_back(th2);
+            ++ti;
+            hi = ci = 0;
+        } else if (hl2 != hl) {
+            if (ti == 0) T.half_l.push_back(hl2);
+            ++hi;
+            ci = 0;
+        } else {
+            if (ti == 0 && hi == 0) T.coords.push_back(co2);
+            ++ci;
+        }
+        ent.push_back({{ti, hi, ci}, rx});
+        th = th2;
+        hl = hl2;
+        co = co2;
+    }
+    const int nt = (int)T.theta.size(), nh = (int)T.half_l.size(), nc = (int)T.coords.size();
+    T.curv.assign((size_t)nt * nh * nc, 0.0);
+    // index triples outside the grids (ragged files) cannot be looked up
+    for (const auto& e : ent)
+        if (e.first[0] < nt && e.first[1] < nh && e.first[2] < nc)
+            T.curv[((size_t)e.first[0] * nh + e.first[1]) * nc + e.first[2]] = e.second;
+}
+
+// parseReducedJKRForceTable (simulation.hpp:662-836, first line: the fixed
+// gamma) and parseJKRForceTable (:838-1008, no header, a gamma column). The
+// Rx grid is the first column's values in order of first appearance, keyed
+// by the text of the field (a second-column text the first column never
+// holds maps to index 0, as the reference's map lookup inserts it). Lines are
+// enumerated (Rx1, Rx2, phi, delta[, gamma]); a new Rx1 or Rx2 value resets
+// the inner indices, a new phi value starts the next phi index, and so on;
+// the inner grids come from the first block.
+void parse_force_table(const std::string& path, bool variable_gamma, AnisoTables& T)
+{
+    std::ifstream in(path);
+    if (!in) io_fail(BFM_ERR_ARG, "cannot open " + path);
+    std::string line;
+    int64_t lineno = 0;
+    if (!variable_gamma) {
+        if (!std::getline(in, line)) io_fail(BFM_ERR_ARG, path + " is empty");
+        T.fixed_gamma = stod_like(line, path, ++lineno);
+    }
+    const int64_t first_row = lineno + 1;
+    std::unordered_map<std::string, int> rx_idx;
+    while (std::getline(in, line)) {
+        ++lineno;
+        const std::string tok = tab_fields(line)[0];
+        if (rx_idx.find(tok) == rx_idx.end()) {
+            T.rx.push_back(stod_like(tok, path, lineno));
+            rx_idx[tok] = (int)T.rx.size() - 1;
+        }
+    }
+    in.clear();
+    in.seekg(0);
+    lineno = 0;
+    if (!variable_gamma) { std::getline(in, line); ++lineno; }
+    if (!std::getline(in, line)) io_fail(BFM_ERR_ARG, path + " has no force rows");
+    lineno = first_row;
+    auto rx_index = [&](const std::string& tok) {
+        auto it = rx_idx.find(tok);
+        if (it == rx_idx.end()) { rx_idx[tok] = 0; return 0; }
+        return it->second;
+    };
+    const int nv = variable_gamma ? 7 : 6;
+    auto f = tab_fields(line);
+    std::vector<double> v(nv);
+    for (int k = 0; k < nv; ++k) v[k] = field(f, k, path, lineno);
+    int i1 = rx_index(f[0]), i2 = rx_index(f.size() > 1 ? f[1] : std::string());
+    T.phi.assign(1, v[2]);
+    T.delta.assign(1, v[3]);
+    if (variable_gamma) T.gamma.assign(1, v[4]);
+    std::vector<std::pair<std::array<int, 5>, std::array<double, 2>>> ent;
+    ent.push_back({{i1, i2, 0, 0, 0}, {v[nv - 2], v[nv - 1]}});
+    double r1 = v[0], r2 = v[1], ph = v[2], de = v[3], ga = variable_gamma ? v[4] : 0.0;
+    int pi = 0, di = 0, gi = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        f = tab_fields(line);
+        for (int k = 0; k < nv; ++k) v[k] = field(f, k, path, lineno);
+        i1 = rx_index(f[0]);
+        i2 = rx_index(f[1]);
+        if (v[0] != r1 || v[1] != r2) {
+            pi = di = gi = 0;
+        } else if (v[2] != ph) {
+            if (i1 == 0 && i2 == 0) T.phi.push_back(v[2]);
+            ++pi;
+            di = gi = 0;
+        } else if (!variable_gamma || v[3] != de) {
+            if (i1 == 0 && i2 == 0 && pi == 0) T.delta.push_back(v[3]);
+            ++di;
+            gi = 0;
+        } else {
+            if (i1 == 0 && i2 == 0 && pi == 0 && di == 0) T.gamma.push_back(v[4]);
+            ++gi;
+        }
+        ent.push_back({{i1, i2, pi, di, gi}, {v[nv - 2], v[nv - 1]}});
+        r1 = v[0];
+        r2 = v[1];
+        ph = v[2];
+        de = v[3];
+        if (variable_gamma) ga = v[4];
+    }
+    (void)ga;
+    const int nr = (int)T.rx.size(), np = (int)T.phi.size(), nd = (int)T.delta.size();
+    const int ng = variable_gamma ? (int)T.gamma.size() : 1;
+    T.fr.assign((size_t)nr * nr * np * nd * ng * 2, 0.0);
+    for (const auto& e : ent) {
+        const auto& k = e.first;
+        if (k[0] >= nr || k[1] >= nr || k[2] >= np || k[3] >= nd || k[4] >= ng) continue;
+        const size_t q = (((((size_t)k[0] * nr + k[1]) * np + k[2]) * nd + k[3]) * ng + k[4]) * 2;
+        T.fr[q] = e.second[0];
+        T.fr[q + 1] = e.second[1];
+    }
+}
+
+void aniso_view(AnisoTables& T, bool variable_gamma)
+{
+    bfm_aniso_jkr_tables& v = T.view;
+    v.theta = T.theta.data();   v.n_theta = (int)T.theta.size();
+    v.half_l = T.half_l.data(); v.n_half_l = (int)T.half_l.size();
+    v.coords = T.coords.data(); v.n_coords = (int)T.coords.size();
+    v.curvature_rx = T.curv.data();
+    v.rx = T.rx.data();         v.n_rx = (int)T.rx.size();
+    v.phi = T.phi.data();       v.n_phi = (int)T.phi.size();
+    v.delta = T.delta.data();   v.n_delta = (int)T.delta.size();
+    v.gamma = variable_gamma ? T.gamma.data() : nullptr;
+    v.n_gamma = variable_gamma ? (int)T.gamma.size() : 0;
+    v.force_radius = T.fr.data();
+}
+
+std::mutex g_aniso_mu;
+std::map<const bfm_aniso_jkr_tables*, std::unique_ptr<AnisoTables>> g_aniso;
 
 // ---- JSON (the subset Boost.JSON's parser accepts without extensions) ------
 struct JVal {
@@ -513,6 +720,38 @@ void load_config(const std::string& path, bfm_run_config* o)
         o->n_mesh_overlap = (int)c.opt_i64("adhesion_n_mesh_overlap", 100);
         o->n_mesh_gamma = (int)c.opt_i64("adhesion_n_mesh_gamma", 100);
     }
+    // anisotropic tables (abmConstAdhesion.cpp:133-138, 242-249)
+    o->precompute_jkr_forces = 0;
+    o->curvature_filename[0] = o->jkr_forces_filename[0] = '\0';
+    o->calibrate_endpoint_radii = 1;
+    o->min_aspect_ratio = 0.01;   // not read from the file (abmConstAdhesion.cpp:146)
+    o->ellipsoid_project_tol = 1e-8;
+    o->brent_tol = 1e-8;
+    o->init_bracket_dx = 1e-3;
+    o->n_mesh_theta = o->n_mesh_half_l = o->n_mesh_centerline_coords = o->n_mesh_curvature_radii = 100;
+    o->ellipsoid_project_max_iter = 100;
+    o->brent_max_iter = 1000;
+    o->n_tries_bracket = 5;
+    if (o->adhesion_mode == 2) {
+        o->precompute_jkr_forces = (int)c.i64("adhesion_precompute_jkr_forces");
+        o->calibrate_endpoint_radii = (double)c.opt_i64("adhesion_calibrate_endpoint_radii", 1);
+        o->n_mesh_theta = (int)c.opt_i64("adhesion_n_mesh_theta", 100);
+        o->n_mesh_half_l = (int)c.opt_i64("adhesion_n_mesh_half_l", 100);
+        o->n_mesh_centerline_coords = (int)c.opt_i64("adhesion_n_mesh_centerline_coords", 100);
+        o->n_mesh_curvature_radii = (int)c.opt_i64("adhesion_n_mesh_curvature_radii", 100);
+        o->ellipsoid_project_tol = c.opt_dbl("adhesion_ellipsoid_project_tol", 1e-8);
+        o->ellipsoid_project_max_iter = (int)c.opt_i64("adhesion_ellipsoid_project_max_iter", 100);
+        o->brent_tol = c.opt_dbl("adhesion_brent_tol", 1e-8);
+        o->brent_max_iter = (int)c.opt_i64("adhesion_brent_max_iter", 1000);
+        o->init_bracket_dx = c.opt_dbl("adhesion_init_bracket_dx", 1e-3);
+        o->n_tries_bracket = (int)c.opt_i64("adhesion_n_tries_bracket", 5);
+        if (o->precompute_jkr_forces == 0) {
+            copy_name(o->curvature_filename, sizeof(o->curvature_filename), c.string("adhesion_curvature_filename"),
+                      "adhesion_curvature_filename");
+            copy_name(o->jkr_forces_filename, sizeof(o->jkr_forces_filename),
+                      c.string("adhesion_jkr_forces_filename"), "adhesion_jkr_forces_filename");
+        }
+    }
     o->eta_cell_cell = o->friction_mode == 1 ? c.dbl("eta_cell_cell") : 0.0;
     o->cell_surface_coulomb_coeff = c.opt_dbl("cell_surface_coulomb_coeff", 1.0);
     o->cell_cell_coulomb_coeff = o->friction_mode == 1 ? c.opt_dbl("cell_cell_coulomb_coeff", 1.0) : 1.0;
@@ -576,6 +815,22 @@ std::map<std::string, std::string> frame_params(const bfm_run_config& o, int rng
         p["adhesion_jkr_aberth_tol"] = fmt_g(o.jkr_aberth_tol);
         p["adhesion_n_mesh_overlap"] = fmt_g((double)o.n_mesh_overlap);
         p["adhesion_n_mesh_gamma"] = fmt_g((double)o.n_mesh_gamma);
+    }
+    if (o.adhesion_mode == 2) {
+        // the anisotropic entries of adhesion_params (abmConstAdhesion.cpp:136-240)
+        p["adhesion_precompute_jkr_forces"] = fmt_g((double)o.precompute_jkr_forces);
+        p["adhesion_calibrate_endpoint_radii"] = fmt_g(o.calibrate_endpoint_radii);
+        p["adhesion_n_mesh_theta"] = fmt_g((double)o.n_mesh_theta);
+        p["adhesion_n_mesh_half_l"] = fmt_g((double)o.n_mesh_half_l);
+        p["adhesion_n_mesh_centerline_coords"] = fmt_g((double)o.n_mesh_centerline_coords);
+        p["adhesion_n_mesh_curvature_radii"] = fmt_g((double)o.n_mesh_curvature_radii);
+        p["adhesion_min_aspect_ratio"] = fmt_g(o.min_aspect_ratio);
+        p["adhesion_ellipsoid_project_tol"] = fmt_g(o.ellipsoid_project_tol);
+        p["adhesion_ellipsoid_project_max_iter"] = fmt_g((double)o.ellipsoid_project_max_iter);
+        p["adhesion_brent_tol"] = fmt_g(o.brent_tol);
+        p["adhesion_brent_max_iter"] = fmt_g((double)o.brent_max_iter);
+        p["adhesion_init_bracket_dx"] = fmt_g(o.init_bracket_dx);
+        p["adhesion_n_tries_bracket"] = fmt_g((double)o.n_tries_bracket);
     }
     p["cell_cell_friction_mode"] = std::to_string(o.friction_mode);
     p["cell_cell_coulomb_coeff"] = fmt_g(o.cell_cell_coulomb_coeff);
@@ -668,6 +923,28 @@ int bfm_load_config(const char* json_path, bfm_run_config* out)
     });
 }
 
+int bfm_read_aniso_jkr_tables(const char* curvature_path, const char* forces_path, int variable_gamma,
+                              bfm_aniso_jkr_tables** out, double* fixed_gamma_out)
+{
+    return guarded([&] {
+        if (!curvature_path || !forces_path || !out) io_fail(BFM_ERR_ARG, "bfm_read_aniso_jkr_tables: NULL argument");
+        auto T = std::make_unique<AnisoTables>();
+        parse_curvature_table(curvature_path, *T);
+        parse_force_table(forces_path, variable_gamma != 0, *T);
+        aniso_view(*T, variable_gamma != 0);
+        if (fixed_gamma_out) *fixed_gamma_out = T->fixed_gamma;
+        *out = &T->view;
+        std::lock_guard<std::mutex> lk(g_aniso_mu);
+        g_aniso[&T->view] = std::move(T);
+    });
+}
+
+void bfm_free_aniso_jkr_tables(bfm_aniso_jkr_tables* tables)
+{
+    std::lock_guard<std::mutex> lk(g_aniso_mu);
+    g_aniso.erase(tables);
+}
+
 int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int device, int jkr_mode, int max_cells,
             int verbose, bfm_sim_stats* stats_out)
 {
@@ -676,13 +953,17 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
         if (!json_path || !outprefix) io_fail(BFM_ERR_ARG, "bfm_run: need the config path and the output prefix");
         bfm_run_config o;
         load_config(json_path, &o);
-        if (o.adhesion_mode == 2) io_fail(BFM_ERR_UNSUPPORTED, "anisotropic JKR adhesion is not supported");
+        // runSimulation reads the anisotropic tables from the two files
+        // (simulation.hpp:1405-1441); computing them at start-up is not built
+        if (o.adhesion_mode == 2 && o.precompute_jkr_forces != 0)
+            io_fail(BFM_ERR_UNSUPPORTED,
+                    "anisotropic JKR tables computed at start-up (adhesion_precompute_jkr_forces != 0) are not supported");
         if (o.integration_mode == 0 && !(o.M0 > 0))
             io_fail(BFM_ERR_ARG, "Initial cell mass must be defined when using velocity Verlet");
         if (o.basal_only) io_fail(BFM_ERR_UNSUPPORTED, "basal_only is not supported");
         if (o.max_rxy_noise > 0 || o.max_rz_noise > 0 || o.max_nxy_noise > 0 || o.max_nz_noise > 0)
             io_fail(BFM_ERR_UNSUPPORTED, "noise in the growth loop is not supported");
-        if (jkr_mode != BFM_JKR_EXACT && jkr_mode != BFM_JKR_TABLE_1D)
+        if (o.adhesion_mode != 2 && jkr_mode != BFM_JKR_EXACT && jkr_mode != BFM_JKR_TABLE_1D)
             io_fail(BFM_ERR_ARG, "bfm_run: jkr_mode must be BFM_JKR_EXACT or BFM_JKR_TABLE_1D");
         const bool adhesion = o.adhesion_mode != 0, friction = o.friction_mode != 0;
         // extra columns: gamma (21 with adhesion), then the friction
@@ -729,6 +1010,14 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
         // (simulation.hpp:1354-1374), and the 1-D table of runSimulation (:1376-1405)
         double max_gamma = 0;
         std::vector<double> overlaps, radii;
+        std::unique_ptr<AnisoTables> aniso;
+        if (o.adhesion_mode == 2) {
+            // fixed gamma (no switching): the reduced force table
+            aniso = std::make_unique<AnisoTables>();
+            parse_curvature_table(o.curvature_filename, *aniso);
+            parse_force_table(o.jkr_forces_filename, false, *aniso);
+            aniso_view(*aniso, false);
+        }
         if (adhesion) {
             double resid = 0;
             const int grc = bfm_jkr_optimal_surface_energy_density(o.R, o.Rcell, o.E0, o.adhesion_eqdist, 100.0, 1e-6,
@@ -736,6 +1025,8 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
             if (grc != BFM_OK) io_fail(grc, "surface energy density for adhesion_eqdist could not be found");
             for (int i = 0; i < n0; ++i)
                 cells[(size_t)col_gamma * n0 + i] = cells[(size_t)kColGroup * n0 + i] == 1 ? max_gamma : 0.0;
+        }
+        if (o.adhesion_mode == 1) {
             const int nm = o.n_mesh_overlap;
             if (nm < 2) io_fail(BFM_ERR_ARG, "adhesion_n_mesh_overlap must be >= 2");
             overlaps.resize(nm);
@@ -771,14 +1062,17 @@ int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int devi
         prm.cell_cell_coulomb_coeff = o.cell_cell_coulomb_coeff;
         prm.cell_surface_coulomb_coeff = o.cell_surface_coulomb_coeff;
         prm.no_surface = o.no_surface;
-        prm.adhesion_mode = adhesion ? BFM_ADHESION_JKR_ISOTROPIC : BFM_ADHESION_NONE;
-        prm.jkr_mode = jkr_mode;
+        prm.adhesion_mode = o.adhesion_mode == 2 ? BFM_ADHESION_JKR_ANISOTROPIC
+                            : adhesion          ? BFM_ADHESION_JKR_ISOTROPIC
+                                                : BFM_ADHESION_NONE;
+        prm.jkr_mode = o.adhesion_mode == 2 ? BFM_JKR_EXACT : jkr_mode;
         prm.colidx_gamma = col_gamma;
         prm.friction_mode = friction ? 1 : 0;
         prm.ncols = ncols;
         prm.colidx_eta_cell_cell = col_eta;
         if ((rc2 = bfm_set_params(ctx, &prm)) != BFM_OK) io_fail(rc2, bfm_last_error(ctx));
-        if (adhesion && jkr_mode == BFM_JKR_TABLE_1D &&
+        if (aniso && (rc2 = bfm_set_aniso_jkr_tables(ctx, &aniso->view)) != BFM_OK) io_fail(rc2, bfm_last_error(ctx));
+        if (o.adhesion_mode == 1 && jkr_mode == BFM_JKR_TABLE_1D &&
             (rc2 = bfm_set_jkr_table(ctx, overlaps.data(), (int)overlaps.size(), nullptr, 0, radii.data())) != BFM_OK)
             io_fail(rc2, bfm_last_error(ctx));
 

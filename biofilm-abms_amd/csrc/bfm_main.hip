@@ -1179,22 +1179,34 @@ struct ForceParams {
 // multiplies the unit distance vector of the main term (Hertz, EPS-capped
 // Hertz or JKR term1 - term2), hard that of the separately accumulated
 // hard-core term (adhesive overload only; 0 otherwise). Rows without overlap
-// get 0 and are skipped by the gather.
+// get 0 and are skipped by the gather. ANISO: the anisotropic JKR tables
+// give the JKR force (forcesAnisotropicJKRLagrange, adhesion.hpp:734-1046;
+// the grids are staged in LDS, dynamic shared memory of an.total doubles);
+// the orientations come from nst (3 x ld_nst, an RK stage's) when given,
+// else from the cells.
+template <bool ANISO>
 __global__ void __launch_bounds__(256)
 k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int ld,
               const int* __restrict__ pi, const int* __restrict__ pj,
               const double* __restrict__ pdx, const double* __restrict__ pdy,
               const double* __restrict__ pdz, ForceParams fp,
               const double* __restrict__ tab_ov, const double* __restrict__ tab_g,
-              const double* __restrict__ tab_r, int tab_nd, int tab_ng,
+              const double* __restrict__ tab_r, int tab_nd, int tab_ng, AnisoJKR an,
+              const double* __restrict__ nst, int ld_nst,
               const double* __restrict__ ps, const double* __restrict__ pt,
               const double* __restrict__ sur_radius, double* __restrict__ rec,
               int* __restrict__ jcnt, int* __restrict__ status)
 {
+    extern __shared__ double s_grid[];
+    if (ANISO) {
+        for (int q = threadIdx.x; q < an.total; q += blockDim.x) s_grid[q] = an.grid[q];
+        __syncthreads();
+    }
     const int m = *d_m;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
         const int i = pi[k], j = pj[k];
-        if ((unsigned)i >= (unsigned)ld || (unsigned)j >= (unsigned)ld) {
+        const unsigned lim = nst ? (unsigned)min(ld, ld_nst) : (unsigned)ld;
+        if ((unsigned)i >= lim || (unsigned)j >= lim) {
             // a row of a build that overflowed its capacity (reported by status)
             double2* r = reinterpret_cast<double2*>(rec + (int64_t)k * 8);
             r[0] = r[1] = r[2] = r[3] = make_double2(0.0, 0.0);
@@ -1216,7 +1228,30 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
             } else {
                 const double gamma = dmin(cellv(cells, ld, i, fp.colidx_gamma),
                                           cellv(cells, ld, j, fp.colidx_gamma));
-                if (gamma > 0) {
+                if (gamma > 0 && ANISO) {
+                    double delta = overlap;
+                    if (fp.max_overlap >= 0 && delta > fp.max_overlap) delta = fp.max_overlap;
+                    const double d[3] = {dx, dy, dz};
+                    double n1[3], n2[3];
+                    if (nst) {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            n1[q] = nst[(int64_t)q * ld_nst + i];
+                            n2[q] = nst[(int64_t)q * ld_nst + j];
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            n1[q] = cellv(cells, ld, i, C_NX + q);
+                            n2[q] = cellv(cells, ld, j, C_NX + q);
+                        }
+                    }
+                    const double2 fr = aniso_jkr_lookup(an, s_grid, d, dist, n1, cellv(cells, ld, i, C_HALF_L),
+                                                        n2, cellv(cells, ld, j, C_HALF_L), ps[k], pt[k], delta,
+                                                        gamma);
+                    mm = fr.x;
+                    jkr = 1.0;
+                } else if (gamma > 0) {
                     double delta = overlap;
                     if (fp.max_overlap >= 0 && delta > fp.max_overlap) delta = fp.max_overlap;
                     double radius;
@@ -1934,11 +1969,36 @@ k_stage_z(int n, const int* __restrict__ dn, const double* __restrict__ cells, i
     }
 }
 
+// Stage orientations (3 x n) for the anisotropic JKR lookups of stage st > 0
+// (cellCellForces reads n from the stage cells, mechanics.hpp:1446-1449).
+// Cells of inactive replicates keep their orientation.
+__global__ void __launch_bounds__(256)
+k_stage_n(int n, const int* __restrict__ dn, const double* __restrict__ cells, int ld, const Tableau tb, int st,
+          double dt, const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
+          const double* __restrict__ kbuf, double* __restrict__ nst, int ld_nst)
+{
+    if (dn) n = *dn;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double y[6];
+        if (rep && !active[rep[c]]) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) y[3 + q] = cellv(cells, ld, c, C_NX + q);
+        } else {
+            stage_coords(cells, ld, n, c, kbuf, tb, st, rep ? dt_rep[rep[c]] : dt, y);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) nst[(int64_t)q * ld_nst + c] = y[3 + q];
+    }
+}
+
 // One RK stage st >= 0 of stepRungeKuttaAdaptive (mechanics.hpp:2534-2557):
 // cells_i = cells + dt * sum_j A(st, j) k_j, normalizeOrientations, then
 // getVelocities with the step's frozen neighbor geometry (the conservative
 // cell-cell forces cc are identical in every stage, because they only read
 // the neighbor rows; friction rows, when on, are recomputed per stage).
+// With the anisotropic JKR tables the forces also read the stage
+// orientations: rec (non-null) holds the stage's pair records and the
+// cell-cell forces are gathered from them instead of cc.
 // Batched replicates: dn (device cell count) replaces n, and rep / dt_rep /
 // active give each cell its replicate's step size; cells of inactive
 // replicates are skipped.
@@ -1947,8 +2007,8 @@ __global__ void __launch_bounds__(256)
 k_rk_stage(int n, const int* __restrict__ dn, const double* __restrict__ cells, int ld,
            const int* __restrict__ assume_2d, const double* __restrict__ noise, VelParams vp, Tableau tb, int st,
            double dt, const int* __restrict__ rep, const double* __restrict__ dt_rep,
-           const int* __restrict__ active, const double* __restrict__ cc, RowLists L, FricGather fg,
-           double* __restrict__ kbuf, int* __restrict__ status)
+           const int* __restrict__ active, const double* __restrict__ cc, const double* __restrict__ rec,
+           RowLists L, FricGather fg, double* __restrict__ kbuf, int* __restrict__ status)
 {
     if (dn) n = *dn;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
@@ -1965,8 +2025,12 @@ k_rk_stage(int n, const int* __restrict__ dn, const double* __restrict__ cells, 
         }
         const bool a2d = assume_2d[c] != 0;
         double ccl[6], nz6[6];
+        if (rec) {
+            gather_cc<CSR>(c, L, rec, vp.sc.R, ccl);
+        } else {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
+            for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
+        }
         if (fg.frec) {
             double fr[6];
             gather_fric<CSR>(c, L, fg.frec, fg.ps, fg.pt, fr);
@@ -2402,6 +2466,23 @@ ForceParams force_params(const bfm_ctx* c)
     return fp;
 }
 
+AnisoJKR aniso_params(const bfm_ctx* c)
+{
+    require(c->d_an_grid != nullptr, BFM_ERR_ARG, "anisotropic JKR tables not set (bfm_set_aniso_jkr_tables)");
+    AnisoJKR an;
+    an.grid = c->d_an_grid;
+    an.curv = c->d_an_curv;
+    an.fr = reinterpret_cast<const double2*>(c->d_an_fr);
+    int off = 0;
+    for (int k = 0; k < kAnisoGrids; ++k) {
+        an.n[k] = c->an_n[k];
+        an.off[k] = off;
+        off += c->an_n[k];
+    }
+    an.total = off;
+    return an;
+}
+
 VelParams vel_params(const bfm_ctx* c)
 {
     VelParams vp;
@@ -2543,9 +2624,18 @@ void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
     const bool need_t = !c->list_csr && !c->transposed;
     if (need_t)
         KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
-    KLAUNCH(c, "pair_forces", k_pair_forces, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj, c->pdx,
-            c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng, c->ps, c->pt,
-            sur_radius, c->d_rec, need_t ? c->d_jcnt : (int*)nullptr, c->d_status);
+    if (c->p.adhesion_mode == BFM_ADHESION_JKR_ANISOTROPIC) {
+        const AnisoJKR an = aniso_params(c);
+        KLAUNCH(c, "pair_forces", k_pair_forces<true>, kPairGrid, 256, sizeof(double) * an.total, c->d_m, d_cells,
+                ld, c->pi, c->pj, c->pdx, c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd,
+                c->tab_ng, an, (const double*)nullptr, 0, c->ps, c->pt, sur_radius, c->d_rec,
+                need_t ? c->d_jcnt : (int*)nullptr, c->d_status);
+    } else {
+        KLAUNCH(c, "pair_forces", k_pair_forces<false>, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj,
+                c->pdx, c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng,
+                AnisoJKR{}, (const double*)nullptr, 0, c->ps, c->pt, sur_radius, c->d_rec,
+                need_t ? c->d_jcnt : (int*)nullptr, c->d_status);
+    }
     if (!need_t) return;
     // lists imported from the caller: transposes for the merge-form gather
     scan(c, c->d_jcnt, c->d_joff, (int64_t)n + 1, "scan_j");
@@ -2609,12 +2699,12 @@ void launch_friction(bfm_ctx* c, const double* d_cells, int ld, const double* zs
 void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld)
 {
     const double* sur = nullptr;
-    if (c->p.adhesion_mode != BFM_ADHESION_NONE && c->p.jkr_mode == BFM_JKR_SURROGATE) {
+    if (c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC && c->p.jkr_mode == BFM_JKR_SURROGATE) {
         require(c->sur != nullptr, BFM_ERR_ARG, "surrogate weights not set (bfm_set_surrogate)");
         sur = bfm::surrogate_pair_radii(c, d_cells, ld, n);
     }
     if ((c->p.jkr_mode == BFM_JKR_TABLE_1D || c->p.jkr_mode == BFM_JKR_TABLE_2D) &&
-        c->p.adhesion_mode != BFM_ADHESION_NONE)
+        c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC)
         require(c->d_tab_ov != nullptr, BFM_ERR_ARG, "JKR table not set (bfm_set_jkr_table)");
     pair_forces_and_transpose(c, d_cells, ld, n, sur);
 }
@@ -2641,6 +2731,25 @@ void launch_stage(bfm_ctx* c, int n, const int* d_n, const double* d_cells, int 
 {
     RowLists L = row_lists(c);
     FricGather fg = fric_gather(c);
+    const double* rec = nullptr;
+    if (c->p.adhesion_mode == BFM_ADHESION_JKR_ANISOTROPIC) {
+        // the pair records at the stage orientations (stage 0: the cells'),
+        // recomputed in every attempt
+        const AnisoJKR an = aniso_params(c);
+        const double* nst = nullptr;
+        if (st > 0) {
+            hipLaunchKernelGGL(k_stage_n, dim3(grid), dim3(256), 0, c->stream, n, d_n, d_cells, ld, tb, st, dt,
+                               d_rep, d_dt, d_active, c->d_k, c->d_nst, c->max_cells);
+            HIPCHK(hipGetLastError());
+            nst = c->d_nst;
+        }
+        hipLaunchKernelGGL(k_pair_forces<true>, dim3(kPairGrid), dim3(256), sizeof(double) * an.total, c->stream,
+                           c->d_m, d_cells, ld, c->pi, c->pj, c->pdx, c->pdy, c->pdz, force_params(c),
+                           c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng, an, nst, c->max_cells,
+                           c->ps, c->pt, (const double*)nullptr, c->d_rec, (int*)nullptr, c->d_status);
+        HIPCHK(hipGetLastError());
+        rec = c->d_rec;
+    }
     if (fg.frec) {
         const double* zst = nullptr;
         if (st > 0) {
@@ -2664,19 +2773,19 @@ void launch_stage(bfm_ctx* c, int n, const int* d_n, const double* d_cells, int 
     if (timed) {
         if (c->list_csr)
             KLAUNCH(c, "rk_stage", k_rk_stage<true>, grid, 256, 0, n, d_n, d_cells, ld, c->d_assume2d, d_noise, vp,
-                    tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg, c->d_k, c->d_status);
+                    tb, st, dt, d_rep, d_dt, d_active, c->d_cc, rec, L, fg, c->d_k, c->d_status);
         else
             KLAUNCH(c, "rk_stage", k_rk_stage<false>, grid, 256, 0, n, d_n, d_cells, ld, c->d_assume2d, d_noise, vp,
-                    tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg, c->d_k, c->d_status);
+                    tb, st, dt, d_rep, d_dt, d_active, c->d_cc, rec, L, fg, c->d_k, c->d_status);
     } else {
         if (c->list_csr)
             hipLaunchKernelGGL(k_rk_stage<true>, dim3(grid), dim3(256), 0, c->stream, n, d_n, d_cells, ld,
-                               c->d_assume2d, d_noise, vp, tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg,
-                               c->d_k, c->d_status);
+                               c->d_assume2d, d_noise, vp, tb, st, dt, d_rep, d_dt, d_active, c->d_cc, rec, L,
+                               fg, c->d_k, c->d_status);
         else
             hipLaunchKernelGGL(k_rk_stage<false>, dim3(grid), dim3(256), 0, c->stream, n, d_n, d_cells, ld,
-                               c->d_assume2d, d_noise, vp, tb, st, dt, d_rep, d_dt, d_active, c->d_cc, L, fg,
-                               c->d_k, c->d_status);
+                               c->d_assume2d, d_noise, vp, tb, st, dt, d_rep, d_dt, d_active, c->d_cc, rec, L,
+                               fg, c->d_k, c->d_status);
         HIPCHK(hipGetLastError());
     }
 }
@@ -2988,7 +3097,8 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
-                    c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst};
+                    c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst,
+                    c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     bfm::surrogate_free(c);
@@ -3009,8 +3119,8 @@ int bfm_set_params(bfm_ctx* c, const bfm_params* p)
                 "invalid radii or modulus");
         require(p->friction_mode == 0 || p->friction_mode == 1, BFM_ERR_UNSUPPORTED,
                 "friction_mode must be 0 (none) or 1 (kinetic)");
-        require(p->adhesion_mode == BFM_ADHESION_NONE || p->adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC,
-                BFM_ERR_UNSUPPORTED, "only NONE and JKR_ISOTROPIC adhesion are supported");
+        require(p->adhesion_mode >= BFM_ADHESION_NONE && p->adhesion_mode <= BFM_ADHESION_JKR_ANISOTROPIC,
+                BFM_ERR_ARG, "invalid adhesion_mode");
         require(p->jkr_mode >= 0 && p->jkr_mode <= 3, BFM_ERR_ARG, "invalid jkr_mode");
         require(p->ncols >= 21 && p->ncols <= kMaxCols, BFM_ERR_ARG, "ncols must be in 21..32");
         if (p->adhesion_mode != BFM_ADHESION_NONE)
@@ -3069,6 +3179,52 @@ int bfm_set_jkr_table(bfm_ctx* c, const double* ov, int nd, const double* gm, in
         HIPCHK(hipMemcpy(c->d_tab_r, rad, sizeof(double) * nr, hipMemcpyHostToDevice));
         c->tab_nd = nd;
         c->tab_ng = ng;
+    });
+}
+
+int bfm_set_aniso_jkr_tables(bfm_ctx* c, const bfm_aniso_jkr_tables* t)
+{
+    return guarded(c, [&] {
+        require(t != nullptr, BFM_ERR_ARG, "tables is NULL");
+        const double* grids[kAnisoGrids] = {t->theta, t->half_l, t->coords, t->rx, t->phi, t->delta, t->gamma};
+        const int ns[kAnisoGrids] = {t->n_theta, t->n_half_l, t->n_coords, t->n_rx, t->n_phi, t->n_delta,
+                                     t->n_gamma};
+        int64_t total = 0;
+        for (int k = 0; k < kAnisoGrids; ++k) {
+            const bool optional = k == kAnisoGrids - 1;
+            require(ns[k] >= (optional ? 0 : 1) && (ns[k] == 0 || grids[k] != nullptr), BFM_ERR_ARG,
+                    "invalid anisotropic JKR grid");
+            for (int q = 0; q < ns[k]; ++q) {
+                require(std::isfinite(grids[k][q]), BFM_ERR_ARG, "non-finite anisotropic JKR grid value");
+                require(q == 0 || grids[k][q - 1] <= grids[k][q], BFM_ERR_ARG,
+                        "anisotropic JKR grids must be non-decreasing");
+            }
+            total += ns[k];
+        }
+        require(total <= kAnisoMaxGrid, BFM_ERR_ARG, "anisotropic JKR grids exceed 8192 points in total");
+        require(t->curvature_rx != nullptr && t->force_radius != nullptr, BFM_ERR_ARG,
+                "anisotropic JKR tables are NULL");
+        const int64_t n_curv = (int64_t)t->n_theta * t->n_half_l * t->n_coords;
+        const int64_t n_fr =
+            (int64_t)t->n_rx * t->n_rx * t->n_phi * t->n_delta * std::max(t->n_gamma, 1) * 2;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (double** q : {&c->d_an_grid, &c->d_an_curv, &c->d_an_fr}) {
+            if (*q) HIPCHK(hipFree(*q));
+            *q = nullptr;
+        }
+        if (!c->d_nst) dalloc(&c->d_nst, (size_t)c->max_cells * 3);
+        dalloc(&c->d_an_grid, (size_t)total);
+        dalloc(&c->d_an_curv, (size_t)n_curv);
+        dalloc(&c->d_an_fr, (size_t)n_fr);
+        int64_t off = 0;
+        for (int k = 0; k < kAnisoGrids; ++k) {
+            if (ns[k] > 0)
+                HIPCHK(hipMemcpy(c->d_an_grid + off, grids[k], sizeof(double) * ns[k], hipMemcpyHostToDevice));
+            off += ns[k];
+            c->an_n[k] = ns[k];
+        }
+        HIPCHK(hipMemcpy(c->d_an_curv, t->curvature_rx, sizeof(double) * n_curv, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_an_fr, t->force_radius, sizeof(double) * n_fr, hipMemcpyHostToDevice));
     });
 }
 

@@ -45,8 +45,14 @@ enum bfm_status {
     BFM_ERR_UNSUPPORTED = 6   /* mode not implemented behind this ABI */
 };
 
-/* Adhesion modes: AdhesionMode, mechanics.hpp:55-60. */
-enum bfm_adhesion_mode { BFM_ADHESION_NONE = 0, BFM_ADHESION_JKR_ISOTROPIC = 1 };
+/* Adhesion modes: AdhesionMode, mechanics.hpp:55-60. JKR_ANISOTROPIC takes
+ * the JKR force from the precomputed tables of bfm_set_aniso_jkr_tables (the
+ * reference's jkr_data path); jkr_mode does not apply to it. */
+enum bfm_adhesion_mode {
+    BFM_ADHESION_NONE = 0,
+    BFM_ADHESION_JKR_ISOTROPIC = 1,
+    BFM_ADHESION_JKR_ANISOTROPIC = 2
+};
 
 /* How the isotropic JKR contact radius is obtained for a contact pair. */
 enum bfm_jkr_mode {
@@ -94,6 +100,32 @@ int bfm_sync(bfm_ctx* ctx);
  * radii[i * n_gamma + j] (2-D). */
 int bfm_set_jkr_table(bfm_ctx* ctx, const double* overlaps, int n_overlap,
                       const double* gammas, int n_gamma, const double* radii);
+
+/* Anisotropic JKR tables (AnisotropicJKRDataFixedGamma / VariableGamma,
+ * mechanics.hpp:199-327; looked up by forcesAnisotropicJKRLagrange,
+ * adhesion.hpp:734-1046): the principal radius of curvature Rx at a contact
+ * point per (theta = angle between overlap vector and orientation, half_l,
+ * normalized centerline coordinate), and the JKR force and contact radius
+ * per (Rx1, Rx2, phi = angle between the cells, overlap[, gamma]). Grids are
+ * non-decreasing, at most 8192 points together; n_gamma == 0 selects the
+ * fixed-gamma (4-D) force table. Dense row-major tables:
+ *   curvature_rx[(i_theta * n_half_l + i_half_l) * n_coords + i_coord]
+ *   force_radius[2 * ((((i_rx1 * n_rx + i_rx2) * n_phi + i_phi) * n_delta
+ *                + i_delta) * max(n_gamma, 1) + i_gamma) + {0: force, 1: radius}]
+ * Combinations a precomputed file does not hold are 0 (what the reference's
+ * hash-map lookup returns for them). The tables are copied to the device. */
+typedef struct bfm_aniso_jkr_tables {
+    const double* theta;  int n_theta;
+    const double* half_l; int n_half_l;
+    const double* coords; int n_coords;
+    const double* curvature_rx;
+    const double* rx;     int n_rx;
+    const double* phi;    int n_phi;
+    const double* delta;  int n_delta;
+    const double* gamma;  int n_gamma;
+    const double* force_radius;
+} bfm_aniso_jkr_tables;
+int bfm_set_aniso_jkr_tables(bfm_ctx* ctx, const bfm_aniso_jkr_tables* tables);
 
 /* JKR surrogate network: 4 -> 64 -> 64 -> 64 -> 1 with SiLU, weights as BF16
  * bit patterns (uint16) row-major [out][in], biases FP32. envelope =
@@ -403,8 +435,29 @@ typedef struct bfm_run_config {
     int basal_only, adhesion_mode, friction_mode, integration_mode, no_surface;
     int n_mesh_overlap, n_mesh_gamma;
     char init_filename[1024], lineage_filename[1024];
+    /* adhesion_mode 2: adhesion_precompute_jkr_forces and, when it is 0, the
+     * table files (abmConstAdhesion.cpp:133-138, 242-249) */
+    int precompute_jkr_forces;
+    char curvature_filename[1024], jkr_forces_filename[1024];
+    /* adhesion_mode 2: the optional keys of the ellipsoid pipeline, with the
+     * driver's defaults, as the frames record them (abmConstAdhesion.cpp:139-240) */
+    double calibrate_endpoint_radii, min_aspect_ratio, ellipsoid_project_tol, brent_tol, init_bracket_dx;
+    int n_mesh_theta, n_mesh_half_l, n_mesh_centerline_coords, n_mesh_curvature_radii,
+        ellipsoid_project_max_iter, brent_max_iter, n_tries_bracket;
 } bfm_run_config;
 int bfm_load_config(const char* json_path, bfm_run_config* out);
+
+/* parseCurvatureRadiiTable and parseReducedJKRForceTable / parseJKRForceTable
+ * (simulation.hpp:565-1008): the two tab-separated files of the anisotropic
+ * JKR tables read into the dense tables of bfm_set_aniso_jkr_tables.
+ * variable_gamma = 0: the force file's first line is the fixed gamma
+ * (returned in *fixed_gamma_out, may be NULL) and its rows are (Rx1, Rx2,
+ * phi, overlap, force, radius); else no header and rows (Rx1, Rx2, phi,
+ * overlap, gamma, force, radius). *out stays valid until
+ * bfm_free_aniso_jkr_tables. */
+int bfm_read_aniso_jkr_tables(const char* curvature_path, const char* forces_path, int variable_gamma,
+                              bfm_aniso_jkr_tables** out, double* fixed_gamma_out);
+void bfm_free_aniso_jkr_tables(bfm_aniso_jkr_tables* tables);
 
 /* The driver main() of abmConstAdhesion.cpp:31-457 with runSimulation's output
  * (simulation.hpp:1444-1572, 2023-2079): reads the config, starts from the
@@ -414,11 +467,13 @@ int bfm_load_config(const char* json_path, bfm_run_config* out);
  * <outprefix>_init.txt, <outprefix>_iter<k>.txt whenever t passes a multiple
  * of dt_write, <outprefix>_final.txt and <outprefix>_lineage.txt. jkr_mode:
  * BFM_JKR_TABLE_1D (runSimulation's table of n_mesh_overlap exact radii) or
- * BFM_JKR_EXACT. max_cells: capacity (0: 2 max(n_cells, initial) + 16).
+ * BFM_JKR_EXACT (adhesion_mode 1; adhesion_mode 2 reads the anisotropic
+ * tables from adhesion_curvature_filename and adhesion_jkr_forces_filename).
+ * max_cells: capacity (0: 2 max(n_cells, initial) + 16).
  * verbose prints the reference's per-frame progress line on stdout (and a wall-clock
- * progress line on stderr every 30 s). Modes the
- * device loop lacks (anisotropic JKR, basal_only, noise)
- * return BFM_ERR_UNSUPPORTED. stats_out may be NULL. */
+ * progress line on stderr every 30 s). Modes the device loop lacks
+ * (anisotropic tables computed at start-up, basal_only, noise) return
+ * BFM_ERR_UNSUPPORTED. stats_out may be NULL. */
 int bfm_run(const char* json_path, const char* outprefix, int rng_seed, int device, int jkr_mode,
             int max_cells, int verbose, bfm_sim_stats* stats_out);
 

@@ -47,6 +47,12 @@ inline double norm3(double x, double y, double z) { return std::sqrt((x * x + y 
 // JKR table registered by oracle_set_jkr_table (the JKRData of the reference).
 std::vector<double> g_tab_overlaps, g_tab_gammas, g_tab_radii;
 
+// Anisotropic JKR tables registered by oracle_set_aniso_tables
+// (AnisotropicJKRDataFixedGamma / VariableGamma, mechanics.hpp:199-327).
+struct AnisoTab {
+    std::vector<double> theta, half_l, coords, curv, rx, phi, delta, gamma, fr;
+} g_an;
+
 }  // namespace
 
 extern "C" {
@@ -512,6 +518,83 @@ void oracle_set_jkr_table(const double* overlaps, int n_overlap, const double* g
     g_tab_radii.assign(radii, radii + (int64_t)n_overlap * std::max(n_gamma, 1));
 }
 
+void oracle_set_aniso_tables(const double* theta, int nt, const double* half_l, int nh,
+                             const double* coords, int nc, const double* curv, const double* rx, int nr,
+                             const double* phi, int np, const double* delta, int nd,
+                             const double* gamma, int ng, const double* fr)
+{
+    g_an.theta.assign(theta, theta + nt);
+    g_an.half_l.assign(half_l, half_l + nh);
+    g_an.coords.assign(coords, coords + nc);
+    g_an.curv.assign(curv, curv + (int64_t)nt * nh * nc);
+    g_an.rx.assign(rx, rx + nr);
+    g_an.phi.assign(phi, phi + np);
+    g_an.delta.assign(delta, delta + nd);
+    g_an.gamma.assign(gamma ? gamma : rx, gamma ? gamma + ng : rx);
+    g_an.fr.assign(fr, fr + (int64_t)nr * nr * np * nd * std::max(ng, 1) * 2);
+}
+
+/* acosSafe, utils.hpp:629-638. */
+static double acos_safe(double x)
+{
+    if (x >= 1) return 0;
+    else if (x <= -1) return kPi;
+    return std::acos(x);
+}
+
+static inline double dot3(const double* a, const double* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+static int nearest_or_zero(const std::vector<double>& v, double x)
+{
+    // nearestValue throws where its bisection fails (NaN); 0 keeps the oracle total
+    if (std::isnan(x)) return 0;
+    if (v.size() == 1) return 0;
+    return oracle_nearest_value(v.data(), (int)v.size(), x);
+}
+
+/* forcesAnisotropicJKRLagrange, table overloads (adhesion.hpp:734-840 with a
+ * fixed gamma, :842-1046 with a gamma grid): force magnitude (repulsion
+ * positive) and contact radius of a pair at distance vector d12. The hash-map
+ * lookups of the reference become reads of the dense tables, in which the
+ * combinations absent from the files hold 0 (what operator[] returns). */
+void oracle_aniso_jkr_force(const double* n1, double half_l1, const double* n2, double half_l2,
+                            const double* d12, double R, double gamma, double s, double t,
+                            double max_overlap, double* force_out, double* radius_out)
+{
+    *force_out = 0;
+    *radius_out = 0;
+    const double dist = norm3(d12[0], d12[1], d12[2]);
+    if (!(dist < 2 * R)) return;
+    const double d12n[3] = {d12[0] / dist, d12[1] / dist, d12[2] / dist};
+    double delta = 2 * R - dist;
+    if (max_overlap >= 0 && delta > max_overlap) delta = max_overlap;
+    const double halfpi = kPi / 2;
+    const double mn1[3] = {-n1[0], -n1[1], -n1[2]}, mn2[3] = {-n2[0], -n2[1], -n2[2]};
+    const double md[3] = {-d12n[0], -d12n[1], -d12n[2]};
+    double theta1 = acos_safe(dot3(d12n, n1));
+    if (theta1 > halfpi) theta1 = acos_safe(dot3(d12n, mn1));
+    double theta2 = acos_safe(dot3(md, n2));
+    if (theta2 > halfpi) theta2 = acos_safe(dot3(md, mn2));
+    const int it1 = nearest_or_zero(g_an.theta, theta1), it2 = nearest_or_zero(g_an.theta, theta2);
+    const int ih1 = nearest_or_zero(g_an.half_l, half_l1), ih2 = nearest_or_zero(g_an.half_l, half_l2);
+    const int is = nearest_or_zero(g_an.coords, std::abs(s) / half_l1);
+    const int itt = nearest_or_zero(g_an.coords, std::abs(t) / half_l2);
+    const int64_t nh = (int64_t)g_an.half_l.size(), nc = (int64_t)g_an.coords.size();
+    double Rx1 = g_an.curv[(it1 * nh + ih1) * nc + is];
+    double Rx2 = g_an.curv[(it2 * nh + ih2) * nc + itt];
+    if (Rx1 > Rx2) std::swap(Rx1, Rx2);
+    double phi = acos_safe(dot3(n1, n2));
+    if (phi > halfpi) phi = kPi - phi;
+    const int ir1 = nearest_or_zero(g_an.rx, Rx1), ir2 = nearest_or_zero(g_an.rx, Rx2);
+    const int ip = nearest_or_zero(g_an.phi, phi), id = nearest_or_zero(g_an.delta, delta);
+    const int64_t nr = (int64_t)g_an.rx.size(), np = (int64_t)g_an.phi.size(), nd = (int64_t)g_an.delta.size();
+    const int64_t ng = std::max<int64_t>((int64_t)g_an.gamma.size(), 1);
+    const int ig = g_an.gamma.empty() ? 0 : nearest_or_zero(g_an.gamma, gamma);
+    const int64_t q = ((((ir1 * nr + ir2) * np + ip) * nd + id) * ng + ig) * 2;
+    *force_out = g_an.fr[q];
+    *radius_out = g_an.fr[q + 1];
+}
+
 /* integrals.hpp:56-64 (phi). */
 static inline double phi_(double rz, double nz, double R, double s) { return R - rz - s * nz; }
 
@@ -765,7 +848,17 @@ static void cell_cell_forces_impl(const double* cells, int n, int ld, const doub
         } else {
             double gamma = std::min(cell(cells, ld, i, p->colidx_gamma),
                                     cell(cells, ld, j, p->colidx_gamma));
-            if (gamma > 0) {
+            if (gamma > 0 && p->adhesion_mode == BFM_ADHESION_JKR_ANISOTROPIC) {
+                // jkr_data->getJKRForces with anisotropic tables (mechanics.hpp:1468-1479)
+                double ni_[3], nj_[3], force, radius;
+                cell_n(cells, ld, i, ni_);
+                cell_n(cells, ld, j, nj_);
+                oracle_aniso_jkr_force(ni_, cell(cells, ld, i, C_HALF_L), nj_, cell(cells, ld, j, C_HALF_L),
+                                       dij, R, gamma, si, sj, max_overlap, &force, &radius);
+                // v = -force d12n; dEdq(i) gets -v = force d12n (adhesion.hpp:813-822)
+                for (int c = 0; c < 3; ++c) w[c] = force * dijn[c];
+                jkr = true;
+            } else if (gamma > 0) {
                 double delta = overlap;
                 if (max_overlap >= 0 && delta > max_overlap) delta = max_overlap;
                 double radius;

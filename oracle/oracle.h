@@ -75,6 +75,16 @@ double oracle_jkr_table_radius_2d(const double* overlaps, int nd, const double* 
 void oracle_set_jkr_table(const double* overlaps, int n_overlap, const double* gammas,
                           int n_gamma, const double* radii);
 
+/* Anisotropic JKR tables (dense, the layouts of bfm_aniso_jkr_tables) and the
+ * table form of forcesAnisotropicJKRLagrange, adhesion.hpp:734-1046. */
+void oracle_set_aniso_tables(const double* theta, int nt, const double* half_l, int nh,
+                             const double* coords, int nc, const double* curv, const double* rx, int nr,
+                             const double* phi, int np, const double* delta, int nd,
+                             const double* gamma, int ng, const double* fr);
+void oracle_aniso_jkr_force(const double* n1, double half_l1, const double* n2, double half_l2,
+                            const double* d12, double R, double gamma, double s, double t,
+                            double max_overlap, double* force_out, double* radius_out);
+
 /* Surface integrals, integrals.hpp:35-463. */
 double oracle_integral1(double rz, double nz, double R, double half_l, double g, double ss);
 double oracle_integral2(double rz, double nz, double R, double half_l, double g, double ss);

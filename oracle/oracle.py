@@ -56,6 +56,11 @@ def lib():
         L.oracle_jkr_table_radius_2d.restype = d
         L.oracle_set_jkr_table.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp]
         L.oracle_set_jkr_table.restype = None
+        L.oracle_set_aniso_tables.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp,
+                                              C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
+        L.oracle_set_aniso_tables.restype = None
+        L.oracle_aniso_jkr_force.argtypes = [_dp, d, _dp, d, _dp, d, d, d, d, d, _dp, _dp]
+        L.oracle_aniso_jkr_force.restype = None
         for nm in ("oracle_integral1", "oracle_integral2", "oracle_integral3"):
             getattr(L, nm).argtypes = [d] * 6
             getattr(L, nm).restype = d
@@ -189,6 +194,27 @@ def set_jkr_table(overlaps, radii, gammas=None):
     rad = np.ascontiguousarray(radii, dtype=np.float64)
     g = None if gammas is None else np.ascontiguousarray(gammas, dtype=np.float64)
     lib().oracle_set_jkr_table(_p(ov), len(ov), _p(g), 0 if g is None else len(g), _p(rad))
+
+
+def set_aniso_tables(t):
+    """Registers an AnisoJKRTables (bfm.core) with the oracle."""
+    a = lambda x: np.ascontiguousarray(x, dtype=np.float64)   # noqa: E731
+    keep = [a(t.theta), a(t.half_l), a(t.coords), a(t.curvature_rx), a(t.rx), a(t.phi),
+            a(t.delta), None if t.gamma is None else a(t.gamma), a(t.force_radius)]
+    th, hl, co, cv, rx, ph, de, ga, fr = keep
+    lib().oracle_set_aniso_tables(_p(th), len(th), _p(hl), len(hl), _p(co), len(co), _p(cv),
+                                  _p(rx), len(rx), _p(ph), len(ph), _p(de), len(de), _p(ga),
+                                  0 if ga is None else len(ga), _p(fr))
+
+
+def aniso_jkr_force(n1, half_l1, n2, half_l2, d12, R, gamma, s, t, max_overlap=-1.0):
+    """(force, radius) of forcesAnisotropicJKRLagrange's table form."""
+    a = lambda x: np.ascontiguousarray(x, dtype=np.float64)   # noqa: E731
+    n1, n2, d12 = a(n1), a(n2), a(d12)
+    f, r = C.c_double(), C.c_double()
+    lib().oracle_aniso_jkr_force(_p(n1), half_l1, _p(n2), half_l2, _p(d12), R, gamma, s, t,
+                                 max_overlap, C.byref(f), C.byref(r))
+    return f.value, r.value
 
 
 def table_radius_1d(overlaps, radii, delta):

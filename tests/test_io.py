@@ -173,8 +173,34 @@ def test_load_config_json_syntax(tmp_path):
 def test_run_rejects_unsupported_modes_before_the_device(tmp_path):
     from bfm import io, BFMError
     from bfm.core import BFM_ERR_UNSUPPORTED
-    for over in ({"adhesion_mode": 2}, {"max_rxy_noise": 0.1}, {"basal_only": 1, "basal_min_overlap": 0.0}):
+    for over in ({"adhesion_mode": 2, "adhesion_precompute_jkr_forces": 1}, {"max_rxy_noise": 0.1},
+                 {"basal_only": 1, "basal_min_overlap": 0.0}):
         p = _write_json(tmp_path / "u.json", _config(**over))
         with pytest.raises(BFMError) as e:
             io.run(p, tmp_path / "out", 42)
         assert e.value.status == BFM_ERR_UNSUPPORTED
+
+
+def test_anisotropic_config_keys(tmp_path):
+    """adhesion_mode 2 (abmConstAdhesion.cpp:133-249): the precompute flag is
+    required, the table file names when it is 0; the ellipsoid-pipeline keys
+    are optional with the driver's defaults."""
+    from bfm import io, BFMError
+    from bfm.core import BFM_ERR_ARG
+    p = _write_json(tmp_path / "a.json", _config(adhesion_mode=2))
+    with pytest.raises(BFMError) as e:
+        io.load_config(p)
+    assert e.value.status == BFM_ERR_ARG and "adhesion_precompute_jkr_forces" in str(e.value)
+    p = _write_json(tmp_path / "a.json", _config(adhesion_mode=2, adhesion_precompute_jkr_forces=0))
+    with pytest.raises(BFMError):
+        io.load_config(p)                  # the file names are then required
+    p = _write_json(tmp_path / "a.json", _config(
+        adhesion_mode=2, adhesion_precompute_jkr_forces=0, adhesion_curvature_filename="c.txt",
+        adhesion_jkr_forces_filename="f.txt", adhesion_n_mesh_theta=40, adhesion_brent_tol=1e-9,
+        adhesion_n_tries_bracket=2.0))
+    c = io.load_config(p)
+    assert (c.adhesion_mode, c.precompute_jkr_forces) == (2, 0)
+    assert c.curvature_filename == b"c.txt" and c.jkr_forces_filename == b"f.txt"
+    assert c.n_mesh_theta == 40 and c.n_mesh_half_l == 100 and c.brent_tol == 1e-9
+    assert c.n_tries_bracket == 5           # a double literal for an integer key keeps the default
+    assert c.min_aspect_ratio == 0.01 and c.calibrate_endpoint_radii == 1.0
