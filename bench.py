@@ -59,8 +59,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=200000)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--jkr", default="exact", choices=["exact", "table", "surrogate", "none"],
-                    help="JKR radius: exact FP64 solve, 1-D table, learned radius, or no adhesion (Hertz only)")
+    ap.add_argument("--jkr", default="exact", choices=["exact", "table", "surrogate", "aniso", "none"],
+                    help="JKR radius: exact FP64 solve, 1-D table, learned radius, anisotropic tables "
+                         "(adhesion_mode 2, synthetic tables), or no adhesion (Hertz only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU work for the sampled O(N^2) neighbor search")
@@ -87,13 +88,16 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(cells, thr, R, Ldiv, params, budget):
+def cpu_baseline(cells, thr, R, Ldiv, params, budget, aniso_tables=None):
     """Times the oracle on the same snapshot: C1 = getVelocities on a prebuilt
     list (all N cells, OpenMP at the reference sites), C2 = the serial O(N^2)
     getCellNeighbors, sampled on a block of rows centred on N/2 (row i costs
     O(i), so the block has the mean row cost) and scaled to N rows."""
     import numpy as np
     from oracle import oracle as orc
+
+    if aniso_tables is not None:
+        orc.set_aniso_tables(aniso_tables)
 
     n = cells.shape[0]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
@@ -151,9 +155,18 @@ def main():
     thr = 0.5 * (2 * R + Ldiv)
     cells, dome_r = bfm.make_dome(n, seed=args.seed)
     jkr_mode = {"exact": bfm.core.JKR_EXACT, "table": bfm.core.JKR_TABLE_1D,
-                "surrogate": bfm.core.JKR_SURROGATE, "none": bfm.core.JKR_EXACT}[args.jkr]
-    params = bfm.core.make_params(jkr_mode=jkr_mode, adhesion=args.jkr != "none")
+                "surrogate": bfm.core.JKR_SURROGATE, "none": bfm.core.JKR_EXACT,
+                "aniso": bfm.core.JKR_EXACT}[args.jkr]
+    params = bfm.core.make_params(jkr_mode=jkr_mode, adhesion=args.jkr != "none",
+                                  anisotropic=args.jkr == "aniso")
     ctx = bfm.Context(n, max_pairs=16 * n, params=params)
+    aniso = None
+    if args.jkr == "aniso":
+        # tables of the reference's shapes: 100 x 50 x 100 contact-point radii,
+        # 40 x 40 x 20 x 50 (force, radius) entries (25.6 MB)
+        from bfm.jkr import synthetic_aniso_tables
+        aniso = synthetic_aniso_tables(n_theta=100, n_half_l=50, n_coords=100, n_rx=40, n_phi=20, n_delta=50)
+        ctx.set_aniso_jkr_tables(aniso)
     if args.bin_scale or args.block or args.variant >= 0:
         ctx.set_tuning(args.bin_scale, args.block, args.variant)
     if args.jkr == "table":
@@ -225,7 +238,7 @@ def main():
     stage_bytes = n * 328 + m * 96
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cells, thr, R, Ldiv, params, args.cpu_budget)
+        cpu = cpu_baseline(cells, thr, R, Ldiv, params, args.cpu_budget, aniso)
     out = {
         "metric": "RK-stage cell-updates/s, 200k-cell 3D FP64",
         "value": value,
