@@ -1,0 +1,160 @@
+"""The 2-D model of ABM-2d/ABM-2d-cpp through the C ABI (bfm2d_set_params,
+bfm2d_get_cell_neighbors, bfm2d_get_velocities, bfm2d_step_rk), with the
+reference's function names: getCellNeighbors (mechanics.hpp:293-374 of
+ABM-2d), getVelocities (:1503-1708, no confinement) and
+stepRungeKuttaAdaptive (:1710-1817). Cells use the 2-D tree's column layout
+(include/indices.hpp of ABM-2d); neighbor rows are (i, j, dx, dy, s, t)."""
+import ctypes as C
+
+import numpy as np
+
+from .core import BFMError, Context, load_library  # noqa: F401
+
+# Column indices, include/indices.hpp of ABM-2d.
+COL2D = dict(id=0, rx=1, ry=2, nx=3, ny=4, drx=5, dry=6, dnx=7, dny=8, l=9, half_l=10, t0=11,
+             growth=12, eta0=13, eta1=14, maxeta1=15, group=16)
+NCOLS2D = 17
+ADHESION2D_NONE, ADHESION2D_JKR, ADHESION2D_KIHARA = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Params2D(C.Structure):
+    """bfm2d_params."""
+    _fields_ = [("R", C.c_double), ("Rcell", C.c_double), ("prefactors", C.c_double * 3),
+                ("surface_contact_density", C.c_double), ("adhesion_mode", C.c_int),
+                ("ncols", C.c_int), ("strength", C.c_double), ("mindist", C.c_double),
+                ("distance_exp", C.c_double)]
+
+
+def cell_cell_prefactors(R, Rcell, E0, Ecell, cap_exponent=2.5):
+    """The repulsion prefactors as the 2-D driver forms them
+    (simulation.hpp:231-234 of ABM-2d: 2.5 E0 sqrt(R), 2.5 E0 sqrt(R)
+    (2 (R - Rcell))^2.5, 2.5 Ecell sqrt(Rcell)). The force of the Hertz
+    potential (potentialHertz, adhesion.hpp:70-92 of ABM-2d) at the cap is
+    2.5 E0 sqrt(R) (2 (R - Rcell))^1.5, the exponent the reference's own
+    repulsion tests use (cap_exponent=1.5)."""
+    return (2.5 * E0 * np.sqrt(R), 2.5 * E0 * np.sqrt(R) * pow(2 * (R - Rcell), cap_exponent),
+            2.5 * Ecell * np.sqrt(Rcell))
+
+
+def make_params(R=0.8, Rcell=0.5, E0=3900.0, Ecell=3.9e6, sigma0=100.0, adhesion_mode=ADHESION2D_NONE,
+                strength=1.0, mindist=1.2, distance_exp=6.0, ncols=NCOLS2D, cap_exponent=2.5):
+    p = Params2D()
+    p.R, p.Rcell = R, Rcell
+    for k, v in enumerate(cell_cell_prefactors(R, Rcell, E0, Ecell, cap_exponent)):
+        p.prefactors[k] = v
+    p.surface_contact_density = pow(sigma0 * R * R / (4 * E0), 1. / 3.)
+    p.adhesion_mode = adhesion_mode
+    p.ncols = ncols
+    p.strength, p.mindist, p.distance_exp = strength, mindist, distance_exp
+    return p
+
+
+def _f64(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _lib():
+    lib = load_library()
+    if not getattr(lib, "_bfm2d_sigs", False):
+        vp = C.c_void_p
+        sig = {
+            "bfm2d_set_params": [vp, C.POINTER(Params2D)],
+            "bfm2d_get_cell_neighbors": [vp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         _dp, C.c_int64, C.POINTER(C.c_int64)],
+            "bfm2d_get_velocities": [vp, _dp, C.c_int, C.c_int, _dp, C.c_int64, C.c_int64, _ip, _dp, _dp],
+            "bfm2d_step_rk": [vp, _dp, _dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_int, _dp,
+                              C.c_int64, C.c_int64, _ip, _dp, _dp, C.c_int, _dp],
+        }
+        for name, args in sig.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        lib._bfm2d_sigs = True
+    return lib
+
+
+class Model2D:
+    """A context running the 2-D mechanics (its own bfm_ctx)."""
+
+    def __init__(self, max_cells, params, max_pairs=None, device=0):
+        self.ctx = Context(max_cells, max_pairs=max_pairs, device=device)
+        self.lib = _lib()
+        self.params = params
+        self.ctx._check(self.lib.bfm2d_set_params(self.ctx.h, C.byref(params)))
+
+    def getCellNeighbors(self, cells, neighbor_threshold, R, Ldiv):
+        cells = _f64(cells)
+        n = cells.shape[0]
+        cap = max(8 * n, 16)
+        while True:
+            out = np.zeros((cap, 6), order="F")
+            m = C.c_int64()
+            rc = self.lib.bfm2d_get_cell_neighbors(self.ctx.h, _ptr(cells), n, n, neighbor_threshold, R, Ldiv,
+                                                   _ptr(out), cap, C.byref(m))
+            if rc == 2 and m.value > cap:
+                cap = m.value
+                continue
+            self.ctx._check(rc)
+            return np.asfortranarray(out[:m.value])
+
+    def _adhere(self, nb, to_adhere):
+        if to_adhere is None:
+            to_adhere = np.ones(nb.shape[0], dtype=np.int32)
+        return np.ascontiguousarray(to_adhere, dtype=np.int32)
+
+    def getVelocities(self, cells, neighbors, to_adhere=None, noise=None):
+        cells, nb = _f64(cells), _f64(neighbors)
+        n = cells.shape[0]
+        ta = self._adhere(nb, to_adhere)
+        nz = None if noise is None else _f64(noise)
+        out = np.zeros((n, 4), order="F")
+        self.ctx._check(self.lib.bfm2d_get_velocities(
+            self.ctx.h, _ptr(cells), n, n, _ptr(nb), nb.shape[0], max(nb.shape[0], 1),
+            ta.ctypes.data_as(_ip), _ptr(nz), _ptr(out)))
+        return out
+
+    def stepRungeKuttaAdaptive(self, A, b, bs, cells, neighbors, dt, to_adhere=None, noise=None):
+        cells, nb = _f64(cells), _f64(neighbors)
+        n = cells.shape[0]
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        bs = np.ascontiguousarray(bs, dtype=np.float64)
+        ta = self._adhere(nb, to_adhere)
+        nz = None if noise is None else _f64(noise)
+        out = np.zeros((n, cells.shape[1]), order="F")
+        err = np.zeros((n, 4), order="F")
+        self.ctx._check(self.lib.bfm2d_step_rk(
+            self.ctx.h, _ptr(A), _ptr(b), _ptr(bs), len(b), dt, _ptr(cells), n, n, _ptr(nb), nb.shape[0],
+            max(nb.shape[0], 1), ta.ctypes.data_as(_ip), _ptr(nz), _ptr(out), n, _ptr(err)))
+        return out, err
+
+
+def random_layer(n, seed, spacing=(3.0, 1.4), tilt=0.3, hl_lo=0.4, hl_hi=1.2, eta0=0.7, eta1=720.0):
+    """n 2-D cells in rows (for tests and benchmarks): centers on a lattice
+    of the given spacing with jitter (0.2 along a row, 0.1 across), nearly
+    horizontal orientations (angle within +-tilt, either direction) and
+    half-lengths in [hl_lo, hl_hi], so that no two centerlines cross (a zero
+    distance has no direction) while neighbors along and across the rows are
+    in contact. The 2-D tree's columns."""
+    rng = np.random.default_rng(seed)
+    sx, sy = spacing
+    cols = max(1, int(np.ceil(np.sqrt(n * sy / sx))))
+    k = np.arange(n)
+    c = np.zeros((n, NCOLS2D), order="F")
+    c[:, 0] = k
+    c[:, 1] = (k % cols) * sx + rng.uniform(-0.2, 0.2, n)
+    c[:, 2] = (k // cols) * sy + rng.uniform(-0.1, 0.1, n)
+    th = rng.uniform(-tilt, tilt, n) + np.pi * (rng.uniform(0, 1, n) < 0.5)
+    c[:, 3], c[:, 4] = np.cos(th), np.sin(th)
+    hl = rng.uniform(hl_lo, hl_hi, n)
+    c[:, 9], c[:, 10] = 2 * hl, hl
+    c[:, 12], c[:, 13], c[:, 14], c[:, 15], c[:, 16] = 1.5, eta0, eta1, eta1, 1
+    c[:, 5:9] = rng.normal(size=(n, 4))
+    return c

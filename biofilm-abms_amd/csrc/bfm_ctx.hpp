@@ -73,6 +73,9 @@ struct bfm_ctx {
 
     bfm_params p{};
     bool params_set = false;
+    bfm2d_params p2d{};             // the 2-D model (bfm2d_set_params)
+    bool p2d_set = false;
+    int* d_adhere = nullptr;        // max_pairs to_adhere flags of the 2-D model (adhesion on)
     std::string err;
 
     // device status word and counters

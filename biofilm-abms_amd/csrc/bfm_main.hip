@@ -2617,6 +2617,8 @@ void build_neighbors_sync(bfm_ctx* c, const double* d_cells, int n, int ld, doub
 
 // Transposes of the current list: rows per cell as j (sorted), and as i when
 // the list is not known to be sorted by i. k_pair_forces counts the j rows.
+void build_transposes(bfm_ctx* c, int n);
+
 void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
                                const double* sur_radius)
 {
@@ -2637,7 +2639,13 @@ void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
                 need_t ? c->d_jcnt : (int*)nullptr, c->d_status);
     }
     if (!need_t) return;
-    // lists imported from the caller: transposes for the merge-form gather
+    build_transposes(c, n);
+}
+
+// Transposes of a list imported from the caller for the merge-form gather,
+// after d_jcnt holds the rows per cell as j.
+void build_transposes(bfm_ctx* c, int n)
+{
     scan(c, c->d_jcnt, c->d_joff, (int64_t)n + 1, "scan_j");
     KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
     KLAUNCH(c, "fill_perm", k_fill_perm, kPairGrid, 256, 0, c->d_m, c->pj, c->d_joff, c->d_jcnt, c->d_perm_j);
@@ -2923,6 +2931,261 @@ int guarded(bfm_ctx* c, F&& f)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// the 2-D model (ABM-2d/ABM-2d-cpp): getVelocities and stepRungeKuttaAdaptive
+// ---------------------------------------------------------------------------
+// 2-D cell columns (include/indices.hpp of ABM-2d).
+enum Col2D { D_RX = 1, D_RY = 2, D_NX = 3, D_NY = 4, D_DRX = 5, D_L = 9, D_HALF_L = 10, D_ETA0 = 13, D_ETA1 = 14 };
+
+struct Params2D {
+    double R, Rcell, p0, p1, p2, scd, strength, mindist, dexp;
+    int mode;
+};
+
+// Per-row magnitudes of cellCellRepulsiveForces and cellCellAdhesiveForces
+// (mechanics.hpp:448-694 of ABM-2d), which read only the frozen distance:
+// {dx, dy, s, t, repulsion prefactor, adhesion magnitude a (v = a d/|d|,
+// before the strength factor), repulsion on, adhesion on}. The torques read
+// the (stage) orientations and are formed in the per-cell gather.
+__global__ void __launch_bounds__(256)
+k2d_pair(const int* __restrict__ d_m, int n, const int* __restrict__ pi, const int* __restrict__ pj,
+         const double* __restrict__ pdx, const double* __restrict__ pdy, const double* __restrict__ ps,
+         const double* __restrict__ pt, const int* __restrict__ to_adhere, Params2D P, double* __restrict__ rec,
+         int* __restrict__ jcnt)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        double2* r = reinterpret_cast<double2*>(rec + (int64_t)k * 8);
+        const int i = pi[k], j = pj[k];
+        if ((unsigned)i >= (unsigned)n || (unsigned)j >= (unsigned)n) {
+            r[0] = r[1] = r[2] = r[3] = make_double2(0.0, 0.0);
+            continue;
+        }
+        if (jcnt) atomicAdd(&jcnt[j], 1);
+        const double dx = pdx[k], dy = pdy[k];
+        const double mag = sqrt(dx * dx + dy * dy);
+        const double overlap = 2 * P.R - mag;
+        const double cap = 2 * (P.R - P.Rcell);
+        double pref = 0;
+        if (overlap > 0 && overlap < cap) pref = P.p0 * pow(overlap, 1.5);
+        else if (overlap >= cap) pref = P.p1 + P.p2 * pow(overlap - cap, 1.5);
+        double a = 0, adh = 0;
+        if (P.mode != BFM2D_ADHESION_NONE && to_adhere[k] && overlap > 0) {
+            adh = 1;
+            if (P.mode == BFM2D_ADHESION_JKR) {
+                a = kPi * P.R;
+                a *= (mag <= P.mindist) ? (2 * P.R - P.mindist) : (2 * P.R - mag);
+            } else {
+                const double t1 = (mag <= P.mindist) ? 1.0 / pow(P.mindist, P.dexp + 1) : 1.0 / pow(mag, P.dexp + 1);
+                const double t2 = 1.0 / pow(2 * P.R, P.dexp + 1);
+                a = P.dexp * (t1 - t2);
+            }
+        }
+        r[0] = make_double2(dx, dy);
+        r[1] = make_double2(ps[k], pt[k]);
+        r[2] = make_double2(pref, a);
+        r[3] = make_double2(overlap > 0 ? 1.0 : 0.0, adh);
+    }
+}
+
+// A cell's coordinates at stage st (cells + dt sum_j A(st, j) k_j, then the
+// 2-D normalizeOrientations, mechanics.hpp:1458-1465 of ABM-2d: no flip).
+__device__ __forceinline__ void stage_coords_2d(const double* __restrict__ cells, int ld, int n, int c,
+                                                const double* __restrict__ kbuf, const Tableau& tb, int st,
+                                                double dt, double y[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q] = cellv(cells, ld, c, D_RX + q);
+    if (st == 0) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double mult = 0;
+        for (int j = 0; j < st; ++j) mult += kbuf[((int64_t)j * 4 + q) * n + c] * tb.A[st * tb.s + j];
+        y[q] += mult * dt;
+    }
+    const double nrm = sqrt(y[2] * y[2] + y[3] * y[3]);
+    y[2] /= nrm;
+    y[3] /= nrm;
+}
+
+// One stage of the 2-D stepRungeKuttaAdaptive (mechanics.hpp:1770-1792 of
+// ABM-2d), or getVelocities alone (tb.s == 0 form: st = 0, out = kbuf):
+// repulsion and adhesion summed per cell in the order of the rows (the
+// reference's two sequential loops), the constrained torque forms with the
+// stage orientation, (repulsion + adhesion) + noise, then -dE/dq divided by
+// the composite drag l (eta0 + eta1 scd / R) and l^3 (...) / 12.
+template <bool CSR>
+__global__ void __launch_bounds__(256)
+k2d_stage(int n, const double* __restrict__ cells, int ld, const double* __restrict__ noise, Params2D P, Tableau tb,
+          int st, double dt, const double* __restrict__ rec, RowLists L, double* __restrict__ kbuf,
+          int* __restrict__ status)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double y[4];
+        stage_coords_2d(cells, ld, n, c, kbuf, tb, st, dt, y);
+        const double nx = y[2], ny = y[3];
+        double rep[4] = {0, 0, 0, 0}, adh[4] = {0, 0, 0, 0};
+        for_rows<CSR>(c, L, [&](int row, bool role_i) {
+            const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)row * 8);
+            const double2 d = r[0], st2 = r[1], mg = r[2], fl = r[3];
+            if (fl.x == 0) return;   // no overlap: neither term
+            const double mag = sqrt(d.x * d.x + d.y * d.y);
+            const double ux = d.x / mag, uy = d.y / mag;
+            {
+                const double vx = mg.x * ux, vy = mg.x * uy;
+                const double w = nx * vx + ny * vy;
+                if (role_i) {
+                    const double s = st2.x;
+                    rep[0] += vx; rep[1] += vy;
+                    rep[2] += s * (-w * nx + vx);
+                    rep[3] += s * (-w * ny + vy);
+                } else {
+                    const double t = st2.y;
+                    rep[0] += -vx; rep[1] += -vy;
+                    rep[2] += t * (w * nx - vx);
+                    rep[3] += t * (w * ny - vy);
+                }
+            }
+            if (fl.y != 0) {
+                const double vx = mg.y * ux, vy = mg.y * uy;
+                const double w = nx * (-vx) + ny * (-vy);
+                double f[4];
+                if (role_i) {
+                    const double s = st2.x;
+                    f[0] = -vx; f[1] = -vy;
+                    f[2] = s * (-w * nx - vx);
+                    f[3] = s * (-w * ny - vy);
+                } else {
+                    const double t = st2.y;
+                    f[0] = vx; f[1] = vy;
+                    f[2] = t * (w * nx + vx);
+                    f[3] = t * (w * ny + vy);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) adh[q] += P.strength * f[q];
+            }
+        });
+        const double l = cellv(cells, ld, c, D_L);
+        const double drag = cellv(cells, ld, c, D_ETA0) + cellv(cells, ld, c, D_ETA1) * P.scd / P.R;
+        const double K = l * drag, Lr = l * l * l * drag / 12;
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double e = rep[q] + adh[q];
+            if (noise) e += noise[(int64_t)q * n + c];
+            const double v = -e / (q < 2 ? K : Lr);
+            kbuf[((int64_t)st * 4 + q) * n + c] = v;
+            bad |= !isfinite(v);
+        }
+        if (bad) atomicOr(status, ST_NAN);
+    }
+}
+
+// RK update of the 2-D step (mechanics.hpp:1794-1816 of ABM-2d).
+__global__ void __launch_bounds__(256)
+k2d_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau tb, double dt,
+          const double* __restrict__ kbuf, double* __restrict__ out, int ld_out, double* __restrict__ errors)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        for (int col = 0; col < ncols; ++col)
+            if (col < D_RX || col > D_DRX + 3) out[(int64_t)col * ld_out + c] = cellv(cells, ld, c, col);
+        double y[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double v1 = 0, v2 = 0;
+            for (int st = 0; st < tb.s; ++st) {
+                const double kv = kbuf[((int64_t)st * 4 + q) * n + c];
+                v1 += kv * tb.b[st];
+                v2 += kv * tb.bs[st];
+            }
+            const double d1 = v1 * dt, d2 = v2 * dt;
+            y[q] = cellv(cells, ld, c, D_RX + q) + d1;
+            errors[(int64_t)q * n + c] = d1 - d2;
+            out[(int64_t)(D_DRX + q) * ld_out + c] = v1;
+        }
+        const double nrm = sqrt(y[2] * y[2] + y[3] * y[3]);
+        y[2] /= nrm;
+        y[3] /= nrm;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[(int64_t)(D_RX + q) * ld_out + c] = y[q];
+    }
+}
+
+// Places 2-D cells in the z = 0 plane in the 3-D layout the neighbor search
+// reads (r, n, l, half_l).
+__global__ void k2d_embed(int n, const double* __restrict__ c2, int ld2, double* __restrict__ c3, int ld3)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        c3[(int64_t)C_ID * ld3 + c] = c2[c];
+        c3[(int64_t)C_RX * ld3 + c] = c2[(int64_t)D_RX * ld2 + c];
+        c3[(int64_t)C_RY * ld3 + c] = c2[(int64_t)D_RY * ld2 + c];
+        c3[(int64_t)C_RZ * ld3 + c] = 0.0;
+        c3[(int64_t)C_NX * ld3 + c] = c2[(int64_t)D_NX * ld2 + c];
+        c3[(int64_t)C_NY * ld3 + c] = c2[(int64_t)D_NY * ld2 + c];
+        c3[(int64_t)C_NZ * ld3 + c] = 0.0;
+        c3[(int64_t)C_L * ld3 + c] = c2[(int64_t)D_L * ld2 + c];
+        c3[(int64_t)C_HALF_L * ld3 + c] = c2[(int64_t)D_HALF_L * ld2 + c];
+    }
+}
+
+Params2D params_2d(const bfm_ctx* c)
+{
+    require(c->p2d_set, BFM_ERR_ARG, "bfm2d_set_params has not been called");
+    const bfm2d_params& q = c->p2d;
+    Params2D P;
+    P.R = q.R; P.Rcell = q.Rcell;
+    P.p0 = q.prefactors[0]; P.p1 = q.prefactors[1]; P.p2 = q.prefactors[2];
+    P.scd = q.surface_contact_density;
+    P.strength = q.strength; P.mindist = q.mindist; P.dexp = q.distance_exp;
+    P.mode = q.adhesion_mode;
+    return P;
+}
+
+// Host arrays in: cells to c->d_cells, the 6-column list as the internal
+// list (dz = 0), to_adhere and noise to device buffers; then the pair
+// records and the transposes of the list.
+void prepare_2d(bfm_ctx* c, const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                const int* to_adhere, const double* noise)
+{
+    require(n >= 0 && n <= c->max_cells, BFM_ERR_CAPACITY, "n exceeds max_cells of the context");
+    require(ld >= n, BFM_ERR_ARG, "ld < n");
+    const int ncols = c->p2d.ncols;
+    if (n > 0)
+        HIPCHK(hipMemcpy2DAsync(c->d_cells, sizeof(double) * c->max_cells, cells, sizeof(double) * ld,
+                                sizeof(double) * n, ncols, hipMemcpyHostToDevice, c->stream));
+    require(m >= 0 && m <= c->max_pairs, BFM_ERR_CAPACITY, "m exceeds max_pairs of the context");
+    require(m == 0 || ld_nb >= m, BFM_ERR_ARG, "ld of neighbors < m");
+    std::vector<double> nb7((size_t)7 * std::max<int64_t>(m, 1), 0.0);
+    const int src[7] = {0, 1, 2, 3, -1, 4, 5};
+    for (int q = 0; q < 7; ++q)
+        if (src[q] >= 0)
+            for (int64_t k = 0; k < m; ++k) nb7[(size_t)q * m + k] = nb[(size_t)src[q] * ld_nb + k];
+    import_host_neighbors(c, nb7.data(), m, std::max<int64_t>(m, 1), n);
+    const bool adh = c->p2d.adhesion_mode != BFM2D_ADHESION_NONE;
+    require(!adh || m == 0 || to_adhere != nullptr, BFM_ERR_ARG, "to_adhere is required with adhesion");
+    if (adh && m > 0)
+        HIPCHK(hipMemcpyAsync(c->d_adhere, to_adhere, sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+    if (noise && n > 0)
+        HIPCHK(hipMemcpyAsync(c->d_noise, noise, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));   // the staged host arrays (nb7) end with this call
+    const Params2D P = params_2d(c);
+    KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
+    KLAUNCH(c, "pair_2d", k2d_pair, kPairGrid, 256, 0, c->d_m, n, c->pi, c->pj, c->pdx, c->pdy, c->ps, c->pt,
+            c->d_adhere, P, c->d_rec, c->d_jcnt);
+    build_transposes(c, n);
+}
+
+// The s stages of the 2-D step (s == 1 with tb.A = 0: getVelocities).
+void stages_2d(bfm_ctx* c, const Tableau& tb, double dt, int n, const double* d_noise)
+{
+    const Params2D P = params_2d(c);
+    const RowLists L = row_lists(c);
+    for (int st = 0; st < tb.s; ++st)
+        KLAUNCH(c, "stage_2d", k2d_stage<false>, grid_for(n, 256), 256, 0, n, c->d_cells, c->max_cells, d_noise,
+                P, tb, st, dt, c->d_rec, L, c->d_k, c->d_status);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3098,7 +3361,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst,
-                    c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst};
+                    c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     bfm::surrogate_free(c);
@@ -3585,6 +3848,99 @@ int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads, int kernel_v
                     BFM_ERR_ARG, "block_threads must be 64, 128, 256 or 512");
             c->block_threads = block_threads;
         }
+    });
+}
+
+// ---- the 2-D model ----------------------------------------------------------
+
+int bfm2d_set_params(bfm_ctx* c, const bfm2d_params* p)
+{
+    return guarded(c, [&] {
+        require(p != nullptr, BFM_ERR_ARG, "params is NULL");
+        require(p->R > 0 && p->Rcell > 0 && p->Rcell <= p->R, BFM_ERR_ARG, "invalid radii");
+        require(p->ncols >= 17 && p->ncols <= kMaxCols, BFM_ERR_ARG, "ncols must be in 17..32");
+        require(p->adhesion_mode >= BFM2D_ADHESION_NONE && p->adhesion_mode <= 3, BFM_ERR_ARG,
+                "invalid adhesion_mode");
+        require(p->adhesion_mode != 3, BFM_ERR_UNSUPPORTED, "GBK adhesion is not supported");
+        if (p->adhesion_mode != BFM2D_ADHESION_NONE && !c->d_adhere) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            dalloc(&c->d_adhere, (size_t)c->max_pairs);
+        }
+        c->p2d = *p;
+        c->p2d_set = true;
+    });
+}
+
+int bfm2d_get_cell_neighbors(bfm_ctx* c, const double* cells, int n, int ld, double thr, double R, double Ldiv,
+                             double* out, int64_t ld_out, int64_t* m_out)
+{
+    return guarded(c, [&] {
+        require(c->p2d_set, BFM_ERR_ARG, "bfm2d_set_params has not been called");
+        require(n >= 0 && n <= c->max_cells, BFM_ERR_CAPACITY, "n exceeds max_cells of the context");
+        require(ld >= n, BFM_ERR_ARG, "ld < n");
+        require(m_out != nullptr, BFM_ERR_ARG, "m_out is NULL");
+        if (n > 0)
+            HIPCHK(hipMemcpy2DAsync(c->d_cells, sizeof(double) * c->max_cells, cells, sizeof(double) * ld,
+                                    sizeof(double) * n, c->p2d.ncols, hipMemcpyHostToDevice, c->stream));
+        KLAUNCH(c, "embed_2d", k2d_embed, grid_for(n, 256), 256, 0, n, c->d_cells, c->max_cells, c->d_cells_new,
+                c->max_cells);
+        build_neighbors_sync(c, c->d_cells_new, n, c->max_cells, thr, R, Ldiv);
+        int m = 0;
+        HIPCHK(hipMemcpy(&m, c->d_m, sizeof(int), hipMemcpyDeviceToHost));
+        c->h_m = m;
+        *m_out = m;
+        require(m <= ld_out, BFM_ERR_CAPACITY, "neighbors_out is too small (see m_out)");
+        if (m > 0) {
+            KLAUNCH(c, "export_nb", k_export_nb, grid_for(m, 256, 4096), 256, 0, c->d_m, c->d_nb, (int64_t)m, c->pi,
+                    c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+            std::vector<double> nb7((size_t)7 * m);
+            HIPCHK(hipMemcpyAsync(nb7.data(), c->d_nb, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            const int src[6] = {0, 1, 2, 3, 5, 6};
+            for (int q = 0; q < 6; ++q)
+                std::memcpy(out + (size_t)q * ld_out, nb7.data() + (size_t)src[q] * m, sizeof(double) * m);
+        }
+        check_status(c);
+    });
+}
+
+int bfm2d_get_velocities(bfm_ctx* c, const double* cells, int n, int ld, const double* nb, int64_t m,
+                         int64_t ld_nb, const int* to_adhere, const double* noise, double* vel)
+{
+    return guarded(c, [&] {
+        require(c->p2d_set, BFM_ERR_ARG, "bfm2d_set_params has not been called");
+        require(vel != nullptr, BFM_ERR_ARG, "velocities_out is NULL");
+        prepare_2d(c, cells, n, ld, nb, m, ld_nb, to_adhere, noise);
+        Tableau tb;
+        std::memset(&tb, 0, sizeof(tb));
+        tb.s = 1;
+        stages_2d(c, tb, 0.0, n, noise ? c->d_noise : nullptr);
+        if (n > 0)
+            HIPCHK(hipMemcpyAsync(vel, c->d_k, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+        check_status(c);
+    });
+}
+
+int bfm2d_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs, int s, double dt,
+                  const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                  const int* to_adhere, const double* noise, double* cells_out, int ld_out, double* errors)
+{
+    return guarded(c, [&] {
+        require(c->p2d_set, BFM_ERR_ARG, "bfm2d_set_params has not been called");
+        require(A && b && bs && cells_out && errors, BFM_ERR_ARG, "NULL argument");
+        require(ld_out >= n, BFM_ERR_ARG, "ld_out < n");
+        const Tableau tb = make_tableau(A, b, bs, s);
+        prepare_2d(c, cells, n, ld, nb, m, ld_nb, to_adhere, noise);
+        stages_2d(c, tb, dt, n, noise ? c->d_noise : nullptr);
+        KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, c->d_cells, c->max_cells, tb,
+                dt, c->d_k, c->d_cells_new, c->max_cells, c->d_err6);
+        if (n > 0) {
+            HIPCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * ld_out, c->d_cells_new,
+                                    sizeof(double) * c->max_cells, sizeof(double) * n, c->p2d.ncols,
+                                    hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(errors, c->d_err6, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+        }
+        check_status(c);
     });
 }
 

@@ -390,6 +390,58 @@ int bfm_stats(bfm_ctx* ctx, int64_t* m, int64_t* screened, int64_t* tested, doub
  * No knob changes the neighbor list, only the time to build it. */
 int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads, int kernel_variant);
 
+/* ---- the 2-D model (ABM-2d/ABM-2d-cpp) ------------------------------------
+ * Cells: the 2-D tree's column-major layout (include/indices.hpp of ABM-2d):
+ * id 0, r 1-2, n 3-4, dr 5-6, dn 7-8, l 9, half_l 10, t0 11, growth 12,
+ * eta0 13, eta1 14, maxeta1 15, group 16 (ncols >= 17). Neighbor lists are
+ * m x 6: i, j, dx, dy, s_i, s_j (getCellNeighbors, mechanics.hpp:293-374 of
+ * ABM-2d). to_adhere: one int per neighbor row (the driver's to_adhere). */
+enum bfm2d_adhesion_mode {
+    BFM2D_ADHESION_NONE = 0,
+    BFM2D_ADHESION_JKR = 1,      /* forcesJKRLagrange<T, 2> (adhesion.hpp:402-462 of ABM-2d) */
+    BFM2D_ADHESION_KIHARA = 2    /* forcesKiharaLagrange<T, 2> (adhesion.hpp:334-400 of ABM-2d) */
+    /* GBK (3) is not built */
+};
+typedef struct bfm2d_params {
+    double R, Rcell;
+    double prefactors[3];            /* cell_cell_prefactors of cellCellRepulsiveForces
+                                        (the driver forms them at simulation.hpp:231-234) */
+    double surface_contact_density;  /* (sigma0 R^2 / (4 E0))^(1/3), simulation.hpp:237 */
+    int adhesion_mode;               /* enum bfm2d_adhesion_mode */
+    int ncols;                       /* columns of the cells array (>= 17) */
+    double strength, mindist, distance_exp;   /* adhesion_params "strength",
+                                        "mindist", "distance_exp" */
+} bfm2d_params;
+
+/* The 2-D mechanics use the context's device buffers; bfm2d_set_params may
+ * be called on a context with or without bfm_set_params. */
+int bfm2d_set_params(bfm_ctx* ctx, const bfm2d_params* params);
+
+/* getCellNeighbors of the 2-D tree (mechanics.hpp:293-374 of ABM-2d): pairs
+ * j < i whose segment distance is below thr, rows (i asc, j asc). The
+ * reference takes the distance of non-parallel cells from CGAL's segment
+ * distance; here it is the closest-point computation of the 3-D path on the
+ * cells placed in the z = 0 plane (the parallel branch is the reference's). */
+int bfm2d_get_cell_neighbors(bfm_ctx* ctx, const double* cells, int n, int ld, double thr, double R,
+                             double Ldiv, double* neighbors_out, int64_t ld_out, int64_t* m_out);
+
+/* getVelocities of the 2-D tree (mechanics.hpp:1503-1708 of ABM-2d) without
+ * confinement (ConfinementMode::NONE): repulsion + adhesion + noise (n x 4,
+ * may be NULL) divided by the composite drag prefactors. velocities_out: n x 4. */
+int bfm2d_get_velocities(bfm_ctx* ctx, const double* cells, int n, int ld, const double* neighbors,
+                         int64_t m, int64_t ld_nb, const int* to_adhere, const double* noise,
+                         double* velocities_out);
+
+/* stepRungeKuttaAdaptive of the 2-D tree (mechanics.hpp:1710-1817 of
+ * ABM-2d): s stages with the step's neighbor rows, the stage orientations in
+ * the constrained torques; cells_out gets the new coordinates, the stored
+ * velocities and the other columns copied; errors_out (n x 4) the embedded
+ * error. noise: n x 4 drawn by the caller once per call, or NULL. */
+int bfm2d_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* bs, int s, double dt,
+                  const double* cells, int n, int ld, const double* neighbors, int64_t m, int64_t ld_nb,
+                  const int* to_adhere, const double* noise, double* cells_out, int ld_out,
+                  double* errors_out);
+
 /* ---- frames, lineage, configuration and the driver (host only) ---------- */
 /* Message of the last failed call below (per thread). */
 const char* bfm_io_last_error(void);

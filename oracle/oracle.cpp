@@ -1822,3 +1822,199 @@ int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double
 }
 
 }  // extern "C"
+
+/* ---- the 2-D model (ABM-2d/ABM-2d-cpp) -------------------------------------
+ * Cells in the 2-D tree's layout (id 0, r 1-2, n 3-4, dr 5-6, dn 7-8, l 9,
+ * half_l 10, eta0 13, eta1 14), neighbor rows (i, j, dx, dy, s, t). */
+namespace {
+enum { D2_RX = 1, D2_RY = 2, D2_NX = 3, D2_NY = 4, D2_DRX = 5, D2_L = 9, D2_HALF_L = 10, D2_ETA0 = 13, D2_ETA1 = 14 };
+
+/* cellCellRepulsiveForces (mechanics.hpp:448-570 of ABM-2d) and
+ * cellCellAdhesiveForces (:572-694) with forcesJKRLagrange<T, 2> /
+ * forcesKiharaLagrange<T, 2> (adhesion.hpp:334-462 of ABM-2d). */
+void forces_2d(const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+               const int* to_adhere, const bfm2d_params* p, double* rep, double* adh)
+{
+    std::memset(rep, 0, sizeof(double) * 4 * (size_t)n);
+    std::memset(adh, 0, sizeof(double) * 4 * (size_t)n);
+    if (n == 1) return;
+    const double R = p->R, Rcell = p->Rcell;
+    for (int64_t k = 0; k < m; ++k) {
+        const int i = (int)nb[k], j = (int)nb[ld_nb + k];
+        const double dx = nb[2 * ld_nb + k], dy = nb[3 * ld_nb + k];
+        const double si = nb[4 * ld_nb + k], sj = nb[5 * ld_nb + k];
+        const double mag = std::sqrt(dx * dx + dy * dy);
+        const double dir[2] = {dx / mag, dy / mag};
+        const double overlap = 2 * R - mag;
+        const double ni[2] = {cell(cells, ld, i, D2_NX), cell(cells, ld, i, D2_NY)};
+        const double nj[2] = {cell(cells, ld, j, D2_NX), cell(cells, ld, j, D2_NY)};
+        double prefactor = 0;
+        if (overlap > 0 && overlap < 2 * (R - Rcell)) {
+            prefactor = p->prefactors[0] * std::pow(overlap, 1.5);
+        } else if (overlap >= 2 * (R - Rcell)) {
+            double term = p->prefactors[2] * std::pow(overlap - 2 * (R - Rcell), 1.5);
+            prefactor = p->prefactors[1] + term;
+        }
+        if (overlap > 0) {
+            const double vij[2] = {prefactor * dir[0], prefactor * dir[1]};
+            const double wi = ni[0] * vij[0] + ni[1] * vij[1];
+            const double wj = nj[0] * vij[0] + nj[1] * vij[1];
+            const double fi[4] = {vij[0], vij[1], si * (-wi * ni[0] + vij[0]), si * (-wi * ni[1] + vij[1])};
+            const double fj[4] = {-vij[0], -vij[1], sj * (wj * nj[0] - vij[0]), sj * (wj * nj[1] - vij[1])};
+            for (int q = 0; q < 4; ++q) {
+                rep[(size_t)q * n + i] += fi[q];
+                rep[(size_t)q * n + j] += fj[q];
+            }
+        }
+    }
+    if (p->adhesion_mode == BFM2D_ADHESION_NONE) return;
+    for (int64_t k = 0; k < m; ++k) {
+        const int i = (int)nb[k], j = (int)nb[ld_nb + k];
+        const double d12[2] = {nb[2 * ld_nb + k], nb[3 * ld_nb + k]};
+        const double s = nb[4 * ld_nb + k], t = nb[5 * ld_nb + k];
+        const double overlap = 2 * R - std::sqrt(d12[0] * d12[0] + d12[1] * d12[1]);
+        if (!(to_adhere[k] && overlap > 0)) continue;
+        const double n1[2] = {cell(cells, ld, i, D2_NX), cell(cells, ld, i, D2_NY)};
+        const double n2[2] = {cell(cells, ld, j, D2_NX), cell(cells, ld, j, D2_NY)};
+        const double dist = std::sqrt(d12[0] * d12[0] + d12[1] * d12[1]);
+        double f0[4] = {0, 0, 0, 0}, f1[4] = {0, 0, 0, 0};
+        if (dist <= 2 * R) {
+            const double d12n[2] = {d12[0] / dist, d12[1] / dist};
+            double v[2];
+            if (p->adhesion_mode == BFM2D_ADHESION_JKR) {
+                double term = kPi * R;
+                if (dist <= p->mindist) term *= (2 * R - p->mindist);
+                else term *= (2 * R - dist);
+                v[0] = term * d12n[0];
+                v[1] = term * d12n[1];
+            } else {
+                const double e = p->distance_exp;
+                double term1 = (dist <= p->mindist ? 1.0 / std::pow(p->mindist, e + 1) : 1.0 / std::pow(dist, e + 1));
+                double term2 = 1.0 / std::pow(2 * R, e + 1);
+                const double a = e * (term1 - term2);
+                v[0] = a * d12n[0];
+                v[1] = a * d12n[1];
+            }
+            const double w1 = n1[0] * (-v[0]) + n1[1] * (-v[1]);
+            const double w2 = n2[0] * (-v[0]) + n2[1] * (-v[1]);
+            f0[0] = -v[0]; f0[1] = -v[1];
+            f1[0] = v[0]; f1[1] = v[1];
+            f0[2] = s * (-w1 * n1[0] - v[0]); f0[3] = s * (-w1 * n1[1] - v[1]);
+            f1[2] = t * (w2 * n2[0] + v[0]); f1[3] = t * (w2 * n2[1] + v[1]);
+        }
+        for (int q = 0; q < 4; ++q) {
+            adh[(size_t)q * n + i] += p->strength * f0[q];
+            adh[(size_t)q * n + j] += p->strength * f1[q];
+        }
+    }
+}
+
+void normalize_2d(double* cells, int n, int ld)
+{
+    for (int i = 0; i < n; ++i) {
+        const double nx = cells[(size_t)D2_NX * ld + i], ny = cells[(size_t)D2_NY * ld + i];
+        const double nrm = std::sqrt(nx * nx + ny * ny);
+        cells[(size_t)D2_NX * ld + i] = nx / nrm;
+        cells[(size_t)D2_NY * ld + i] = ny / nrm;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+/* getCellNeighbors of the 2-D tree (mechanics.hpp:293-374 of ABM-2d): the
+ * 3-D closest-point computation of oracle_dist_between_cells on the cells in
+ * the z = 0 plane (the reference's non-parallel branch calls CGAL's segment
+ * distance, CGAL::Distance_3::internal::squared_distance, not available here;
+ * its parallel branch is the same as the 3-D tree's). Rows i, j, dx, dy, s, t. */
+int64_t oracle2d_get_cell_neighbors(const double* cells, int n, int ld, double thr, double R, double Ldiv,
+                                    double* out, int64_t ld_out)
+{
+    std::vector<double> c3((size_t)15 * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        c3[(size_t)0 * n + i] = cell(cells, ld, i, 0);
+        c3[(size_t)C_RX * n + i] = cell(cells, ld, i, D2_RX);
+        c3[(size_t)C_RY * n + i] = cell(cells, ld, i, D2_RY);
+        c3[(size_t)C_NX * n + i] = cell(cells, ld, i, D2_NX);
+        c3[(size_t)C_NY * n + i] = cell(cells, ld, i, D2_NY);
+        c3[(size_t)C_L * n + i] = cell(cells, ld, i, D2_L);
+        c3[(size_t)C_HALF_L * n + i] = cell(cells, ld, i, D2_HALF_L);
+    }
+    const int64_t cap = std::max<int64_t>(ld_out, 1);
+    std::vector<double> nb7((size_t)7 * cap);
+    const int64_t m = oracle_get_cell_neighbors(c3.data(), n, n, thr, R, Ldiv, nb7.data(), cap);
+    const int src[6] = {0, 1, 2, 3, 5, 6};
+    for (int q = 0; q < 6; ++q)
+        for (int64_t k = 0; k < std::min(m, ld_out); ++k) out[q * ld_out + k] = nb7[(size_t)src[q] * cap + k];
+    return m;
+}
+
+/* The repulsive and adhesive dE/dq (n x 4 each) of the 2-D tree. */
+void oracle2d_forces(const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                     const int* to_adhere, const bfm2d_params* p, double* rep, double* adh)
+{
+    forces_2d(cells, n, ld, nb, m, ld_nb, to_adhere, p, rep, adh);
+}
+
+/* getVelocities of the 2-D tree (mechanics.hpp:1503-1708 of ABM-2d) without
+ * confinement: (repulsion + adhesion + 0) + noise, divided by the composite
+ * drag prefactors (compositeViscosityForcePrefactors, :264-291). */
+void oracle2d_get_velocities(const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                             const int* to_adhere, const double* noise, const bfm2d_params* p, double* vel)
+{
+    std::vector<double> rep(4 * (size_t)n), adh(4 * (size_t)n);
+    forces_2d(cells, n, ld, nb, m, ld_nb, to_adhere, p, rep.data(), adh.data());
+    for (int i = 0; i < n; ++i) {
+        const double l = cell(cells, ld, i, D2_L);
+        const double drag = cell(cells, ld, i, D2_ETA0) + cell(cells, ld, i, D2_ETA1) * p->surface_contact_density / p->R;
+        const double K = l * drag;
+        const double L = l * l * l * drag / 12;
+        for (int q = 0; q < 4; ++q) {
+            double e = rep[(size_t)q * n + i] + adh[(size_t)q * n + i] + 0.0;
+            e = e + (noise ? noise[(size_t)q * n + i] : 0.0);
+            vel[(size_t)q * n + i] = -e / (q < 2 ? K : L);
+        }
+    }
+}
+
+/* stepRungeKuttaAdaptive of the 2-D tree (mechanics.hpp:1710-1817 of ABM-2d). */
+void oracle2d_step_rk(const double* A, const double* b, const double* bs, int s, double dt, const double* cells,
+                      int n, int ld, int ncols, const double* nb, int64_t m, int64_t ld_nb, const int* to_adhere,
+                      const double* noise, const bfm2d_params* p, double* cells_new, double* errors)
+{
+    std::vector<std::vector<double>> k(s, std::vector<double>(4 * (size_t)n));
+    std::vector<double> ci((size_t)ncols * n);
+    auto copy_in = [&]() {
+        for (int c = 0; c < ncols; ++c)
+            for (int i = 0; i < n; ++i) ci[(size_t)c * n + i] = cell(cells, ld, i, c);
+    };
+    copy_in();
+    oracle2d_get_velocities(ci.data(), n, n, nb, m, ld_nb, to_adhere, noise, p, k[0].data());
+    for (int st = 1; st < s; ++st) {
+        std::vector<double> mult(4 * (size_t)n, 0.0);
+        for (int j = 0; j < st; ++j)
+            for (size_t q = 0; q < 4 * (size_t)n; ++q) mult[q] += k[j][q] * A[st * s + j];
+        copy_in();
+        for (int c = 0; c < 4; ++c)
+            for (int i = 0; i < n; ++i) ci[(size_t)(c + D2_RX) * n + i] += mult[(size_t)c * n + i] * dt;
+        normalize_2d(ci.data(), n, n);
+        oracle2d_get_velocities(ci.data(), n, n, nb, m, ld_nb, to_adhere, noise, p, k[st].data());
+    }
+    std::vector<double> v1(4 * (size_t)n, 0.0), v2(4 * (size_t)n, 0.0);
+    for (int st = 0; st < s; ++st)
+        for (size_t q = 0; q < 4 * (size_t)n; ++q) {
+            v1[q] += k[st][q] * b[st];
+            v2[q] += k[st][q] * bs[st];
+        }
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < n; ++i) cells_new[(size_t)c * n + i] = cell(cells, ld, i, c);
+    for (size_t q = 0; q < 4 * (size_t)n; ++q) {
+        const double d1 = v1[q] * dt, d2 = v2[q] * dt;
+        cells_new[(size_t)D2_RX * n + q] += d1;
+        errors[q] = d1 - d2;
+        cells_new[(size_t)D2_DRX * n + q] = v1[q];
+    }
+    normalize_2d(cells_new, n, n);
+}
+
+}  // extern "C"

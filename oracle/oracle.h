@@ -135,6 +135,19 @@ int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double
                     double* cells_out, int cap, int* n_out, double* dt_trace, int cap_trace,
                     int64_t* iters_out, int64_t* rk_steps_out, int nthreads);
 
+
+/* The 2-D model (ABM-2d/ABM-2d-cpp): neighbor rows (i, j, dx, dy, s, t),
+ * repulsive / adhesive dE/dq, getVelocities and stepRungeKuttaAdaptive. */
+int64_t oracle2d_get_cell_neighbors(const double* cells, int n, int ld, double thr, double R, double Ldiv,
+                                    double* out, int64_t ld_out);
+void oracle2d_forces(const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                     const int* to_adhere, const bfm2d_params* p, double* rep, double* adh);
+void oracle2d_get_velocities(const double* cells, int n, int ld, const double* nb, int64_t m, int64_t ld_nb,
+                             const int* to_adhere, const double* noise, const bfm2d_params* p, double* vel);
+void oracle2d_step_rk(const double* A, const double* b, const double* bs, int s, double dt, const double* cells,
+                      int n, int ld, int ncols, const double* nb, int64_t m, int64_t ld_nb, const int* to_adhere,
+                      const double* noise, const bfm2d_params* p, double* cells_new, double* errors);
+
 #ifdef __cplusplus
 }
 #endif

@@ -408,3 +408,78 @@ def simulate(cfg, params, cells, cap, trace_cap=1_000_000, nthreads=0):
         raise RuntimeError(f"oracle_simulate: status {rc}")
     it = min(iters.value, trace_cap)
     return np.asfortranarray(out[:n_out.value]), trace[:it].copy(), iters.value, steps.value
+
+
+# ---- the 2-D model -----------------------------------------------------------
+def _lib2d():
+    L = lib()
+    if not getattr(L, "_o2d", False):
+        d, i64 = C.c_double, C.c_int64
+        vp = C.c_void_p
+        L.oracle2d_get_cell_neighbors.argtypes = [_dp, C.c_int, C.c_int, d, d, d, _dp, i64]
+        L.oracle2d_get_cell_neighbors.restype = i64
+        L.oracle2d_forces.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, _ip, vp, _dp, _dp]
+        L.oracle2d_forces.restype = None
+        L.oracle2d_get_velocities.argtypes = [_dp, C.c_int, C.c_int, _dp, i64, i64, _ip, _dp, vp, _dp]
+        L.oracle2d_get_velocities.restype = None
+        L.oracle2d_step_rk.argtypes = [_dp, _dp, _dp, C.c_int, d, _dp, C.c_int, C.c_int, C.c_int, _dp, i64,
+                                       i64, _ip, _dp, vp, _dp, _dp]
+        L.oracle2d_step_rk.restype = None
+        L._o2d = True
+    return L
+
+
+def _adh(nb, to_adhere):
+    if to_adhere is None:
+        to_adhere = np.ones(nb.shape[0], dtype=np.int32)
+    return np.ascontiguousarray(to_adhere, dtype=np.int32)
+
+
+def get_cell_neighbors_2d(cells, thr, R, Ldiv):
+    cells = _f(cells)
+    n = cells.shape[0]
+    cap = max(8 * n, 16)
+    while True:
+        out = np.zeros((cap, 6), order="F")
+        m = _lib2d().oracle2d_get_cell_neighbors(_p(cells), n, n, thr, R, Ldiv, _p(out), cap)
+        if m <= cap:
+            return np.asfortranarray(out[:m])
+        cap = m
+
+
+def forces_2d(cells, nb, params, to_adhere=None):
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    rep, adh = np.zeros((n, 4), order="F"), np.zeros((n, 4), order="F")
+    ta = _adh(nb, to_adhere)
+    _lib2d().oracle2d_forces(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                             ta.ctypes.data_as(_ip), C.cast(C.byref(params), C.c_void_p), _p(rep), _p(adh))
+    return rep, adh
+
+
+def get_velocities_2d(cells, nb, params, to_adhere=None, noise=None):
+    cells, nb = _f(cells), _f(nb)
+    n = cells.shape[0]
+    out = np.zeros((n, 4), order="F")
+    ta = _adh(nb, to_adhere)
+    nz = None if noise is None else _f(noise)
+    _lib2d().oracle2d_get_velocities(_p(cells), n, n, _p(nb), nb.shape[0], max(nb.shape[0], 1),
+                                     ta.ctypes.data_as(_ip), _p(nz), C.cast(C.byref(params), C.c_void_p),
+                                     _p(out))
+    return out
+
+
+def step_rk_2d(A, b, bs, dt, cells, nb, params, to_adhere=None, noise=None):
+    cells, nb = _f(cells), _f(nb)
+    n, ncols = cells.shape
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    bs = np.ascontiguousarray(bs, dtype=np.float64)
+    out = np.zeros((n, ncols), order="F")
+    err = np.zeros((n, 4), order="F")
+    ta = _adh(nb, to_adhere)
+    nz = None if noise is None else _f(noise)
+    _lib2d().oracle2d_step_rk(_p(A), _p(b), _p(bs), len(b), dt, _p(cells), n, n, ncols, _p(nb), nb.shape[0],
+                              max(nb.shape[0], 1), ta.ctypes.data_as(_ip), _p(nz),
+                              C.cast(C.byref(params), C.c_void_p), _p(out), _p(err))
+    return out, err
