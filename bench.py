@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
-    ap.add_argument("--workload", default="stage", choices=["stage", "growth", "ensemble"],
+    ap.add_argument("--workload", default="stage", choices=["stage", "growth", "ensemble", "layer2d"],
                     help="stage: the RK-stage bench (default); growth: the founder-to-N growth run; "
                          "ensemble: batched replicates")
     ap.add_argument("--ens-reps", type=int, default=256)
@@ -136,6 +136,8 @@ def main():
         return growth(args)
     if args.workload == "ensemble":
         return ensemble(args)
+    if args.workload == "layer2d":
+        return layer2d(args)
     import numpy as np
     import torch
 
@@ -324,6 +326,44 @@ def growth(args):
                       "daughter_angle_xy_bound": cfg.daughter_angle_xy_bound,
                       "daughter_angle_z_bound": cfg.daughter_angle_z_bound},
            "gpu": gpu, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+def layer2d(args):
+    """The 2-D model (ABM-2d) on a layer of --cells cells in rows (JKR
+    adhesion): the neighbor search, then getVelocities and a Bogacki-Shampine
+    step on host arrays; the per-kernel device times (HIP events) are the
+    figures, the host calls include the PCIe copies of cells and rows."""
+    import numpy as np
+
+    import bfm
+    from bfm.model2d import Model2D, make_params, random_layer
+
+    n = args.cells
+    c = random_layer(n, args.seed)
+    p = make_params(adhesion_mode=1, strength=40.0, mindist=1.2)
+    m2 = Model2D(n, p, max_pairs=8 * n)
+    nb = m2.getCellNeighbors(c, 2.0, 0.8, 2.4)
+    A, b, bs, _ = bfm.tableau("bogacki_shampine")
+    for _ in range(args.warmup):
+        m2.getVelocities(c, nb)
+        m2.stepRungeKuttaAdaptive(A, b, bs, c, nb, 1e-5)
+    m2.ctx.enable_timing(True)
+    m2.ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m2.stepRungeKuttaAdaptive(A, b, bs, c, nb, 1e-5)
+    wall = (time.perf_counter() - t0) / args.steps
+    kt = m2.ctx.kernel_times()
+    per = {k: 1e3 * ms / max(cnt, 1) for k, (ms, cnt) in kt.items()}
+    dev_step_us = sum(1e3 * ms for k, (ms, cnt) in kt.items()) / args.steps
+    out = {"metric": "2-D RK step cell-updates/s (device kernels, FP64)", "value": n / (dev_step_us * 1e-6),
+           "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n}-cell 2-D layer in rows, JKR adhesion, Bogacki-Shampine step",
+                      "cells": n, "rows": int(nb.shape[0]), "seed": args.seed},
+           "device_us_per_step": dev_step_us, "host_call_ms_per_step": 1e3 * wall,
+           "kernel_us": {k: round(v, 2) for k, v in sorted(per.items(), key=lambda kv: -kv[1])}}
     print(json.dumps(out))
 
 

@@ -41,8 +41,8 @@ def test_neighbors_match_oracle(orc, n, seed):
     assert got.shape == want.shape and np.array_equal(got, want)
 
 
-MODES = {"none": dict(), "jkr": dict(adhesion_mode=1, strength=0.4, mindist=1.2),
-         "kihara": dict(adhesion_mode=2, strength=0.05, mindist=1.1, distance_exp=6.0)}
+MODES = {"none": dict(), "jkr": dict(adhesion_mode=1, strength=40.0, mindist=1.2),
+         "kihara": dict(adhesion_mode=2, strength=50.0, mindist=1.1, distance_exp=6.0)}
 
 
 @pytest.mark.parametrize("mode", list(MODES))
