@@ -71,6 +71,11 @@ def _lib():
             "bfm2d_get_velocities": [vp, _dp, C.c_int, C.c_int, _dp, C.c_int64, C.c_int64, _ip, _dp, _dp],
             "bfm2d_step_rk": [vp, _dp, _dp, _dp, C.c_int, C.c_double, _dp, C.c_int, C.c_int, _dp,
                               C.c_int64, C.c_int64, _ip, _dp, _dp, C.c_int, _dp],
+            "bfm2d_dev_build_neighbors": [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip,
+                                          C.c_int, C.POINTER(C.c_int64)],
+            "bfm2d_dev_update_neighbor_distances": [vp, vp, C.c_int, C.c_int],
+            "bfm2d_dev_step_rk": [vp, _dp, _dp, _dp, C.c_int, C.c_double, vp, C.c_int, C.c_int, vp, vp, C.c_int,
+                                  vp],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -134,6 +139,45 @@ class Model2D:
             self.ctx.h, _ptr(A), _ptr(b), _ptr(bs), len(b), dt, _ptr(cells), n, n, _ptr(nb), nb.shape[0],
             max(nb.shape[0], 1), ta.ctypes.data_as(_ip), _ptr(nz), _ptr(out), n, _ptr(err)))
         return out, err
+
+
+def dev_build_neighbors(model, d_cells, n, ld, thr, R, Ldiv, adhesion_map=None, sync=True):
+    """bfm2d_dev_build_neighbors on a device array (pointer); adhesion_map:
+    an n_groups x n_groups 0/1 matrix or None. Returns the row count (sync)."""
+    amap = None if adhesion_map is None else np.ascontiguousarray(adhesion_map, dtype=np.int32)
+    m = C.c_int64()
+    model.ctx._check(model.lib.bfm2d_dev_build_neighbors(
+        model.ctx.h, C.c_void_p(d_cells), n, ld, thr, R, Ldiv,
+        None if amap is None else amap.ctypes.data_as(_ip), 0 if amap is None else amap.shape[0],
+        C.byref(m) if sync else None))
+    return m.value if sync else None
+
+
+def dev_update_neighbor_distances(model, d_cells, n, ld):
+    model.ctx._check(model.lib.bfm2d_dev_update_neighbor_distances(model.ctx.h, C.c_void_p(d_cells), n, ld))
+
+
+def dev_step_rk(model, A, b, bs, dt, d_cells, n, ld, d_out, ld_out, d_noise=None, d_errors=None):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    bs = np.ascontiguousarray(bs, dtype=np.float64)
+    model.ctx._check(model.lib.bfm2d_dev_step_rk(
+        model.ctx.h, _ptr(A), _ptr(b), _ptr(bs), len(b), dt, C.c_void_p(d_cells), n, ld,
+        C.c_void_p(d_noise) if d_noise else None, C.c_void_p(d_out), ld_out,
+        C.c_void_p(d_errors) if d_errors else None))
+
+
+def to_adhere_rule(cells, nb, R, Rcell, adhesion_map):
+    """The 2-D driver's to_adhere for host rows (simulation.hpp:242-262 of ABM-2d)."""
+    amap = np.asarray(adhesion_map)
+    g = cells[:, 16].astype(int)
+    gi, gj = g[nb[:, 0].astype(int)], g[nb[:, 1].astype(int)]
+    lo, hi = np.minimum(gi, gj), np.maximum(gi, gj)
+    dist = np.sqrt(nb[:, 2] ** 2 + nb[:, 3] ** 2)
+    ok = (lo >= 0) & (hi < amap.shape[0])
+    flag = np.zeros(nb.shape[0], dtype=np.int32)
+    flag[ok] = amap[lo[ok], hi[ok]] != 0
+    return (flag.astype(bool) & (dist > R + Rcell) & (dist < 2 * R)).astype(np.int32)
 
 
 def random_layer(n, seed, spacing=(3.0, 1.4), tilt=0.3, hl_lo=0.4, hl_hi=1.2, eta0=0.7, eta1=720.0):

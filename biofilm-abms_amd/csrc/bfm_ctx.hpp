@@ -75,7 +75,8 @@ struct bfm_ctx {
     bool params_set = false;
     bfm2d_params p2d{};             // the 2-D model (bfm2d_set_params)
     bool p2d_set = false;
-    int* d_adhere = nullptr;        // max_pairs to_adhere flags of the 2-D model (adhesion on)
+    int* d_adhere = nullptr;        // max_pairs to_adhere flags of the 2-D model
+    int* d_amap = nullptr;          // 64 x 64 group adhesion map of the 2-D device list
     std::string err;
 
     // device status word and counters

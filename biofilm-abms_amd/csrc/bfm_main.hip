@@ -3129,6 +3129,29 @@ __global__ void k2d_embed(int n, const double* __restrict__ c2, int ld2, double*
     }
 }
 
+// to_adhere of the 2-D driver (simulation.hpp:242-262 of ABM-2d) for the
+// rows of the internal list: the pair's groups (min, max) adhere in the map
+// and R + Rcell < |d| < 2R.
+__global__ void k2d_adhere(const int* __restrict__ d_m, const double* __restrict__ cells, int ld,
+                           const int* __restrict__ pi, const int* __restrict__ pj, const double* __restrict__ pdx,
+                           const double* __restrict__ pdy, const int* __restrict__ amap, int ng, double R,
+                           double Rcell, int* __restrict__ adhere)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const int i = pi[k], j = pj[k];
+        int flag = 0;
+        if ((unsigned)i < (unsigned)ld && (unsigned)j < (unsigned)ld && amap) {
+            const int gi = (int)cellv(cells, ld, i, 16), gj = (int)cellv(cells, ld, j, 16);
+            const int g0 = min(gi, gj), g1 = max(gi, gj);
+            const double dx = pdx[k], dy = pdy[k];
+            const double dist = sqrt(dx * dx + dy * dy);
+            flag = g0 >= 0 && g1 < ng && amap[g0 * ng + g1] != 0 && dist > R + Rcell && dist < 2 * R;
+        }
+        adhere[k] = flag;
+    }
+}
+
 Params2D params_2d(const bfm_ctx* c)
 {
     require(c->p2d_set, BFM_ERR_ARG, "bfm2d_set_params has not been called");
@@ -3176,14 +3199,21 @@ void prepare_2d(bfm_ctx* c, const double* cells, int n, int ld, const double* nb
     build_transposes(c, n);
 }
 
-// The s stages of the 2-D step (s == 1 with tb.A = 0: getVelocities).
-void stages_2d(bfm_ctx* c, const Tableau& tb, double dt, int n, const double* d_noise)
+// The s stages of the 2-D step (s == 1 with tb.A = 0: getVelocities) on
+// cells d_cells (leading dimension ld), with the current list's row lists.
+void stages_2d(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int ld, int n,
+               const double* d_noise)
 {
     const Params2D P = params_2d(c);
     const RowLists L = row_lists(c);
-    for (int st = 0; st < tb.s; ++st)
-        KLAUNCH(c, "stage_2d", k2d_stage<false>, grid_for(n, 256), 256, 0, n, c->d_cells, c->max_cells, d_noise,
-                P, tb, st, dt, c->d_rec, L, c->d_k, c->d_status);
+    for (int st = 0; st < tb.s; ++st) {
+        if (c->list_csr)
+            KLAUNCH(c, "stage_2d", k2d_stage<true>, grid_for(n, 256), 256, 0, n, d_cells, ld, d_noise, P, tb, st,
+                    dt, c->d_rec, L, c->d_k, c->d_status);
+        else
+            KLAUNCH(c, "stage_2d", k2d_stage<false>, grid_for(n, 256), 256, 0, n, d_cells, ld, d_noise, P, tb, st,
+                    dt, c->d_rec, L, c->d_k, c->d_status);
+    }
 }
 
 }  // namespace
@@ -3361,7 +3391,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst,
-                    c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere};
+                    c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere, c->d_amap};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     bfm::surrogate_free(c);
@@ -3914,7 +3944,7 @@ int bfm2d_get_velocities(bfm_ctx* c, const double* cells, int n, int ld, const d
         Tableau tb;
         std::memset(&tb, 0, sizeof(tb));
         tb.s = 1;
-        stages_2d(c, tb, 0.0, n, noise ? c->d_noise : nullptr);
+        stages_2d(c, tb, 0.0, c->d_cells, c->max_cells, n, noise ? c->d_noise : nullptr);
         if (n > 0)
             HIPCHK(hipMemcpyAsync(vel, c->d_k, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->stream));
         check_status(c);
@@ -3931,7 +3961,7 @@ int bfm2d_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs
         require(ld_out >= n, BFM_ERR_ARG, "ld_out < n");
         const Tableau tb = make_tableau(A, b, bs, s);
         prepare_2d(c, cells, n, ld, nb, m, ld_nb, to_adhere, noise);
-        stages_2d(c, tb, dt, n, noise ? c->d_noise : nullptr);
+        stages_2d(c, tb, dt, c->d_cells, c->max_cells, n, noise ? c->d_noise : nullptr);
         KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, c->d_cells, c->max_cells, tb,
                 dt, c->d_k, c->d_cells_new, c->max_cells, c->d_err6);
         if (n > 0) {
@@ -3941,6 +3971,71 @@ int bfm2d_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs
             HIPCHK(hipMemcpyAsync(errors, c->d_err6, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->stream));
         }
         check_status(c);
+    });
+}
+
+int bfm2d_dev_build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv,
+                              const int* adhesion_map, int n_groups, int64_t* m_out)
+{
+    return guarded(c, [&] {
+        require(c->p2d_set, BFM_ERR_ARG, "bfm2d_set_params has not been called");
+        require(n >= 0 && n <= c->max_cells, BFM_ERR_CAPACITY, "n exceeds max_cells of the context");
+        require(ld >= n, BFM_ERR_ARG, "ld < n");
+        require(n_groups >= 0 && n_groups <= 64, BFM_ERR_ARG, "n_groups must be 0..64");
+        KLAUNCH(c, "embed_2d", k2d_embed, grid_for(n, 256), 256, 0, n, d_cells, ld, c->d_cells_new, c->max_cells);
+        if (m_out) build_neighbors_sync(c, c->d_cells_new, n, c->max_cells, thr, R, Ldiv);
+        else build_neighbors(c, c->d_cells_new, n, c->max_cells, thr, R, Ldiv);
+        if (!c->d_adhere) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            dalloc(&c->d_adhere, (size_t)c->max_pairs);
+        }
+        const int* amap = nullptr;
+        if (adhesion_map && n_groups > 0) {
+            if (!c->d_amap) dalloc(&c->d_amap, (size_t)64 * 64);
+            HIPCHK(hipMemcpyAsync(c->d_amap, adhesion_map, sizeof(int) * n_groups * n_groups,
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            amap = c->d_amap;
+        }
+        KLAUNCH(c, "adhere_2d", k2d_adhere, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj, c->pdx, c->pdy,
+                amap, n_groups, c->p2d.R, c->p2d.Rcell, c->d_adhere);
+        if (m_out) {
+            int m = 0;
+            HIPCHK(hipMemcpy(&m, c->d_m, sizeof(int), hipMemcpyDeviceToHost));
+            c->h_m = m;
+            *m_out = m;
+        }
+    });
+}
+
+int bfm2d_dev_update_neighbor_distances(bfm_ctx* c, const double* d_cells, int n, int ld)
+{
+    return guarded(c, [&] {
+        require(n >= 0 && n <= c->max_cells && ld >= n, BFM_ERR_ARG, "invalid n or ld");
+        require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
+        KLAUNCH(c, "embed_2d", k2d_embed, grid_for(n, 256), 256, 0, n, d_cells, ld, c->d_cells_new, c->max_cells);
+        KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, c->d_cells_new, c->max_cells,
+                c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+    });
+}
+
+int bfm2d_dev_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs, int s, double dt,
+                      const double* d_cells, int n, int ld, const double* d_noise, double* d_cells_out, int ld_out,
+                      double* d_errors)
+{
+    return guarded(c, [&] {
+        require(c->p2d_set, BFM_ERR_ARG, "bfm2d_set_params has not been called");
+        require(A && b && bs && d_cells_out, BFM_ERR_ARG, "NULL argument");
+        require(n >= 0 && n <= c->max_cells && ld >= n && ld_out >= n, BFM_ERR_ARG, "invalid n, ld or ld_out");
+        require(c->list_n == n && c->list_csr, BFM_ERR_ARG,
+                "no device neighbor list for these cells (bfm2d_dev_build_neighbors)");
+        const Tableau tb = make_tableau(A, b, bs, s);
+        const Params2D P = params_2d(c);
+        KLAUNCH(c, "pair_2d", k2d_pair, kPairGrid, 256, 0, c->d_m, n, c->pi, c->pj, c->pdx, c->pdy, c->ps, c->pt,
+                c->d_adhere, P, c->d_rec, (int*)nullptr);
+        stages_2d(c, tb, dt, d_cells, ld, n, d_noise);
+        KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, d_cells, ld, tb, dt, c->d_k,
+                d_cells_out, ld_out, d_errors ? d_errors : c->d_err6);
     });
 }
 

@@ -442,6 +442,24 @@ int bfm2d_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* 
                   const int* to_adhere, const double* noise, double* cells_out, int ld_out,
                   double* errors_out);
 
+/* Device-resident 2-D forms (device arrays, the context's stream, the
+ * context's internal list). bfm2d_dev_build_neighbors builds the list of
+ * bfm2d_get_cell_neighbors and the rows' to_adhere flags by the 2-D driver's
+ * rule (simulation.hpp:242-262 of ABM-2d): the groups (min, max) of the pair
+ * adhere in adhesion_map (host, n_groups x n_groups row-major, group = column
+ * 16 as an integer; NULL: no row adheres) and R + Rcell < |d| < 2R at build
+ * time. m_out == NULL: no synchronisation. The list exports with
+ * bfm_dev_export_neighbors (7 columns, dz = 0). */
+int bfm2d_dev_build_neighbors(bfm_ctx* ctx, const double* d_cells, int n, int ld, double thr, double R,
+                              double Ldiv, const int* adhesion_map, int n_groups, int64_t* m_out);
+/* updateNeighborDistances of the 2-D tree (mechanics.hpp:376-446 of ABM-2d). */
+int bfm2d_dev_update_neighbor_distances(bfm_ctx* ctx, const double* d_cells, int n, int ld);
+/* bfm2d_step_rk on device arrays with the internal list (d_noise n x 4 or
+ * NULL, d_errors n x 4 or NULL). */
+int bfm2d_dev_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* bs, int s, double dt,
+                      const double* d_cells, int n, int ld, const double* d_noise, double* d_cells_out,
+                      int ld_out, double* d_errors);
+
 /* ---- frames, lineage, configuration and the driver (host only) ---------- */
 /* Message of the last failed call below (per thread). */
 const char* bfm_io_last_error(void);

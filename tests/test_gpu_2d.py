@@ -101,3 +101,60 @@ def test_device_list_drives_the_step(orc):
     new, _ = m2.stepRungeKuttaAdaptive(A, b, bs, c, nb, 1e-5)
     new_ref, _ = orc.step_rk_2d(A, b, bs, 1e-5, c, orc.get_cell_neighbors_2d(c, THR, R, LDIV), p)
     assert rel_err(new[:, 1:9], new_ref[:, 1:9]) <= TOL
+
+
+def _embed(c):
+    """2-D cells in the 3-D layout at z = 0 (for the oracle's distance update)."""
+    c3 = np.zeros((c.shape[0], 22), order="F")
+    c3[:, 0] = c[:, 0]
+    c3[:, 1:3], c3[:, 4:6], c3[:, 13], c3[:, 14] = c[:, 1:3], c[:, 3:5], c[:, 9], c[:, 10]
+    return c3
+
+
+def test_device_path_trajectory(orc):
+    """bfm2d_dev_build_neighbors (to_adhere by the driver's group rule),
+    bfm2d_dev_step_rk and bfm2d_dev_update_neighbor_distances on device
+    arrays: the exported list equals the oracle's, and 8 Bogacki-Shampine
+    steps with the distances updated between them stay within 1e-9 of the
+    oracle's loop using the same rule on the host."""
+    import torch
+
+    import bfm
+    from bfm.model2d import (Model2D, make_params, dev_build_neighbors, dev_step_rk,
+                             dev_update_neighbor_distances, to_adhere_rule)
+    n = 2000
+    c = _layer(n, 78)
+    c[:, 16] = np.random.default_rng(79).integers(1, 3, n)
+    amap = np.array([[0, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.int32)
+    p = make_params(adhesion_mode=1, strength=40.0, mindist=1.2)
+    m2 = Model2D(n, p, max_pairs=8 * n)
+    a = torch.from_numpy(np.ascontiguousarray(c.T)).cuda()
+    bbuf = torch.empty_like(a)
+    m = dev_build_neighbors(m2, a.data_ptr(), n, n, THR, R, LDIV, amap)
+    nb_ref = orc.get_cell_neighbors_2d(c, THR, R, LDIV)
+    exp = torch.zeros((7, max(m, 1)), dtype=torch.float64, device="cuda")
+    m2.ctx._check(m2.ctx.lib.bfm_dev_export_neighbors(m2.ctx.h, exp.data_ptr(), max(m, 1)))
+    torch.cuda.synchronize()
+    nb_dev = exp.cpu().numpy().T[:m][:, [0, 1, 2, 3, 5, 6]]
+    assert m == nb_ref.shape[0] and np.array_equal(nb_dev, nb_ref)
+    ta = to_adhere_rule(c, nb_ref, R, 0.5, amap)
+    assert 0 < ta.sum() < len(ta)
+    A, b, bs, _ = bfm.tableau("bogacki_shampine")
+    dt = 1e-5
+    x_ref = c.copy(order="F")
+    nb = nb_ref.copy(order="F")
+    src, dst = a, bbuf
+    for k in range(8):
+        dev_step_rk(m2, A, b, bs, dt, src.data_ptr(), n, n, dst.data_ptr(), n)
+        dev_update_neighbor_distances(m2, dst.data_ptr(), n, n)
+        src, dst = dst, src
+        x_ref, _ = orc.step_rk_2d(A, b, bs, dt, x_ref, nb, p, ta)
+        nb7 = np.asfortranarray(np.c_[nb[:, :4], np.zeros(len(nb)), nb[:, 4:]])
+        orc.update_neighbor_distances(_embed(x_ref), nb7)
+        nb = np.asfortranarray(nb7[:, [0, 1, 2, 3, 5, 6]])
+    torch.cuda.synchronize()
+    got = src.cpu().numpy().T
+    e = rel_err(got[:, 1:9], x_ref[:, 1:9])
+    print(f"8-step device trajectory rel_err={e:.3e}", flush=True)
+    assert e <= 1e-9
+    assert np.abs(got[:, 1:3] - c[:, 1:3]).max() > 1e-6
