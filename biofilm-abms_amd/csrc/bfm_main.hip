@@ -1838,8 +1838,8 @@ __device__ __forceinline__ CellVel load_cellvel(const double* __restrict__ cells
 // Conservative forces (-F_cc - F_rep - F_adh, mechanics.hpp:1768) + noise and
 // the velocity solve for one cell.
 __device__ __forceinline__ void cell_velocity(const VelParams& vp, const CellVel& cv, double sigma0,
-                                              bool a2d, const double cc[6], const double* noise6,
-                                              double v[6])
+                                              bool a2d, const double cc[6], bool has_noise,
+                                              const double noise6[6], double v[6])
 {
     const double ss = a2d ? NAN : (vp.sc.R - cv.rz) / cv.nz;
     double rep2 = 0, rep5 = 0, adh2 = 0, adh5 = 0;
@@ -1850,7 +1850,7 @@ __device__ __forceinline__ void cell_velocity(const VelParams& vp, const CellVel
     for (int q = 0; q < 6; ++q) f[q] = -cc[q];
     f[2] = (f[2] - rep2) - adh2;
     f[5] = (f[5] - rep5) - adh5;
-    if (noise6) {
+    if (has_noise) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) f[q] += noise6[q];
     }
@@ -1882,13 +1882,13 @@ k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restr
         }
         const CellVel cv = load_cellvel(cells, ld, c);
         const bool a2d = assume_2d ? (assume_2d[c] != 0) : (cv.nz < vp.nz_threshold);
-        double nz6[6];
+        double nz6[6] = {0, 0, 0, 0, 0, 0};
         if (noise) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) nz6[q] = noise[(int64_t)q * n + c];
         }
         double v[6];
-        cell_velocity(vp, cv, cellv(cells, ld, c, C_SIGMA0), a2d, cc, noise ? nz6 : nullptr, v);
+        cell_velocity(vp, cv, cellv(cells, ld, c, C_SIGMA0), a2d, cc, noise != nullptr, nz6, v);
         bool bad = false;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -2024,7 +2024,7 @@ k_rk_stage(int n, const int* __restrict__ dn, const double* __restrict__ cells, 
             cv.rz = y[2]; cv.nx = y[3]; cv.ny = y[4]; cv.nz = y[5];
         }
         const bool a2d = assume_2d[c] != 0;
-        double ccl[6], nz6[6];
+        double ccl[6], nz6[6] = {0, 0, 0, 0, 0, 0};
         if (rec) {
             gather_cc<CSR>(c, L, rec, vp.sc.R, ccl);
         } else {
@@ -2042,7 +2042,7 @@ k_rk_stage(int n, const int* __restrict__ dn, const double* __restrict__ cells, 
             for (int q = 0; q < 6; ++q) nz6[q] = noise[(int64_t)q * n + c];
         }
         double v[6];
-        cell_velocity(vp, cv, cellv(cells, ld, c, C_SIGMA0), a2d, ccl, noise ? nz6 : nullptr, v);
+        cell_velocity(vp, cv, cellv(cells, ld, c, C_SIGMA0), a2d, ccl, noise != nullptr, nz6, v);
         bool bad = false;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
