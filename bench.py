@@ -375,12 +375,37 @@ def layer2d(args):
     torch.cuda.synchronize()
     kt = m2.ctx.kernel_times()
     per = {k: 1e3 * ms / max(cnt, 1) for k, (ms, cnt) in kt.items()}
+    # k2d_stage, per launch: each row's 64-byte record and 4-byte CSR entry
+    # once per role, seven cell columns, the earlier stages' velocities (on
+    # average 1.5 of 4 x 8 B for the 4 stages) and its own 4 x 8 B
+    stage_bytes = rows * 2 * (64 + 4) + n * (7 * 8 + 48 + 32)
+    st_us = per.get("stage_2d", 0.0)
+    roof = {"kernel": "stage_2d", "bound": "hbm", "achieved": stage_bytes / (st_us * 1e-6) / 1e9 if st_us else None,
+            "peak": 8000.0, "unit": "GB/s", "traffic": None, "algorithmic_bytes": stage_bytes, "avg_us": st_us}
+    roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        from bfm.model2d import to_adhere_rule
+        # the device list (bit-identical to the oracle's O(N^2) search, which
+        # takes about a minute here) feeds the oracle's step
+        exp = torch.zeros((7, max(rows, 1)), dtype=torch.float64, device="cuda")
+        m2.ctx._check(m2.ctx.lib.bfm_dev_export_neighbors(m2.ctx.h, exp.data_ptr(), max(rows, 1)))
+        torch.cuda.synchronize()
+        nb = np.asfortranarray(exp.cpu().numpy().T[:rows][:, [0, 1, 2, 3, 5, 6]])
+        ta = to_adhere_rule(c, nb, 0.8, 0.5, amap)
+        t0 = time.perf_counter()
+        orc.step_rk_2d(A, b, bs, 1e-5, c, nb, p, ta)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": n / t_cpu, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"oracle stepRungeKuttaAdaptive of the 2-D tree (Bogacki-Shampine) on the same "
+                         f"{n}-cell layer and rows, one step, one thread: {t_cpu:.3f} s"}
     out = {"metric": "2-D RK step cell-updates/s (FP64, device arrays)", "value": n / (step_us * 1e-6),
            "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None,
            "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"{n}-cell 2-D layer in rows, JKR adhesion, Bogacki-Shampine step",
                       "cells": n, "rows": int(rows), "seed": args.seed},
-           "us_per_step": step_us,
+           "us_per_step": step_us, "roofline": roof, "cpu_baseline": cpu,
            "kernel_us": {k: round(v, 2) for k, v in sorted(per.items(), key=lambda kv: -kv[1])}}
     print(json.dumps(out))
 

@@ -151,6 +151,7 @@ struct bfm_ctx {
     double* d_an_curv = nullptr;
     double* d_an_fr = nullptr;
     int an_n[7] = {0, 0, 0, 0, 0, 0, 0};
+    double an_inv_h[7] = {0, 0, 0, 0, 0, 0, 0};
     double* d_nst = nullptr;        // 3 x max_cells stage orientations (anisotropic tables set)
 
     // surrogate

@@ -276,6 +276,7 @@ struct AnisoJKR {
     int n[kAnisoGrids];
     int off[kAnisoGrids];
     int total;
+    double inv_h[kAnisoGrids];   // 1 / spacing of a (nearly) evenly spaced grid, else 0
 };
 
 // nearestValue (utils.hpp:1169-1215) on a non-decreasing grid. The
@@ -294,6 +295,22 @@ __device__ __forceinline__ int nearest_sorted(const double* v, int n, double x)
         else hi = mid;
     }
     return (x - v[lo] < v[hi] - x) ? lo : hi;
+}
+
+// nearest_sorted with a first guess for evenly spaced grids (inv_h > 0): the
+// guess is moved to the same interval v[lo] <= x < v[lo + 1] by comparisons
+// with the grid values, so the result is nearest_sorted's for any sorted
+// grid; the spacing only decides how many steps that takes.
+__device__ __forceinline__ int nearest_grid(const double* v, int n, double x, double inv_h)
+{
+    if (inv_h <= 0) return nearest_sorted(v, n, x);
+    if (!(x > v[0])) return 0;
+    if (x >= v[n - 1]) return n - 1;
+    int lo = (int)((x - v[0]) * inv_h);
+    lo = min(max(lo, 0), n - 2);
+    while (lo > 0 && v[lo] > x) --lo;
+    while (lo < n - 2 && v[lo + 1] <= x) ++lo;
+    return (x - v[lo] < v[lo + 1] - x) ? lo : lo + 1;
 }
 
 // acosSafe, utils.hpp:629-638.
@@ -324,30 +341,32 @@ __device__ __forceinline__ double2 aniso_jkr_lookup(const AnisoJKR& an, const do
 {
     constexpr double kHalfPi = 1.57079632679489661923;
     const double dn[3] = {d[0] / dist, d[1] / dist, d[2] / dist};
-    // (-d12n).n2 and (-d12n).(-n2) are -c2 and c2 exactly (sign flips are exact)
+    // The reference folds theta1 = acos(c1) into [0, pi/2] by taking acos(-c1)
+    // when acos(c1) > pi/2, i.e. when c1 < 0, and theta2 likewise from
+    // (-d12n).n2 = -c2 (sign flips are exact): both are acos(|c|). One
+    // arccosine per angle instead of two; the two-step form differs only when
+    // acos of a |c| below an ulp rounds to pi/2, a difference under one ulp.
     const double c1 = dot3_rn(dn, n1), c2 = dot3_rn(dn, n2);
-    double th1 = acos_safe(c1);
-    if (th1 > kHalfPi) th1 = acos_safe(-c1);
-    double th2 = acos_safe(-c2);
-    if (th2 > kHalfPi) th2 = acos_safe(c2);
+    const double th1 = acos_safe(fabs(c1));
+    const double th2 = acos_safe(fabs(c2));
     const double* g_th = g + an.off[0];
     const double* g_hl = g + an.off[1];
     const double* g_co = g + an.off[2];
-    const int it1 = nearest_sorted(g_th, an.n[0], th1), it2 = nearest_sorted(g_th, an.n[0], th2);
-    const int ih1 = nearest_sorted(g_hl, an.n[1], hl1), ih2 = nearest_sorted(g_hl, an.n[1], hl2);
-    const int ic1 = nearest_sorted(g_co, an.n[2], fabs(s) / hl1);
-    const int ic2 = nearest_sorted(g_co, an.n[2], fabs(t) / hl2);
+    const int it1 = nearest_grid(g_th, an.n[0], th1, an.inv_h[0]), it2 = nearest_grid(g_th, an.n[0], th2, an.inv_h[0]);
+    const int ih1 = nearest_grid(g_hl, an.n[1], hl1, an.inv_h[1]), ih2 = nearest_grid(g_hl, an.n[1], hl2, an.inv_h[1]);
+    const int ic1 = nearest_grid(g_co, an.n[2], fabs(s) / hl1, an.inv_h[2]);
+    const int ic2 = nearest_grid(g_co, an.n[2], fabs(t) / hl2, an.inv_h[2]);
     double rx1 = an.curv[((int64_t)it1 * an.n[1] + ih1) * an.n[2] + ic1];
     double rx2 = an.curv[((int64_t)it2 * an.n[1] + ih2) * an.n[2] + ic2];
     if (rx1 > rx2) { const double tmp = rx1; rx1 = rx2; rx2 = tmp; }
     double phi = acos_safe(dot3_rn(n1, n2));
     if (phi > kHalfPi) phi = kPi - phi;
-    const int ir1 = nearest_sorted(g + an.off[3], an.n[3], rx1);
-    const int ir2 = nearest_sorted(g + an.off[3], an.n[3], rx2);
-    const int ip = nearest_sorted(g + an.off[4], an.n[4], phi);
-    const int id = nearest_sorted(g + an.off[5], an.n[5], delta);
+    const int ir1 = nearest_grid(g + an.off[3], an.n[3], rx1, an.inv_h[3]);
+    const int ir2 = nearest_grid(g + an.off[3], an.n[3], rx2, an.inv_h[3]);
+    const int ip = nearest_grid(g + an.off[4], an.n[4], phi, an.inv_h[4]);
+    const int id = nearest_grid(g + an.off[5], an.n[5], delta, an.inv_h[5]);
     const int ng = an.n[6] > 0 ? an.n[6] : 1;
-    const int ig = an.n[6] > 0 ? nearest_sorted(g + an.off[6], an.n[6], gamma) : 0;
+    const int ig = an.n[6] > 0 ? nearest_grid(g + an.off[6], an.n[6], gamma, an.inv_h[6]) : 0;
     const int64_t q = ((((int64_t)ir1 * an.n[3] + ir2) * an.n[4] + ip) * an.n[5] + id) * ng + ig;
     return an.fr[q];
 }

@@ -2477,6 +2477,7 @@ AnisoJKR aniso_params(const bfm_ctx* c)
     for (int k = 0; k < kAnisoGrids; ++k) {
         an.n[k] = c->an_n[k];
         an.off[k] = off;
+        an.inv_h[k] = c->an_inv_h[k];
         off += c->an_n[k];
     }
     an.total = off;
@@ -3515,6 +3516,16 @@ int bfm_set_aniso_jkr_tables(bfm_ctx* c, const bfm_aniso_jkr_tables* t)
                 HIPCHK(hipMemcpy(c->d_an_grid + off, grids[k], sizeof(double) * ns[k], hipMemcpyHostToDevice));
             off += ns[k];
             c->an_n[k] = ns[k];
+            // a first guess of the lookups' interval when the grid is evenly
+            // spaced to within a quarter step (it only saves bisection steps)
+            c->an_inv_h[k] = 0.0;
+            if (ns[k] >= 3) {
+                const double* v = grids[k];
+                const double h = (v[ns[k] - 1] - v[0]) / (ns[k] - 1);
+                bool even = h > 0;
+                for (int q = 0; even && q < ns[k]; ++q) even = std::fabs(v[q] - (v[0] + q * h)) <= 0.25 * h;
+                if (even) c->an_inv_h[k] = 1.0 / h;
+            }
         }
         HIPCHK(hipMemcpy(c->d_an_curv, t->curvature_rx, sizeof(double) * n_curv, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_an_fr, t->force_radius, sizeof(double) * n_fr, hipMemcpyHostToDevice));
