@@ -254,7 +254,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "config-1 static 3-D dome snapshot: one RK stage (binning, sort, "
+        "config": {"workload": "static 200,000-cell 3-D dome snapshot: one RK stage (binning, sort, "
                                "neighbor list, Hertz+JKR force/torque, velocities)",
                    "cells": n, "pairs": m, "pairs_per_cell": m / n, "dome_radius_um": dome_r,
                    "seed": args.seed, "jkr": args.jkr, "neighbor_threshold_um": thr,

@@ -223,8 +223,9 @@ int bfm_dev_get_velocities(bfm_ctx* ctx, const double* d_cells, int n, int ld,
                            const int* d_assume_2d, const double* d_noise,
                            double* d_vel);
 
-/* One config-1 RK stage: binning, sort, neighbor list, Hertz+JKR force and
- * torque, surface forces and velocities, all on device. */
+/* One RK stage of a snapshot with its neighbor list rebuilt: binning, sort,
+ * neighbor list, Hertz+JKR force and torque, surface forces and velocities,
+ * all on device. */
 int bfm_dev_stage(bfm_ctx* ctx, const double* d_cells, int n, int ld,
                   double neighbor_threshold, double R, double Ldiv, double* d_vel);
 
