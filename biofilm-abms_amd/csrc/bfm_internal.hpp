@@ -29,6 +29,10 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
                          double* d_cells_new, double tol, unsigned long long* d_errmax, int* d_bad);
 
 // growCells with per-replicate dt and active flags, cell count from d_n.
+// The columns the attempts do not write (all of them for inactive
+// replicates), from d_cells into d_cells_new, with the last attempt's mask.
+int copy_static_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, const int* d_rep,
+                        const int* d_active, double* d_cells_new);
 int grow_batched(bfm_ctx* c, double* d_cells, int ld, const int* d_n, double R, const int* d_rep,
                  const double* d_dt, const int* d_active);
 

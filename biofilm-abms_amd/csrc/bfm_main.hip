@@ -2144,12 +2144,9 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
         if (c < n) {
             r = rep[c];
             if (!active[r]) {
-                for (int col = 0; col < ncols; ++col) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
-                r = -1;
+                r = -1;   // rows of inactive replicates: k_copy_static_b after the attempts
             } else {
                 const double dt = dt_rep[r];
-                for (int col = 0; col < ncols; ++col)
-                    if (col < 1 || col > 12) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
                 double y[6];
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
@@ -2190,6 +2187,22 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
     }
 }
 
+// The columns the RK attempts do not write, once per iteration after them:
+// all columns but the coordinates and velocities (1-12) of active replicates'
+// cells, every column of the other cells (finished or failed replicates keep
+// the state the iteration started from).
+__global__ void __launch_bounds__(256)
+k_copy_static_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld,
+                const int* __restrict__ rep, const int* __restrict__ active, double* __restrict__ cells_new)
+{
+    const int n = *dn;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const bool on = active[rep[c]] != 0;
+        for (int col = 0; col < ncols; ++col)
+            if (!on || col < 1 || col > 12) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
+    }
+}
+
 // growCells (growth.hpp:51-58) with each replicate's dt (inactive: unchanged).
 __global__ void k_grow_b(const int* __restrict__ dn, double* __restrict__ cells, int ld, double R,
                          const int* __restrict__ rep, const double* __restrict__ dt_rep,
@@ -2220,44 +2233,74 @@ __global__ void k_box_init(unsigned long long* __restrict__ box, int n_rep)
     for (int k = 0; k < 7; ++k) box[r * 7 + k] = order_key(k < 3 ? INFINITY : -INFINITY);
 }
 
-// Per-replicate bounding boxes and max half-length (ordered-key atomics; the
-// block first reduces each run of equal replicate ids).
+// Per-replicate bounding boxes and max half-length (ordered-key atomics).
+// Replicates occupy runs of consecutive cells, so a wavefront whose 64 cells
+// belong to one replicate (the common case) reduces them by shuffles and
+// issues one set of atomics; a wavefront that straddles replicates reduces
+// each run serially from its first lane (at most 64 cells).
 __global__ void __launch_bounds__(256)
 k_bbox_rep(const double* __restrict__ cells, int n, int ld, const int* __restrict__ rep,
            unsigned long long* __restrict__ box)
 {
     __shared__ double s_v[7][256];
     __shared__ int s_rep[256];
+    const int t = threadIdx.x, lane = t & 63, w0 = t & ~63;
     for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
-        const int i = base + threadIdx.x;
+        const int i = base + t;
         const int r = i < n ? rep[i] : -1;
+        double v[7];
         if (r >= 0) {
-            for (int k = 0; k < 3; ++k) {
-                s_v[k][threadIdx.x] = cellv(cells, ld, i, C_RX + k);
-                s_v[3 + k][threadIdx.x] = s_v[k][threadIdx.x];
-            }
-            s_v[6][threadIdx.x] = cellv(cells, ld, i, C_HALF_L);
+            for (int k = 0; k < 3; ++k) v[k] = v[3 + k] = cellv(cells, ld, i, C_RX + k);
+            v[6] = cellv(cells, ld, i, C_HALF_L);
+        } else {
+            for (int k = 0; k < 3; ++k) { v[k] = INFINITY; v[3 + k] = -INFINITY; }
+            v[6] = -INFINITY;
         }
-        s_rep[threadIdx.x] = r;
-        __syncthreads();
-        const int t = threadIdx.x;
-        if (r >= 0 && (t == 0 || s_rep[t - 1] != r)) {
-            double v[7];
-            for (int k = 0; k < 7; ++k) v[k] = s_v[k][t];
-            for (int u = t + 1; u < 256 && s_rep[u] == r; ++u) {
-                for (int k = 0; k < 3; ++k) {
-                    v[k] = fmin(v[k], s_v[k][u]);
-                    v[3 + k] = fmax(v[3 + k], s_v[3 + k][u]);
+        const int r0 = __shfl(r, 0);
+        const bool uniform = __all(r == r0);
+        if (uniform) {
+            if (r0 >= 0) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        v[k] = fmin(v[k], __shfl_xor(v[k], off));
+                        v[3 + k] = fmax(v[3 + k], __shfl_xor(v[3 + k], off));
+                    }
+                    v[6] = fmax(v[6], __shfl_xor(v[6], off));
                 }
-                v[6] = fmax(v[6], s_v[6][u]);
+                if (lane == 0) {
+                    for (int k = 0; k < 3; ++k) {
+                        atomicMin(&box[r0 * 7 + k], order_key(v[k]));
+                        atomicMax(&box[r0 * 7 + 3 + k], order_key(v[3 + k]));
+                    }
+                    atomicMax(&box[r0 * 7 + 6], order_key(v[6]));
+                }
             }
-            for (int k = 0; k < 3; ++k) {
-                atomicMin(&box[r * 7 + k], order_key(v[k]));
-                atomicMax(&box[r * 7 + 3 + k], order_key(v[3 + k]));
+        } else {
+            for (int k = 0; k < 7; ++k) s_v[k][t] = v[k];
+            s_rep[t] = r;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (r >= 0 && (lane == 0 || s_rep[t - 1] != r)) {
+                for (int u = t + 1; u < w0 + 64 && s_rep[u] == r; ++u) {
+                    for (int k = 0; k < 3; ++k) {
+                        v[k] = fmin(v[k], s_v[k][u]);
+                        v[3 + k] = fmax(v[3 + k], s_v[3 + k][u]);
+                    }
+                    v[6] = fmax(v[6], s_v[6][u]);
+                }
+                for (int k = 0; k < 3; ++k) {
+                    atomicMin(&box[r * 7 + k], order_key(v[k]));
+                    atomicMax(&box[r * 7 + 3 + k], order_key(v[3 + k]));
+                }
+                atomicMax(&box[r * 7 + 6], order_key(v[6]));
             }
-            atomicMax(&box[r * 7 + 6], order_key(v[6]));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        __syncthreads();
     }
 }
 
@@ -3262,6 +3305,16 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
             launch_stage(c, 0, d_n, d_cells, ld, nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active, kBatchGrid, false);
         hipLaunchKernelGGL(k_rk_final_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld,
                            tb, d_rep, d_dt, d_active, c->d_k, d_cells_new, tol, d_errmax, d_bad);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+int copy_static_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, const int* d_rep,
+                        const int* d_active, double* d_cells_new)
+{
+    return guarded(c, [&] {
+        hipLaunchKernelGGL(k_copy_static_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld,
+                           d_rep, d_active, d_cells_new);
         HIPCHK(hipGetLastError());
     });
 }

@@ -1043,6 +1043,8 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         // a replicate that failed in the accepted attempt: redo the attempt
         // without it, so its rows are carried over unchanged
         while (check_failures(act, false)) replay(act, dt);
+        // the columns the attempts leave alone, with the last attempt's mask (act)
+        abi(c, copy_static_batched(c, d_cur, ld, d_n.p, c->d_rep, d_active.p, d_new));
         }
         std::swap(d_cur, d_new);
         abi(c, grow_batched(c, d_cur, ld, d_n.p, Rr, c->d_rep, d_dt.p, d_active.p));
