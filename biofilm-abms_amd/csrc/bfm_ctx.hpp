@@ -102,6 +102,7 @@ struct bfm_ctx {
     bool bins_dirty = true;         // histogram not known to be zero (see build_neighbors)
     int* d_bin_off = nullptr;       // max_bins + 1
     double* d_sorted = nullptr;     // 7 x max_cells SoA: x y z nx ny nz hl
+    double* d_geom = nullptr;       // max_cells x 8 packed segments (distance updates)
     int* d_sid = nullptr;           // sorted -> original index
     int* d_skey = nullptr;          // sorted keys
     int* d_count = nullptr;         // per-cell neighbor counts (max_cells + 1)

@@ -1112,19 +1112,44 @@ __device__ __forceinline__ Seg load_cell_seg(const double* __restrict__ cells, i
     return c;
 }
 
-// updateNeighborDistances, mechanics.hpp:604-648.
-__global__ void k_update_distances(const int* __restrict__ d_m, const double* __restrict__ cells, int ld,
-                                   const int* __restrict__ pi, const int* __restrict__ pj,
-                                   double* __restrict__ pdx, double* __restrict__ pdy,
-                                   double* __restrict__ pdz, double* __restrict__ ps,
-                                   double* __restrict__ pt)
+// The segment of every cell (r, n, half_l) as one 64-byte record, so the
+// distance update reads each partner with one cache line instead of seven
+// column reads.
+__global__ void k_pack_segs(int n, const double* __restrict__ cells, int ld, double* __restrict__ geom)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double2* g = reinterpret_cast<double2*>(geom + (int64_t)c * 8);
+        g[0] = make_double2(cellv(cells, ld, c, C_RX), cellv(cells, ld, c, C_RY));
+        g[1] = make_double2(cellv(cells, ld, c, C_RZ), cellv(cells, ld, c, C_NX));
+        g[2] = make_double2(cellv(cells, ld, c, C_NY), cellv(cells, ld, c, C_NZ));
+        g[3] = make_double2(cellv(cells, ld, c, C_HALF_L), 0.0);
+    }
+}
+
+__device__ __forceinline__ Seg load_packed_seg(const double* __restrict__ geom, int c)
+{
+    const double2* g = reinterpret_cast<const double2*>(geom + (int64_t)c * 8);
+    const double2 a = g[0], b = g[1], e = g[2], f = g[3];
+    Seg s;
+    s.r[0] = a.x; s.r[1] = a.y; s.r[2] = b.x;
+    s.n[0] = b.y; s.n[1] = e.x; s.n[2] = e.y;
+    s.hl = f.x;
+    return s;
+}
+
+// updateNeighborDistances (mechanics.hpp:604-648) on packed segments.
+__global__ void k_update_distances_packed(const int* __restrict__ d_m, const double* __restrict__ geom, int n,
+                                          const int* __restrict__ pi, const int* __restrict__ pj,
+                                          double* __restrict__ pdx, double* __restrict__ pdy,
+                                          double* __restrict__ pdz, double* __restrict__ ps,
+                                          double* __restrict__ pt)
 {
     const int m = *d_m;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
         // rows of a build that overflowed its capacity (reported by status) may be stale
-        if ((unsigned)pi[k] >= (unsigned)ld || (unsigned)pj[k] >= (unsigned)ld) continue;
-        Seg ci = load_cell_seg(cells, ld, pi[k]);
-        Seg cj = load_cell_seg(cells, ld, pj[k]);
+        if ((unsigned)pi[k] >= (unsigned)n || (unsigned)pj[k] >= (unsigned)n) continue;
+        const Seg ci = load_packed_seg(geom, pi[k]);
+        const Seg cj = load_packed_seg(geom, pj[k]);
         double d[3], s, t;
         dist_between_cells(ci, cj, d, s, t);
         pdx[k] = d[0]; pdy[k] = d[1]; pdz[k] = d[2]; ps[k] = s; pt[k] = t;
@@ -2894,6 +2919,15 @@ Tableau make_tableau(const double* A, const double* b, const double* bs, int s)
     return tb;
 }
 
+// updateNeighborDistances of the internal list for cells d_cells (n cells).
+void update_distances(bfm_ctx* c, const double* d_cells, int ld, int n)
+{
+    if (n <= 0) return;
+    KLAUNCH(c, "pack_segs", k_pack_segs, grid_for(n, 256), 256, 0, n, d_cells, ld, c->d_geom);
+    KLAUNCH(c, "update_distances", k_update_distances_packed, kPairGrid, 256, 0, c->d_m, c->d_geom, n, c->pi,
+            c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+}
+
 // RK step on device data with the internal list (stepRungeKuttaAdaptive).
 void step_rk_device(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld,
                     const double* d_noise, double* d_cells_new, int ld_new, double* d_errors, double tol,
@@ -2949,8 +2983,7 @@ void step_verlet_device(bfm_ctx* c, double dt, double rho, const double* d_cells
     forces(d_cells, ld);
     KLAUNCH(c, "verlet_half", k_verlet_half, grid_for(n, 256), 256, 0, n, d_cells, ld, F, P, c->p.R, d_new, ld_new,
             wh);
-    KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, d_new, ld_new, c->pi, c->pj,
-            c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+    update_distances(c, d_new, ld_new, n);
     forces(d_new, ld_new);
     KLAUNCH(c, "verlet_final", k_verlet_final, grid_for(n, 256), 256, 0, n, d_new, ld_new, F, P, c->p.R, wh);
 }
@@ -3387,6 +3420,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_bin_count, (size_t)c->max_bins + 1);
         dalloc(&c->d_bin_off, (size_t)c->max_bins + 1);
         dalloc(&c->d_sorted, N * 7);
+        dalloc(&c->d_geom, N * 8);
         dalloc(&c->d_sid, N);
         dalloc(&c->d_skey, N);
         dalloc(&c->d_count, N + 1);
@@ -3444,7 +3478,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
-                    c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst,
+                    c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst, c->d_geom,
                     c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere, c->d_amap};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -3618,8 +3652,7 @@ int bfm_update_neighbor_distances(bfm_ctx* c, const double* cells, int n, int ld
         upload_cells(c, cells, n, ld);
         import_host_neighbors(c, nb, m, ld_nb, n);
         if (m > 0) {
-            KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, c->d_cells,
-                    c->max_cells, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+            update_distances(c, c->d_cells, c->max_cells, n);
             KLAUNCH(c, "export_nb", k_export_nb, grid_for(m, 256, 4096), 256, 0, c->d_m, c->d_nb, (int64_t)m,
                     c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
             HIPCHK(hipMemcpy2DAsync(nb + 2 * ld_nb, sizeof(double) * ld_nb, c->d_nb + 2 * m, sizeof(double) * m,
@@ -3756,8 +3789,7 @@ int bfm_dev_update_neighbor_distances(bfm_ctx* c, const double* d_cells, int n, 
     return guarded(c, [&] {
         check_cells(c, n, ld);
         require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
-        KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj,
-                c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+        update_distances(c, d_cells, ld, n);
     });
 }
 
@@ -4078,8 +4110,7 @@ int bfm2d_dev_update_neighbor_distances(bfm_ctx* c, const double* d_cells, int n
         require(n >= 0 && n <= c->max_cells && ld >= n, BFM_ERR_ARG, "invalid n or ld");
         require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
         KLAUNCH(c, "embed_2d", k2d_embed, grid_for(n, 256), 256, 0, n, d_cells, ld, c->d_cells_new, c->max_cells);
-        KLAUNCH(c, "update_distances", k_update_distances, kPairGrid, 256, 0, c->d_m, c->d_cells_new, c->max_cells,
-                c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+        update_distances(c, c->d_cells_new, c->max_cells, n);
     });
 }
 
