@@ -2151,8 +2151,9 @@ k_rk_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau t
 // k_rk_final for batched replicates: device cell count, per-replicate dt and
 // active mask (inactive replicates' rows are copied unchanged), and the
 // scaled-error maximum per replicate (errmax[r]). Replicates occupy runs of
-// consecutive rows, so each block reduces its 256 values per run before one
-// atomic per run.
+// consecutive rows, so a wavefront whose 64 rows belong to one replicate (the
+// common case) reduces them by shuffles before one atomic; a wavefront that
+// straddles replicates reduces each run serially from its first lane.
 __global__ void __launch_bounds__(256)
 k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld, Tableau tb,
              const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
@@ -2162,8 +2163,9 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
     __shared__ double s_val[256];
     __shared__ int s_rep[256];
     const int n = *dn;
+    const int t = threadIdx.x, lane = t & 63, w0 = t & ~63;
     for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
-        const int c = base + threadIdx.x;
+        const int c = base + t;
         double local = -INFINITY;
         int r = -1;
         if (c < n) {
@@ -2199,16 +2201,28 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
                 if (bad) atomicOr(&bad_rep[r], 1);
             }
         }
-        s_val[threadIdx.x] = local;
-        s_rep[threadIdx.x] = r;
-        __syncthreads();
-        const int t = threadIdx.x;
-        if (r >= 0 && (t == 0 || s_rep[t - 1] != r)) {
-            double m = local;
-            for (int u = t + 1; u < 256 && s_rep[u] == r; ++u) m = fmax(m, s_val[u]);
-            atomicMax(&errmax[r], order_key(m));
+        const int r0 = __shfl(r, 0);
+        if (__all(r == r0)) {
+            if (r0 >= 0) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) local = fmax(local, __shfl_xor(local, off));
+                if (lane == 0) atomicMax(&errmax[r0], order_key(local));
+            }
+        } else {
+            s_val[t] = local;
+            s_rep[t] = r;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (r >= 0 && (lane == 0 || s_rep[t - 1] != r)) {
+                double m = local;
+                for (int u = t + 1; u < w0 + 64 && s_rep[u] == r; ++u) m = fmax(m, s_val[u]);
+                atomicMax(&errmax[r], order_key(m));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        __syncthreads();
     }
 }
 
