@@ -2394,27 +2394,46 @@ k_plan_rep(const unsigned long long* __restrict__ box, int n_rep, double thr, do
         if (t == 0) s_edge = edge * 1.25;
         __syncthreads();
     }
-    if (t != 0) return;
     const double edge = s_edge;
+    // each thread builds its replicates' grids; bases are the exclusive
+    // prefix sums of the bin counts, 256 replicates per block scan
+    __shared__ int s_scan[256];
+    int base = 0;
+    for (int c0 = 0; c0 < n_rep; c0 += 256) {
+        const int r = c0 + t;
+        RepGrid G;
+        int cnt = 0;
+        if (r < n_rep) {
+            const double lo0 = order_val(box[r * 7]);
+            if (lo0 <= order_val(box[r * 7 + 3])) {
+                for (int k = 0; k < 3; ++k) {
+                    G.lo[k] = order_val(box[r * 7 + k]);
+                    G.dims[k] = (int)floor((order_val(box[r * 7 + 3 + k]) - G.lo[k]) / edge) + 1;
+                }
+            } else {
+                for (int k = 0; k < 3; ++k) { G.lo[k] = 0; G.dims[k] = 1; }
+            }
+            cnt = G.dims[0] * G.dims[1] * G.dims[2];
+        }
+        s_scan[t] = cnt;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int v = t >= off ? s_scan[t - off] : 0;
+            __syncthreads();
+            s_scan[t] += v;
+            __syncthreads();
+        }
+        if (r < n_rep) {
+            G.base = base + s_scan[t] - cnt;
+            grid[r] = G;
+        }
+        base += s_scan[255];
+        __syncthreads();
+    }
+    if (t != 0) return;
     int reach = (int)ceil(cut / edge - 1e-12);
     if (reach < 1) reach = 1;
     if (reach * edge < cut) reach++;
-    int base = 0;
-    for (int r = 0; r < n_rep; ++r) {
-        RepGrid G;
-        const double lo0 = order_val(box[r * 7]);
-        if (lo0 <= order_val(box[r * 7 + 3])) {
-            for (int k = 0; k < 3; ++k) {
-                G.lo[k] = order_val(box[r * 7 + k]);
-                G.dims[k] = (int)floor((order_val(box[r * 7 + 3 + k]) - G.lo[k]) / edge) + 1;
-            }
-        } else {
-            for (int k = 0; k < 3; ++k) { G.lo[k] = 0; G.dims[k] = 1; }
-        }
-        G.base = base;
-        base += G.dims[0] * G.dims[1] * G.dims[2];
-        grid[r] = G;
-    }
     Plan P;
     for (int k = 0; k < 3; ++k) { P.lo[k] = 0; P.dims[k] = 1; }
     P.edge = edge;
