@@ -2273,10 +2273,13 @@ __global__ void k_box_init(unsigned long long* __restrict__ box, int n_rep)
 }
 
 // Per-replicate bounding boxes and max half-length (ordered-key atomics).
-// Replicates occupy runs of consecutive cells, so a wavefront whose 64 cells
-// belong to one replicate (the common case) reduces them by shuffles and
-// issues one set of atomics; a wavefront that straddles replicates reduces
-// each run serially from its first lane (at most 64 cells).
+// Each block reduces one contiguous chunk of cells. Replicates occupy runs of
+// consecutive cells, so a wavefront keeps per-lane minima and maxima while its
+// successive 64-cell slices stay in one replicate and reduces them by shuffles
+// into one set of atomics when the replicate changes or the chunk ends; a
+// slice that straddles replicates reduces each run serially from its first
+// lane (at most 64 cells). Minima and maxima are exact, so the order of the
+// reduction does not change the result.
 __global__ void __launch_bounds__(256)
 k_bbox_rep(const double* __restrict__ cells, int n, int ld, const int* __restrict__ rep,
            unsigned long long* __restrict__ box)
@@ -2284,63 +2287,87 @@ k_bbox_rep(const double* __restrict__ cells, int n, int ld, const int* __restric
     __shared__ double s_v[7][256];
     __shared__ int s_rep[256];
     const int t = threadIdx.x, lane = t & 63, w0 = t & ~63;
-    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
-        const int i = base + t;
-        const int r = i < n ? rep[i] : -1;
+    const int64_t chunk = (((int64_t)n + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+    const int64_t b0 = blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : (int64_t)n;
+    double acc[7];
+    int wr = -1;   // replicate of the wavefront's accumulators (wavefront-uniform)
+    auto reset = [&] {
+        for (int k = 0; k < 3; ++k) { acc[k] = INFINITY; acc[3 + k] = -INFINITY; }
+        acc[6] = -INFINITY;
+    };
+    auto flush = [&] {
+        if (wr < 0) return;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                acc[k] = fmin(acc[k], __shfl_xor(acc[k], off));
+                acc[3 + k] = fmax(acc[3 + k], __shfl_xor(acc[3 + k], off));
+            }
+            acc[6] = fmax(acc[6], __shfl_xor(acc[6], off));
+        }
+        if (lane == 0) {
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&box[wr * 7 + k], order_key(acc[k]));
+                atomicMax(&box[wr * 7 + 3 + k], order_key(acc[3 + k]));
+            }
+            atomicMax(&box[wr * 7 + 6], order_key(acc[6]));
+        }
+    };
+    reset();
+    for (int64_t base = b0; base < b1; base += 256) {
+        const int64_t i = base + t;
+        const int r = i < b1 ? rep[i] : -1;
         double v[7];
         if (r >= 0) {
-            for (int k = 0; k < 3; ++k) v[k] = v[3 + k] = cellv(cells, ld, i, C_RX + k);
-            v[6] = cellv(cells, ld, i, C_HALF_L);
+            for (int k = 0; k < 3; ++k) v[k] = v[3 + k] = cellv(cells, ld, (int)i, C_RX + k);
+            v[6] = cellv(cells, ld, (int)i, C_HALF_L);
         } else {
             for (int k = 0; k < 3; ++k) { v[k] = INFINITY; v[3 + k] = -INFINITY; }
             v[6] = -INFINITY;
         }
         const int r0 = __shfl(r, 0);
         const bool uniform = __all(r == r0);
+        if (uniform && r0 == wr) {
+            for (int k = 0; k < 3; ++k) { acc[k] = fmin(acc[k], v[k]); acc[3 + k] = fmax(acc[3 + k], v[3 + k]); }
+            acc[6] = fmax(acc[6], v[6]);
+            continue;
+        }
+        flush();
+        reset();
+        wr = -1;
         if (uniform) {
             if (r0 >= 0) {
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        v[k] = fmin(v[k], __shfl_xor(v[k], off));
-                        v[3 + k] = fmax(v[3 + k], __shfl_xor(v[3 + k], off));
-                    }
-                    v[6] = fmax(v[6], __shfl_xor(v[6], off));
-                }
-                if (lane == 0) {
-                    for (int k = 0; k < 3; ++k) {
-                        atomicMin(&box[r0 * 7 + k], order_key(v[k]));
-                        atomicMax(&box[r0 * 7 + 3 + k], order_key(v[3 + k]));
-                    }
-                    atomicMax(&box[r0 * 7 + 6], order_key(v[6]));
-                }
+                wr = r0;
+                for (int k = 0; k < 7; ++k) acc[k] = v[k];
             }
-        } else {
-            for (int k = 0; k < 7; ++k) s_v[k][t] = v[k];
-            s_rep[t] = r;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (r >= 0 && (lane == 0 || s_rep[t - 1] != r)) {
-                for (int u = t + 1; u < w0 + 64 && s_rep[u] == r; ++u) {
-                    for (int k = 0; k < 3; ++k) {
-                        v[k] = fmin(v[k], s_v[k][u]);
-                        v[3 + k] = fmax(v[3 + k], s_v[3 + k][u]);
-                    }
-                    v[6] = fmax(v[6], s_v[6][u]);
-                }
-                for (int k = 0; k < 3; ++k) {
-                    atomicMin(&box[r * 7 + k], order_key(v[k]));
-                    atomicMax(&box[r * 7 + 3 + k], order_key(v[3 + k]));
-                }
-                atomicMax(&box[r * 7 + 6], order_key(v[6]));
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            continue;
         }
+        for (int k = 0; k < 7; ++k) s_v[k][t] = v[k];
+        s_rep[t] = r;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (r >= 0 && (lane == 0 || s_rep[t - 1] != r)) {
+            for (int u = t + 1; u < w0 + 64 && s_rep[u] == r; ++u) {
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = fmin(v[k], s_v[k][u]);
+                    v[3 + k] = fmax(v[3 + k], s_v[3 + k][u]);
+                }
+                v[6] = fmax(v[6], s_v[6][u]);
+            }
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&box[r * 7 + k], order_key(v[k]));
+                atomicMax(&box[r * 7 + 3 + k], order_key(v[3 + k]));
+            }
+            atomicMax(&box[r * 7 + 6], order_key(v[6]));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    flush();
 }
 
 // Grid plans of all replicates with one shared edge (so one stencil reach):
@@ -2658,7 +2685,7 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     const double pre = thr + Ldiv + 2 * R;
     if (c->n_rep > 0) {
         KLAUNCH(c, "bbox", k_box_init, grid_for(c->n_rep, 256), 256, 0, c->d_rep_box, c->n_rep);
-        KLAUNCH(c, "bbox", k_bbox_rep, std::min(grid_for(n, 256), 2048), 256, 0, d_cells, n, ld, c->d_rep,
+        KLAUNCH(c, "bbox", k_bbox_rep, std::min(grid_for(n, 2048), 1024), 256, 0, d_cells, n, ld, c->d_rep,
                 c->d_rep_box);
         KLAUNCH(c, "plan", k_plan_rep, 1, 256, 0, c->d_rep_box, c->n_rep, thr, pre, c->bin_scale, c->max_bins,
                 c->d_plan, c->d_grid);
