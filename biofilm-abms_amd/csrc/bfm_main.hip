@@ -2150,10 +2150,12 @@ k_rk_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau t
 
 // k_rk_final for batched replicates: device cell count, per-replicate dt and
 // active mask (inactive replicates' rows are copied unchanged), and the
-// scaled-error maximum per replicate (errmax[r]). Replicates occupy runs of
-// consecutive rows, so a wavefront whose 64 rows belong to one replicate (the
-// common case) reduces them by shuffles before one atomic; a wavefront that
-// straddles replicates reduces each run serially from its first lane.
+// scaled-error maximum per replicate (errmax[r]). Each block updates one
+// contiguous chunk of rows. Replicates occupy runs of consecutive rows, so a
+// wavefront keeps a per-lane maximum while its successive 64-row slices stay
+// in one replicate and reduces it by shuffles into one atomic when the
+// replicate changes or the chunk ends; a slice that straddles replicates
+// reduces each run serially from its first lane.
 __global__ void __launch_bounds__(256)
 k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld, Tableau tb,
              const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
@@ -2164,11 +2166,22 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
     __shared__ int s_rep[256];
     const int n = *dn;
     const int t = threadIdx.x, lane = t & 63, w0 = t & ~63;
-    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
-        const int c = base + t;
+    const int64_t chunk = (((int64_t)n + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+    const int64_t b0 = blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : (int64_t)n;
+    double acc = -INFINITY;
+    int wr = -1;   // replicate of the wavefront's running maximum (wavefront-uniform)
+    auto flush = [&] {
+        if (wr < 0) return;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
+        if (lane == 0) atomicMax(&errmax[wr], order_key(acc));
+    };
+    for (int64_t base = b0; base < b1; base += 256) {
+        const int c = (int)(base + t);
         double local = -INFINITY;
         int r = -1;
-        if (c < n) {
+        if (c < b1) {
             r = rep[c];
             if (!active[r]) {
                 r = -1;   // rows of inactive replicates: k_copy_static_b after the attempts
@@ -2202,11 +2215,18 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
             }
         }
         const int r0 = __shfl(r, 0);
-        if (__all(r == r0)) {
+        const bool uniform = __all(r == r0);
+        if (uniform && r0 == wr) {
+            acc = fmax(acc, local);
+            continue;
+        }
+        flush();
+        acc = -INFINITY;
+        wr = -1;
+        if (uniform) {
             if (r0 >= 0) {
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) local = fmax(local, __shfl_xor(local, off));
-                if (lane == 0) atomicMax(&errmax[r0], order_key(local));
+                wr = r0;
+                acc = local;
             }
         } else {
             s_val[t] = local;
@@ -2224,6 +2244,7 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
+    flush();
 }
 
 // The columns the RK attempts do not write, once per iteration after them:
