@@ -146,6 +146,57 @@ def test_tilted_velocity_solve_residual(orc, nz):
     assert np.allclose(v, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
 
 
+def _integral1(rz, nz, hl, g):
+    ss = (R - rz) / nz
+    return float(_quad(lambda s: _delta(rz, nz, s) ** g if _delta(rz, nz, s) > 0 else 0, hl, ss))
+
+
+@pytest.mark.parametrize("nz", [0.25, 0.6, 0.9])
+@pytest.mark.parametrize("coeff", [0.5, 1.0])
+def test_tilted_velocity_solve_coulomb_cap(orc, nz, coeff):
+    """Tilted contacting cell with large stored velocities: the surface friction
+    coefficient is truncated so that (eta1_eff / R)(A0 |dr_xy| + A1 |dn_xy|)
+    equals 2 E0 coeff [(1 - nz^2) I1(1) + sqrt(R) nz^2 I1(1/2)]
+    (mechanics.hpp:2246-2262), and the velocities solve the 7x7 constrained
+    system built with eta1_eff. Integrals by quadrature."""
+    hl, eta0, eta1, E0 = 0.9, 0.7, 720.0, 3900.0
+    rz = R + hl * nz - 0.15
+    dr, dn = (40.0, -25.0), (12.0, 30.0)
+    row = _cell_row(rz, nz, hl, eta0=eta0, eta1=eta1, nx=0.6 * math.sqrt(1 - nz * nz), dr=dr, dn=dn)
+    cells = row[None, :]
+    p = _params(cell_surface_coulomb_coeff=coeff)
+    a2d = np.zeros(1, np.int32)
+    noise = np.array([[3.0, -2.0, 1.5, 0.7, -0.4, 0.9]])
+    nb = np.zeros((0, 7), order="F")
+    F = orc.conservative_forces(cells, nb, a2d, p)[0] + noise[0]
+    v = orc.get_velocities(cells, nb, a2d, noise, p)[0]
+    mrz, mnz, mhl = mp.mpf(rz), mp.mpf(nz), mp.mpf(hl)
+    A1, A2, A3 = _area(mrz, mnz, mhl)
+    drxy, dnxy = math.hypot(*dr), math.hypot(*dn)
+    max_norm = 2 * E0 * coeff * ((1 - nz * nz) * _integral1(mrz, mnz, mhl, 1)
+                                 + math.sqrt(R) * nz * nz * _integral1(mrz, mnz, mhl, mp.mpf(0.5)))
+    friction = (eta1 / R) * (A1 * drxy + A2 * dnxy)
+    assert friction > 2 * max_norm          # the cap is active in these cases
+    eta1_eff = max_norm * R / (A1 * drxy + A2 * dnxy)
+    assert abs((eta1_eff / R) * (A1 * drxy + A2 * dnxy) - max_norm) < 1e-12 * max_norm
+    l = 2 * hl
+
+    def drag(e1):
+        M = np.zeros((6, 6))
+        M[0, 0] = M[1, 1] = eta0 * l + e1 / R * A1
+        M[0, 3] = M[1, 4] = M[3, 0] = M[4, 1] = e1 / R * A2
+        M[2, 2] = eta0 * l
+        M[3, 3] = M[4, 4] = eta0 * l ** 3 / 12 + e1 / R * A3
+        M[5, 5] = eta0 * l ** 3 / 12
+        return M
+
+    ref = _solve7(drag(eta1_eff), row[4:7], F)
+    assert np.allclose(v, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    untruncated = _solve7(drag(eta1), row[4:7], F)
+    assert not np.allclose(v, untruncated, rtol=1e-3)
+    assert abs(np.dot(v[3:], row[4:7])) < 1e-12 * np.abs(v[3:]).max()
+
+
 def test_free_cell_velocity_solve(orc):
     row = _cell_row(5.0, 0.6, 0.8)
     cells = row[None, :]
