@@ -205,6 +205,8 @@ def load_library():
         "bfm_dev_import_neighbors": [vp, vp, C.c_int64, C.c_int64, C.c_int],
         "bfm_dev_get_velocities": [vp, vp, C.c_int, C.c_int, vp, vp, vp],
         "bfm_dev_stage": [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, vp],
+        "bfm_dev_get_velocities_noalloc": [C.POINTER(Params), vp, C.c_int, C.c_int, vp, C.c_int64,
+                                           C.c_int64, vp, vp, vp, vp],
         "bfm_dev_step_rk": [vp, _dp, _dp, _dp, C.c_int, C.c_double, vp, C.c_int, C.c_int, vp, vp,
                             vp, C.c_double, vp],
         "bfm_enable_timing": [vp, C.c_int],
@@ -248,6 +250,20 @@ def load_library():
     lib.bfm_free_aniso_jkr_tables.restype = None
     _LIB = lib
     return lib
+
+
+def dev_get_velocities_noalloc(params, d_cells, n, ld, d_neighbors, m, ld_nb, d_vel, d_a2d=0,
+                               d_noise=0, stream=0):
+    """bfm_dev_get_velocities_noalloc: getVelocities on device pointers with no
+    device allocation and no context (n <= 65536, m <= 262144). Raises
+    BFMError with the returned status."""
+    lib = load_library()
+    rc = lib.bfm_dev_get_velocities_noalloc(C.byref(params), C.c_void_p(d_cells), int(n), int(ld),
+                                            C.c_void_p(d_neighbors or None), int(m), int(ld_nb),
+                                            C.c_void_p(d_a2d or None), C.c_void_p(d_noise or None),
+                                            C.c_void_p(d_vel), C.c_void_p(stream or None))
+    if rc != BFM_OK:
+        raise BFMError(rc, "bfm_dev_get_velocities_noalloc")
 
 
 def _f64(a):

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2599,21 +2600,23 @@ void check_cells(bfm_ctx* c, int n, int ld)
     require(ld >= n, BFM_ERR_ARG, "ld < n");
 }
 
-ForceParams force_params(const bfm_ctx* c)
+ForceParams force_params_of(const bfm_params& p)
 {
     ForceParams fp;
-    fp.R = c->p.R; fp.Rcell = c->p.Rcell; fp.E0 = c->p.E0;
-    fp.Req = c->p.R / 2;
-    fp.max_overlap = 2 * (c->p.R - c->p.Rcell);
-    fp.pref0 = c->p.repulsion_prefactors[0];
-    fp.pref1 = c->p.repulsion_prefactors[1];
-    fp.pref2 = c->p.repulsion_prefactors[2];
-    fp.adhesion = c->p.adhesion_mode != BFM_ADHESION_NONE;
-    fp.jkr_mode = c->p.jkr_mode;
-    fp.colidx_gamma = c->p.colidx_gamma;
+    fp.R = p.R; fp.Rcell = p.Rcell; fp.E0 = p.E0;
+    fp.Req = p.R / 2;
+    fp.max_overlap = 2 * (p.R - p.Rcell);
+    fp.pref0 = p.repulsion_prefactors[0];
+    fp.pref1 = p.repulsion_prefactors[1];
+    fp.pref2 = p.repulsion_prefactors[2];
+    fp.adhesion = p.adhesion_mode != BFM_ADHESION_NONE;
+    fp.jkr_mode = p.jkr_mode;
+    fp.colidx_gamma = p.colidx_gamma;
     fp.pad = 0;
     return fp;
 }
+
+ForceParams force_params(const bfm_ctx* c) { return force_params_of(c->p); }
 
 AnisoJKR aniso_params(const bfm_ctx* c)
 {
@@ -2633,10 +2636,10 @@ AnisoJKR aniso_params(const bfm_ctx* c)
     return an;
 }
 
-VelParams vel_params(const bfm_ctx* c)
+VelParams vel_params_of(const bfm_params& p)
 {
     VelParams vp;
-    const double R = c->p.R, E0 = c->p.E0;
+    const double R = p.R, E0 = p.E0;
     vp.sc.R = R; vp.sc.E0 = E0; vp.sc.sqR = std::sqrt(R);
     vp.sc.rep0 = 2 * E0;
     vp.sc.rep1 = (8. / 3.) * E0 * std::pow(R, 0.5);
@@ -2644,12 +2647,14 @@ VelParams vel_params(const bfm_ctx* c)
     vp.sc.adh0 = std::pow(R, 0.5) / 2;
     vp.sc.adh1 = 2 * bfm::kPi * R;
     vp.sc.adh2 = 2 * std::pow(R, 0.5);
-    vp.nz_threshold = c->p.nz_threshold;
-    vp.coeff = c->p.cell_surface_coulomb_coeff;
-    vp.no_surface = c->p.no_surface;
+    vp.nz_threshold = p.nz_threshold;
+    vp.coeff = p.cell_surface_coulomb_coeff;
+    vp.no_surface = p.no_surface;
     vp.has_noise = 0;
     return vp;
 }
+
+VelParams vel_params(const bfm_ctx* c) { return vel_params_of(c->p); }
 
 void scan(bfm_ctx* c, const int* in, int* out, int64_t items, const char* name)
 {
@@ -4216,3 +4221,185 @@ int bfm2d_dev_step_rk(bfm_ctx* c, const double* A, const double* b, const double
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// getVelocities without allocation, for callers that cannot allocate device
+// memory (code under stream capture, serving loops). Every buffer beyond the
+// caller's inputs and outputs lives in `scratch`, a fixed-size device array
+// of this library; each GPU holds its own copy until the process ends.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNaMaxCells = 65536;
+constexpr int64_t kNaMaxRows = 262144;
+constexpr int kNaMaxDevices = 64;
+// int region after the pair records: pi, pj, perm_i, perm_j (kNaMaxRows each),
+// icnt, jcnt, ioff, joff (kNaMaxCells + 1 each), m, status.
+constexpr int64_t kNaInts = 4 * kNaMaxRows + 4 * ((int64_t)kNaMaxCells + 1) + 2;
+constexpr int64_t kNaDoubles = 8 * kNaMaxRows + (kNaInts + 1) / 2;   // ~21 MB
+constexpr int kNaBadIndex = 1 << 16;     // status bit: a row index out of range
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+std::mutex g_na_mu;
+double* g_na_ptr[kNaMaxDevices] = {};
+
+// The device address of `scratch` on device dev (current), looked up once per
+// device; nullptr on failure.
+double* na_scratch(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_na_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_na_ptr[dev];
+}
+
+__global__ void k_na_init(int* __restrict__ d_m, int m, int* __restrict__ status)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *d_m = m;
+        *status = 0;
+    }
+}
+
+// Row indices of an m x 7 neighbor array as int32; a row whose i or j is not
+// an integer in [0, n) is mapped to cell n (counted in the spare last entry of
+// the per-cell counts, gathered by no cell) and flagged.
+__global__ void k_na_ij(const double* __restrict__ nb, int64_t m, int64_t ld, int n, int* __restrict__ pi,
+                        int* __restrict__ pj, int* __restrict__ status)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+        const double vi = nb[k], vj = nb[ld + k];
+        const bool ok = vi >= 0 && vi < n && vj >= 0 && vj < n && vi == (double)(int)vi && vj == (double)(int)vj;
+        pi[k] = ok ? (int)vi : n;
+        pj[k] = ok ? (int)vj : n;
+        if (!ok) atomicOr(status, kNaBadIndex);
+    }
+}
+
+// Exclusive prefix sum of len <= kNaMaxCells + 1 ints in one work-group:
+// each thread sums a contiguous chunk, the 1024 chunk sums are scanned in LDS.
+__global__ void __launch_bounds__(1024) k_na_scan(const int* __restrict__ in, int* __restrict__ out, int len)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (len + 1023) / 1024;
+    const int a = min(t * chunk, len), b = min(a + chunk, len);
+    int s = 0;
+    for (int q = a; q < b; ++q) s += in[q];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int q = a; q < b; ++q) {
+        const int v = in[q];
+        out[q] = run;
+        run += v;
+    }
+}
+
+int na_status_code(int st)
+{
+    if (st & kNaBadIndex) return BFM_ERR_ARG;
+    if (st & ST_JKR_ROOT) return BFM_ERR_JKR_ROOT;
+    if (st & ST_NAN) return BFM_ERR_NAN;
+    return st ? BFM_ERR_HIP : BFM_OK;
+}
+
+}  // namespace
+
+extern "C" int bfm_dev_get_velocities_noalloc(const bfm_params* p, const double* d_cells, int n, int ld,
+                                              const double* d_neighbors, int64_t m, int64_t ld_nb,
+                                              const int* d_assume_2d, const double* d_noise, double* d_vel,
+                                              void* stream)
+{
+    if (!p) return BFM_ERR_ARG;
+    if (n < 0 || m < 0 || ld < n || (m > 0 && ld_nb < m)) return BFM_ERR_ARG;
+    if (n > kNaMaxCells || m > kNaMaxRows) return BFM_ERR_CAPACITY;
+    if (p->adhesion_mode != BFM_ADHESION_NONE &&
+        !(p->adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC && p->jkr_mode == BFM_JKR_EXACT))
+        return BFM_ERR_UNSUPPORTED;
+    if (p->friction_mode != 0) return BFM_ERR_UNSUPPORTED;
+    if (p->adhesion_mode != BFM_ADHESION_NONE && (p->colidx_gamma < 0 || p->colidx_gamma >= p->ncols))
+        return BFM_ERR_ARG;
+    if (n == 0) return BFM_OK;
+    if (!d_cells || !d_vel || (m > 0 && !d_neighbors)) return BFM_ERR_ARG;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDevices) return BFM_ERR_HIP;
+    double* base = na_scratch(dev);
+    if (!base) return BFM_ERR_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    double* rec = base;
+    int* ip = reinterpret_cast<int*>(base + 8 * kNaMaxRows);
+    int* pi = ip;
+    int* pj = pi + kNaMaxRows;
+    int* perm_i = pj + kNaMaxRows;
+    int* perm_j = perm_i + kNaMaxRows;
+    int* icnt = perm_j + kNaMaxRows;
+    int* jcnt = icnt + (kNaMaxCells + 1);
+    int* ioff = jcnt + (kNaMaxCells + 1);
+    int* joff = ioff + (kNaMaxCells + 1);
+    int* d_m = joff + (kNaMaxCells + 1);
+    int* status = d_m + 1;
+
+    const int64_t nc = (int64_t)n + 1;
+    const int gc = grid_for(nc, 256, 4096);
+    hipLaunchKernelGGL(k_na_init, dim3(1), dim3(64), 0, s, d_m, (int)m, status);
+    hipLaunchKernelGGL(k_fill_int, dim3(gc), dim3(256), 0, s, jcnt, 0, nc);
+    hipLaunchKernelGGL(k_fill_int, dim3(gc), dim3(256), 0, s, icnt, 0, nc);
+    if (m > 0)
+        hipLaunchKernelGGL(k_na_ij, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, d_neighbors, m, ld_nb, n, pi,
+                           pj, status);
+    // pair records and the j-role counts (the imported-list path of
+    // bfm_dev_get_velocities: the same kernels in the same order)
+    const double* pdx = m > 0 ? d_neighbors + 2 * ld_nb : d_cells;   // not read when m == 0
+    hipLaunchKernelGGL(k_pair_forces<false>, dim3(kPairGrid), dim3(256), 0, s, (const int*)d_m, d_cells, ld,
+                       (const int*)pi, (const int*)pj, pdx, pdx + ld_nb, pdx + 2 * ld_nb, force_params_of(*p),
+                       (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, 0, 0, AnisoJKR{},
+                       (const double*)nullptr, 0, pdx + 3 * ld_nb, pdx + 4 * ld_nb, (const double*)nullptr, rec,
+                       jcnt, status);
+    hipLaunchKernelGGL(k_na_scan, dim3(1), dim3(1024), 0, s, (const int*)jcnt, joff, (int)nc);
+    hipLaunchKernelGGL(k_fill_int, dim3(gc), dim3(256), 0, s, jcnt, 0, nc);
+    hipLaunchKernelGGL(k_fill_perm, dim3(kPairGrid), dim3(256), 0, s, (const int*)d_m, (const int*)pj,
+                       (const int*)joff, jcnt, perm_j);
+    hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const int*)joff, perm_j);
+    hipLaunchKernelGGL(k_count_key, dim3(kPairGrid), dim3(256), 0, s, (const int*)d_m, (const int*)pi, icnt);
+    hipLaunchKernelGGL(k_na_scan, dim3(1), dim3(1024), 0, s, (const int*)icnt, ioff, (int)nc);
+    hipLaunchKernelGGL(k_fill_int, dim3(gc), dim3(256), 0, s, icnt, 0, nc);
+    hipLaunchKernelGGL(k_fill_perm, dim3(kPairGrid), dim3(256), 0, s, (const int*)d_m, (const int*)pi,
+                       (const int*)ioff, icnt, perm_i);
+    hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const int*)ioff, perm_i);
+    RowLists L;
+    L.off64 = nullptr;
+    L.csr = nullptr;
+    L.cap = kNaMaxRows;
+    L.ioff = ioff;
+    L.perm_i = perm_i;
+    L.joff = joff;
+    L.perm_j = perm_j;
+    FricGather fg;
+    fg.frec = nullptr;
+    fg.ps = nullptr;
+    fg.pt = nullptr;
+    hipLaunchKernelGGL(k_velocities<false>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, d_cells, ld, d_assume_2d,
+                       d_noise, vel_params_of(*p), L, (const double*)rec, fg, d_vel, (double*)nullptr, status);
+    if (hipGetLastError() != hipSuccess) return BFM_ERR_HIP;
+    // Outside a stream capture the call is synchronous and reports the
+    // kernels' status (bad index, JKR root, non-finite velocity).
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) return BFM_ERR_HIP;
+    if (cs != hipStreamCaptureStatusNone) return BFM_OK;
+    int st = 0;
+    if (hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return BFM_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return BFM_ERR_HIP;
+    return na_status_code(st);
+}

@@ -223,6 +223,27 @@ int bfm_dev_get_velocities(bfm_ctx* ctx, const double* d_cells, int n, int ld,
                            const int* d_assume_2d, const double* d_noise,
                            double* d_vel);
 
+/* getVelocities on device arrays without any device allocation and without
+ * a context. No counterpart in the reference: this is for callers that cannot
+ * allocate device memory (code under HIP stream capture, serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and results as
+ * bfm_dev_import_neighbors + bfm_dev_get_velocities (bit-identical
+ * velocities): d_neighbors is a device m x 7 column-major array (leading
+ * dimension ld_nb) in the reference's row order; d_assume_2d and d_noise may
+ * be NULL; stream may be NULL (the default stream); runs on the current
+ * device. Limits: n <= 65536 cells, m <= 262144 rows (BFM_ERR_CAPACITY
+ * beyond); adhesion NONE or isotropic JKR with the exact solve, no friction
+ * (BFM_ERR_UNSUPPORTED otherwise). Its working memory is a fixed ~21 MB
+ * device array inside the library, one per GPU, held until the process ends
+ * and shared by every call on that GPU: calls that overlap on one GPU (other
+ * host threads, other streams) race. Outside a stream capture the call waits
+ * for its kernels and returns BFM_ERR_ARG for a row index out of range,
+ * BFM_ERR_JKR_ROOT or BFM_ERR_NAN; under capture it only enqueues. */
+int bfm_dev_get_velocities_noalloc(const bfm_params* params, const double* d_cells, int n, int ld,
+                                   const double* d_neighbors, int64_t m, int64_t ld_nb,
+                                   const int* d_assume_2d, const double* d_noise, double* d_vel,
+                                   void* stream);
+
 /* One RK stage of a snapshot with its neighbor list rebuilt: binning, sort,
  * neighbor list, Hertz+JKR force and torque, surface forces and velocities,
  * all on device. */
