@@ -412,8 +412,8 @@ def layer2d(args):
 
 def ensemble(args):
     """Batched replicates: --ens-reps domes with log-uniform cell counts in
-    [--ens-lo, --ens-hi] (seeds 0..R-1), two phenotypes (gamma 50 / 500 Pa um)
-    assigned 50/50 by a seeded draw, advanced --ens-minutes simulated minutes
+    [--ens-lo, --ens-hi] (seeds 0..R-1), two phenotypes (gamma 500 / 50 Pa um,
+    groups 1 / 2) assigned 50/50 per cell by a seeded draw, advanced --ens-minutes simulated minutes
     (or until --ens-max-wall seconds) by bfm_ensemble_simulate with the
     autotuner and the learned JKR radius. value = RK stage cell-updates per
     second over the whole run (every attempt of every replicate)."""
@@ -425,7 +425,8 @@ def ensemble(args):
 
     sizes = bfm.ensemble_sizes(args.ens_reps, args.ens_lo, args.ens_hi, seed=0)
     # half-lengths below Ldiv / 2 - R: no cell starts past the division length
-    reps, pheno = bfm.make_ensemble(sizes, seeds=range(args.ens_reps), hl_hi=1.2)
+    reps, groups = bfm.make_ensemble(sizes, seeds=range(args.ens_reps), hl_hi=1.2)
+    group_all = np.concatenate(groups)
     total = int(sizes.sum())
     cap = int(total * 1.3) + 1024
     params = make_params(jkr_mode=JKR_SURROGATE if args.ens_jkr == "surrogate" else JKR_EXACT)
@@ -460,10 +461,11 @@ def ensemble(args):
              "value": value, "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True,
              "vs_baseline": None, "dtype": "f64", "data": "synthetic",
              "config": {"workload": f"{args.ens_reps} replicate domes, {args.ens_lo}-{args.ens_hi} cells "
-                                    f"log-uniform, gamma 50/500 Pa um 50/50, {args.ens_minutes} simulated min",
+                                    f"log-uniform, gamma 500/50 Pa um per cell 50/50, {args.ens_minutes} simulated min",
                         "replicates": args.ens_reps, "cells_start": total, "cells_end": st["n_final"],
                         "tableau": args.ens_tableau, "jkr": args.ens_jkr,
-                        "phenotype_counts": [int((pheno == 0).sum()), int((pheno == 1).sum())]},
+                        "phenotype_cells": {"group 1 (gamma 500)": int((group_all == 1).sum()),
+                                            "group 2 (gamma 50)": int((group_all == 2).sum())}},
              "ensemble": {"wall_s": st["wall_seconds"], "iterations": st["iterations"],
                           "attempts": st["attempts"], "divisions": st["divisions"],
                           "neighbor_builds": st["neighbor_builds"],

@@ -44,16 +44,32 @@ def ensemble_sizes(n_rep=256, lo=500, hi=20000, seed=0):
     return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n_rep))).astype(np.int64)
 
 
-def make_ensemble(sizes, seeds=None, gammas=(50.0, 500.0), phenotype_seed=1234, **dome_kw):
-    """One dome snapshot per replicate (replicate r: make_dome(sizes[r], seed=seeds[r])),
-    each replicate assigned one of two phenotypes (surface energy gammas[0] or
-    gammas[1], Pa um) by a seeded 50/50 draw. Returns (list of cell arrays,
-    phenotype per replicate)."""
+def assign_phenotypes(cells, gammas=(500.0, 50.0), seed=1234):
+    """Two phenotypes at a seeded 50/50 assignment per cell (half of the cells,
+    rounded down, in group 1): the group column is 1 or 2 and the surface
+    energy density column follows the group, gamma = gammas[group - 1]
+    (the reference initialises gamma per cell from its group,
+    simulation.hpp:1368-1374; a pair adheres with min(gamma_i, gamma_j),
+    mechanics.hpp:1464). Daughters inherit both columns. In place; returns
+    the group per cell."""
+    n = cells.shape[0]
+    group = np.random.default_rng(seed).permutation(np.arange(n) % 2) + 1
+    cells[:, 20] = group
+    cells[:, 21] = np.asarray(gammas, dtype=np.float64)[group - 1]
+    return group
+
+
+def make_ensemble(sizes, seeds=None, gammas=(500.0, 50.0), phenotype_seed=1234, **dome_kw):
+    """One dome snapshot per replicate (replicate r: make_dome(sizes[r], seed=seeds[r]))
+    whose cells carry one of two phenotypes (surface energy gammas[0] for group
+    1, gammas[1] for group 2, Pa um) by a seeded 50/50 draw per cell
+    (assign_phenotypes with seed phenotype_seed + r). Returns (list of cell
+    arrays, list of group vectors)."""
     n_rep = len(sizes)
     seeds = list(range(n_rep)) if seeds is None else list(seeds)
-    pheno = np.random.default_rng(phenotype_seed).permutation(np.arange(n_rep) % 2)
-    reps = []
+    reps, groups = [], []
     for r in range(n_rep):
-        cells, _ = make_dome(int(sizes[r]), seed=seeds[r], gamma=gammas[int(pheno[r])], **dome_kw)
+        cells, _ = make_dome(int(sizes[r]), seed=seeds[r], **dome_kw)
+        groups.append(assign_phenotypes(cells, gammas, phenotype_seed + r))
         reps.append(cells)
-    return reps, pheno
+    return reps, groups

@@ -47,3 +47,36 @@ def test_dome_generator_rules():
     assert np.all(np.linalg.norm(nb[:, 2:5], axis=1) >= 1.0 - 1e-12)   # cores disjoint
     again, _ = bfm.make_dome(n, seed=4)
     assert np.array_equal(cells, again)
+
+
+def test_struct_layout_matches_ctypes():
+    """sizeof / offsetof of bfm_params, bfm_sim_config and bfm_sim_stats as the
+    C++ compiler lays them out (tests/abi_cpp/abi_check --layout) against the
+    ctypes mirrors the Python binding passes across the ABI."""
+    import ctypes as C
+    import json
+    import subprocess
+
+    from bfm.core import Params
+    from bfm.sim import SimConfig, SimStats
+    exe = os.path.join(ROOT, "tests", "abi_cpp", "abi_check")
+    out = json.loads(subprocess.run([exe, "--layout"], capture_output=True, text=True, check=True).stdout)
+    for S, name in ((Params, "bfm_params"), (SimConfig, "bfm_sim_config"), (SimStats, "bfm_sim_stats")):
+        assert out[f"sizeof({name})"] == C.sizeof(S), name
+        for field, _ in S._fields_:
+            assert out[f"{name}.{field}"] == getattr(S, field).offset, (name, field)
+
+
+def test_phenotypes_per_cell():
+    """The batched-ensemble workload: two phenotypes drawn 50/50 per cell in
+    every replicate, group 1 -> gamma 500, group 2 -> gamma 50."""
+    import bfm
+    sizes = bfm.ensemble_sizes(8, 40, 400, seed=0)
+    reps, groups = bfm.make_ensemble(sizes, seeds=range(8))
+    for cells, g in zip(reps, groups):
+        assert set(np.unique(g)) <= {1, 2}
+        assert abs(int((g == 1).sum()) - int((g == 2).sum())) <= 1
+        assert np.array_equal(cells[:, 20], g)
+        assert np.array_equal(cells[:, 21], np.where(g == 1, 500.0, 50.0))
+    again, g2 = bfm.make_ensemble(sizes, seeds=range(8))
+    assert all(np.array_equal(a, b) for a, b in zip(reps, again))

@@ -68,7 +68,7 @@ def test_ensemble_many_replicates_autotuned():
     cap = 2 * int(sizes.sum()) + 64
     print(f"ensemble of 64 replicates, {int(sizes.sum())} cells", flush=True)
     out, t_out, it_out, st = sim.ensemble_simulate(_ctx(cap), cfg, reps, seeds, autotune=True, cap=cap)
-    assert 0 <= st["tuned_variant"] <= 6 and st["tuned_bin_scale"] in (0.5, 0.75, 1.0, 1.25)
+    assert 0 <= st["tuned_variant"] <= 7 and st["tuned_bin_scale"] in (0.5, 0.75, 1.0, 1.25)
     assert st["failed_replicates"] == [47]
     ok = np.ones(64, bool)
     ok[47] = False
@@ -93,3 +93,31 @@ def test_failed_replicate_matches_independent_failure():
     with pytest.raises(BFMError) as e:
         sim.simulate(_ctx(cap), cfg, cells, cap=cap)
     assert e.value.status == 3
+
+
+def test_large_batch_multi_slice_error_maxima():
+    """618k cells in 48 uneven replicates: the RK update's blocks then cover
+    more than one 256-row slice each (chunk = n / 2048 blocks rounded up to
+    256), so the per-lane running error maximum that a wavefront keeps while
+    its slices stay in one replicate, and its flush when the replicate
+    changes, decide every replicate's step-size control. A wrong maximum
+    changes the accepted step or the retries, so two lockstep iterations must
+    match independent runs of replicates that span several slices, bit for
+    bit."""
+    import bfm
+    from bfm import sim
+    sizes = bfm.ensemble_sizes(48, 4000, 30000, seed=3)
+    reps, _ = bfm.make_ensemble(sizes, seeds=range(48), hl_hi=1.2)
+    n = int(sizes.sum())
+    assert n > 2048 * 256   # every block of the update takes two or more slices
+    cfg = sim.sim_config(max_iter=2, tableau="bogacki_shampine", seed=0, iter_update_neighbors=1)
+    cap = n + 4096
+    print(f"multi-slice ensemble: {n} cells", flush=True)
+    out, t_out, it_out, st = sim.ensemble_simulate(_ctx(cap), cfg, reps, list(range(48)), autotune=False, cap=cap)
+    assert np.all(it_out == 2) and st["failed_replicates"] == []
+    for r in (int(np.argmax(sizes)), int(np.argmin(sizes)), 17):
+        cfg.seed = r
+        capr = reps[r].shape[0] + 1024
+        want, _, _, s1 = sim.simulate(_ctx(capr), cfg, reps[r], cap=capr)
+        assert t_out[r] == s1["t_end"], r
+        assert np.array_equal(out[r], want), (r, np.max(np.abs(out[r] - want)))
