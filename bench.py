@@ -21,11 +21,14 @@ sys.path.insert(0, os.path.join(ROOT, "biofilm-abms_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
-# FP64 operations per unit of work of the candidate kernel, counted from
-# bfm_device.hpp / k_candidates (adds, muls, divides and square roots each 1):
-SCREEN_FLOPS = 12      # r_ij^2 (8), bound and its square (4) per screened pair
-TEST_FLOPS = 80        # segment-segment closest points + |d| per queued pair
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64, vector and matrix alike
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector
+# Operations per unit of work of the candidate kernel (adds, muls, divides
+# and square roots each 1): the FP64 segment test of every pair that passed
+# the FP32 pre-screen (the reference's prefilter, closest points, |d|), and
+# the FP32 pre-screen of every screened pair (|r_ij|^2 and the bound).
+TEST_FLOPS = 80        # FP64, per queued pair
+SCREEN_FLOPS = 12      # FP32, per screened pair
 
 
 # HBM traffic per launch (FETCH_SIZE and WRITE_SIZE, copy-calibrated) of the
@@ -63,6 +66,8 @@ def parse():
                     help="JKR radius: exact FP64 solve, 1-D table, learned radius, anisotropic tables "
                          "(adhesion_mode 2, synthetic tables), or no adhesion (Hertz only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-JKR force error and stage time on the snapshot")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU work for the sampled O(N^2) neighbor search")
     ap.add_argument("--bin-scale", type=float, default=0.0)
@@ -88,6 +93,93 @@ def parse():
     return ap.parse_args()
 
 
+def learned_jkr_check(ctx, d_cells, cells, n, thr, R, Ldiv, exact_ms, steps):
+    """The learned JKR radius on the snapshot's own contacts: the stage's
+    list (exported from the device), overlap capped at the shell thickness as
+    the force uses it (adhesion.hpp:195-196), gamma = min(gamma_i, gamma_j);
+    force error |F - F_exact| / (|term1| + |term2|) of the network alone and
+    after the one FP64 Newton step the product applies, against the converged
+    FP64 Newton root (40 steps); contacts outside the training envelope take
+    the exact solve. Also the stage time with the learned radius."""
+    import numpy as np
+    import torch
+
+    import bfm
+    from bfm import surrogate as sur
+
+    m = ctx.stats()["m"]
+    out = torch.zeros((7, max(m, 1)), dtype=torch.float64, device="cuda")
+    ctx.dev_export_neighbors(out.data_ptr(), max(m, 1))
+    ctx.sync()
+    torch.cuda.synchronize()
+    nb = out.cpu().numpy().T[:m]
+    i, j = nb[:, 0].astype(np.int64), nb[:, 1].astype(np.int64)
+    dist = np.sqrt((nb[:, 2] ** 2 + nb[:, 3] ** 2) + nb[:, 4] ** 2)
+    delta = 2 * R - dist
+    contact = delta > 0
+    Rcell, E0 = 0.5, 3900.0
+    Req = R / 2
+    delta_c = np.minimum(delta[contact], 2 * (R - Rcell))
+    gamma = np.minimum(cells[i[contact], 21], cells[j[contact], 21])
+    env = (delta_c >= -0.2 * R) & (delta_c <= 0.3 * R) & (gamma >= 10.0) & (gamma <= 1000.0)
+    p_s = bfm.core.make_params(jkr_mode=bfm.core.JKR_SURROGATE)
+    ctx_s = bfm.Context(n, max_pairs=16 * n, params=p_s)
+    sur.install(ctx_s, sur.load())
+    dd, gg = delta_c[env], gamma[env]
+    _, r0 = sur.evaluate(ctx_s, dd, gg, polish=0)
+    _, r1 = sur.evaluate(ctx_s, dd, gg, polish=1)
+    _, rx = sur.evaluate(ctx_s, dd, gg, polish=40)
+
+    def force(a):
+        a3 = a ** 3
+        t1 = 4 * E0 * a3 / (3 * Req)
+        t2 = 4 * np.sqrt(np.pi * a3 * gg * E0)
+        return t1 - t2, np.abs(t1) + np.abs(t2)
+
+    fx, scale = force(rx)
+    e0 = np.abs(force(r0)[0] - fx) / scale
+    e1 = np.abs(force(r1)[0] - fx) / scale
+    stream = torch.cuda.Stream()
+    ctx_s.set_stream(stream.cuda_stream)
+    d_vel = torch.zeros((6, n), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ctx_s.dev_stage(d_cells.data_ptr(), n, n, thr, R, Ldiv, d_vel.data_ptr())
+    ctx_s.sync()
+    e_a, e_b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e_a.record(stream)
+    for _ in range(steps):
+        ctx_s.dev_stage(d_cells.data_ptr(), n, n, thr, R, Ldiv, d_vel.data_ptr())
+    e_b.record(stream)
+    torch.cuda.synchronize()
+    ms_s = e_a.elapsed_time(e_b) / steps
+    return {"contacts": int(contact.sum()), "in_envelope": int(env.sum()),
+            "force_err_max_polished": float(e1.max()) if e1.size else None,
+            "force_err_rms_polished": float(np.sqrt(np.mean(e1 ** 2))) if e1.size else None,
+            "force_err_max_network_only": float(e0.max()) if e0.size else None,
+            "meets_1e-4": bool(e1.size and e1.max() <= 1e-4),
+            "stage_ms_learned": ms_s, "stage_ms_exact": exact_ms,
+            "learned_vs_exact_speed": exact_ms / ms_s,
+            "reference_radius": "FP64 Newton on the JKR quartic from the network's guess, 40 steps"}
+
+
+def host_cores():
+    """(threads the baseline uses, cores available to this process, host
+    logical CPUs, CPU model): every core the process may run on, unless
+    OMP_NUM_THREADS limits OpenMP."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or avail
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, avail, os.cpu_count(), model
+
+
 def cpu_baseline(cells, thr, R, Ldiv, params, budget, aniso_tables=None):
     """Times the oracle on the same snapshot: C1 = getVelocities on a prebuilt
     list (all N cells, OpenMP at the reference sites), C2 = the serial O(N^2)
@@ -100,7 +192,7 @@ def cpu_baseline(cells, thr, R, Ldiv, params, budget, aniso_tables=None):
         orc.set_aniso_tables(aniso_tables)
 
     n = cells.shape[0]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads, avail, nproc, model = host_cores()
     nb = orc.get_cell_neighbors(cells, thr, R, Ldiv, binned=True, nthreads=threads)
     a2d = (cells[:, 6] < params.nz_threshold).astype(np.int32)
     times = []
@@ -119,6 +211,7 @@ def cpu_baseline(cells, thr, R, Ldiv, params, budget, aniso_tables=None):
     t_rows = time.perf_counter() - t0
     c2 = t_rows * n / rows
     return dict(value=n / (c1 + c2), unit="cell-updates/s", cores=threads, kind="port",
+                host={"cores_available": avail, "nproc": nproc, "cpu_model": model},
                 sample=(f"oracle on the same {n}-cell snapshot: getVelocities on a prebuilt list "
                         f"(all cells, {threads} OpenMP threads, median of 3: {c1*1e3:.1f} ms) + "
                         f"serial O(N^2) getCellNeighbors timed on {rows} rows centred on N/2 "
@@ -220,15 +313,26 @@ def main():
     tests_per_launch = st["segment_tests"] / args.steps
     screened_per_launch = st["screened"] / args.steps
     if dom == "candidates":
-        flops = tests_per_launch * TEST_FLOPS + screened_per_launch * SCREEN_FLOPS
-        achieved = flops / (per_kernel[dom] * 1e-3) / 1e12
+        # FP64 work only: the segment tests of the pairs that passed the FP32
+        # pre-screen; the pre-screen itself is FP32 and reported beside it
+        t_s = per_kernel[dom] * 1e-3
+        flops64 = tests_per_launch * TEST_FLOPS
+        achieved = flops64 / t_s / 1e12
+        screen32 = screened_per_launch * SCREEN_FLOPS / t_s / 1e12
         roof = dict(kernel=dom, bound="mfma", achieved=achieved, peak=FP64_PEAK_TFLOPS,
                     unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
                     traffic=profiled_traffic(dom, n),
-                    peak_note="FP64 dense peak of MI355X (matrix and vector alike); "
-                              "the kernel's FP64 work runs on the vector ALUs",
-                    avg_us=per_kernel[dom] * 1e3, algorithmic_flops=flops,
-                    screened_per_launch=screened_per_launch, tests_per_launch=tests_per_launch)
+                    engine="FP64 vector ALU: MI355X's FP64 vector and matrix peaks are both 78.6 "
+                           "TFLOP/s; the kernel is bound by the latency of its load-screen-test chain, "
+                           "not by either peak",
+                    avg_us=per_kernel[dom] * 1e3, fp64_flops_per_launch=flops64,
+                    tests_per_launch=tests_per_launch,
+                    fp32_screen={"screened_per_launch": screened_per_launch,
+                                 "flops_per_launch": screened_per_launch * SCREEN_FLOPS,
+                                 "achieved_tflops": screen32, "peak_tflops": FP32_PEAK_TFLOPS,
+                                 "frac": screen32 / FP32_PEAK_TFLOPS},
+                    hbm={"algorithmic_bytes": n * 56 + m * 48,
+                         "achieved_gbs": (n * 56 + m * 48) / t_s / 1e9, "peak_gbs": HBM_PEAK_GBS})
     else:
         # HBM-bound per-cell / per-pair kernels: algorithmic bytes 176 per cell + 48 per pair
         bytes_ = n * 176 + m * 48
@@ -238,11 +342,14 @@ def main():
                     avg_us=per_kernel[dom] * 1e3,
                     algorithmic_bytes=bytes_)
     stage_bytes = n * 328 + m * 96
+    learned = None
+    if args.jkr == "exact" and not args.no_learned:
+        learned = learned_jkr_check(ctx, d_cells, cells, n, thr, R, Ldiv, ms, args.steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cells, thr, R, Ldiv, params, args.cpu_budget, aniso)
     out = {
-        "metric": "RK-stage cell-updates/s, 200k-cell 3D FP64",
+        "metric": "RK-stage cell-updates/s, 200k-cell 3D FP64; surrogate vs exact JKR @1e-4 err",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -262,6 +369,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "learned_jkr": learned,
         "stage_hbm_frac": (stage_bytes / (ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
         "jkr_path_us": round(1e3 * (per_kernel.get("pair_forces", 0) + per_kernel.get("sur_radius", 0)), 2),
         "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(per_kernel.items(),
@@ -315,7 +423,7 @@ def growth(args):
                    iterations=it_c, same_iterations=bool(it_c == st_g["iterations"]),
                    max_abs_state_diff=float(np.max(np.abs(cc[:, 1:7] - cg[:, 1:7])))
                    if cc.shape == cg.shape else None, kind="port",
-                   cores=int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1))
+                   cores=host_cores()[0])
     out = {"metric": f"founder-to-{target} growth wall time (3D, FP64, tol 1e-8)",
            "value": st["wall_seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -453,7 +561,7 @@ def ensemble(args):
             t_cpu += time.perf_counter() - t0
             upd += cells.shape[0] * len(b)
             used.append(int(cells.shape[0]))
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        threads = host_cores()[0]
         cpu = dict(value=upd / t_cpu, unit="stage cell-updates/s", cores=threads, kind="port",
                    sample=f"oracle stepRungeKuttaAdaptive ({args.ens_tableau}, exact JKR, prebuilt lists) "
                           f"on 4 median-size replicates {used}, one step each")

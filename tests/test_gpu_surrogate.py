@@ -38,15 +38,18 @@ def test_gradients_match_cpu_oracle(ctx):
     T = rng.uniform(0.6, 1.4, 512).astype(np.float32)
     loss, y, g = sur.grad(ctx, params, X, T)
     loss_r, y_r, g_r = mlp_ref.loss_and_grad(params.astype(np.float64), X.astype(np.float64), T)
-    assert abs(loss - loss_r) <= 1e-2 * loss_r
-    assert np.max(np.abs(y - y_r)) <= 2e-2 * np.max(np.abs(y_r))
+    y_err = np.max(np.abs(y - y_r)) / np.max(np.abs(y_r))
+    print(f"loss rel err {abs(loss - loss_r) / loss_r:.2e}, output rel err {y_err:.2e}", flush=True)
+    # the training kernels run in FP32: agreement to ~1e-6 is measured
+    assert abs(loss - loss_r) <= 1e-4 * loss_r
+    assert y_err <= 1e-4
     off = 0
     for name, shape in mlp_ref.SIZES:
         k = int(np.prod(shape))
         a, b = g[off:off + k], g_r[off:off + k]
         err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
         print(f"{name}: rel grad err {err:.2e}", flush=True)
-        assert err < 3e-2, name
+        assert err < 1e-4, name
         off += k
 
 
@@ -105,6 +108,7 @@ def test_surrogate_accuracy_and_gain(ctx):
           f"1 Newton from bound (no network) {err_nn:.2e}, 100-pt table at fixed gamma "
           f"{err_tab1:.2e}, 100x100 (overlap, gamma) table {err_tab2:.2e}", flush=True)
     assert err1 <= 1e-4
+    assert err0 <= 2e-2              # the BF16 network alone (measured 1.0e-2)
     assert err_nn > 10 * err1        # what the network gains at equal Newton work
     assert err_tab2 > err1           # variable gamma: better than the reference's 2-D table
 
