@@ -73,7 +73,7 @@ def parse():
     ap.add_argument("--bin-scale", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
-    ap.add_argument("--workload", default="stage", choices=["stage", "growth", "ensemble", "layer2d"],
+    ap.add_argument("--workload", default="stage", choices=["stage", "growth", "ensemble", "layer2d", "grow2d"],
                     help="stage: the RK-stage bench (default); growth: the founder-to-N growth run; "
                          "ensemble: batched replicates")
     ap.add_argument("--ens-reps", type=int, default=256)
@@ -86,6 +86,9 @@ def parse():
     ap.add_argument("--growth-cells", type=int, default=100000)
     ap.add_argument("--growth-tableau", default="bogacki_shampine")
     ap.add_argument("--growth-max-wall", type=float, default=900.0)
+    ap.add_argument("--grow2d-cells", type=int, default=2000)
+    ap.add_argument("--grow2d-cpu-cells", type=int, default=2000,
+                    help="the oracle's 2-D loop runs the same founder to this many cells (0: skip)")
     ap.add_argument("--growth-cpu-cells", type=int, default=64,
                     help="target of the oracle growth run timed beside the GPU one (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -229,6 +232,8 @@ def main():
         return growth(args)
     if args.workload == "ensemble":
         return ensemble(args)
+    if args.workload == "grow2d":
+        return grow2d(args)
     if args.workload == "layer2d":
         return layer2d(args)
     import numpy as np
@@ -433,6 +438,70 @@ def growth(args):
                       "iter_update_neighbors": cfg.iter_update_neighbors,
                       "daughter_angle_xy_bound": cfg.daughter_angle_xy_bound,
                       "daughter_angle_z_bound": cfg.daughter_angle_z_bound},
+           "gpu": gpu, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+def grow2d(args):
+    """The 2-D tree's growth run (runSimulationAdaptiveLagrangian of ABM-2d) from
+    the 2-D driver's founder cell to --grow2d-cells cells on the GPU
+    (bfm2d_simulate): R 0.8, L0 0.4 (division at l > 2.4), growth 1.5 +- 0.15
+    per h, daughters N(0.5, 0.125) of l - 2R, E0 3.9 kPa / Ecell 3.9 MPa,
+    isotropic JKR adhesion at 0.1 mJ/m^2 between group-1 cells, drag 0.7 Pa h
+    (surface 720), Bogacki-Shampine at tolerance 1e-8, seed 42. The oracle's
+    loop runs the same founder with the same division stream on one host core
+    (--grow2d-cpu-cells), and the two runs are compared."""
+    import numpy as np
+
+    from bfm.model2d import (Model2D, founder_cell_2d, make_params, sim_config_2d, simulate_2d,
+                             ADHESION2D_JKR_ISOTROPIC)
+
+    target = args.grow2d_cells
+    cap = int(target * 1.1) + 64
+    p = make_params(adhesion_mode=ADHESION2D_JKR_ISOTROPIC, gamma=100.0)
+    amap = np.array([[0, 0], [0, 1]], dtype=np.int32)
+    cfg = sim_config_2d(n_cells=target, seed=args.seed, report_seconds=20.0, max_wall_seconds=args.growth_max_wall)
+    m2 = Model2D(cap, p, max_pairs=16 * cap + 1024)
+    c0 = founder_cell_2d()
+    cells, parents, trace, st = simulate_2d(m2, cfg, c0, amap, cap=cap, trace_cap=5_000_000)
+    n = cells.shape[0]
+    # stage evaluations: every RK attempt evaluates the s = 4 stages for all cells
+    gpu = dict(n_final=n, reached_target=bool(n >= target), wall_s=st["wall_seconds"],
+               iterations=st["iterations"], rk_attempts=st["rk_steps"], divisions=st["divisions"],
+               neighbor_builds=st["neighbor_builds"], t_end_h=st["t_end"],
+               host_division_s=st["host_division_seconds"],
+               us_per_iteration=1e6 * st["wall_seconds"] / max(st["iterations"], 1),
+               dt_percentiles_h={str(q): float(np.percentile(trace, q)) for q in (1, 10, 50, 90, 99)})
+    cpu = None
+    if args.grow2d_cpu_cells > 0 and not args.no_cpu:
+        from oracle import oracle as orc
+        tgt_c = min(args.grow2d_cpu_cells, target)
+        cfg_c = sim_config_2d(n_cells=tgt_c, seed=args.seed)
+        cap_c = int(tgt_c * 1.1) + 64
+        t0 = time.perf_counter()
+        cc, par_c, tr_c, it_c, steps_c = orc.simulate_2d(cfg_c, p, amap, c0, cap=cap_c, trace_cap=5_000_000)
+        t_cpu = time.perf_counter() - t0
+        if tgt_c == target:
+            cg, par_g, tr_g, st_g = cells, parents, trace, st
+        else:
+            m2c = Model2D(cap_c, p, max_pairs=16 * cap_c + 1024)
+            cg, par_g, tr_g, st_g = simulate_2d(m2c, cfg_c, c0, amap, cap=cap_c, trace_cap=5_000_000)
+        same = cc.shape == cg.shape and it_c == st_g["iterations"]
+        cpu = {"value": t_cpu, "unit": "s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's 2-D growth loop from the same founder to {tgt_c} cells, one thread",
+               "iterations": it_c, "rk_attempts": steps_c, "gpu_wall_s_same_run": st_g["wall_seconds"],
+               "same_events": bool(same and np.array_equal(par_c, par_g)),
+               "max_abs_state_diff": float(np.max(np.abs(cc[:, 1:5] - cg[:, 1:5]))) if same else None,
+               "max_rel_dt_diff": float(np.max(np.abs(tr_c - tr_g) / tr_c)) if same else None,
+               "cpu_model": host_cores()[3], "host_logical_cpus": host_cores()[2]}
+    out = {"metric": f"founder-to-{target} 2-D growth wall time (FP64, tol 1e-8)",
+           "value": st["wall_seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"2-D growth from one founder cell to {target} cells (isotropic JKR, "
+                                  f"gamma 100 Pa um, Bogacki-Shampine)",
+                      "seed": args.seed, "tol": 1e-8, "max_stepsize_h": cfg.max_stepsize,
+                      "iter_update_neighbors": cfg.iter_update_neighbors,
+                      "daughter_angle_bound": cfg.daughter_angle_xy_bound},
            "gpu": gpu, "cpu_baseline": cpu}
     print(json.dumps(out))
 

@@ -14,7 +14,7 @@ from .core import BFMError, Context, load_library  # noqa: F401
 COL2D = dict(id=0, rx=1, ry=2, nx=3, ny=4, drx=5, dry=6, dnx=7, dny=8, l=9, half_l=10, t0=11,
              growth=12, eta0=13, eta1=14, maxeta1=15, group=16)
 NCOLS2D = 17
-ADHESION2D_NONE, ADHESION2D_JKR, ADHESION2D_KIHARA = 0, 1, 2
+ADHESION2D_NONE, ADHESION2D_JKR, ADHESION2D_KIHARA, ADHESION2D_JKR_ISOTROPIC = 0, 1, 2, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -25,7 +25,7 @@ class Params2D(C.Structure):
     _fields_ = [("R", C.c_double), ("Rcell", C.c_double), ("prefactors", C.c_double * 3),
                 ("surface_contact_density", C.c_double), ("adhesion_mode", C.c_int),
                 ("ncols", C.c_int), ("strength", C.c_double), ("mindist", C.c_double),
-                ("distance_exp", C.c_double)]
+                ("distance_exp", C.c_double), ("E0", C.c_double), ("gamma", C.c_double)]
 
 
 def cell_cell_prefactors(R, Rcell, E0, Ecell, cap_exponent=2.5):
@@ -40,7 +40,8 @@ def cell_cell_prefactors(R, Rcell, E0, Ecell, cap_exponent=2.5):
 
 
 def make_params(R=0.8, Rcell=0.5, E0=3900.0, Ecell=3.9e6, sigma0=100.0, adhesion_mode=ADHESION2D_NONE,
-                strength=1.0, mindist=1.2, distance_exp=6.0, ncols=NCOLS2D, cap_exponent=2.5):
+                strength=1.0, mindist=1.2, distance_exp=6.0, ncols=NCOLS2D, cap_exponent=2.5, gamma=100.0):
+    """bfm2d_params; gamma (Pa um) and E0 feed ADHESION2D_JKR_ISOTROPIC."""
     p = Params2D()
     p.R, p.Rcell = R, Rcell
     for k, v in enumerate(cell_cell_prefactors(R, Rcell, E0, Ecell, cap_exponent)):
@@ -49,6 +50,7 @@ def make_params(R=0.8, Rcell=0.5, E0=3900.0, Ecell=3.9e6, sigma0=100.0, adhesion
     p.adhesion_mode = adhesion_mode
     p.ncols = ncols
     p.strength, p.mindist, p.distance_exp = strength, mindist, distance_exp
+    p.E0, p.gamma = E0, gamma
     return p
 
 
@@ -76,6 +78,9 @@ def _lib():
             "bfm2d_dev_update_neighbor_distances": [vp, vp, C.c_int, C.c_int],
             "bfm2d_dev_step_rk": [vp, _dp, _dp, _dp, C.c_int, C.c_double, vp, C.c_int, C.c_int, vp, vp, C.c_int,
                                   vp],
+            "bfm2d_simulate": [vp, vp, _ip, C.c_int, _dp, C.c_int, _ip, C.c_int, _dp, C.c_int, _ip, _ip, _ip, _dp,
+                               C.c_int, vp],
+            "bfm2d_divide_cells": [vp, _dp, _ip, C.c_int, C.c_int, C.c_double, C.c_uint64, _ip, _ip, C.c_int, _ip],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -202,3 +207,88 @@ def random_layer(n, seed, spacing=(3.0, 1.4), tilt=0.3, hl_lo=0.4, hl_hi=1.2, et
     c[:, 12], c[:, 13], c[:, 14], c[:, 15], c[:, 16] = 1.5, eta0, eta1, eta1, 1
     c[:, 5:9] = rng.normal(size=(n, 4))
     return c
+
+
+# ---- the growth loop (runSimulationAdaptiveLagrangian of ABM-2d) -------------
+
+def sim_config_2d(n_cells=0, max_iter=0, max_time=0.0, R=0.8, Rcell=0.5, L0=0.4, max_stepsize=1e-4,
+                  min_stepsize=1e-9, max_error_allowed=1e-8, min_error=1e-30, growth_mean=1.5, growth_std=0.15,
+                  daughter_length_std=0.125, daughter_angle_bound=0.05, iter_update_neighbors=10,
+                  iter_update_stepsize=1, max_tries_update_stepsize=3, tableau="bogacki_shampine", seed=42,
+                  report_seconds=0.0, max_wall_seconds=0.0):
+    """bfm_sim_config for bfm2d_simulate, named as the 2-D driver's JSON keys
+    (abmDualAdhesionSwitch.cpp:37-61 of ABM-2d): Ldiv = 2 L0 + 2 R,
+    neighbor_threshold = 2 R + Ldiv, one growth mean/std for both groups,
+    max_tries_update_stepsize 3 and min_error 1e-30 (the driver's constants,
+    :25-29); the tableau defaults to the 2-D tree's Bogacki-Shampine."""
+    from .sim import sim_config
+    c = sim_config(n_cells=n_cells, max_iter=max_iter, max_time=max_time, R=R, Rcell=Rcell, L0=L0,
+                   max_stepsize=max_stepsize, min_stepsize=min_stepsize, max_error_allowed=max_error_allowed,
+                   min_error=min_error, growth_mean=(growth_mean, growth_mean), growth_std=(growth_std, growth_std),
+                   daughter_length_std=daughter_length_std, daughter_angle_xy_bound=daughter_angle_bound,
+                   daughter_angle_z_bound=0.0, iter_update_neighbors=iter_update_neighbors,
+                   iter_update_stepsize=iter_update_stepsize, max_tries_update_stepsize=max_tries_update_stepsize,
+                   tableau=tableau, seed=seed, report_seconds=report_seconds, max_wall_seconds=max_wall_seconds)
+    c.neighbor_threshold = 2 * R + c.Ldiv
+    return c
+
+
+def founder_cell_2d(L0=0.4, growth_mean=1.5, eta_ambient=0.7, eta_surface=720.0, ncols=NCOLS2D):
+    """The 2-D driver's founder (abmDualAdhesionSwitch.cpp:133-135 of ABM-2d):
+    at the origin along x, length L0, mean growth rate, group 1."""
+    c = np.zeros((1, ncols), order="F")
+    c[0, COL2D["nx"]] = 1.0
+    c[0, COL2D["l"]] = L0
+    c[0, COL2D["half_l"]] = L0 / 2
+    c[0, COL2D["growth"]] = growth_mean
+    c[0, COL2D["eta0"]] = eta_ambient
+    c[0, COL2D["eta1"]] = eta_surface
+    c[0, COL2D["maxeta1"]] = eta_surface
+    c[0, COL2D["group"]] = 1
+    return c
+
+
+def simulate_2d(model, cfg, cells, adhesion_map=((0, 0), (0, 1)), cap=None, trace_cap=2_000_000):
+    """bfm2d_simulate on the model's context from `cells` (n x ncols, the
+    2-D columns); adhesion_map: n_groups x n_groups 0/1 (groups are column
+    16; the default lets group-1 cells adhere to each other, as the 2-D
+    drivers do) or None. Returns (cells, parents, dt_trace, stats dict)."""
+    from .sim import SimStats
+    lib = _lib()
+    cells = _f64(cells)
+    n_in, ncols = cells.shape
+    cap = int(cap or model.ctx.max_cells)
+    amap = None if adhesion_map is None else np.ascontiguousarray(adhesion_map, dtype=np.int32)
+    out = np.zeros((cap, ncols), order="F")
+    parents = np.zeros(int(cells[:, 0].max()) + 1 + 2 * cap + 8, dtype=np.int32)
+    n_out, n_par = C.c_int(), C.c_int()
+    trace = np.zeros(trace_cap)
+    st = SimStats()
+    model.ctx._check(lib.bfm2d_simulate(
+        model.ctx.h, C.cast(C.byref(cfg), C.c_void_p), None if amap is None else amap.ctypes.data_as(_ip),
+        0 if amap is None else amap.shape[0], _ptr(cells), n_in, None, 0, _ptr(out), cap, C.byref(n_out),
+        parents.ctypes.data_as(_ip), C.byref(n_par), _ptr(trace), trace_cap, C.cast(C.byref(st), C.c_void_p)))
+    it = min(st.iterations, trace_cap)
+    stats = {k: getattr(st, k) for k, _ in st._fields_}
+    return np.asfortranarray(out[:n_out.value]), parents[:n_par.value].copy(), trace[:it].copy(), stats
+
+
+def divide_cells_2d(cfg, cells, t, seed, parents=None):
+    """bfm2d_divide_cells (host only) on a copy; returns (cells, parents, tries)."""
+    lib = _lib()
+    cells = np.asarray(cells, dtype=np.float64)
+    n, ncols = cells.shape
+    cap = 2 * n + 1
+    buf = np.zeros((cap, ncols), order="F")
+    buf[:n] = cells
+    if parents is None:
+        parents = -np.ones(int(cells[:, 0].max()) + 1, dtype=np.int32)
+    pcap = len(parents) + 2 * n + 2
+    par = np.zeros(pcap, dtype=np.int32)
+    par[:len(parents)] = parents
+    n_io, n_par, tries = C.c_int(n), C.c_int(len(parents)), C.c_int()
+    rc = lib.bfm2d_divide_cells(C.cast(C.byref(cfg), C.c_void_p), _ptr(buf), C.byref(n_io), cap, ncols, t, seed,
+                                par.ctypes.data_as(_ip), C.byref(n_par), pcap, C.byref(tries))
+    if rc != 0:
+        raise BFMError(rc, "bfm2d_divide_cells failed")
+    return np.asfortranarray(buf[:n_io.value]), par[:n_par.value].copy(), tries.value

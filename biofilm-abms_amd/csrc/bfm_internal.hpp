@@ -50,6 +50,16 @@ int step_verlet(bfm_ctx* c, double dt, double rho, const double* d_cells, int n,
 // Ordered-key <-> double for the scaled-error maxima.
 double order_key_value(unsigned long long k);
 
+// The 2-D step for the 2-D growth loop, on the internal list
+// (bfm2d_dev_build_neighbors) and the context's bfm2d_params: the pair
+// records of the list's frozen rows and to_adhere flags (once per iteration),
+// then each RK attempt: the stages and the update into d_new; tol > 0 also
+// writes the controller's scaled-error maximum to d_errmax (ordered key).
+// Non-finite stage velocities set ST_NAN in the context's status word.
+int step2d_prepare(bfm_ctx* c, int n);
+int step2d_attempt(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld, double* d_new,
+                   int ld_new, double tol, unsigned long long* d_errmax);
+
 // Frame output of a one-colony run: after every iteration in which the
 // simulated time passes a multiple of dt_write (simulation.hpp:2023-2063),
 // fn receives the cells on the host (n rows, leading dimension ld), the

@@ -3113,7 +3113,7 @@ int guarded(bfm_ctx* c, F&& f)
 enum Col2D { D_RX = 1, D_RY = 2, D_NX = 3, D_NY = 4, D_DRX = 5, D_L = 9, D_HALF_L = 10, D_ETA0 = 13, D_ETA1 = 14 };
 
 struct Params2D {
-    double R, Rcell, p0, p1, p2, scd, strength, mindist, dexp;
+    double R, Rcell, p0, p1, p2, scd, strength, mindist, dexp, E0, gamma;
     int mode;
 };
 
@@ -3150,6 +3150,13 @@ k2d_pair(const int* __restrict__ d_m, int n, const int* __restrict__ pi, const i
             if (P.mode == BFM2D_ADHESION_JKR) {
                 a = kPi * P.R;
                 a *= (mag <= P.mindist) ? (2 * P.R - P.mindist) : (2 * P.R - mag);
+            } else if (P.mode == BFM2D_ADHESION_JKR_ISOTROPIC) {
+                // forcesIsotropicJKRLagrange<T, 2> (adhesion.hpp:171-233 of
+                // ABM-3d) at the overlap capped at 2 (R - Rcell): a = term2 - term1
+                const double delta = overlap > cap ? cap : overlap;
+                const double Req = P.R / 2;
+                const double radius = jkr_radius_newton(delta, Req, P.E0, P.gamma, 0.0, 200, nullptr);
+                a = -jkr_magnitude(radius, Req, P.E0, P.gamma);
             } else {
                 const double t1 = (mag <= P.mindist) ? 1.0 / pow(P.mindist, P.dexp + 1) : 1.0 / pow(mag, P.dexp + 1);
                 const double t2 = 1.0 / pow(2 * P.R, P.dexp + 1);
@@ -3256,11 +3263,16 @@ k2d_stage(int n, const double* __restrict__ cells, int ld, const double* __restr
     }
 }
 
-// RK update of the 2-D step (mechanics.hpp:1794-1816 of ABM-2d).
+// RK update of the 2-D step (mechanics.hpp:1794-1816 of ABM-2d); errors may
+// be NULL; tol > 0 also reduces the controller's scaled error
+// max e / (tol (1 + |y|)) over the four coordinates (simulation.hpp:639-650
+// of ABM-2d) into errmax (ordered key).
 __global__ void __launch_bounds__(256)
 k2d_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau tb, double dt,
-          const double* __restrict__ kbuf, double* __restrict__ out, int ld_out, double* __restrict__ errors)
+          const double* __restrict__ kbuf, double* __restrict__ out, int ld_out, double* __restrict__ errors,
+          double tol, unsigned long long* __restrict__ errmax)
 {
+    double local = -INFINITY;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         for (int col = 0; col < ncols; ++col)
             if (col < D_RX || col > D_DRX + 3) out[(int64_t)col * ld_out + c] = cellv(cells, ld, c, col);
@@ -3274,8 +3286,10 @@ k2d_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau tb
                 v2 += kv * tb.bs[st];
             }
             const double d1 = v1 * dt, d2 = v2 * dt;
-            y[q] = cellv(cells, ld, c, D_RX + q) + d1;
-            errors[(int64_t)q * n + c] = d1 - d2;
+            const double y0 = cellv(cells, ld, c, D_RX + q);
+            y[q] = y0 + d1;
+            if (errors) errors[(int64_t)q * n + c] = d1 - d2;
+            if (tol > 0) local = fmax(local, (d1 - d2) / (tol * (1.0 + fabs(y0))));
             out[(int64_t)(D_DRX + q) * ld_out + c] = v1;
         }
         const double nrm = sqrt(y[2] * y[2] + y[3] * y[3]);
@@ -3283,6 +3297,10 @@ k2d_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau tb
         y[3] /= nrm;
 #pragma unroll
         for (int q = 0; q < 4; ++q) out[(int64_t)(D_RX + q) * ld_out + c] = y[q];
+    }
+    if (tol > 0) {
+        for (int off = 32; off > 0; off >>= 1) local = fmax(local, __shfl_xor(local, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(errmax, order_key(local));
     }
 }
 
@@ -3335,6 +3353,7 @@ Params2D params_2d(const bfm_ctx* c)
     P.p0 = q.prefactors[0]; P.p1 = q.prefactors[1]; P.p2 = q.prefactors[2];
     P.scd = q.surface_contact_density;
     P.strength = q.strength; P.mindist = q.mindist; P.dexp = q.distance_exp;
+    P.E0 = q.E0; P.gamma = q.gamma;
     P.mode = q.adhesion_mode;
     return P;
 }
@@ -3479,6 +3498,29 @@ int set_replicate_count(bfm_ctx* c, int n_rep)
 }
 
 double order_key_value(unsigned long long k) { return order_val_host(k); }
+
+int step2d_prepare(bfm_ctx* c, int n)
+{
+    return guarded(c, [&] {
+        require(c->list_n == n && c->list_csr, BFM_ERR_ARG,
+                "no device neighbor list for these cells (bfm2d_dev_build_neighbors)");
+        const Params2D P = params_2d(c);
+        KLAUNCH(c, "pair_2d", k2d_pair, kPairGrid, 256, 0, c->d_m, n, c->pi, c->pj, c->pdx, c->pdy, c->ps, c->pt,
+                c->d_adhere, P, c->d_rec, (int*)nullptr);
+    });
+}
+
+int step2d_attempt(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld, double* d_new,
+                   int ld_new, double tol, unsigned long long* d_errmax)
+{
+    return guarded(c, [&] {
+        require(n >= 1 && n <= c->max_cells && ld >= n && ld_new >= n, BFM_ERR_ARG, "invalid n, ld or ld_new");
+        if (tol > 0) KLAUNCH(c, "err_init", k_err_init, 1, 1, 0, d_errmax);
+        stages_2d(c, tb, dt, d_cells, ld, n, nullptr);
+        KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, d_cells, ld, tb, dt, c->d_k,
+                d_new, ld_new, (double*)nullptr, tol, d_errmax);
+    });
+}
 
 }  // namespace bfm
 
@@ -4088,9 +4130,11 @@ int bfm2d_set_params(bfm_ctx* c, const bfm2d_params* p)
         require(p != nullptr, BFM_ERR_ARG, "params is NULL");
         require(p->R > 0 && p->Rcell > 0 && p->Rcell <= p->R, BFM_ERR_ARG, "invalid radii");
         require(p->ncols >= 17 && p->ncols <= kMaxCols, BFM_ERR_ARG, "ncols must be in 17..32");
-        require(p->adhesion_mode >= BFM2D_ADHESION_NONE && p->adhesion_mode <= 3, BFM_ERR_ARG,
-                "invalid adhesion_mode");
+        require(p->adhesion_mode >= BFM2D_ADHESION_NONE && p->adhesion_mode <= BFM2D_ADHESION_JKR_ISOTROPIC,
+                BFM_ERR_ARG, "invalid adhesion_mode");
         require(p->adhesion_mode != 3, BFM_ERR_UNSUPPORTED, "GBK adhesion is not supported");
+        require(p->adhesion_mode != BFM2D_ADHESION_JKR_ISOTROPIC || (p->E0 > 0 && p->gamma >= 0), BFM_ERR_ARG,
+                "isotropic JKR needs E0 > 0 and gamma >= 0");
         if (p->adhesion_mode != BFM2D_ADHESION_NONE && !c->d_adhere) {
             HIPCHK(hipStreamSynchronize(c->stream));
             dalloc(&c->d_adhere, (size_t)c->max_pairs);
@@ -4162,7 +4206,7 @@ int bfm2d_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs
         prepare_2d(c, cells, n, ld, nb, m, ld_nb, to_adhere, noise);
         stages_2d(c, tb, dt, c->d_cells, c->max_cells, n, noise ? c->d_noise : nullptr);
         KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, c->d_cells, c->max_cells, tb,
-                dt, c->d_k, c->d_cells_new, c->max_cells, c->d_err6);
+                dt, c->d_k, c->d_cells_new, c->max_cells, c->d_err6, 0.0, (unsigned long long*)nullptr);
         if (n > 0) {
             HIPCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * ld_out, c->d_cells_new,
                                     sizeof(double) * c->max_cells, sizeof(double) * n, c->p2d.ncols,
@@ -4233,7 +4277,7 @@ int bfm2d_dev_step_rk(bfm_ctx* c, const double* A, const double* b, const double
                 c->d_adhere, P, c->d_rec, (int*)nullptr);
         stages_2d(c, tb, dt, d_cells, ld, n, d_noise);
         KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, d_cells, ld, tb, dt, c->d_k,
-                d_cells_out, ld_out, d_errors ? d_errors : c->d_err6);
+                d_cells_out, ld_out, d_errors ? d_errors : c->d_err6, 0.0, (unsigned long long*)nullptr);
     });
 }
 

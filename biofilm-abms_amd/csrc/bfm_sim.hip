@@ -1139,6 +1139,341 @@ void simulate(bfm_ctx* c, const bfm_sim_config* cfg, const double* cells_in, int
     if (failed) sim_fail(BFM_ERR_NAN, "non-finite positions or orientations (the run stopped at the last finite state)");
 }
 
+// ---------------------------------------------------------------------------
+// the 2-D tree's growth loop (runSimulationAdaptiveLagrangian of ABM-2d)
+// ---------------------------------------------------------------------------
+// 2-D cell columns (include/indices.hpp of ABM-2d).
+enum Col2 { E2_ID = 0, E2_RX = 1, E2_RY = 2, E2_NX = 3, E2_NY = 4, E2_L = 9, E2_HALF_L = 10, E2_T0 = 11,
+            E2_GROWTH = 12, E2_GROUP = 16 };
+
+// rotate (utils.hpp:373-380 of ABM-2d): the 2x2 rotation by theta.
+void rotate_2d(const double n[2], double theta, double out[2])
+{
+    const double s = sin(theta), c = cos(theta);
+    out[0] = c * n[0] + (-s) * n[1];
+    out[1] = s * n[0] + c * n[1];
+}
+
+Seg seg_2d(const double* cells, int ld, int i)
+{
+    Seg g;
+    g.r[0] = cells[(size_t)E2_RX * ld + i]; g.r[1] = cells[(size_t)E2_RY * ld + i]; g.r[2] = 0.0;
+    g.n[0] = cells[(size_t)E2_NX * ld + i]; g.n[1] = cells[(size_t)E2_NY * ld + i]; g.n[2] = 0.0;
+    g.hl = cells[(size_t)E2_HALF_L * ld + i];
+    return g;
+}
+
+// One division batch of the 2-D tree (divideMaxLength + divideCells,
+// growth.hpp:67-71, 438-697 of ABM-2d) on host cells (column-major, leading
+// dimension ld, capacity cap): the no-room test against the population, then
+// placement tries (two re-orientation draws per cell with room except on the
+// last try, the length ratios, the mothers' then the daughters' growth
+// rates) until every cell with room has both daughters at least 2 Rcell from
+// all other cells, or ten tries; ids and lineage last. minDistToCell (:82-117)
+// is only compared with 2 Rcell, so pairs whose bounding distance exceeds
+// that are skipped. Returns the number of tries (0: nothing divided).
+int divide_2d(const bfm_sim_config* cfg, int ncols, Mt19937& rng, double t, double* cells, int ld, int& n, int cap,
+              std::vector<int>& parents)
+{
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i)
+        if (cells[(size_t)E2_L * ld + i] > cfg->Ldiv) idx.push_back(i);
+    const int nd = (int)idx.size();
+    if (nd == 0) return 0;
+    if (n + nd > cap) sim_fail(BFM_ERR_CAPACITY, "a division would exceed the capacity");
+    const Draws dr{cfg};
+    const double R = cfg->R, cut = 2 * cfg->Rcell;
+    std::vector<char> dividing(n, 0);
+    for (int i : idx) dividing[i] = 1;
+    // minimum over the cells that do not divide (their rows stay as they are)
+    auto min_to_rest = [&](const Seg& a, int self) {
+        double best = INFINITY;
+        for (int j = 0; j < n; ++j) {
+            if (j == self || (self < 0 && dividing[j])) continue;
+            const Seg b = seg_2d(cells, ld, j);
+            if (beyond_cut(a, b, cut)) continue;
+            best = std::min(best, seg_dist(a, b));
+        }
+        return best;
+    };
+    std::vector<int> check(nd, 1), satisfies(nd, 0);
+    for (int k = 0; k < nd; ++k)
+        if (min_to_rest(seg_2d(cells, ld, idx[k]), idx[k]) < cut) check[k] = 0, satisfies[k] = 1;
+    // the dividing rows as they were, and this try's mothers and daughters
+    std::vector<double> orig((size_t)nd * ncols), mo, da;
+    for (int k = 0; k < nd; ++k)
+        for (int col = 0; col < ncols; ++col) orig[(size_t)k * ncols + col] = cells[(size_t)col * ld + idx[k]];
+    auto sat_all = [&] {
+        int v = 0;
+        for (int x : satisfies) v += x;
+        return v == nd;
+    };
+    int ntries = 0;
+    while (ntries == 0 || (ntries < 10 && !sat_all())) {
+        mo = orig;
+        da = orig;
+        std::vector<double> th1(nd, 0.0), th2(nd, 0.0);
+        for (int k = 0; k < nd; ++k)
+            if (ntries < 10 - 1 && check[k] == 1) {
+                th1[k] = dr.angle_xy(rng);
+                th2[k] = dr.angle_xy(rng);
+            }
+        for (int k = 0; k < nd; ++k) {
+            double* m = &mo[(size_t)k * ncols];
+            double* d = &da[(size_t)k * ncols];
+            const double u[2] = {m[E2_NX], m[E2_NY]};
+            double v[2];
+            rotate_2d(u, th1[k], v);
+            m[E2_NX] = v[0]; m[E2_NY] = v[1];
+            rotate_2d(u, th2[k], v);
+            d[E2_NX] = v[0]; d[E2_NY] = v[1];
+        }
+        std::vector<double> M(nd);
+        for (int k = 0; k < nd; ++k) M[k] = dr.daughter_length(rng);
+        for (int k = 0; k < nd; ++k) {
+            double* m = &mo[(size_t)k * ncols];
+            double* d = &da[(size_t)k * ncols];
+            const double Ld = m[E2_L] - 2 * R;
+            const double L1 = M[k] * Ld, L2 = (1 - M[k]) * Ld;
+            const double div = -m[E2_HALF_L] + L1 + R;
+            const double delta1 = L1 / 2 + R, delta2 = L2 / 2 + R;
+            m[E2_L] = L1; m[E2_HALF_L] = L1 / 2;
+            d[E2_L] = L2; d[E2_HALF_L] = L2 / 2;
+            m[E2_RX] = m[E2_RX] + (div - delta1) * m[E2_NX];
+            d[E2_RX] = d[E2_RX] + (div + delta2) * d[E2_NX];
+            m[E2_RY] = m[E2_RY] + (div - delta1) * m[E2_NY];
+            d[E2_RY] = d[E2_RY] + (div + delta2) * d[E2_NY];
+            m[E2_T0] = t;
+            d[E2_T0] = t;
+        }
+        for (int k = 0; k < nd; ++k) mo[(size_t)k * ncols + E2_GROWTH] = dr.growth(rng, (int)mo[(size_t)k * ncols + E2_GROUP]);
+        for (int k = 0; k < nd; ++k) da[(size_t)k * ncols + E2_GROWTH] = dr.growth(rng, (int)da[(size_t)k * ncols + E2_GROUP]);
+        auto seg_row = [&](const double* row) {
+            Seg g;
+            g.r[0] = row[E2_RX]; g.r[1] = row[E2_RY]; g.r[2] = 0.0;
+            g.n[0] = row[E2_NX]; g.n[1] = row[E2_NY]; g.n[2] = 0.0;
+            g.hl = row[E2_HALF_L];
+            return g;
+        };
+        // minimum over the whole new population: the unchanged cells, then the
+        // 2 nd rewritten rows other than the query itself
+        auto min_new = [&](int self_row) {
+            const Seg a = seg_row(self_row < nd ? &mo[(size_t)self_row * ncols] : &da[(size_t)(self_row - nd) * ncols]);
+            double best = min_to_rest(a, -1);
+            for (int r = 0; r < 2 * nd; ++r) {
+                if (r == self_row) continue;
+                const Seg b = seg_row(r < nd ? &mo[(size_t)r * ncols] : &da[(size_t)(r - nd) * ncols]);
+                if (beyond_cut(a, b, cut)) continue;
+                best = std::min(best, seg_dist(a, b));
+            }
+            return best;
+        };
+        for (int k = 0; k < nd; ++k)
+            if (check[k] == 1 && min_new(k) >= cut && min_new(nd + k) >= cut) satisfies[k] = 1;
+        ntries++;
+    }
+    const int max_id = (int)parents.size() - 1;
+    for (int k = 0; k < nd; ++k) {
+        const int parent_id = (int)mo[(size_t)k * ncols + E2_ID];
+        mo[(size_t)k * ncols + E2_ID] = max_id + (2 * k + 1);
+        da[(size_t)k * ncols + E2_ID] = max_id + (2 * k + 2);
+        parents.push_back(parent_id);
+        parents.push_back(parent_id);
+    }
+    for (int k = 0; k < nd; ++k)
+        for (int col = 0; col < ncols; ++col) {
+            cells[(size_t)col * ld + idx[k]] = mo[(size_t)k * ncols + col];
+            cells[(size_t)col * ld + n + k] = da[(size_t)k * ncols + col];
+        }
+    n += nd;
+    return ntries;
+}
+
+// growCells of the 2-D tree (growth.hpp:51-57 of ABM-2d) when grow != 0,
+// then divideMaxLength's count (l > Ldiv) into *count.
+__global__ void __launch_bounds__(256)
+k2d_grow_count(double* __restrict__ cells, int ld, int n, double R, double dt, double Ldiv, int grow,
+               int* __restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int div = 0;
+    if (i < n) {
+        double l = cells[(size_t)E2_L * ld + i];
+        if (grow) {
+            const double g = cells[(size_t)E2_GROWTH * ld + i];
+            l = l + (g * (4 * R / 3 + l)) * dt;
+            cells[(size_t)E2_L * ld + i] = l;
+            cells[(size_t)E2_HALF_L * ld + i] = l / 2;
+        }
+        div = l > Ldiv;
+    }
+    const unsigned long long ballot = __ballot(div);
+    if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(count, (int)__popcll(ballot));
+}
+
+// runSimulationAdaptiveLagrangian's loop of the 2-D tree (simulation.hpp:
+// 549-765 of ABM-2d) with the cells on the device (leading dimension cap).
+// Per iteration the host reads the division count and the first attempt's
+// scaled error; a division copies the population to the host, divides it
+// there with the reference's draws and copies it back.
+void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, const double* cells_in, int n_in,
+                const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
+                int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats)
+{
+    if (!c->p2d_set) sim_fail(BFM_ERR_ARG, "bfm2d_set_params has not been called");
+    if (!cfg) sim_fail(BFM_ERR_ARG, "NULL config");
+    const int ncols = c->p2d.ncols;
+    if (!cells_in || !cells_out || !n_out || n_in < 1 || cap < n_in || cap > c->max_cells)
+        sim_fail(BFM_ERR_ARG, "need 1 <= n_in <= cap <= max_cells and non-NULL arrays");
+    if (cfg->iter_update_neighbors < 1 || cfg->iter_update_stepsize < 1 || cfg->max_tries_update_stepsize < 0)
+        sim_fail(BFM_ERR_ARG, "iter_update_* must be >= 1 and max_tries_update_stepsize >= 0");
+    if (!(cfg->max_stepsize > 0) || !(cfg->min_stepsize > 0) || cfg->min_stepsize > cfg->max_stepsize)
+        sim_fail(BFM_ERR_ARG, "need 0 < min_stepsize <= max_stepsize");
+    if (!(cfg->max_error_allowed > 0)) sim_fail(BFM_ERR_ARG, "max_error_allowed must be > 0");
+    if (ng < 0 || ng > 64) sim_fail(BFM_ERR_ARG, "n_groups must be 0..64");
+    const TableauDef tdef = tableau(cfg->tableau);
+    Tableau tb;
+    std::memset(&tb, 0, sizeof(tb));
+    tb.s = tdef.s;
+    for (int i = 0; i < 49; ++i) tb.A[i] = tdef.A[i];
+    for (int i = 0; i < 7; ++i) { tb.b[i] = tdef.b[i]; tb.bs[i] = tdef.bs[i]; }
+    const auto wall0 = std::chrono::steady_clock::now();
+    auto wall = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count(); };
+    hipStream_t st = c->stream;
+    DevBuf<double> bufA((size_t)cap * ncols), bufB((size_t)cap * ncols);
+    DevBuf<unsigned long long> d_err(1);
+    DevBuf<int> d_count(1);
+    PinnedBuf pin(64), host(sizeof(double) * (size_t)cap * ncols);
+    unsigned long long* h_err = (unsigned long long*)pin.p;
+    int* h_count = (int*)((char*)pin.p + 16);
+    int* h_status = (int*)((char*)pin.p + 32);
+    double* hc = (double*)host.p;
+    SCHK(hipMemcpy2DAsync(bufA.p, sizeof(double) * cap, cells_in, sizeof(double) * n_in, sizeof(double) * n_in,
+                          ncols, hipMemcpyHostToDevice, st));
+    std::vector<int> parents;
+    if (parents_in) {
+        parents.assign(parents_in, parents_in + n_parents_in);
+    } else {
+        int max_id = 0;
+        for (int i = 0; i < n_in; ++i) max_id = std::max(max_id, (int)cells_in[i]);
+        parents.assign(max_id + 1, -1);
+    }
+    Mt19937 rng((uint32_t)cfg->seed);
+    double* cur = bufA.p;
+    double* nxt = bufB.p;
+    int n = n_in;
+    int64_t iter = 0, attempts = 0, divisions = 0, builds = 0;
+    double t = 0, dt = cfg->max_stepsize, host_div_s = 0, next_report = cfg->report_seconds;
+    const double thr = cfg->neighbor_threshold, R = cfg->R, Ldiv = cfg->Ldiv, tol = cfg->max_error_allowed;
+    auto build = [&] {
+        int64_t m = 0;
+        abi(c, bfm2d_dev_build_neighbors(c, cur, n, cap, thr, R, Ldiv, amap, ng, &m));
+        ++builds;
+    };
+    auto count_dividing = [&](int grow, double h) {
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
+        hipLaunchKernelGGL(k2d_grow_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, cur, cap, n, R, h, Ldiv,
+                           grow, d_count.p);
+        SCHK(hipGetLastError());
+        SCHK(hipMemcpyAsync(h_count, d_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    };
+    auto status_check = [&] {
+        if (*h_status) {
+            const int zero = 0;
+            SCHK(hipMemcpy(c->d_status, &zero, sizeof(int), hipMemcpyHostToDevice));
+            const int s0 = *h_status;
+            *h_status = 0;
+            if (s0 & ST_NAN) sim_fail(BFM_ERR_NAN, "non-finite stage velocity in the 2-D growth loop");
+            sim_fail(BFM_ERR_HIP, "device status error in the 2-D growth loop");
+        }
+    };
+    // one RK attempt; with want_error the scaled-error maximum is read back
+    auto attempt = [&](double h, bool want_error) {
+        ++attempts;
+        abi(c, step2d_attempt(c, tb, h, cur, n, cap, nxt, cap, want_error ? tol : 0.0, d_err.p));
+        if (!want_error) return 0.0;
+        SCHK(hipMemcpyAsync(h_err, d_err.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        SCHK(hipMemcpyAsync(h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        status_check();
+        return std::max(order_key_value(*h_err), cfg->min_error);
+    };
+    *h_status = 0;
+    build();
+    count_dividing(0, 0.0);
+    for (;;) {
+        SCHK(hipStreamSynchronize(st));
+        if (terminate_now(cfg, n, iter, t)) break;
+        if (cfg->max_wall_seconds > 0 && wall() > cfg->max_wall_seconds) break;
+        if (cfg->report_seconds > 0 && wall() >= next_report) {
+            std::fprintf(stderr, "[bfm2d] iter %lld  n %d  t %.6g h  dt %.3g h  wall %.1f s\n", (long long)iter, n, t,
+                         dt, wall());
+            next_report += cfg->report_seconds;
+        }
+        const int nd = *h_count;
+        if (nd > 0) {
+            const auto h0 = std::chrono::steady_clock::now();
+            SCHK(hipMemcpy2DAsync(hc, sizeof(double) * cap, cur, sizeof(double) * cap, sizeof(double) * n, ncols,
+                                  hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+            const int n0 = n;
+            divide_2d(cfg, ncols, rng, t, hc, cap, n, cap, parents);
+            divisions += n - n0;
+            SCHK(hipMemcpy2DAsync(cur, sizeof(double) * cap, hc, sizeof(double) * cap, sizeof(double) * n, ncols,
+                                  hipMemcpyHostToDevice, st));
+            SCHK(hipStreamSynchronize(st));
+            host_div_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
+            build();
+        }
+        abi(c, step2d_prepare(c, n));
+        double max_error = attempt(dt, true);
+        if (iter % cfg->iter_update_stepsize == 0) {   // simulation.hpp:639-712 of ABM-2d
+            const double inv = 1.0 / (tdef.order + 1);
+            bool exceeded = max_error > 1.0;
+            double factor = 0.9 * pow(1.0 / max_error, inv);
+            if (factor >= 10) factor = 10;
+            else if (factor < 0.2) factor = 0.2;
+            int j = 0;
+            while (exceeded && j < cfg->max_tries_update_stepsize) {
+                max_error = attempt(dt * factor, true);
+                exceeded = max_error > 1.0;
+                factor *= 0.9 * pow(1.0 / max_error, inv);
+                if (factor >= 10) { factor = 10; break; }
+                else if (factor < 0.2) { factor = 0.2; break; }
+                j++;
+            }
+            if (dt * factor < cfg->min_stepsize) factor = cfg->min_stepsize / dt;
+            else if (dt * factor > cfg->max_stepsize) factor = cfg->max_stepsize / dt;
+            dt *= factor;
+            attempt(dt, false);
+        }
+        std::swap(cur, nxt);
+        count_dividing(1, dt);
+        abi(c, bfm2d_dev_update_neighbor_distances(c, cur, n, cap));
+        if (dt_trace && iter < cap_trace) dt_trace[iter] = dt;
+        t += dt;
+        iter++;
+        if (iter % cfg->iter_update_neighbors == 0) build();
+    }
+    SCHK(hipMemcpyAsync(h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, st));
+    SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, cur, sizeof(double) * cap, sizeof(double) * n, ncols,
+                          hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    status_check();
+    *n_out = n;
+    if (parents_out) std::copy(parents.begin(), parents.end(), parents_out);
+    if (n_parents_out) *n_parents_out = (int)parents.size();
+    if (stats) {
+        stats->iterations = iter;
+        stats->rk_steps = attempts;
+        stats->divisions = divisions;
+        stats->neighbor_builds = builds;
+        stats->t_end = t;
+        stats->wall_seconds = wall();
+        stats->host_division_seconds = host_div_s;
+    }
+}
+
 }  // namespace
 
 namespace bfm {
@@ -1282,6 +1617,49 @@ int bfm_rng_uniform(uint64_t seed, int count, double* out)
     Mt19937 g((uint32_t)seed);
     for (int i = 0; i < count; ++i) out[i] = uniform01(g);
     return BFM_OK;
+}
+
+int bfm2d_simulate(bfm_ctx* c, const bfm_sim_config* cfg, const int* adhesion_map, int n_groups,
+                   const double* cells_in, int n_in, const int* parents_in, int n_parents_in, double* cells_out,
+                   int cap, int* n_out, int* parents_out, int* n_parents_out, double* dt_trace, int cap_trace,
+                   bfm_sim_stats* stats)
+{
+    if (!c) return BFM_ERR_ARG;
+    try {
+        SCHK(hipSetDevice(c->device));
+        simulate2d(c, cfg, adhesion_map, n_groups, cells_in, n_in, parents_in, n_parents_in, cells_out, cap, n_out,
+                   parents_out, n_parents_out, dt_trace, cap_trace, stats);
+        return BFM_OK;
+    } catch (const SimError& e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return BFM_ERR_ARG;
+    }
+}
+
+int bfm2d_divide_cells(const bfm_sim_config* cfg, double* cells, int* n_io, int cap, int ncols, double t,
+                       uint64_t seed, int* parents, int* n_parents, int parents_cap, int* tries_out)
+{
+    try {
+        if (!cfg || !cells || !n_io || !parents || !n_parents || ncols < 17 || *n_io < 0 || *n_io > cap)
+            return BFM_ERR_ARG;
+        int nd = 0;
+        for (int i = 0; i < *n_io; ++i) nd += cells[(size_t)E2_L * cap + i] > cfg->Ldiv;
+        if (*n_io + nd > cap || *n_parents + 2 * nd > parents_cap) return BFM_ERR_CAPACITY;
+        Mt19937 rng((uint32_t)seed);
+        std::vector<int> par(parents, parents + *n_parents);
+        const int tries = divide_2d(cfg, ncols, rng, t, cells, cap, *n_io, cap, par);
+        std::copy(par.begin(), par.end(), parents);
+        *n_parents = (int)par.size();
+        if (tries_out) *tries_out = tries;
+        return BFM_OK;
+    } catch (const SimError& e) {
+        return e.code;
+    } catch (const std::exception&) {
+        return BFM_ERR_ARG;
+    }
 }
 
 }  // extern "C"

@@ -421,8 +421,15 @@ int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads, int kernel
 enum bfm2d_adhesion_mode {
     BFM2D_ADHESION_NONE = 0,
     BFM2D_ADHESION_JKR = 1,      /* forcesJKRLagrange<T, 2> (adhesion.hpp:402-462 of ABM-2d) */
-    BFM2D_ADHESION_KIHARA = 2    /* forcesKiharaLagrange<T, 2> (adhesion.hpp:334-400 of ABM-2d) */
+    BFM2D_ADHESION_KIHARA = 2,   /* forcesKiharaLagrange<T, 2> (adhesion.hpp:334-400 of ABM-2d) */
     /* GBK (3) is not built */
+    BFM2D_ADHESION_JKR_ISOTROPIC = 4   /* the 3-D tree's dimension-templated
+        forcesIsotropicJKRLagrange<T, 2> (adhesion.hpp:171-233 of ABM-3d) in the
+        2-D adhesive loop: contact radius of the JKR quartic (exact FP64 solve)
+        at the overlap capped at 2 (R - Rcell), as the 3-D pair loop calls it
+        (mechanics.hpp:1482-1487 of ABM-3d), E0 and gamma below, strength 1
+        for the reference's scale. The 2-D tree has no gamma-parametrised law;
+        this composition gives 2-D growth runs a surface energy. */
 };
 typedef struct bfm2d_params {
     double R, Rcell;
@@ -433,6 +440,8 @@ typedef struct bfm2d_params {
     int ncols;                       /* columns of the cells array (>= 17) */
     double strength, mindist, distance_exp;   /* adhesion_params "strength",
                                         "mindist", "distance_exp" */
+    double E0, gamma;                /* BFM2D_ADHESION_JKR_ISOTROPIC: EPS modulus and
+                                        surface energy density (Pa um) */
 } bfm2d_params;
 
 /* The 2-D mechanics use the context's device buffers; bfm2d_set_params may
@@ -481,6 +490,38 @@ int bfm2d_dev_update_neighbor_distances(bfm_ctx* ctx, const double* d_cells, int
 int bfm2d_dev_step_rk(bfm_ctx* ctx, const double* A, const double* b, const double* bs, int s, double dt,
                       const double* d_cells, int n, int ld, const double* d_noise, double* d_cells_out,
                       int ld_out, double* d_errors);
+
+/* The growth loop of the 2-D tree, runSimulationAdaptiveLagrangian
+ * (simulation.hpp:549-765 of ABM-2d), on the device 2-D step with the
+ * context's bfm2d_params: per iteration divide the cells with l > Ldiv
+ * (divideCells, growth.hpp:438-697 of ABM-2d, on the host with the
+ * reference's random draws in its order), rebuild the list and to_adhere
+ * after a division, take an adaptive RK step with the controller
+ * (simulation.hpp:636-712), grow (growCells, growth.hpp:51-57), refresh the
+ * distances, advance t and iter, rebuild every iter_update_neighbors.
+ * cfg fields read: R, Rcell, Ldiv, max/min_stepsize, max_error_allowed,
+ * min_error, neighbor_threshold (the 2-D driver uses 2R + Ldiv), growth
+ * mean/std of groups 1-2, daughter_length_std, daughter_angle_xy_bound (the
+ * 2-D daughter_angle_bound), n_cells / max_iter / max_time, the
+ * iter_update_* and max_tries counts, tableau 0..3 (the 2-D tree fixes
+ * Bogacki-Shampine, 1), seed, report_seconds, max_wall_seconds. Not built:
+ * noise (max_noise = 0), switching, confinement, growth voids, pole tracking.
+ * adhesion_map as in bfm2d_dev_build_neighbors. Cells: ncols of the params,
+ * leading dimension n_in in, cap out. parents_in NULL: one -1 entry per id
+ * up to the largest id; parents_out must hold n_parents_in + 2 (cap - n_in)
+ * ints. BFM_ERR_CAPACITY when a division would exceed cap (cap <= max_cells),
+ * BFM_ERR_NAN when a stage velocity is not finite. */
+int bfm2d_simulate(bfm_ctx* ctx, const bfm_sim_config* cfg, const int* adhesion_map, int n_groups,
+                   const double* cells_in, int n_in, const int* parents_in, int n_parents_in,
+                   double* cells_out, int cap, int* n_out, int* parents_out, int* n_parents_out,
+                   double* dt_trace, int cap_trace, bfm_sim_stats* stats);
+
+/* divideMaxLength + divideCells of the 2-D tree (growth.hpp:67-71, 438-697
+ * of ABM-2d) on host data, the 2-D columns: cells n x ncols (ncols >= 17)
+ * with leading dimension cap, daughters appended; the random stream starts
+ * fresh from seed. Needs no device. */
+int bfm2d_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, int ncols, double t,
+                       uint64_t seed, int* parents, int* n_parents, int parents_cap, int* tries_out);
 
 /* ---- frames, lineage, configuration and the driver (host only) ---------- */
 /* Message of the last failed call below (per thread). */

@@ -1681,16 +1681,13 @@ int oracle_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int ca
     return BFM_OK;
 }
 
-/* runSimulation's loop, simulation.hpp:1616-1864 (noise, switching, basal
- * pruning and file output off): divide, rebuild the neighbor list after a
- * division, adaptive RK step with the controller, grow, refresh the neighbor
- * distances, advance t and iter, periodic rebuild. The neighbor list comes
- * from the binned builder (the same list as getCellNeighbors). */
-int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double* cells_in, int n_in,
-                    double* cells_out, int cap, int* n_out, double* dt_trace, int cap_trace,
-                    int64_t* iters_out, int64_t* rk_steps_out, int nthreads)
+namespace {
+/* Butcher tableaux of runSimulation (simulation.hpp:1207-1268 of ABM-3d;
+ * Bogacki-Shampine is also the 2-D tree's fixed method, simulation.hpp:216-228
+ * of ABM-2d); -1 is velocity Verlet (no tableau, Heun-Euler placeholders).
+ * Returns false for an unknown id. */
+bool orc_tableau(int which, const double*& A, const double*& b, const double*& bs, int& s, double& order)
 {
-    const int ncols = p->ncols;
     static const double HE_A[4] = {0, 0, 1, 0}, HE_b[2] = {0.5, 0.5}, HE_bs[2] = {1, 0};
     static const double BS_A[16] = {0, 0, 0, 0, 1. / 2., 0, 0, 0, 0, 3. / 4., 0, 0, 2. / 9., 1. / 3., 4. / 9., 0};
     static const double BS_b[4] = {2. / 9., 1. / 3., 4. / 9., 0}, BS_bs[4] = {7. / 24., 1. / 4., 1. / 3., 1. / 8.};
@@ -1708,17 +1705,32 @@ int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double
     static const double DP_b[7] = {35. / 384., 0, 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0};
     static const double DP_bs[7] = {5179. / 57600., 0, 7571. / 16695., 393. / 640., -92097. / 339200.,
                                     187. / 2100., 1. / 40.};
-    const double *A, *b, *bs;
-    int s;
-    double order;
-    switch (cfg->tableau) {
+    switch (which) {
     case 0: A = HE_A; b = HE_b; bs = HE_bs; s = 2; order = 1; break;
     case 1: A = BS_A; b = BS_b; bs = BS_bs; s = 4; order = 2; break;
     case 2: A = RKF_A; b = RKF_b; bs = RKF_bs; s = 6; order = 4; break;
     case 3: A = DP_A; b = DP_b; bs = DP_bs; s = 7; order = 4; break;
     case -1: A = HE_A; b = HE_b; bs = HE_bs; s = 2; order = 1; break;   // velocity Verlet (no tableau)
-    default: return BFM_ERR_ARG;
+    default: return false;
     }
+    return true;
+}
+}  // namespace
+
+/* runSimulation's loop, simulation.hpp:1616-1864 (noise, switching, basal
+ * pruning and file output off): divide, rebuild the neighbor list after a
+ * division, adaptive RK step with the controller, grow, refresh the neighbor
+ * distances, advance t and iter, periodic rebuild. The neighbor list comes
+ * from the binned builder (the same list as getCellNeighbors). */
+int oracle_simulate(const bfm_sim_config* cfg, const bfm_params* p, const double* cells_in, int n_in,
+                    double* cells_out, int cap, int* n_out, double* dt_trace, int cap_trace,
+                    int64_t* iters_out, int64_t* rk_steps_out, int nthreads)
+{
+    const int ncols = p->ncols;
+    const double *A, *b, *bs;
+    int s;
+    double order;
+    if (!orc_tableau(cfg->tableau, A, b, bs, s, order)) return BFM_ERR_ARG;
     const bool verlet = cfg->tableau == -1;
     if (verlet && !(cfg->rho > 0)) return BFM_ERR_ARG;
     if (n_in < 1 || n_in > cap || cfg->iter_update_neighbors < 1 || cfg->iter_update_stepsize < 1)
@@ -1887,6 +1899,19 @@ void forces_2d(const double* cells, int n, int ld, const double* nb, int64_t m, 
                 else term *= (2 * R - dist);
                 v[0] = term * d12n[0];
                 v[1] = term * d12n[1];
+            } else if (p->adhesion_mode == BFM2D_ADHESION_JKR_ISOTROPIC) {
+                // forcesIsotropicJKRLagrange<T, 2> (adhesion.hpp:171-233 of
+                // ABM-3d) with max_overlap = 2 (R - Rcell) (the 3-D call,
+                // mechanics.hpp:1482-1487 of ABM-3d): v = (term2 - term1) d12n
+                double delta = 2 * R - dist;
+                if (delta > 2 * (R - Rcell)) delta = 2 * (R - Rcell);
+                const double Req = R / 2;
+                const double radius = oracle_jkr_contact_radius(delta, Req, p->E0, p->gamma, nullptr);
+                const double a3 = radius * radius * radius;
+                const double term1 = 4 * p->E0 * a3 / (3 * Req);
+                const double term2 = 4 * std::sqrt(kPi * a3 * p->gamma * p->E0);
+                v[0] = (term2 - term1) * d12n[0];
+                v[1] = (term2 - term1) * d12n[1];
             } else {
                 const double e = p->distance_exp;
                 double term1 = (dist <= p->mindist ? 1.0 / std::pow(p->mindist, e + 1) : 1.0 / std::pow(dist, e + 1));
@@ -2015,6 +2040,307 @@ void oracle2d_step_rk(const double* A, const double* b, const double* bs, int s,
         cells_new[(size_t)D2_DRX * n + q] = v1[q];
     }
     normalize_2d(cells_new, n, n);
+}
+
+}  // extern "C"
+
+/* ---- the 2-D growth loop (ABM-2d/ABM-2d-cpp) -------------------------------- */
+namespace {
+enum { D2_ID = 0, D2_T0 = 11, D2_GROWTH = 12, D2_GROUP = 16 };
+
+/* rotate, utils.hpp:373-380 of ABM-2d: the 2x2 rotation applied to n. */
+void orc2d_rotate(const double* n, double theta, double* out)
+{
+    const double rot[2][2] = {{std::cos(theta), -std::sin(theta)}, {std::sin(theta), std::cos(theta)}};
+    for (int r = 0; r < 2; ++r) out[r] = rot[r][0] * n[0] + rot[r][1] * n[1];
+}
+
+/* minDistToCell, growth.hpp:82-117 of ABM-2d: brute force over all other
+ * cells, |d| of distBetweenCells on the cells in the z = 0 plane. */
+double orc2d_min_dist_to_cell(const double* cells, int n, int ld, int i)
+{
+    const double ri[3] = {cell(cells, ld, i, D2_RX), cell(cells, ld, i, D2_RY), 0.0};
+    const double ni[3] = {cell(cells, ld, i, D2_NX), cell(cells, ld, i, D2_NY), 0.0};
+    const double hli = cell(cells, ld, i, D2_HALF_L);
+    double mindist = INFINITY;
+    for (int j = 0; j < n; ++j) {
+        if (i == j) continue;
+        const double rj[3] = {cell(cells, ld, j, D2_RX), cell(cells, ld, j, D2_RY), 0.0};
+        const double nj[3] = {cell(cells, ld, j, D2_NX), cell(cells, ld, j, D2_NY), 0.0};
+        double d[3], si, sj;
+        oracle_dist_between_cells(ri, ni, hli, rj, nj, cell(cells, ld, j, D2_HALF_L), d, &si, &sj);
+        const double dist = std::sqrt(d[0] * d[0] + d[1] * d[1]);
+        if (dist < mindist) mindist = dist;
+    }
+    return mindist;
+}
+
+/* divideCells of the 2-D tree (growth.hpp:438-697 of ABM-2d; the overload
+ * with one growth distribution per group) on a column-major population with
+ * leading dimension cap; daughters are appended. Returns the number of tries. */
+int orc2d_divide(const bfm_sim_config* cfg, std::mt19937& rng, double t, double* cells, int& n, int cap, int ncols,
+                 std::vector<int>& parents)
+{
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i)
+        if (cells[(int64_t)D2_L * cap + i] > cfg->Ldiv) idx.push_back(i);   // divideMaxLength, :67-71
+    const int nd = (int)idx.size();
+    if (nd == 0) return 0;
+    const double R = cfg->R, mindist_default = 2 * cfg->Rcell;
+    auto growth = [&](int group) {   // simulation.hpp:267-280 of ABM-2d
+        return cfg->growth_mean[group - 1] + cfg->growth_std[group - 1] * orc_normal(rng);
+    };
+    auto dlen = [&] {                // simulation.hpp:302-315
+        double r = 0.5 + cfg->daughter_length_std * orc_normal(rng);
+        return r < 0 ? 0.0 : (r > 1 ? 1.0 : r);
+    };
+    auto angle = [&] {               // simulation.hpp:316-324
+        double r = orc_uniform(rng);
+        return -cfg->daughter_angle_xy_bound + 2 * cfg->daughter_angle_xy_bound * r;
+    };
+    std::vector<int> check(nd, 1), satisfies(nd, 0);
+    for (int m = 0; m < nd; ++m)
+        if (orc2d_min_dist_to_cell(cells, n, cap, idx[m]) < mindist_default) check[m] = 0;
+    for (int m = 0; m < nd; ++m)
+        if (check[m] == 0) satisfies[m] = 1;
+    const int nt = n + nd;
+    std::vector<double> total((size_t)nt * ncols);
+    auto T = [&](int i, int c) -> double& { return total[(size_t)c * nt + i]; };
+    int ntries = 0, sat = 0;
+    for (int v : satisfies) sat += v;
+    while (ntries == 0 || (ntries < 10 && sat < nd)) {
+        for (int c = 0; c < ncols; ++c)
+            for (int i = 0; i < n; ++i) T(i, c) = cells[(int64_t)c * cap + i];
+        for (int m = 0; m < nd; ++m)
+            for (int c = 0; c < ncols; ++c) T(n + m, c) = cells[(int64_t)c * cap + idx[m]];
+        std::vector<double> th1(nd, 0), th2(nd, 0);
+        for (int m = 0; m < nd; ++m)
+            if (ntries < 9 && check[m] == 1) {
+                th1[m] = angle();
+                th2[m] = angle();
+            }
+        for (int m = 0; m < nd; ++m) {
+            const double u[2] = {T(idx[m], D2_NX), T(idx[m], D2_NY)};
+            double v[2];
+            orc2d_rotate(u, th1[m], v);
+            T(idx[m], D2_NX) = v[0]; T(idx[m], D2_NY) = v[1];
+            orc2d_rotate(u, th2[m], v);
+            T(n + m, D2_NX) = v[0]; T(n + m, D2_NY) = v[1];
+        }
+        std::vector<double> M(nd);
+        for (int m = 0; m < nd; ++m) M[m] = dlen();
+        for (int m = 0; m < nd; ++m) {
+            const int i = idx[m], d = n + m;
+            const double Ld = T(i, D2_L) - 2 * R;
+            const double L1 = M[m] * Ld, L2 = (1 - M[m]) * Ld;
+            const double div = -T(i, D2_HALF_L) + L1 + R;
+            const double delta1 = L1 / 2 + R, delta2 = L2 / 2 + R;
+            T(i, D2_L) = L1; T(i, D2_HALF_L) = L1 / 2;
+            T(d, D2_L) = L2; T(d, D2_HALF_L) = L2 / 2;
+            T(i, D2_RX) = T(i, D2_RX) + (div - delta1) * T(i, D2_NX);
+            T(d, D2_RX) = T(d, D2_RX) + (div + delta2) * T(d, D2_NX);
+            T(i, D2_RY) = T(i, D2_RY) + (div - delta1) * T(i, D2_NY);
+            T(d, D2_RY) = T(d, D2_RY) + (div + delta2) * T(d, D2_NY);
+            T(i, D2_T0) = t;
+            T(d, D2_T0) = t;
+        }
+        for (int m = 0; m < nd; ++m) T(idx[m], D2_GROWTH) = growth((int)T(idx[m], D2_GROUP));
+        for (int m = 0; m < nd; ++m) T(n + m, D2_GROWTH) = growth((int)T(n + m, D2_GROUP));
+        for (int m = 0; m < nd; ++m) {
+            if (check[m] != 1) continue;
+            const double d1 = orc2d_min_dist_to_cell(total.data(), nt, nt, idx[m]);
+            const double d2 = orc2d_min_dist_to_cell(total.data(), nt, nt, n + m);
+            if (d1 >= mindist_default && d2 >= mindist_default) satisfies[m] = 1;
+        }
+        sat = 0;
+        for (int v : satisfies) sat += v;
+        ntries++;
+    }
+    const int max_id = (int)parents.size() - 1;   // ids and lineage, growth.hpp:672-684
+    for (int m = 0; m < nd; ++m) {
+        const int parent_id = (int)T(idx[m], D2_ID);
+        T(idx[m], D2_ID) = max_id + (2 * m + 1);
+        T(n + m, D2_ID) = max_id + (2 * m + 2);
+        parents.push_back(parent_id);
+        parents.push_back(parent_id);
+    }
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < nt; ++i) cells[(int64_t)c * cap + i] = T(i, c);
+    n = nt;
+    return ntries;
+}
+}  // namespace
+
+extern "C" {
+
+int oracle2d_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, int ncols, double t,
+                          uint64_t seed, int* parents, int* n_parents, int parents_cap, int* tries_out)
+{
+    int nd = 0;
+    for (int i = 0; i < *n; ++i) nd += cells[(int64_t)D2_L * cap + i] > cfg->Ldiv;
+    if (*n + nd > cap || *n_parents + 2 * nd > parents_cap) return BFM_ERR_CAPACITY;
+    std::mt19937 rng((uint32_t)seed);
+    std::vector<int> par(parents, parents + *n_parents);
+    const int tries = orc2d_divide(cfg, rng, t, cells, *n, cap, ncols, par);
+    std::copy(par.begin(), par.end(), parents);
+    *n_parents = (int)par.size();
+    if (tries_out) *tries_out = tries;
+    return BFM_OK;
+}
+
+/* runSimulationAdaptiveLagrangian's loop of the 2-D tree (simulation.hpp:
+ * 549-765 of ABM-2d; no noise, switching, confinement, growth void or file
+ * output): divide, rebuild the list and to_adhere after a division (:551-625),
+ * adaptive RK step with the controller (:627-729), grow (growth.hpp:51-57),
+ * updateNeighborDistances, advance t and iter, periodic rebuild (:742-764).
+ * The list comes from the binned builder on the cells placed in the z = 0
+ * plane (the list of getCellNeighbors, oracle2d_get_cell_neighbors). */
+int oracle2d_simulate(const bfm_sim_config* cfg, const bfm2d_params* p, const int* amap, int n_groups,
+                      const double* cells_in, int n_in, double* cells_out, int cap, int* n_out, int* parents_out,
+                      int* n_parents_out, double* dt_trace, int cap_trace, int64_t* iters_out,
+                      int64_t* rk_steps_out)
+{
+    const int ncols = p->ncols;
+    const double *A, *b, *bs;
+    int s;
+    double order;
+    if (cfg->tableau < 0 || !orc_tableau(cfg->tableau, A, b, bs, s, order)) return BFM_ERR_ARG;
+    if (n_in < 1 || n_in > cap || ncols < 17 || cfg->iter_update_neighbors < 1 || cfg->iter_update_stepsize < 1)
+        return BFM_ERR_ARG;
+    std::vector<double> cells((size_t)cap * ncols, 0.0);
+    for (int c = 0; c < ncols; ++c)
+        for (int i = 0; i < n_in; ++i) cells[(size_t)c * cap + i] = cells_in[(size_t)c * n_in + i];
+    int max_id = 0;
+    for (int i = 0; i < n_in; ++i) max_id = std::max(max_id, (int)cells_in[i]);
+    std::vector<int> parents(max_id + 1, -1);
+    std::mt19937 rng((uint32_t)cfg->seed);
+    int n = n_in;
+    int64_t iter = 0, rk_steps = 0;
+    double t = 0, dt = cfg->max_stepsize;
+    const double R = cfg->R, Rcell = cfg->Rcell;
+    std::vector<double> nb6, c3;
+    std::vector<int> to_adhere;
+    int64_t m = 0;
+    auto embed = [&] {
+        c3.assign((size_t)15 * n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            c3[(size_t)0 * n + i] = cells[i];
+            c3[(size_t)C_RX * n + i] = cells[(size_t)D2_RX * cap + i];
+            c3[(size_t)C_RY * n + i] = cells[(size_t)D2_RY * cap + i];
+            c3[(size_t)C_NX * n + i] = cells[(size_t)D2_NX * cap + i];
+            c3[(size_t)C_NY * n + i] = cells[(size_t)D2_NY * cap + i];
+            c3[(size_t)C_L * n + i] = cells[(size_t)D2_L * cap + i];
+            c3[(size_t)C_HALF_L * n + i] = cells[(size_t)D2_HALF_L * cap + i];
+        }
+    };
+    auto build = [&] {   // getCellNeighbors + the driver's to_adhere rule (:242-262)
+        embed();
+        const int64_t need = oracle_get_cell_neighbors_binned(c3.data(), n, n, cfg->neighbor_threshold, R,
+                                                              cfg->Ldiv, nullptr, 0, 1);
+        const int64_t ld = std::max<int64_t>(need, 1);
+        std::vector<double> nb7((size_t)ld * 7, 0.0);
+        m = oracle_get_cell_neighbors_binned(c3.data(), n, n, cfg->neighbor_threshold, R, cfg->Ldiv, nb7.data(), ld, 1);
+        nb6.assign((size_t)std::max<int64_t>(m, 1) * 6, 0.0);
+        const int src[6] = {0, 1, 2, 3, 5, 6};
+        for (int q = 0; q < 6; ++q)
+            for (int64_t k = 0; k < m; ++k) nb6[(size_t)q * std::max<int64_t>(m, 1) + k] = nb7[(size_t)src[q] * ld + k];
+        to_adhere.assign(std::max<int64_t>(m, 1), 0);
+        const int64_t L = std::max<int64_t>(m, 1);
+        for (int64_t k = 0; k < m; ++k) {
+            const int i = (int)nb6[k], j = (int)nb6[L + k];
+            const int gi = (int)cells[(size_t)D2_GROUP * cap + i], gj = (int)cells[(size_t)D2_GROUP * cap + j];
+            const int g0 = std::min(gi, gj), g1 = std::max(gi, gj);
+            const double dx = nb6[2 * L + k], dy = nb6[3 * L + k];
+            const double dist = std::sqrt(dx * dx + dy * dy);
+            const bool in_map = amap && g0 >= 0 && g1 < n_groups && amap[g0 * n_groups + g1] != 0;
+            to_adhere[k] = in_map && dist > R + Rcell && dist < 2 * R;
+        }
+    };
+    auto update_distances = [&] {   // updateNeighborDistances (mechanics.hpp:376-446 of ABM-2d)
+        embed();
+        const int64_t L = std::max<int64_t>(m, 1);
+        std::vector<double> nb7((size_t)L * 7, 0.0);
+        const int dst[6] = {0, 1, 2, 3, 5, 6};
+        for (int q = 0; q < 6; ++q)
+            for (int64_t k = 0; k < m; ++k) nb7[(size_t)dst[q] * L + k] = nb6[(size_t)q * L + k];
+        oracle_update_neighbor_distances(c3.data(), n, n, nb7.data(), m, L);
+        for (int q = 0; q < 6; ++q)
+            for (int64_t k = 0; k < m; ++k) nb6[(size_t)q * L + k] = nb7[(size_t)dst[q] * L + k];
+    };
+    std::vector<double> packed, packed_new, errors;
+    auto rk = [&](double h) {   // stepRungeKuttaAdaptive + the scaled error (:639-650)
+        packed.assign((size_t)n * ncols, 0.0);
+        for (int c = 0; c < ncols; ++c)
+            std::memcpy(&packed[(size_t)c * n], &cells[(size_t)c * cap], sizeof(double) * n);
+        packed_new.assign((size_t)n * ncols, 0.0);
+        errors.assign((size_t)n * 4, 0.0);
+        const int64_t L = std::max<int64_t>(m, 1);
+        oracle2d_step_rk(A, b, bs, s, h, packed.data(), n, n, ncols, nb6.data(), m, L, to_adhere.data(), nullptr, p,
+                         packed_new.data(), errors.data());
+        ++rk_steps;
+        double mx = -INFINITY;
+        for (int q = 0; q < 4; ++q)
+            for (int i = 0; i < n; ++i) {
+                const double z = 1.0 + std::fabs(packed[(size_t)(D2_RX + q) * n + i]);
+                mx = std::max(mx, errors[(size_t)q * n + i] / (cfg->max_error_allowed * z));
+            }
+        return std::max(mx, cfg->min_error);
+    };
+    auto terminate = [&] {
+        bool hn = cfg->n_cells > 0, hi = cfg->max_iter > 0, ht = cfg->max_time > 0;
+        if (!hn && !hi && !ht) return true;
+        return (hn && n >= cfg->n_cells) || (hi && iter >= cfg->max_iter) || (ht && t >= cfg->max_time);
+    };
+    build();
+    while (!terminate()) {
+        int nd = 0;
+        for (int i = 0; i < n; ++i) nd += cells[(size_t)D2_L * cap + i] > cfg->Ldiv;
+        if (nd > 0) {
+            if (n + nd > cap) return BFM_ERR_CAPACITY;
+            orc2d_divide(cfg, rng, t, cells.data(), n, cap, ncols, parents);
+            build();
+        }
+        double max_error = rk(dt);
+        if (iter % cfg->iter_update_stepsize == 0) {
+            bool exceeded = max_error > 1.0;
+            double factor = 0.9 * std::pow(1.0 / max_error, 1.0 / (order + 1));
+            if (factor >= 10) factor = 10;
+            else if (factor < 0.2) factor = 0.2;
+            int j = 0;
+            while (exceeded && j < cfg->max_tries_update_stepsize) {
+                max_error = rk(dt * factor);
+                exceeded = max_error > 1.0;
+                factor *= 0.9 * std::pow(1.0 / max_error, 1.0 / (order + 1));
+                if (factor >= 10) { factor = 10; break; }
+                else if (factor < 0.2) { factor = 0.2; break; }
+                j++;
+            }
+            if (dt * factor < cfg->min_stepsize) factor = cfg->min_stepsize / dt;
+            else if (dt * factor > cfg->max_stepsize) factor = cfg->max_stepsize / dt;
+            dt *= factor;
+            rk(dt);
+        }
+        for (int c = 0; c < ncols; ++c)
+            std::memcpy(&cells[(size_t)c * cap], &packed_new[(size_t)c * n], sizeof(double) * n);
+        for (int i = 0; i < n; ++i) {   // growCells, growth.hpp:51-57 of ABM-2d
+            double& l = cells[(size_t)D2_L * cap + i];
+            l += (cells[(size_t)D2_GROWTH * cap + i] * (4 * R / 3 + l)) * dt;
+            cells[(size_t)D2_HALF_L * cap + i] = l / 2;
+        }
+        update_distances();
+        if (dt_trace && iter < cap_trace) dt_trace[iter] = dt;
+        t += dt;
+        iter++;
+        if (iter % cfg->iter_update_neighbors == 0) build();
+    }
+    for (int c = 0; c < ncols; ++c)
+        std::memcpy(&cells_out[(size_t)c * cap], &cells[(size_t)c * cap], sizeof(double) * n);
+    *n_out = n;
+    if (parents_out) std::copy(parents.begin(), parents.end(), parents_out);
+    if (n_parents_out) *n_parents_out = (int)parents.size();
+    if (iters_out) *iters_out = iter;
+    if (rk_steps_out) *rk_steps_out = rk_steps;
+    return BFM_OK;
 }
 
 }  // extern "C"

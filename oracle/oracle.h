@@ -147,6 +147,14 @@ void oracle2d_get_velocities(const double* cells, int n, int ld, const double* n
 void oracle2d_step_rk(const double* A, const double* b, const double* bs, int s, double dt, const double* cells,
                       int n, int ld, int ncols, const double* nb, int64_t m, int64_t ld_nb, const int* to_adhere,
                       const double* noise, const bfm2d_params* p, double* cells_new, double* errors);
+/* divideCells of the 2-D tree on host data (bfm2d_divide_cells' contract). */
+int oracle2d_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, int ncols, double t,
+                          uint64_t seed, int* parents, int* n_parents, int parents_cap, int* tries_out);
+/* The 2-D growth loop (bfm2d_simulate's contract; tableau 0..3, one thread). */
+int oracle2d_simulate(const bfm_sim_config* cfg, const bfm2d_params* p, const int* amap, int n_groups,
+                      const double* cells_in, int n_in, double* cells_out, int cap, int* n_out, int* parents_out,
+                      int* n_parents_out, double* dt_trace, int cap_trace, int64_t* iters_out,
+                      int64_t* rk_steps_out);
 
 #ifdef __cplusplus
 }

@@ -425,6 +425,11 @@ def _lib2d():
         L.oracle2d_step_rk.argtypes = [_dp, _dp, _dp, C.c_int, d, _dp, C.c_int, C.c_int, C.c_int, _dp, i64,
                                        i64, _ip, _dp, vp, _dp, _dp]
         L.oracle2d_step_rk.restype = None
+        L.oracle2d_divide_cells.argtypes = [vp, _dp, _ip, C.c_int, C.c_int, d, C.c_uint64, _ip, _ip, C.c_int, _ip]
+        L.oracle2d_divide_cells.restype = C.c_int
+        L.oracle2d_simulate.argtypes = [vp, vp, _ip, C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, _ip, _ip, _dp,
+                                        C.c_int, C.POINTER(i64), C.POINTER(i64)]
+        L.oracle2d_simulate.restype = C.c_int
         L._o2d = True
     return L
 
@@ -483,3 +488,45 @@ def step_rk_2d(A, b, bs, dt, cells, nb, params, to_adhere=None, noise=None):
                               max(nb.shape[0], 1), ta.ctypes.data_as(_ip), _p(nz),
                               C.cast(C.byref(params), C.c_void_p), _p(out), _p(err))
     return out, err
+
+
+def divide_cells_2d(cfg, cells, t, seed, parents=None):
+    """oracle2d_divide_cells on a copy (2-D columns); returns (cells, parents, tries)."""
+    cells = np.asarray(cells, dtype=np.float64)
+    n, ncols = cells.shape
+    cap = 2 * n + 1
+    buf = np.zeros((cap, ncols), order="F")
+    buf[:n] = cells
+    if parents is None:
+        parents = -np.ones(int(cells[:, 0].max()) + 1, dtype=np.int32)
+    pcap = len(parents) + 2 * n + 2
+    par = np.zeros(pcap, dtype=np.int32)
+    par[:len(parents)] = parents
+    n_io, n_par, tries = C.c_int(n), C.c_int(len(parents)), C.c_int()
+    rc = _lib2d().oracle2d_divide_cells(C.cast(C.byref(cfg), C.c_void_p), _p(buf), C.byref(n_io), cap, ncols, t,
+                                        seed, par.ctypes.data_as(_ip), C.byref(n_par), pcap, C.byref(tries))
+    if rc != 0:
+        raise RuntimeError(f"oracle2d_divide_cells: status {rc}")
+    return np.asfortranarray(buf[:n_io.value]), par[:n_par.value].copy(), tries.value
+
+
+def simulate_2d(cfg, params, adhesion_map, cells, cap, trace_cap=1_000_000):
+    """oracle2d_simulate; returns (cells, parents, dt_trace, iterations, rk_steps)."""
+    cells = _f(cells)
+    n_in = cells.shape[0]
+    out = np.zeros((cap, params.ncols), order="F")
+    amap = None if adhesion_map is None else np.ascontiguousarray(adhesion_map, dtype=np.int32)
+    ng = 0 if amap is None else amap.shape[0]
+    n_out, n_par = C.c_int(), C.c_int()
+    par = np.zeros(int(cells[:, 0].max()) + 1 + 2 * cap + 2, dtype=np.int32)
+    trace = np.zeros(trace_cap)
+    iters, steps = C.c_int64(), C.c_int64()
+    rc = _lib2d().oracle2d_simulate(C.cast(C.byref(cfg), C.c_void_p), C.cast(C.byref(params), C.c_void_p),
+                                    None if amap is None else amap.ctypes.data_as(_ip), ng, _p(cells), n_in,
+                                    _p(out), cap, C.byref(n_out), par.ctypes.data_as(_ip), C.byref(n_par),
+                                    _p(trace), trace_cap, C.byref(iters), C.byref(steps))
+    if rc != 0:
+        raise RuntimeError(f"oracle2d_simulate: status {rc}")
+    it = min(iters.value, trace_cap)
+    return (np.asfortranarray(out[:n_out.value]), par[:n_par.value].copy(), trace[:it].copy(), iters.value,
+            steps.value)
