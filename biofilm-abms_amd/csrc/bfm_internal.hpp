@@ -51,14 +51,23 @@ int step_verlet(bfm_ctx* c, double dt, double rho, const double* d_cells, int n,
 double order_key_value(unsigned long long k);
 
 // The 2-D step for the 2-D growth loop, on the internal list
-// (bfm2d_dev_build_neighbors) and the context's bfm2d_params: the pair
-// records of the list's frozen rows and to_adhere flags (once per iteration),
-// then each RK attempt: the stages and the update into d_new; tol > 0 also
-// writes the controller's scaled-error maximum to d_errmax (ordered key).
-// Non-finite stage velocities set ST_NAN in the context's status word.
-int step2d_prepare(bfm_ctx* c, int n);
-int step2d_attempt(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld, double* d_new,
-                   int ld_new, double tol, unsigned long long* d_errmax);
+// (bfm2d_dev_build_neighbors) and the context's bfm2d_params.
+// step2d_prepare: the pair records of the list's rows and to_adhere flags,
+// then the per-cell frozen-geometry force sums d_sums (8 n doubles).
+// step2d_rows_update: updateNeighborDistances on the rows, their records and
+// the sums. step2d_attempt: one RK attempt from the sums into d_new (one
+// launch); tol > 0 also folds the controller's scaled-error maximum into
+// d_errmax (ordered key, starting from -inf); d_count != NULL also grows the
+// new cells by dt (growCells) and adds the number with l > Ldiv to *d_count.
+// step2d_publish: writes {error key, count, status} to host-mapped memory and
+// resets the key and the count. Non-finite stage velocities set ST_NAN in
+// the context's status word. All only enqueue.
+int step2d_prepare(bfm_ctx* c, int n, double* d_sums);
+int step2d_rows_update(bfm_ctx* c, const double* d_cells, int n, int ld, double* d_sums);
+int step2d_attempt(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld,
+                   const double* d_sums, double* d_new, int ld_new, double tol, unsigned long long* d_errmax,
+                   int* d_count, double R, double Ldiv);
+int step2d_publish(bfm_ctx* c, unsigned long long* d_errmax, int* d_count, unsigned long long* host_mapped);
 
 // Frame output of a one-colony run: after every iteration in which the
 // simulated time passes a multiple of dt_write (simulation.hpp:2023-2063),

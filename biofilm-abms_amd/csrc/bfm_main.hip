@@ -3117,6 +3117,43 @@ struct Params2D {
     int mode;
 };
 
+// One row's record for the 2-D stage kernel: {dx, dy, s, t, repulsion
+// prefactor, adhesion magnitude, repulsion on, adhesion on}.
+__device__ __forceinline__ void pair_record_2d(const Params2D& P, double dx, double dy, double s_i, double s_j,
+                                               int adhere, double* __restrict__ rec)
+{
+    double2* r = reinterpret_cast<double2*>(rec);
+    const double mag = sqrt(dx * dx + dy * dy);
+    const double overlap = 2 * P.R - mag;
+    const double cap = 2 * (P.R - P.Rcell);
+    double pref = 0;
+    if (overlap > 0 && overlap < cap) pref = P.p0 * pow(overlap, 1.5);
+    else if (overlap >= cap) pref = P.p1 + P.p2 * pow(overlap - cap, 1.5);
+    double a = 0, adh = 0;
+    if (P.mode != BFM2D_ADHESION_NONE && adhere && overlap > 0) {
+        adh = 1;
+        if (P.mode == BFM2D_ADHESION_JKR) {
+            a = kPi * P.R;
+            a *= (mag <= P.mindist) ? (2 * P.R - P.mindist) : (2 * P.R - mag);
+        } else if (P.mode == BFM2D_ADHESION_JKR_ISOTROPIC) {
+            // forcesIsotropicJKRLagrange<T, 2> (adhesion.hpp:171-233 of
+            // ABM-3d) at the overlap capped at 2 (R - Rcell): a = term2 - term1
+            const double delta = overlap > cap ? cap : overlap;
+            const double Req = P.R / 2;
+            const double radius = jkr_radius_newton(delta, Req, P.E0, P.gamma, 0.0, 200, nullptr);
+            a = -jkr_magnitude(radius, Req, P.E0, P.gamma);
+        } else {
+            const double t1 = (mag <= P.mindist) ? 1.0 / pow(P.mindist, P.dexp + 1) : 1.0 / pow(mag, P.dexp + 1);
+            const double t2 = 1.0 / pow(2 * P.R, P.dexp + 1);
+            a = P.dexp * (t1 - t2);
+        }
+    }
+    r[0] = make_double2(dx, dy);
+    r[1] = make_double2(s_i, s_j);
+    r[2] = make_double2(pref, a);
+    r[3] = make_double2(overlap > 0 ? 1.0 : 0.0, adh);
+}
+
 // Per-row magnitudes of cellCellRepulsiveForces and cellCellAdhesiveForces
 // (mechanics.hpp:448-694 of ABM-2d), which read only the frozen distance:
 // {dx, dy, s, t, repulsion prefactor, adhesion magnitude a (v = a d/|d|,
@@ -3137,36 +3174,8 @@ k2d_pair(const int* __restrict__ d_m, int n, const int* __restrict__ pi, const i
             continue;
         }
         if (jcnt) atomicAdd(&jcnt[j], 1);
-        const double dx = pdx[k], dy = pdy[k];
-        const double mag = sqrt(dx * dx + dy * dy);
-        const double overlap = 2 * P.R - mag;
-        const double cap = 2 * (P.R - P.Rcell);
-        double pref = 0;
-        if (overlap > 0 && overlap < cap) pref = P.p0 * pow(overlap, 1.5);
-        else if (overlap >= cap) pref = P.p1 + P.p2 * pow(overlap - cap, 1.5);
-        double a = 0, adh = 0;
-        if (P.mode != BFM2D_ADHESION_NONE && to_adhere[k] && overlap > 0) {
-            adh = 1;
-            if (P.mode == BFM2D_ADHESION_JKR) {
-                a = kPi * P.R;
-                a *= (mag <= P.mindist) ? (2 * P.R - P.mindist) : (2 * P.R - mag);
-            } else if (P.mode == BFM2D_ADHESION_JKR_ISOTROPIC) {
-                // forcesIsotropicJKRLagrange<T, 2> (adhesion.hpp:171-233 of
-                // ABM-3d) at the overlap capped at 2 (R - Rcell): a = term2 - term1
-                const double delta = overlap > cap ? cap : overlap;
-                const double Req = P.R / 2;
-                const double radius = jkr_radius_newton(delta, Req, P.E0, P.gamma, 0.0, 200, nullptr);
-                a = -jkr_magnitude(radius, Req, P.E0, P.gamma);
-            } else {
-                const double t1 = (mag <= P.mindist) ? 1.0 / pow(P.mindist, P.dexp + 1) : 1.0 / pow(mag, P.dexp + 1);
-                const double t2 = 1.0 / pow(2 * P.R, P.dexp + 1);
-                a = P.dexp * (t1 - t2);
-            }
-        }
-        r[0] = make_double2(dx, dy);
-        r[1] = make_double2(ps[k], pt[k]);
-        r[2] = make_double2(pref, a);
-        r[3] = make_double2(overlap > 0 ? 1.0 : 0.0, adh);
+        pair_record_2d(P, pdx[k], pdy[k], ps[k], pt[k], P.mode != BFM2D_ADHESION_NONE ? to_adhere[k] : 0,
+                       rec + (int64_t)k * 8);
     }
 }
 
@@ -3263,19 +3272,37 @@ k2d_stage(int n, const double* __restrict__ cells, int ld, const double* __restr
     }
 }
 
+// Growth of the 2-D loop fused into the accepted RK update: growCells
+// (growth.hpp:51-57 of ABM-2d) on the new cells and divideMaxLength's count
+// (l > Ldiv, growth.hpp:67-71) into *count.
+struct Grow2D {
+    int on;
+    double R, Ldiv, dt;
+    int* count;
+};
+
 // RK update of the 2-D step (mechanics.hpp:1794-1816 of ABM-2d); errors may
 // be NULL; tol > 0 also reduces the controller's scaled error
 // max e / (tol (1 + |y|)) over the four coordinates (simulation.hpp:639-650
-// of ABM-2d) into errmax (ordered key).
+// of ABM-2d) into errmax (ordered key); gr.on applies the growth step.
 __global__ void __launch_bounds__(256)
 k2d_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau tb, double dt,
           const double* __restrict__ kbuf, double* __restrict__ out, int ld_out, double* __restrict__ errors,
-          double tol, unsigned long long* __restrict__ errmax)
+          double tol, unsigned long long* __restrict__ errmax, Grow2D gr)
 {
     double local = -INFINITY;
+    int divide = 0;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         for (int col = 0; col < ncols; ++col)
             if (col < D_RX || col > D_DRX + 3) out[(int64_t)col * ld_out + c] = cellv(cells, ld, c, col);
+        if (gr.on) {
+            double l = cellv(cells, ld, c, D_L);
+            const double g = cellv(cells, ld, c, 12);
+            l = l + (g * (4 * gr.R / 3 + l)) * gr.dt;
+            out[(int64_t)D_L * ld_out + c] = l;
+            out[(int64_t)D_HALF_L * ld_out + c] = l / 2;
+            divide += l > gr.Ldiv;
+        }
         double y[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -3301,6 +3328,208 @@ k2d_final(int n, int ncols, const double* __restrict__ cells, int ld, Tableau tb
     if (tol > 0) {
         for (int off = 32; off > 0; off >>= 1) local = fmax(local, __shfl_xor(local, off));
         if ((threadIdx.x & 63) == 0) atomicMax(errmax, order_key(local));
+    }
+    if (gr.on) {
+        for (int off = 32; off > 0; off >>= 1) divide += __shfl_xor(divide, off);
+        if ((threadIdx.x & 63) == 0 && divide) atomicAdd(gr.count, divide);
+    }
+}
+
+// updateNeighborDistances of the 2-D tree (mechanics.hpp:376-446 of ABM-2d)
+// on the internal list's rows from the 2-D columns (the segments in the
+// z = 0 plane, as the neighbor search sees them), followed by each row's
+// pair record (k2d_pair) for the next step.
+__global__ void __launch_bounds__(256)
+k2d_rows_update(const int* __restrict__ d_m, int n, const double* __restrict__ cells, int ld,
+                const int* __restrict__ pi, const int* __restrict__ pj, double* __restrict__ pdx,
+                double* __restrict__ pdy, double* __restrict__ pdz, double* __restrict__ ps, double* __restrict__ pt,
+                const int* __restrict__ to_adhere, Params2D P, double* __restrict__ rec)
+{
+    const int m = *d_m;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const int i = pi[k], j = pj[k];
+        if ((unsigned)i >= (unsigned)n || (unsigned)j >= (unsigned)n) {
+            double2* r = reinterpret_cast<double2*>(rec + (int64_t)k * 8);
+            r[0] = r[1] = r[2] = r[3] = make_double2(0.0, 0.0);
+            continue;
+        }
+        Seg a, b;
+        a.r[0] = cellv(cells, ld, i, D_RX); a.r[1] = cellv(cells, ld, i, D_RY); a.r[2] = 0.0;
+        a.n[0] = cellv(cells, ld, i, D_NX); a.n[1] = cellv(cells, ld, i, D_NY); a.n[2] = 0.0;
+        a.hl = cellv(cells, ld, i, D_HALF_L);
+        b.r[0] = cellv(cells, ld, j, D_RX); b.r[1] = cellv(cells, ld, j, D_RY); b.r[2] = 0.0;
+        b.n[0] = cellv(cells, ld, j, D_NX); b.n[1] = cellv(cells, ld, j, D_NY); b.n[2] = 0.0;
+        b.hl = cellv(cells, ld, j, D_HALF_L);
+        double d[3], si, sj;
+        dist_between_cells(a, b, d, si, sj);
+        pdx[k] = d[0]; pdy[k] = d[1]; pdz[k] = d[2]; ps[k] = si; pt[k] = sj;
+        pair_record_2d(P, d[0], d[1], si, sj, P.mode != BFM2D_ADHESION_NONE ? to_adhere[k] : 0,
+                       rec + (int64_t)k * 8);
+    }
+}
+
+// Frozen-geometry sums of the 2-D forces for the whole RK step. The pair
+// geometry (d, s, t) of every row is frozen during a step and only the
+// cell's own stage orientation n enters its constrained torques, each of
+// the form c (v - (n . v) n) (cellCellRepulsiveForces / the adhesive loop,
+// mechanics.hpp:448-694 of ABM-2d). Summed over a cell's rows in row order
+// that is V - (n . V) n with V = sum c v, so per cell and step four vectors
+// carry all of them: the repulsive and adhesive position forces and their
+// torque sums V (adhesion before the strength factor). sums: 8 doubles per
+// cell {Frep, Vrep, Fadh, Vadh}.
+template <bool CSR>
+__global__ void __launch_bounds__(256)
+k2d_gather(int n, const double* __restrict__ rec, RowLists L, double* __restrict__ sums)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for_rows<CSR>(c, L, [&](int row, bool role_i) {
+            const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)row * 8);
+            const double2 d = r[0], st2 = r[1], mg = r[2], fl = r[3];
+            if (fl.x == 0) return;
+            const double mag = sqrt(d.x * d.x + d.y * d.y);
+            const double ux = d.x / mag, uy = d.y / mag;
+            const double vx = mg.x * ux, vy = mg.x * uy;
+            if (role_i) {           // +v, s (v - (n.v) n)
+                f[0] += vx; f[1] += vy;
+                f[2] += st2.x * vx; f[3] += st2.x * vy;
+            } else {                // -v, -t (v - (n.v) n)
+                f[0] += -vx; f[1] += -vy;
+                f[2] += -(st2.y * vx); f[3] += -(st2.y * vy);
+            }
+            if (fl.y != 0) {
+                const double ax = mg.y * ux, ay = mg.y * uy;
+                if (role_i) {       // -v, -s (v - (n.v) n)
+                    f[4] += -ax; f[5] += -ay;
+                    f[6] += -(st2.x * ax); f[7] += -(st2.x * ay);
+                } else {            // +v, +t (v - (n.v) n)
+                    f[4] += ax; f[5] += ay;
+                    f[6] += st2.y * ax; f[7] += st2.y * ay;
+                }
+            }
+        });
+        double2* o = reinterpret_cast<double2*>(sums + (int64_t)c * 8);
+        o[0] = make_double2(f[0], f[1]);
+        o[1] = make_double2(f[2], f[3]);
+        o[2] = make_double2(f[4], f[5]);
+        o[3] = make_double2(f[6], f[7]);
+    }
+}
+
+// One whole RK attempt of the 2-D step per cell (stepRungeKuttaAdaptive,
+// mechanics.hpp:1710-1817 of ABM-2d) from the frozen-geometry sums: the S
+// stages in registers (stage coordinates, 2-D normalizeOrientations,
+// (repulsion + adhesion) + noise, the composite drag), then the update of
+// k2d_final (error key, growth and division count as there). Cells are
+// independent within a step, so the attempt is one launch.
+template <int S>
+__global__ void __launch_bounds__(256)
+k2d_attempt(int n, int ncols, const double* __restrict__ cells, int ld, const double* __restrict__ sums,
+            const double* __restrict__ noise, double strength, double scd, double R, Tableau tb, double dt,
+            double* __restrict__ out, int ld_out, double* __restrict__ errors, double tol,
+            unsigned long long* __restrict__ errmax, Grow2D gr, int* __restrict__ status)
+{
+    double local = -INFINITY;
+    int divide = 0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const double2* sp = reinterpret_cast<const double2*>(sums + (int64_t)c * 8);
+        const double2 Fr = sp[0], Vr = sp[1], Fa = sp[2], Va = sp[3];
+        double y0[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y0[q] = cellv(cells, ld, c, D_RX + q);
+        double nz[4] = {0, 0, 0, 0};
+        if (noise) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) nz[q] = noise[(int64_t)q * n + c];
+        }
+        const double l = cellv(cells, ld, c, D_L);
+        const double drag = cellv(cells, ld, c, D_ETA0) + cellv(cells, ld, c, D_ETA1) * scd / R;
+        const double K = l * drag, Lr = l * l * l * drag / 12;
+        double k[S][4];
+        bool bad = false;
+#pragma unroll
+        for (int st = 0; st < S; ++st) {
+            double y[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                y[q] = y0[q];
+                if (st > 0) {
+                    double mult = 0;
+#pragma unroll
+                    for (int j = 0; j < st; ++j) mult += k[j][q] * tb.A[st * S + j];
+                    y[q] += mult * dt;
+                }
+            }
+            if (st > 0) {
+                const double nrm = sqrt(y[2] * y[2] + y[3] * y[3]);
+                y[2] /= nrm;
+                y[3] /= nrm;
+            }
+            const double nx = y[2], ny = y[3];
+            const double pr = nx * Vr.x + ny * Vr.y, pa = nx * Va.x + ny * Va.y;
+            const double e[4] = {Fr.x + strength * Fa.x, Fr.y + strength * Fa.y,
+                                 (Vr.x - pr * nx) + strength * (Va.x - pa * nx),
+                                 (Vr.y - pr * ny) + strength * (Va.y - pa * ny)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double v = -(e[q] + nz[q]) / (q < 2 ? K : Lr);
+                k[st][q] = v;
+                bad |= !isfinite(v);
+            }
+        }
+        if (bad) atomicOr(status, ST_NAN);
+        for (int col = 0; col < ncols; ++col)
+            if (col < D_RX || col > D_DRX + 3) out[(int64_t)col * ld_out + c] = cellv(cells, ld, c, col);
+        if (gr.on) {
+            const double g = cellv(cells, ld, c, 12);
+            const double l1 = l + (g * (4 * gr.R / 3 + l)) * gr.dt;
+            out[(int64_t)D_L * ld_out + c] = l1;
+            out[(int64_t)D_HALF_L * ld_out + c] = l1 / 2;
+            divide += l1 > gr.Ldiv;
+        }
+        double y[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double v1 = 0, v2 = 0;
+#pragma unroll
+            for (int st = 0; st < S; ++st) {
+                v1 += k[st][q] * tb.b[st];
+                v2 += k[st][q] * tb.bs[st];
+            }
+            const double d1 = v1 * dt, d2 = v2 * dt;
+            y[q] = y0[q] + d1;
+            if (errors) errors[(int64_t)q * n + c] = d1 - d2;
+            if (tol > 0) local = fmax(local, (d1 - d2) / (tol * (1.0 + fabs(y0[q]))));
+            out[(int64_t)(D_DRX + q) * ld_out + c] = v1;
+        }
+        const double nrm = sqrt(y[2] * y[2] + y[3] * y[3]);
+        y[2] /= nrm;
+        y[3] /= nrm;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[(int64_t)(D_RX + q) * ld_out + c] = y[q];
+    }
+    if (tol > 0) {
+        for (int off = 32; off > 0; off >>= 1) local = fmax(local, __shfl_xor(local, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(errmax, order_key(local));
+    }
+    if (gr.on) {
+        for (int off = 32; off > 0; off >>= 1) divide += __shfl_xor(divide, off);
+        if ((threadIdx.x & 63) == 0 && divide) atomicAdd(gr.count, divide);
+    }
+}
+
+// Readback of one 2-D loop attempt: the scaled-error maximum, the division
+// count and the status word into host-mapped memory; the maximum and the
+// count start again from their identities.
+__global__ void k2d_publish(unsigned long long* __restrict__ errmax, int* __restrict__ count,
+                            const int* __restrict__ status, unsigned long long* __restrict__ host)
+{
+    if (threadIdx.x == 0) {
+        host[0] = *errmax;
+        host[1] = (unsigned long long)(unsigned)*count;
+        host[2] = (unsigned long long)(unsigned)*status;
+        *errmax = order_key(-INFINITY);
+        *count = 0;
     }
 }
 
@@ -3499,7 +3728,7 @@ int set_replicate_count(bfm_ctx* c, int n_rep)
 
 double order_key_value(unsigned long long k) { return order_val_host(k); }
 
-int step2d_prepare(bfm_ctx* c, int n)
+int step2d_prepare(bfm_ctx* c, int n, double* d_sums)
 {
     return guarded(c, [&] {
         require(c->list_n == n && c->list_csr, BFM_ERR_ARG,
@@ -3507,18 +3736,61 @@ int step2d_prepare(bfm_ctx* c, int n)
         const Params2D P = params_2d(c);
         KLAUNCH(c, "pair_2d", k2d_pair, kPairGrid, 256, 0, c->d_m, n, c->pi, c->pj, c->pdx, c->pdy, c->ps, c->pt,
                 c->d_adhere, P, c->d_rec, (int*)nullptr);
+        KLAUNCH(c, "gather_2d", k2d_gather<true>, grid_for(n, 256), 256, 0, n, c->d_rec, row_lists(c), d_sums);
     });
 }
 
-int step2d_attempt(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld, double* d_new,
-                   int ld_new, double tol, unsigned long long* d_errmax)
+int step2d_rows_update(bfm_ctx* c, const double* d_cells, int n, int ld, double* d_sums)
+{
+    return guarded(c, [&] {
+        require(c->list_n == n && c->list_csr, BFM_ERR_ARG,
+                "no device neighbor list for these cells (bfm2d_dev_build_neighbors)");
+        const Params2D P = params_2d(c);
+        KLAUNCH(c, "rows_2d", k2d_rows_update, kPairGrid, 256, 0, c->d_m, n, d_cells, ld, c->pi, c->pj, c->pdx,
+                c->pdy, c->pdz, c->ps, c->pt, c->d_adhere, P, c->d_rec);
+        KLAUNCH(c, "gather_2d", k2d_gather<true>, grid_for(n, 256), 256, 0, n, c->d_rec, row_lists(c), d_sums);
+    });
+}
+
+int step2d_attempt(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, int n, int ld,
+                   const double* d_sums, double* d_new, int ld_new, double tol, unsigned long long* d_errmax,
+                   int* d_count, double R, double Ldiv)
 {
     return guarded(c, [&] {
         require(n >= 1 && n <= c->max_cells && ld >= n && ld_new >= n, BFM_ERR_ARG, "invalid n, ld or ld_new");
-        if (tol > 0) KLAUNCH(c, "err_init", k_err_init, 1, 1, 0, d_errmax);
-        stages_2d(c, tb, dt, d_cells, ld, n, nullptr);
-        KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, d_cells, ld, tb, dt, c->d_k,
-                d_new, ld_new, (double*)nullptr, tol, d_errmax);
+        const Params2D P = params_2d(c);
+        Grow2D gr{};
+        if (d_count) gr = Grow2D{1, R, Ldiv, dt, d_count};
+        const int g = grid_for(n, 256);
+        const int ncols = c->p2d.ncols;
+        const double* nz = nullptr;
+        switch (tb.s) {
+        case 2:
+            KLAUNCH(c, "attempt_2d", k2d_attempt<2>, g, 256, 0, n, ncols, d_cells, ld, d_sums, nz, P.strength, P.scd,
+                    P.R, tb, dt, d_new, ld_new, (double*)nullptr, tol, d_errmax, gr, c->d_status);
+            break;
+        case 4:
+            KLAUNCH(c, "attempt_2d", k2d_attempt<4>, g, 256, 0, n, ncols, d_cells, ld, d_sums, nz, P.strength, P.scd,
+                    P.R, tb, dt, d_new, ld_new, (double*)nullptr, tol, d_errmax, gr, c->d_status);
+            break;
+        case 6:
+            KLAUNCH(c, "attempt_2d", k2d_attempt<6>, g, 256, 0, n, ncols, d_cells, ld, d_sums, nz, P.strength, P.scd,
+                    P.R, tb, dt, d_new, ld_new, (double*)nullptr, tol, d_errmax, gr, c->d_status);
+            break;
+        case 7:
+            KLAUNCH(c, "attempt_2d", k2d_attempt<7>, g, 256, 0, n, ncols, d_cells, ld, d_sums, nz, P.strength, P.scd,
+                    P.R, tb, dt, d_new, ld_new, (double*)nullptr, tol, d_errmax, gr, c->d_status);
+            break;
+        default:
+            require(false, BFM_ERR_ARG, "the 2-D loop's tableaux have 2, 4, 6 or 7 stages");
+        }
+    });
+}
+
+int step2d_publish(bfm_ctx* c, unsigned long long* d_errmax, int* d_count, unsigned long long* host_mapped)
+{
+    return guarded(c, [&] {
+        KLAUNCH(c, "publish_2d", k2d_publish, 1, 64, 0, d_errmax, d_count, c->d_status, host_mapped);
     });
 }
 
@@ -4206,7 +4478,7 @@ int bfm2d_step_rk(bfm_ctx* c, const double* A, const double* b, const double* bs
         prepare_2d(c, cells, n, ld, nb, m, ld_nb, to_adhere, noise);
         stages_2d(c, tb, dt, c->d_cells, c->max_cells, n, noise ? c->d_noise : nullptr);
         KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, c->d_cells, c->max_cells, tb,
-                dt, c->d_k, c->d_cells_new, c->max_cells, c->d_err6, 0.0, (unsigned long long*)nullptr);
+                dt, c->d_k, c->d_cells_new, c->max_cells, c->d_err6, 0.0, (unsigned long long*)nullptr, Grow2D{});
         if (n > 0) {
             HIPCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * ld_out, c->d_cells_new,
                                     sizeof(double) * c->max_cells, sizeof(double) * n, c->p2d.ncols,
@@ -4277,7 +4549,8 @@ int bfm2d_dev_step_rk(bfm_ctx* c, const double* A, const double* b, const double
                 c->d_adhere, P, c->d_rec, (int*)nullptr);
         stages_2d(c, tb, dt, d_cells, ld, n, d_noise);
         KLAUNCH(c, "final_2d", k2d_final, grid_for(n, 256), 256, 0, n, c->p2d.ncols, d_cells, ld, tb, dt, c->d_k,
-                d_cells_out, ld_out, d_errors ? d_errors : c->d_err6, 0.0, (unsigned long long*)nullptr);
+                d_cells_out, ld_out, d_errors ? d_errors : c->d_err6, 0.0, (unsigned long long*)nullptr,
+                Grow2D{});
     });
 }
 

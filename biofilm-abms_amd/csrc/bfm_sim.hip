@@ -1289,8 +1289,8 @@ int divide_2d(const bfm_sim_config* cfg, int ncols, Mt19937& rng, double t, doub
     return ntries;
 }
 
-// growCells of the 2-D tree (growth.hpp:51-57 of ABM-2d) when grow != 0,
-// then divideMaxLength's count (l > Ldiv) into *count.
+// growCells of the 2-D tree (growth.hpp:51-57 of ABM-2d) for an accepted
+// step that was not the controller's re-run, and divideMaxLength's count.
 __global__ void __launch_bounds__(256)
 k2d_grow_count(double* __restrict__ cells, int ld, int n, double R, double dt, double Ldiv, int grow,
                int* __restrict__ count)
@@ -1311,11 +1311,31 @@ k2d_grow_count(double* __restrict__ cells, int ld, int n, double R, double dt, d
     if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(count, (int)__popcll(ballot));
 }
 
+__global__ void k_key_init(unsigned long long* key) { *key = 0x000FFFFFFFFFFFFFull; }   // ordered key of -inf
+
+// Host memory the device writes the per-attempt readback into.
+struct MappedBuf {
+    void* h = nullptr;
+    void* d = nullptr;
+    explicit MappedBuf(size_t bytes)
+    {
+        SCHK(hipHostMalloc(&h, bytes, hipHostMallocMapped));
+        std::memset(h, 0, bytes);
+        SCHK(hipHostGetDevicePointer(&d, h, 0));
+    }
+    ~MappedBuf() { if (h) (void)hipHostFree(h); }
+};
+
 // runSimulationAdaptiveLagrangian's loop of the 2-D tree (simulation.hpp:
 // 549-765 of ABM-2d) with the cells on the device (leading dimension cap).
-// Per iteration the host reads the division count and the first attempt's
-// scaled error; a division copies the population to the host, divides it
-// there with the reference's draws and copies it back.
+// Each iteration enqueues its first RK attempt right after the previous
+// iteration's work and synchronises once: a small kernel publishes the
+// attempt's scaled error, the division count of the current cells (counted
+// when they grew) and the status word into host-mapped memory. The rare
+// iteration that divides discards that attempt: the population goes to the
+// host, divides there with the reference's draws, comes back, and the list
+// is rebuilt before the attempt is taken again. The controller's re-run also
+// grows the cells and counts the next divisions.
 void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, const double* cells_in, int n_in,
                 const int* parents_in, int n_parents_in, double* cells_out, int cap, int* n_out, int* parents_out,
                 int* n_parents_out, double* dt_trace, int cap_trace, bfm_sim_stats* stats)
@@ -1340,13 +1360,12 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
     const auto wall0 = std::chrono::steady_clock::now();
     auto wall = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count(); };
     hipStream_t st = c->stream;
-    DevBuf<double> bufA((size_t)cap * ncols), bufB((size_t)cap * ncols);
+    DevBuf<double> bufA((size_t)cap * ncols), bufB((size_t)cap * ncols), d_sums((size_t)cap * 8);
     DevBuf<unsigned long long> d_err(1);
     DevBuf<int> d_count(1);
-    PinnedBuf pin(64), host(sizeof(double) * (size_t)cap * ncols);
-    unsigned long long* h_err = (unsigned long long*)pin.p;
-    int* h_count = (int*)((char*)pin.p + 16);
-    int* h_status = (int*)((char*)pin.p + 32);
+    MappedBuf pub(64);
+    volatile unsigned long long* h_pub = (volatile unsigned long long*)pub.h;
+    PinnedBuf host(sizeof(double) * (size_t)cap * ncols);
     double* hc = (double*)host.p;
     SCHK(hipMemcpy2DAsync(bufA.p, sizeof(double) * cap, cells_in, sizeof(double) * n_in, sizeof(double) * n_in,
                           ncols, hipMemcpyHostToDevice, st));
@@ -1365,44 +1384,41 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
     int64_t iter = 0, attempts = 0, divisions = 0, builds = 0;
     double t = 0, dt = cfg->max_stepsize, host_div_s = 0, next_report = cfg->report_seconds;
     const double thr = cfg->neighbor_threshold, R = cfg->R, Ldiv = cfg->Ldiv, tol = cfg->max_error_allowed;
-    auto build = [&] {
+    unsigned long long* d_pub = (unsigned long long*)pub.d;
+    auto build = [&] {   // list, to_adhere and pair records (a synchronising build)
         int64_t m = 0;
         abi(c, bfm2d_dev_build_neighbors(c, cur, n, cap, thr, R, Ldiv, amap, ng, &m));
+        abi(c, step2d_prepare(c, n, d_sums.p));
         ++builds;
     };
-    auto count_dividing = [&](int grow, double h) {
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
-        hipLaunchKernelGGL(k2d_grow_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, cur, cap, n, R, h, Ldiv,
-                           grow, d_count.p);
-        SCHK(hipGetLastError());
-        SCHK(hipMemcpyAsync(h_count, d_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    auto enqueue_attempt = [&](double h) {
+        ++attempts;
+        abi(c, step2d_attempt(c, tb, h, cur, n, cap, d_sums.p, nxt, cap, tol, d_err.p, nullptr, R, Ldiv));
     };
-    auto status_check = [&] {
-        if (*h_status) {
+    // publishes and waits: returns the attempt's max(scaled error, min_error)
+    auto sync_read = [&]() {
+        abi(c, step2d_publish(c, d_err.p, d_count.p, d_pub));
+        SCHK(hipStreamSynchronize(st));
+        const int status = (int)h_pub[2];
+        if (status) {
             const int zero = 0;
             SCHK(hipMemcpy(c->d_status, &zero, sizeof(int), hipMemcpyHostToDevice));
-            const int s0 = *h_status;
-            *h_status = 0;
-            if (s0 & ST_NAN) sim_fail(BFM_ERR_NAN, "non-finite stage velocity in the 2-D growth loop");
+            if (status & ST_NAN) sim_fail(BFM_ERR_NAN, "non-finite stage velocity in the 2-D growth loop");
             sim_fail(BFM_ERR_HIP, "device status error in the 2-D growth loop");
         }
+        return std::max(order_key_value(h_pub[0]), cfg->min_error);
     };
-    // one RK attempt; with want_error the scaled-error maximum is read back
-    auto attempt = [&](double h, bool want_error) {
-        ++attempts;
-        abi(c, step2d_attempt(c, tb, h, cur, n, cap, nxt, cap, want_error ? tol : 0.0, d_err.p));
-        if (!want_error) return 0.0;
-        SCHK(hipMemcpyAsync(h_err, d_err.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        SCHK(hipMemcpyAsync(h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, st));
-        SCHK(hipStreamSynchronize(st));
-        status_check();
-        return std::max(order_key_value(*h_err), cfg->min_error);
-    };
-    *h_status = 0;
+    hipLaunchKernelGGL(k_key_init, dim3(1), dim3(1), 0, st, d_err.p);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
+    hipLaunchKernelGGL(k2d_grow_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, cur, cap, n, R, 0.0, Ldiv, 0,
+                       d_count.p);
+    SCHK(hipGetLastError());
     build();
-    count_dividing(0, 0.0);
     for (;;) {
-        SCHK(hipStreamSynchronize(st));
+        // the iteration's first attempt, taken before the division decision is known
+        enqueue_attempt(dt);
+        double max_error = sync_read();
+        const int nd = (int)h_pub[1];
         if (terminate_now(cfg, n, iter, t)) break;
         if (cfg->max_wall_seconds > 0 && wall() > cfg->max_wall_seconds) break;
         if (cfg->report_seconds > 0 && wall() >= next_report) {
@@ -1410,8 +1426,7 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
                          dt, wall());
             next_report += cfg->report_seconds;
         }
-        const int nd = *h_count;
-        if (nd > 0) {
+        if (nd > 0) {   // divide, rebuild, take the first attempt again
             const auto h0 = std::chrono::steady_clock::now();
             SCHK(hipMemcpy2DAsync(hc, sizeof(double) * cap, cur, sizeof(double) * cap, sizeof(double) * n, ncols,
                                   hipMemcpyDeviceToHost, st));
@@ -1421,12 +1436,11 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
             divisions += n - n0;
             SCHK(hipMemcpy2DAsync(cur, sizeof(double) * cap, hc, sizeof(double) * cap, sizeof(double) * n, ncols,
                                   hipMemcpyHostToDevice, st));
-            SCHK(hipStreamSynchronize(st));
             host_div_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
             build();
+            enqueue_attempt(dt);
+            max_error = sync_read();
         }
-        abi(c, step2d_prepare(c, n));
-        double max_error = attempt(dt, true);
         if (iter % cfg->iter_update_stepsize == 0) {   // simulation.hpp:639-712 of ABM-2d
             const double inv = 1.0 / (tdef.order + 1);
             bool exceeded = max_error > 1.0;
@@ -1435,7 +1449,8 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
             else if (factor < 0.2) factor = 0.2;
             int j = 0;
             while (exceeded && j < cfg->max_tries_update_stepsize) {
-                max_error = attempt(dt * factor, true);
+                enqueue_attempt(dt * factor);
+                max_error = sync_read();
                 exceeded = max_error > 1.0;
                 factor *= 0.9 * pow(1.0 / max_error, inv);
                 if (factor >= 10) { factor = 10; break; }
@@ -1445,21 +1460,25 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
             if (dt * factor < cfg->min_stepsize) factor = cfg->min_stepsize / dt;
             else if (dt * factor > cfg->max_stepsize) factor = cfg->max_stepsize / dt;
             dt *= factor;
-            attempt(dt, false);
+            // the re-run, with growCells and the next division count fused in
+            ++attempts;
+            abi(c, step2d_attempt(c, tb, dt, cur, n, cap, d_sums.p, nxt, cap, 0.0, d_err.p, d_count.p, R, Ldiv));
+            std::swap(cur, nxt);
+        } else {
+            std::swap(cur, nxt);
+            hipLaunchKernelGGL(k2d_grow_count, dim3(blocks_for(n, 256)), dim3(256), 0, st, cur, cap, n, R, dt, Ldiv,
+                               1, d_count.p);
+            SCHK(hipGetLastError());
         }
-        std::swap(cur, nxt);
-        count_dividing(1, dt);
-        abi(c, bfm2d_dev_update_neighbor_distances(c, cur, n, cap));
+        abi(c, step2d_rows_update(c, cur, n, cap, d_sums.p));
         if (dt_trace && iter < cap_trace) dt_trace[iter] = dt;
         t += dt;
         iter++;
         if (iter % cfg->iter_update_neighbors == 0) build();
     }
-    SCHK(hipMemcpyAsync(h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, st));
     SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, cur, sizeof(double) * cap, sizeof(double) * n, ncols,
                           hipMemcpyDeviceToHost, st));
     SCHK(hipStreamSynchronize(st));
-    status_check();
     *n_out = n;
     if (parents_out) std::copy(parents.begin(), parents.end(), parents_out);
     if (n_parents_out) *n_parents_out = (int)parents.size();
