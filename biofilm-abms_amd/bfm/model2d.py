@@ -248,11 +248,14 @@ def founder_cell_2d(L0=0.4, growth_mean=1.5, eta_ambient=0.7, eta_surface=720.0,
     return c
 
 
-def simulate_2d(model, cfg, cells, adhesion_map=((0, 0), (0, 1)), cap=None, trace_cap=2_000_000):
+def simulate_2d(model, cfg, cells, adhesion_map=((0, 0), (0, 1)), cap=None, trace_cap=2_000_000,
+                allow_nan=False):
     """bfm2d_simulate on the model's context from `cells` (n x ncols, the
     2-D columns); adhesion_map: n_groups x n_groups 0/1 (groups are column
     16; the default lets group-1 cells adhere to each other, as the 2-D
-    drivers do) or None. Returns (cells, parents, dt_trace, stats dict)."""
+    drivers do) or None. Returns (cells, parents, dt_trace, stats dict).
+    allow_nan: a run stopped by a non-finite stage velocity returns its
+    last state with stats["nan"] = True instead of raising."""
     from .sim import SimStats
     lib = _lib()
     cells = _f64(cells)
@@ -264,12 +267,15 @@ def simulate_2d(model, cfg, cells, adhesion_map=((0, 0), (0, 1)), cap=None, trac
     n_out, n_par = C.c_int(), C.c_int()
     trace = np.zeros(trace_cap)
     st = SimStats()
-    model.ctx._check(lib.bfm2d_simulate(
+    rc = lib.bfm2d_simulate(
         model.ctx.h, C.cast(C.byref(cfg), C.c_void_p), None if amap is None else amap.ctypes.data_as(_ip),
         0 if amap is None else amap.shape[0], _ptr(cells), n_in, None, 0, _ptr(out), cap, C.byref(n_out),
-        parents.ctypes.data_as(_ip), C.byref(n_par), _ptr(trace), trace_cap, C.cast(C.byref(st), C.c_void_p)))
+        parents.ctypes.data_as(_ip), C.byref(n_par), _ptr(trace), trace_cap, C.cast(C.byref(st), C.c_void_p))
+    if not (allow_nan and rc == 3):
+        model.ctx._check(rc)
     it = min(st.iterations, trace_cap)
     stats = {k: getattr(st, k) for k, _ in st._fields_}
+    stats["nan"] = rc == 3
     return np.asfortranarray(out[:n_out.value]), parents[:n_par.value].copy(), trace[:it].copy(), stats
 
 

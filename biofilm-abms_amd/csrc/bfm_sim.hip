@@ -1395,7 +1395,26 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
         ++attempts;
         abi(c, step2d_attempt(c, tb, h, cur, n, cap, d_sums.p, nxt, cap, tol, d_err.p, nullptr, R, Ldiv));
     };
-    // publishes and waits: returns the attempt's max(scaled error, min_error)
+    auto finish = [&] {   // outputs: the current cells, lineage and counters
+        SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, cur, sizeof(double) * cap, sizeof(double) * n, ncols,
+                              hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        *n_out = n;
+        if (parents_out) std::copy(parents.begin(), parents.end(), parents_out);
+        if (n_parents_out) *n_parents_out = (int)parents.size();
+        if (stats) {
+            stats->iterations = iter;
+            stats->rk_steps = attempts;
+            stats->divisions = divisions;
+            stats->neighbor_builds = builds;
+            stats->t_end = t;
+            stats->wall_seconds = wall();
+            stats->host_division_seconds = host_div_s;
+        }
+    };
+    // publishes and waits: returns the attempt's max(scaled error, min_error).
+    // A non-finite stage velocity ends the run with the cells the attempt
+    // started from in cells_out.
     auto sync_read = [&]() {
         abi(c, step2d_publish(c, d_err.p, d_count.p, d_pub));
         SCHK(hipStreamSynchronize(st));
@@ -1403,7 +1422,11 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
         if (status) {
             const int zero = 0;
             SCHK(hipMemcpy(c->d_status, &zero, sizeof(int), hipMemcpyHostToDevice));
-            if (status & ST_NAN) sim_fail(BFM_ERR_NAN, "non-finite stage velocity in the 2-D growth loop");
+            finish();
+            char msg[160];
+            std::snprintf(msg, sizeof(msg), "non-finite stage velocity in the 2-D growth loop (iteration %lld, "
+                          "t = %.9g h, n = %d; cells_out holds the attempt's starting state)", (long long)iter, t, n);
+            if (status & ST_NAN) sim_fail(BFM_ERR_NAN, msg);
             sim_fail(BFM_ERR_HIP, "device status error in the 2-D growth loop");
         }
         return std::max(order_key_value(h_pub[0]), cfg->min_error);
@@ -1476,21 +1499,7 @@ void simulate2d(bfm_ctx* c, const bfm_sim_config* cfg, const int* amap, int ng, 
         iter++;
         if (iter % cfg->iter_update_neighbors == 0) build();
     }
-    SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, cur, sizeof(double) * cap, sizeof(double) * n, ncols,
-                          hipMemcpyDeviceToHost, st));
-    SCHK(hipStreamSynchronize(st));
-    *n_out = n;
-    if (parents_out) std::copy(parents.begin(), parents.end(), parents_out);
-    if (n_parents_out) *n_parents_out = (int)parents.size();
-    if (stats) {
-        stats->iterations = iter;
-        stats->rk_steps = attempts;
-        stats->divisions = divisions;
-        stats->neighbor_builds = builds;
-        stats->t_end = t;
-        stats->wall_seconds = wall();
-        stats->host_division_seconds = host_div_s;
-    }
+    finish();
 }
 
 }  // namespace

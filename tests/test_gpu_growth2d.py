@@ -102,3 +102,23 @@ def test_stop_rules_and_errors(orc):
     with pytest.raises(BFMError) as e:
         simulate_2d(bare, sim_config_2d(max_iter=5), founder_cell_2d(), AMAP, cap=16)
     assert e.value.status == 1
+
+
+def test_zero_length_daughter_stops_with_nan():
+    """divideCells clips the length ratio to [0, 1] (simulation.hpp:302-315 of
+    ABM-2d), so a daughter can have length 0, where the composite drag l eta
+    vanishes and the reference's velocities of that cell are not finite. A
+    wide length-ratio distribution makes that happen at the first division:
+    the loop stops with BFM_ERR_NAN and hands back the state the failing
+    attempt started from."""
+    from bfm.core import BFMError
+    from bfm.model2d import founder_cell_2d, sim_config_2d, simulate_2d
+    print("2-D growth zero-length daughter", flush=True)
+    cfg = sim_config_2d(n_cells=8, daughter_length_std=50.0, seed=2)
+    m2 = _model(16)
+    with pytest.raises(BFMError) as e:
+        simulate_2d(m2, cfg, founder_cell_2d(), AMAP, cap=16)
+    assert e.value.status == 3
+    cells, _, _, st = simulate_2d(m2, cfg, founder_cell_2d(), AMAP, cap=16, allow_nan=True)
+    assert st["nan"] and cells.shape[0] == 2 and np.all(np.isfinite(cells))
+    assert np.min(cells[:, 9]) == 0.0
