@@ -127,6 +127,11 @@ void print_layout()
     F(bfm_sim_config, iter_update_stepsize); F(bfm_sim_config, max_tries_update_stepsize); F(bfm_sim_config, n_cells);
     F(bfm_sim_config, max_iter); F(bfm_sim_config, tableau); F(bfm_sim_config, seed);
     F(bfm_sim_config, report_seconds); F(bfm_sim_config, max_wall_seconds); F(bfm_sim_config, rho);
+    std::printf("\"sizeof(bfm2d_params)\": %zu, ", sizeof(bfm2d_params));
+    F(bfm2d_params, R); F(bfm2d_params, Rcell); F(bfm2d_params, prefactors);
+    F(bfm2d_params, surface_contact_density); F(bfm2d_params, adhesion_mode); F(bfm2d_params, ncols);
+    F(bfm2d_params, strength); F(bfm2d_params, mindist); F(bfm2d_params, distance_exp); F(bfm2d_params, E0);
+    F(bfm2d_params, gamma);
     std::printf("\"sizeof(bfm_sim_stats)\": %zu, ", sizeof(bfm_sim_stats));
     F(bfm_sim_stats, iterations); F(bfm_sim_stats, rk_steps); F(bfm_sim_stats, divisions);
     F(bfm_sim_stats, neighbor_builds); F(bfm_sim_stats, t_end); F(bfm_sim_stats, wall_seconds);

@@ -50,7 +50,7 @@ def test_dome_generator_rules():
 
 
 def test_struct_layout_matches_ctypes():
-    """sizeof / offsetof of bfm_params, bfm_sim_config and bfm_sim_stats as the
+    """sizeof / offsetof of bfm_params, bfm_sim_config, bfm_sim_stats and bfm2d_params as the
     C++ compiler lays them out (tests/abi_cpp/abi_check --layout) against the
     ctypes mirrors the Python binding passes across the ABI."""
     import ctypes as C
@@ -58,10 +58,12 @@ def test_struct_layout_matches_ctypes():
     import subprocess
 
     from bfm.core import Params
+    from bfm.model2d import Params2D
     from bfm.sim import SimConfig, SimStats
     exe = os.path.join(ROOT, "tests", "abi_cpp", "abi_check")
     out = json.loads(subprocess.run([exe, "--layout"], capture_output=True, text=True, check=True).stdout)
-    for S, name in ((Params, "bfm_params"), (SimConfig, "bfm_sim_config"), (SimStats, "bfm_sim_stats")):
+    for S, name in ((Params, "bfm_params"), (SimConfig, "bfm_sim_config"), (SimStats, "bfm_sim_stats"),
+                    (Params2D, "bfm2d_params")):
         assert out[f"sizeof({name})"] == C.sizeof(S), name
         for field, _ in S._fields_:
             assert out[f"{name}.{field}"] == getattr(S, field).offset, (name, field)
