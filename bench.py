@@ -381,7 +381,8 @@ def main():
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "learned_jkr": learned,
         "stage_hbm_frac": (stage_bytes / (ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
-        "jkr_path_us": round(1e3 * (per_kernel.get("pair_forces", 0) + per_kernel.get("sur_radius", 0)), 2),
+        "jkr_path_us": round(1e3 * (per_kernel.get("pair_forces", 0) + per_kernel.get("sur_stage", 0)
+                                    + per_kernel.get("sur_network", 0)), 2),
         "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(per_kernel.items(),
                                                               key=lambda x: -x[1])},
         "wall_s_timed": wall,

@@ -1208,6 +1208,7 @@ struct ForceParams {
     int jkr_mode;
     int colidx_gamma;
     int pad;
+    SurrogateEnvelope env;   // rows inside it take the learned radius (jkr_mode surrogate)
 };
 
 // Per-pair force magnitudes (the scalar factors of cellCellInteractionForces,
@@ -1296,7 +1297,7 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
                         radius = table_radius_1d(tab_ov, tab_r, tab_nd, delta);
                     else if (fp.jkr_mode == BFM_JKR_TABLE_2D)
                         radius = table_radius_2d(tab_ov, tab_nd, tab_g, tab_ng, tab_r, delta, gamma);
-                    else if (fp.jkr_mode == BFM_JKR_SURROGATE && sur_radius)
+                    else if (fp.jkr_mode == BFM_JKR_SURROGATE && sur_radius && sur_inside(delta, gamma, fp.env))
                         radius = sur_radius[k];
                     else
                         radius = jkr_radius_newton(delta, fp.Req, fp.E0, gamma, 0.0, 200, nullptr);
@@ -2624,10 +2625,16 @@ ForceParams force_params_of(const bfm_params& p)
     fp.jkr_mode = p.jkr_mode;
     fp.colidx_gamma = p.colidx_gamma;
     fp.pad = 0;
+    fp.env = SurrogateEnvelope{1, 0, 1, 0, 1};   // contains nothing
     return fp;
 }
 
-ForceParams force_params(const bfm_ctx* c) { return force_params_of(c->p); }
+ForceParams force_params(const bfm_ctx* c)
+{
+    ForceParams fp = force_params_of(c->p);
+    fp.env = bfm::surrogate_envelope(c);
+    return fp;
+}
 
 AnisoJKR aniso_params(const bfm_ctx* c)
 {

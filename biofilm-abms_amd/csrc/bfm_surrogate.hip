@@ -7,7 +7,8 @@
 // function of the inputs. The network maps the configured inputs
 // (normalised overlap delta / R, both cell radii, surface energy) through
 // 4 -> 64 -> 64 -> 64 -> 1 (SiLU) to t. Weights and activations are BF16, the
-// three 64-wide layers run on MFMA (v_mfma_f32_16x16x16_bf16) with FP32
+// three 64-wide layers run on MFMA (v_mfma_f32_16x16x16_bf16 for the first,
+// gfx950's v_mfma_f32_16x16x32_bf16 for the two 64 x 64 layers) with FP32
 // accumulation; the last layer is an FP32 dot product. The guess
 // u = t p^(1/3) is then polished by FP64 Newton steps on u^4 - p u - q.
 // Pairs outside the training envelope use the exact FP64 solve.
@@ -47,13 +48,9 @@ constexpr int kW4 = kB3 + kH;             // [64]
 constexpr int kB4 = kW4 + kH;             // [1]
 constexpr int kNParams = kB4 + 1;
 
-// Feature normalisation of the training envelope (config: overlap in
-// [-0.2 R, 0.3 R], gamma in [10, 1000] Pa um, R1 = R2 = 0.8 um).
-struct Envelope {
-    double dlo, dhi;     // delta / R range
-    double glo, ghi;     // gamma range (Pa um)
-    double R;            // reference radius
-};
+// Feature normalisation of the training envelope (overlap in [-0.2 R, 0.3 R],
+// gamma in [10, 1000] Pa um, R1 = R2 = 0.8 um).
+using Envelope = SurrogateEnvelope;
 
 struct SurrogateWeights {
     // device: BF16 weights (row-major [out][in]; W1 padded to [64][16]) and FP32 biases
@@ -63,6 +60,10 @@ struct SurrogateWeights {
     uint16_t* w4 = nullptr;   // 64
     float* b = nullptr;       // b1(64) b2(64) b3(64) b4(1)
     double* radius = nullptr; // per-pair output, max_pairs
+    int* st_row = nullptr;    // staged rows inside the envelope (max_pairs) ...
+    double* st_delta = nullptr;
+    double* st_gamma = nullptr;
+    int* st_count = nullptr;  // ... and each staging block's count
     int64_t cap = 0;
     Envelope env{};
     int polish = 1;
@@ -112,10 +113,9 @@ __host__ __device__ inline void features(double delta, double r1, double r2, dou
 // lane l holds W[16 t + l % 16][16 s + 4 (l / 16) + i], i = 0..3.
 __device__ inline bf16x4 load_bfrag(const uint16_t* __restrict__ W, int K, int s, int t, int lane)
 {
+    // four consecutive BF16 (K is a multiple of 4, so the 8-byte load is aligned)
     const uint16_t* p = W + (16 * t + (lane & 15)) * K + 16 * s + 4 * (lane >> 4);
-    bf16x4 v;
-    v[0] = (short)p[0]; v[1] = (short)p[1]; v[2] = (short)p[2]; v[3] = (short)p[3];
-    return v;
+    return *reinterpret_cast<const bf16x4*>(p);
 }
 
 struct WFrags {
@@ -293,12 +293,33 @@ __device__ inline uint32_t pack_bf16x2(float a, float b)
 // SiLU with the hardware exp2 and reciprocal (1 ulp) instead of an IEEE divide.
 __device__ inline float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
-__device__ inline float mlp_forward64(const bf16x4 (*wl)[64], const float* bs, const float x[4], int lane)
+typedef short short8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+
+// One K = 32 step of gfx950's v_mfma_f32_16x16x32_bf16 from two K = 16
+// fragments. Its operand lane l holds K positions 8 (l / 16) .. + 7; the two
+// 16-wide fragments give lane l the K indices 4 (l / 16) + i of k-step 2 s'
+// and of 2 s' + 1, so position 8 (l / 16) + j stands for input
+// 32 s' + (j < 4 ? 4 (l / 16) + j : 16 + 4 (l / 16) + j - 4). The weight and
+// activation operands use the same order, so the product is the sum of the
+// two K = 16 products.
+__device__ inline f32x4 mfma_k32(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c)
+{
+    const short8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    const short8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, b),
+                                                   c, 0, 0, 0);
+}
+
+// Forward of up to 64 samples (ntiles 16-sample tiles); x: the four features
+// of sample `lane`. Returns y of sample `lane` (0 for lanes past the tiles).
+__device__ inline float mlp_forward64(const bf16x4 (*wl)[64], const float* bs, const float x[4], int lane,
+                                      int ntiles = 4)
 {
     const int col = lane & 15, g = lane >> 4;
     float y_mine = 0.f;
 #pragma unroll 1
-    for (int T = 0; T < 4; ++T) {
+    for (int T = 0; T < ntiles; ++T) {
         // the weight fragments are re-read from LDS for every tile rather than
         // held in 72 registers across the loop
         asm volatile("" ::: "memory");
@@ -325,14 +346,15 @@ __device__ inline float mlp_forward64(const bf16x4 (*wl)[64], const float* bs, c
                 h[t] = __builtin_bit_cast(bf16x4, hw[t]);
             }
             if (layer == 2) break;
-            // the accumulator starts at the next layer's bias
+            // the accumulator starts at the next layer's bias; two K = 32 steps
             const int fb = layer == 0 ? 4 : 20;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 f32x4 z = *(const f32x4*)(bs + 64 * (layer + 1) + 16 * t + 4 * g);
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[fb + 4 * s + t][lane], h[s], z, 0, 0, 0);
+                for (int s2 = 0; s2 < 2; ++s2)
+                    z = mfma_k32(wl[fb + 4 * (2 * s2) + t][lane], wl[fb + 4 * (2 * s2 + 1) + t][lane], h[2 * s2],
+                                 h[2 * s2 + 1], z);
                 acc[t] = z;
             }
         }
@@ -378,100 +400,133 @@ __device__ inline void stage_bias(float* bs, const float* __restrict__ b, const 
     __syncthreads();
 }
 
-// Per-row contact radius for the JKR pairs of the neighbor list: rows with
-// overlap > 0 and gamma > 0 inside the envelope take the network's guess and
-// the polish; other JKR rows take the exact solve at once. Each wavefront
-// reads 64 rows per iteration and appends the rows inside the envelope to a
-// wavefront-private LDS queue; the network runs on full 64-row queue slices
-// (and once on the remainder at the end), so its work follows the rows inside
-// the envelope (about a quarter of the rows of a dense colony) rather than
-// every block of 64 rows that holds one.
+// Staging blocks of the learned radius (k_sur_stage's grid), samples per
+// wavefront pass of the network (four 16-sample tiles) and the network
+// kernel's grid (32-sample passes on twice the wavefronts measured slower).
+constexpr int kSurStageBlocks = 1024;
+constexpr int kSurSlice = 64;
+constexpr int kSurNetBlocks = 640;
+
+// First pass of the learned radius: block b owns the rows [b C, (b + 1) C),
+// C = ceil(m / kSurStageBlocks), and copies those that overlap, adhere
+// (gamma = min of the two cells' > 0) and lie inside the envelope into its
+// staging slots [b C, b C + count) (row, capped overlap, gamma), in any order;
+// counts[b] = count. The test is the pair force kernel's, expression for
+// expression, so both kernels agree on which rows the network serves.
 __global__ void __launch_bounds__(256)
-k_sur_radius(const int* __restrict__ d_m, const double* __restrict__ cells, int ld, const int* __restrict__ pi,
-             const int* __restrict__ pj, const double* __restrict__ pdx, const double* __restrict__ pdy,
-             const double* __restrict__ pdz, RadiusArgs ra, Envelope env, const uint16_t* __restrict__ w1,
-             const uint16_t* __restrict__ w2, const uint16_t* __restrict__ w3, const uint16_t* __restrict__ w4,
-             const float* __restrict__ bias, double* __restrict__ radius)
+k_sur_stage(const int* __restrict__ d_m, const double* __restrict__ cells, int ld, const int* __restrict__ pi,
+            const int* __restrict__ pj, const double* __restrict__ pdx, const double* __restrict__ pdy,
+            const double* __restrict__ pdz, RadiusArgs ra, Envelope env, int* __restrict__ st_row,
+            double* __restrict__ st_delta, double* __restrict__ st_gamma, int* __restrict__ counts)
+{
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int m = *d_m;
+    const int C = (m + kSurStageBlocks - 1) / kSurStageBlocks;
+    const int lo = blockIdx.x * C, hi = min(m, lo + C);
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int base = lo; base < hi; base += blockDim.x) {
+        const int k = base + threadIdx.x;
+        bool inside = false;
+        double delta = 0, gamma = 0;
+        // rows of a build that overflowed its capacity (reported by status) may be stale
+        if (k < hi && (unsigned)pi[k] < (unsigned)ld && (unsigned)pj[k] < (unsigned)ld) {
+            const double dist = norm3(pdx[k], pdy[k], pdz[k]);
+            const double overlap = 2 * ra.R - dist;
+            if (overlap > 0) {
+                gamma = dmin(cells[(int64_t)ra.colidx_gamma * ld + pi[k]],
+                             cells[(int64_t)ra.colidx_gamma * ld + pj[k]]);
+                delta = overlap;
+                if (ra.max_overlap >= 0 && delta > ra.max_overlap) delta = ra.max_overlap;
+                inside = gamma > 0 && sur_inside(delta, gamma, env);
+            }
+        }
+        const uint64_t mk = __ballot(inside);
+        int wbase = 0;
+        if (lane == 0 && mk) wbase = atomicAdd(&cnt, (int)__popcll(mk));
+        wbase = __shfl(wbase, 0);
+        if (inside) {
+            const int pos = lo + wbase + (int)__popcll(mk & below);
+            st_row[pos] = k;
+            st_delta[pos] = delta;
+            st_gamma[pos] = gamma;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = cnt;
+}
+
+// Second pass: the network on the staged rows as consecutive kSurSlice-sample
+// slices, one per wavefront at a time (the last slice runs only its filled
+// 16-sample tiles), then the FP64 polish; radius[row] for each staged row.
+// Sample g of the concatenated staging lists is found from the blocks'
+// exclusive prefix counts by a binary search in LDS.
+__global__ void __launch_bounds__(256)
+k_sur_network(const int* __restrict__ d_m, const int* __restrict__ counts, const int* __restrict__ st_row,
+              const double* __restrict__ st_delta, const double* __restrict__ st_gamma, RadiusArgs ra, Envelope env,
+              const uint16_t* __restrict__ w1, const uint16_t* __restrict__ w2, const uint16_t* __restrict__ w3,
+              const uint16_t* __restrict__ w4, const float* __restrict__ bias, double* __restrict__ radius)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __shared__ __align__(16) float bs[260];
     __shared__ bf16x4 wl[36][64];
-    __shared__ int q_row[4][128];
-    __shared__ double q_delta[4][128], q_gamma[4][128];
+    __shared__ int pre[kSurStageBlocks + 1];
+    __shared__ int wsum[4];
+    // exclusive prefix of the counts: four per thread, then wavefront and block sums
+    constexpr int per = kSurStageBlocks / 256;
+    int v[per], tsum = 0;
+#pragma unroll
+    for (int i = 0; i < per; ++i) {
+        v[i] = counts[threadIdx.x * per + i];
+        tsum += v[i];
+    }
+    int incl = tsum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
     stage_frags(wl, w1, w2, w3);
-    stage_bias(bs, bias, w4);
+    stage_bias(bs, bias, w4);   // ends with a barrier
+    int wofs = 0;
+    for (int w = 0; w < wave; ++w) wofs += wsum[w];
+    int run = wofs + incl - tsum;
+#pragma unroll
+    for (int i = 0; i < per; ++i) {
+        pre[threadIdx.x * per + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == 255) pre[kSurStageBlocks] = run;
+    __syncthreads();
+    const int total = pre[kSurStageBlocks];
     const int m = *d_m;
-    const int waves_total = gridDim.x * 4;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int qn = 0;   // queue fill, uniform across the wavefront
-    // network + polish for queue entries [0, cnt), cnt <= 64
-    auto run_queue = [&](int cnt) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int C = (m + kSurStageBlocks - 1) / kSurStageBlocks;
+    const int nslices = (total + kSurSlice - 1) / kSurSlice;
+    for (int sl = blockIdx.x * 4 + wave; sl < nslices; sl += gridDim.x * 4) {
+        const int cnt = min(kSurSlice, total - kSurSlice * sl);
+        const int g = kSurSlice * sl + lane;
         const bool on = lane < cnt;
         int kq = 0;
         double dq = 0, gq = 0;
         float x[4] = {0.f, 0.f, 0.f, 0.f};
         if (on) {
-            kq = q_row[wave][lane];
-            dq = q_delta[wave][lane];
-            gq = q_gamma[wave][lane];
+            int lo = 0, hi = kSurStageBlocks - 1;   // the largest b with pre[b] <= g
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pre[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const int idx = lo * C + (g - pre[lo]);
+            kq = st_row[idx];
+            dq = st_delta[idx];
+            gq = st_gamma[idx];
             features_f32(dq, ra.R, gq, env, x);
         }
-        const float y = mlp_forward64(wl, bs, x, lane);
+        const float y = mlp_forward64(wl, bs, x, lane, (cnt + 15) >> 4);
         if (on) radius[kq] = polish_radius_fast(y, dq, ra.Req, ra.E0, gq, ra.polish);
-    };
-    for (int base = (blockIdx.x * 4 + wave) * 64; base < m; base += waves_total * 64) {
-        const int k = base + lane;
-        // rows of a build that overflowed its capacity (reported by status) may be stale
-        bool valid = k < m && (unsigned)pi[k] < (unsigned)ld && (unsigned)pj[k] < (unsigned)ld, inside = false;
-        double delta = 0, gamma = 0;
-        if (valid) {
-            const double dist = norm3(pdx[k], pdy[k], pdz[k]);
-            const double overlap = 2 * ra.R - dist;
-            gamma = dmin(cells[(int64_t)ra.colidx_gamma * ld + pi[k]], cells[(int64_t)ra.colidx_gamma * ld + pj[k]]);
-            valid = overlap > 0 && gamma > 0;
-            delta = overlap;
-            if (ra.max_overlap >= 0 && delta > ra.max_overlap) delta = ra.max_overlap;
-            inside = valid && delta >= env.dlo * env.R && delta <= env.dhi * env.R && gamma >= env.glo &&
-                     gamma <= env.ghi;
-        }
-        if (valid && !inside) radius[k] = jkr_radius_newton(delta, ra.Req, ra.E0, gamma, 0.0, 200, nullptr);
-        const uint64_t mk = __ballot(inside);
-        if (inside) {
-            const int pos = qn + __popcll(mk & below);
-            q_row[wave][pos] = k;
-            q_delta[wave][pos] = delta;
-            q_gamma[wave][pos] = gamma;
-        }
-        qn += __popcll(mk);
-        if (qn >= 64) {
-            run_queue(64);
-            // move the overflow (< 64 entries) to the front
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int rest = qn - 64;
-            int r0 = 0;
-            double r1 = 0, r2 = 0;
-            if (lane < rest) {
-                r0 = q_row[wave][64 + lane];
-                r1 = q_delta[wave][64 + lane];
-                r2 = q_gamma[wave][64 + lane];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane < rest) {
-                q_row[wave][lane] = r0;
-                q_delta[wave][lane] = r1;
-                q_gamma[wave][lane] = r2;
-            }
-            qn = rest;
-        }
     }
-    if (qn > 0) run_queue(qn);
 }
 
 // Network evaluation for given (delta, gamma) samples (tests and reports):
@@ -752,8 +807,14 @@ const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, in
     (void)n;
     SurrogateWeights* s = c->sur;
     if (s->cap < c->max_pairs) {
-        if (s->radius) SHIP(hipFree(s->radius));
+        for (void* q : {(void*)s->radius, (void*)s->st_row, (void*)s->st_delta, (void*)s->st_gamma,
+                        (void*)s->st_count})
+            if (q) SHIP(hipFree(q));
         dmal(&s->radius, c->max_pairs);
+        dmal(&s->st_row, c->max_pairs);
+        dmal(&s->st_delta, c->max_pairs);
+        dmal(&s->st_gamma, c->max_pairs);
+        dmal(&s->st_count, kSurStageBlocks);
         s->cap = c->max_pairs;
     }
     RadiusArgs ra;
@@ -761,18 +822,32 @@ const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, in
     ra.max_overlap = 2 * (c->p.R - c->p.Rcell);
     ra.colidx_gamma = c->p.colidx_gamma;
     ra.polish = s->polish;
-    bfm::launch_timed(c, "sur_radius", [&] {
-        hipLaunchKernelGGL(k_sur_radius, dim3(1024), dim3(256), 0, c->stream, c->d_m, d_cells, ld, c->pi, c->pj,
-                           c->pdx, c->pdy, c->pdz, ra, s->env, s->w1, s->w2, s->w3, s->w4, s->b, s->radius);
+    bfm::launch_timed(c, "sur_stage", [&] {
+        hipLaunchKernelGGL(k_sur_stage, dim3(kSurStageBlocks), dim3(256), 0, c->stream, c->d_m, d_cells, ld, c->pi,
+                           c->pj, c->pdx, c->pdy, c->pdz, ra, s->env, s->st_row, s->st_delta, s->st_gamma,
+                           s->st_count);
+    });
+    bfm::launch_timed(c, "sur_network", [&] {
+        hipLaunchKernelGGL(k_sur_network, dim3(kSurNetBlocks), dim3(256), 0, c->stream, c->d_m, s->st_count, s->st_row,
+                           s->st_delta, s->st_gamma, ra, s->env, s->w1, s->w2, s->w3, s->w4, s->b, s->radius);
     });
     return s->radius;
+}
+
+SurrogateEnvelope surrogate_envelope(const bfm_ctx* c)
+{
+    if (c->sur) return c->sur->env;
+    SurrogateEnvelope e;
+    e.dlo = 1; e.dhi = 0; e.glo = 1; e.ghi = 0; e.R = 1;   // contains nothing
+    return e;
 }
 
 void surrogate_free(bfm_ctx* c)
 {
     if (!c->sur) return;
     for (void* q : {(void*)c->sur->w1, (void*)c->sur->w2, (void*)c->sur->w3, (void*)c->sur->w4,
-                    (void*)c->sur->b, (void*)c->sur->radius})
+                    (void*)c->sur->b, (void*)c->sur->radius, (void*)c->sur->st_row, (void*)c->sur->st_delta,
+                    (void*)c->sur->st_gamma, (void*)c->sur->st_count})
         if (q) (void)hipFree(q);
     delete c->sur;
     c->sur = nullptr;
