@@ -101,6 +101,9 @@ struct bfm_ctx {
     int* d_bin_count = nullptr;     // max_bins + 1
     bool bins_dirty = true;         // histogram not known to be zero (see build_neighbors)
     int* d_bin_off = nullptr;       // max_bins + 1
+    int* d_tile_sum = nullptr;      // per tile of the histogram (bin-offset scan)
+    unsigned* d_scan_flag = nullptr;  // tile flags of the bin-offset scan
+    unsigned scan_epoch = 0;          // launch epoch of the bin-offset scan
     double* d_sorted = nullptr;     // 7 x max_cells SoA: x y z nx ny nz hl
     double* d_geom = nullptr;       // max_cells x 8 packed segments (distance updates)
     int* d_sid = nullptr;           // sorted -> original index

@@ -289,6 +289,80 @@ __global__ void k_keys(const double* __restrict__ cells, int n, int ld, const Pl
     }
 }
 
+// Exclusive scan of the bin histogram over the plan's nbins + 1 entries only
+// (its capacity is max_bins + 1, several times the bins a plan uses), over
+// 2048-entry tiles. Entries: plan->nbins + 1 when plan is given, else
+// `items`.
+constexpr int kScanTile = 2048;
+constexpr int kMaxScanTiles = 1 << 14;   // 33.5M entries
+
+// One pass: tile b scans its entries and publishes its aggregate; then its
+// threads read the aggregates of all earlier tiles in parallel (a thread per
+// earlier tile, waiting for its flag), and the block sums them into the
+// tile's prefix. A flag word holds epoch << 2 | 1; the epoch changes with
+// every launch, so the flags need no clearing between launches. Blocks are
+// dispatched in index order and a tile waits only for earlier ones, which
+// publish first thing; at a few hundred tiles all are resident at once and
+// the wait is about one round trip;
+// it gives up after a bounded number of polls with ST_QUEUE in the status
+// word instead of waiting forever.
+constexpr int kLookbackPolls = 1 << 16;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_scan_onepass(const Plan* __restrict__ plan, int64_t items_arg, const T* __restrict__ in, T* __restrict__ out,
+               T* __restrict__ agg_val, unsigned* __restrict__ flag, unsigned epoch, int* __restrict__ status)
+{
+    __shared__ T wtot[4];
+    __shared__ T wpre[4];
+    const int64_t items = plan ? (int64_t)plan->nbins + 1 : items_arg;
+    const int b = blockIdx.x;
+    if ((int64_t)b * kScanTile >= items) return;
+    constexpr int per = kScanTile / 256;
+    const int64_t i0 = (int64_t)b * kScanTile + per * threadIdx.x;
+    T v[per], tot = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        v[k] = i0 + k < items ? in[i0 + k] : (T)0;
+        tot += v[k];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T incl = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+        const T u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    const unsigned tag = (epoch << 2) | 1u;
+    if (threadIdx.x == 0) {
+        agg_val[b] = (wtot[0] + wtot[1]) + (wtot[2] + wtot[3]);
+        __hip_atomic_store(&flag[b], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    T pre = 0;
+    for (int p = threadIdx.x; p < b; p += 256) {
+        int polls = 0;
+        while (__hip_atomic_load(&flag[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != tag) {
+            if (++polls > kLookbackPolls) {
+                atomicOr(status, ST_QUEUE);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        pre += agg_val[p];
+    }
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
+    if (lane == 0) wpre[w] = pre;
+    __syncthreads();
+    T run = ((wpre[0] + wpre[1]) + (wpre[2] + wpre[3])) + incl - tot;
+    for (int k = 0; k < w; ++k) run += wtot[k];
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        if (i0 + k < items) out[i0 + k] = run;
+        run += v[k];
+    }
+}
+
 // Counting-sort scatter: each cell goes to its bin's offset plus its rank
 // within the bin (taken by the histogram atomic of k_keys), with its geometry. The
 // order inside a bin depends on atomic timing, but nothing downstream does:
@@ -2752,13 +2826,19 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     else
         KLAUNCH(c, "keys", k_keys, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_plan, c->d_keys, c->d_vals,
                 c->d_bin_count);
-    scan(c, c->d_bin_count, c->d_bin_off, (int64_t)c->max_bins + 1, "scan_bins");
+    {
+        const int tiles = (int)((c->max_bins + kScanTile) / kScanTile);   // tiles of max_bins + 1 entries
+        const unsigned epoch = ++c->scan_epoch & 0x3fffffffu;
+        KLAUNCH(c, "scan_bins", k_scan_onepass<int>, tiles, 256, 0, c->d_plan, (int64_t)0, c->d_bin_count,
+                c->d_bin_off, c->d_tile_sum, c->d_scan_flag, epoch, c->d_status);
+    }
     const int64_t stride = c->max_cells;
     KLAUNCH(c, "scatter", k_scatter, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals, c->d_bin_off,
             c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count);
     c->bins_dirty = false;
     launch_candidates(c, n, stride);
     {
+        // (the one-pass scan measured no faster than hipCUB on the n + 1 row counts)
         LaunchScope ls(c, "scan_counts");
         size_t tb = c->temp_bytes;
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, c->d_cnt64, c->d_off64, n + 1, c->stream));
@@ -3837,6 +3917,11 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_vals, N); dalloc(&c->d_vals_alt, N);
         dalloc(&c->d_bin_count, (size_t)c->max_bins + 1);
         dalloc(&c->d_bin_off, (size_t)c->max_bins + 1);
+        require((c->max_bins + kScanTile) / kScanTile <= kMaxScanTiles, BFM_ERR_CAPACITY,
+                "max_cells too large for the bin-offset scan");
+        dalloc(&c->d_tile_sum, (size_t)(c->max_bins + kScanTile) / kScanTile);
+        dalloc(&c->d_scan_flag, (size_t)kMaxScanTiles);
+        HIPCHK(hipMemset(c->d_scan_flag, 0, sizeof(unsigned) * kMaxScanTiles));
         dalloc(&c->d_sorted, N * 7);
         dalloc(&c->d_geom, N * 8);
         dalloc(&c->d_sid, N);
@@ -3897,7 +3982,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst, c->d_geom,
-                    c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere, c->d_amap};
+                    c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere, c->d_amap, c->d_tile_sum, c->d_scan_flag};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     bfm::surrogate_free(c);
