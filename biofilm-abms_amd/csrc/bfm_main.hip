@@ -1058,6 +1058,84 @@ k_candidates_wave(int n, const Plan* __restrict__ plan, const double* __restrict
     }
 }
 
+struct ForceParams {
+    double R, Rcell, E0, Req, max_overlap;
+    double pref0, pref1, pref2;
+    int adhesion;
+    int jkr_mode;
+    int colidx_gamma;
+    int pad;
+    SurrogateEnvelope env;   // rows inside it take the learned radius (jkr_mode surrogate)
+};
+
+// The main-term magnitude of the adhesive overload for an isotropic pair
+// (gamma = min of the two cells'): JKR term1 - term2 at the overlap capped at
+// the shell thickness, the contact radius from the exact FP64 solve, a table
+// or the learned radius (rows inside its envelope); Hertz (EPS-capped) when
+// gamma is 0.
+__device__ __forceinline__ void isotropic_magnitudes(const ForceParams& fp, double overlap, double gamma,
+                                                     const double* __restrict__ tab_ov,
+                                                     const double* __restrict__ tab_g,
+                                                     const double* __restrict__ tab_r, int tab_nd, int tab_ng,
+                                                     const double* __restrict__ sur_radius, int64_t k,
+                                                     int* __restrict__ status, double& mm, double& jkr)
+{
+    if (gamma > 0) {
+        double delta = overlap;
+        if (fp.max_overlap >= 0 && delta > fp.max_overlap) delta = fp.max_overlap;
+        double radius;
+        if (fp.jkr_mode == BFM_JKR_TABLE_1D)
+            radius = table_radius_1d(tab_ov, tab_r, tab_nd, delta);
+        else if (fp.jkr_mode == BFM_JKR_TABLE_2D)
+            radius = table_radius_2d(tab_ov, tab_nd, tab_g, tab_ng, tab_r, delta, gamma);
+        else if (fp.jkr_mode == BFM_JKR_SURROGATE && sur_radius && sur_inside(delta, gamma, fp.env))
+            radius = sur_radius[k];
+        else
+            radius = jkr_radius_newton(delta, fp.Req, fp.E0, gamma, 0.0, 200, nullptr);
+        if (!(radius > 0)) atomicOr(status, ST_JKR_ROOT);
+        mm = jkr_magnitude(radius, fp.Req, fp.E0, gamma);
+        jkr = 1.0;
+    } else {
+        if (overlap <= fp.max_overlap) mm = fp.pref0 * pow_half<3>(overlap);
+        else mm = fp.pref2;
+    }
+}
+
+// The 64-byte record of one row for an isotropic interaction ({dx, dy, dz,
+// s_i, s_j, main, hard, jkr}, as k_pair_forces writes it).
+__device__ __forceinline__ void isotropic_record(const ForceParams& fp, const double* __restrict__ cells, int ld,
+                                                 int i, int j, double dx, double dy, double dz, double si,
+                                                 double sj, const double* __restrict__ tab_ov,
+                                                 const double* __restrict__ tab_g,
+                                                 const double* __restrict__ tab_r, int tab_nd, int tab_ng,
+                                                 int* __restrict__ status, double* __restrict__ rec)
+{
+    const double dist = norm3(dx, dy, dz);
+    const double overlap = 2 * fp.R - dist;
+    double mm = 0, mh = 0, jkr = 0;
+    if (overlap > 0) {
+        if (!fp.adhesion) {
+            if (overlap <= fp.max_overlap) {
+                mm = fp.pref0 * pow_half<3>(overlap);
+            } else {
+                mm = fp.pref2;
+                mm += fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
+            }
+        } else {
+            const double gamma = dmin(cellv(cells, ld, i, fp.colidx_gamma), cellv(cells, ld, j, fp.colidx_gamma));
+            isotropic_magnitudes(fp, overlap, gamma, tab_ov, tab_g, tab_r, tab_nd, tab_ng, nullptr, 0, status, mm,
+                                 jkr);
+            if (overlap > fp.max_overlap)
+                mh = fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
+        }
+    }
+    double2* r = reinterpret_cast<double2*>(rec);
+    r[0] = make_double2(dx, dy);
+    r[1] = make_double2(dz, si);
+    r[2] = make_double2(sj, mm);
+    r[3] = make_double2(mh, jkr);
+}
+
 // Packs the slots into the reference's row order and builds the per-cell
 // force lists. Rows of cell i start at hi(off64[i]) and are sorted by
 // partner j (rank by comparison). The force list of cell c (starting at
@@ -1070,14 +1148,29 @@ k_candidates_wave(int n, const Plan* __restrict__ plan, const double* __restrict
 // width-8 shuffles, so the work per cell is O(entries) per lane and no wave
 // waits for its most connected cell's O(entries^2) loop.
 // Pass 1 (per cell i): rows, i-role entries, and the row index of each pair
-// stored at the pair's slot in j's partner list (jrow).
+// stored at the pair's slot in j's partner list (jrow). FORCES: also the
+// row's force record of an isotropic interaction (Hertz, or JKR with the
+// exact radius or a table), the record k_pair_forces would write, while the
+// row's geometry is in registers.
+struct FusedForces {
+    ForceParams fp;
+    const double* cells;
+    int ld;
+    const double* tab_ov;
+    const double* tab_g;
+    const double* tab_r;
+    int tab_nd, tab_ng;
+    double* rec;
+};
+
+template <bool FORCES>
 __global__ void __launch_bounds__(256)
 k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
                const unsigned long long* __restrict__ off64, const int* __restrict__ slot_j,
                const double* __restrict__ slot_rec, int64_t max_pairs, int* __restrict__ pi, int* __restrict__ pj,
                double* __restrict__ pdx, double* __restrict__ pdy, double* __restrict__ pdz,
                double* __restrict__ ps, double* __restrict__ pt, int* __restrict__ csr,
-               int* __restrict__ jrow, int* __restrict__ d_m, int* __restrict__ status)
+               int* __restrict__ jrow, int* __restrict__ d_m, int* __restrict__ status, FusedForces ff)
 {
     const bool fits = (int64_t)(off64[n] >> 32) <= max_pairs;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
@@ -1115,6 +1208,9 @@ k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __
             pdx[row] = r0.x; pdy[row] = r0.y; pdz[row] = r1.x;
             ps[row] = r1.y; pt[row] = r2.x;
             csr[base2 + rank] = (int)row;
+            if constexpr (FORCES)
+                isotropic_record(ff.fp, ff.cells, ff.ld, c, je, r0.x, r0.y, r1.x, r1.y, r2.x, ff.tab_ov, ff.tab_g,
+                                 ff.tab_r, ff.tab_nd, ff.tab_ng, status, ff.rec + 8 * row);
             const int s2 = (int)__double_as_longlong(r2.y);
             if (s2 < slot_cap) jrow[(int64_t)s2 * stride + je] = (int)row;
         }
@@ -1275,15 +1371,7 @@ __global__ void k_sort_segments(int n, const int* __restrict__ off, int* __restr
     }
 }
 
-struct ForceParams {
-    double R, Rcell, E0, Req, max_overlap;
-    double pref0, pref1, pref2;
-    int adhesion;
-    int jkr_mode;
-    int colidx_gamma;
-    int pad;
-    SurrogateEnvelope env;   // rows inside it take the learned radius (jkr_mode surrogate)
-};
+
 
 // Per-pair force magnitudes (the scalar factors of cellCellInteractionForces,
 // mechanics.hpp:1209-1350 and :1390-1659), written with the pair geometry as
@@ -1363,24 +1451,9 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
                                                         gamma);
                     mm = fr.x;
                     jkr = 1.0;
-                } else if (gamma > 0) {
-                    double delta = overlap;
-                    if (fp.max_overlap >= 0 && delta > fp.max_overlap) delta = fp.max_overlap;
-                    double radius;
-                    if (fp.jkr_mode == BFM_JKR_TABLE_1D)
-                        radius = table_radius_1d(tab_ov, tab_r, tab_nd, delta);
-                    else if (fp.jkr_mode == BFM_JKR_TABLE_2D)
-                        radius = table_radius_2d(tab_ov, tab_nd, tab_g, tab_ng, tab_r, delta, gamma);
-                    else if (fp.jkr_mode == BFM_JKR_SURROGATE && sur_radius && sur_inside(delta, gamma, fp.env))
-                        radius = sur_radius[k];
-                    else
-                        radius = jkr_radius_newton(delta, fp.Req, fp.E0, gamma, 0.0, 200, nullptr);
-                    if (!(radius > 0)) atomicOr(status, ST_JKR_ROOT);
-                    mm = jkr_magnitude(radius, fp.Req, fp.E0, gamma);
-                    jkr = 1.0;
                 } else {
-                    if (overlap <= fp.max_overlap) mm = fp.pref0 * pow_half<3>(overlap);
-                    else mm = fp.pref2;
+                    isotropic_magnitudes(fp, overlap, gamma, tab_ov, tab_g, tab_r, tab_nd, tab_ng, sur_radius, k,
+                                         status, mm, jkr);
                 }
                 if (overlap > fp.max_overlap)
                     mh = fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
@@ -2786,7 +2859,10 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
 }
 
 // Device neighbor build into the context's internal list.
-void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv)
+// fuse: the compaction also writes the pair-force records of an isotropic
+// interaction (see fused_forces_ok); n > 1 only.
+void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv,
+                     bool fuse = false)
 {
     c->transposed = false;
     c->list_n = n;
@@ -2843,9 +2919,17 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
         size_t tb = c->temp_bytes;
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, c->d_cnt64, c->d_off64, n + 1, c->stream));
     }
-    KLAUNCH(c, "compact_rows", k_compact_rows, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64, c->d_off64,
-            c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps,
-            c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status);
+    if (fuse) {
+        const FusedForces ff{force_params(c), d_cells, ld, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng,
+                             c->d_rec};
+        KLAUNCH(c, "compact_rows", k_compact_rows<true>, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride,
+                c->slot_cap, c->d_cnt64, c->d_off64, c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi, c->pj,
+                c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status, ff);
+    } else {
+        KLAUNCH(c, "compact_rows", k_compact_rows<false>, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride,
+                c->slot_cap, c->d_cnt64, c->d_off64, c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi, c->pj,
+                c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status, FusedForces{});
+    }
     KLAUNCH(c, "compact_jrole", k_compact_jrole, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64,
             c->d_off64, c->d_jslot, c->d_jrow, c->max_pairs, c->d_csr);
     c->h_m = -1;
@@ -2956,6 +3040,18 @@ void launch_friction(bfm_ctx* c, const double* d_cells, int ld, const double* zs
 }
 
 
+
+// The interactions whose pair forces the compaction kernel can write: no
+// adhesion, or isotropic JKR with the exact radius or a table (not the
+// learned radius, which runs on the finished list, nor the anisotropic
+// tables).
+bool fused_forces_ok(const bfm_ctx* c)
+{
+    if (c->p.adhesion_mode == BFM_ADHESION_NONE) return true;
+    if (c->p.adhesion_mode != BFM_ADHESION_JKR_ISOTROPIC) return false;
+    if (c->p.jkr_mode == BFM_JKR_EXACT) return true;
+    return (c->p.jkr_mode == BFM_JKR_TABLE_1D || c->p.jkr_mode == BFM_JKR_TABLE_2D) && c->d_tab_ov != nullptr;
+}
 
 void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld)
 {
@@ -4346,8 +4442,10 @@ int bfm_dev_stage(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, 
     return guarded(c, [&] {
         need_params(c);
         check_cells(c, n, ld);
-        build_neighbors(c, d_cells, n, ld, thr, R, Ldiv);
-        compute_forces(c, d_cells, n, ld);
+        // an isotropic interaction's pair forces come out of the compaction
+        const bool fuse = n > 1 && fused_forces_ok(c);
+        build_neighbors(c, d_cells, n, ld, thr, R, Ldiv, fuse);
+        if (!fuse) compute_forces(c, d_cells, n, ld);
         launch_velocities(c, d_cells, n, ld, nullptr, nullptr, d_vel, nullptr);
     });
 }

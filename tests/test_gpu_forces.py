@@ -118,3 +118,65 @@ def test_empty_neighbor_list(orc, dome):
     import bfm
     cells, _ = dome
     _check(orc, cells, np.zeros((0, 7), order="F"), bfm.core.make_params())
+
+
+@pytest.mark.parametrize("mode", ["hertz", "jkr_exact", "gamma_zero_mix", "table_1d", "table_2d", "surrogate"])
+def test_stage_pair_forces_fused_into_compaction(orc, dome, mode):
+    """bfm_dev_stage writes the isotropic pair-force records inside the
+    neighbor-list compaction (Hertz, exact JKR, tables); the learned radius
+    keeps the separate kernels. The stage's velocities equal, bit for bit, a
+    separate build followed by getVelocities on the device list (which runs
+    the pair-force kernel), and the oracle's velocities within 1e-10 (the
+    learned radius: within its 1e-4 force tolerance of the exact oracle)."""
+    import torch
+
+    import bfm
+    from bfm import surrogate as sur
+    from bfm.jkr import reference_table, reference_table_2d
+    cells, nb = dome
+    cells = cells.copy(order="F")
+    table = None
+    if mode == "hertz":
+        p = bfm.core.make_params(adhesion=False, ncols=21)
+        cells = np.asfortranarray(cells[:, :21])
+    elif mode == "jkr_exact":
+        p = bfm.core.make_params()
+    elif mode == "gamma_zero_mix":
+        cells[::3, 21] = 0.0
+        p = bfm.core.make_params()
+    elif mode == "table_1d":
+        p = bfm.core.make_params(jkr_mode=bfm.core.JKR_TABLE_1D)
+        table = reference_table(R, RCELL, 3900.0, 100.0)
+    elif mode == "table_2d":
+        rng = np.random.default_rng(22)
+        cells[:, 21] = np.where(rng.random(cells.shape[0]) < 0.5, 50.0, 500.0)
+        ov, gm, rad = reference_table_2d(R, RCELL, 3900.0, 500.0)
+        p = bfm.core.make_params(jkr_mode=bfm.core.JKR_TABLE_2D)
+        table = (ov, np.ascontiguousarray(rad.ravel()), gm)
+    else:
+        p = bfm.core.make_params(jkr_mode=bfm.core.JKR_SURROGATE)
+    n = cells.shape[0]
+    print(f"fused stage mode={mode} n={n}", flush=True)
+    c = bfm.Context(n, max_pairs=16 * n, params=p)
+    if table is not None:
+        c.set_jkr_table(*table)
+        orc.set_jkr_table(*table)
+    if mode == "surrogate":
+        sur.install(c, sur.load())
+    d_cells = torch.from_numpy(np.ascontiguousarray(cells.T)).cuda()
+    d_v1 = torch.zeros((6, n), dtype=torch.float64, device="cuda")
+    d_v2 = torch.zeros((6, n), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    c.dev_stage(d_cells.data_ptr(), n, n, THR, R, LDIV, d_v1.data_ptr())
+    c.dev_build_neighbors(d_cells.data_ptr(), n, n, THR, R, LDIV)
+    c.dev_get_velocities(d_cells.data_ptr(), n, n, d_v2.data_ptr())
+    c.sync()
+    torch.cuda.synchronize()
+    v1, v2 = d_v1.cpu().numpy().T, d_v2.cpu().numpy().T
+    assert np.array_equal(v1, v2)
+    p_ref = bfm.core.make_params(adhesion=False, ncols=21) if mode == "hertz" else (
+        bfm.core.make_params() if mode == "surrogate" else p)
+    v_ref = orc.get_velocities(cells, nb, _a2d(cells, p), None, p_ref)
+    err = rel_err(v1, v_ref)
+    print(f"rel_err={err:.3e}", flush=True)
+    assert err <= (1e-4 if mode == "surrogate" else TOL)
