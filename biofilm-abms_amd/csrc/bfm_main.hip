@@ -1479,11 +1479,9 @@ struct VelParams {
 // s_i w; row j: -w, -s_j w; then the hard-core term the same way), exactly as
 // the reference's sequential loop adds forces.row(0/1) and forces_hard
 // (mechanics.hpp:1589-1591, 1643-1644).
-__device__ __forceinline__ void add_row(const double* __restrict__ rec, int k, bool role_i, double R,
-                                        double acc[6])
+__device__ __forceinline__ void add_row_vals(double2 a, double2 b, double2 c, double2 d, bool role_i, double R,
+                                             double acc[6])
 {
-    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)k * 8);
-    const double2 a = r[0], b = r[1], c = r[2], d = r[3];
     const double dx = a.x, dy = a.y, dz = b.x;
     const double dist = norm3(dx, dy, dz);
     const double overlap = 2 * R - dist;
@@ -1506,6 +1504,13 @@ __device__ __forceinline__ void add_row(const double* __restrict__ rec, int k, b
             acc[q + 3] += st * w;
         }
     }
+}
+
+__device__ __forceinline__ void add_row(const double* __restrict__ rec, int k, bool role_i, double R,
+                                        double acc[6])
+{
+    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)k * 8);
+    add_row_vals(r[0], r[1], r[2], r[3], role_i, R, acc);
 }
 
 // Row lists of a cell. CSR form (lists built by the device neighbor search):
@@ -1535,12 +1540,30 @@ __device__ __forceinline__ void gather_cc(int c, const RowLists& L, const double
         const unsigned long long o0 = L.off64[c], o1 = L.off64[c + 1];
         const int64_t e0 = (int64_t)(o0 >> 32) + (int64_t)(o0 & 0xffffffffull);
         const int64_t e1 = min((int64_t)(o1 >> 32) + (int64_t)(o1 & 0xffffffffull), 2 * L.cap);
-        for (int64_t e = e0; e < e1; ++e) {
-            // a build whose slots or pair capacity overflowed is redone or
-            // reported; until then its entries may be stale: stay in bounds
-            const int ent = L.csr[e];
-            const int row = ent & 0x7fffffff;
-            if (row < L.cap) add_row(rec, row, ent >= 0, R, acc);
+        // rows in groups of four: the four entries, then the four records, are
+        // loaded before any is used (two round trips per group instead of two
+        // per row), and added in row order
+        constexpr int G = 4;
+        for (int64_t e = e0; e < e1; e += G) {
+            int ent[G];
+            bool ok[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) ent[u] = e + u < e1 ? L.csr[e + u] : 0;
+            double2 ra[G], rb[G], rc[G], rd[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                // a build whose slots or pair capacity overflowed is redone or
+                // reported; until then its entries may be stale: stay in bounds
+                const int row = ent[u] & 0x7fffffff;
+                ok[u] = e + u < e1 && row < L.cap;
+                if (ok[u]) {
+                    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)row * 8);
+                    ra[u] = r[0]; rb[u] = r[1]; rc[u] = r[2]; rd[u] = r[3];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (ok[u]) add_row_vals(ra[u], rb[u], rc[u], rd[u], ent[u] >= 0, R, acc);
         }
     } else {
         int a = L.ioff[c], ae = L.ioff[c + 1], b = L.joff[c], be = L.joff[c + 1];
