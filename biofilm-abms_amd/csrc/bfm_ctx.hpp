@@ -69,7 +69,7 @@ struct bfm_ctx {
     int slot_cap = 32;
     double bin_scale = 1.0;
     int block_threads = 256;
-    int cand_variant = 6;           // LDS candidate kernel configuration (HOMES, CAP, BATCH)
+    int cand_variant = 8;           // candidate kernel (8: lane per home; 6 above 2^26 cells)
 
     bfm_params p{};
     bool params_set = false;
@@ -105,6 +105,7 @@ struct bfm_ctx {
     unsigned* d_scan_flag = nullptr;  // tile flags of the bin-offset scan
     unsigned scan_epoch = 0;          // launch epoch of the bin-offset scan
     double* d_sorted = nullptr;     // 7 x max_cells SoA: x y z nx ny nz hl
+    float4* d_sortf = nullptr;      // 2 x max_cells FP32: (x y z hl), then (nx ny nz 0), sorted order
     double* d_geom = nullptr;       // max_cells x 8 packed segments (distance updates)
     int* d_sid = nullptr;           // sorted -> original index
     int* d_skey = nullptr;          // sorted keys

@@ -374,7 +374,7 @@ k_scan_onepass(const Plan* __restrict__ plan, int64_t items_arg, const T* __rest
 __global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const int* __restrict__ keys,
                           const int* __restrict__ rank, const int* __restrict__ bin_off,
                           double* __restrict__ sorted, int* __restrict__ sid, int* __restrict__ skey, int64_t stride,
-                          unsigned long long* __restrict__ cnt64, int* __restrict__ hist)
+                          unsigned long long* __restrict__ cnt64, int* __restrict__ hist, float4* __restrict__ sortf)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt64[n] = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -391,6 +391,11 @@ __global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const
         sorted[4 * stride + q] = cellv(cells, ld, i, C_NY);
         sorted[5 * stride + q] = cellv(cells, ld, i, C_NZ);
         sorted[6 * stride + q] = cellv(cells, ld, i, C_HALF_L);
+        // FP32 copies for the lane-per-home candidate search
+        sortf[q] = make_float4((float)cellv(cells, ld, i, C_RX), (float)cellv(cells, ld, i, C_RY),
+                               (float)cellv(cells, ld, i, C_RZ), (float)cellv(cells, ld, i, C_HALF_L));
+        sortf[stride + q] = make_float4((float)cellv(cells, ld, i, C_NX), (float)cellv(cells, ld, i, C_NY),
+                                        (float)cellv(cells, ld, i, C_NZ), 0.f);
     }
 }
 
@@ -1054,6 +1059,251 @@ k_candidates_wave(int n, const Plan* __restrict__ plan, const double* __restrict
     __syncthreads();
     if (tid == 0) {
         count_add(counters, 0, s_screened);
+        count_add(counters, 1, tested);
+    }
+}
+
+// Number of set bits of m in the lanes below this one.
+__device__ __forceinline__ int lanes_below(uint64_t m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Squared distance from the point p to the segment c +- [0, c.w] n (FP32).
+__device__ __forceinline__ float point_seg_dist2(float4 p, float4 c, float4 n)
+{
+    const float ux = p.x - c.x, uy = p.y - c.y, uz = p.z - c.z;
+    const float t = fminf(fmaxf(ux * n.x + uy * n.y + uz * n.z, -c.w), c.w);
+    const float wx = ux - t * n.x, wy = uy - t * n.y, wz = uz - t * n.z;
+    return wx * wx + wy * wy + wz * wz;
+}
+
+// Second FP32 screen of a pair that passed the centre-distance screen: the
+// segment-segment distance is at least dist(centre of one, segment of the
+// other) minus the first's half length (every point of a segment lies within
+// its half length of its centre), so a pair is kept only if both such lower
+// bounds are at most ft = thr + margin. The FP32 rounding of the bounds
+// (~1e-5 um at coordinates of a few hundred um) is far below the margin, and
+// the reference's |d| joins two points of the segments, so it is never below
+// the true segment distance: no pair with |d| < thr is dropped.
+__device__ __forceinline__ bool seg_bounds_pass(float4 af, float4 an, float4 bf, float4 bn, float ft)
+{
+    const float ra = ft + bf.w, rb = ft + af.w;
+    return point_seg_dist2(bf, af, an) <= ra * ra && point_seg_dist2(af, bf, bn) <= rb * rb;
+}
+
+// Lane-per-home candidate search, without work-group barriers: each wavefront
+// owns 64 consecutive bin-ordered homes, one per lane, and every lane walks its
+// own home's half stencil -- the stencil rows (dz, dy) before (0, 0) over bins
+// [bx - reach, bx + reach] and the home row over [bx - reach, bx] up to the
+// home's sorted position -- reading the candidates' FP32 (x, y, z, half length)
+// from a packed copy in sorted order (lanes of one bin read the same lines).
+// Survivors of the centre-distance screen go to a wavefront-private LDS queue
+// (ballot + mbcnt, no atomics); 64 at a time they take the point-segment
+// screen, whose survivors go to a second queue that is drained 64 at a time
+// through the FP64 segment test, one pair per lane. The pair rule and
+// everything written are those of k_candidates_lds. Sorted positions must be
+// below 2^26 (queue entries carry the home lane in the top six bits).
+template <int UNROLL>
+__global__ void __launch_bounds__(256)
+k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
+                  const float4* __restrict__ sortf, const float4* __restrict__ sortn,
+                  const int* __restrict__ sid, const int* __restrict__ skey,
+                  const int* __restrict__ rep, const RepGrid* __restrict__ grid,
+                  const int* __restrict__ bin_off, int slot_cap, unsigned long long* __restrict__ cnt64,
+                  int* __restrict__ slot_j, double* __restrict__ slot_rec, int* __restrict__ jslot,
+                  unsigned long long* __restrict__ counters)
+{
+    constexpr int WAVES = 4, QA = 64 + 64 * UNROLL, QB = 128, QBITS = 26;
+    constexpr unsigned QMASK = (1u << QBITS) - 1u;
+    __shared__ unsigned s_qa[WAVES][QA];
+    __shared__ unsigned s_qb[WAVES][QB];
+    __shared__ float4 s_hf[WAVES][64], s_hn[WAVES][64];
+    __shared__ int s_hid[WAVES][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int hbase = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wave) * 64;
+    if (hbase >= n) return;   // the whole wavefront
+    const Plan P = *plan;
+    const double thr = P.thr, pre = P.pre;
+    const float ft = (float)(thr + P.fmargin);
+    const int reach = P.reach, side = 2 * reach + 1, nseg = reach * side + reach + 1;
+    unsigned* qa = s_qa[wave];
+    unsigned* qb = s_qb[wave];
+
+    const int hs = hbase + lane;
+    const bool hv = hs < n;
+    float4 hf = make_float4(0.f, 0.f, 0.f, 0.f), hn = hf;
+    int hid = 0, bx = 0, by = 0, bz = 0, gbase = 0, gd0 = 1, gd1 = 1, gd2 = 1;
+    if (hv) {
+        hf = sortf[hs];
+        hn = sortn[hs];
+        hid = sid[hs];
+        const int key = skey[hs];
+        const int r = rep ? rep[hid] : 0;
+        gbase = grid[r].base;
+        gd0 = grid[r].dims[0];
+        gd1 = grid[r].dims[1];
+        gd2 = grid[r].dims[2];
+        const int local = key - gbase;
+        bx = local % gd0;
+        const int rr = local / gd0;
+        by = rr % gd1;
+        bz = rr / gd1;
+    }
+    s_hf[wave][lane] = hf;
+    s_hn[wave][lane] = hn;
+    s_hid[wave][lane] = hid;
+    const float hb = ft + hf.w;   // thr + margin + the home's half length
+    unsigned long long screened = 0, tested = 0;
+    int na = 0, nb = 0;           // queue fills, uniform across the wavefront
+
+    // FP64 test of queue-B entries, 64 per round: all of them (all) or whole
+    // rounds, the rest moved to the front
+    auto drain_b = [&](bool all) __attribute__((always_inline)) {
+        wave_sync();
+        int e0 = 0;
+        for (; e0 + 64 <= nb || (all && e0 < nb); e0 += 64) {
+            const int e = e0 + lane;
+            if (e < nb) {
+                const unsigned ent = qb[e];
+                const int h = (int)(ent >> QBITS), q = (int)(ent & QMASK);
+                const Seg ch = load_sorted(sorted, stride, hbase + h);
+                const Seg cc = load_sorted(sorted, stride, q);
+                const int hid_h = s_hid[wave][h], cid = sid[q];
+                const bool home_is_i = hid_h > cid;
+                const int i = home_is_i ? hid_h : cid;
+                const int j = home_is_i ? cid : hid_h;
+                const double dx = ch.r[0] - cc.r[0], dy = ch.r[1] - cc.r[1], dz = ch.r[2] - cc.r[2];
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                const double bound = (thr + ch.hl + cc.hl) * (1 + 1e-9) + 1e-9;
+                if (r2 <= bound * bound && sqrt(r2) < pre) {
+                    // one inlined segment test for every lane: the operands are
+                    // selected per lane (cell i first) instead of two branches
+                    Seg c1, c2;
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        c1.r[x] = home_is_i ? ch.r[x] : cc.r[x];
+                        c2.r[x] = home_is_i ? cc.r[x] : ch.r[x];
+                        c1.n[x] = home_is_i ? ch.n[x] : cc.n[x];
+                        c2.n[x] = home_is_i ? cc.n[x] : ch.n[x];
+                    }
+                    c1.hl = home_is_i ? ch.hl : cc.hl;
+                    c2.hl = home_is_i ? cc.hl : ch.hl;
+                    double d[3], sv, tv;
+                    dist_between_cells(c1, c2, d, sv, tv);
+                    if (norm3(d[0], d[1], d[2]) < thr) {
+                        const int slot = (int)(atomicAdd(&cnt64[i], 1ull << 32) >> 32);
+                        const int slot2 = (int)(atomicAdd(&cnt64[j], 1ull) & 0xffffffffull);
+                        if (slot < slot_cap) {
+                            const int64_t es = (int64_t)slot * stride + i;
+                            slot_j[es] = j;
+                            double2* rr = reinterpret_cast<double2*>(slot_rec + 8 * es);
+                            rr[0] = make_double2(d[0], d[1]);
+                            rr[1] = make_double2(d[2], sv);
+                            rr[2] = make_double2(tv, __longlong_as_double((long long)slot2));
+                        }
+                        if (slot2 < slot_cap) jslot[(int64_t)slot2 * stride + j] = i;
+                    }
+                }
+            }
+            tested += (unsigned long long)min(64, nb - e0);
+        }
+        const int rem = nb - e0;   // < 64; 0 when all
+        wave_sync();
+        const unsigned v = lane < rem ? qb[e0 + lane] : 0u;
+        wave_sync();
+        if (lane < rem) qb[lane] = v;
+        nb = max(rem, 0);
+        wave_sync();
+    };
+    // point-segment screen of queue-A entries, 64 per round, survivors to queue B
+    auto drain_a = [&](bool all) __attribute__((always_inline)) {
+        wave_sync();
+        int e0 = 0;
+        for (; e0 + 64 <= na || (all && e0 < na); e0 += 64) {
+            const int e = e0 + lane;
+            bool pass = false;
+            unsigned ent = 0;
+            if (e < na) {
+                ent = qa[e];
+                const int h = (int)(ent >> QBITS), q = (int)(ent & QMASK);
+                pass = seg_bounds_pass(s_hf[wave][h], s_hn[wave][h], sortf[q], sortn[q], ft);
+            }
+            const uint64_t m = __ballot(pass);
+            if (pass) qb[nb + lanes_below(m)] = ent;
+            nb += __popcll(m);
+            if (nb >= 64) drain_b(false);
+        }
+        const int rem = na - e0;
+        wave_sync();
+        const unsigned v = lane < rem ? qa[e0 + lane] : 0u;
+        wave_sync();
+        if (lane < rem) qa[lane] = v;
+        na = max(rem, 0);
+        wave_sync();
+    };
+
+    // bounds of stencil row k as a run of sorted positions (len 0 outside the grid)
+    auto seg_bounds = [&](int k, int& lo, int& len) __attribute__((always_inline)) {
+        const int dz = k / side - reach, dy = k % side - reach;
+        lo = 0;
+        len = 0;
+        if (hv) {
+            const int zz = bz + dz, yy = by + dy;
+            if (zz >= 0 && zz < gd2 && yy >= 0 && yy < gd1) {
+                const int rb = gbase + (zz * gd1 + yy) * gd0;
+                const bool home_row = dz == 0 && dy == 0;
+                const int xlo = max(0, bx - reach);
+                const int xhi = home_row ? bx : min(gd0 - 1, bx + reach);
+                lo = bin_off[rb + xlo];
+                int hi = bin_off[rb + xhi + 1];
+                if (home_row) hi = min(hi, hs);   // candidates before the home in bin order
+                len = max(0, hi - lo);
+            }
+        }
+    };
+    int lo, len;
+    seg_bounds(0, lo, len);
+    for (int k = 0; k < nseg; ++k) {
+        int lo_next = 0, len_next = 0;   // the next row's bounds, loaded ahead
+        if (k + 1 < nseg) seg_bounds(k + 1, lo_next, len_next);
+        int maxlen = len;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, off));
+        for (int i0 = 0; i0 < maxlen; i0 += UNROLL) {
+            // loads past the lane's run read a valid position and are masked below
+            float4 cf[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) cf[u] = sortf[min(lo + i0 + u, n - 1)];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const float fx = hf.x - cf[u].x, fy = hf.y - cf[u].y, fz = hf.z - cf[u].z;
+                const float fb = hb + cf[u].w;
+                const bool pass = (i0 + u < len) & (fx * fx + fy * fy + fz * fz <= fb * fb);
+                const uint64_t m = __ballot(pass);
+                if (m) {
+                    if (pass) qa[na + lanes_below(m)] = ((unsigned)lane << QBITS) | (unsigned)(lo + i0 + u);
+                    na += __popcll(m);
+                }
+            }
+            screened += (unsigned long long)max(0, min(UNROLL, len - i0));
+            if (na >= 64) drain_a(false);
+        }
+        lo = lo_next;
+        len = len_next;
+    }
+    drain_a(true);
+    drain_b(true);
+    for (int off = 32; off > 0; off >>= 1) screened += __shfl_xor(screened, off);
+    if (lane == 0) {
+        count_add(counters, 0, screened);
         count_add(counters, 1, tested);
     }
 }
@@ -2864,6 +3114,12 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
             c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_jslot,        \
             c->d_counters)
     switch (c->cand_variant) {
+        case 8:
+            KLAUNCH(c, "candidates", k_candidates_lane<4>, grid_for(n, 256), 256, 0, n, c->d_plan, c->d_sorted,
+                    stride, c->d_sortf, c->d_sortf + stride, c->d_sid, c->d_skey,
+                    c->n_rep > 0 ? c->d_rep : (const int*)nullptr, c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64,
+                    c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_counters);
+            break;
         case 7:
             KLAUNCH(c, "candidates", k_candidates_bins<256>, 2048, 256, 0, c->d_plan, c->d_sorted, stride, c->d_sid,
                     c->d_grid, std::max(c->n_rep, 1), c->d_bin_off, c->slot_cap, c->d_cnt64, c->d_slot_j,
@@ -2933,7 +3189,7 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     }
     const int64_t stride = c->max_cells;
     KLAUNCH(c, "scatter", k_scatter, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals, c->d_bin_off,
-            c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count);
+            c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count, c->d_sortf);
     c->bins_dirty = false;
     launch_candidates(c, n, stride);
     {
@@ -4014,6 +4270,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
     bfm_ctx* c = new bfm_ctx();
     c->device = device;
     c->max_cells = max_cells;
+    if (max_cells > (1 << 26)) c->cand_variant = 6;   // the lane-per-home search packs positions in 26 bits
     c->max_pairs = max_pairs;
     c->max_bins = std::max(2 * max_cells, 4096);
     int rc = guarded(c, [&] {
@@ -4042,6 +4299,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_scan_flag, (size_t)kMaxScanTiles);
         HIPCHK(hipMemset(c->d_scan_flag, 0, sizeof(unsigned) * kMaxScanTiles));
         dalloc(&c->d_sorted, N * 7);
+        dalloc(&c->d_sortf, N * 2);
         dalloc(&c->d_geom, N * 8);
         dalloc(&c->d_sid, N);
         dalloc(&c->d_skey, N);
@@ -4097,7 +4355,7 @@ int bfm_destroy(bfm_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_status, c->d_counters, c->d_cells, c->d_cells_new, c->d_nb, c->d_noise, c->d_out6,
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
-                    c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, c->d_count,
+                    c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sortf, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst, c->d_geom,
@@ -4491,7 +4749,7 @@ int bfm_autotune_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, dou
         need_params(c);
         check_cells(c, n, ld);
         static const double scales[] = {0.5, 0.75, 1.0, 1.25};
-        static const int variants[] = {0, 1, 2, 3, 4, 5, 6, 7};
+        static const int variants[] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
         double best = INFINITY, bs = c->bin_scale;
         int bv = c->cand_variant;
         hipEvent_t e0, e1;
@@ -4590,9 +4848,11 @@ int bfm_set_tuning(bfm_ctx* c, double bin_scale, int block_threads, int kernel_v
 {
     return guarded(c, [&] {
         if (kernel_variant >= 0) {
-            require(kernel_variant <= 7, BFM_ERR_ARG, "kernel_variant must be 0..7");
+            require(kernel_variant <= 8, BFM_ERR_ARG, "kernel_variant must be 0..8");
             require(kernel_variant != 7 || c->max_cells < (1 << 25), BFM_ERR_ARG,
                     "kernel_variant 7 needs max_cells < 2^25");
+            require(kernel_variant != 8 || c->max_cells <= (1 << 26), BFM_ERR_ARG,
+                    "kernel_variant 8 needs max_cells <= 2^26");
             c->cand_variant = kernel_variant;
         }
         if (bin_scale > 0) {

@@ -381,7 +381,7 @@ int bfm_ensemble_simulate(bfm_ctx* ctx, const bfm_sim_config* cfg, int n_rep, co
                           int* failed_out, bfm_ensemble_stats* stats);
 
 /* Times the device neighbor build on these cells for bin scales {0.5, 0.75,
- * 1, 1.25} x candidate-kernel variants 0..7 and keeps the fastest (every
+ * 1, 1.25} x candidate-kernel variants 0..8 and keeps the fastest (every
  * choice gives the same list). Rebuilds the list with the choice. */
 int bfm_autotune_neighbors(bfm_ctx* ctx, const double* d_cells, int n, int ld, double neighbor_threshold,
                            double R, double Ldiv, double* best_scale, int* best_variant, double* best_ms);
@@ -404,11 +404,14 @@ int bfm_stats(bfm_ctx* ctx, int64_t* m, int64_t* screened, int64_t* tested, doub
 /* Tuning knobs of the neighbor build: bin edge as a multiple of the search
  * cutoff (1/3 .. 4; 1 gives the 27-bin stencil, 0.5 the 125-bin stencil),
  * threads per workgroup (reserved, 0), and the candidate-kernel configuration
- * 0..7 (0..3 and 6: thread-per-candidate screen with (homes, LDS chunk) =
- * (32, 640), (64, 640), (64, 1024), (64, 512), (128, 512), the default 6;
+ * 0..8 (0..3 and 6: thread-per-candidate screen with (homes, LDS chunk) =
+ * (32, 640), (64, 640), (64, 1024), (64, 512), (128, 512);
  * 4, 5: wave-per-candidate screen with LDS chunk 512 or 1024; 7: one
  * wavefront per home bin, lane per candidate, no work-group barriers; needs
- * max_cells < 2^25). 0 / 0 / -1 keep the current values.
+ * max_cells < 2^25; 8: lane per home walking its own half stencil, a second
+ * point-segment screen before the FP64 test, no work-group barriers; needs
+ * max_cells <= 2^26; the default, 6 for larger contexts). 0 / 0 / -1 keep
+ * the current values.
  * No knob changes the neighbor list, only the time to build it. */
 int bfm_set_tuning(bfm_ctx* ctx, double bin_scale, int block_threads, int kernel_variant);
 

@@ -68,7 +68,7 @@ def test_ensemble_many_replicates_autotuned():
     cap = 2 * int(sizes.sum()) + 64
     print(f"ensemble of 64 replicates, {int(sizes.sum())} cells", flush=True)
     out, t_out, it_out, st = sim.ensemble_simulate(_ctx(cap), cfg, reps, seeds, autotune=True, cap=cap)
-    assert 0 <= st["tuned_variant"] <= 7 and st["tuned_bin_scale"] in (0.5, 0.75, 1.0, 1.25)
+    assert 0 <= st["tuned_variant"] <= 8 and st["tuned_bin_scale"] in (0.5, 0.75, 1.0, 1.25)
     assert st["failed_replicates"] == [47]
     ok = np.ones(64, bool)
     ok[47] = False

@@ -77,7 +77,8 @@ def test_tiny_inputs(ctx, orc, n):
     (1.0, 0, 2), (0.5, 0, 3), (1.25, 0, 3),
     (1.0, 0, 4), (0.5, 0, 4), (0.34, 0, 4), (2.0, 0, 4), (1.25, 0, 5), (0.75, 0, 5),
     (1.0, 0, 6), (0.5, 0, 6), (1.25, 0, 6),
-    (1.0, 0, 7), (0.5, 0, 7), (1.25, 0, 7), (0.34, 0, 7), (2.0, 0, 7)])
+    (1.0, 0, 7), (0.5, 0, 7), (1.25, 0, 7), (0.34, 0, 7), (2.0, 0, 7),
+    (1.0, 0, 8), (0.5, 0, 8), (0.75, 0, 8), (1.25, 0, 8), (0.34, 0, 8), (2.0, 0, 8)])
 def test_tuning_knobs_do_not_change_the_list(orc, bin_scale, block, variant):
     import bfm
     cells, _ = bfm.make_dome(8000, seed=12)
@@ -88,7 +89,7 @@ def test_tuning_knobs_do_not_change_the_list(orc, bin_scale, block, variant):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("variant", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [3, 4, 5, 6, 7, 8])
 def test_dense_packing_queue_overflow_path(orc, variant):
     """A packing dense enough that a group's survivors overflow the LDS queue
     in one pass, so the candidate kernels redo the chunk in home batches."""
