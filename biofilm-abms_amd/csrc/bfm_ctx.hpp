@@ -128,6 +128,7 @@ struct bfm_ctx {
     unsigned long long* d_cnt64 = nullptr;  // per cell: (rows as i) << 32 | (rows as j)
     unsigned long long* d_off64 = nullptr;  // exclusive scan of d_cnt64
     int* d_jslot = nullptr;         // max_cells x slot_cap: partners with larger index
+    int* d_jslotk = nullptr;        // slot_cap x max_cells: the i-role slot of each partner-list entry
     int* d_jrow = nullptr;          // max_cells x slot_cap: row index of each jslot pair
     int* d_csr = nullptr;           // 2 x max_pairs row indices (bit 31: j role)
     double* d_rec = nullptr;        // 8 x max_pairs: per-row {d, s, t, main, hard, 0}

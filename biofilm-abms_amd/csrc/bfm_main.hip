@@ -1063,6 +1063,127 @@ k_candidates_wave(int n, const Plan* __restrict__ plan, const double* __restrict
     }
 }
 
+struct ForceParams {
+    double R, Rcell, E0, Req, max_overlap;
+    double pref0, pref1, pref2;
+    int adhesion;
+    int jkr_mode;
+    int colidx_gamma;
+    int pad;
+    SurrogateEnvelope env;   // rows inside it take the learned radius (jkr_mode surrogate)
+};
+
+// The main-term magnitude of the adhesive overload for an isotropic pair
+// (gamma = min of the two cells'): JKR term1 - term2 at the overlap capped at
+// the shell thickness, the contact radius from the exact FP64 solve, a table
+// or the learned radius (rows inside its envelope); Hertz (EPS-capped) when
+// gamma is 0.
+__device__ __forceinline__ void isotropic_magnitudes(const ForceParams& fp, double overlap, double gamma,
+                                                     const double* __restrict__ tab_ov,
+                                                     const double* __restrict__ tab_g,
+                                                     const double* __restrict__ tab_r, int tab_nd, int tab_ng,
+                                                     const double* __restrict__ sur_radius, int64_t k,
+                                                     int* __restrict__ status, double& mm, double& jkr)
+{
+    if (gamma > 0) {
+        double delta = overlap;
+        if (fp.max_overlap >= 0 && delta > fp.max_overlap) delta = fp.max_overlap;
+        double radius;
+        if (fp.jkr_mode == BFM_JKR_TABLE_1D)
+            radius = table_radius_1d(tab_ov, tab_r, tab_nd, delta);
+        else if (fp.jkr_mode == BFM_JKR_TABLE_2D)
+            radius = table_radius_2d(tab_ov, tab_nd, tab_g, tab_ng, tab_r, delta, gamma);
+        else if (fp.jkr_mode == BFM_JKR_SURROGATE && sur_radius && sur_inside(delta, gamma, fp.env))
+            radius = sur_radius[k];
+        else
+            radius = jkr_radius_newton(delta, fp.Req, fp.E0, gamma, 0.0, 200, nullptr);
+        if (!(radius > 0)) atomicOr(status, ST_JKR_ROOT);
+        mm = jkr_magnitude(radius, fp.Req, fp.E0, gamma);
+        jkr = 1.0;
+    } else {
+        if (overlap <= fp.max_overlap) mm = fp.pref0 * pow_half<3>(overlap);
+        else mm = fp.pref2;
+    }
+}
+
+// Main-term, hard-core and JKR-radius values of an isotropic row (i, j) with
+// centerline vector d: Hertz with the EPS cap, or the adhesive overload's
+// JKR main term plus the separate hard-core term (mechanics.hpp:1209-1350,
+// 1390-1659).
+__device__ __forceinline__ void isotropic_vals(const ForceParams& fp, const double* __restrict__ cells, int ld,
+                                               int i, int j, double dx, double dy, double dz,
+                                               const double* __restrict__ tab_ov, const double* __restrict__ tab_g,
+                                               const double* __restrict__ tab_r, int tab_nd, int tab_ng,
+                                               int* __restrict__ status, double& mm, double& mh, double& jkr)
+{
+    const double dist = norm3(dx, dy, dz);
+    const double overlap = 2 * fp.R - dist;
+    mm = 0;
+    mh = 0;
+    jkr = 0;
+    if (overlap > 0) {
+        if (!fp.adhesion) {
+            if (overlap <= fp.max_overlap) {
+                mm = fp.pref0 * pow_half<3>(overlap);
+            } else {
+                mm = fp.pref2;
+                mm += fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
+            }
+        } else {
+            const double gamma = dmin(cellv(cells, ld, i, fp.colidx_gamma), cellv(cells, ld, j, fp.colidx_gamma));
+            isotropic_magnitudes(fp, overlap, gamma, tab_ov, tab_g, tab_r, tab_nd, tab_ng, nullptr, 0, status, mm,
+                                 jkr);
+            if (overlap > fp.max_overlap)
+                mh = fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
+        }
+    }
+}
+
+// The 64-byte record of one row for an isotropic interaction ({dx, dy, dz,
+// s_i, s_j, main, hard, jkr}, as k_pair_forces writes it).
+__device__ __forceinline__ void isotropic_record(const ForceParams& fp, const double* __restrict__ cells, int ld,
+                                                 int i, int j, double dx, double dy, double dz, double si,
+                                                 double sj, const double* __restrict__ tab_ov,
+                                                 const double* __restrict__ tab_g,
+                                                 const double* __restrict__ tab_r, int tab_nd, int tab_ng,
+                                                 int* __restrict__ status, double* __restrict__ rec)
+{
+    double mm, mh, jkr;
+    isotropic_vals(fp, cells, ld, i, j, dx, dy, dz, tab_ov, tab_g, tab_r, tab_nd, tab_ng, status, mm, mh, jkr);
+    double2* r = reinterpret_cast<double2*>(rec);
+    r[0] = make_double2(dx, dy);
+    r[1] = make_double2(dz, si);
+    r[2] = make_double2(sj, mm);
+    r[3] = make_double2(mh, jkr);
+}
+
+// Packs the slots into the reference's row order and builds the per-cell
+// force lists. Rows of cell i start at hi(off64[i]) and are sorted by
+// partner j (rank by comparison). The force list of cell c (starting at
+// hi(off64[c]) + lo(off64[c])) holds the rows where c is i (ascending j, i.e.
+// ascending row) followed by the rows where c is j (ascending i), i.e. all
+// rows touching c in row order -- the order the reference's sequential pair
+// loop adds them (mechanics.hpp:1438-1647); bit 31 marks the j role.
+// Eight lanes per cell: lane k of a cell's group holds the cell's entries
+// k, k + 8, ...; an entry's rank among the cell's partners comes from
+// width-8 shuffles, so the work per cell is O(entries) per lane and no wave
+// waits for its most connected cell's O(entries^2) loop.
+// Pass 1 (per cell i): rows, i-role entries, and the row index of each pair
+// stored at the pair's slot in j's partner list (jrow). FORCES: also the
+// row's force record of an isotropic interaction (Hertz, or JKR with the
+// exact radius or a table), the record k_pair_forces would write, while the
+// row's geometry is in registers.
+struct FusedForces {
+    ForceParams fp;
+    const double* cells;
+    int ld;
+    const double* tab_ov;
+    const double* tab_g;
+    const double* tab_r;
+    int tab_nd, tab_ng;
+    double* rec;
+};
+
 // Number of set bits of m in the lanes below this one.
 __device__ __forceinline__ int lanes_below(uint64_t m)
 {
@@ -1119,7 +1240,7 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
                   const int* __restrict__ rep, const RepGrid* __restrict__ grid,
                   const int* __restrict__ bin_off, int slot_cap, unsigned long long* __restrict__ cnt64,
                   int* __restrict__ slot_j, double* __restrict__ slot_rec, int* __restrict__ jslot,
-                  unsigned long long* __restrict__ counters)
+                  int* __restrict__ jslotk, unsigned long long* __restrict__ counters)
 {
     constexpr int WAVES = 4, QA = 64 + 64 * UNROLL, QB = 128, QBITS = 26;
     constexpr unsigned QMASK = (1u << QBITS) - 1u;
@@ -1209,7 +1330,10 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
                             rr[1] = make_double2(d[2], sv);
                             rr[2] = make_double2(tv, __longlong_as_double((long long)slot2));
                         }
-                        if (slot2 < slot_cap) jslot[(int64_t)slot2 * stride + j] = i;
+                        if (slot2 < slot_cap) {
+                            jslot[(int64_t)slot2 * stride + j] = i;
+                            jslotk[(int64_t)slot2 * stride + j] = slot;   // where j finds the row's record
+                        }
                     }
                 }
             }
@@ -1308,110 +1432,6 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
     }
 }
 
-struct ForceParams {
-    double R, Rcell, E0, Req, max_overlap;
-    double pref0, pref1, pref2;
-    int adhesion;
-    int jkr_mode;
-    int colidx_gamma;
-    int pad;
-    SurrogateEnvelope env;   // rows inside it take the learned radius (jkr_mode surrogate)
-};
-
-// The main-term magnitude of the adhesive overload for an isotropic pair
-// (gamma = min of the two cells'): JKR term1 - term2 at the overlap capped at
-// the shell thickness, the contact radius from the exact FP64 solve, a table
-// or the learned radius (rows inside its envelope); Hertz (EPS-capped) when
-// gamma is 0.
-__device__ __forceinline__ void isotropic_magnitudes(const ForceParams& fp, double overlap, double gamma,
-                                                     const double* __restrict__ tab_ov,
-                                                     const double* __restrict__ tab_g,
-                                                     const double* __restrict__ tab_r, int tab_nd, int tab_ng,
-                                                     const double* __restrict__ sur_radius, int64_t k,
-                                                     int* __restrict__ status, double& mm, double& jkr)
-{
-    if (gamma > 0) {
-        double delta = overlap;
-        if (fp.max_overlap >= 0 && delta > fp.max_overlap) delta = fp.max_overlap;
-        double radius;
-        if (fp.jkr_mode == BFM_JKR_TABLE_1D)
-            radius = table_radius_1d(tab_ov, tab_r, tab_nd, delta);
-        else if (fp.jkr_mode == BFM_JKR_TABLE_2D)
-            radius = table_radius_2d(tab_ov, tab_nd, tab_g, tab_ng, tab_r, delta, gamma);
-        else if (fp.jkr_mode == BFM_JKR_SURROGATE && sur_radius && sur_inside(delta, gamma, fp.env))
-            radius = sur_radius[k];
-        else
-            radius = jkr_radius_newton(delta, fp.Req, fp.E0, gamma, 0.0, 200, nullptr);
-        if (!(radius > 0)) atomicOr(status, ST_JKR_ROOT);
-        mm = jkr_magnitude(radius, fp.Req, fp.E0, gamma);
-        jkr = 1.0;
-    } else {
-        if (overlap <= fp.max_overlap) mm = fp.pref0 * pow_half<3>(overlap);
-        else mm = fp.pref2;
-    }
-}
-
-// The 64-byte record of one row for an isotropic interaction ({dx, dy, dz,
-// s_i, s_j, main, hard, jkr}, as k_pair_forces writes it).
-__device__ __forceinline__ void isotropic_record(const ForceParams& fp, const double* __restrict__ cells, int ld,
-                                                 int i, int j, double dx, double dy, double dz, double si,
-                                                 double sj, const double* __restrict__ tab_ov,
-                                                 const double* __restrict__ tab_g,
-                                                 const double* __restrict__ tab_r, int tab_nd, int tab_ng,
-                                                 int* __restrict__ status, double* __restrict__ rec)
-{
-    const double dist = norm3(dx, dy, dz);
-    const double overlap = 2 * fp.R - dist;
-    double mm = 0, mh = 0, jkr = 0;
-    if (overlap > 0) {
-        if (!fp.adhesion) {
-            if (overlap <= fp.max_overlap) {
-                mm = fp.pref0 * pow_half<3>(overlap);
-            } else {
-                mm = fp.pref2;
-                mm += fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
-            }
-        } else {
-            const double gamma = dmin(cellv(cells, ld, i, fp.colidx_gamma), cellv(cells, ld, j, fp.colidx_gamma));
-            isotropic_magnitudes(fp, overlap, gamma, tab_ov, tab_g, tab_r, tab_nd, tab_ng, nullptr, 0, status, mm,
-                                 jkr);
-            if (overlap > fp.max_overlap)
-                mh = fp.pref1 * pow_half<3>(overlap - fp.max_overlap);
-        }
-    }
-    double2* r = reinterpret_cast<double2*>(rec);
-    r[0] = make_double2(dx, dy);
-    r[1] = make_double2(dz, si);
-    r[2] = make_double2(sj, mm);
-    r[3] = make_double2(mh, jkr);
-}
-
-// Packs the slots into the reference's row order and builds the per-cell
-// force lists. Rows of cell i start at hi(off64[i]) and are sorted by
-// partner j (rank by comparison). The force list of cell c (starting at
-// hi(off64[c]) + lo(off64[c])) holds the rows where c is i (ascending j, i.e.
-// ascending row) followed by the rows where c is j (ascending i), i.e. all
-// rows touching c in row order -- the order the reference's sequential pair
-// loop adds them (mechanics.hpp:1438-1647); bit 31 marks the j role.
-// Eight lanes per cell: lane k of a cell's group holds the cell's entries
-// k, k + 8, ...; an entry's rank among the cell's partners comes from
-// width-8 shuffles, so the work per cell is O(entries) per lane and no wave
-// waits for its most connected cell's O(entries^2) loop.
-// Pass 1 (per cell i): rows, i-role entries, and the row index of each pair
-// stored at the pair's slot in j's partner list (jrow). FORCES: also the
-// row's force record of an isotropic interaction (Hertz, or JKR with the
-// exact radius or a table), the record k_pair_forces would write, while the
-// row's geometry is in registers.
-struct FusedForces {
-    ForceParams fp;
-    const double* cells;
-    int ld;
-    const double* tab_ov;
-    const double* tab_g;
-    const double* tab_r;
-    int tab_nd, tab_ng;
-    double* rec;
-};
 
 template <bool FORCES>
 __global__ void __launch_bounds__(256)
@@ -2357,6 +2377,102 @@ k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restr
     }
 }
 
+// Isotropic row values of the lane-per-home search's per-cell lists: for each
+// slot entry (i = c, j = its partner) the main and hard-core terms of the
+// row, computed as the compaction computes them, stored in the last two
+// doubles of the entry's record {d, s_i, s_j, partner slot, main, hard}.
+// Eight lanes per cell.
+__global__ void __launch_bounds__(256)
+k_slot_values(int n, FusedForces ff, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+              const int* __restrict__ slot_j, double* __restrict__ slot_rec, int* __restrict__ status)
+{
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(g >> 3), k0 = (int)(g & 7);
+        const int ci = min((int)(cnt64[c] >> 32), slot_cap);
+        for (int k = k0; k < ci; k += 8) {
+            const int64_t e = (int64_t)k * stride + c;
+            double2* rr = reinterpret_cast<double2*>(slot_rec + 8 * e);
+            const double2 r0 = rr[0], r1 = rr[1];
+            double mm, mh, jkr;
+            isotropic_vals(ff.fp, ff.cells, ff.ld, c, slot_j[e], r0.x, r0.y, r1.x, ff.tab_ov, ff.tab_g, ff.tab_r,
+                           ff.tab_nd, ff.tab_ng, status, mm, mh, jkr);
+            rr[3] = make_double2(mm, mh);
+        }
+    }
+}
+
+// One slot-list row {d, s_i, s_j, partner slot, main, hard} added as
+// k_velocities adds the compacted row.
+__device__ __forceinline__ void slot_row(const double2* __restrict__ rr, bool role_i, double R, double acc[6])
+{
+    const double2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3];
+    add_row_vals(r0, r1, make_double2(r2.x, r3.x), make_double2(r3.y, 0.0), role_i, R, acc);
+}
+
+// getVelocities of one rebuilt stage straight from the lane-per-home search's
+// per-cell lists, without compacting them into rows: the entries of cell c
+// where c is i (partners j < c: slot_j and the record at c's slots) in
+// ascending j, then those where c is j (partners i > c: jslot, the record at
+// slot jslotk of i) in ascending i -- the reference's row order for c
+// (mechanics.hpp:1438-1647) -- each row's values (k_slot_values) added as
+// k_velocities adds them, so the result is bit-identical to the compacted
+// path. The next entry is the smallest partner above the previous one
+// (partners of a cell are distinct; lists are a few entries long).
+// Isotropic interactions without friction only.
+__global__ void __launch_bounds__(256)
+k_velocities_slots(int n, const double* __restrict__ cells, int ld, VelParams vp, int64_t stride,
+                   int slot_cap, const unsigned long long* __restrict__ cnt64, const int* __restrict__ slot_j,
+                   const double* __restrict__ slot_rec, const int* __restrict__ jslot,
+                   const int* __restrict__ jslotk, double* __restrict__ vel, int* __restrict__ status)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const unsigned long long cv = cnt64[c];
+        int ci = (int)(cv >> 32), cj = (int)(cv & 0xffffffffull);
+        if (ci > slot_cap || cj > slot_cap) {
+            atomicOr(status, ST_SLOT_OVERFLOW);
+            ci = min(ci, slot_cap);
+            cj = min(cj, slot_cap);
+        }
+        double cc[6] = {0, 0, 0, 0, 0, 0};
+        int prev = -1;
+        for (int r = 0; r < ci; ++r) {   // rows where c is i, ascending partner j
+            int best = INT_MAX, bk = 0;
+            for (int k = 0; k < ci; ++k) {
+                const int j = slot_j[(int64_t)k * stride + c];
+                if (j > prev && j < best) { best = j; bk = k; }
+            }
+            prev = best;
+            slot_row(reinterpret_cast<const double2*>(slot_rec + 8 * ((int64_t)bk * stride + c)), true, vp.sc.R, cc);
+        }
+        prev = -1;
+        for (int r = 0; r < cj; ++r) {   // rows where c is j, ascending partner i
+            int best = INT_MAX, bk = 0;
+            for (int k = 0; k < cj; ++k) {
+                const int i = jslot[(int64_t)k * stride + c];
+                if (i > prev && i < best) { best = i; bk = k; }
+            }
+            prev = best;
+            const int si = jslotk[(int64_t)bk * stride + c];
+            if (si >= slot_cap) continue;   // i's slots overflowed: reported by i's thread
+            slot_row(reinterpret_cast<const double2*>(slot_rec + 8 * ((int64_t)si * stride + best)), false, vp.sc.R,
+                     cc);
+        }
+        const CellVel cv6 = load_cellvel(cells, ld, c);
+        const bool a2d = cv6.nz < vp.nz_threshold;
+        double nz6[6] = {0, 0, 0, 0, 0, 0};
+        double v[6];
+        cell_velocity(vp, cv6, cellv(cells, ld, c, C_SIGMA0), a2d, cc, false, nz6, v);
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            vel[(int64_t)q * n + c] = v[q];
+            bad |= !isfinite(v[q]);
+        }
+        if (bad) atomicOr(status, ST_NAN);
+    }
+}
+
 // Conservative forces only (getConservativeForces, include_constraint = false).
 template <bool CSR>
 __global__ void k_conservative(int n, const double* __restrict__ cells, int ld,
@@ -2992,8 +3108,12 @@ void grow_slots(bfm_ctx* c)
     int *js = nullptr, *jr = nullptr;
     HIPCHK(hipMalloc((void**)&js, sizeof(int) * (size_t)c->max_cells * cap));
     HIPCHK(hipMalloc((void**)&jr, sizeof(int) * (size_t)c->max_cells * cap));
-    for (void* q : {(void*)c->d_slot_j, (void*)c->d_slot_rec, (void*)c->d_jslot, (void*)c->d_jrow})
+    int* jk = nullptr;
+    HIPCHK(hipMalloc((void**)&jk, sizeof(int) * (size_t)c->max_cells * cap));
+    for (void* q : {(void*)c->d_slot_j, (void*)c->d_slot_rec, (void*)c->d_jslot, (void*)c->d_jrow,
+                    (void*)c->d_jslotk})
         HIPCHK(hipFree(q));
+    c->d_jslotk = jk;
     c->d_slot_j = sj;
     c->d_slot_rec = sr;
     c->d_jslot = js;
@@ -3118,7 +3238,7 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
             KLAUNCH(c, "candidates", k_candidates_lane<4>, grid_for(n, 256), 256, 0, n, c->d_plan, c->d_sorted,
                     stride, c->d_sortf, c->d_sortf + stride, c->d_sid, c->d_skey,
                     c->n_rep > 0 ? c->d_rep : (const int*)nullptr, c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64,
-                    c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_counters);
+                    c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_jslotk, c->d_counters);
             break;
         case 7:
             KLAUNCH(c, "candidates", k_candidates_bins<256>, 2048, 256, 0, c->d_plan, c->d_sorted, stride, c->d_sid,
@@ -3141,7 +3261,7 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
 // fuse: the compaction also writes the pair-force records of an isotropic
 // interaction (see fused_forces_ok); n > 1 only.
 void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, double R, double Ldiv,
-                     bool fuse = false)
+                     bool fuse = false, bool compact = true)
 {
     c->transposed = false;
     c->list_n = n;
@@ -3192,6 +3312,12 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
             c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count, c->d_sortf);
     c->bins_dirty = false;
     launch_candidates(c, n, stride);
+    if (!compact) {   // per-cell lists only: no rows, no reusable list in the context
+        c->list_n = -1;
+        c->h_m = -1;
+        c->list_csr = false;
+        return;
+    }
     {
         // (the one-pass scan measured no faster than hipCUB on the n + 1 row counts)
         LaunchScope ls(c, "scan_counts");
@@ -4320,6 +4446,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_cnt64, N + 1);
         dalloc(&c->d_off64, N + 1);
         dalloc(&c->d_jslot, N * c->slot_cap);
+        dalloc(&c->d_jslotk, N * c->slot_cap);
         dalloc(&c->d_jrow, N * c->slot_cap);
         dalloc(&c->d_csr, 2 * P);
         dalloc(&c->d_cc, 6 * N);
@@ -4357,7 +4484,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sortf, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
-                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
+                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jslotk, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst, c->d_geom,
                     c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere, c->d_amap, c->d_tile_sum, c->d_scan_flag};
     for (void* q : ptrs)
@@ -4723,8 +4850,21 @@ int bfm_dev_stage(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, 
     return guarded(c, [&] {
         need_params(c);
         check_cells(c, n, ld);
-        // an isotropic interaction's pair forces come out of the compaction
         const bool fuse = n > 1 && fused_forces_ok(c);
+        if (fuse && c->cand_variant == 8 && c->p.friction_mode == 0) {
+            // an isotropic interaction without friction: velocities straight
+            // from the search's per-cell lists (no row compaction)
+            build_neighbors(c, d_cells, n, ld, thr, R, Ldiv, false, false);
+            const FusedForces ff{force_params(c), d_cells, ld, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd,
+                                 c->tab_ng, nullptr};
+            KLAUNCH(c, "pair_forces", k_slot_values, grid_for(8 * (int64_t)n, 256), 256, 0, n, ff,
+                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_status);
+            KLAUNCH(c, "velocities", k_velocities_slots, grid_for(n, 256), 256, 0, n, d_cells, ld, vel_params(c),
+                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_jslot,
+                    c->d_jslotk, d_vel, c->d_status);
+            return;
+        }
+        // an isotropic interaction's pair forces come out of the compaction
         build_neighbors(c, d_cells, n, ld, thr, R, Ldiv, fuse);
         if (!fuse) compute_forces(c, d_cells, n, ld);
         launch_velocities(c, d_cells, n, ld, nullptr, nullptr, d_vel, nullptr);

@@ -246,7 +246,12 @@ int bfm_dev_get_velocities_noalloc(const bfm_params* params, const double* d_cel
 
 /* One RK stage of a snapshot with its neighbor list rebuilt: binning, sort,
  * neighbor list, Hertz+JKR force and torque, surface forces and velocities,
- * all on device. */
+ * all on device. With an isotropic interaction (Hertz, exact or tabulated
+ * JKR), no friction and the default candidate kernel, the velocities come
+ * straight from the per-cell neighbor lists of the search, in the
+ * reference's row order for each cell, and the context keeps no reusable
+ * list (build one with bfm_dev_build_neighbors); otherwise the list is
+ * compacted into rows and kept. */
 int bfm_dev_stage(bfm_ctx* ctx, const double* d_cells, int n, int ld,
                   double neighbor_threshold, double R, double Ldiv, double* d_vel);
 
