@@ -412,6 +412,8 @@ __global__ void k_find_dividing(const double* __restrict__ cells, int ld, int n,
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
+__global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst) { *dst = *src; }
+
 __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -658,6 +660,190 @@ struct DevBuf {
     ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
+// Host memory the device writes the per-attempt readback into.
+struct MappedBuf {
+    void* h = nullptr;
+    void* d = nullptr;
+    explicit MappedBuf(size_t bytes)
+    {
+        SCHK(hipHostMalloc(&h, bytes, hipHostMallocMapped));
+        std::memset(h, 0, bytes);
+        SCHK(hipHostGetDevicePointer(&d, h, 0));
+    }
+    ~MappedBuf() { if (h) (void)hipHostFree(h); }
+};
+
+// ---------------------------------------------------------------------------
+// per-replicate step-size controller on the device
+// ---------------------------------------------------------------------------
+
+// The controller of runSimulation (simulation.hpp:1688-1769) for every
+// replicate of a batch, run by small kernels between the RK attempts of the
+// captured graph, so the host does not read the error maxima back after
+// each attempt. A replicate's iteration is a state machine: FIRST (the
+// attempt at dt), RETRY (attempts at dt * factor while the scaled error
+// exceeds 1, at most max_tries, the factor accumulated and clipped to
+// [0.2, 10], clipping ending the retries), FINAL (the unconditional re-run at
+// the clamped dt * factor, or the single attempt of an iteration without
+// step control), DONE. Each kernel sets the next attempt's dt and mask.
+enum : int { CS_DONE = 0, CS_FIRST = 1, CS_RETRY = 2, CS_FINAL = 3 };
+
+struct CtrlState {
+    double* dt;          // step size; after the iteration, the one taken
+    double* factor;      // the iteration's accumulated factor
+    double* t;           // simulated time
+    double* err;         // max(scaled error, min_error) of the last attempt
+    int* tries;          // retries counted as the reference's j
+    int* state;
+    int* stepped;        // takes this iteration's step
+    int* failed;         // frozen after a non-finite attempt
+    long long* iters;
+};
+
+struct CtrlConsts {
+    double min_error, p1, min_step, max_step;   // p1 = 1 / (order + 1)
+    int max_tries, n_rep, stages;
+};
+
+// Published after each controller kernel (host-mapped memory); per replicate
+// after the iteration: t, dt, err, stepped | failed << 1.
+struct CtrlPub {
+    int pending;          // replicates with an attempt still to take
+    int failed_total;     // replicates frozen so far
+    long long cells_total;   // cells of all attempts scheduled so far (RK-stage updates / stages)
+    int ndiv;             // cells past the division length (end of the iteration)
+    int pad;
+};
+
+__device__ __forceinline__ double ctrl_key_value(unsigned long long k)
+{
+    const unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+// Block-wide sums into pub (one block): the next attempt's replicates and
+// cells, and the replicates frozen by the last attempt.
+__device__ __forceinline__ void ctrl_publish(int pending, long long cells, int newf, CtrlPub* pub)
+{
+    __shared__ int s_p[4], s_f[4];
+    __shared__ long long s_c[4];
+    for (int off = 32; off > 0; off >>= 1) {
+        pending += __shfl_xor(pending, off);
+        cells += __shfl_xor(cells, off);
+        newf += __shfl_xor(newf, off);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_p[w] = pending; s_c[w] = cells; s_f[w] = newf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        pub->pending = (s_p[0] + s_p[1]) + (s_p[2] + s_p[3]);
+        pub->cells_total += (s_c[0] + s_c[1]) + (s_c[2] + s_c[3]);
+        pub->failed_total += (s_f[0] + s_f[1]) + (s_f[2] + s_f[3]);
+    }
+}
+
+// The iteration's first attempt: replicates with go[r] (not finished on the
+// host's counts) and not failed take dt; control selects step control.
+__global__ void __launch_bounds__(256)
+k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control, const int* __restrict__ starts,
+             double* __restrict__ trial, int* __restrict__ active, CtrlPub* pub)
+{
+    int pending = 0;
+    long long cells = 0;
+    for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
+        const bool on = go[r] && !S.failed[r];
+        S.stepped[r] = on;
+        active[r] = on;
+        S.state[r] = on ? (control ? CS_FIRST : CS_FINAL) : CS_DONE;
+        S.factor[r] = 1;
+        S.tries[r] = 0;
+        trial[r] = S.dt[r];
+        if (on) { ++pending; cells += starts[r + 1] - starts[r]; }
+    }
+    ctrl_publish(pending, cells, 0, pub);
+}
+
+// After an attempt: each replicate that took it moves on (a non-finite
+// result freezes it at the state the iteration started from).
+__global__ void __launch_bounds__(256)
+k_ctrl_after(CtrlState S, CtrlConsts K, const unsigned long long* __restrict__ errmax, int* __restrict__ bad,
+             const int* __restrict__ starts, double* __restrict__ trial, int* __restrict__ active, CtrlPub* pub)
+{
+    int pending = 0, newf = 0;
+    long long cells = 0;
+    for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
+        if (!active[r]) continue;
+        if (bad[r]) {
+            bad[r] = 0;
+            S.failed[r] = 1;
+            S.stepped[r] = 0;
+            S.state[r] = CS_DONE;
+            active[r] = 0;
+            ++newf;
+            continue;
+        }
+        const double e = fmax(ctrl_key_value(errmax[r]), K.min_error);
+        S.err[r] = e;
+        const double dt = S.dt[r];
+        int st = S.state[r];
+        bool finish = false;
+        if (st == CS_FIRST) {
+            double f = 0.9 * pow(1.0 / e, K.p1);
+            if (f >= 10) f = 10;
+            else if (f < 0.2) f = 0.2;
+            S.factor[r] = f;
+            if (e > 1.0 && 0 < K.max_tries) { st = CS_RETRY; trial[r] = dt * f; }
+            else finish = true;
+        } else if (st == CS_RETRY) {
+            double f = S.factor[r] * (0.9 * pow(1.0 / e, K.p1));
+            bool stop = false;
+            if (f >= 10) { f = 10; stop = true; }
+            else if (f < 0.2) { f = 0.2; stop = true; }
+            S.factor[r] = f;
+            int j = S.tries[r];
+            if (!stop) S.tries[r] = ++j;
+            if (!stop && e > 1.0 && j < K.max_tries) trial[r] = dt * f;
+            else finish = true;
+        } else {   // CS_FINAL: the iteration's step is taken
+            st = CS_DONE;
+            active[r] = 0;
+        }
+        if (finish) {   // clamp to [min, max] step size, the unconditional re-run
+            double f = S.factor[r];
+            if (dt * f < K.min_step) f = K.min_step / dt;
+            else if (dt * f > K.max_step) f = K.max_step / dt;
+            S.dt[r] = dt * f;
+            trial[r] = dt * f;
+            st = CS_FINAL;
+        }
+        S.state[r] = st;
+        if (active[r]) { ++pending; cells += starts[r + 1] - starts[r]; }
+    }
+    ctrl_publish(pending, cells, newf, pub);
+}
+
+// End of the iteration: time and iteration counts of the replicates that
+// stepped; the grow / static-copy mask (stepped) and dt; the per-replicate
+// values for the host.
+__global__ void __launch_bounds__(256)
+k_ctrl_end(CtrlState S, CtrlConsts K, double* __restrict__ trial, int* __restrict__ active,
+           double* __restrict__ h_t, double* __restrict__ h_dt, double* __restrict__ h_err, int* __restrict__ h_flags)
+{
+    for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
+        const int on = S.stepped[r];
+        if (on) {
+            S.t[r] += S.dt[r];
+            S.iters[r] += 1;
+        }
+        active[r] = on;
+        trial[r] = S.dt[r];
+        h_t[r] = S.t[r];
+        h_dt[r] = S.dt[r];
+        h_err[r] = S.err[r];
+        h_flags[r] = on | (S.failed[r] << 1);
+    }
+}
+
 // runSimulation for a batch of independent replicates in lockstep: every
 // iteration advances each unfinished replicate by its own dt with its own
 // controller, division stream and lineage. Replicates occupy contiguous row
@@ -711,13 +897,28 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         d_starts(R + 1), d_shift(R);
     DevBuf<unsigned long long> d_errmax(R);
     DevBuf<int> d_bad(R);
-    PinnedBuf h_dt(sizeof(double) * R), h_act(sizeof(int) * R), h_err(sizeof(unsigned long long) * R),
-        h_bad(sizeof(int) * R), h_small(64);
+    // the device controller's state and its host-mapped readback
+    DevBuf<double> d_cdt(R), d_cfactor(R), d_ct(R), d_cerr(R);
+    DevBuf<int> d_ctries(R), d_cstate(R), d_cstepped(R), d_cfailed(R), d_go(R);
+    DevBuf<long long> d_citers(R);
+    MappedBuf pub(sizeof(CtrlPub) + (size_t)R * (3 * sizeof(double) + sizeof(int)));
+    CtrlPub* h_pub = (CtrlPub*)pub.h;
+    CtrlPub* d_pub = (CtrlPub*)pub.d;
+    double* h_pt = (double*)((char*)pub.h + sizeof(CtrlPub));
+    double* h_pdt = h_pt + R;
+    double* h_perr = h_pdt + R;
+    int* h_pflags = (int*)(h_perr + R);
+    double* d_pt = (double*)((char*)pub.d + sizeof(CtrlPub));
+    double* d_pdt = d_pt + R;
+    double* d_perr = d_pdt + R;
+    int* d_pflags = (int*)(d_perr + R);
+    const CtrlState cs{d_cdt.p, d_cfactor.p, d_ct.p, d_cerr.p, d_ctries.p, d_cstate.p, d_cstepped.p, d_cfailed.p,
+                       d_citers.p};
+    PinnedBuf h_dt(sizeof(double) * R), h_act(sizeof(int) * R), h_bad(sizeof(int) * R);
     int* p_bad = (int*)h_bad.p;
     SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
     double* p_dt = (double*)h_dt.p;
     int* p_act = (int*)h_act.p;
-    unsigned long long* p_err = (unsigned long long*)h_err.p;
     double* d_cur = bufA.p;
     double* d_new = bufB.p;
     SCHK(hipMemsetAsync(d_flags.p, 0, sizeof(int) * cap, st));
@@ -743,6 +944,17 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         parents[r].assign(max_id + 1, -1);
     }
     abi(c, set_replicate_count(c, R));
+    {
+        std::vector<long long> zl(R, 0);
+        std::vector<int> zi(R, 0);
+        std::vector<double> zd(R, 0.0);
+        SCHK(hipMemcpyAsync(d_cdt.p, dt.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_ct.p, zd.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_cerr.p, zd.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_cfailed.p, zi.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_citers.p, zl.data(), sizeof(long long) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipStreamSynchronize(st));
+    }
     auto upload_layout = [&] {
         SCHK(hipMemcpyAsync(d_starts.p, start.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice, st));
         SCHK(hipMemcpyAsync(d_n.p, &n, sizeof(int), hipMemcpyHostToDevice, st));
@@ -802,56 +1014,44 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         dmin.run(q, d_cur, ld, d_flags.p, cut, out);
     };
     std::vector<char> act(R, 0);
-    auto replay = [&](const std::vector<char>& mask, const std::vector<double>& h) {
-        int64_t cells_on = 0;
-        for (int r = 0; r < R; ++r) {
-            p_act[r] = mask[r] ? 1 : 0;
-            p_dt[r] = h[r];
-            if (mask[r]) cells_on += nr[r];
-        }
-        SCHK(hipMemcpyAsync(d_dt.p, p_dt, sizeof(double) * R, hipMemcpyHostToDevice, st));
-        SCHK(hipMemcpyAsync(d_active.p, p_act, sizeof(int) * R, hipMemcpyHostToDevice, st));
-        SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
-        ++attempts;
-        stage_updates += cells_on * tb.s;
-    };
     std::vector<char> failed(R, 0);
     int n_failed = 0;
-    // Non-finite results freeze their replicate at the state the iteration
-    // started from (the reference stops such a run, simulation.hpp:1803-1822);
-    // the others go on. Returns whether a replicate failed in the last replay.
-    auto check_failures = [&](std::vector<char>& mask, bool flags_read) {
-        if (!flags_read) {
-            SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
-            SCHK(hipStreamSynchronize(st));
-        }
-        bool any_new = false;
-        for (int r = 0; r < R; ++r)
-            if (p_bad[r] && !failed[r]) {
-                failed[r] = 1;
-                done[r] = 1;
-                act[r] = 0;
-                mask[r] = 0;
-                ++n_failed;
-                any_new = true;
-            }
-        if (any_new) SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
-        return any_new;
-    };
     double last_err0 = 0;
     double last_frame_wall = 0;
     std::vector<double> frame_buf;
-    // error maxima and failure flags of the last replay (one synchronisation)
-    auto read_errors = [&](std::vector<double>& e) {
-        SCHK(hipMemcpyAsync(p_err, d_errmax.p, sizeof(unsigned long long) * R, hipMemcpyDeviceToHost, st));
-        SCHK(hipMemcpyAsync(p_bad, d_bad.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
-        SCHK(hipStreamSynchronize(st));
-        for (int r = 0; r < R; ++r) e[r] = std::max(order_key_value(p_err[r]), cfg->min_error);
-        last_err0 = e[0];
+    const CtrlConsts kc{cfg->min_error, 1.0 / (tdef.order + 1), cfg->min_stepsize, cfg->max_stepsize,
+                        cfg->max_tries_update_stepsize, R, tb.s};
+    std::vector<int> go_host(R, -1);
+    auto launch_graph = [&] {
+        SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
+        ++attempts;
     };
+    // the per-replicate values k_ctrl_end published for the iteration just
+    // finished (read after a synchronisation): time, step counts, dt trace,
+    // failures (a non-finite result freezes its replicate at the state the
+    // iteration started from; the reference stops such a run,
+    // simulation.hpp:1803-1822; the others go on)
+    auto take_iteration = [&] {
+        for (int r = 0; r < R; ++r) {
+            const int fl = h_pflags[r];
+            if (fl & 1) {
+                if (io && io->dt_trace && r == 0 && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = h_pdt[0];
+                iters[r]++;
+            }
+            t[r] = h_pt[r];
+            dt[r] = h_pdt[r];
+            if ((fl & 2) && !failed[r]) {
+                failed[r] = 1;
+                done[r] = 1;
+                ++n_failed;
+            }
+        }
+        last_err0 = h_perr[0];
+        stage_updates = h_pub->cells_total * tb.s;
+    };
+    h_pub->ndiv = -1;
     double next_report = cfg->report_seconds;
     std::vector<int> idx;
-    const double p1 = 1.0 / (tdef.order + 1);
     for (;;) {
         bool any = false;
         for (int r = 0; r < R; ++r) {
@@ -877,13 +1077,17 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             next_report += cfg->report_seconds;
         }
 
-        // divideMaxLength + divideCells per active replicate
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
-        hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
-                           d_count.p, d_idx.p);
-        SCHK(hipMemcpyAsync(h_small.p, d_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        SCHK(hipStreamSynchronize(st));
-        const int nd_all = *(int*)h_small.p;
+        // divideMaxLength + divideCells per active replicate; the count of
+        // the current cells comes with the previous iteration's readback
+        if (h_pub->ndiv < 0) {
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
+            hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
+                               d_count.p, d_idx.p);
+            hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, st, d_count.p, &d_pub->ndiv);
+            SCHK(hipStreamSynchronize(st));
+        }
+        const int nd_all = h_pub->ndiv;
+        h_pub->ndiv = -1;
         if (nd_all > 0) {
             auto td = std::chrono::steady_clock::now();
             idx.resize(nd_all);
@@ -995,64 +1199,61 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             }
         } else {
         // adaptive RK step of every active replicate (simulation.hpp:1677-1769)
+        // with the controller on the device: the first attempt, the re-run
+        // or first retry (every stepping replicate takes a second attempt when
+        // step control is on), then one synchronisation; more attempts only
+        // for replicates still retrying
         abi(c, step_prepare(c, d_cur, n, ld));
-        replay(act, dt);
-        if (iter % cfg->iter_update_stepsize == 0) {
-            std::vector<double> err(R), factor(R, 1.0), trial(R);
-            std::vector<char> exceeded(R, 0), retry(R, 0);
-            std::vector<int> j(R, 0);
-            read_errors(err);
-            check_failures(act, true);
-            bool any_retry = false;
-            for (int r = 0; r < R; ++r) {
-                if (!act[r]) continue;
-                exceeded[r] = err[r] > 1.0;
-                double f = 0.9 * pow(1.0 / err[r], p1);
-                if (f >= 10) f = 10;
-                else if (f < 0.2) f = 0.2;
-                factor[r] = f;
-                retry[r] = exceeded[r] && j[r] < cfg->max_tries_update_stepsize;
-                any_retry |= retry[r] != 0;
-            }
-            while (any_retry) {
-                for (int r = 0; r < R; ++r) trial[r] = retry[r] ? dt[r] * factor[r] : dt[r];
-                replay(retry, trial);
-                read_errors(err);
-                check_failures(retry, true);
-                any_retry = false;
-                for (int r = 0; r < R; ++r) {
-                    if (!retry[r]) continue;
-                    exceeded[r] = err[r] > 1.0;
-                    factor[r] *= 0.9 * pow(1.0 / err[r], p1);
-                    bool stop = false;
-                    if (factor[r] >= 10) { factor[r] = 10; stop = true; }
-                    else if (factor[r] < 0.2) { factor[r] = 0.2; stop = true; }
-                    if (!stop) j[r]++;
-                    retry[r] = !stop && exceeded[r] && j[r] < cfg->max_tries_update_stepsize;
-                    any_retry |= retry[r] != 0;
-                }
-            }
-            for (int r = 0; r < R; ++r) {
-                if (!act[r]) continue;
-                if (dt[r] * factor[r] < cfg->min_stepsize) factor[r] = cfg->min_stepsize / dt[r];
-                else if (dt[r] * factor[r] > cfg->max_stepsize) factor[r] = cfg->max_stepsize / dt[r];
-                dt[r] *= factor[r];
-            }
-            replay(act, dt);
+        bool go_changed = false;
+        for (int r = 0; r < R; ++r)
+            if (go_host[r] != (act[r] ? 1 : 0)) { go_host[r] = act[r] ? 1 : 0; go_changed = true; }
+        if (go_changed) {
+            for (int r = 0; r < R; ++r) p_act[r] = go_host[r];
+            SCHK(hipMemcpyAsync(d_go.p, p_act, sizeof(int) * R, hipMemcpyHostToDevice, st));
         }
-        // a replicate that failed in the accepted attempt: redo the attempt
-        // without it, so its rows are carried over unchanged
-        while (check_failures(act, false)) replay(act, dt);
-        // the columns the attempts leave alone, with the last attempt's mask (act)
+        const int control = iter % cfg->iter_update_stepsize == 0;
+        hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
+                           d_active.p, d_pub);
+        launch_graph();
+        hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
+                           d_dt.p, d_active.p, d_pub);
+        if (control) {
+            launch_graph();
+            hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
+                               d_dt.p, d_active.p, d_pub);
+        }
+        SCHK(hipGetLastError());
+        SCHK(hipStreamSynchronize(st));
+        while (h_pub->pending > 0) {
+            launch_graph();
+            hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
+                               d_dt.p, d_active.p, d_pub);
+            SCHK(hipGetLastError());
+            SCHK(hipStreamSynchronize(st));
+        }
+        hipLaunchKernelGGL(k_ctrl_end, dim3(1), dim3(256), 0, st, cs, kc, d_dt.p, d_active.p, d_pt, d_pdt, d_perr,
+                           d_pflags);
+        // the columns the attempts leave alone (all columns of replicates
+        // that did not step)
         abi(c, copy_static_batched(c, d_cur, ld, d_n.p, c->d_rep, d_active.p, d_new));
         }
         std::swap(d_cur, d_new);
         abi(c, grow_batched(c, d_cur, ld, d_n.p, Rr, c->d_rep, d_dt.p, d_active.p));
         abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
+        if (!verlet) {   // the next iteration's division count, read with this one's values
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
+            hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
+                               d_count.p, d_idx.p);
+            hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, st, d_count.p, &d_pub->ndiv);
+        }
         abi(c, bfm_sync(c));
-        if (io && io->dt_trace && act[0] && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = dt[0];
-        for (int r = 0; r < R; ++r)
-            if (act[r]) { t[r] += dt[r]; iters[r]++; }
+        if (verlet) {
+            if (io && io->dt_trace && act[0] && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = dt[0];
+            for (int r = 0; r < R; ++r)
+                if (act[r]) { t[r] += dt[r]; iters[r]++; }
+        } else {
+            take_iteration();
+        }
         iter++;
         if (iter % cfg->iter_update_neighbors == 0) build();
         // a frame when t has just passed a multiple of dt_write (simulation.hpp:2023-2063)
@@ -1312,19 +1513,6 @@ k2d_grow_count(double* __restrict__ cells, int ld, int n, double R, double dt, d
 }
 
 __global__ void k_key_init(unsigned long long* key) { *key = 0x000FFFFFFFFFFFFFull; }   // ordered key of -inf
-
-// Host memory the device writes the per-attempt readback into.
-struct MappedBuf {
-    void* h = nullptr;
-    void* d = nullptr;
-    explicit MappedBuf(size_t bytes)
-    {
-        SCHK(hipHostMalloc(&h, bytes, hipHostMallocMapped));
-        std::memset(h, 0, bytes);
-        SCHK(hipHostGetDevicePointer(&d, h, 0));
-    }
-    ~MappedBuf() { if (h) (void)hipHostFree(h); }
-};
 
 // runSimulationAdaptiveLagrangian's loop of the 2-D tree (simulation.hpp:
 // 549-765 of ABM-2d) with the cells on the device (leading dimension cap).

@@ -10,17 +10,21 @@ agrees to 4e-12 per iteration (at least 1e-9): the in-plane orientation has
 no restoring force, so rounding differences add up linearly over the
 iterations (3e-8 after 25,000 Dormand-Prince iterations). The number of RK
 step evaluations may differ by a few: each retry of the controller is a
-threshold test on the error estimate. The accepted dt sequences agree to 1e-6 relative: dt follows
+threshold test on the error estimate. The accepted dt sequences agree to 2e-6 relative: dt follows
 err^(-1/(p+1)) and err is the difference of two nearly equal weighted sums
 (Delta1 - Delta2), so its relative rounding is that of the sums times
-|Delta| / |err| (about 1e-8 here)."""
+|Delta| / |err| (about 1e-8 here). Most accepted dt are identical; the
+largest deviation sits at one ill-conditioned iteration (22,522 of the
+Dormand-Prince run: 4.5e-7 with the controller on the host, 1.2e-6 with
+the device controller, whose pow can differ from the C library's in the
+last bit)."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 TOL_STATE_PER_ITER = 4e-12
-TOL_DT = 1e-6
+TOL_DT = 2e-6
 
 
 def _ctx(cap):
