@@ -129,6 +129,7 @@ struct bfm_ctx {
     unsigned long long* d_off64 = nullptr;  // exclusive scan of d_cnt64
     int* d_jslot = nullptr;         // max_cells x slot_cap: partners with larger index
     int* d_jslotk = nullptr;        // slot_cap x max_cells: the i-role slot of each partner-list entry
+    unsigned long long* d_ordj = nullptr;   // slot_cap x max_cells: j-role rows in order, i << 32 | slot
     int* d_jrow = nullptr;          // max_cells x slot_cap: row index of each jslot pair
     int* d_csr = nullptr;           // 2 x max_pairs row indices (bit 31: j role)
     double* d_rec = nullptr;        // 8 x max_pairs: per-row {d, s, t, main, hard, 0}

@@ -2377,37 +2377,67 @@ k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restr
     }
 }
 
-// Isotropic row values of the lane-per-home search's per-cell lists: for each
-// slot entry (i = c, j = its partner) the main and hard-core terms of the
-// row, computed as the compaction computes them, stored in the last two
-// doubles of the entry's record {d, s_i, s_j, partner slot, main, hard}.
-// Eight lanes per cell.
+// Isotropic row values and row order of the lane-per-home search's per-cell
+// lists, eight lanes per cell: each i-role entry (i = c, j = its partner)
+// gets the row's main and hard-core terms, computed as the compaction
+// computes them, in the last two doubles of its record {d, s_i, s_j,
+// partner slot, main, hard}; each entry's rank among the cell's partners
+// comes from width-8 shuffles, and ord_i[rank][c] = its slot (ascending j),
+// ord_j[rank][c] = i << 32 | i's slot of the row (ascending i).
 __global__ void __launch_bounds__(256)
-k_slot_values(int n, FusedForces ff, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
-              const int* __restrict__ slot_j, double* __restrict__ slot_rec, int* __restrict__ status)
+k_slot_order(int n, FusedForces ff, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+             const int* __restrict__ slot_j, double* __restrict__ slot_rec, const int* __restrict__ jslot,
+             const int* __restrict__ jslotk, int* __restrict__ ord_i, unsigned long long* __restrict__ ord_j,
+             int* __restrict__ status)
 {
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
          g += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(g >> 3), k0 = (int)(g & 7);
-        const int ci = min((int)(cnt64[c] >> 32), slot_cap);
-        for (int k = k0; k < ci; k += 8) {
-            const int64_t e = (int64_t)k * stride + c;
-            double2* rr = reinterpret_cast<double2*>(slot_rec + 8 * e);
-            const double2 r0 = rr[0], r1 = rr[1];
-            double mm, mh, jkr;
-            isotropic_vals(ff.fp, ff.cells, ff.ld, c, slot_j[e], r0.x, r0.y, r1.x, ff.tab_ov, ff.tab_g, ff.tab_r,
-                           ff.tab_nd, ff.tab_ng, status, mm, mh, jkr);
-            rr[3] = make_double2(mm, mh);
+        const int c = (int)(g >> 3), k = (int)(g & 7);
+        const unsigned long long cv = cnt64[c];
+        const int ci = min((int)(cv >> 32), slot_cap), cj = min((int)(cv & 0xffffffffull), slot_cap);
+        for (int a = 0; a * 8 < ci; ++a) {
+            const int e = a * 8 + k;
+            const int je = e < ci ? slot_j[(int64_t)e * stride + c] : INT_MAX;
+            int rank = 0;
+            for (int b = 0; b * 8 < ci; ++b) {
+                const int eb = b * 8 + k;
+                const int jb = b == a ? je : (eb < ci ? slot_j[(int64_t)eb * stride + c] : INT_MAX);
+#pragma unroll
+                for (int f = 0; f < 8; ++f) rank += __shfl(jb, f, 8) < je;
+            }
+            if (e < ci) {
+                const int64_t se = (int64_t)e * stride + c;
+                double2* rr = reinterpret_cast<double2*>(slot_rec + 8 * se);
+                const double2 r0 = rr[0], r1 = rr[1];
+                double mm, mh, jkr;
+                isotropic_vals(ff.fp, ff.cells, ff.ld, c, je, r0.x, r0.y, r1.x, ff.tab_ov, ff.tab_g, ff.tab_r,
+                               ff.tab_nd, ff.tab_ng, status, mm, mh, jkr);
+                rr[3] = make_double2(mm, mh);
+                ord_i[(int64_t)rank * stride + c] = e;
+            }
+        }
+        for (int a = 0; a * 8 < cj; ++a) {
+            const int e = a * 8 + k;
+            const int ie = e < cj ? jslot[(int64_t)e * stride + c] : INT_MAX;
+            int rank = 0;
+            for (int b = 0; b * 8 < cj; ++b) {
+                const int eb = b * 8 + k;
+                const int ib = b == a ? ie : (eb < cj ? jslot[(int64_t)eb * stride + c] : INT_MAX);
+#pragma unroll
+                for (int f = 0; f < 8; ++f) rank += __shfl(ib, f, 8) < ie;
+            }
+            if (e < cj)
+                ord_j[(int64_t)rank * stride + c] =
+                    ((unsigned long long)(unsigned)ie << 32) | (unsigned)jslotk[(int64_t)e * stride + c];
         }
     }
 }
 
 // One slot-list row {d, s_i, s_j, partner slot, main, hard} added as
 // k_velocities adds the compacted row.
-__device__ __forceinline__ void slot_row(const double2* __restrict__ rr, bool role_i, double R, double acc[6])
+__device__ __forceinline__ void slot_row(const double2 (&r)[4], bool role_i, double R, double acc[6])
 {
-    const double2 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3];
-    add_row_vals(r0, r1, make_double2(r2.x, r3.x), make_double2(r3.y, 0.0), role_i, R, acc);
+    add_row_vals(r[0], r[1], make_double2(r[2].x, r[3].x), make_double2(r[3].y, 0.0), role_i, R, acc);
 }
 
 // getVelocities of one rebuilt stage straight from the lane-per-home search's
@@ -2415,17 +2445,16 @@ __device__ __forceinline__ void slot_row(const double2* __restrict__ rr, bool ro
 // where c is i (partners j < c: slot_j and the record at c's slots) in
 // ascending j, then those where c is j (partners i > c: jslot, the record at
 // slot jslotk of i) in ascending i -- the reference's row order for c
-// (mechanics.hpp:1438-1647) -- each row's values (k_slot_values) added as
-// k_velocities adds them, so the result is bit-identical to the compacted
-// path. The next entry is the smallest partner above the previous one
-// (partners of a cell are distinct; lists are a few entries long).
-// Isotropic interactions without friction only.
+// (mechanics.hpp:1438-1647), in the order k_slot_order wrote -- each row's
+// values added as k_velocities adds them, so the result is bit-identical to
+// the compacted path. Isotropic interactions without friction only.
 __global__ void __launch_bounds__(256)
 k_velocities_slots(int n, const double* __restrict__ cells, int ld, VelParams vp, int64_t stride,
-                   int slot_cap, const unsigned long long* __restrict__ cnt64, const int* __restrict__ slot_j,
-                   const double* __restrict__ slot_rec, const int* __restrict__ jslot,
-                   const int* __restrict__ jslotk, double* __restrict__ vel, int* __restrict__ status)
+                   int slot_cap, const unsigned long long* __restrict__ cnt64,
+                   const double* __restrict__ slot_rec, const int* __restrict__ ord_i,
+                   const unsigned long long* __restrict__ ord_j, double* __restrict__ vel, int* __restrict__ status)
 {
+    constexpr int G = 4;   // rows whose order entries, then records, are loaded before any is added
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         const unsigned long long cv = cnt64[c];
         int ci = (int)(cv >> 32), cj = (int)(cv & 0xffffffffull);
@@ -2435,28 +2464,37 @@ k_velocities_slots(int n, const double* __restrict__ cells, int ld, VelParams vp
             cj = min(cj, slot_cap);
         }
         double cc[6] = {0, 0, 0, 0, 0, 0};
-        int prev = -1;
-        for (int r = 0; r < ci; ++r) {   // rows where c is i, ascending partner j
-            int best = INT_MAX, bk = 0;
-            for (int k = 0; k < ci; ++k) {
-                const int j = slot_j[(int64_t)k * stride + c];
-                if (j > prev && j < best) { best = j; bk = k; }
+        // rows where c is i (ascending partner j), then where c is j
+        // (ascending partner i), G at a time
+        const int ct = ci + cj;
+        for (int r0 = 0; r0 < ct; r0 += G) {
+            int64_t rec_at[G];
+            bool ok[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int r = r0 + u;
+                ok[u] = false;
+                rec_at[u] = 0;
+                if (r < ci) {
+                    rec_at[u] = (int64_t)ord_i[(int64_t)r * stride + c] * stride + c;
+                    ok[u] = true;
+                } else if (r < ct) {
+                    const unsigned long long o = ord_j[(int64_t)(r - ci) * stride + c];
+                    const int si = (int)(o & 0xffffffffull);
+                    rec_at[u] = (int64_t)si * stride + (int)(o >> 32);
+                    ok[u] = si < slot_cap;   // else i's slots overflowed: reported by i's thread
+                }
             }
-            prev = best;
-            slot_row(reinterpret_cast<const double2*>(slot_rec + 8 * ((int64_t)bk * stride + c)), true, vp.sc.R, cc);
-        }
-        prev = -1;
-        for (int r = 0; r < cj; ++r) {   // rows where c is j, ascending partner i
-            int best = INT_MAX, bk = 0;
-            for (int k = 0; k < cj; ++k) {
-                const int i = jslot[(int64_t)k * stride + c];
-                if (i > prev && i < best) { best = i; bk = k; }
-            }
-            prev = best;
-            const int si = jslotk[(int64_t)bk * stride + c];
-            if (si >= slot_cap) continue;   // i's slots overflowed: reported by i's thread
-            slot_row(reinterpret_cast<const double2*>(slot_rec + 8 * ((int64_t)si * stride + best)), false, vp.sc.R,
-                     cc);
+            double2 ra[G][4];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (ok[u]) {
+                    const double2* rr = reinterpret_cast<const double2*>(slot_rec + 8 * rec_at[u]);
+                    ra[u][0] = rr[0]; ra[u][1] = rr[1]; ra[u][2] = rr[2]; ra[u][3] = rr[3];
+                }
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (ok[u]) slot_row(ra[u], r0 + u < ci, vp.sc.R, cc);
         }
         const CellVel cv6 = load_cellvel(cells, ld, c);
         const bool a2d = cv6.nz < vp.nz_threshold;
@@ -3109,11 +3147,14 @@ void grow_slots(bfm_ctx* c)
     HIPCHK(hipMalloc((void**)&js, sizeof(int) * (size_t)c->max_cells * cap));
     HIPCHK(hipMalloc((void**)&jr, sizeof(int) * (size_t)c->max_cells * cap));
     int* jk = nullptr;
+    unsigned long long* oj = nullptr;
     HIPCHK(hipMalloc((void**)&jk, sizeof(int) * (size_t)c->max_cells * cap));
+    HIPCHK(hipMalloc((void**)&oj, sizeof(unsigned long long) * (size_t)c->max_cells * cap));
     for (void* q : {(void*)c->d_slot_j, (void*)c->d_slot_rec, (void*)c->d_jslot, (void*)c->d_jrow,
-                    (void*)c->d_jslotk})
+                    (void*)c->d_jslotk, (void*)c->d_ordj})
         HIPCHK(hipFree(q));
     c->d_jslotk = jk;
+    c->d_ordj = oj;
     c->d_slot_j = sj;
     c->d_slot_rec = sr;
     c->d_jslot = js;
@@ -4447,6 +4488,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_off64, N + 1);
         dalloc(&c->d_jslot, N * c->slot_cap);
         dalloc(&c->d_jslotk, N * c->slot_cap);
+        dalloc(&c->d_ordj, N * c->slot_cap);
         dalloc(&c->d_jrow, N * c->slot_cap);
         dalloc(&c->d_csr, 2 * P);
         dalloc(&c->d_cc, 6 * N);
@@ -4484,7 +4526,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sortf, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
-                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jslotk, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
+                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jslotk, c->d_ordj, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst, c->d_geom,
                     c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere, c->d_amap, c->d_tile_sum, c->d_scan_flag};
     for (void* q : ptrs)
@@ -4857,11 +4899,12 @@ int bfm_dev_stage(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, 
             build_neighbors(c, d_cells, n, ld, thr, R, Ldiv, false, false);
             const FusedForces ff{force_params(c), d_cells, ld, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd,
                                  c->tab_ng, nullptr};
-            KLAUNCH(c, "pair_forces", k_slot_values, grid_for(8 * (int64_t)n, 256), 256, 0, n, ff,
-                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_status);
-            KLAUNCH(c, "velocities", k_velocities_slots, grid_for(n, 256), 256, 0, n, d_cells, ld, vel_params(c),
+            KLAUNCH(c, "pair_forces", k_slot_order, grid_for(8 * (int64_t)n, 256), 256, 0, n, ff,
                     (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_jslot,
-                    c->d_jslotk, d_vel, c->d_status);
+                    c->d_jslotk, c->d_jrow, c->d_ordj, c->d_status);
+            KLAUNCH(c, "velocities", k_velocities_slots, grid_for(n, 256), 256, 0, n, d_cells, ld, vel_params(c),
+                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_rec, c->d_jrow, c->d_ordj, d_vel,
+                    c->d_status);
             return;
         }
         // an isotropic interaction's pair forces come out of the compaction
