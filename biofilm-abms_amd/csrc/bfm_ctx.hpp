@@ -106,6 +106,7 @@ struct bfm_ctx {
     unsigned scan_epoch = 0;          // launch epoch of the bin-offset scan
     double* d_sorted = nullptr;     // 7 x max_cells SoA: x y z nx ny nz hl
     float4* d_sortf = nullptr;      // 2 x max_cells FP32: (x y z hl), then (nx ny nz 0), sorted order
+    double* d_sgeom = nullptr;      // max_cells x 8: segment records {r, n, half_l, 0}, sorted order
     double* d_geom = nullptr;       // max_cells x 8 packed segments (distance updates)
     int* d_sid = nullptr;           // sorted -> original index
     int* d_skey = nullptr;          // sorted keys

@@ -371,10 +371,12 @@ k_scan_onepass(const Plan* __restrict__ plan, int64_t items_arg, const T* __rest
 // It also clears the per-cell pair counters (n + 1) that the candidate kernel
 // increments next, and the histogram entries of the occupied bins (every
 // nonzero entry has a cell), which leaves the histogram zero for the next build.
+template <bool SOA>
 __global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const int* __restrict__ keys,
                           const int* __restrict__ rank, const int* __restrict__ bin_off,
                           double* __restrict__ sorted, int* __restrict__ sid, int* __restrict__ skey, int64_t stride,
-                          unsigned long long* __restrict__ cnt64, int* __restrict__ hist, float4* __restrict__ sortf)
+                          unsigned long long* __restrict__ cnt64, int* __restrict__ hist, float4* __restrict__ sortf,
+                          double* __restrict__ sgeom)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt64[n] = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -384,18 +386,27 @@ __global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const
         const int q = bin_off[key] + rank[i];
         sid[q] = i;
         skey[q] = key;
-        sorted[0 * stride + q] = cellv(cells, ld, i, C_RX);
-        sorted[1 * stride + q] = cellv(cells, ld, i, C_RY);
-        sorted[2 * stride + q] = cellv(cells, ld, i, C_RZ);
-        sorted[3 * stride + q] = cellv(cells, ld, i, C_NX);
-        sorted[4 * stride + q] = cellv(cells, ld, i, C_NY);
-        sorted[5 * stride + q] = cellv(cells, ld, i, C_NZ);
-        sorted[6 * stride + q] = cellv(cells, ld, i, C_HALF_L);
-        // FP32 copies for the lane-per-home candidate search
-        sortf[q] = make_float4((float)cellv(cells, ld, i, C_RX), (float)cellv(cells, ld, i, C_RY),
-                               (float)cellv(cells, ld, i, C_RZ), (float)cellv(cells, ld, i, C_HALF_L));
-        sortf[stride + q] = make_float4((float)cellv(cells, ld, i, C_NX), (float)cellv(cells, ld, i, C_NY),
-                                        (float)cellv(cells, ld, i, C_NZ), 0.f);
+        const double rx = cellv(cells, ld, i, C_RX), ry = cellv(cells, ld, i, C_RY), rz = cellv(cells, ld, i, C_RZ);
+        const double nx = cellv(cells, ld, i, C_NX), ny = cellv(cells, ld, i, C_NY), nz = cellv(cells, ld, i, C_NZ);
+        const double hl = cellv(cells, ld, i, C_HALF_L);
+        if constexpr (SOA) {   // the LDS-staged candidate kernels
+            sorted[0 * stride + q] = rx;
+            sorted[1 * stride + q] = ry;
+            sorted[2 * stride + q] = rz;
+            sorted[3 * stride + q] = nx;
+            sorted[4 * stride + q] = ny;
+            sorted[5 * stride + q] = nz;
+            sorted[6 * stride + q] = hl;
+        } else {   // the lane-per-home search: one 64-byte segment record per cell
+            double2* g = reinterpret_cast<double2*>(sgeom + 8 * (int64_t)q);
+            g[0] = make_double2(rx, ry);
+            g[1] = make_double2(rz, nx);
+            g[2] = make_double2(ny, nz);
+            g[3] = make_double2(hl, 0.0);
+            // FP32 copies for its screens
+            sortf[q] = make_float4((float)rx, (float)ry, (float)rz, (float)hl);
+            sortf[stride + q] = make_float4((float)nx, (float)ny, (float)nz, 0.f);
+        }
     }
 }
 
@@ -1184,6 +1195,18 @@ struct FusedForces {
     double* rec;
 };
 
+// The 64-byte segment record {r, n, half_l, 0} of sorted position q.
+__device__ __forceinline__ Seg load_geom(const double* __restrict__ g, int q)
+{
+    const double2* p = reinterpret_cast<const double2*>(g + 8 * (int64_t)q);
+    const double2 a = p[0], b = p[1], c = p[2], d = p[3];
+    Seg s;
+    s.r[0] = a.x; s.r[1] = a.y; s.r[2] = b.x;
+    s.n[0] = b.y; s.n[1] = c.x; s.n[2] = c.y;
+    s.hl = d.x;
+    return s;
+}
+
 // Number of set bits of m in the lanes below this one.
 __device__ __forceinline__ int lanes_below(uint64_t m)
 {
@@ -1234,7 +1257,7 @@ __device__ __forceinline__ bool seg_bounds_pass(float4 af, float4 an, float4 bf,
 // below 2^26 (queue entries carry the home lane in the top six bits).
 template <int UNROLL>
 __global__ void __launch_bounds__(256)
-k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict__ sorted, int64_t stride,
+k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict__ sgeom, int64_t stride,
                   const float4* __restrict__ sortf, const float4* __restrict__ sortn,
                   const int* __restrict__ sid, const int* __restrict__ skey,
                   const int* __restrict__ rep, const RepGrid* __restrict__ grid,
@@ -1295,8 +1318,8 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
             if (e < nb) {
                 const unsigned ent = qb[e];
                 const int h = (int)(ent >> QBITS), q = (int)(ent & QMASK);
-                const Seg ch = load_sorted(sorted, stride, hbase + h);
-                const Seg cc = load_sorted(sorted, stride, q);
+                const Seg ch = load_geom(sgeom, hbase + h);
+                const Seg cc = load_geom(sgeom, q);
                 const int hid_h = s_hid[wave][h], cid = sid[q];
                 const bool home_is_i = hid_h > cid;
                 const int i = home_is_i ? hid_h : cid;
@@ -3276,7 +3299,7 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
             c->d_counters)
     switch (c->cand_variant) {
         case 8:
-            KLAUNCH(c, "candidates", k_candidates_lane<4>, grid_for(n, 256), 256, 0, n, c->d_plan, c->d_sorted,
+            KLAUNCH(c, "candidates", k_candidates_lane<4>, grid_for(n, 256), 256, 0, n, c->d_plan, c->d_sgeom,
                     stride, c->d_sortf, c->d_sortf + stride, c->d_sid, c->d_skey,
                     c->n_rep > 0 ? c->d_rep : (const int*)nullptr, c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64,
                     c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_jslotk, c->d_counters);
@@ -3349,8 +3372,14 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
                 c->d_bin_off, c->d_tile_sum, c->d_scan_flag, epoch, c->d_status);
     }
     const int64_t stride = c->max_cells;
-    KLAUNCH(c, "scatter", k_scatter, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals, c->d_bin_off,
-            c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count, c->d_sortf);
+    if (c->cand_variant == 8)
+        KLAUNCH(c, "scatter", k_scatter<false>, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals,
+                c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count, c->d_sortf,
+                c->d_sgeom);
+    else
+        KLAUNCH(c, "scatter", k_scatter<true>, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals,
+                c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count, c->d_sortf,
+                c->d_sgeom);
     c->bins_dirty = false;
     launch_candidates(c, n, stride);
     if (!compact) {   // per-cell lists only: no rows, no reusable list in the context
@@ -4467,6 +4496,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         HIPCHK(hipMemset(c->d_scan_flag, 0, sizeof(unsigned) * kMaxScanTiles));
         dalloc(&c->d_sorted, N * 7);
         dalloc(&c->d_sortf, N * 2);
+        dalloc(&c->d_sgeom, N * 8);
         dalloc(&c->d_geom, N * 8);
         dalloc(&c->d_sid, N);
         dalloc(&c->d_skey, N);
@@ -4524,7 +4554,7 @@ int bfm_destroy(bfm_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_status, c->d_counters, c->d_cells, c->d_cells_new, c->d_nb, c->d_noise, c->d_out6,
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
-                    c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sortf, c->d_sid, c->d_skey, c->d_count,
+                    c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sortf, c->d_sgeom, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
                     c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jslotk, c->d_ordj, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst, c->d_geom,
