@@ -4,16 +4,17 @@
 // coalesced SoA stream, include/bfm.h). The neighbor list is SoA: int32 i, j
 // and five FP64 columns (dx, dy, dz, s, t), rows sorted (i asc, j asc).
 //
-// Pipeline of one RK stage (bfm_dev_stage):
+// Pipeline of one RK stage (bfm_dev_stage, isotropic interaction):
 //   k_bbox_partial, k_plan        bounding box and uniform-grid plan (device side)
 //   k_keys + scan + k_scatter     counting sort of the cells into bin order
-//   k_candidates_lds              half-stencil candidate screening from LDS-staged
-//                                 bins, LDS queue, segment-segment tests
-//   scan (64-bit), k_compact_rows, k_compact_jrole
-//                                 sorted (i asc, j asc) rows + per-cell row lists
-//   k_pair_forces                 Hertz / JKR magnitude per row (64-byte records)
-//   k_velocities                  per cell: forces summed in the reference's row
+//   k_candidates_lane             lane-per-home half-stencil search: FP32 centre and
+//                                 point-segment screens, FP64 segment tests, per-cell slots
+//   k_slot_order                  row values (Hertz / JKR) and each cell's row order
+//   k_velocities_slots            per cell: forces summed in the reference's row
 //                                 order, surface forces, velocity solve
+// The neighbor list proper (bfm_dev_build_neighbors) compacts the slots into
+// the reference's rows: scan (64-bit), k_compact_rows, k_compact_jrole; the
+// other interactions use k_pair_forces and k_velocities over those rows.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -166,8 +167,8 @@ __global__ void k_iota(int* __restrict__ p, int n)
 }
 
 // Block partials of min/max of the centers and max half-length.
-__global__ void __launch_bounds__(256) k_bbox_partial(const double* __restrict__ cells, int n, int ld,
-                                                      double* __restrict__ partials)
+__device__ __forceinline__ void bbox_block(const double* __restrict__ cells, int n, int ld,
+                                           double* __restrict__ partials)
 {
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     double hl = 0;
@@ -200,12 +201,18 @@ __global__ void __launch_bounds__(256) k_bbox_partial(const double* __restrict__
         for (int k = 0; k < 7; ++k) partials[blockIdx.x * 7 + k] = red[k][0];
 }
 
+__global__ void __launch_bounds__(256) k_bbox_partial(const double* __restrict__ cells, int n, int ld,
+                                                      double* __restrict__ partials)
+{
+    bbox_block(cells, n, ld, partials);
+}
+
 // Final reduction and grid plan. The search cutoff covers every pair the
 // reference can keep: |r_i - r_j| < thr + Ldiv + 2R (its prefilter) and,
 // since |d| >= |r_i - r_j| - hl_i - hl_j, |r_i - r_j| < thr + 2 max(hl).
-__global__ void __launch_bounds__(256) k_plan(const double* __restrict__ partials, int nparts,
-                                              double thr, double pre, double bin_scale, int max_bins,
-                                              Plan* __restrict__ plan, RepGrid* __restrict__ grid)
+__device__ __forceinline__ void plan_from_partials(const double* __restrict__ partials, int nparts, double thr,
+                                                   double pre, double bin_scale, int max_bins,
+                                                   Plan* __restrict__ plan, RepGrid* __restrict__ grid)
 {
     __shared__ double red[7][256];
     const int tid = threadIdx.x;
@@ -267,6 +274,13 @@ __global__ void __launch_bounds__(256) k_plan(const double* __restrict__ partial
     for (int k = 0; k < 3; ++k) { G.lo[k] = P.lo[k]; G.dims[k] = P.dims[k]; }
     G.base = 0;
     *grid = G;
+}
+
+__global__ void __launch_bounds__(256) k_plan(const double* __restrict__ partials, int nparts,
+                                              double thr, double pre, double bin_scale, int max_bins,
+                                              Plan* __restrict__ plan, RepGrid* __restrict__ grid)
+{
+    plan_from_partials(partials, nparts, thr, pre, bin_scale, max_bins, plan, grid);
 }
 
 __device__ __forceinline__ int bin_coord(double x, double lo, double edge, int dim)
