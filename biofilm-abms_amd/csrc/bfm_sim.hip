@@ -12,6 +12,7 @@
 // unchanged population (a device kernel, exact-safe screening) and the part
 // among the rewritten rows (host, the same segment arithmetic).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -186,7 +187,7 @@ Query query_of_row(const double* row, int self, int lo, int hi, int skip_dividin
 }
 
 // |d| of distBetweenCells(query, other), as minDistToCell (growth.hpp:128-162).
-double seg_dist(const Seg& a, const Seg& b)
+__host__ __device__ inline double seg_dist(const Seg& a, const Seg& b)
 {
     double d[3], s, t;
     dist_between_cells(a, b, d, s, t);
@@ -844,6 +845,466 @@ k_ctrl_end(CtrlState S, CtrlConsts K, double* __restrict__ trial, int* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// divideCells on the device (growth.hpp:456-720), all replicates at once
+// ---------------------------------------------------------------------------
+
+// Each replicate's MT19937 stream lives in device memory (624 words and the
+// index) and is advanced by one wavefront: every lane computes the same draws
+// (wave-uniform control flow), the 624-word twist is split over the lanes in
+// the three phases its recurrence allows (words 0-226 read only old words,
+// 227-453 read words 0-226 of this twist, 454-622 read words 227-395, and
+// word 623 reads words 0 and 396), so the stream equals the sequential
+// generator's. uniform_01 = x 2^-32 and the Box-Muller normal follow the host
+// restatement (utils.hpp:648-655).
+constexpr int kMtWords = 625;   // 624 state words + the index
+
+__device__ __forceinline__ void wave_sync_sim()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct WaveMt {
+    uint32_t* mt;   // 624 words in LDS
+    int idx;        // uniform across the wavefront
+    __device__ void twist(int lane)
+    {
+        auto step = [&](int i) __attribute__((always_inline)) {
+            const uint32_t y = (mt[i] & 0x80000000u) | (mt[(i + 1) % 624] & 0x7fffffffu);
+            uint32_t v = mt[(i + 397) % 624] ^ (y >> 1);
+            if (y & 1u) v ^= 0x9908b0dfu;
+            return v;
+        };
+        for (int b = 0; b < 227; b += 64) {
+            const int i = b + lane;
+            uint32_t v = 0;
+            if (i < 227) v = step(i);
+            wave_sync_sim();
+            if (i < 227) mt[i] = v;
+            wave_sync_sim();
+        }
+        for (int b = 227; b < 454; b += 64) {
+            const int i = b + lane;
+            uint32_t v = 0;
+            if (i < 454) v = step(i);
+            wave_sync_sim();
+            if (i < 454) mt[i] = v;
+            wave_sync_sim();
+        }
+        for (int b = 454; b < 623; b += 64) {
+            const int i = b + lane;
+            uint32_t v = 0;
+            if (i < 623) v = step(i);
+            wave_sync_sim();
+            if (i < 623) mt[i] = v;
+            wave_sync_sim();
+        }
+        uint32_t v = step(623);
+        wave_sync_sim();
+        if (lane == 0) mt[623] = v;
+        wave_sync_sim();
+        idx = 0;
+    }
+    __device__ uint32_t next(int lane)
+    {
+        if (idx >= 624) twist(lane);
+        uint32_t y = mt[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+    __device__ double uniform01(int lane) { return (double)next(lane) * (1.0 / 4294967296.0); }
+    __device__ double normal(int lane)
+    {
+        const double u = uniform01(lane);
+        const double v = uniform01(lane);
+        const double c = sqrt(-2 * log(u));
+        return c * cos(6.283185307179586476925286766559 * v);
+    }
+};
+
+// The distributions of runSimulation (simulation.hpp:1294-1339) and the
+// geometry of divideCells.
+struct DivParams {
+    double R, cut;                       // cut = 2 Rcell
+    double gmean[2], gstd[2];
+    double dl_std, axy, az;
+    int ncols;
+};
+
+// The dividing rows of every replicate that steps, ascending, from the
+// sorted list of rows past the division length: job r covers
+// dlist[doff[r] .. doff[r] + nd[r]); rows of replicates that do not step are
+// left out (nd = 0). One block.
+__global__ void __launch_bounds__(256)
+k_div_jobs(int R, const int* __restrict__ starts, const int* __restrict__ go, const int* __restrict__ sorted_rows,
+           int nd_all, int* __restrict__ doff, int* __restrict__ nd, int* __restrict__ ntries,
+           int* __restrict__ jobact)
+{
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        int lo = 0, hi = nd_all;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sorted_rows[mid] < starts[r]) lo = mid + 1; else hi = mid;
+        }
+        int e = lo;
+        hi = nd_all;
+        while (e < hi) {
+            const int mid = (e + hi) >> 1;
+            if (sorted_rows[mid] < starts[r + 1]) e = mid + 1; else hi = mid;
+        }
+        doff[r] = lo;
+        nd[r] = go[r] ? e - lo : 0;
+        ntries[r] = 0;
+        jobact[r] = nd[r] > 0;
+    }
+}
+
+__device__ __forceinline__ int replicate_of(const int* __restrict__ starts, int R, int row)
+{
+    int lo = 0, hi = R - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The no-room check's queries (growth.hpp:480-520): each dividing cell's
+// segment against every other cell of its replicate; flags mark the rows
+// being divided (left out of the placement checks).
+__global__ void k_div_start(int R, int nd_all, const int* __restrict__ starts, const int* __restrict__ sorted_rows,
+                            const int* __restrict__ doff, const int* __restrict__ nd, const double* __restrict__ cells,
+                            int ld, int* __restrict__ flags, int* __restrict__ check, int* __restrict__ sat,
+                            double* __restrict__ q, int4* __restrict__ meta)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nd_all) return;
+    const int row = sorted_rows[k];
+    const int r = replicate_of(starts, R, row);
+    const bool on = k - doff[r] < nd[r];
+    check[k] = 1;
+    sat[k] = 0;
+    if (on) flags[row] = 1;
+    for (int a = 0; a < 3; ++a) {
+        q[(int64_t)k * 7 + a] = cells[(int64_t)(C_RX + a) * ld + row];
+        q[(int64_t)k * 7 + 3 + a] = cells[(int64_t)(C_NX + a) * ld + row];
+    }
+    q[(int64_t)k * 7 + 6] = cells[(int64_t)C_HALF_L * ld + row];
+    meta[k] = on ? make_int4(row, starts[r], starts[r + 1], 0) : make_int4(-1, 0, 0, 0);
+}
+
+__device__ __forceinline__ double dist_key_value(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+__global__ void k_div_noroom(int nd_all, const unsigned long long* __restrict__ md, double cut,
+                             int* __restrict__ check, int* __restrict__ sat)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nd_all) return;
+    if (dist_key_value(md[k]) < cut) check[k] = 0;
+    if (check[k] == 0) sat[k] = 1;
+}
+
+// rotateXY and rotateOutOfXY (utils.hpp:791-829), as the host restatement.
+__device__ __forceinline__ void dev_rotate_xy(const double n[3], double theta, double out[3])
+{
+    double s = sin(theta), c = cos(theta);
+    out[0] = (c * n[0] + (-s) * n[1]) + 0.0 * n[2];
+    out[1] = (s * n[0] + c * n[1]) + 0.0 * n[2];
+    out[2] = (0.0 * n[0] + 0.0 * n[1]) + 1.0 * n[2];
+}
+
+__device__ __forceinline__ void dev_rotate_out_of_xy(const double n[3], double theta, double out[3])
+{
+    double v0 = n[1] * 1.0 - n[2] * 0.0;
+    double v1 = n[2] * 0.0 - n[0] * 1.0;
+    double v2 = n[0] * 0.0 - n[1] * 0.0;
+    double nrm = sqrt((v0 * v0 + v1 * v1) + v2 * v2);
+    v0 /= nrm; v1 /= nrm; v2 /= nrm;
+    double s = sin(theta), c = cos(theta);
+    double w0 = v1 * n[2] - v2 * n[1];
+    double w1 = v2 * n[0] - v0 * n[2];
+    double w2 = v0 * n[1] - v1 * n[0];
+    out[0] = n[0] * c + w0 * s;
+    out[1] = n[1] * c + w1 * s;
+    out[2] = n[2] * c + w2 * s;
+}
+
+// One placement attempt per active job (one wavefront per replicate): the
+// re-orientation draws (four per cell with room, none on the last try), the
+// length ratios, the mothers' then the daughters' growth rates, in the
+// reference's order; the proposal rows mo / da (column-major, leading
+// dimension ldp) and the queries of the cells with room (2k: mother, 2k + 1:
+// daughter) against the unchanged rows of the replicate.
+__global__ void __launch_bounds__(64)
+k_div_propose(int R, DivParams P, double t_unused, const int* __restrict__ starts,
+              const int* __restrict__ sorted_rows, const int* __restrict__ doff, const int* __restrict__ nd,
+              const int* __restrict__ ntries, const int* __restrict__ jobact, const int* __restrict__ check,
+              const double* __restrict__ t_rep, const double* __restrict__ cells, int ld,
+              uint32_t* __restrict__ mt_state, double* __restrict__ mo, double* __restrict__ da, int ldp,
+              double* __restrict__ q, int4* __restrict__ meta, int* __restrict__ status)
+{
+    __shared__ uint32_t s_mt[624];
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= R) return;
+    const int n0 = doff[r], m = nd[r], tries = ntries[r];
+    if (!jobact[r]) {   // a finished job asks nothing
+        for (int i = lane; i < m; i += 64) {
+            meta[2 * (int64_t)(n0 + i)] = make_int4(-1, 0, 0, 0);
+            meta[2 * (int64_t)(n0 + i) + 1] = make_int4(-1, 0, 0, 0);
+        }
+        return;
+    }
+    uint32_t* gst = mt_state + (int64_t)r * kMtWords;
+    for (int i = lane; i < 624; i += 64) s_mt[i] = gst[i];
+    WaveMt g{s_mt, (int)gst[624]};
+    wave_sync_sim();
+    const double t = t_rep[r];
+    // rows start as the mothers' originals
+    for (int col = 0; col < P.ncols; ++col)
+        for (int i = lane; i < m; i += 64) {
+            const double v = cells[(int64_t)col * ld + sorted_rows[n0 + i]];
+            mo[(int64_t)col * ldp + n0 + i] = v;
+            da[(int64_t)col * ldp + n0 + i] = v;
+        }
+    wave_sync_sim();
+    // re-orientation draws, then the rotations (lane i % 64 writes cell i)
+    for (int i = 0; i < m; ++i) {
+        double txy1 = 0, tz1 = 0, txy2 = 0, tz2 = 0;
+        if (tries < 10 - 1 && check[n0 + i] == 1) {
+            txy1 = -P.axy + 2 * P.axy * g.uniform01(lane);
+            tz1 = -P.az + 2 * P.az * g.uniform01(lane);
+            txy2 = -P.axy + 2 * P.axy * g.uniform01(lane);
+            tz2 = -P.az + 2 * P.az * g.uniform01(lane);
+        }
+        if (lane == (i & 63)) {
+            const int64_t k = n0 + i;
+            const double u[3] = {mo[(int64_t)C_NX * ldp + k], mo[(int64_t)C_NY * ldp + k], mo[(int64_t)C_NZ * ldp + k]};
+            double w[3], v[3];
+            dev_rotate_xy(u, txy1, w);
+            dev_rotate_out_of_xy(w, tz1, v);
+            mo[(int64_t)C_NX * ldp + k] = v[0]; mo[(int64_t)C_NY * ldp + k] = v[1]; mo[(int64_t)C_NZ * ldp + k] = v[2];
+            dev_rotate_xy(u, txy2, w);
+            dev_rotate_out_of_xy(w, tz2, v);
+            da[(int64_t)C_NX * ldp + k] = v[0]; da[(int64_t)C_NY * ldp + k] = v[1]; da[(int64_t)C_NZ * ldp + k] = v[2];
+        }
+    }
+    // length ratios (clipped N(0.5, std)), positions and lengths
+    for (int i = 0; i < m; ++i) {
+        double M = 0.5 + P.dl_std * g.normal(lane);
+        M = M < 0 ? 0.0 : (M > 1 ? 1.0 : M);
+        if (lane == (i & 63)) {
+            const int64_t k = n0 + i;
+            const double l = mo[(int64_t)C_L * ldp + k], hl = mo[(int64_t)C_HALF_L * ldp + k];
+            const double Ld = l - 2 * P.R;
+            const double L1 = M * Ld, L2 = (1 - M) * Ld;
+            const double div = -hl + L1 + P.R;
+            const double delta1 = L1 / 2 + P.R, delta2 = L2 / 2 + P.R;
+            mo[(int64_t)C_L * ldp + k] = L1;
+            mo[(int64_t)C_HALF_L * ldp + k] = L1 / 2;
+            da[(int64_t)C_L * ldp + k] = L2;
+            da[(int64_t)C_HALF_L * ldp + k] = L2 / 2;
+            for (int a = 0; a < 3; ++a) {
+                const double mn = mo[(int64_t)(C_NX + a) * ldp + k], dn = da[(int64_t)(C_NX + a) * ldp + k];
+                mo[(int64_t)(C_RX + a) * ldp + k] = mo[(int64_t)(C_RX + a) * ldp + k] + (div - delta1) * mn;
+                da[(int64_t)(C_RX + a) * ldp + k] = da[(int64_t)(C_RX + a) * ldp + k] + (div + delta2) * dn;
+            }
+            mo[(int64_t)C_T0 * ldp + k] = t;
+            da[(int64_t)C_T0 * ldp + k] = t;
+        }
+    }
+    // growth rates: the mothers', then the daughters'
+    for (int pass = 0; pass < 2; ++pass) {
+        double* rows = pass == 0 ? mo : da;
+        for (int i = 0; i < m; ++i) {
+            const int64_t k = n0 + i;
+            const int group = (int)rows[(int64_t)C_GROUP * ldp + k];
+            const bool okg = group == 1 || group == 2;
+            const double z = g.normal(lane);
+            if (lane == (i & 63)) {
+                if (okg) rows[(int64_t)C_GROWTH * ldp + k] = P.gmean[group - 1] + P.gstd[group - 1] * z;
+                else atomicOr(status, 1);
+            }
+        }
+    }
+    wave_sync_sim();
+    for (int i = lane; i < 624; i += 64) gst[i] = s_mt[i];
+    if (lane == 0) gst[624] = (uint32_t)g.idx;
+    // queries of the cells with room
+    for (int i = lane; i < m; i += 64) {
+        const int64_t k = n0 + i;
+        for (int h = 0; h < 2; ++h) {
+            const double* rows = h == 0 ? mo : da;
+            const int64_t qk = 2 * k + h;
+            for (int a = 0; a < 3; ++a) {
+                q[qk * 7 + a] = rows[(int64_t)(C_RX + a) * ldp + k];
+                q[qk * 7 + 3 + a] = rows[(int64_t)(C_NX + a) * ldp + k];
+            }
+            q[qk * 7 + 6] = rows[(int64_t)C_HALF_L * ldp + k];
+            meta[qk] = check[k] == 1 ? make_int4(-1, starts[r], starts[r + 1], 1) : make_int4(-1, 0, 0, 0);
+        }
+    }
+}
+
+__device__ __forceinline__ Seg seg_of_prop(const double* __restrict__ rows, int ldp, int64_t k)
+{
+    Seg s;
+    s.r[0] = rows[(int64_t)C_RX * ldp + k]; s.r[1] = rows[(int64_t)C_RY * ldp + k];
+    s.r[2] = rows[(int64_t)C_RZ * ldp + k];
+    s.n[0] = rows[(int64_t)C_NX * ldp + k]; s.n[1] = rows[(int64_t)C_NY * ldp + k];
+    s.n[2] = rows[(int64_t)C_NZ * ldp + k];
+    s.hl = rows[(int64_t)C_HALF_L * ldp + k];
+    return s;
+}
+
+// Completes an attempt (one wavefront per active job): each cell with room
+// takes the minimum over the other new rows of its replicate as well, and
+// is satisfied when its mother and daughter are both at least cut from
+// everything; the job goes on while a cell is unsatisfied and fewer than ten
+// attempts were made. any_active counts the jobs that go on.
+__global__ void __launch_bounds__(64)
+k_div_eval(int R, double cut, const int* __restrict__ doff, const int* __restrict__ nd, int* __restrict__ ntries,
+           int* __restrict__ jobact, const int* __restrict__ check, int* __restrict__ sat,
+           const double* __restrict__ mo, const double* __restrict__ da, int ldp,
+           const unsigned long long* __restrict__ md, int* __restrict__ any_active)
+{
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= R || !jobact[r]) return;
+    const int n0 = doff[r], m = nd[r];
+    int unsat = 0;
+    for (int i = lane; i < m; i += 64) {
+        const int64_t k = n0 + i;
+        if (check[k] == 1) {
+            double best[2];
+            for (int h = 0; h < 2; ++h) {
+                const Seg a = seg_of_prop(h == 0 ? mo : da, ldp, k);
+                double b = dist_key_value(md[2 * k + h]);
+                for (int x = 0; x < 2 * m; ++x) {
+                    if (x == i + h * m) continue;
+                    const Seg o = seg_of_prop(x < m ? mo : da, ldp, n0 + (x < m ? x : x - m));
+                    if (beyond_cut(a, o, cut)) continue;
+                    const double dd = seg_dist(a, o);
+                    if (dd < b) b = dd;
+                }
+                best[h] = b;
+            }
+            if (best[0] >= cut && best[1] >= cut) sat[k] = 1;
+        }
+        unsat += sat[k] == 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) unsat += __shfl_xor(unsat, off);
+    if (lane == 0) {
+        const int tr = ntries[r] + 1;
+        ntries[r] = tr;
+        const int on = tr < 10 && unsat > 0;
+        jobact[r] = on;
+        if (on) atomicAdd(any_active, 1);
+    }
+}
+
+// New ids (growth.hpp:700-712: the mother of pair i gets max_id + 2i + 1,
+// the daughter max_id + 2i + 2, both the parent's id as parent), appended to
+// the lineage log as (replicate, id, parent); the new layout (each replicate
+// gains its daughters at its end) and the row shifts. One block.
+__global__ void __launch_bounds__(256)
+k_div_layout(int R, const int* __restrict__ doff, const int* __restrict__ nd, double* __restrict__ mo,
+             double* __restrict__ da, int ldp, long long* __restrict__ next_id, int* __restrict__ log_n,
+             int4* __restrict__ lineage, int log_cap, int* __restrict__ starts, int* __restrict__ shift,
+             int* __restrict__ old_end, int* __restrict__ d_n, int* __restrict__ sizes_out, int* __restrict__ status)
+{
+    __shared__ int s_tot[256];
+    // ids and lineage
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        const int m = nd[r], n0 = doff[r];
+        if (m == 0) continue;
+        const long long base = next_id[r];
+        const int pos = atomicAdd(log_n, 2 * m);
+        for (int i = 0; i < m; ++i) {
+            const int64_t k = n0 + i;
+            const int parent = (int)mo[(int64_t)C_ID * ldp + k];
+            const long long id1 = base + 2 * i, id2 = base + 2 * i + 1;
+            mo[(int64_t)C_ID * ldp + k] = (double)id1;
+            da[(int64_t)C_ID * ldp + k] = (double)id2;
+            if (pos + 2 * i + 1 < log_cap) {
+                lineage[pos + 2 * i] = make_int4(r, (int)id1, parent, 0);
+                lineage[pos + 2 * i + 1] = make_int4(r, (int)id2, parent, 0);
+            } else {
+                atomicOr(status, 2);
+            }
+        }
+        next_id[r] = base + 2 * m;
+    }
+    // new starts: an exclusive scan of the new sizes, in chunks of the block
+    __syncthreads();
+    int carry = 0;
+    for (int r0 = 0; r0 < R; r0 += blockDim.x) {
+        const int r = r0 + threadIdx.x;
+        const int sz = r < R ? (starts[r + 1] - starts[r]) + nd[r] : 0;
+        s_tot[threadIdx.x] = sz;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+            const int v = threadIdx.x >= (unsigned)o ? s_tot[threadIdx.x - o] : 0;
+            __syncthreads();
+            s_tot[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const int incl = s_tot[threadIdx.x];
+        const int chunk = s_tot[blockDim.x - 1];
+        __syncthreads();
+        if (r < R) {
+            const int ns = carry + incl - sz;
+            shift[r] = ns - starts[r];
+            old_end[r] = starts[r + 1];
+            sizes_out[r] = sz;
+        }
+        carry += chunk;
+        __syncthreads();
+    }
+    __syncthreads();
+    // starts <- new starts (after every thread has read the old ones)
+    for (int r = threadIdx.x; r < R; r += blockDim.x) starts[r] = starts[r] + shift[r];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        starts[R] = carry;
+        *d_n = carry;
+    }
+}
+
+// Writes the rewritten mothers at their shifted rows and the daughters at
+// the end of their replicate (old end + shift + i), and clears the dividing
+// flags of the old rows.
+__global__ void k_div_place(int R, int nd_all, int ncols, const int* __restrict__ sorted_rows,
+                            const int* __restrict__ starts_new, const int* __restrict__ doff,
+                            const int* __restrict__ nd, const int* __restrict__ shift, const int* __restrict__ old_end,
+                            const double* __restrict__ mo, const double* __restrict__ da, int ldp,
+                            double* __restrict__ dst, int ld, int* __restrict__ flags)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nd_all) return;
+    const int row = sorted_rows[k];
+    flags[row] = 0;
+    // the replicate of the old row: shifted starts are new starts, so search
+    // the old ends (ascending)
+    int lo = 0, hi = R - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (old_end[mid] <= row) lo = mid + 1; else hi = mid;
+    }
+    const int r = lo;
+    const int i = k - doff[r];
+    if (i < 0 || i >= nd[r]) return;
+    const int64_t mrow = row + shift[r], drow = old_end[r] + shift[r] + i;
+    for (int col = 0; col < ncols; ++col) {
+        dst[(int64_t)col * ld + mrow] = mo[(int64_t)col * ldp + k];
+        dst[(int64_t)col * ld + drow] = da[(int64_t)col * ldp + k];
+    }
+    (void)starts_new;
+}
+
 // runSimulation for a batch of independent replicates in lockstep: every
 // iteration advances each unfinished replicate by its own dt with its own
 // controller, division stream and lineage. Replicates occupy contiguous row
@@ -892,9 +1353,8 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     const int ld = cap;
     int n = (int)total64;
 
-    DevBuf<double> bufA((size_t)ld * ncols), bufB((size_t)ld * ncols), d_dt(R), d_rows((size_t)2 * cap * ncols);
-    DevBuf<int> d_n(1), d_active(R), d_flags(cap), d_idx(cap), d_count(1), d_rowidx(2 * (size_t)cap),
-        d_starts(R + 1), d_shift(R);
+    DevBuf<double> bufA((size_t)ld * ncols), bufB((size_t)ld * ncols), d_dt(R);
+    DevBuf<int> d_n(1), d_active(R), d_flags(cap), d_idx(cap), d_count(1), d_starts(R + 1), d_shift(R);
     DevBuf<unsigned long long> d_errmax(R);
     DevBuf<int> d_bad(R);
     // the device controller's state and its host-mapped readback
@@ -944,6 +1404,43 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         parents[r].assign(max_id + 1, -1);
     }
     abi(c, set_replicate_count(c, R));
+    // division state on the device: each replicate's random stream, its next
+    // cell id and the lineage log (replicate, id, parent)
+    DevBuf<uint32_t> d_mt((size_t)R * kMtWords);
+    DevBuf<long long> d_next_id(R);
+    const int log_cap = 2 * cap + 16;
+    DevBuf<int4> d_lineage(log_cap);
+    DevBuf<int> d_log_n(1);
+    int log_read = 0;
+    {
+        std::vector<uint32_t> mt((size_t)R * kMtWords);
+        std::vector<long long> nid(R);
+        for (int r = 0; r < R; ++r) {
+            std::memcpy(&mt[(size_t)r * kMtWords], rng[r].mt, sizeof(uint32_t) * 624);
+            mt[(size_t)r * kMtWords + 624] = (uint32_t)rng[r].idx;
+            nid[r] = (long long)parents[r].size();
+        }
+        SCHK(hipMemcpyAsync(d_mt.p, mt.data(), sizeof(uint32_t) * mt.size(), hipMemcpyHostToDevice, st));
+        SCHK(hipMemcpyAsync(d_next_id.p, nid.data(), sizeof(long long) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipMemsetAsync(d_log_n.p, 0, sizeof(int), st));
+        SCHK(hipStreamSynchronize(st));
+    }
+    // appends the lineage the device logged since the last call to parents
+    auto take_lineage = [&] {
+        int nlog = 0;
+        SCHK(hipMemcpyAsync(&nlog, d_log_n.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCHK(hipStreamSynchronize(st));
+        if (nlog > log_cap) sim_fail(BFM_ERR_CAPACITY, "lineage log overflow");
+        if (nlog <= log_read) return;
+        std::vector<int4> ent(nlog - log_read);
+        SCHK(hipMemcpy(ent.data(), d_lineage.p + log_read, sizeof(int4) * ent.size(), hipMemcpyDeviceToHost));
+        for (const int4& e : ent) {
+            std::vector<int>& pr = parents[e.x];
+            if ((int)pr.size() <= e.y) pr.resize(e.y + 1, -1);
+            pr[e.y] = e.z;
+        }
+        log_read = nlog;
+    };
     {
         std::vector<long long> zl(R, 0);
         std::vector<int> zi(R, 0);
@@ -1009,10 +1506,54 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     }
     c->timing = was_timing;
 
-    DeviceMinDist dmin(2 * cap, st);
-    BaseMinFn base_min = [&](const std::vector<Query>& q, double cut, std::vector<double>& out) {
-        dmin.run(q, d_cur, ld, d_flags.p, cut, out);
+    // device division buffers, grown to the largest batch of dividing cells
+    struct DivBufs {
+        int cap = 0;
+        double *mo = nullptr, *da = nullptr, *q = nullptr;
+        int4* meta = nullptr;
+        unsigned long long* md = nullptr;
+        int *sorted = nullptr, *check = nullptr, *sat = nullptr;
+        void* temp = nullptr;
+        size_t temp_bytes = 0;
+        ~DivBufs() { release(); }
+        void release()
+        {
+            for (void* p : {(void*)mo, (void*)da, (void*)q, (void*)meta, (void*)md, (void*)sorted, (void*)check,
+                            (void*)sat, temp})
+                if (p) (void)hipFree(p);
+            mo = da = q = nullptr;
+            meta = nullptr;
+            md = nullptr;
+            sorted = check = sat = nullptr;
+            temp = nullptr;
+        }
+    } dv;
+    DevBuf<int> d_doff(R), d_nd(R), d_ntries(R), d_jobact(R), d_old_end(R), d_divstat(1);
+    MappedBuf divpub(sizeof(int) * (4 + (size_t)R));
+    int* h_div = (int*)divpub.h;    // [0] jobs still placing, [1] status, [2..] new sizes
+    int* d_divp = (int*)divpub.d;
+    const int key_bits = std::max(1, (int)std::ceil(std::log2((double)cap + 1)));
+    auto ensure_div = [&](int nd_all) {
+        if (nd_all <= dv.cap) return;
+        SCHK(hipStreamSynchronize(st));
+        dv.release();
+        const int k = std::max(nd_all, 64);
+        SCHK(hipMalloc((void**)&dv.mo, sizeof(double) * (size_t)k * ncols));
+        SCHK(hipMalloc((void**)&dv.da, sizeof(double) * (size_t)k * ncols));
+        SCHK(hipMalloc((void**)&dv.q, sizeof(double) * 7 * 2 * (size_t)k));
+        SCHK(hipMalloc((void**)&dv.meta, sizeof(int4) * 2 * (size_t)k));
+        SCHK(hipMalloc((void**)&dv.md, sizeof(unsigned long long) * 2 * (size_t)k));
+        SCHK(hipMalloc((void**)&dv.sorted, sizeof(int) * (size_t)k));
+        SCHK(hipMalloc((void**)&dv.check, sizeof(int) * (size_t)k));
+        SCHK(hipMalloc((void**)&dv.sat, sizeof(int) * (size_t)k));
+        dv.temp_bytes = 0;
+        SCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, dv.temp_bytes, d_idx.p, dv.sorted, k, 0, key_bits, st));
+        SCHK(hipMalloc(&dv.temp, std::max<size_t>(dv.temp_bytes, 16)));
+        dv.cap = k;
     };
+    const DivParams dpar{cfg->R, 2 * cfg->Rcell, {cfg->growth_mean[0], cfg->growth_mean[1]},
+                         {cfg->growth_std[0], cfg->growth_std[1]}, cfg->daughter_length_std,
+                         cfg->daughter_angle_xy_bound, cfg->daughter_angle_z_bound, ncols};
     std::vector<char> act(R, 0);
     std::vector<char> failed(R, 0);
     int n_failed = 0;
@@ -1022,6 +1563,17 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     const CtrlConsts kc{cfg->min_error, 1.0 / (tdef.order + 1), cfg->min_stepsize, cfg->max_stepsize,
                         cfg->max_tries_update_stepsize, R, tb.s};
     std::vector<int> go_host(R, -1);
+    // the replicates that take this iteration (the host's termination rules)
+    auto upload_go = [&] {
+        bool changed = false;
+        for (int r = 0; r < R; ++r)
+            if (go_host[r] != (act[r] ? 1 : 0)) { go_host[r] = act[r] ? 1 : 0; changed = true; }
+        if (changed) {
+            SCHK(hipStreamSynchronize(st));   // p_act may still feed an earlier copy
+            for (int r = 0; r < R; ++r) p_act[r] = go_host[r];
+            SCHK(hipMemcpyAsync(d_go.p, p_act, sizeof(int) * R, hipMemcpyHostToDevice, st));
+        }
+    };
     auto launch_graph = [&] {
         SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
         ++attempts;
@@ -1089,97 +1641,77 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         const int nd_all = h_pub->ndiv;
         h_pub->ndiv = -1;
         if (nd_all > 0) {
+            // divideCells of every replicate that steps, on the device
             auto td = std::chrono::steady_clock::now();
-            idx.resize(nd_all);
-            SCHK(hipMemcpyAsync(idx.data(), d_idx.p, sizeof(int) * nd_all, hipMemcpyDeviceToHost, st));
-            SCHK(hipStreamSynchronize(st));
-            std::sort(idx.begin(), idx.end());
-            std::vector<int> keep, keep_rep;
-            for (int i : idx) {
-                const int r = (int)(std::upper_bound(start.begin(), start.end(), i) - start.begin()) - 1;
-                if (act[r]) { keep.push_back(i); keep_rep.push_back(r); }
+            upload_go();
+            ensure_div(nd_all);
+            size_t tb_bytes = dv.temp_bytes;
+            SCHK(hipcub::DeviceRadixSort::SortKeys(dv.temp, tb_bytes, d_idx.p, dv.sorted, nd_all, 0, key_bits, st));
+            hipLaunchKernelGGL(k_div_jobs, dim3(1), dim3(256), 0, st, R, d_starts.p, d_go.p, dv.sorted, nd_all,
+                               d_doff.p, d_nd.p, d_ntries.p, d_jobact.p);
+            const int nb = blocks_for(nd_all, 256);
+            hipLaunchKernelGGL(k_div_start, dim3(nb), dim3(256), 0, st, R, nd_all, d_starts.p, dv.sorted, d_doff.p,
+                               d_nd.p, d_cur, ld, d_flags.p, dv.check, dv.sat, dv.q, dv.meta);
+            int span = 1;
+            for (int r = 0; r < R; ++r) span = std::max(span, nr[r]);
+            const int gx = std::min(blocks_for(span, 256), 64);
+            hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, st, dv.md, nd_all,
+                               (unsigned long long)0x7FF0000000000000ull);
+            // minimum distances of nq queries, in launches of at most 65,535 (the grid's y limit)
+            auto min_dist = [&](int nq) {
+                for (int q0 = 0; q0 < nq; q0 += 65535)
+                    hipLaunchKernelGGL(k_min_dist, dim3(gx, std::min(65535, nq - q0)), dim3(256), 0, st, d_cur, ld,
+                                       dv.q + 7 * (size_t)q0, dv.meta + q0, d_flags.p, dpar.cut, dv.md + q0);
+            };
+            min_dist(nd_all);
+            hipLaunchKernelGGL(k_div_noroom, dim3(nb), dim3(256), 0, st, nd_all, dv.md, dpar.cut, dv.check, dv.sat);
+            SCHK(hipMemsetAsync(d_divstat.p, 0, sizeof(int), st));
+            for (int round = 0; round < 10; ++round) {   // placement attempts until every job is done
+                hipLaunchKernelGGL(k_div_propose, dim3(R), dim3(64), 0, st, R, dpar, 0.0, d_starts.p, dv.sorted,
+                                   d_doff.p, d_nd.p, d_ntries.p, d_jobact.p, dv.check, d_ct.p, d_cur, ld, d_mt.p,
+                                   dv.mo, dv.da, dv.cap, dv.q, dv.meta, d_divstat.p);
+                hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(2 * nd_all, 256)), dim3(256), 0, st, dv.md,
+                                   2 * nd_all, (unsigned long long)0x7FF0000000000000ull);
+                min_dist(2 * nd_all);
+                hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_divp);
+                hipLaunchKernelGGL(k_div_eval, dim3(R), dim3(64), 0, st, R, dpar.cut, d_doff.p, d_nd.p, d_ntries.p,
+                                   d_jobact.p, dv.check, dv.sat, dv.mo, dv.da, dv.cap, dv.md, d_divp);
+                SCHK(hipGetLastError());
+                SCHK(hipStreamSynchronize(st));
+                if (h_div[0] == 0) break;
             }
-            const int nd = (int)keep.size();
+            // dividing cells of the stepping replicates (the capacity check
+            // comes before anything is written)
+            std::vector<int> ndr(R, 0);
+            SCHK(hipMemcpyAsync(ndr.data(), d_nd.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+            int dstat = 0;
+            SCHK(hipMemcpyAsync(&dstat, d_divstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            SCHK(hipStreamSynchronize(st));
+            if (dstat & 1) sim_fail(BFM_ERR_ARG, "cell group must be 1 or 2");
+            int nd = 0;
+            for (int r = 0; r < R; ++r) nd += ndr[r];
             if (nd > 0) {
                 if (n + nd > cap) sim_fail(BFM_ERR_CAPACITY, "division would exceed the cell capacity");
-                SCHK(hipMemcpyAsync(d_idx.p, keep.data(), sizeof(int) * nd, hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for((int64_t)nd * ncols, 256)), dim3(256), 0, st,
-                                   d_cur, ld, ncols, d_idx.p, nd, d_rows.p);
-                hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p,
-                                   nd, 1);
-                std::vector<double> colm((size_t)nd * ncols);
-                SCHK(hipMemcpyAsync(colm.data(), d_rows.p, sizeof(double) * colm.size(), hipMemcpyDeviceToHost,
-                                    st));
-                SCHK(hipStreamSynchronize(st));
-                std::vector<DivisionJob> jobs;
-                std::vector<int> job_of(R, -1);
-                for (int k = 0; k < nd; ++k) {
-                    const int r = keep_rep[k];
-                    if (job_of[r] < 0) {
-                        job_of[r] = (int)jobs.size();
-                        jobs.emplace_back();
-                        DivisionJob& jb = jobs.back();
-                        jb.rng = &rng[r];
-                        jb.parents = &parents[r];
-                        jb.t = t[r];
-                        jb.lo = start[r];
-                        jb.hi = start[r + 1];
-                    }
-                    DivisionJob& jb = jobs[job_of[r]];
-                    jb.idx.push_back(keep[k]);
-                    for (int col = 0; col < ncols; ++col) jb.mrows.push_back(colm[(size_t)col * nd + k]);
-                }
-                divide_batch(cfg, ncols, jobs, base_min);
-                // re-pack so that every replicate gains its daughters at its end
-                std::vector<int> nstart(R + 1, 0), shift(R), ndr(R, 0);
-                for (auto& jb : jobs) ndr[(int)(std::upper_bound(start.begin(), start.end(), jb.lo) - start.begin()) - 1] = jb.nd;
-                for (int r = 0; r < R; ++r) {
-                    nstart[r + 1] = nstart[r] + nr[r] + ndr[r];
-                    shift[r] = nstart[r] - start[r];
-                }
-                SCHK(hipMemcpyAsync(d_shift.p, shift.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_div_layout, dim3(1), dim3(256), 0, st, R, d_doff.p, d_nd.p, dv.mo, dv.da,
+                                   dv.cap, d_next_id.p, d_log_n.p, d_lineage.p, log_cap, d_starts.p, d_shift.p,
+                                   d_old_end.p, d_n.p, d_divp + 2, d_divstat.p);
                 hipLaunchKernelGGL(k_repack, dim3(std::min(blocks_for((int64_t)n * ncols, 256), 8192)), dim3(256), 0,
                                    st, d_cur, d_new, ld, ncols, n, c->d_rep, d_shift.p);
-                std::vector<int> dst;
-                std::vector<double> rows((size_t)2 * nd * ncols);
-                dst.reserve(2 * nd);
-                int k = 0;
-                for (auto& jb : jobs) {
-                    const int r = (int)(std::upper_bound(start.begin(), start.end(), jb.lo) - start.begin()) - 1;
-                    for (int i = 0; i < jb.nd; ++i, ++k) dst.push_back(jb.idx[i] + shift[r]);
-                }
-                for (auto& jb : jobs) {
-                    const int r = (int)(std::upper_bound(start.begin(), start.end(), jb.lo) - start.begin()) - 1;
-                    for (int i = 0; i < jb.nd; ++i) dst.push_back(nstart[r] + nr[r] + i);
-                }
-                k = 0;
-                for (auto& jb : jobs)
-                    for (int i = 0; i < jb.nd; ++i, ++k)
-                        for (int col = 0; col < ncols; ++col)
-                            rows[(size_t)col * 2 * nd + k] = jb.mo[(size_t)i * ncols + col];
-                for (auto& jb : jobs)
-                    for (int i = 0; i < jb.nd; ++i, ++k)
-                        for (int col = 0; col < ncols; ++col)
-                            rows[(size_t)col * 2 * nd + k] = jb.da[(size_t)i * ncols + col];
-                SCHK(hipMemcpyAsync(d_rowidx.p, dst.data(), sizeof(int) * dst.size(), hipMemcpyHostToDevice, st));
-                SCHK(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice,
-                                    st));
-                hipLaunchKernelGGL(k_scatter_rows, dim3(blocks_for((int64_t)2 * nd * ncols, 256)), dim3(256), 0, st,
-                                   d_new, ld, ncols, d_rowidx.p, 2 * nd, d_rows.p);
-                hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(nd, 256)), dim3(256), 0, st, d_flags.p, d_idx.p,
-                                   nd, 0);
+                hipLaunchKernelGGL(k_div_place, dim3(nb), dim3(256), 0, st, R, nd_all, ncols, dv.sorted, d_starts.p,
+                                   d_doff.p, d_nd.p, d_shift.p, d_old_end.p, dv.mo, dv.da, dv.cap, d_new, ld,
+                                   d_flags.p);
+                hipLaunchKernelGGL(k_fill_rep, dim3(R), dim3(256), 0, st, c->d_rep, d_starts.p);
                 SCHK(hipGetLastError());
+                SCHK(hipStreamSynchronize(st));
                 std::swap(d_cur, d_new);
-                for (int r = 0; r < R; ++r) nr[r] += ndr[r];
-                start = nstart;
-                n += nd;
+                for (int r = 0; r < R; ++r) nr[r] = h_div[2 + r];
+                for (int r = 0; r < R; ++r) start[r + 1] = start[r] + nr[r];
+                n = start[R];
                 divisions += nd;
-                upload_layout();
                 div_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - td).count();
                 build();
             }
         }
-
         if (verlet) {
             // stepVelocityVerlet (simulation.hpp:1787-1798); it leaves the list's
             // distances at the new positions, refreshed again after growth below
@@ -1204,13 +1736,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         // step control is on), then one synchronisation; more attempts only
         // for replicates still retrying
         abi(c, step_prepare(c, d_cur, n, ld));
-        bool go_changed = false;
-        for (int r = 0; r < R; ++r)
-            if (go_host[r] != (act[r] ? 1 : 0)) { go_host[r] = act[r] ? 1 : 0; go_changed = true; }
-        if (go_changed) {
-            for (int r = 0; r < R; ++r) p_act[r] = go_host[r];
-            SCHK(hipMemcpyAsync(d_go.p, p_act, sizeof(int) * R, hipMemcpyHostToDevice, st));
-        }
+        upload_go();
         const int control = iter % cfg->iter_update_stepsize == 0;
         hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
                            d_active.p, d_pub);
@@ -1266,6 +1792,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
                                       sizeof(double) * n, ncols, hipMemcpyDeviceToHost, st));
                 SCHK(hipStreamSynchronize(st));
                 const double now = wall();
+                take_lineage();
                 io->hook->fn(frame_buf.data(), n, n, iters[0], t[0], dt[0], last_err0, now - last_frame_wall,
                              parents[0]);
                 last_frame_wall = now;
@@ -1273,6 +1800,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         }
     }
     abi(c, bfm_sync(c));
+    take_lineage();
     SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, d_cur, sizeof(double) * ld, sizeof(double) * n, ncols,
                           hipMemcpyDeviceToHost, st));
     SCHK(hipStreamSynchronize(st));
