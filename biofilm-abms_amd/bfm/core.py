@@ -220,6 +220,7 @@ def load_library():
         "bfm_divide_cells": [vp, _dp, _ip, C.c_int, C.c_int, C.c_double, C.c_uint64, _ip, _ip,
                              C.c_int, _ip],
         "bfm_rng_uniform": [C.c_uint64, C.c_int, _dp],
+        "bfm_rng_draws_device": [C.c_int, C.c_uint64, C.c_int, _dp],
         "bfm_jkr_equilibrium_distance": [C.c_double] * 7 + [C.c_int, _dp, _dp],
         "bfm_jkr_optimal_surface_energy_density": [C.c_double] * 9 + [C.c_int, C.c_int, _dp, _dp],
         "bfm_ensemble_simulate": [vp, vp, C.c_int, _ip, _dp, C.c_int, vp, C.c_int, _dp, C.c_int, _ip,

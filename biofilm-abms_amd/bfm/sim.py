@@ -149,6 +149,17 @@ def divide_cells(cfg: SimConfig, cells, t, seed, parents=None):
     return np.asfortranarray(buf[:n_io.value]), par[:n_par.value].copy(), tries.value
 
 
+def rng_draws_device(seed, count, device=0):
+    """count uniform_01 draws, then count standard normals, of the division
+    random stream as the device generator of the growth loop draws them."""
+    lib = load_library()
+    out = np.zeros(2 * count)
+    rc = lib.bfm_rng_draws_device(device, seed, count, out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise BFMError(rc, "bfm_rng_draws_device failed")
+    return out[:count], out[count:]
+
+
 def rng_uniform(seed, count):
     """The first `count` uniform_01 draws of the division random stream."""
     lib = load_library()

@@ -2363,6 +2363,50 @@ int bfm_rng_uniform(uint64_t seed, int count, double* out)
     return BFM_OK;
 }
 
+namespace {
+// count uniform_01 draws, then count standard normals, of one stream on
+// the device generator (one wavefront)
+__global__ void __launch_bounds__(64)
+k_rng_draws(uint32_t* __restrict__ state, int count, double* __restrict__ out)
+{
+    __shared__ uint32_t s_mt[624];
+    const int lane = threadIdx.x;
+    for (int i = lane; i < 624; i += 64) s_mt[i] = state[i];
+    WaveMt g{s_mt, (int)state[624]};
+    wave_sync_sim();
+    for (int k = 0; k < count; ++k) {
+        const double u = g.uniform01(lane);
+        if (lane == 0) out[k] = u;
+    }
+    for (int k = 0; k < count; ++k) {
+        const double z = g.normal(lane);
+        if (lane == 0) out[count + k] = z;
+    }
+}
+}   // namespace
+
+int bfm_rng_draws_device(int device, uint64_t seed, int count, double* out)
+{
+    if (count < 0 || (count > 0 && !out)) return BFM_ERR_ARG;
+    if (count == 0) return BFM_OK;
+    try {
+        SCHK(hipSetDevice(device));
+        Mt19937 g((uint32_t)seed);
+        std::vector<uint32_t> st(kMtWords);
+        std::memcpy(st.data(), g.mt, sizeof(uint32_t) * 624);
+        st[624] = (uint32_t)g.idx;
+        DevBuf<uint32_t> d_st(kMtWords);
+        DevBuf<double> d_out(2 * (size_t)count);
+        SCHK(hipMemcpy(d_st.p, st.data(), sizeof(uint32_t) * kMtWords, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_rng_draws, dim3(1), dim3(64), 0, 0, d_st.p, count, d_out.p);
+        SCHK(hipGetLastError());
+        SCHK(hipMemcpy(out, d_out.p, sizeof(double) * 2 * count, hipMemcpyDeviceToHost));
+    } catch (const SimError& e) {
+        return e.code;
+    }
+    return BFM_OK;
+}
+
 int bfm2d_simulate(bfm_ctx* c, const bfm_sim_config* cfg, const int* adhesion_map, int n_groups,
                    const double* cells_in, int n_in, const int* parents_in, int n_parents_in, double* cells_out,
                    int cap, int* n_out, int* parents_out, int* n_parents_out, double* dt_trace, int cap_trace,

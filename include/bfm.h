@@ -326,6 +326,13 @@ int bfm_divide_cells(const bfm_sim_config* cfg, double* cells, int* n, int cap, 
 /* The first `count` uniform_01 draws of the random stream seeded with seed. */
 int bfm_rng_uniform(uint64_t seed, int count, double* out);
 
+/* The same stream drawn by the device generator of the growth loop's
+ * divisions (one wavefront on GPU `device`): out[0 .. count) the first count
+ * uniform_01 draws, out[count .. 2 count) the next count standard normals
+ * (Box-Muller, two draws each). No counterpart in the reference: a check of
+ * the device stream against the host one for callers and tests. */
+int bfm_rng_draws_device(int device, uint64_t seed, int count, double* out);
+
 /* ---- JKR setup (host only, no device) ------------------------------------ */
 /* jkrEquilibriumDistance (jkr.hpp:1310-1340): the centerline distance 2R - delta
  * at which the JKR force of two cells of radius R (equivalent radius R / 2)
