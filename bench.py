@@ -25,18 +25,17 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X FP64, vector and matrix alike
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector
 # Operations per unit of work of the candidate kernel (adds, muls, divides
 # and square roots each 1): the FP64 segment test of every pair that passed
-# the FP32 pre-screen (the reference's prefilter, closest points, |d|), and
-# the FP32 pre-screen of every screened pair (|r_ij|^2 and the bound).
-TEST_FLOPS = 80        # FP64, per queued pair
+# both FP32 screens (the reference's prefilter, closest points, |d|), and the
+# FP32 centre-distance screen of every screened pair (|r_ij|^2 and the bound).
+TEST_FLOPS = 80        # FP64, per tested pair
 SCREEN_FLOPS = 12      # FP32, per screened pair
 
 
 # HBM traffic per launch (FETCH_SIZE and WRITE_SIZE, copy-calibrated) of the
 # stage kernels, from the committed rocprofv3 summary of the same workload.
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stage_200k_dome_kernels.md")
-KERNEL_SYMBOL = {"candidates": "k_candidates_lds", "compact_rows": "k_compact_rows",
-                 "velocities": "k_velocities", "pair_forces": "k_pair_forces",
-                 "scatter": "k_scatter", "compact_jrole": "k_compact_jrole"}
+KERNEL_SYMBOL = {"candidates": "k_candidates_lane", "velocities": "k_velocities_slots",
+                 "pair_forces": "k_slot_order", "scatter": "k_scatter", "keys": "k_keys"}
 
 
 def profiled_traffic(kernel, cells):
@@ -317,14 +316,17 @@ def main():
     kt = ctx.kernel_times()
     st = ctx.stats()
     ctx.enable_timing(False)
-    m = st["m"]
+    # the pair count from the neighbor list proper (the stage itself keeps
+    # only per-cell lists), built after the timed work
+    m = ctx.dev_build_neighbors(d_cells.data_ptr(), n, n, thr, R, Ldiv)
+    ctx.stats()
     per_kernel = {k: v[0] / v[1] for k, v in kt.items()}
     dom = max(kt, key=lambda k: kt[k][0])
     tests_per_launch = st["segment_tests"] / args.steps
     screened_per_launch = st["screened"] / args.steps
     if dom == "candidates":
-        # FP64 work only: the segment tests of the pairs that passed the FP32
-        # pre-screen; the pre-screen itself is FP32 and reported beside it
+        # FP64 work only: the segment tests of the pairs that passed the two
+        # FP32 screens; the centre-distance screen is FP32 and reported beside it
         t_s = per_kernel[dom] * 1e-3
         flops64 = tests_per_launch * TEST_FLOPS
         achieved = flops64 / t_s / 1e12
@@ -333,16 +335,17 @@ def main():
                     unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
                     traffic=profiled_traffic(dom, n),
                     engine="FP64 vector ALU: MI355X's FP64 vector and matrix peaks are both 78.6 "
-                           "TFLOP/s; the kernel is bound by the latency of its load-screen-test chain, "
-                           "not by either peak",
+                           "TFLOP/s; the kernel is bound by its two returning device-scope atomics per kept "
+                           "pair (the per-cell slots) and the latency of its screen-test chain, not by "
+                           "either peak",
                     avg_us=per_kernel[dom] * 1e3, fp64_flops_per_launch=flops64,
                     tests_per_launch=tests_per_launch,
                     fp32_screen={"screened_per_launch": screened_per_launch,
                                  "flops_per_launch": screened_per_launch * SCREEN_FLOPS,
                                  "achieved_tflops": screen32, "peak_tflops": FP32_PEAK_TFLOPS,
                                  "frac": screen32 / FP32_PEAK_TFLOPS},
-                    hbm={"algorithmic_bytes": n * 56 + m * 48,
-                         "achieved_gbs": (n * 56 + m * 48) / t_s / 1e9, "peak_gbs": HBM_PEAK_GBS})
+                    hbm={"algorithmic_bytes": n * 104 + m * 60,
+                         "achieved_gbs": (n * 104 + m * 60) / t_s / 1e9, "peak_gbs": HBM_PEAK_GBS})
     else:
         # HBM-bound per-cell / per-pair kernels: algorithmic bytes 176 per cell + 48 per pair
         bytes_ = n * 176 + m * 48
