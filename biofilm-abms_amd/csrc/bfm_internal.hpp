@@ -21,12 +21,30 @@ int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld);
 // the update, with the cell count read from d_n and each replicate's dt and
 // active flag from d_dt / d_active (inactive rows are copied unchanged);
 // d_errmax[r] receives replicate r's scaled-error maximum as an ordered key
-// (reset inside); d_bad[r] is or-ed with 1 when replicate r produced a
-// non-finite position or orientation (the caller clears it). Enqueues only
-// kernels, so a stream capture records it.
+// (atomic maxima: the caller resets it to the key of -inf before the
+// attempt); d_bad[r] is or-ed with 1 when replicate r produced a non-finite
+// position or orientation (the caller clears it). Enqueues only kernels, so
+// a stream capture records it. With an isotropic interaction and no
+// friction the attempt is one launch.
 int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld, const int* d_n,
                          const int* d_rep, const double* d_dt, const int* d_active, int n_rep,
                          double* d_cells_new, double tol, unsigned long long* d_errmax, int* d_bad);
+
+// The end of a batched RK iteration in one launch: copy_static_batched into
+// d_cells_new, then growCells on d_cells_new with d_dt / d_active (as
+// grow_batched), and the indices of the grown cells with l > Ldiv appended
+// to d_idx through *d_count (which the caller zeroes first); n (the host's
+// count, equal to *d_n) sizes the grid. Nothing when *d_skip != 0.
+// refresh_distances: then updateNeighborDistances of the internal list at
+// the new state (as bfm_dev_update_neighbor_distances on d_cells_new).
+int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
+                           const int* d_active, const double* d_dt, double R, double Ldiv, double* d_cells_new,
+                           int* d_count, int* d_idx, const int* d_skip, bool refresh_distances);
+
+// Whether a batched RK attempt is one launch that reads only the step's
+// frozen cell-cell sums (an isotropic interaction without friction), not
+// the neighbor list.
+bool attempt_fused(const bfm_ctx* c);
 
 // growCells with per-replicate dt and active flags, cell count from d_n.
 // The columns the attempts do not write (all of them for inactive

@@ -2871,6 +2871,163 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
     flush();
 }
 
+// One whole RK attempt of batched replicates when no stage needs another
+// cell: with an isotropic interaction, no friction and no noise every
+// stage's cell-cell forces are the step's frozen sums cc, so a cell's stages
+// depend only on its own earlier stages. Per cell, the s stages of k_rk_stage
+// one after the other (the stage velocities go to kbuf, written and read back
+// by the same thread), then the update and the per-replicate scaled-error
+// maximum of k_rk_final_b, with the same arithmetic and the same row chunks:
+// the staged attempt's result, in one launch instead of s + 1.
+__global__ void __launch_bounds__(256)
+k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int ld, VelParams vp, Tableau tb,
+               const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
+               const double* __restrict__ cc, double* __restrict__ kbuf, double* __restrict__ cells_new,
+               double tol, unsigned long long* __restrict__ errmax, int* __restrict__ bad_rep)
+{
+    __shared__ double s_val[256];
+    __shared__ int s_rep[256];
+    const int n = *dn;
+    const int t = threadIdx.x, lane = t & 63, w0 = t & ~63;
+    const int64_t chunk = (((int64_t)n + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+    const int64_t b0 = blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : (int64_t)n;
+    double acc = -INFINITY;
+    int wr = -1;   // replicate of the wavefront's running maximum (wavefront-uniform)
+    auto flush = [&] {
+        if (wr < 0) return;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off));
+        if (lane == 0) atomicMax(&errmax[wr], order_key(acc));
+    };
+    for (int64_t base = b0; base < b1; base += 256) {
+        const int c = (int)(base + t);
+        double local = -INFINITY;
+        int r = -1;
+        if (c < b1) {
+            r = rep[c];
+            if (!active[r]) {
+                r = -1;   // rows of inactive replicates: k_copy_static_b after the attempts
+            } else {
+                const double dt = dt_rep[r];
+                const bool a2d = cellv(cells, ld, c, C_NZ) < vp.nz_threshold;
+                const double sigma0 = cellv(cells, ld, c, C_SIGMA0);
+                double ccl[6];
+                const double nz6[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
+                for (int st = 0; st < tb.s; ++st) {
+                    CellVel cv = load_cellvel(cells, ld, c);
+                    if (st > 0) {
+                        double y[6];
+                        stage_coords(cells, ld, n, c, kbuf, tb, st, dt, y);
+                        cv.rz = y[2]; cv.nx = y[3]; cv.ny = y[4]; cv.nz = y[5];
+                    }
+                    double v[6];
+                    cell_velocity(vp, cv, sigma0, a2d, ccl, false, nz6, v);
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) kbuf[((int64_t)st * 6 + q) * n + c] = v[q];
+                }
+                double y[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    double v1 = 0, v2 = 0;
+                    for (int st = 0; st < tb.s; ++st) {
+                        const double kv = kbuf[((int64_t)st * 6 + q) * n + c];
+                        v1 += kv * tb.b[st];
+                        v2 += kv * tb.bs[st];
+                    }
+                    const double d1 = v1 * dt, d2 = v2 * dt;
+                    const double y0 = cellv(cells, ld, c, C_RX + q);
+                    y[q] = y0 + d1;
+                    cells_new[(int64_t)(C_DRX + q) * ld + c] = v1;
+                    local = fmax(local, (d1 - d2) / (tol * (1.0 + fabs(y0))));
+                }
+                double nrm = norm3(y[3], y[4], y[5]);
+                y[3] /= nrm; y[4] /= nrm; y[5] /= nrm;
+                if (y[5] < 0) { y[3] *= -1; y[4] *= -1; y[5] *= -1; }
+                bool bad = false;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    cells_new[(int64_t)(C_RX + q) * ld + c] = y[q];
+                    bad |= !isfinite(y[q]);
+                }
+                if (bad) atomicOr(&bad_rep[r], 1);
+            }
+        }
+        const int r0 = __shfl(r, 0);
+        const bool uniform = __all(r == r0);
+        if (uniform && r0 == wr) {
+            acc = fmax(acc, local);
+            continue;
+        }
+        flush();
+        acc = -INFINITY;
+        wr = -1;
+        if (uniform) {
+            if (r0 >= 0) {
+                wr = r0;
+                acc = local;
+            }
+        } else {
+            s_val[t] = local;
+            s_rep[t] = r;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (r >= 0 && (lane == 0 || s_rep[t - 1] != r)) {
+                double m = local;
+                for (int u = t + 1; u < w0 + 64 && s_rep[u] == r; ++u) m = fmax(m, s_val[u]);
+                atomicMax(&errmax[r], order_key(m));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    flush();
+}
+
+// The end of a batched iteration in one launch, for every cell: the columns
+// the RK attempts leave alone (as k_copy_static_b), growCells on the new
+// array (as k_grow_b, growth.hpp:51-58) and the rows past the division
+// length (divideMaxLength's test, growth.hpp:68-73) appended to idx with
+// count (zeroed by the caller); geom (when not null) receives the new
+// state's packed segments as k_pack_segs writes them. Nothing when *skip != 0.
+__global__ void __launch_bounds__(256)
+k_iter_tail_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld,
+              const int* __restrict__ rep, const int* __restrict__ active, const double* __restrict__ dt_rep,
+              double R, double Ldiv, double* __restrict__ cells_new, int* __restrict__ count, int* __restrict__ idx,
+              double* __restrict__ geom, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    const int n = *dn;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const int r = rep[c];
+        const bool on = active[r] != 0;
+        for (int col = 0; col < ncols; ++col)
+            if (!on || col < 1 || col > 12) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
+        double l = cellv(cells, ld, c, C_L);
+        double hl = cellv(cells, ld, c, C_HALF_L);
+        if (on) {
+            const double g = cellv(cells, ld, c, C_GROWTH);
+            l = l + (g * (4 * R / 3 + l)) * dt_rep[r];
+            hl = l / 2;
+            cells_new[(int64_t)C_L * ld + c] = l;
+            cells_new[(int64_t)C_HALF_L * ld + c] = hl;
+        }
+        if (l > Ldiv) idx[atomicAdd(count, 1)] = c;
+        if (geom) {
+            const double* src = on ? cells_new : cells;   // the new state's coordinates
+            double2* gp = reinterpret_cast<double2*>(geom + (int64_t)c * 8);
+            gp[0] = make_double2(cellv(src, ld, c, C_RX), cellv(src, ld, c, C_RY));
+            gp[1] = make_double2(cellv(src, ld, c, C_RZ), cellv(src, ld, c, C_NX));
+            gp[2] = make_double2(cellv(src, ld, c, C_NY), cellv(src, ld, c, C_NZ));
+            gp[3] = make_double2(hl, 0.0);
+        }
+    }
+}
+
 // The columns the RK attempts do not write, once per iteration after them:
 // all columns but the coordinates and velocities (1-12) of active replicates'
 // cells, every column of the other cells (finished or failed replicates keep
@@ -4329,6 +4486,14 @@ int step_verlet(bfm_ctx* c, double dt, double rho, const double* d_cells, int n,
     });
 }
 
+// Whether a batched RK attempt runs as one launch (k_rk_attempt_b): every
+// stage's cell-cell forces are the step's frozen sums (no anisotropic
+// tables, no friction).
+bool attempt_fused(const bfm_ctx* c)
+{
+    return c->p.adhesion_mode != BFM_ADHESION_JKR_ANISOTROPIC && !friction_on(c);
+}
+
 int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld)
 {
     return guarded(c, [&] {
@@ -4336,8 +4501,9 @@ int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld)
         check_cells(c, n, ld);
         require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
         compute_forces(c, d_cells, n, ld);
-        KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
-                c->d_assume2d);
+        if (!attempt_fused(c))   // the one-launch attempt tests nz itself
+            KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
+                    c->d_assume2d);
         launch_velocities(c, d_cells, n, ld, c->d_assume2d, nullptr, nullptr, c->d_cc);
     });
 }
@@ -4348,14 +4514,37 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
 {
     return guarded(c, [&] {
         require(tol > 0, BFM_ERR_ARG, "tol must be > 0");
+        require(n_rep >= 1, BFM_ERR_ARG, "n_rep must be >= 1");
         VelParams vp = vel_params(c);
-        hipLaunchKernelGGL(k_fill_key, dim3(grid_for(n_rep, 256)), dim3(256), 0, c->stream, d_errmax, n_rep,
-                           -INFINITY);
-        for (int st = 0; st < tb.s; ++st)
-            launch_stage(c, 0, d_n, d_cells, ld, nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active, kBatchGrid, false);
-        hipLaunchKernelGGL(k_rk_final_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld,
-                           tb, d_rep, d_dt, d_active, c->d_k, d_cells_new, tol, d_errmax, d_bad);
+        if (attempt_fused(c)) {
+            hipLaunchKernelGGL(k_rk_attempt_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, d_cells, ld, vp, tb,
+                               d_rep, d_dt, d_active, c->d_cc, c->d_k, d_cells_new, tol, d_errmax, d_bad);
+        } else {
+            for (int st = 0; st < tb.s; ++st)
+                launch_stage(c, 0, d_n, d_cells, ld, nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active, kBatchGrid,
+                             false);
+            hipLaunchKernelGGL(k_rk_final_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells,
+                               ld, tb, d_rep, d_dt, d_active, c->d_k, d_cells_new, tol, d_errmax, d_bad);
+        }
         HIPCHK(hipGetLastError());
+    });
+}
+
+int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
+                           const int* d_active, const double* d_dt, double R, double Ldiv, double* d_cells_new,
+                           int* d_count, int* d_idx, const int* d_skip, bool refresh_distances)
+{
+    return guarded(c, [&] {
+        check_cells(c, n, ld);
+        if (refresh_distances)
+            require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
+        hipLaunchKernelGGL(k_iter_tail_b, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells,
+                           ld, d_rep, d_active, d_dt, R, Ldiv, d_cells_new, d_count, d_idx,
+                           refresh_distances ? c->d_geom : (double*)nullptr, d_skip);
+        HIPCHK(hipGetLastError());
+        if (refresh_distances && n > 0)
+            KLAUNCH(c, "update_distances", k_update_distances_packed, kPairGrid, 256, 0, c->d_m, c->d_geom, n,
+                    c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
     });
 }
 

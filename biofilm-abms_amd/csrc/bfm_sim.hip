@@ -699,6 +699,7 @@ struct CtrlState {
     int* stepped;        // takes this iteration's step
     int* failed;         // frozen after a non-finite attempt
     long long* iters;
+    int* pending;        // one int: replicates with an attempt still to take (device copy)
 };
 
 struct CtrlConsts {
@@ -713,8 +714,16 @@ struct CtrlPub {
     int failed_total;     // replicates frozen so far
     long long cells_total;   // cells of all attempts scheduled so far (RK-stage updates / stages)
     int ndiv;             // cells past the division length (end of the iteration)
-    int pad;
+    int status;           // the context's status flags at the end of the iteration
 };
+
+// The iteration's last kernel: the next division count and the context's
+// status flags to the host-mapped block, read after one synchronisation.
+__global__ void k_publish_end(const int* __restrict__ div_count, const int* __restrict__ status, CtrlPub* pub)
+{
+    pub->ndiv = *div_count;
+    pub->status = *status;
+}
 
 __device__ __forceinline__ double ctrl_key_value(unsigned long long k)
 {
@@ -724,7 +733,8 @@ __device__ __forceinline__ double ctrl_key_value(unsigned long long k)
 
 // Block-wide sums into pub (one block): the next attempt's replicates and
 // cells, and the replicates frozen by the last attempt.
-__device__ __forceinline__ void ctrl_publish(int pending, long long cells, int newf, CtrlPub* pub)
+__device__ __forceinline__ void ctrl_publish(int pending, long long cells, int newf, CtrlPub* pub,
+                                             int* dev_pending)
 {
     __shared__ int s_p[4], s_f[4];
     __shared__ long long s_c[4];
@@ -738,20 +748,27 @@ __device__ __forceinline__ void ctrl_publish(int pending, long long cells, int n
     __syncthreads();
     if (threadIdx.x == 0) {
         pub->pending = (s_p[0] + s_p[1]) + (s_p[2] + s_p[3]);
+        *dev_pending = pub->pending;
         pub->cells_total += (s_c[0] + s_c[1]) + (s_c[2] + s_c[3]);
         pub->failed_total += (s_f[0] + s_f[1]) + (s_f[2] + s_f[3]);
     }
 }
 
+// The ordered key of -inf, the start of an attempt's scaled-error maximum
+// (the attempts take atomic maxima of ordered keys).
+constexpr unsigned long long kKeyNegInf = 0x000FFFFFFFFFFFFFull;
+
 // The iteration's first attempt: replicates with go[r] (not finished on the
 // host's counts) and not failed take dt; control selects step control.
 __global__ void __launch_bounds__(256)
 k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control, const int* __restrict__ starts,
-             double* __restrict__ trial, int* __restrict__ active, CtrlPub* pub)
+             double* __restrict__ trial, int* __restrict__ active, unsigned long long* __restrict__ errmax,
+             CtrlPub* pub)
 {
     int pending = 0;
     long long cells = 0;
     for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
+        errmax[r] = kKeyNegInf;
         const bool on = go[r] && !S.failed[r];
         S.stepped[r] = on;
         active[r] = on;
@@ -761,19 +778,22 @@ k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control,
         trial[r] = S.dt[r];
         if (on) { ++pending; cells += starts[r + 1] - starts[r]; }
     }
-    ctrl_publish(pending, cells, 0, pub);
+    ctrl_publish(pending, cells, 0, pub, S.pending);
 }
 
 // After an attempt: each replicate that took it moves on (a non-finite
-// result freezes it at the state the iteration started from).
+// result freezes it at the state the iteration started from); its error
+// maximum is reset for the next attempt.
 __global__ void __launch_bounds__(256)
-k_ctrl_after(CtrlState S, CtrlConsts K, const unsigned long long* __restrict__ errmax, int* __restrict__ bad,
+k_ctrl_after(CtrlState S, CtrlConsts K, unsigned long long* __restrict__ errmax, int* __restrict__ bad,
              const int* __restrict__ starts, double* __restrict__ trial, int* __restrict__ active, CtrlPub* pub)
 {
     int pending = 0, newf = 0;
     long long cells = 0;
     for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
         if (!active[r]) continue;
+        const unsigned long long ekey = errmax[r];
+        errmax[r] = kKeyNegInf;
         if (bad[r]) {
             bad[r] = 0;
             S.failed[r] = 1;
@@ -783,7 +803,7 @@ k_ctrl_after(CtrlState S, CtrlConsts K, const unsigned long long* __restrict__ e
             ++newf;
             continue;
         }
-        const double e = fmax(ctrl_key_value(errmax[r]), K.min_error);
+        const double e = fmax(ctrl_key_value(ekey), K.min_error);
         S.err[r] = e;
         const double dt = S.dt[r];
         int st = S.state[r];
@@ -820,16 +840,21 @@ k_ctrl_after(CtrlState S, CtrlConsts K, const unsigned long long* __restrict__ e
         S.state[r] = st;
         if (active[r]) { ++pending; cells += starts[r + 1] - starts[r]; }
     }
-    ctrl_publish(pending, cells, newf, pub);
+    ctrl_publish(pending, cells, newf, pub, S.pending);
 }
 
 // End of the iteration: time and iteration counts of the replicates that
 // stepped; the grow / static-copy mask (stepped) and dt; the per-replicate
-// values for the host.
+// values for the host; the division count of the iteration's tail zeroed.
+// Enqueued before the host knows whether a replicate still retries: it does
+// nothing while one does (the host then enqueues it again after the retries).
 __global__ void __launch_bounds__(256)
 k_ctrl_end(CtrlState S, CtrlConsts K, double* __restrict__ trial, int* __restrict__ active,
-           double* __restrict__ h_t, double* __restrict__ h_dt, double* __restrict__ h_err, int* __restrict__ h_flags)
+           double* __restrict__ h_t, double* __restrict__ h_dt, double* __restrict__ h_err, int* __restrict__ h_flags,
+           int* __restrict__ div_count)
 {
+    if (*S.pending) return;
+    if (threadIdx.x == 0) *div_count = 0;
     for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
         const int on = S.stepped[r];
         if (on) {
@@ -1361,6 +1386,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     DevBuf<double> d_cdt(R), d_cfactor(R), d_ct(R), d_cerr(R);
     DevBuf<int> d_ctries(R), d_cstate(R), d_cstepped(R), d_cfailed(R), d_go(R);
     DevBuf<long long> d_citers(R);
+    DevBuf<int> d_cpending(1);
     MappedBuf pub(sizeof(CtrlPub) + (size_t)R * (3 * sizeof(double) + sizeof(int)));
     CtrlPub* h_pub = (CtrlPub*)pub.h;
     CtrlPub* d_pub = (CtrlPub*)pub.d;
@@ -1373,7 +1399,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     double* d_perr = d_pdt + R;
     int* d_pflags = (int*)(d_perr + R);
     const CtrlState cs{d_cdt.p, d_cfactor.p, d_ct.p, d_cerr.p, d_ctries.p, d_cstate.p, d_cstepped.p, d_cfailed.p,
-                       d_citers.p};
+                       d_citers.p, d_cpending.p};
     PinnedBuf h_dt(sizeof(double) * R), h_act(sizeof(int) * R), h_bad(sizeof(int) * R);
     int* p_bad = (int*)h_bad.p;
     SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
@@ -1450,6 +1476,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         SCHK(hipMemcpyAsync(d_cerr.p, zd.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemcpyAsync(d_cfailed.p, zi.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemcpyAsync(d_citers.p, zl.data(), sizeof(long long) * R, hipMemcpyHostToDevice, st));
+        SCHK(hipMemsetAsync(d_cpending.p, 0, sizeof(int), st));
         SCHK(hipStreamSynchronize(st));
     }
     auto upload_layout = [&] {
@@ -1729,17 +1756,21 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
                 ++n_failed;
                 break;
             }
+            std::swap(d_cur, d_new);
+            abi(c, grow_batched(c, d_cur, ld, d_n.p, Rr, c->d_rep, d_dt.p, d_active.p));
+            abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
+            abi(c, bfm_sync(c));
         } else {
         // adaptive RK step of every active replicate (simulation.hpp:1677-1769)
         // with the controller on the device: the first attempt, the re-run
         // or first retry (every stepping replicate takes a second attempt when
-        // step control is on), then one synchronisation; more attempts only
-        // for replicates still retrying
+        // step control is on), then the iteration's end and one
+        // synchronisation; more attempts only for replicates still retrying
         abi(c, step_prepare(c, d_cur, n, ld));
         upload_go();
         const int control = iter % cfg->iter_update_stepsize == 0;
         hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
-                           d_active.p, d_pub);
+                           d_active.p, d_errmax.p, d_pub);
         launch_graph();
         hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
                            d_dt.p, d_active.p, d_pub);
@@ -1749,30 +1780,39 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
                                d_dt.p, d_active.p, d_pub);
         }
         SCHK(hipGetLastError());
-        SCHK(hipStreamSynchronize(st));
-        while (h_pub->pending > 0) {
-            launch_graph();
-            hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
-                               d_dt.p, d_active.p, d_pub);
+        // the iteration's end: time and counts, the columns the attempts
+        // leave alone (all columns of replicates that did not step), growth,
+        // the next iteration's dividing cells, the list's distances at the
+        // new state, then the division count and the status flags to the
+        // host. k_ctrl_end and the tail do nothing while a replicate still
+        // retries; when the attempts read only the step's frozen sums (not
+        // the list), the end is enqueued before the host knows that and
+        // enqueued again after the retries
+        auto enqueue_end = [&] {
+            hipLaunchKernelGGL(k_ctrl_end, dim3(1), dim3(256), 0, st, cs, kc, d_dt.p, d_active.p, d_pt, d_pdt,
+                               d_perr, d_pflags, d_count.p);
+            abi(c, iteration_tail_batched(c, d_cur, ld, d_n.p, n, c->d_rep, d_active.p, d_dt.p, Rr, Ldiv, d_new,
+                                          d_count.p, d_idx.p, d_cpending.p, true));
+            hipLaunchKernelGGL(k_publish_end, dim3(1), dim3(1), 0, st, d_count.p, c->d_status, d_pub);
             SCHK(hipGetLastError());
+        };
+        const bool ahead = attempt_fused(c);
+        if (ahead) enqueue_end();
+        SCHK(hipStreamSynchronize(st));
+        if (!ahead || h_pub->pending > 0) {
+            while (h_pub->pending > 0) {
+                launch_graph();
+                hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p,
+                                   d_starts.p, d_dt.p, d_active.p, d_pub);
+                SCHK(hipGetLastError());
+                SCHK(hipStreamSynchronize(st));
+            }
+            enqueue_end();
             SCHK(hipStreamSynchronize(st));
         }
-        hipLaunchKernelGGL(k_ctrl_end, dim3(1), dim3(256), 0, st, cs, kc, d_dt.p, d_active.p, d_pt, d_pdt, d_perr,
-                           d_pflags);
-        // the columns the attempts leave alone (all columns of replicates
-        // that did not step)
-        abi(c, copy_static_batched(c, d_cur, ld, d_n.p, c->d_rep, d_active.p, d_new));
-        }
+        if (h_pub->status) abi(c, bfm_sync(c));   // reports and clears the flags
         std::swap(d_cur, d_new);
-        abi(c, grow_batched(c, d_cur, ld, d_n.p, Rr, c->d_rep, d_dt.p, d_active.p));
-        abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
-        if (!verlet) {   // the next iteration's division count, read with this one's values
-            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
-            hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
-                               d_count.p, d_idx.p);
-            hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, st, d_count.p, &d_pub->ndiv);
         }
-        abi(c, bfm_sync(c));
         if (verlet) {
             if (io && io->dt_trace && act[0] && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = dt[0];
             for (int r = 0; r < R; ++r)
