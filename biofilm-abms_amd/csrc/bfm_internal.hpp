@@ -14,8 +14,12 @@ namespace bfm {
 
 // Frozen-geometry part of a step (mechanics.hpp:2483-2556): assume_2d from the
 // step-start orientations, pair forces and the per-cell cell-cell forces.
-// Every RK attempt of one iteration reuses them.
-int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld);
+// Every RK attempt of one iteration reuses them. refresh_distances: the
+// list's distances are first brought to the packed segments that
+// iteration_tail_batched(pack_segments) wrote. *cc_in_attempt is set when
+// the cell-cell sums are left to the iteration's first attempt
+// (attempt_direct with gather).
+int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_distances, bool* cc_in_attempt);
 
 // One RK attempt of a batch of replicates after step_prepare: the stages and
 // the update, with the cell count read from d_n and each replicate's dt and
@@ -35,11 +39,23 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
 // grow_batched), and the indices of the grown cells with l > Ldiv appended
 // to d_idx through *d_count (which the caller zeroes first); n (the host's
 // count, equal to *d_n) sizes the grid. Nothing when *d_skip != 0.
-// refresh_distances: then updateNeighborDistances of the internal list at
-// the new state (as bfm_dev_update_neighbor_distances on d_cells_new).
+// pack_segments: the new state's packed segments also go to the context's
+// segment buffer, from which step_prepare(refresh_distances) brings the
+// list's distances to the new state.
 int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
                            const int* d_active, const double* d_dt, double R, double Ldiv, double* d_cells_new,
-                           int* d_count, int* d_idx, const int* d_skip, bool refresh_distances);
+                           int* d_count, int* d_idx, const int* d_skip, bool pack_segments);
+
+// One attempt as a direct launch (attempt_fused configurations), grid sized
+// for n cells (= *d_n). first_go non-null: the iteration's first attempt,
+// whose replicates are those with first_go[r] && !first_failed[r], at
+// first_dt[r] (the controller's state; d_dt / d_active are not read);
+// gather: the cell-cell sums are gathered from the pair records and stored
+// (step_prepare left them to the attempt).
+int attempt_direct(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld, const int* d_n, int n,
+                   const int* d_rep, const double* d_dt, const int* d_active, const int* first_go,
+                   const int* first_failed, const double* first_dt, bool gather, double* d_cells_new, double tol,
+                   unsigned long long* d_errmax, int* d_bad);
 
 // Whether a batched RK attempt is one launch that reads only the step's
 // frozen cell-cell sums (an isotropic interaction without friction), not

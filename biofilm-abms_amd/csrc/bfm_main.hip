@@ -1680,6 +1680,16 @@ __global__ void k_sort_segments(int n, const int* __restrict__ off, int* __restr
 
 
 
+// updateNeighborDistances (mechanics.hpp:604-650) fused into the pair-force
+// kernel: each row's distance vector and contact coordinates recomputed from
+// the cells' packed segments (k_pack_segs layout) and written back to the
+// list before its force. geom == nullptr: the list's stored values are used.
+struct DistRefresh {
+    const double* geom;
+    int n;                  // rows naming a cell >= n keep their stored values
+    double *dx, *dy, *dz, *s, *t;
+};
+
 // Per-pair force magnitudes (the scalar factors of cellCellInteractionForces,
 // mechanics.hpp:1209-1350 and :1390-1659), written with the pair geometry as
 // one 64-byte record per row: {dx, dy, dz, s_i, s_j, main, hard, 0}. main
@@ -1702,7 +1712,7 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
               const double* __restrict__ nst, int ld_nst,
               const double* __restrict__ ps, const double* __restrict__ pt,
               const double* __restrict__ sur_radius, double* __restrict__ rec,
-              int* __restrict__ jcnt, int* __restrict__ status)
+              int* __restrict__ jcnt, int* __restrict__ status, DistRefresh rf)
 {
     extern __shared__ double s_grid[];
     if (ANISO) {
@@ -1720,7 +1730,18 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
             continue;
         }
         if (jcnt) atomicAdd(&jcnt[j], 1);
-        const double dx = pdx[k], dy = pdy[k], dz = pdz[k];
+        double dx, dy, dz, sv, tv;
+        if (rf.geom && (unsigned)i < (unsigned)rf.n && (unsigned)j < (unsigned)rf.n) {
+            // updateNeighborDistances of the row first (as k_update_distances_packed)
+            const Seg ci = load_packed_seg(rf.geom, i);
+            const Seg cj = load_packed_seg(rf.geom, j);
+            double d[3];
+            dist_between_cells(ci, cj, d, sv, tv);
+            dx = d[0]; dy = d[1]; dz = d[2];
+            rf.dx[k] = dx; rf.dy[k] = dy; rf.dz[k] = dz; rf.s[k] = sv; rf.t[k] = tv;
+        } else {
+            dx = pdx[k]; dy = pdy[k]; dz = pdz[k]; sv = ps[k]; tv = pt[k];
+        }
         const double dist = norm3(dx, dy, dz);
         const double overlap = 2 * fp.R - dist;
         double mm = 0, mh = 0, jkr = 0;
@@ -1754,7 +1775,7 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
                         }
                     }
                     const double2 fr = aniso_jkr_lookup(an, s_grid, d, dist, n1, cellv(cells, ld, i, C_HALF_L),
-                                                        n2, cellv(cells, ld, j, C_HALF_L), ps[k], pt[k], delta,
+                                                        n2, cellv(cells, ld, j, C_HALF_L), sv, tv, delta,
                                                         gamma);
                     mm = fr.x;
                     jkr = 1.0;
@@ -1768,8 +1789,8 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
         }
         double2* r = reinterpret_cast<double2*>(rec + (int64_t)k * 8);
         r[0] = make_double2(dx, dy);
-        r[1] = make_double2(dz, ps[k]);
-        r[2] = make_double2(pt[k], mm);
+        r[1] = make_double2(dz, sv);
+        r[2] = make_double2(tv, mm);
         r[3] = make_double2(mh, jkr);
     }
 }
@@ -2879,11 +2900,24 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
 // by the same thread), then the update and the per-replicate scaled-error
 // maximum of k_rk_final_b, with the same arithmetic and the same row chunks:
 // the staged attempt's result, in one launch instead of s + 1.
+// An iteration's first attempt (first.go non-null) takes its replicates and
+// step sizes from the controller state instead of active / dt_rep (those
+// with go[r] and not failed[r], at dt[r]); gather: the cell-cell sums are
+// first gathered from the pair records in the reference's row order (as
+// k_velocities<true>) and stored in cc for the later attempts.
+struct FirstAttempt {
+    const int* go;
+    const int* failed;
+    const double* dt;
+};
+
+template <bool GATHER>
 __global__ void __launch_bounds__(256)
 k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int ld, VelParams vp, Tableau tb,
                const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
-               const double* __restrict__ cc, double* __restrict__ kbuf, double* __restrict__ cells_new,
-               double tol, unsigned long long* __restrict__ errmax, int* __restrict__ bad_rep)
+               FirstAttempt first, RowLists L, const double* __restrict__ rec, double* __restrict__ cc,
+               double* __restrict__ kbuf, double* __restrict__ cells_new, double tol,
+               unsigned long long* __restrict__ errmax, int* __restrict__ bad_rep)
 {
     __shared__ double s_val[256];
     __shared__ int s_rep[256];
@@ -2906,16 +2940,23 @@ k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int
         int r = -1;
         if (c < b1) {
             r = rep[c];
-            if (!active[r]) {
-                r = -1;   // rows of inactive replicates: k_copy_static_b after the attempts
+            const bool on = first.go ? (first.go[r] && !first.failed[r]) : active[r] != 0;
+            if (!on) {
+                r = -1;   // rows of inactive replicates: the iteration's tail after the attempts
             } else {
-                const double dt = dt_rep[r];
+                const double dt = first.go ? first.dt[r] : dt_rep[r];
                 const bool a2d = cellv(cells, ld, c, C_NZ) < vp.nz_threshold;
                 const double sigma0 = cellv(cells, ld, c, C_SIGMA0);
                 double ccl[6];
                 const double nz6[6] = {0, 0, 0, 0, 0, 0};
+                if (GATHER) {
+                    gather_cc<true>(c, L, rec, vp.sc.R, ccl);
 #pragma unroll
-                for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
+                    for (int q = 0; q < 6; ++q) cc[(int64_t)q * n + c] = ccl[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
+                }
                 for (int st = 0; st < tb.s; ++st) {
                     CellVel cv = load_cellvel(cells, ld, c);
                     if (st > 0) {
@@ -3603,9 +3644,11 @@ void build_neighbors_sync(bfm_ctx* c, const double* d_cells, int n, int ld, doub
 void build_transposes(bfm_ctx* c, int n);
 
 void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
-                               const double* sur_radius)
+                               const double* sur_radius, bool refresh = false)
 {
     ForceParams fp = force_params(c);
+    DistRefresh rf{};
+    if (refresh) rf = DistRefresh{c->d_geom, n, c->pdx, c->pdy, c->pdz, c->ps, c->pt};
     const bool need_t = !c->list_csr && !c->transposed;
     if (need_t)
         KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
@@ -3614,12 +3657,12 @@ void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
         KLAUNCH(c, "pair_forces", k_pair_forces<true>, kPairGrid, 256, sizeof(double) * an.total, c->d_m, d_cells,
                 ld, c->pi, c->pj, c->pdx, c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd,
                 c->tab_ng, an, (const double*)nullptr, 0, c->ps, c->pt, sur_radius, c->d_rec,
-                need_t ? c->d_jcnt : (int*)nullptr, c->d_status);
+                need_t ? c->d_jcnt : (int*)nullptr, c->d_status, DistRefresh{});
     } else {
         KLAUNCH(c, "pair_forces", k_pair_forces<false>, kPairGrid, 256, 0, c->d_m, d_cells, ld, c->pi, c->pj,
                 c->pdx, c->pdy, c->pdz, fp, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng,
                 AnisoJKR{}, (const double*)nullptr, 0, c->ps, c->pt, sur_radius, c->d_rec,
-                need_t ? c->d_jcnt : (int*)nullptr, c->d_status);
+                need_t ? c->d_jcnt : (int*)nullptr, c->d_status, rf);
     }
     if (!need_t) return;
     build_transposes(c, n);
@@ -3699,17 +3742,27 @@ bool fused_forces_ok(const bfm_ctx* c)
     return (c->p.jkr_mode == BFM_JKR_TABLE_1D || c->p.jkr_mode == BFM_JKR_TABLE_2D) && c->d_tab_ov != nullptr;
 }
 
-void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld)
+// refresh: the list's distances are first brought to the cells' packed
+// segments in c->d_geom (updateNeighborDistances), inside the pair-force
+// kernel where nothing reads them before it.
+void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh = false)
 {
     const double* sur = nullptr;
-    if (c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC && c->p.jkr_mode == BFM_JKR_SURROGATE) {
+    const bool sur_on = c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC && c->p.jkr_mode == BFM_JKR_SURROGATE;
+    if (refresh && (sur_on || c->p.adhesion_mode == BFM_ADHESION_JKR_ANISOTROPIC)) {
+        if (n > 0)
+            KLAUNCH(c, "update_distances", k_update_distances_packed, kPairGrid, 256, 0, c->d_m, c->d_geom, n,
+                    c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
+        refresh = false;
+    }
+    if (sur_on) {
         require(c->sur != nullptr, BFM_ERR_ARG, "surrogate weights not set (bfm_set_surrogate)");
         sur = bfm::surrogate_pair_radii(c, d_cells, ld, n);
     }
     if ((c->p.jkr_mode == BFM_JKR_TABLE_1D || c->p.jkr_mode == BFM_JKR_TABLE_2D) &&
         c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC)
         require(c->d_tab_ov != nullptr, BFM_ERR_ARG, "JKR table not set (bfm_set_jkr_table)");
-    pair_forces_and_transpose(c, d_cells, ld, n, sur);
+    pair_forces_and_transpose(c, d_cells, ld, n, sur, refresh);
 }
 
 void launch_velocities(bfm_ctx* c, const double* d_cells, int n, int ld, const int* d_a2d,
@@ -3749,7 +3802,8 @@ void launch_stage(bfm_ctx* c, int n, const int* d_n, const double* d_cells, int 
         hipLaunchKernelGGL(k_pair_forces<true>, dim3(kPairGrid), dim3(256), sizeof(double) * an.total, c->stream,
                            c->d_m, d_cells, ld, c->pi, c->pj, c->pdx, c->pdy, c->pdz, force_params(c),
                            c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng, an, nst, c->max_cells,
-                           c->ps, c->pt, (const double*)nullptr, c->d_rec, (int*)nullptr, c->d_status);
+                           c->ps, c->pt, (const double*)nullptr, c->d_rec, (int*)nullptr, c->d_status,
+                           DistRefresh{});
         HIPCHK(hipGetLastError());
         rec = c->d_rec;
     }
@@ -4494,17 +4548,43 @@ bool attempt_fused(const bfm_ctx* c)
     return c->p.adhesion_mode != BFM_ADHESION_JKR_ANISOTROPIC && !friction_on(c);
 }
 
-int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld)
+int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_distances, bool* cc_in_attempt)
 {
     return guarded(c, [&] {
         need_params(c);
         check_cells(c, n, ld);
         require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
-        compute_forces(c, d_cells, n, ld);
-        if (!attempt_fused(c))   // the one-launch attempt tests nz itself
+        compute_forces(c, d_cells, n, ld, refresh_distances);
+        const bool fused = attempt_fused(c);
+        if (!fused)   // the one-launch attempt tests nz itself
             KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
                     c->d_assume2d);
-        launch_velocities(c, d_cells, n, ld, c->d_assume2d, nullptr, nullptr, c->d_cc);
+        *cc_in_attempt = fused && c->list_csr;
+        if (!*cc_in_attempt) launch_velocities(c, d_cells, n, ld, c->d_assume2d, nullptr, nullptr, c->d_cc);
+    });
+}
+
+int attempt_direct(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld, const int* d_n, int n,
+                   const int* d_rep, const double* d_dt, const int* d_active, const int* first_go,
+                   const int* first_failed, const double* first_dt, bool gather, double* d_cells_new, double tol,
+                   unsigned long long* d_errmax, int* d_bad)
+{
+    return guarded(c, [&] {
+        require(attempt_fused(c), BFM_ERR_ARG, "this interaction needs the staged attempt");
+        require(!gather || c->list_csr, BFM_ERR_ARG, "the cell-cell gather needs a device-built list");
+        require(tol > 0, BFM_ERR_ARG, "tol must be > 0");
+        const VelParams vp = vel_params(c);
+        const FirstAttempt fa{first_go, first_failed, first_dt};
+        const int grid = grid_for(n, 256, kBatchGrid);
+        if (gather)
+            hipLaunchKernelGGL(k_rk_attempt_b<true>, dim3(grid), dim3(256), 0, c->stream, d_n, d_cells, ld, vp, tb,
+                               d_rep, d_dt, d_active, fa, row_lists(c), c->d_rec, c->d_cc, c->d_k, d_cells_new,
+                               tol, d_errmax, d_bad);
+        else
+            hipLaunchKernelGGL(k_rk_attempt_b<false>, dim3(grid), dim3(256), 0, c->stream, d_n, d_cells, ld, vp,
+                               tb, d_rep, d_dt, d_active, fa, row_lists(c), c->d_rec, c->d_cc, c->d_k,
+                               d_cells_new, tol, d_errmax, d_bad);
+        HIPCHK(hipGetLastError());
     });
 }
 
@@ -4517,8 +4597,9 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
         require(n_rep >= 1, BFM_ERR_ARG, "n_rep must be >= 1");
         VelParams vp = vel_params(c);
         if (attempt_fused(c)) {
-            hipLaunchKernelGGL(k_rk_attempt_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, d_cells, ld, vp, tb,
-                               d_rep, d_dt, d_active, c->d_cc, c->d_k, d_cells_new, tol, d_errmax, d_bad);
+            hipLaunchKernelGGL(k_rk_attempt_b<false>, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, d_cells, ld,
+                               vp, tb, d_rep, d_dt, d_active, FirstAttempt{}, RowLists{}, (const double*)nullptr,
+                               c->d_cc, c->d_k, d_cells_new, tol, d_errmax, d_bad);
         } else {
             for (int st = 0; st < tb.s; ++st)
                 launch_stage(c, 0, d_n, d_cells, ld, nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active, kBatchGrid,
@@ -4532,19 +4613,14 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
 
 int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
                            const int* d_active, const double* d_dt, double R, double Ldiv, double* d_cells_new,
-                           int* d_count, int* d_idx, const int* d_skip, bool refresh_distances)
+                           int* d_count, int* d_idx, const int* d_skip, bool pack_segments)
 {
     return guarded(c, [&] {
         check_cells(c, n, ld);
-        if (refresh_distances)
-            require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
         hipLaunchKernelGGL(k_iter_tail_b, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells,
                            ld, d_rep, d_active, d_dt, R, Ldiv, d_cells_new, d_count, d_idx,
-                           refresh_distances ? c->d_geom : (double*)nullptr, d_skip);
+                           pack_segments ? c->d_geom : (double*)nullptr, d_skip);
         HIPCHK(hipGetLastError());
-        if (refresh_distances && n > 0)
-            KLAUNCH(c, "update_distances", k_update_distances_packed, kPairGrid, 256, 0, c->d_m, c->d_geom, n,
-                    c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
     });
 }
 
@@ -5589,7 +5665,7 @@ extern "C" int bfm_dev_get_velocities_noalloc(const bfm_params* p, const double*
                        (const int*)pi, (const int*)pj, pdx, pdx + ld_nb, pdx + 2 * ld_nb, force_params_of(*p),
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, 0, 0, AnisoJKR{},
                        (const double*)nullptr, 0, pdx + 3 * ld_nb, pdx + 4 * ld_nb, (const double*)nullptr, rec,
-                       jcnt, status);
+                       jcnt, status, DistRefresh{});
     hipLaunchKernelGGL(k_na_scan, dim3(1), dim3(1024), 0, s, (const int*)jcnt, joff, (int)nc);
     hipLaunchKernelGGL(k_fill_int, dim3(gc), dim3(256), 0, s, jcnt, 0, nc);
     hipLaunchKernelGGL(k_fill_perm, dim3(kPairGrid), dim3(256), 0, s, (const int*)d_m, (const int*)pj,

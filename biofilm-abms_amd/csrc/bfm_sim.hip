@@ -783,14 +783,27 @@ k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control,
 
 // After an attempt: each replicate that took it moves on (a non-finite
 // result freezes it at the state the iteration started from); its error
-// maximum is reset for the next attempt.
+// maximum is reset for the next attempt. first_go non-null: the attempt was
+// the iteration's first, launched without k_ctrl_begin (its replicates
+// were those with first_go[r] and not failed, at dt).
 __global__ void __launch_bounds__(256)
 k_ctrl_after(CtrlState S, CtrlConsts K, unsigned long long* __restrict__ errmax, int* __restrict__ bad,
-             const int* __restrict__ starts, double* __restrict__ trial, int* __restrict__ active, CtrlPub* pub)
+             const int* __restrict__ starts, double* __restrict__ trial, int* __restrict__ active, CtrlPub* pub,
+             const int* __restrict__ first_go, int control)
 {
     int pending = 0, newf = 0;
     long long cells = 0;
     for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
+        if (first_go) {   // after the iteration's first attempt: k_ctrl_begin's set-up first
+            const bool on = first_go[r] && !S.failed[r];
+            S.stepped[r] = on;
+            active[r] = on;
+            S.state[r] = on ? (control ? CS_FIRST : CS_FINAL) : CS_DONE;
+            S.factor[r] = 1;
+            S.tries[r] = 0;
+            trial[r] = S.dt[r];
+            if (on) cells += starts[r + 1] - starts[r];
+        }
         if (!active[r]) continue;
         const unsigned long long ekey = errmax[r];
         errmax[r] = kKeyNegInf;
@@ -1477,6 +1490,9 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         SCHK(hipMemcpyAsync(d_cfailed.p, zi.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemcpyAsync(d_citers.p, zl.data(), sizeof(long long) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemsetAsync(d_cpending.p, 0, sizeof(int), st));
+        // every attempt starts from the key of -inf (k_ctrl_after resets it)
+        hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(R, 256)), dim3(256), 0, st, d_errmax.p, R, kKeyNegInf);
+        SCHK(hipGetLastError());
         SCHK(hipStreamSynchronize(st));
     }
     auto upload_layout = [&] {
@@ -1491,10 +1507,12 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     const double thr = cfg->neighbor_threshold, Rr = cfg->R, Ldiv = cfg->Ldiv;
     int64_t builds = 0, attempts = 0, divisions = 0, iter = 0, stage_updates = 0;
     double div_seconds = 0;
+    bool list_fresh = false;   // the list's distances are those of d_cur
     auto build = [&] {
         int64_t m = 0;
         abi(c, bfm_dev_build_neighbors(c, d_cur, n, ld, thr, Rr, Ldiv, &m));
         ++builds;
+        list_fresh = true;
     };
     build();
     double tuned_scale = c->bin_scale, tuned_ms = 0;
@@ -1518,9 +1536,13 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             }
         }
     } guard{exec, graph};
+    // one-launch attempts (an isotropic interaction without friction) go
+    // straight to the stream; the staged ones are captured once per buffer
+    // orientation
+    const bool direct = !verlet && attempt_fused(c);
     const bool was_timing = c->timing;
     c->timing = false;
-    for (int k = 0; k < 2 && !verlet; ++k) {
+    for (int k = 0; k < 2 && !verlet && !direct; ++k) {
         const double* src = k == 0 ? bufA.p : bufB.p;
         double* dst = k == 0 ? bufB.p : bufA.p;
         SCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1600,10 +1622,6 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             for (int r = 0; r < R; ++r) p_act[r] = go_host[r];
             SCHK(hipMemcpyAsync(d_go.p, p_act, sizeof(int) * R, hipMemcpyHostToDevice, st));
         }
-    };
-    auto launch_graph = [&] {
-        SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
-        ++attempts;
     };
     // the per-replicate values k_ctrl_end published for the iteration just
     // finished (read after a synchronisation): time, step counts, dt trace,
@@ -1766,27 +1784,43 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         // or first retry (every stepping replicate takes a second attempt when
         // step control is on), then the iteration's end and one
         // synchronisation; more attempts only for replicates still retrying
-        abi(c, step_prepare(c, d_cur, n, ld));
+        bool cc_in_attempt = false;
+        abi(c, step_prepare(c, d_cur, n, ld, !list_fresh, &cc_in_attempt));
         upload_go();
         const int control = iter % cfg->iter_update_stepsize == 0;
-        hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
-                           d_active.p, d_errmax.p, d_pub);
-        launch_graph();
-        hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
-                           d_dt.p, d_active.p, d_pub);
-        if (control) {
-            launch_graph();
+        // one attempt after the first: a direct launch (one kernel) or the
+        // captured graph of the staged attempt
+        auto next_attempt = [&] {
+            if (direct)
+                abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, d_dt.p, d_active.p, nullptr, nullptr,
+                                      nullptr, false, d_new, cfg->max_error_allowed, d_errmax.p, d_bad.p));
+            else
+                SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
+            ++attempts;
             hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
-                               d_dt.p, d_active.p, d_pub);
+                               d_dt.p, d_active.p, d_pub, (const int*)nullptr, 0);
+        };
+        if (direct) {   // the first attempt sets the controller up itself
+            abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, d_dt.p, d_active.p, d_go.p, d_cfailed.p,
+                                  d_cdt.p, cc_in_attempt, d_new, cfg->max_error_allowed, d_errmax.p, d_bad.p));
+            ++attempts;
+            hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
+                               d_dt.p, d_active.p, d_pub, (const int*)d_go.p, control);
+        } else {
+            hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
+                               d_active.p, d_errmax.p, d_pub);
+            next_attempt();
         }
+        if (control) next_attempt();
         SCHK(hipGetLastError());
         // the iteration's end: time and counts, the columns the attempts
         // leave alone (all columns of replicates that did not step), growth,
-        // the next iteration's dividing cells, the list's distances at the
-        // new state, then the division count and the status flags to the
-        // host. k_ctrl_end and the tail do nothing while a replicate still
-        // retries; when the attempts read only the step's frozen sums (not
-        // the list), the end is enqueued before the host knows that and
+        // the next iteration's dividing cells, the new state's packed
+        // segments (the next step_prepare brings the list's distances to
+        // them), then the division count and the status flags to the host.
+        // k_ctrl_end and the tail do nothing while a replicate still
+        // retries; with the one-launch attempts (which read only the step's
+        // frozen sums) the end is enqueued before the host knows that and
         // enqueued again after the retries
         auto enqueue_end = [&] {
             hipLaunchKernelGGL(k_ctrl_end, dim3(1), dim3(256), 0, st, cs, kc, d_dt.p, d_active.p, d_pt, d_pdt,
@@ -1796,14 +1830,11 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             hipLaunchKernelGGL(k_publish_end, dim3(1), dim3(1), 0, st, d_count.p, c->d_status, d_pub);
             SCHK(hipGetLastError());
         };
-        const bool ahead = attempt_fused(c);
-        if (ahead) enqueue_end();
+        if (direct) enqueue_end();
         SCHK(hipStreamSynchronize(st));
-        if (!ahead || h_pub->pending > 0) {
+        if (!direct || h_pub->pending > 0) {
             while (h_pub->pending > 0) {
-                launch_graph();
-                hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p,
-                                   d_starts.p, d_dt.p, d_active.p, d_pub);
+                next_attempt();
                 SCHK(hipGetLastError());
                 SCHK(hipStreamSynchronize(st));
             }
@@ -1812,6 +1843,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         }
         if (h_pub->status) abi(c, bfm_sync(c));   // reports and clears the flags
         std::swap(d_cur, d_new);
+        list_fresh = false;
         }
         if (verlet) {
             if (io && io->dt_trace && act[0] && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = dt[0];
@@ -1839,6 +1871,9 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             }
         }
     }
+    // the context's list keeps distances at the final state, as after each
+    // iteration of runSimulation
+    if (!list_fresh && !verlet) abi(c, bfm_dev_update_neighbor_distances(c, d_cur, n, ld));
     abi(c, bfm_sync(c));
     take_lineage();
     SCHK(hipMemcpy2DAsync(cells_out, sizeof(double) * cap, d_cur, sizeof(double) * ld, sizeof(double) * n, ncols,
