@@ -8,6 +8,7 @@
 #include <functional>
 #include <vector>
 
+#include "bfm_ctrl.hpp"
 #include "bfm_ctx.hpp"
 
 namespace bfm {
@@ -34,28 +35,31 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
                          const int* d_rep, const double* d_dt, const int* d_active, int n_rep,
                          double* d_cells_new, double tol, unsigned long long* d_errmax, int* d_bad);
 
-// The end of a batched RK iteration in one launch: copy_static_batched into
-// d_cells_new, then growCells on d_cells_new with d_dt / d_active (as
-// grow_batched), and the indices of the grown cells with l > Ldiv appended
-// to d_idx through *d_count (which the caller zeroes first); n (the host's
-// count, equal to *d_n) sizes the grid. Nothing when *d_skip != 0.
-// pack_segments: the new state's packed segments also go to the context's
-// segment buffer, from which step_prepare(refresh_distances) brings the
-// list's distances to the new state.
+// The end of a batched RK iteration, from the controller state E.S (the
+// replicates that stepped and their dt): the controller's end-of-iteration
+// step (E), the columns the attempts did not write copied from d_cells into
+// d_cells_new, growCells on d_cells_new, the indices of the grown cells with
+// l > Ldiv appended to d_idx through *d_count (zeroed by the controller's
+// first step of the iteration), then *d_count and the context's status
+// flags to d_pub->ndiv / d_pub->status. n (the host's count, equal to *d_n)
+// sizes the grid; d_ticket: one zeroed unsigned of scratch. Nothing when
+// *d_skip != 0. pack_segments: the new state's packed segments also go to
+// the context's segment buffer, from which step_prepare(refresh_distances)
+// brings the list's distances to the new state.
 int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
-                           const int* d_active, const double* d_dt, double R, double Ldiv, double* d_cells_new,
-                           int* d_count, int* d_idx, const int* d_skip, bool pack_segments);
+                           const CtrlEnd& E, double R, double Ldiv, double* d_cells_new, int* d_count, int* d_idx,
+                           const int* d_skip, bool pack_segments, CtrlPub* d_pub, unsigned* d_ticket);
 
-// One attempt as a direct launch (attempt_fused configurations), grid sized
-// for n cells (= *d_n). first_go non-null: the iteration's first attempt,
-// whose replicates are those with first_go[r] && !first_failed[r], at
-// first_dt[r] (the controller's state; d_dt / d_active are not read);
-// gather: the cell-cell sums are gathered from the pair records and stored
-// (step_prepare left them to the attempt).
+// One attempt as a direct launch (attempt_fused configurations) followed by
+// the controller's step A (ctrl_after_block), the grid sized for n cells
+// (= *d_n). The attempt's replicates and dt: A.active / A.trial, or, when
+// A.first_go is set (the iteration's first attempt), those with
+// A.first_go[r] && !A.S.failed[r] at A.S.dt[r]. gather: the cell-cell sums
+// are gathered from the pair records and stored (step_prepare left them to
+// the attempt). d_ticket: one zeroed unsigned of scratch.
 int attempt_direct(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld, const int* d_n, int n,
-                   const int* d_rep, const double* d_dt, const int* d_active, const int* first_go,
-                   const int* first_failed, const double* first_dt, bool gather, double* d_cells_new, double tol,
-                   unsigned long long* d_errmax, int* d_bad);
+                   const int* d_rep, bool gather, double* d_cells_new, double tol, const CtrlAfter& A,
+                   unsigned* d_ticket);
 
 // Whether a batched RK attempt is one launch that reads only the step's
 // frozen cell-cell sums (an isotropic interaction without friction), not
@@ -63,10 +67,6 @@ int attempt_direct(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld,
 bool attempt_fused(const bfm_ctx* c);
 
 // growCells with per-replicate dt and active flags, cell count from d_n.
-// The columns the attempts do not write (all of them for inactive
-// replicates), from d_cells into d_cells_new, with the last attempt's mask.
-int copy_static_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, const int* d_rep,
-                        const int* d_active, double* d_cells_new);
 int grow_batched(bfm_ctx* c, double* d_cells, int ld, const int* d_n, double R, const int* d_rep,
                  const double* d_dt, const int* d_active);
 

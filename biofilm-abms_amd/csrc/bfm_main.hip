@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "bfm_ctx.hpp"
+#include "bfm_ctrl.hpp"
 #include "bfm_internal.hpp"
 #include "bfm_device.hpp"
 #include "bfm_surrogate.hpp"
@@ -2829,7 +2830,7 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
         if (c < b1) {
             r = rep[c];
             if (!active[r]) {
-                r = -1;   // rows of inactive replicates: k_copy_static_b after the attempts
+                r = -1;   // rows of inactive replicates: the iteration's tail after the attempts
             } else {
                 const double dt = dt_rep[r];
                 double y[6];
@@ -2900,25 +2901,25 @@ k_rk_final_b(const int* __restrict__ dn, int ncols, const double* __restrict__ c
 // by the same thread), then the update and the per-replicate scaled-error
 // maximum of k_rk_final_b, with the same arithmetic and the same row chunks:
 // the staged attempt's result, in one launch instead of s + 1.
-// An iteration's first attempt (first.go non-null) takes its replicates and
-// step sizes from the controller state instead of active / dt_rep (those
-// with go[r] and not failed[r], at dt[r]); gather: the cell-cell sums are
-// first gathered from the pair records in the reference's row order (as
-// k_velocities<true>) and stored in cc for the later attempts.
-struct FirstAttempt {
-    const int* go;
-    const int* failed;
-    const double* dt;
-};
-
-template <bool GATHER>
+// An iteration's first attempt (A.first_go non-null) takes its replicates
+// and step sizes from the controller state instead of active / dt_rep (those
+// with first_go[r] and not failed[r], at dt[r]); GATHER: the cell-cell sums
+// are first gathered from the pair records in the reference's row order (as
+// k_velocities<true>) and stored in cc for the later attempts. ticket
+// non-null: the last work-group then runs the controller's step after the
+// attempt (ctrl_after_block), so the attempt and its controller step are one
+// launch (for small grids: every work-group pays a device-scope release).
+// S > 0: the tableau has S stages (tb.s == S) and the stage velocities stay
+// in registers; S == 0: any tb.s, the stage velocities through kbuf.
+template <bool GATHER, int S>
 __global__ void __launch_bounds__(256)
 k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int ld, VelParams vp, Tableau tb,
                const int* __restrict__ rep, const double* __restrict__ dt_rep, const int* __restrict__ active,
-               FirstAttempt first, RowLists L, const double* __restrict__ rec, double* __restrict__ cc,
-               double* __restrict__ kbuf, double* __restrict__ cells_new, double tol,
-               unsigned long long* __restrict__ errmax, int* __restrict__ bad_rep)
+               RowLists L, const double* __restrict__ rec, double* __restrict__ cc, double* __restrict__ kbuf,
+               double* __restrict__ cells_new, double tol, unsigned long long* __restrict__ errmax,
+               int* __restrict__ bad_rep, CtrlAfter A, unsigned* __restrict__ ticket)
 {
+    const int* first_go = A.first_go;
     __shared__ double s_val[256];
     __shared__ int s_rep[256];
     const int n = *dn;
@@ -2940,11 +2941,11 @@ k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int
         int r = -1;
         if (c < b1) {
             r = rep[c];
-            const bool on = first.go ? (first.go[r] && !first.failed[r]) : active[r] != 0;
+            const bool on = first_go ? (first_go[r] && !A.S.failed[r]) : active[r] != 0;
             if (!on) {
                 r = -1;   // rows of inactive replicates: the iteration's tail after the attempts
             } else {
-                const double dt = first.go ? first.dt[r] : dt_rep[r];
+                const double dt = first_go ? A.S.dt[r] : dt_rep[r];
                 const bool a2d = cellv(cells, ld, c, C_NZ) < vp.nz_threshold;
                 const double sigma0 = cellv(cells, ld, c, C_SIGMA0);
                 double ccl[6];
@@ -2957,26 +2958,62 @@ k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int
 #pragma unroll
                     for (int q = 0; q < 6; ++q) ccl[q] = cc[(int64_t)q * n + c];
                 }
-                for (int st = 0; st < tb.s; ++st) {
-                    CellVel cv = load_cellvel(cells, ld, c);
-                    if (st > 0) {
-                        double y[6];
-                        stage_coords(cells, ld, n, c, kbuf, tb, st, dt, y);
-                        cv.rz = y[2]; cv.nx = y[3]; cv.ny = y[4]; cv.nz = y[5];
-                    }
-                    double v[6];
-                    cell_velocity(vp, cv, sigma0, a2d, ccl, false, nz6, v);
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) kbuf[((int64_t)st * 6 + q) * n + c] = v[q];
-                }
                 double y[6];
+                double kr[S > 0 ? S : 1][6];   // S > 0: the stage velocities in registers
+                if (S > 0) {
+                    // stage_coords' arithmetic on register copies of the cell
+                    const CellVel cv0 = load_cellvel(cells, ld, c);
+                    const double x0[6] = {cellv(cells, ld, c, C_RX), cellv(cells, ld, c, C_RY), cv0.rz,
+                                          cv0.nx, cv0.ny, cv0.nz};
+#pragma unroll
+                    for (int st = 0; st < (S > 0 ? S : 1); ++st) {
+                        CellVel cv = cv0;
+                        if (st > 0) {
+                            double ys[6];
+#pragma unroll
+                            for (int q = 0; q < 6; ++q) {
+                                double mult = 0;
+#pragma unroll
+                                for (int j = 0; j < st; ++j) mult += kr[j][q] * tb.A[st * tb.s + j];
+                                ys[q] = x0[q];
+                                ys[q] += mult * dt;
+                            }
+                            double nrm = norm3(ys[3], ys[4], ys[5]);
+                            ys[3] /= nrm; ys[4] /= nrm; ys[5] /= nrm;
+                            if (ys[5] < 0) { ys[3] *= -1; ys[4] *= -1; ys[5] *= -1; }
+                            cv.rz = ys[2]; cv.nx = ys[3]; cv.ny = ys[4]; cv.nz = ys[5];
+                        }
+                        cell_velocity(vp, cv, sigma0, a2d, ccl, false, nz6, kr[st]);
+                    }
+                } else {
+                    for (int st = 0; st < tb.s; ++st) {
+                        CellVel cv = load_cellvel(cells, ld, c);
+                        if (st > 0) {
+                            double ys[6];
+                            stage_coords(cells, ld, n, c, kbuf, tb, st, dt, ys);
+                            cv.rz = ys[2]; cv.nx = ys[3]; cv.ny = ys[4]; cv.nz = ys[5];
+                        }
+                        double v[6];
+                        cell_velocity(vp, cv, sigma0, a2d, ccl, false, nz6, v);
+#pragma unroll
+                        for (int q = 0; q < 6; ++q) kbuf[((int64_t)st * 6 + q) * n + c] = v[q];
+                    }
+                }
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
                     double v1 = 0, v2 = 0;
-                    for (int st = 0; st < tb.s; ++st) {
-                        const double kv = kbuf[((int64_t)st * 6 + q) * n + c];
-                        v1 += kv * tb.b[st];
-                        v2 += kv * tb.bs[st];
+                    if (S > 0) {
+#pragma unroll
+                        for (int st = 0; st < (S > 0 ? S : 1); ++st) {
+                            v1 += kr[st][q] * tb.b[st];
+                            v2 += kr[st][q] * tb.bs[st];
+                        }
+                    } else {
+                        for (int st = 0; st < tb.s; ++st) {
+                            const double kv = kbuf[((int64_t)st * 6 + q) * n + c];
+                            v1 += kv * tb.b[st];
+                            v2 += kv * tb.bs[st];
+                        }
                     }
                     const double d1 = v1 * dt, d2 = v2 * dt;
                     const double y0 = cellv(cells, ld, c, C_RX + q);
@@ -3027,32 +3064,43 @@ k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int
         }
     }
     flush();
+    if (ticket && last_block_done(ticket)) ctrl_after_block(A);
 }
 
+__global__ void __launch_bounds__(256) k_ctrl_after_step(CtrlAfter A) { ctrl_after_block(A); }
+
 // The end of a batched iteration in one launch, for every cell: the columns
-// the RK attempts leave alone (as k_copy_static_b), growCells on the new
-// array (as k_grow_b, growth.hpp:51-58) and the rows past the division
+// the RK attempts leave alone (all but the coordinates and velocities 1-12
+// of the replicates that stepped, every column of the others: finished or
+// failed replicates keep the state the iteration started from), growCells on
+// the new array (as k_grow_b, growth.hpp:51-58) and the rows past the division
 // length (divideMaxLength's test, growth.hpp:68-73) appended to idx with
-// count (zeroed by the caller); geom (when not null) receives the new
-// state's packed segments as k_pack_segs writes them. Nothing when *skip != 0.
+// count (zeroed by the controller's first step of the iteration); geom (when
+// not null) receives the new state's packed segments as k_pack_segs writes
+// them. The replicates that stepped and their dt come from the controller
+// state, whose end-of-iteration step (ctrl_end_block) work-group 0 runs;
+// ticket non-null: the last work-group publishes the division count and the
+// status flags to pub. Nothing when *skip != 0 (a replicate still retries).
 __global__ void __launch_bounds__(256)
 k_iter_tail_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld,
-              const int* __restrict__ rep, const int* __restrict__ active, const double* __restrict__ dt_rep,
-              double R, double Ldiv, double* __restrict__ cells_new, int* __restrict__ count, int* __restrict__ idx,
-              double* __restrict__ geom, const int* __restrict__ skip)
+              const int* __restrict__ rep, CtrlEnd E, double R, double Ldiv, double* __restrict__ cells_new,
+              int* __restrict__ count, int* __restrict__ idx, double* __restrict__ geom,
+              const int* __restrict__ skip, CtrlPub* pub, const int* __restrict__ status,
+              unsigned* __restrict__ ticket)
 {
     if (*skip) return;
+    if (blockIdx.x == 0) ctrl_end_block(E);
     const int n = *dn;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         const int r = rep[c];
-        const bool on = active[r] != 0;
+        const bool on = E.S.stepped[r] != 0;
         for (int col = 0; col < ncols; ++col)
             if (!on || col < 1 || col > 12) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
         double l = cellv(cells, ld, c, C_L);
         double hl = cellv(cells, ld, c, C_HALF_L);
         if (on) {
             const double g = cellv(cells, ld, c, C_GROWTH);
-            l = l + (g * (4 * R / 3 + l)) * dt_rep[r];
+            l = l + (g * (4 * R / 3 + l)) * E.S.dt[r];
             hl = l / 2;
             cells_new[(int64_t)C_L * ld + c] = l;
             cells_new[(int64_t)C_HALF_L * ld + c] = hl;
@@ -3067,22 +3115,16 @@ k_iter_tail_b(const int* __restrict__ dn, int ncols, const double* __restrict__ 
             gp[3] = make_double2(hl, 0.0);
         }
     }
+    if (ticket && last_block_done(ticket) && threadIdx.x == 0) {
+        pub->ndiv = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pub->status = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
-// The columns the RK attempts do not write, once per iteration after them:
-// all columns but the coordinates and velocities (1-12) of active replicates'
-// cells, every column of the other cells (finished or failed replicates keep
-// the state the iteration started from).
-__global__ void __launch_bounds__(256)
-k_copy_static_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld,
-                const int* __restrict__ rep, const int* __restrict__ active, double* __restrict__ cells_new)
+__global__ void k_publish_tail(const int* __restrict__ count, const int* __restrict__ status, CtrlPub* pub)
 {
-    const int n = *dn;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-        const bool on = active[rep[c]] != 0;
-        for (int col = 0; col < ncols; ++col)
-            if (!on || col < 1 || col > 12) cells_new[(int64_t)col * ld + c] = cellv(cells, ld, c, col);
-    }
+    pub->ndiv = *count;
+    pub->status = *status;
 }
 
 // growCells (growth.hpp:51-58) with each replicate's dt (inactive: unchanged).
@@ -4529,6 +4571,10 @@ void stages_2d(bfm_ctx* c, const Tableau& tb, double dt, const double* d_cells, 
 namespace bfm {
 
 constexpr int kBatchGrid = 2048;   // fixed grid of the graph-captured batched kernels
+// Grids of at most this many work-groups run the controller's steps in
+// their last work-group (a ticket after a device-scope release per group)
+// instead of a launch of their own.
+constexpr int kCtrlFuseGroups = 16;
 
 int step_verlet(bfm_ctx* c, double dt, double rho, const double* d_cells, int n, int ld, double* d_new, int ld_new,
                 int* d_bad)
@@ -4565,25 +4611,39 @@ int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_
 }
 
 int attempt_direct(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld, const int* d_n, int n,
-                   const int* d_rep, const double* d_dt, const int* d_active, const int* first_go,
-                   const int* first_failed, const double* first_dt, bool gather, double* d_cells_new, double tol,
-                   unsigned long long* d_errmax, int* d_bad)
+                   const int* d_rep, bool gather, double* d_cells_new, double tol, const CtrlAfter& A,
+                   unsigned* d_ticket)
 {
     return guarded(c, [&] {
         require(attempt_fused(c), BFM_ERR_ARG, "this interaction needs the staged attempt");
         require(!gather || c->list_csr, BFM_ERR_ARG, "the cell-cell gather needs a device-built list");
         require(tol > 0, BFM_ERR_ARG, "tol must be > 0");
         const VelParams vp = vel_params(c);
-        const FirstAttempt fa{first_go, first_failed, first_dt};
         const int grid = grid_for(n, 256, kBatchGrid);
-        if (gather)
-            hipLaunchKernelGGL(k_rk_attempt_b<true>, dim3(grid), dim3(256), 0, c->stream, d_n, d_cells, ld, vp, tb,
-                               d_rep, d_dt, d_active, fa, row_lists(c), c->d_rec, c->d_cc, c->d_k, d_cells_new,
-                               tol, d_errmax, d_bad);
-        else
-            hipLaunchKernelGGL(k_rk_attempt_b<false>, dim3(grid), dim3(256), 0, c->stream, d_n, d_cells, ld, vp,
-                               tb, d_rep, d_dt, d_active, fa, row_lists(c), c->d_rec, c->d_cc, c->d_k,
-                               d_cells_new, tol, d_errmax, d_bad);
+        // the controller's step in the attempt's last work-group for small
+        // grids; a launch of its own otherwise
+        unsigned* ticket = grid <= kCtrlFuseGroups ? d_ticket : nullptr;
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, c->stream, d_n, d_cells, ld, vp, tb, d_rep, A.trial,
+                               A.active, row_lists(c), c->d_rec, c->d_cc, c->d_k, d_cells_new, tol, A.errmax, A.bad,
+                               A, ticket);
+        };
+        // the stage counts of the reference's tableaux with the stage
+        // velocities in registers
+        switch (tb.s * 2 + (gather ? 1 : 0)) {
+        case 4: launch(k_rk_attempt_b<false, 2>); break;
+        case 5: launch(k_rk_attempt_b<true, 2>); break;
+        case 8: launch(k_rk_attempt_b<false, 4>); break;
+        case 9: launch(k_rk_attempt_b<true, 4>); break;
+        case 12: launch(k_rk_attempt_b<false, 6>); break;
+        case 13: launch(k_rk_attempt_b<true, 6>); break;
+        case 14: launch(k_rk_attempt_b<false, 7>); break;
+        case 15: launch(k_rk_attempt_b<true, 7>); break;
+        default:
+            if (gather) launch(k_rk_attempt_b<true, 0>);
+            else launch(k_rk_attempt_b<false, 0>);
+        }
+        if (!ticket) hipLaunchKernelGGL(k_ctrl_after_step, dim3(1), dim3(256), 0, c->stream, A);
         HIPCHK(hipGetLastError());
     });
 }
@@ -4597,9 +4657,9 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
         require(n_rep >= 1, BFM_ERR_ARG, "n_rep must be >= 1");
         VelParams vp = vel_params(c);
         if (attempt_fused(c)) {
-            hipLaunchKernelGGL(k_rk_attempt_b<false>, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, d_cells, ld,
-                               vp, tb, d_rep, d_dt, d_active, FirstAttempt{}, RowLists{}, (const double*)nullptr,
-                               c->d_cc, c->d_k, d_cells_new, tol, d_errmax, d_bad);
+            hipLaunchKernelGGL((k_rk_attempt_b<false, 0>), dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, d_cells,
+                               ld, vp, tb, d_rep, d_dt, d_active, RowLists{}, (const double*)nullptr, c->d_cc,
+                               c->d_k, d_cells_new, tol, d_errmax, d_bad, CtrlAfter{}, (unsigned*)nullptr);
         } else {
             for (int st = 0; st < tb.s; ++st)
                 launch_stage(c, 0, d_n, d_cells, ld, nullptr, vp, tb, st, 0.0, d_rep, d_dt, d_active, kBatchGrid,
@@ -4612,24 +4672,19 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
 }
 
 int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
-                           const int* d_active, const double* d_dt, double R, double Ldiv, double* d_cells_new,
-                           int* d_count, int* d_idx, const int* d_skip, bool pack_segments)
+                           const CtrlEnd& E, double R, double Ldiv, double* d_cells_new, int* d_count, int* d_idx,
+                           const int* d_skip, bool pack_segments, CtrlPub* d_pub, unsigned* d_ticket)
 {
     return guarded(c, [&] {
         check_cells(c, n, ld);
-        hipLaunchKernelGGL(k_iter_tail_b, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells,
-                           ld, d_rep, d_active, d_dt, R, Ldiv, d_cells_new, d_count, d_idx,
-                           pack_segments ? c->d_geom : (double*)nullptr, d_skip);
-        HIPCHK(hipGetLastError());
-    });
-}
-
-int copy_static_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, const int* d_rep,
-                        const int* d_active, double* d_cells_new)
-{
-    return guarded(c, [&] {
-        hipLaunchKernelGGL(k_copy_static_b, dim3(kBatchGrid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld,
-                           d_rep, d_active, d_cells_new);
+        const int grid = grid_for(n, 256);
+        unsigned* ticket = grid <= kCtrlFuseGroups ? d_ticket : nullptr;
+        hipLaunchKernelGGL(k_iter_tail_b, dim3(grid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld, d_rep,
+                           E, R, Ldiv, d_cells_new, d_count, d_idx, pack_segments ? c->d_geom : (double*)nullptr,
+                           d_skip, d_pub, (const int*)c->d_status, ticket);
+        if (!ticket)
+            hipLaunchKernelGGL(k_publish_tail, dim3(1), dim3(1), 0, c->stream, (const int*)d_count,
+                               (const int*)c->d_status, d_pub);
         HIPCHK(hipGetLastError());
     });
 }

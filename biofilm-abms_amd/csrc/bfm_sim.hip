@@ -23,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "bfm_ctrl.hpp"
 #include "bfm_ctx.hpp"
 #include "bfm_device.hpp"
 #include "bfm_internal.hpp"
@@ -678,95 +679,21 @@ struct MappedBuf {
 // per-replicate step-size controller on the device
 // ---------------------------------------------------------------------------
 
-// The controller of runSimulation (simulation.hpp:1688-1769) for every
-// replicate of a batch, run by small kernels between the RK attempts of the
-// captured graph, so the host does not read the error maxima back after
-// each attempt. A replicate's iteration is a state machine: FIRST (the
-// attempt at dt), RETRY (attempts at dt * factor while the scaled error
-// exceeds 1, at most max_tries, the factor accumulated and clipped to
-// [0.2, 10], clipping ending the retries), FINAL (the unconditional re-run at
-// the clamped dt * factor, or the single attempt of an iteration without
-// step control), DONE. Each kernel sets the next attempt's dt and mask.
-enum : int { CS_DONE = 0, CS_FIRST = 1, CS_RETRY = 2, CS_FINAL = 3 };
-
-struct CtrlState {
-    double* dt;          // step size; after the iteration, the one taken
-    double* factor;      // the iteration's accumulated factor
-    double* t;           // simulated time
-    double* err;         // max(scaled error, min_error) of the last attempt
-    int* tries;          // retries counted as the reference's j
-    int* state;
-    int* stepped;        // takes this iteration's step
-    int* failed;         // frozen after a non-finite attempt
-    long long* iters;
-    int* pending;        // one int: replicates with an attempt still to take (device copy)
-};
-
-struct CtrlConsts {
-    double min_error, p1, min_step, max_step;   // p1 = 1 / (order + 1)
-    int max_tries, n_rep, stages;
-};
-
-// Published after each controller kernel (host-mapped memory); per replicate
-// after the iteration: t, dt, err, stepped | failed << 1.
-struct CtrlPub {
-    int pending;          // replicates with an attempt still to take
-    int failed_total;     // replicates frozen so far
-    long long cells_total;   // cells of all attempts scheduled so far (RK-stage updates / stages)
-    int ndiv;             // cells past the division length (end of the iteration)
-    int status;           // the context's status flags at the end of the iteration
-};
-
-// The iteration's last kernel: the next division count and the context's
-// status flags to the host-mapped block, read after one synchronisation.
-__global__ void k_publish_end(const int* __restrict__ div_count, const int* __restrict__ status, CtrlPub* pub)
-{
-    pub->ndiv = *div_count;
-    pub->status = *status;
-}
-
-__device__ __forceinline__ double ctrl_key_value(unsigned long long k)
-{
-    const unsigned long long u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
-    return __longlong_as_double((long long)u);
-}
-
-// Block-wide sums into pub (one block): the next attempt's replicates and
-// cells, and the replicates frozen by the last attempt.
-__device__ __forceinline__ void ctrl_publish(int pending, long long cells, int newf, CtrlPub* pub,
-                                             int* dev_pending)
-{
-    __shared__ int s_p[4], s_f[4];
-    __shared__ long long s_c[4];
-    for (int off = 32; off > 0; off >>= 1) {
-        pending += __shfl_xor(pending, off);
-        cells += __shfl_xor(cells, off);
-        newf += __shfl_xor(newf, off);
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { s_p[w] = pending; s_c[w] = cells; s_f[w] = newf; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        pub->pending = (s_p[0] + s_p[1]) + (s_p[2] + s_p[3]);
-        *dev_pending = pub->pending;
-        pub->cells_total += (s_c[0] + s_c[1]) + (s_c[2] + s_c[3]);
-        pub->failed_total += (s_f[0] + s_f[1]) + (s_f[2] + s_f[3]);
-    }
-}
-
-// The ordered key of -inf, the start of an attempt's scaled-error maximum
-// (the attempts take atomic maxima of ordered keys).
-constexpr unsigned long long kKeyNegInf = 0x000FFFFFFFFFFFFFull;
+// The controller's steps as kernels of their own (one work-group each), for
+// batches whose attempt and tail kernels do not run them in their last
+// work-group.
 
 // The iteration's first attempt: replicates with go[r] (not finished on the
-// host's counts) and not failed take dt; control selects step control.
+// host's counts) and not failed take dt; control selects step control; the
+// tail's division count is zeroed.
 __global__ void __launch_bounds__(256)
 k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control, const int* __restrict__ starts,
              double* __restrict__ trial, int* __restrict__ active, unsigned long long* __restrict__ errmax,
-             CtrlPub* pub)
+             CtrlPub* pub, int* __restrict__ div_count)
 {
     int pending = 0;
     long long cells = 0;
+    if (threadIdx.x == 0) *div_count = 0;
     for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
         errmax[r] = kKeyNegInf;
         const bool on = go[r] && !S.failed[r];
@@ -781,107 +708,7 @@ k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control,
     ctrl_publish(pending, cells, 0, pub, S.pending);
 }
 
-// After an attempt: each replicate that took it moves on (a non-finite
-// result freezes it at the state the iteration started from); its error
-// maximum is reset for the next attempt. first_go non-null: the attempt was
-// the iteration's first, launched without k_ctrl_begin (its replicates
-// were those with first_go[r] and not failed, at dt).
-__global__ void __launch_bounds__(256)
-k_ctrl_after(CtrlState S, CtrlConsts K, unsigned long long* __restrict__ errmax, int* __restrict__ bad,
-             const int* __restrict__ starts, double* __restrict__ trial, int* __restrict__ active, CtrlPub* pub,
-             const int* __restrict__ first_go, int control)
-{
-    int pending = 0, newf = 0;
-    long long cells = 0;
-    for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
-        if (first_go) {   // after the iteration's first attempt: k_ctrl_begin's set-up first
-            const bool on = first_go[r] && !S.failed[r];
-            S.stepped[r] = on;
-            active[r] = on;
-            S.state[r] = on ? (control ? CS_FIRST : CS_FINAL) : CS_DONE;
-            S.factor[r] = 1;
-            S.tries[r] = 0;
-            trial[r] = S.dt[r];
-            if (on) cells += starts[r + 1] - starts[r];
-        }
-        if (!active[r]) continue;
-        const unsigned long long ekey = errmax[r];
-        errmax[r] = kKeyNegInf;
-        if (bad[r]) {
-            bad[r] = 0;
-            S.failed[r] = 1;
-            S.stepped[r] = 0;
-            S.state[r] = CS_DONE;
-            active[r] = 0;
-            ++newf;
-            continue;
-        }
-        const double e = fmax(ctrl_key_value(ekey), K.min_error);
-        S.err[r] = e;
-        const double dt = S.dt[r];
-        int st = S.state[r];
-        bool finish = false;
-        if (st == CS_FIRST) {
-            double f = 0.9 * pow(1.0 / e, K.p1);
-            if (f >= 10) f = 10;
-            else if (f < 0.2) f = 0.2;
-            S.factor[r] = f;
-            if (e > 1.0 && 0 < K.max_tries) { st = CS_RETRY; trial[r] = dt * f; }
-            else finish = true;
-        } else if (st == CS_RETRY) {
-            double f = S.factor[r] * (0.9 * pow(1.0 / e, K.p1));
-            bool stop = false;
-            if (f >= 10) { f = 10; stop = true; }
-            else if (f < 0.2) { f = 0.2; stop = true; }
-            S.factor[r] = f;
-            int j = S.tries[r];
-            if (!stop) S.tries[r] = ++j;
-            if (!stop && e > 1.0 && j < K.max_tries) trial[r] = dt * f;
-            else finish = true;
-        } else {   // CS_FINAL: the iteration's step is taken
-            st = CS_DONE;
-            active[r] = 0;
-        }
-        if (finish) {   // clamp to [min, max] step size, the unconditional re-run
-            double f = S.factor[r];
-            if (dt * f < K.min_step) f = K.min_step / dt;
-            else if (dt * f > K.max_step) f = K.max_step / dt;
-            S.dt[r] = dt * f;
-            trial[r] = dt * f;
-            st = CS_FINAL;
-        }
-        S.state[r] = st;
-        if (active[r]) { ++pending; cells += starts[r + 1] - starts[r]; }
-    }
-    ctrl_publish(pending, cells, newf, pub, S.pending);
-}
-
-// End of the iteration: time and iteration counts of the replicates that
-// stepped; the grow / static-copy mask (stepped) and dt; the per-replicate
-// values for the host; the division count of the iteration's tail zeroed.
-// Enqueued before the host knows whether a replicate still retries: it does
-// nothing while one does (the host then enqueues it again after the retries).
-__global__ void __launch_bounds__(256)
-k_ctrl_end(CtrlState S, CtrlConsts K, double* __restrict__ trial, int* __restrict__ active,
-           double* __restrict__ h_t, double* __restrict__ h_dt, double* __restrict__ h_err, int* __restrict__ h_flags,
-           int* __restrict__ div_count)
-{
-    if (*S.pending) return;
-    if (threadIdx.x == 0) *div_count = 0;
-    for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
-        const int on = S.stepped[r];
-        if (on) {
-            S.t[r] += S.dt[r];
-            S.iters[r] += 1;
-        }
-        active[r] = on;
-        trial[r] = S.dt[r];
-        h_t[r] = S.t[r];
-        h_dt[r] = S.dt[r];
-        h_err[r] = S.err[r];
-        h_flags[r] = on | (S.failed[r] << 1);
-    }
-}
+__global__ void __launch_bounds__(256) k_ctrl_after(CtrlAfter A) { ctrl_after_block(A); }
 
 // ---------------------------------------------------------------------------
 // divideCells on the device (growth.hpp:456-720), all replicates at once
@@ -1400,6 +1227,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     DevBuf<int> d_ctries(R), d_cstate(R), d_cstepped(R), d_cfailed(R), d_go(R);
     DevBuf<long long> d_citers(R);
     DevBuf<int> d_cpending(1);
+    DevBuf<unsigned> d_tick(2);   // last-work-group tickets of the attempt and tail kernels
     MappedBuf pub(sizeof(CtrlPub) + (size_t)R * (3 * sizeof(double) + sizeof(int)));
     CtrlPub* h_pub = (CtrlPub*)pub.h;
     CtrlPub* d_pub = (CtrlPub*)pub.d;
@@ -1490,6 +1318,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         SCHK(hipMemcpyAsync(d_cfailed.p, zi.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemcpyAsync(d_citers.p, zl.data(), sizeof(long long) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemsetAsync(d_cpending.p, 0, sizeof(int), st));
+        SCHK(hipMemsetAsync(d_tick.p, 0, sizeof(unsigned) * 2, st));
         // every attempt starts from the key of -inf (k_ctrl_after resets it)
         hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(R, 256)), dim3(256), 0, st, d_errmax.p, R, kKeyNegInf);
         SCHK(hipGetLastError());
@@ -1788,47 +1617,50 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         abi(c, step_prepare(c, d_cur, n, ld, !list_fresh, &cc_in_attempt));
         upload_go();
         const int control = iter % cfg->iter_update_stepsize == 0;
-        // one attempt after the first: a direct launch (one kernel) or the
-        // captured graph of the staged attempt
-        auto next_attempt = [&] {
-            if (direct)
-                abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, d_dt.p, d_active.p, nullptr, nullptr,
-                                      nullptr, false, d_new, cfg->max_error_allowed, d_errmax.p, d_bad.p));
-            else
-                SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
-            ++attempts;
-            hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
-                               d_dt.p, d_active.p, d_pub, (const int*)nullptr, 0);
+        // the controller's step after an attempt (first_go: after the
+        // iteration's first attempt, which set nothing up before it)
+        auto after = [&](const int* first_go) {
+            return CtrlAfter{cs, kc, d_errmax.p, d_bad.p, d_starts.p, d_dt.p, d_active.p, d_pub, first_go, control,
+                             d_count.p};
         };
-        if (direct) {   // the first attempt sets the controller up itself
-            abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, d_dt.p, d_active.p, d_go.p, d_cfailed.p,
-                                  d_cdt.p, cc_in_attempt, d_new, cfg->max_error_allowed, d_errmax.p, d_bad.p));
+        // one attempt after the first and the controller's step: a direct
+        // launch (one kernel, the step in its last work-group for small
+        // batches) or the captured graph of the staged attempt
+        auto next_attempt = [&] {
+            if (direct) {
+                abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, false, d_new, cfg->max_error_allowed,
+                                      after(nullptr), d_tick.p));
+            } else {
+                SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
+                hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, after(nullptr));
+            }
             ++attempts;
-            hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, cs, kc, d_errmax.p, d_bad.p, d_starts.p,
-                               d_dt.p, d_active.p, d_pub, (const int*)d_go.p, control);
+        };
+        if (direct) {   // the first attempt's controller step sets the iteration up
+            abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, cc_in_attempt, d_new,
+                                  cfg->max_error_allowed, after(d_go.p), d_tick.p));
+            ++attempts;
         } else {
             hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
-                               d_active.p, d_errmax.p, d_pub);
+                               d_active.p, d_errmax.p, d_pub, d_count.p);
             next_attempt();
         }
         if (control) next_attempt();
         SCHK(hipGetLastError());
-        // the iteration's end: time and counts, the columns the attempts
-        // leave alone (all columns of replicates that did not step), growth,
-        // the next iteration's dividing cells, the new state's packed
-        // segments (the next step_prepare brings the list's distances to
-        // them), then the division count and the status flags to the host.
-        // k_ctrl_end and the tail do nothing while a replicate still
-        // retries; with the one-launch attempts (which read only the step's
-        // frozen sums) the end is enqueued before the host knows that and
-        // enqueued again after the retries
+        // the iteration's end (one kernel and, for large batches, the
+        // publication): the controller's end-of-iteration step, the columns
+        // the attempts leave alone (all columns of replicates that did not
+        // step), growth, the next iteration's dividing cells, the new state's
+        // packed segments (the next step_prepare brings the list's distances
+        // to them), then the division count and the status flags to the host.
+        // It does nothing while a replicate still retries; with the
+        // one-launch attempts (which read only the step's frozen sums) it is
+        // enqueued before the host knows that and enqueued again after the
+        // retries
+        const CtrlEnd ce{cs, kc, d_dt.p, d_active.p, d_pt, d_pdt, d_perr, d_pflags};
         auto enqueue_end = [&] {
-            hipLaunchKernelGGL(k_ctrl_end, dim3(1), dim3(256), 0, st, cs, kc, d_dt.p, d_active.p, d_pt, d_pdt,
-                               d_perr, d_pflags, d_count.p);
-            abi(c, iteration_tail_batched(c, d_cur, ld, d_n.p, n, c->d_rep, d_active.p, d_dt.p, Rr, Ldiv, d_new,
-                                          d_count.p, d_idx.p, d_cpending.p, true));
-            hipLaunchKernelGGL(k_publish_end, dim3(1), dim3(1), 0, st, d_count.p, c->d_status, d_pub);
-            SCHK(hipGetLastError());
+            abi(c, iteration_tail_batched(c, d_cur, ld, d_n.p, n, c->d_rep, ce, Rr, Ldiv, d_new, d_count.p, d_idx.p,
+                                          d_cpending.p, true, d_pub, d_tick.p + 1));
         };
         if (direct) enqueue_end();
         SCHK(hipStreamSynchronize(st));
