@@ -43,6 +43,7 @@ struct CtrlPub {
     long long cells_total;   // cells of all attempts scheduled so far (RK-stage updates / stages)
     int ndiv;             // cells past the division length (end of the iteration)
     int status;           // the context's status flags at the end of the iteration
+    long long attempts;   // RK attempts taken by at least one replicate so far
 };
 
 // The ordered key of -inf, the start of an attempt's scaled-error maximum
@@ -55,26 +56,28 @@ __device__ __forceinline__ double ctrl_key_value(unsigned long long k)
     return __longlong_as_double((long long)u);
 }
 
-// Block-wide sums into pub: the next attempt's replicates and cells, and the
-// replicates frozen by the last attempt.
-__device__ __forceinline__ void ctrl_publish(int pending, long long cells, int newf, CtrlPub* pub,
+// Block-wide sums into pub: the next attempt's replicates and cells, the
+// replicates frozen by the last attempt, and whether any replicate took it.
+__device__ __forceinline__ void ctrl_publish(int pending, long long cells, int newf, int took, CtrlPub* pub,
                                              int* dev_pending)
 {
-    __shared__ int s_p[4], s_f[4];
+    __shared__ int s_p[4], s_f[4], s_t[4];
     __shared__ long long s_c[4];
     for (int off = 32; off > 0; off >>= 1) {
         pending += __shfl_xor(pending, off);
         cells += __shfl_xor(cells, off);
         newf += __shfl_xor(newf, off);
+        took += __shfl_xor(took, off);
     }
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { s_p[w] = pending; s_c[w] = cells; s_f[w] = newf; }
+    if ((threadIdx.x & 63) == 0) { s_p[w] = pending; s_c[w] = cells; s_f[w] = newf; s_t[w] = took; }
     __syncthreads();
     if (threadIdx.x == 0) {
         pub->pending = (s_p[0] + s_p[1]) + (s_p[2] + s_p[3]);
         *dev_pending = pub->pending;
         pub->cells_total += (s_c[0] + s_c[1]) + (s_c[2] + s_c[3]);
         pub->failed_total += (s_f[0] + s_f[1]) + (s_f[2] + s_f[3]);
+        if ((s_t[0] + s_t[1]) + (s_t[2] + s_t[3]) > 0) pub->attempts += 1;
     }
 }
 
@@ -99,12 +102,14 @@ struct CtrlAfter {
 
 // After an attempt: each replicate that took it moves on (a non-finite
 // result freezes it at the state the iteration started from); its error
-// maximum is reset for the next attempt. One block of 256 threads.
+// maximum is reset for the next attempt. One block of 256 threads. After an
+// attempt that no replicate took (nothing pending) it changes nothing.
 __device__ __forceinline__ void ctrl_after_block(const CtrlAfter& A)
 {
     const CtrlState& S = A.S;
     const CtrlConsts& K = A.K;
-    int pending = 0, newf = 0;
+    if (!A.first_go && *S.pending == 0) return;
+    int pending = 0, newf = 0, took = 0;
     long long cells = 0;
     if (A.first_go && A.div_count && threadIdx.x == 0) *A.div_count = 0;
     for (int r = threadIdx.x; r < K.n_rep; r += blockDim.x) {
@@ -119,6 +124,7 @@ __device__ __forceinline__ void ctrl_after_block(const CtrlAfter& A)
             if (on) cells += A.starts[r + 1] - A.starts[r];
         }
         if (!A.active[r]) continue;
+        ++took;
         // the attempt's atomics, possibly from other work-groups of the same kernel
         const unsigned long long ekey =
             __hip_atomic_load(&A.errmax[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -169,7 +175,7 @@ __device__ __forceinline__ void ctrl_after_block(const CtrlAfter& A)
         S.state[r] = st;
         if (A.active[r]) { ++pending; cells += A.starts[r + 1] - A.starts[r]; }
     }
-    ctrl_publish(pending, cells, newf, A.pub, S.pending);
+    ctrl_publish(pending, cells, newf, took, A.pub, S.pending);
 }
 
 // What the end of an iteration writes: time and iteration counts of the

@@ -2920,6 +2920,9 @@ k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int
                int* __restrict__ bad_rep, CtrlAfter A, unsigned* __restrict__ ticket)
 {
     const int* first_go = A.first_go;
+    // an attempt after every replicate finished its iteration: nothing to do
+    // (read before any work-group can reach the controller step below)
+    if (!first_go && A.S.pending && *A.S.pending == 0) return;
     __shared__ double s_val[256];
     __shared__ int s_rep[256];
     const int n = *dn;
@@ -4605,7 +4608,9 @@ int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_
         if (!fused)   // the one-launch attempt tests nz itself
             KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
                     c->d_assume2d);
-        *cc_in_attempt = fused && c->list_csr;
+        // the gather inside the first attempt saves a launch for small
+        // batches; large ones keep the separate, fully parallel gather
+        *cc_in_attempt = fused && c->list_csr && grid_for(n, 256) <= kCtrlFuseGroups;
         if (!*cc_in_attempt) launch_velocities(c, d_cells, n, ld, c->d_assume2d, nullptr, nullptr, c->d_cc);
     });
 }
@@ -4619,6 +4624,8 @@ int attempt_direct(bfm_ctx* c, const Tableau& tb, const double* d_cells, int ld,
         require(!gather || c->list_csr, BFM_ERR_ARG, "the cell-cell gather needs a device-built list");
         require(tol > 0, BFM_ERR_ARG, "tol must be > 0");
         const VelParams vp = vel_params(c);
+        // 256-row slices, several per work-group above kBatchGrid groups
+        // (faster than one group per slice on large batches)
         const int grid = grid_for(n, 256, kBatchGrid);
         // the controller's step in the attempt's last work-group for small
         // grids; a launch of its own otherwise

@@ -705,7 +705,7 @@ k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control,
         trial[r] = S.dt[r];
         if (on) { ++pending; cells += starts[r + 1] - starts[r]; }
     }
-    ctrl_publish(pending, cells, 0, pub, S.pending);
+    ctrl_publish(pending, cells, 0, 0, pub, S.pending);
 }
 
 __global__ void __launch_bounds__(256) k_ctrl_after(CtrlAfter A) { ctrl_after_block(A); }
@@ -1474,6 +1474,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         }
         last_err0 = h_perr[0];
         stage_updates = h_pub->cells_total * tb.s;
+        attempts = h_pub->attempts;
     };
     h_pub->ndiv = -1;
     double next_report = cfg->report_seconds;
@@ -1634,18 +1635,23 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
                 SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
                 hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, after(nullptr));
             }
-            ++attempts;
         };
         if (direct) {   // the first attempt's controller step sets the iteration up
             abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, cc_in_attempt, d_new,
                                   cfg->max_error_allowed, after(d_go.p), d_tick.p));
-            ++attempts;
         } else {
             hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
                                d_active.p, d_errmax.p, d_pub, d_count.p);
             next_attempt();
         }
         if (control) next_attempt();
+        // a batch of replicates nearly always has one still retrying after
+        // two attempts: every attempt the controller may ask for (max_tries
+        // retries and the final re-run) is enqueued at once; those after the
+        // last replicate finished return at once
+        const bool all_attempts = direct && R > 1;
+        if (all_attempts && control)
+            for (int k = 0; k < cfg->max_tries_update_stepsize; ++k) next_attempt();
         SCHK(hipGetLastError());
         // the iteration's end (one kernel and, for large batches, the
         // publication): the controller's end-of-iteration step, the columns
@@ -1662,7 +1668,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
             abi(c, iteration_tail_batched(c, d_cur, ld, d_n.p, n, c->d_rep, ce, Rr, Ldiv, d_new, d_count.p, d_idx.p,
                                           d_cpending.p, true, d_pub, d_tick.p + 1));
         };
-        if (direct) enqueue_end();
+        if (direct) enqueue_end();   // ahead of the retry check
         SCHK(hipStreamSynchronize(st));
         if (!direct || h_pub->pending > 0) {
             while (h_pub->pending > 0) {
