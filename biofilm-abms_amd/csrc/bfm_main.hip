@@ -2436,33 +2436,38 @@ k_velocities(int n, const double* __restrict__ cells, int ld, const int* __restr
     }
 }
 
+// Lanes per cell of k_slot_order (measured on the 200k dome: 1, 2, 4, 8, 16
+// lanes give 76, 55, 42, 46, 68 us).
+constexpr int kSlotLanes = 4;
+
 // Isotropic row values and row order of the lane-per-home search's per-cell
-// lists, eight lanes per cell: each i-role entry (i = c, j = its partner)
-// gets the row's main and hard-core terms, computed as the compaction
-// computes them, in the last two doubles of its record {d, s_i, s_j,
-// partner slot, main, hard}; each entry's rank among the cell's partners
-// comes from width-8 shuffles, and ord_i[rank][c] = its slot (ascending j),
+// lists, L lanes per cell: each i-role entry (i = c, j = its partner) gets
+// the row's main and hard-core terms, computed as the compaction computes
+// them, in the last two doubles of its record {d, s_i, s_j, partner slot,
+// main, hard}; each entry's rank among the cell's partners comes from
+// width-L shuffles, and ord_i[rank][c] = its slot (ascending j),
 // ord_j[rank][c] = i << 32 | i's slot of the row (ascending i).
+template <int L>
 __global__ void __launch_bounds__(256)
 k_slot_order(int n, FusedForces ff, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
              const int* __restrict__ slot_j, double* __restrict__ slot_rec, const int* __restrict__ jslot,
              const int* __restrict__ jslotk, int* __restrict__ ord_i, unsigned long long* __restrict__ ord_j,
              int* __restrict__ status)
 {
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g / L < n;
          g += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(g >> 3), k = (int)(g & 7);
+        const int c = (int)(g / L), k = (int)(g % L);
         const unsigned long long cv = cnt64[c];
         const int ci = min((int)(cv >> 32), slot_cap), cj = min((int)(cv & 0xffffffffull), slot_cap);
-        for (int a = 0; a * 8 < ci; ++a) {
-            const int e = a * 8 + k;
+        for (int a = 0; a * L < ci; ++a) {
+            const int e = a * L + k;
             const int je = e < ci ? slot_j[(int64_t)e * stride + c] : INT_MAX;
             int rank = 0;
-            for (int b = 0; b * 8 < ci; ++b) {
-                const int eb = b * 8 + k;
+            for (int b = 0; b * L < ci; ++b) {
+                const int eb = b * L + k;
                 const int jb = b == a ? je : (eb < ci ? slot_j[(int64_t)eb * stride + c] : INT_MAX);
 #pragma unroll
-                for (int f = 0; f < 8; ++f) rank += __shfl(jb, f, 8) < je;
+                for (int f = 0; f < L; ++f) rank += __shfl(jb, f, L) < je;
             }
             if (e < ci) {
                 const int64_t se = (int64_t)e * stride + c;
@@ -2475,15 +2480,15 @@ k_slot_order(int n, FusedForces ff, int64_t stride, int slot_cap, const unsigned
                 ord_i[(int64_t)rank * stride + c] = e;
             }
         }
-        for (int a = 0; a * 8 < cj; ++a) {
-            const int e = a * 8 + k;
+        for (int a = 0; a * L < cj; ++a) {
+            const int e = a * L + k;
             const int ie = e < cj ? jslot[(int64_t)e * stride + c] : INT_MAX;
             int rank = 0;
-            for (int b = 0; b * 8 < cj; ++b) {
-                const int eb = b * 8 + k;
+            for (int b = 0; b * L < cj; ++b) {
+                const int eb = b * L + k;
                 const int ib = b == a ? ie : (eb < cj ? jslot[(int64_t)eb * stride + c] : INT_MAX);
 #pragma unroll
-                for (int f = 0; f < 8; ++f) rank += __shfl(ib, f, 8) < ie;
+                for (int f = 0; f < L; ++f) rank += __shfl(ib, f, L) < ie;
             }
             if (e < cj)
                 ord_j[(int64_t)rank * stride + c] =
@@ -5270,7 +5275,7 @@ int bfm_dev_stage(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, 
             build_neighbors(c, d_cells, n, ld, thr, R, Ldiv, false, false);
             const FusedForces ff{force_params(c), d_cells, ld, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd,
                                  c->tab_ng, nullptr};
-            KLAUNCH(c, "pair_forces", k_slot_order, grid_for(8 * (int64_t)n, 256), 256, 0, n, ff,
+            KLAUNCH(c, "pair_forces", k_slot_order<kSlotLanes>, grid_for(kSlotLanes * (int64_t)n, 256), 256, 0, n, ff,
                     (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_jslot,
                     c->d_jslotk, c->d_jrow, c->d_ordj, c->d_status);
             KLAUNCH(c, "velocities", k_velocities_slots, grid_for(n, 256), 256, 0, n, d_cells, ld, vel_params(c),
