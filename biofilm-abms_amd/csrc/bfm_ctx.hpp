@@ -126,8 +126,11 @@ struct bfm_ctx {
     bool transposed = false;        // ioff/joff/perm valid for the current list
 
     bool list_csr = false;          // per-cell CSR (device-built lists) instead of transposes
-    unsigned long long* d_cnt64 = nullptr;  // per cell: (rows as i) << 32 | (rows as j)
-    unsigned long long* d_off64 = nullptr;  // exclusive scan of d_cnt64
+    unsigned long long* d_cnt64 = nullptr;  // per cell: (rows as i) << 32 | (rows as j) taken by atomics
+    unsigned long long* d_own64 = nullptr;  // the same counts of the rows a cell's own search kept (slots
+                                            // from the top of its list; 0 for the LDS-staged variants)
+    unsigned long long* d_tot64 = nullptr;  // d_cnt64 + d_own64 (the list path's scan input)
+    unsigned long long* d_off64 = nullptr;  // exclusive scan of d_tot64
     int* d_jslot = nullptr;         // max_cells x slot_cap: partners with larger index
     int* d_jslotk = nullptr;        // slot_cap x max_cells: the i-role slot of each partner-list entry
     unsigned long long* d_ordj = nullptr;   // slot_cap x max_cells: j-role rows in order, i << 32 | slot

@@ -384,18 +384,19 @@ k_scan_onepass(const Plan* __restrict__ plan, int64_t items_arg, const T* __rest
 // every pair is tested from the larger original index, and each cell's list
 // is sorted by partner index during compaction.
 // It also clears the per-cell pair counters (n + 1) that the candidate kernel
-// increments next, and the histogram entries of the occupied bins (every
+// increments or writes next, and the histogram entries of the occupied bins (every
 // nonzero entry has a cell), which leaves the histogram zero for the next build.
 template <bool SOA>
 __global__ void k_scatter(const double* __restrict__ cells, int n, int ld, const int* __restrict__ keys,
                           const int* __restrict__ rank, const int* __restrict__ bin_off,
                           double* __restrict__ sorted, int* __restrict__ sid, int* __restrict__ skey, int64_t stride,
-                          unsigned long long* __restrict__ cnt64, int* __restrict__ hist, float4* __restrict__ sortf,
-                          double* __restrict__ sgeom)
+                          unsigned long long* __restrict__ cnt64, unsigned long long* __restrict__ own64,
+                          int* __restrict__ hist, float4* __restrict__ sortf, double* __restrict__ sgeom)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt64[n] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt64[n] = own64[n] = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         cnt64[i] = 0;
+        own64[i] = 0;
         const int key = keys[i];
         hist[key] = 0;
         const int q = bin_off[key] + rank[i];
@@ -1267,8 +1268,12 @@ __device__ __forceinline__ bool seg_bounds_pass(float4 af, float4 an, float4 bf,
 // Survivors of the centre-distance screen go to a wavefront-private LDS queue
 // (ballot + mbcnt, no atomics); 64 at a time they take the point-segment
 // screen, whose survivors go to a second queue that is drained 64 at a time
-// through the FP64 segment test, one pair per lane. The pair rule and
-// everything written are those of k_candidates_lds. Sorted positions must be
+// through the FP64 segment test, one pair per lane. The pair rule is that of
+// k_candidates_lds; of a kept pair's two list entries, the partner's takes
+// its slot through a device-scope atomic on the partner's counter (cnt64,
+// slots from 0 up) and the home's through an LDS counter of the wavefront
+// (slots from slot_cap - 1 down; the counts go to own64 at the end), which
+// halves the device-scope atomics (see slot_of). Sorted positions must be
 // below 2^26 (queue entries carry the home lane in the top six bits).
 template <int UNROLL>
 __global__ void __launch_bounds__(256)
@@ -1277,8 +1282,8 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
                   const int* __restrict__ sid, const int* __restrict__ skey,
                   const int* __restrict__ rep, const RepGrid* __restrict__ grid,
                   const int* __restrict__ bin_off, int slot_cap, unsigned long long* __restrict__ cnt64,
-                  int* __restrict__ slot_j, double* __restrict__ slot_rec, int* __restrict__ jslot,
-                  int* __restrict__ jslotk, unsigned long long* __restrict__ counters)
+                  unsigned long long* __restrict__ own64, int* __restrict__ slot_j, double* __restrict__ slot_rec,
+                  int* __restrict__ jslot, int* __restrict__ jslotk, unsigned long long* __restrict__ counters)
 {
     constexpr int WAVES = 4, QA = 64 + 64 * UNROLL, QB = 128, QBITS = 26;
     constexpr unsigned QMASK = (1u << QBITS) - 1u;
@@ -1286,6 +1291,7 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
     __shared__ unsigned s_qb[WAVES][QB];
     __shared__ float4 s_hf[WAVES][64], s_hn[WAVES][64];
     __shared__ int s_hid[WAVES][64];
+    __shared__ int s_own[WAVES][64][2];   // the homes' own rows as i and as j (LDS atomics)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int hbase = (xcd_block(blockIdx.x, gridDim.x) * WAVES + wave) * 64;
     if (hbase >= n) return;   // the whole wavefront
@@ -1319,6 +1325,7 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
     s_hf[wave][lane] = hf;
     s_hn[wave][lane] = hn;
     s_hid[wave][lane] = hid;
+    s_own[wave][lane][0] = s_own[wave][lane][1] = 0;
     const float hb = ft + hf.w;   // thr + margin + the home's half length
     unsigned long long screened = 0, tested = 0;
     int na = 0, nb = 0;           // queue fills, uniform across the wavefront
@@ -1358,9 +1365,18 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
                     double d[3], sv, tv;
                     dist_between_cells(c1, c2, d, sv, tv);
                     if (norm3(d[0], d[1], d[2]) < thr) {
-                        const int slot = (int)(atomicAdd(&cnt64[i], 1ull << 32) >> 32);
-                        const int slot2 = (int)(atomicAdd(&cnt64[j], 1ull) & 0xffffffffull);
-                        if (slot < slot_cap) {
+                        // the home's entry from the top of its list through an LDS
+                        // counter of its wavefront, the partner's from the bottom
+                        // through a device-scope atomic
+                        int slot, slot2;
+                        if (home_is_i) {
+                            slot = slot_cap - 1 - atomicAdd(&s_own[wave][h][0], 1);
+                            slot2 = (int)(atomicAdd(&cnt64[j], 1ull) & 0xffffffffull);
+                        } else {
+                            slot2 = slot_cap - 1 - atomicAdd(&s_own[wave][h][1], 1);
+                            slot = (int)(atomicAdd(&cnt64[i], 1ull << 32) >> 32);
+                        }
+                        if (slot >= 0 && slot < slot_cap) {
                             const int64_t es = (int64_t)slot * stride + i;
                             slot_j[es] = j;
                             double2* rr = reinterpret_cast<double2*>(slot_rec + 8 * es);
@@ -1368,7 +1384,7 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
                             rr[1] = make_double2(d[2], sv);
                             rr[2] = make_double2(tv, __longlong_as_double((long long)slot2));
                         }
-                        if (slot2 < slot_cap) {
+                        if (slot2 >= 0 && slot2 < slot_cap) {
                             jslot[(int64_t)slot2 * stride + j] = i;
                             jslotk[(int64_t)slot2 * stride + j] = slot;   // where j finds the row's record
                         }
@@ -1463,6 +1479,9 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
     }
     drain_a(true);
     drain_b(true);
+    wave_sync();
+    if (hv)
+        own64[hid] = ((unsigned long long)(unsigned)s_own[wave][lane][0] << 32) | (unsigned)s_own[wave][lane][1];
     for (int off = 32; off > 0; off >>= 1) screened += __shfl_xor(screened, off);
     if (lane == 0) {
         count_add(counters, 0, screened);
@@ -1471,9 +1490,44 @@ k_candidates_lane(int n, const Plan* __restrict__ plan, const double* __restrict
 }
 
 
+// A cell's per-cell list, one role: the f entries other cells' searches
+// added fill slots upwards from 0
+// (device-scope atomics on the cell's counter), those of the cell's own
+// search fill them downwards from slot_cap - 1 (the lane-per-home search;
+// none for the other variants). Their counts are cnt64 and own64 (as i in
+// the high, as j in the low half); a cell with more than slot_cap entries in
+// a role is reported and skipped (its slots may overlap).
+struct SlotCounts {
+    int fi, fj, ci, cj;   // entries added by other searches, entries in all (as i, as j)
+    bool over;
+};
+__device__ __forceinline__ SlotCounts slot_counts(unsigned long long cv, unsigned long long ow, int slot_cap)
+{
+    SlotCounts k;
+    k.fi = (int)(cv >> 32);
+    k.fj = (int)(cv & 0xffffffffull);
+    k.ci = k.fi + (int)(ow >> 32);
+    k.cj = k.fj + (int)(ow & 0xffffffffull);
+    k.over = k.ci > slot_cap || k.cj > slot_cap;
+    if (k.over) k.fi = k.fj = k.ci = k.cj = 0;
+    return k;
+}
+
+// Slot of entry e (0 <= e < count) of one role, f of them from the bottom.
+__device__ __forceinline__ int slot_of(int e, int f, int slot_cap) { return e < f ? e : slot_cap - 1 - (e - f); }
+
+// out = a + b over n entries (per-cell row counts: the atomically taken and
+// the own-search ones, each packed as i << 32 | j).
+__global__ void k_add_u64(int n, const unsigned long long* __restrict__ a, const unsigned long long* __restrict__ b,
+                          unsigned long long* __restrict__ out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = a[i] + b[i];
+}
+
 template <bool FORCES>
 __global__ void __launch_bounds__(256)
 k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+               const unsigned long long* __restrict__ own64,
                const unsigned long long* __restrict__ off64, const int* __restrict__ slot_j,
                const double* __restrict__ slot_rec, int64_t max_pairs, int* __restrict__ pi, int* __restrict__ pj,
                double* __restrict__ pdx, double* __restrict__ pdy, double* __restrict__ pdz,
@@ -1484,31 +1538,30 @@ k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
          g += (int64_t)gridDim.x * blockDim.x) {
     const int c = (int)(g >> 3), k = (int)(g & 7);
-    int ci = 0;
+    int ci = 0, fi = 0;
     int64_t oi = 0, base2 = 0;
     if (c < n) {
-        const unsigned long long cv = cnt64[c], ov = off64[c];
-        ci = (int)(cv >> 32);
-        const int cj = (int)(cv & 0xffffffffull);
-        if (ci > slot_cap || cj > slot_cap) {
-            if (k == 0) atomicOr(status, ST_SLOT_OVERFLOW);
-            ci = min(ci, slot_cap);
-        }
+        const unsigned long long ov = off64[c];
+        const SlotCounts sc = slot_counts(cnt64[c], own64[c], slot_cap);
+        if (sc.over && k == 0) atomicOr(status, ST_SLOT_OVERFLOW);
+        ci = sc.ci;
+        fi = sc.fi;
         oi = (int64_t)(ov >> 32);
         base2 = oi + (int64_t)(ov & 0xffffffffull);
     }
     for (int a = 0; a * 8 < ci; ++a) {
         const int e = a * 8 + k;
-        const int je = e < ci ? slot_j[(int64_t)e * stride + c] : INT_MAX;
+        const int je = e < ci ? slot_j[(int64_t)slot_of(e, fi, slot_cap) * stride + c] : INT_MAX;
         int rank = 0;
         for (int b = 0; b * 8 < ci; ++b) {
             const int eb = b * 8 + k;
-            const int jb = b == a ? je : (eb < ci ? slot_j[(int64_t)eb * stride + c] : INT_MAX);
+            const int jb =
+                b == a ? je : (eb < ci ? slot_j[(int64_t)slot_of(eb, fi, slot_cap) * stride + c] : INT_MAX);
 #pragma unroll
             for (int f = 0; f < 8; ++f) rank += __shfl(jb, f, 8) < je;
         }
         if (e < ci && fits) {
-            const int64_t se = (int64_t)e * stride + c;
+            const int64_t se = (int64_t)slot_of(e, fi, slot_cap) * stride + c;
             const int64_t row = oi + rank;
             const double2* r = reinterpret_cast<const double2*>(slot_rec + 8 * se);
             const double2 r0 = r[0], r1 = r[1], r2 = r[2];
@@ -1520,7 +1573,7 @@ k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __
                 isotropic_record(ff.fp, ff.cells, ff.ld, c, je, r0.x, r0.y, r1.x, r1.y, r2.x, ff.tab_ov, ff.tab_g,
                                  ff.tab_r, ff.tab_nd, ff.tab_ng, status, ff.rec + 8 * row);
             const int s2 = (int)__double_as_longlong(r2.y);
-            if (s2 < slot_cap) jrow[(int64_t)s2 * stride + je] = (int)row;
+            if (s2 >= 0 && s2 < slot_cap) jrow[(int64_t)s2 * stride + je] = (int)row;
         }
     }
     if (c == n - 1 && k == 0) {
@@ -1534,6 +1587,7 @@ k_compact_rows(int n, int64_t stride, int slot_cap, const unsigned long long* __
 // Pass 2 (per cell c, eight lanes): j-role entries sorted by partner i.
 __global__ void __launch_bounds__(256)
 k_compact_jrole(int n, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
+                const unsigned long long* __restrict__ own64,
                 const unsigned long long* __restrict__ off64, const int* __restrict__ jslot,
                 const int* __restrict__ jrow, int64_t max_pairs, int* __restrict__ csr)
 {
@@ -1541,25 +1595,29 @@ k_compact_jrole(int n, int64_t stride, int slot_cap, const unsigned long long* _
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; (g >> 3) < n;
          g += (int64_t)gridDim.x * blockDim.x) {
     const int c = (int)(g >> 3), k = (int)(g & 7);
-    int ci = 0, cj = 0;
+    int ci = 0, cj = 0, fj = 0;
     int64_t base2 = 0;
     if (c < n) {
-        const unsigned long long cv = cnt64[c], ov = off64[c];
-        ci = min((int)(cv >> 32), slot_cap);
-        cj = min((int)(cv & 0xffffffffull), slot_cap);
+        const unsigned long long ov = off64[c];
+        const SlotCounts sc = slot_counts(cnt64[c], own64[c], slot_cap);
+        ci = sc.ci;
+        cj = sc.cj;
+        fj = sc.fj;
         base2 = (int64_t)(ov >> 32) + (int64_t)(ov & 0xffffffffull);
     }
     for (int a = 0; a * 8 < cj; ++a) {
         const int e = a * 8 + k;
-        const int ie = e < cj ? jslot[(int64_t)e * stride + c] : INT_MAX;
+        const int ie = e < cj ? jslot[(int64_t)slot_of(e, fj, slot_cap) * stride + c] : INT_MAX;
         int rank = 0;
         for (int b = 0; b * 8 < cj; ++b) {
             const int eb = b * 8 + k;
-            const int ib = b == a ? ie : (eb < cj ? jslot[(int64_t)eb * stride + c] : INT_MAX);
+            const int ib =
+                b == a ? ie : (eb < cj ? jslot[(int64_t)slot_of(eb, fj, slot_cap) * stride + c] : INT_MAX);
 #pragma unroll
             for (int f = 0; f < 8; ++f) rank += __shfl(ib, f, 8) < ie;
         }
-        if (e < cj) csr[base2 + ci + rank] = jrow[(int64_t)e * stride + c] | (int)0x80000000u;
+        if (e < cj)
+            csr[base2 + ci + rank] = jrow[(int64_t)slot_of(e, fj, slot_cap) * stride + c] | (int)0x80000000u;
     }
     }
 }
@@ -2450,49 +2508,53 @@ constexpr int kSlotLanes = 4;
 template <int L>
 __global__ void __launch_bounds__(256)
 k_slot_order(int n, FusedForces ff, int64_t stride, int slot_cap, const unsigned long long* __restrict__ cnt64,
-             const int* __restrict__ slot_j, double* __restrict__ slot_rec, const int* __restrict__ jslot,
-             const int* __restrict__ jslotk, int* __restrict__ ord_i, unsigned long long* __restrict__ ord_j,
-             int* __restrict__ status)
+             const unsigned long long* __restrict__ own64, const int* __restrict__ slot_j,
+             double* __restrict__ slot_rec, const int* __restrict__ jslot, const int* __restrict__ jslotk,
+             int* __restrict__ ord_i, unsigned long long* __restrict__ ord_j, int* __restrict__ status)
 {
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g / L < n;
          g += (int64_t)gridDim.x * blockDim.x) {
         const int c = (int)(g / L), k = (int)(g % L);
-        const unsigned long long cv = cnt64[c];
-        const int ci = min((int)(cv >> 32), slot_cap), cj = min((int)(cv & 0xffffffffull), slot_cap);
+        const SlotCounts sc = slot_counts(cnt64[c], own64[c], slot_cap);   // overflow: k_velocities_slots
+        const int ci = sc.ci, cj = sc.cj, fi = sc.fi, fj = sc.fj;
         for (int a = 0; a * L < ci; ++a) {
             const int e = a * L + k;
-            const int je = e < ci ? slot_j[(int64_t)e * stride + c] : INT_MAX;
+            const int pe = e < ci ? slot_of(e, fi, slot_cap) : 0;
+            const int je = e < ci ? slot_j[(int64_t)pe * stride + c] : INT_MAX;
             int rank = 0;
             for (int b = 0; b * L < ci; ++b) {
                 const int eb = b * L + k;
-                const int jb = b == a ? je : (eb < ci ? slot_j[(int64_t)eb * stride + c] : INT_MAX);
+                const int jb =
+                    b == a ? je : (eb < ci ? slot_j[(int64_t)slot_of(eb, fi, slot_cap) * stride + c] : INT_MAX);
 #pragma unroll
                 for (int f = 0; f < L; ++f) rank += __shfl(jb, f, L) < je;
             }
             if (e < ci) {
-                const int64_t se = (int64_t)e * stride + c;
+                const int64_t se = (int64_t)pe * stride + c;
                 double2* rr = reinterpret_cast<double2*>(slot_rec + 8 * se);
                 const double2 r0 = rr[0], r1 = rr[1];
                 double mm, mh, jkr;
                 isotropic_vals(ff.fp, ff.cells, ff.ld, c, je, r0.x, r0.y, r1.x, ff.tab_ov, ff.tab_g, ff.tab_r,
                                ff.tab_nd, ff.tab_ng, status, mm, mh, jkr);
                 rr[3] = make_double2(mm, mh);
-                ord_i[(int64_t)rank * stride + c] = e;
+                ord_i[(int64_t)rank * stride + c] = pe;
             }
         }
         for (int a = 0; a * L < cj; ++a) {
             const int e = a * L + k;
-            const int ie = e < cj ? jslot[(int64_t)e * stride + c] : INT_MAX;
+            const int pe = e < cj ? slot_of(e, fj, slot_cap) : 0;
+            const int ie = e < cj ? jslot[(int64_t)pe * stride + c] : INT_MAX;
             int rank = 0;
             for (int b = 0; b * L < cj; ++b) {
                 const int eb = b * L + k;
-                const int ib = b == a ? ie : (eb < cj ? jslot[(int64_t)eb * stride + c] : INT_MAX);
+                const int ib =
+                    b == a ? ie : (eb < cj ? jslot[(int64_t)slot_of(eb, fj, slot_cap) * stride + c] : INT_MAX);
 #pragma unroll
                 for (int f = 0; f < L; ++f) rank += __shfl(ib, f, L) < ie;
             }
             if (e < cj)
                 ord_j[(int64_t)rank * stride + c] =
-                    ((unsigned long long)(unsigned)ie << 32) | (unsigned)jslotk[(int64_t)e * stride + c];
+                    ((unsigned long long)(unsigned)ie << 32) | (unsigned)jslotk[(int64_t)pe * stride + c];
         }
     }
 }
@@ -2515,18 +2577,15 @@ __device__ __forceinline__ void slot_row(const double2 (&r)[4], bool role_i, dou
 __global__ void __launch_bounds__(256)
 k_velocities_slots(int n, const double* __restrict__ cells, int ld, VelParams vp, int64_t stride,
                    int slot_cap, const unsigned long long* __restrict__ cnt64,
+                   const unsigned long long* __restrict__ own64,
                    const double* __restrict__ slot_rec, const int* __restrict__ ord_i,
                    const unsigned long long* __restrict__ ord_j, double* __restrict__ vel, int* __restrict__ status)
 {
     constexpr int G = 4;   // rows whose order entries, then records, are loaded before any is added
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-        const unsigned long long cv = cnt64[c];
-        int ci = (int)(cv >> 32), cj = (int)(cv & 0xffffffffull);
-        if (ci > slot_cap || cj > slot_cap) {
-            atomicOr(status, ST_SLOT_OVERFLOW);
-            ci = min(ci, slot_cap);
-            cj = min(cj, slot_cap);
-        }
+        const SlotCounts sc = slot_counts(cnt64[c], own64[c], slot_cap);
+        if (sc.over) atomicOr(status, ST_SLOT_OVERFLOW);
+        const int ci = sc.ci, cj = sc.cj;
         double cc[6] = {0, 0, 0, 0, 0, 0};
         // rows where c is i (ascending partner j), then where c is j
         // (ascending partner i), G at a time
@@ -2546,7 +2605,7 @@ k_velocities_slots(int n, const double* __restrict__ cells, int ld, VelParams vp
                     const unsigned long long o = ord_j[(int64_t)(r - ci) * stride + c];
                     const int si = (int)(o & 0xffffffffull);
                     rec_at[u] = (int64_t)si * stride + (int)(o >> 32);
-                    ok[u] = si < slot_cap;   // else i's slots overflowed: reported by i's thread
+                    ok[u] = si >= 0 && si < slot_cap;   // else i's slots overflowed: reported by i's thread
                 }
             }
             double2 ra[G][4];
@@ -3564,7 +3623,7 @@ void launch_candidates(bfm_ctx* c, int n, int64_t stride)
             KLAUNCH(c, "candidates", k_candidates_lane<4>, grid_for(n, 256), 256, 0, n, c->d_plan, c->d_sgeom,
                     stride, c->d_sortf, c->d_sortf + stride, c->d_sid, c->d_skey,
                     c->n_rep > 0 ? c->d_rep : (const int*)nullptr, c->d_grid, c->d_bin_off, c->slot_cap, c->d_cnt64,
-                    c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_jslotk, c->d_counters);
+                    c->d_own64, c->d_slot_j, c->d_slot_rec, c->d_jslot, c->d_jslotk, c->d_counters);
             break;
         case 7:
             KLAUNCH(c, "candidates", k_candidates_bins<256>, 2048, 256, 0, c->d_plan, c->d_sorted, stride, c->d_sid,
@@ -3636,11 +3695,11 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
     const int64_t stride = c->max_cells;
     if (c->cand_variant == 8)
         KLAUNCH(c, "scatter", k_scatter<false>, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals,
-                c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count, c->d_sortf,
+                c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_own64, c->d_bin_count, c->d_sortf,
                 c->d_sgeom);
     else
         KLAUNCH(c, "scatter", k_scatter<true>, grid_for(n, 256), 256, 0, d_cells, n, ld, c->d_keys, c->d_vals,
-                c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_bin_count, c->d_sortf,
+                c->d_bin_off, c->d_sorted, c->d_sid, c->d_skey, stride, c->d_cnt64, c->d_own64, c->d_bin_count, c->d_sortf,
                 c->d_sgeom);
     c->bins_dirty = false;
     launch_candidates(c, n, stride);
@@ -3654,21 +3713,25 @@ void build_neighbors(bfm_ctx* c, const double* d_cells, int n, int ld, double th
         // (the one-pass scan measured no faster than hipCUB on the n + 1 row counts)
         LaunchScope ls(c, "scan_counts");
         size_t tb = c->temp_bytes;
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, c->d_cnt64, c->d_off64, n + 1, c->stream));
+        hipLaunchKernelGGL(k_add_u64, dim3(grid_for(n + 1, 256)), dim3(256), 0, c->stream, n + 1, c->d_cnt64,
+                           c->d_own64, c->d_tot64);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->d_temp, tb, c->d_tot64, c->d_off64, n + 1, c->stream));
     }
     if (fuse) {
         const FusedForces ff{force_params(c), d_cells, ld, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd, c->tab_ng,
                              c->d_rec};
         KLAUNCH(c, "compact_rows", k_compact_rows<true>, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride,
-                c->slot_cap, c->d_cnt64, c->d_off64, c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi, c->pj,
-                c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status, ff);
+                c->slot_cap, c->d_cnt64, c->d_own64, c->d_off64, c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi,
+                c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status, ff);
     } else {
         KLAUNCH(c, "compact_rows", k_compact_rows<false>, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride,
-                c->slot_cap, c->d_cnt64, c->d_off64, c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi, c->pj,
-                c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status, FusedForces{});
+                c->slot_cap, c->d_cnt64, c->d_own64, c->d_off64, c->d_slot_j, c->d_slot_rec, c->max_pairs, c->pi,
+                c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt, c->d_csr, c->d_jrow, c->d_m, c->d_status,
+                FusedForces{});
     }
-    KLAUNCH(c, "compact_jrole", k_compact_jrole, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride, c->slot_cap, c->d_cnt64,
-            c->d_off64, c->d_jslot, c->d_jrow, c->max_pairs, c->d_csr);
+    KLAUNCH(c, "compact_jrole", k_compact_jrole, grid_for(8 * (int64_t)n, 256), 256, 0, n, stride, c->slot_cap,
+            c->d_cnt64, c->d_own64, c->d_off64, c->d_jslot, c->d_jrow, c->max_pairs, c->d_csr);
     c->h_m = -1;
     c->list_csr = true;
 }
@@ -4861,6 +4924,8 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         dalloc(&c->d_perm_j, P);
         dalloc(&c->d_rec, 8 * P);
         dalloc(&c->d_cnt64, N + 1);
+        dalloc(&c->d_own64, N + 1);
+        dalloc(&c->d_tot64, N + 1);
         dalloc(&c->d_off64, N + 1);
         dalloc(&c->d_jslot, N * c->slot_cap);
         dalloc(&c->d_jslotk, N * c->slot_cap);
@@ -4880,7 +4945,7 @@ int bfm_create(bfm_ctx** out, int device, int max_cells, int64_t max_pairs)
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, c->d_bin_count, c->d_bin_off, c->max_bins + 1));
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t3, c->d_count, c->d_offsets, max_cells + 1));
         size_t t4 = 0;
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, c->d_cnt64, c->d_off64, max_cells + 1));
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, c->d_tot64, c->d_off64, max_cells + 1));
         c->temp_bytes = std::max(std::max(t1, t4), std::max(t2, t3));
         HIPCHK(hipMalloc(&c->d_temp, c->temp_bytes));
         HIPCHK(hipDeviceSynchronize());
@@ -4902,7 +4967,7 @@ int bfm_destroy(bfm_ctx* c)
                     c->d_err6, c->d_assume2d, c->d_partials, c->d_plan, c->d_keys, c->d_keys_alt, c->d_vals,
                     c->d_vals_alt, c->d_bin_count, c->d_bin_off, c->d_sorted, c->d_sortf, c->d_sgeom, c->d_sid, c->d_skey, c->d_count,
                     c->d_offsets, c->d_slot_j, c->d_slot_rec, c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt,
-                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_off64, c->d_jslot, c->d_jslotk, c->d_ordj, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
+                    c->d_m, c->d_joff, c->d_icnt, c->d_jcnt, c->d_perm_i, c->d_perm_j, c->d_rec, c->d_cnt64, c->d_own64, c->d_tot64, c->d_off64, c->d_jslot, c->d_jslotk, c->d_ordj, c->d_jrow, c->d_csr, c->d_cc, c->d_rep, c->d_grid, c->d_rep_box,
                     c->d_k, c->d_errmax, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->d_temp, c->d_frec, c->d_zst, c->d_geom,
                     c->d_an_grid, c->d_an_curv, c->d_an_fr, c->d_nst, c->d_adhere, c->d_amap, c->d_tile_sum, c->d_scan_flag};
     for (void* q : ptrs)
@@ -5276,11 +5341,11 @@ int bfm_dev_stage(bfm_ctx* c, const double* d_cells, int n, int ld, double thr, 
             const FusedForces ff{force_params(c), d_cells, ld, c->d_tab_ov, c->d_tab_g, c->d_tab_r, c->tab_nd,
                                  c->tab_ng, nullptr};
             KLAUNCH(c, "pair_forces", k_slot_order<kSlotLanes>, grid_for(kSlotLanes * (int64_t)n, 256), 256, 0, n, ff,
-                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_j, c->d_slot_rec, c->d_jslot,
-                    c->d_jslotk, c->d_jrow, c->d_ordj, c->d_status);
+                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_own64, c->d_slot_j, c->d_slot_rec,
+                    c->d_jslot, c->d_jslotk, c->d_jrow, c->d_ordj, c->d_status);
             KLAUNCH(c, "velocities", k_velocities_slots, grid_for(n, 256), 256, 0, n, d_cells, ld, vel_params(c),
-                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_slot_rec, c->d_jrow, c->d_ordj, d_vel,
-                    c->d_status);
+                    (int64_t)c->max_cells, c->slot_cap, c->d_cnt64, c->d_own64, c->d_slot_rec, c->d_jrow,
+                    c->d_ordj, d_vel, c->d_status);
             return;
         }
         // an isotropic interaction's pair forces come out of the compaction
