@@ -339,9 +339,9 @@ def main():
                     unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
                     traffic=profiled_traffic(dom, n),
                     engine="FP64 vector ALU: MI355X's FP64 vector and matrix peaks are both 78.6 "
-                           "TFLOP/s; the kernel is bound by its two returning device-scope atomics per kept "
-                           "pair (the per-cell slots) and the latency of its screen-test chain, not by "
-                           "either peak",
+                           "TFLOP/s; the kernel is bound by the returning device-scope atomic per kept pair "
+                           "(the partner's per-cell slot), the VALU issue of its FP32 screen loop and the "
+                           "latency of its screen-test chain, not by either peak",
                     avg_us=per_kernel[dom] * 1e3, fp64_flops_per_launch=flops64,
                     tests_per_launch=tests_per_launch,
                     fp32_screen={"screened_per_launch": screened_per_launch,
