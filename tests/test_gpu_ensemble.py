@@ -2,9 +2,13 @@
 runs of the same replicates with the same division streams.
 
 The batched loop shares the neighbor build (per-replicate grids, no pairs
-across replicates), one captured graph per RK attempt and per-replicate dt,
+across replicates), one kernel per RK attempt and per-replicate dt,
 controller and division; with the list rebuilt every iteration in both runs
-the arithmetic is the same, so each replicate must come out bit-identical."""
+the arithmetic is the same, so each replicate must come out bit-identical.
+The 64- and 48-replicate batches run the controller's steps as kernels of
+their own (more than 16 work-groups) while most single runs fold them into
+the attempt's last work-group, so the two forms are checked against each
+other too."""
 import numpy as np
 import pytest
 
