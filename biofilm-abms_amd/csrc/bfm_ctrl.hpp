@@ -28,11 +28,13 @@ struct CtrlState {
     int* failed;         // frozen after a non-finite attempt
     long long* iters;
     int* pending;        // one int: replicates with an attempt still to take (device copy)
+    long long* totals;   // device copies of CtrlPub's cells_total, attempts, failed_total
 };
 
 struct CtrlConsts {
     double min_error, p1, min_step, max_step;   // p1 = 1 / (order + 1)
     int max_tries, n_rep, stages;
+    double max_time;      // the run's simulated-time limit (0: none)
 };
 
 // Published after each controller step (host-mapped memory); per replicate
@@ -44,6 +46,18 @@ struct CtrlPub {
     int ndiv;             // cells past the division length (end of the iteration)
     int status;           // the context's status flags at the end of the iteration
     long long attempts;   // RK attempts taken by at least one replicate so far
+};
+
+// What the host reads at the end of an iteration (host-mapped, one per
+// parity of the iteration so that the next iteration may already run):
+// whether the tail ran (0: a replicate still retries), the next division
+// count, the status flags, whether the next iteration may run without the
+// host (spec), the running totals, then per replicate t, dt, err (doubles)
+// and stepped | failed << 1 (ints).
+struct IterSnap {
+    int tail_ran, ndiv, status, spec;
+    long long cells_total, attempts;
+    long long failed_total;
 };
 
 // The ordered key of -inf, the start of an attempt's scaled-error maximum
@@ -59,7 +73,7 @@ __device__ __forceinline__ double ctrl_key_value(unsigned long long k)
 // Block-wide sums into pub: the next attempt's replicates and cells, the
 // replicates frozen by the last attempt, and whether any replicate took it.
 __device__ __forceinline__ void ctrl_publish(int pending, long long cells, int newf, int took, CtrlPub* pub,
-                                             int* dev_pending)
+                                             int* dev_pending, long long* totals)
 {
     __shared__ int s_p[4], s_f[4], s_t[4];
     __shared__ long long s_c[4];
@@ -78,6 +92,9 @@ __device__ __forceinline__ void ctrl_publish(int pending, long long cells, int n
         pub->cells_total += (s_c[0] + s_c[1]) + (s_c[2] + s_c[3]);
         pub->failed_total += (s_f[0] + s_f[1]) + (s_f[2] + s_f[3]);
         if ((s_t[0] + s_t[1]) + (s_t[2] + s_t[3]) > 0) pub->attempts += 1;
+        totals[0] = pub->cells_total;
+        totals[1] = pub->attempts;
+        totals[2] = pub->failed_total;
     }
 }
 
@@ -98,6 +115,7 @@ struct CtrlAfter {
     const int* first_go;
     int control;
     int* div_count;
+    const int* gate;      // non-null: the attempt and this step run only if *gate != 0
 };
 
 // After an attempt: each replicate that took it moves on (a non-finite
@@ -108,6 +126,7 @@ __device__ __forceinline__ void ctrl_after_block(const CtrlAfter& A)
 {
     const CtrlState& S = A.S;
     const CtrlConsts& K = A.K;
+    if (A.gate && *A.gate == 0) return;
     if (!A.first_go && *S.pending == 0) return;
     int pending = 0, newf = 0, took = 0;
     long long cells = 0;
@@ -175,7 +194,7 @@ __device__ __forceinline__ void ctrl_after_block(const CtrlAfter& A)
         S.state[r] = st;
         if (A.active[r]) { ++pending; cells += A.starts[r + 1] - A.starts[r]; }
     }
-    ctrl_publish(pending, cells, newf, took, A.pub, S.pending);
+    ctrl_publish(pending, cells, newf, took, A.pub, S.pending, S.totals);
 }
 
 // What the end of an iteration writes: time and iteration counts of the
@@ -209,6 +228,26 @@ __device__ __forceinline__ void ctrl_end_block(const CtrlEnd& E)
         E.h_err[r] = S.err[r];
         E.h_flags[r] = on | (S.failed[r] << 1);
     }
+}
+
+// The end-of-iteration values for the host (IterSnap) once the tail ran:
+// the division count, the status flags, the totals and whether the next
+// iteration may go ahead without the host: no division, no flag, one
+// replicate inside its time limit and not failed (the host's other
+// termination rules are known to it in advance). One thread.
+__device__ __forceinline__ int publish_iteration(const CtrlEnd& E, int ndiv, int status, IterSnap* snap)
+{
+    const CtrlState& S = E.S;
+    int spec = ndiv == 0 && status == 0 && E.K.n_rep == 1 && !S.failed[0] &&
+               (E.K.max_time <= 0 || S.t[0] < E.K.max_time);
+    snap->ndiv = ndiv;
+    snap->status = status;
+    snap->cells_total = S.totals[0];
+    snap->attempts = S.totals[1];
+    snap->failed_total = S.totals[2];
+    snap->spec = spec;
+    snap->tail_ran = 1;
+    return spec;
 }
 
 // The last work-group of a grid to get here (a ticket after a device-scope

@@ -19,8 +19,10 @@ namespace bfm {
 // list's distances are first brought to the packed segments that
 // iteration_tail_batched(pack_segments) wrote. *cc_in_attempt is set when
 // the cell-cell sums are left to the iteration's first attempt
-// (attempt_direct with gather).
-int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_distances, bool* cc_in_attempt);
+// (attempt_direct with gather). d_gate non-null (an iteration enqueued ahead
+// of the host's check, iteration_ahead_ok): nothing runs unless *d_gate.
+int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_distances, bool* cc_in_attempt,
+                 const int* d_gate = nullptr);
 
 // One RK attempt of a batch of replicates after step_prepare: the stages and
 // the update, with the cell count read from d_n and each replicate's dt and
@@ -40,15 +42,26 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
 // step (E), the columns the attempts did not write copied from d_cells into
 // d_cells_new, growCells on d_cells_new, the indices of the grown cells with
 // l > Ldiv appended to d_idx through *d_count (zeroed by the controller's
-// first step of the iteration), then *d_count and the context's status
-// flags to d_pub->ndiv / d_pub->status. n (the host's count, equal to *d_n)
-// sizes the grid; d_ticket: one zeroed unsigned of scratch. Nothing when
-// *d_skip != 0. pack_segments: the new state's packed segments also go to
+// first step of the iteration), then the iteration's values for the host
+// in *d_snap (IterSnap: the division count, the status flags, the totals,
+// the per-replicate t, dt, err, flags after it) and in *d_spec whether the
+// next iteration may run ahead (d_gate of its kernels). n (the host's
+// count, equal to *d_n) sizes the grid; d_ticket: one zeroed unsigned of
+// scratch. Nothing when *d_gate == 0 (d_gate non-null); when *d_skip != 0
+// only d_snap->tail_ran = 0 and *d_spec = 0. pack_segments: the new state's
+// packed segments also go to
 // the context's segment buffer, from which step_prepare(refresh_distances)
 // brings the list's distances to the new state.
 int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
                            const CtrlEnd& E, double R, double Ldiv, double* d_cells_new, int* d_count, int* d_idx,
-                           const int* d_skip, bool pack_segments, CtrlPub* d_pub, unsigned* d_ticket);
+                           const int* d_skip, bool pack_segments, const int* d_gate, IterSnap* d_snap,
+                           int* d_spec, unsigned* d_ticket);
+
+// Whether an iteration of n cells may be enqueued before the host has read
+// the previous one (its kernels gated on the previous tail's *d_spec): the
+// one-kernel attempts with their controller steps in the last work-group,
+// a device-built list, the exact or tabulated JKR radius.
+bool iteration_ahead_ok(const bfm_ctx* c, int n);
 
 // One attempt as a direct launch (attempt_fused configurations) followed by
 // the controller's step A (ctrl_after_block), the grid sized for n cells

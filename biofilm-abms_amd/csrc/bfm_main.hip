@@ -1752,6 +1752,7 @@ struct DistRefresh {
     const double* geom;
     int n;                  // rows naming a cell >= n keep their stored values
     double *dx, *dy, *dz, *s, *t;
+    const int* gate;        // non-null: the kernel runs only if *gate != 0
 };
 
 // Per-pair force magnitudes (the scalar factors of cellCellInteractionForces,
@@ -1779,6 +1780,7 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
               int* __restrict__ jcnt, int* __restrict__ status, DistRefresh rf)
 {
     extern __shared__ double s_grid[];
+    if (rf.gate && *rf.gate == 0) return;
     if (ANISO) {
         for (int q = threadIdx.x; q < an.total; q += blockDim.x) s_grid[q] = an.grid[q];
         __syncthreads();
@@ -2989,6 +2991,7 @@ k_rk_attempt_b(const int* __restrict__ dn, const double* __restrict__ cells, int
                int* __restrict__ bad_rep, CtrlAfter A, unsigned* __restrict__ ticket)
 {
     const int* first_go = A.first_go;
+    if (A.gate && *A.gate == 0) return;   // an iteration enqueued ahead that may not run
     // an attempt after every replicate finished its iteration: nothing to do
     // (read before any work-group can reach the controller step below)
     if (!first_go && A.S.pending && *A.S.pending == 0) return;
@@ -3150,18 +3153,29 @@ __global__ void __launch_bounds__(256) k_ctrl_after_step(CtrlAfter A) { ctrl_aft
 // count (zeroed by the controller's first step of the iteration); geom (when
 // not null) receives the new state's packed segments as k_pack_segs writes
 // them. The replicates that stepped and their dt come from the controller
-// state, whose end-of-iteration step (ctrl_end_block) work-group 0 runs;
-// ticket non-null: the last work-group publishes the division count and the
-// status flags to pub. Nothing when *skip != 0 (a replicate still retries).
+// state, whose end-of-iteration step (ctrl_end_block, writing the per-replicate
+// values after snap) one work-group runs. Nothing when *gate == 0 (gate
+// non-null: an iteration enqueued ahead that may not run); only
+// snap->tail_ran = 0 and *spec_out = 0 when *skip != 0 (a replicate still
+// retries). ticket non-null: the last work-group runs the end step and
+// publishes the iteration (snap, *spec_out); without a ticket work-group 0
+// runs the end step and k_publish_tail publishes.
 __global__ void __launch_bounds__(256)
 k_iter_tail_b(const int* __restrict__ dn, int ncols, const double* __restrict__ cells, int ld,
               const int* __restrict__ rep, CtrlEnd E, double R, double Ldiv, double* __restrict__ cells_new,
               int* __restrict__ count, int* __restrict__ idx, double* __restrict__ geom,
-              const int* __restrict__ skip, CtrlPub* pub, const int* __restrict__ status,
-              unsigned* __restrict__ ticket)
+              const int* __restrict__ skip, const int* __restrict__ gate, IterSnap* snap,
+              int* __restrict__ spec_out, const int* __restrict__ status, unsigned* __restrict__ ticket)
 {
-    if (*skip) return;
-    if (blockIdx.x == 0) ctrl_end_block(E);
+    if (gate && *gate == 0) return;
+    if (*skip) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            snap->tail_ran = 0;
+            *spec_out = 0;
+        }
+        return;
+    }
+    if (!ticket && blockIdx.x == 0) ctrl_end_block(E);
     const int n = *dn;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         const int r = rep[c];
@@ -3187,16 +3201,23 @@ k_iter_tail_b(const int* __restrict__ dn, int ncols, const double* __restrict__ 
             gp[3] = make_double2(hl, 0.0);
         }
     }
-    if (ticket && last_block_done(ticket) && threadIdx.x == 0) {
-        pub->ndiv = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        pub->status = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ticket && last_block_done(ticket)) {
+        ctrl_end_block(E);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            *spec_out = publish_iteration(E, __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                          __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                          snap);
     }
 }
 
-__global__ void k_publish_tail(const int* __restrict__ count, const int* __restrict__ status, CtrlPub* pub)
+// The publication of a tail without a ticket (see k_iter_tail_b).
+__global__ void k_publish_tail(const int* __restrict__ count, const int* __restrict__ status, CtrlEnd E,
+                               const int* __restrict__ skip, const int* __restrict__ gate, IterSnap* snap,
+                               int* __restrict__ spec_out)
 {
-    pub->ndiv = *count;
-    pub->status = *status;
+    if ((gate && *gate == 0) || *skip) return;
+    *spec_out = publish_iteration(E, *count, *status, snap);
 }
 
 // growCells (growth.hpp:51-58) with each replicate's dt (inactive: unchanged).
@@ -3762,11 +3783,11 @@ void build_neighbors_sync(bfm_ctx* c, const double* d_cells, int n, int ld, doub
 void build_transposes(bfm_ctx* c, int n);
 
 void pair_forces_and_transpose(bfm_ctx* c, const double* d_cells, int ld, int n,
-                               const double* sur_radius, bool refresh = false)
+                               const double* sur_radius, bool refresh = false, const int* gate = nullptr)
 {
     ForceParams fp = force_params(c);
     DistRefresh rf{};
-    if (refresh) rf = DistRefresh{c->d_geom, n, c->pdx, c->pdy, c->pdz, c->ps, c->pt};
+    if (refresh) rf = DistRefresh{c->d_geom, n, c->pdx, c->pdy, c->pdz, c->ps, c->pt, gate};
     const bool need_t = !c->list_csr && !c->transposed;
     if (need_t)
         KLAUNCH(c, "fill", k_fill_int, grid_for(n + 1, 256, 4096), 256, 0, c->d_jcnt, 0, (int64_t)n + 1);
@@ -3863,7 +3884,8 @@ bool fused_forces_ok(const bfm_ctx* c)
 // refresh: the list's distances are first brought to the cells' packed
 // segments in c->d_geom (updateNeighborDistances), inside the pair-force
 // kernel where nothing reads them before it.
-void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh = false)
+void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh = false,
+                    const int* gate = nullptr)
 {
     const double* sur = nullptr;
     const bool sur_on = c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC && c->p.jkr_mode == BFM_JKR_SURROGATE;
@@ -3880,7 +3902,7 @@ void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld, bool refre
     if ((c->p.jkr_mode == BFM_JKR_TABLE_1D || c->p.jkr_mode == BFM_JKR_TABLE_2D) &&
         c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC)
         require(c->d_tab_ov != nullptr, BFM_ERR_ARG, "JKR table not set (bfm_set_jkr_table)");
-    pair_forces_and_transpose(c, d_cells, ld, n, sur, refresh);
+    pair_forces_and_transpose(c, d_cells, ld, n, sur, refresh, gate);
 }
 
 void launch_velocities(bfm_ctx* c, const double* d_cells, int n, int ld, const int* d_a2d,
@@ -4670,13 +4692,16 @@ bool attempt_fused(const bfm_ctx* c)
     return c->p.adhesion_mode != BFM_ADHESION_JKR_ANISOTROPIC && !friction_on(c);
 }
 
-int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_distances, bool* cc_in_attempt)
+int step_prepare(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh_distances, bool* cc_in_attempt,
+                 const int* d_gate)
 {
     return guarded(c, [&] {
         need_params(c);
         check_cells(c, n, ld);
         require(c->list_n == n, BFM_ERR_ARG, "neighbor list was built for a different number of cells");
-        compute_forces(c, d_cells, n, ld, refresh_distances);
+        require(!d_gate || (refresh_distances && iteration_ahead_ok(c, n)), BFM_ERR_ARG,
+                "a gated step needs the one-kernel attempts and the list's refresh");
+        compute_forces(c, d_cells, n, ld, refresh_distances, d_gate);
         const bool fused = attempt_fused(c);
         if (!fused)   // the one-launch attempt tests nz itself
             KLAUNCH(c, "assume2d", k_assume2d, grid_for(n, 256), 256, 0, n, d_cells, ld, c->p.nz_threshold,
@@ -4751,9 +4776,16 @@ int step_attempt_batched(bfm_ctx* c, const Tableau& tb, const double* d_cells, i
     });
 }
 
+bool iteration_ahead_ok(const bfm_ctx* c, int n)
+{
+    return attempt_fused(c) && c->list_csr && c->p.jkr_mode != BFM_JKR_SURROGATE &&
+           grid_for(n, 256) <= kCtrlFuseGroups;
+}
+
 int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int* d_n, int n, const int* d_rep,
                            const CtrlEnd& E, double R, double Ldiv, double* d_cells_new, int* d_count, int* d_idx,
-                           const int* d_skip, bool pack_segments, CtrlPub* d_pub, unsigned* d_ticket)
+                           const int* d_skip, bool pack_segments, const int* d_gate, IterSnap* d_snap,
+                           int* d_spec, unsigned* d_ticket)
 {
     return guarded(c, [&] {
         check_cells(c, n, ld);
@@ -4761,10 +4793,10 @@ int iteration_tail_batched(bfm_ctx* c, const double* d_cells, int ld, const int*
         unsigned* ticket = grid <= kCtrlFuseGroups ? d_ticket : nullptr;
         hipLaunchKernelGGL(k_iter_tail_b, dim3(grid), dim3(256), 0, c->stream, d_n, c->p.ncols, d_cells, ld, d_rep,
                            E, R, Ldiv, d_cells_new, d_count, d_idx, pack_segments ? c->d_geom : (double*)nullptr,
-                           d_skip, d_pub, (const int*)c->d_status, ticket);
+                           d_skip, d_gate, d_snap, d_spec, (const int*)c->d_status, ticket);
         if (!ticket)
             hipLaunchKernelGGL(k_publish_tail, dim3(1), dim3(1), 0, c->stream, (const int*)d_count,
-                               (const int*)c->d_status, d_pub);
+                               (const int*)c->d_status, E, d_skip, d_gate, d_snap, d_spec);
         HIPCHK(hipGetLastError());
     });
 }

@@ -705,7 +705,7 @@ k_ctrl_begin(CtrlState S, CtrlConsts K, const int* __restrict__ go, int control,
         trial[r] = S.dt[r];
         if (on) { ++pending; cells += starts[r + 1] - starts[r]; }
     }
-    ctrl_publish(pending, cells, 0, 0, pub, S.pending);
+    ctrl_publish(pending, cells, 0, 0, pub, S.pending, S.totals);
 }
 
 __global__ void __launch_bounds__(256) k_ctrl_after(CtrlAfter A) { ctrl_after_block(A); }
@@ -1228,19 +1228,45 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     DevBuf<long long> d_citers(R);
     DevBuf<int> d_cpending(1);
     DevBuf<unsigned> d_tick(2);   // last-work-group tickets of the attempt and tail kernels
-    MappedBuf pub(sizeof(CtrlPub) + (size_t)R * (3 * sizeof(double) + sizeof(int)));
+    MappedBuf pub(sizeof(CtrlPub));
     CtrlPub* h_pub = (CtrlPub*)pub.h;
     CtrlPub* d_pub = (CtrlPub*)pub.d;
-    double* h_pt = (double*)((char*)pub.h + sizeof(CtrlPub));
-    double* h_pdt = h_pt + R;
-    double* h_perr = h_pdt + R;
-    int* h_pflags = (int*)(h_perr + R);
-    double* d_pt = (double*)((char*)pub.d + sizeof(CtrlPub));
-    double* d_pdt = d_pt + R;
-    double* d_perr = d_pdt + R;
-    int* d_pflags = (int*)(d_perr + R);
+    // the end-of-iteration values, one block per parity of the iteration
+    // (IterSnap, then per replicate t, dt, err and flags)
+    const size_t snap_stride = (sizeof(IterSnap) + (size_t)R * (3 * sizeof(double) + sizeof(int)) + 255) / 256 * 256;
+    MappedBuf snapbuf(2 * snap_stride);
+    IterSnap* snap_h[2];
+    IterSnap* snap_d[2];
+    double *t_h[2], *dt_h[2], *err_h[2];
+    int* flags_h[2];
+    double *t_d[2], *dt_d[2], *err_d[2];
+    int* flags_d[2];
+    for (int q = 0; q < 2; ++q) {
+        snap_h[q] = (IterSnap*)((char*)snapbuf.h + q * snap_stride);
+        snap_d[q] = (IterSnap*)((char*)snapbuf.d + q * snap_stride);
+        t_h[q] = (double*)(snap_h[q] + 1);
+        dt_h[q] = t_h[q] + R;
+        err_h[q] = dt_h[q] + R;
+        flags_h[q] = (int*)(err_h[q] + R);
+        t_d[q] = (double*)(snap_d[q] + 1);
+        dt_d[q] = t_d[q] + R;
+        err_d[q] = dt_d[q] + R;
+        flags_d[q] = (int*)(err_d[q] + R);
+    }
+    DevBuf<long long> d_totals(3);
+    DevBuf<int> d_spec(1);
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    for (int q = 0; q < 2; ++q) SCHK(hipEventCreateWithFlags(&evs.e[q], hipEventDisableTiming));
+    hipEvent_t* ev_end = evs.e;
     const CtrlState cs{d_cdt.p, d_cfactor.p, d_ct.p, d_cerr.p, d_ctries.p, d_cstate.p, d_cstepped.p, d_cfailed.p,
-                       d_citers.p, d_cpending.p};
+                       d_citers.p, d_cpending.p, d_totals.p};
     PinnedBuf h_dt(sizeof(double) * R), h_act(sizeof(int) * R), h_bad(sizeof(int) * R);
     int* p_bad = (int*)h_bad.p;
     SCHK(hipMemsetAsync(d_bad.p, 0, sizeof(int) * R, st));
@@ -1318,6 +1344,8 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         SCHK(hipMemcpyAsync(d_cfailed.p, zi.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemcpyAsync(d_citers.p, zl.data(), sizeof(long long) * R, hipMemcpyHostToDevice, st));
         SCHK(hipMemsetAsync(d_cpending.p, 0, sizeof(int), st));
+        SCHK(hipMemsetAsync(d_totals.p, 0, sizeof(long long) * 3, st));
+        SCHK(hipMemsetAsync(d_spec.p, 0, sizeof(int), st));
         SCHK(hipMemsetAsync(d_tick.p, 0, sizeof(unsigned) * 2, st));
         // every attempt starts from the key of -inf (k_ctrl_after resets it)
         hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(R, 256)), dim3(256), 0, st, d_errmax.p, R, kKeyNegInf);
@@ -1439,7 +1467,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     double last_frame_wall = 0;
     std::vector<double> frame_buf;
     const CtrlConsts kc{cfg->min_error, 1.0 / (tdef.order + 1), cfg->min_stepsize, cfg->max_stepsize,
-                        cfg->max_tries_update_stepsize, R, tb.s};
+                        cfg->max_tries_update_stepsize, R, tb.s, cfg->max_time};
     std::vector<int> go_host(R, -1);
     // the replicates that take this iteration (the host's termination rules)
     auto upload_go = [&] {
@@ -1457,29 +1485,107 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
     // failures (a non-finite result freezes its replicate at the state the
     // iteration started from; the reference stops such a run,
     // simulation.hpp:1803-1822; the others go on)
+    unsigned iter_par = 0;   // parity of the iteration being processed (its IterSnap block)
+    bool expect_retries = false;   // a single colony's next iterations are likely to retry
     auto take_iteration = [&] {
+        const IterSnap* sn = snap_h[iter_par];
+        const double* pt = t_h[iter_par];
+        const double* pdt = dt_h[iter_par];
+        const int* pflags = flags_h[iter_par];
         for (int r = 0; r < R; ++r) {
-            const int fl = h_pflags[r];
+            const int fl = pflags[r];
             if (fl & 1) {
-                if (io && io->dt_trace && r == 0 && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = h_pdt[0];
+                if (io && io->dt_trace && r == 0 && iters[0] < io->cap_trace) io->dt_trace[iters[0]] = pdt[0];
                 iters[r]++;
             }
-            t[r] = h_pt[r];
-            dt[r] = h_pdt[r];
+            t[r] = pt[r];
+            dt[r] = pdt[r];
             if ((fl & 2) && !failed[r]) {
                 failed[r] = 1;
                 done[r] = 1;
                 ++n_failed;
             }
         }
-        last_err0 = h_perr[0];
-        stage_updates = h_pub->cells_total * tb.s;
-        attempts = h_pub->attempts;
+        last_err0 = err_h[iter_par][0];
+        stage_updates = sn->cells_total * tb.s;
+        // below the largest step the controller sets dt from the error and
+        // retries are frequent
+        expect_retries = dt[0] < cfg->max_stepsize;
+        attempts = sn->attempts;
     };
-    h_pub->ndiv = -1;
+    int ndiv_next = -1;        // the next iteration's dividing cells (-1: not known)
+    bool ahead_ran = false;    // the iteration to process was enqueued ahead and ran
+    const bool pipelined = !verlet && direct && R == 1;
+    int cur_control = 1;
+    const int* cur_gate = nullptr;
+    // the controller's step after an attempt (first_go: after the
+    // iteration's first attempt, which set nothing up before it)
+    auto after = [&](const int* first_go) {
+        return CtrlAfter{cs, kc, d_errmax.p, d_bad.p, d_starts.p, d_dt.p, d_active.p, d_pub, first_go, cur_control,
+                         d_count.p, cur_gate};
+    };
+    // one attempt after the first and the controller's step: a direct
+    // launch (one kernel, the step in its last work-group for small
+    // batches) or the captured graph of the staged attempt
+    auto next_attempt = [&] {
+        if (direct) {
+            abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, false, d_new, cfg->max_error_allowed,
+                                  after(nullptr), d_tick.p));
+        } else {
+            SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
+            hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, after(nullptr));
+        }
+    };
+    // the iteration's end (one kernel and, for large batches, the
+    // publication): the controller's end-of-iteration step, the columns the
+    // attempts leave alone (all columns of replicates that did not step),
+    // growth, the next iteration's dividing cells, the new state's packed
+    // segments (the next step_prepare brings the list's distances to them),
+    // then the iteration's values to the host (IterSnap of parity q). It does
+    // nothing while a replicate still retries; with the one-launch attempts
+    // (which read only the step's frozen sums) it is enqueued before the host
+    // knows that and enqueued again after the retries
+    auto enqueue_end = [&](unsigned q) {
+        const CtrlEnd ce{cs, kc, d_dt.p, d_active.p, t_d[q], dt_d[q], err_d[q], flags_d[q]};
+        abi(c, iteration_tail_batched(c, d_cur, ld, d_n.p, n, c->d_rep, ce, Rr, Ldiv, d_new, d_count.p, d_idx.p,
+                                      d_cpending.p, true, cur_gate, snap_d[q], d_spec.p, d_tick.p + 1));
+        SCHK(hipEventRecord(ev_end[q], st));
+    };
+    // iteration `it` (parity q) from d_cur into d_new; gated: enqueued
+    // before the host has read the previous iteration, its kernels gated on
+    // the previous tail's d_spec
+    auto enqueue_iteration = [&](int64_t it, unsigned q, bool gated) {
+        cur_control = it % cfg->iter_update_stepsize == 0;
+        cur_gate = gated ? d_spec.p : nullptr;
+        bool cc_in_attempt = false;
+        abi(c, step_prepare(c, d_cur, n, ld, !list_fresh, &cc_in_attempt, cur_gate));
+        if (!gated) upload_go();
+        if (direct) {   // the first attempt's controller step sets the iteration up
+            abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, cc_in_attempt, d_new,
+                                  cfg->max_error_allowed, after(d_go.p), d_tick.p));
+        } else {
+            hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, cur_control, d_starts.p,
+                               d_dt.p, d_active.p, d_errmax.p, d_pub, d_count.p);
+            next_attempt();
+        }
+        if (cur_control) next_attempt();
+        // a batch of replicates nearly always has one still retrying after
+        // two attempts, and a single colony whose step the controller holds
+        // below the largest one retries often (an iteration enqueued ahead
+        // would then wait for the host's retries): every attempt the controller may ask for
+        // (max_tries retries and the final re-run) is enqueued at once; those
+        // after the last replicate finished return at once
+        if (direct && (R > 1 || (pipelined && expect_retries)) && cur_control)
+            for (int k = 0; k < cfg->max_tries_update_stepsize; ++k) next_attempt();
+        SCHK(hipGetLastError());
+        if (direct) enqueue_end(q);   // ahead of the retry check
+    };
     double next_report = cfg->report_seconds;
     std::vector<int> idx;
     for (;;) {
+        // an iteration enqueued ahead that ran is processed first (the
+        // termination rules are checked after it)
+        if (!ahead_ran) {
         bool any = false;
         for (int r = 0; r < R; ++r) {
             if (!done[r] && terminate_now(cfg, nr[r], iter, t[r])) done[r] = 1;
@@ -1506,15 +1612,16 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
 
         // divideMaxLength + divideCells per active replicate; the count of
         // the current cells comes with the previous iteration's readback
-        if (h_pub->ndiv < 0) {
+        if (ndiv_next < 0) {
             hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, d_count.p);
             hipLaunchKernelGGL(k_find_dividing, dim3(blocks_for(n, 256)), dim3(256), 0, st, d_cur, ld, n, Ldiv,
                                d_count.p, d_idx.p);
             hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, st, d_count.p, &d_pub->ndiv);
             SCHK(hipStreamSynchronize(st));
+            ndiv_next = h_pub->ndiv;
         }
-        const int nd_all = h_pub->ndiv;
-        h_pub->ndiv = -1;
+        const int nd_all = ndiv_next;
+        ndiv_next = -1;
         if (nd_all > 0) {
             // divideCells of every replicate that steps, on the device
             auto td = std::chrono::steady_clock::now();
@@ -1587,6 +1694,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
                 build();
             }
         }
+        }   // !ahead_ran
         if (verlet) {
             // stepVelocityVerlet (simulation.hpp:1787-1798); it leaves the list's
             // distances at the new positions, refreshed again after growth below
@@ -1613,73 +1721,42 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
         // with the controller on the device: the first attempt, the re-run
         // or first retry (every stepping replicate takes a second attempt when
         // step control is on), then the iteration's end and one
-        // synchronisation; more attempts only for replicates still retrying
-        bool cc_in_attempt = false;
-        abi(c, step_prepare(c, d_cur, n, ld, !list_fresh, &cc_in_attempt));
-        upload_go();
-        const int control = iter % cfg->iter_update_stepsize == 0;
-        // the controller's step after an attempt (first_go: after the
-        // iteration's first attempt, which set nothing up before it)
-        auto after = [&](const int* first_go) {
-            return CtrlAfter{cs, kc, d_errmax.p, d_bad.p, d_starts.p, d_dt.p, d_active.p, d_pub, first_go, control,
-                             d_count.p};
-        };
-        // one attempt after the first and the controller's step: a direct
-        // launch (one kernel, the step in its last work-group for small
-        // batches) or the captured graph of the staged attempt
-        auto next_attempt = [&] {
-            if (direct) {
-                abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, false, d_new, cfg->max_error_allowed,
-                                      after(nullptr), d_tick.p));
-            } else {
-                SCHK(hipGraphLaunch(exec[d_cur == bufA.p ? 0 : 1], st));
-                hipLaunchKernelGGL(k_ctrl_after, dim3(1), dim3(256), 0, st, after(nullptr));
-            }
-        };
-        if (direct) {   // the first attempt's controller step sets the iteration up
-            abi(c, attempt_direct(c, tb, d_cur, ld, d_n.p, n, c->d_rep, cc_in_attempt, d_new,
-                                  cfg->max_error_allowed, after(d_go.p), d_tick.p));
-        } else {
-            hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(256), 0, st, cs, kc, d_go.p, control, d_starts.p, d_dt.p,
-                               d_active.p, d_errmax.p, d_pub, d_count.p);
-            next_attempt();
+        // synchronisation; more attempts only for replicates still retrying.
+        // A single colony on the one-kernel path also enqueues the next
+        // iteration before that synchronisation when the host knows it needs
+        // no rebuild and no termination; its kernels are gated on the tail
+        // (d_spec: no division, no flag, time left) and do nothing otherwise.
+        const unsigned par = iter_par;
+        if (!ahead_ran) enqueue_iteration(iter, par, false);
+        bool ahead = false;
+        if (pipelined && iteration_ahead_ok(c, n) && (iter + 1) % cfg->iter_update_neighbors != 0 &&
+            !(cfg->max_iter > 0 && iter + 1 >= cfg->max_iter) && !(cfg->n_cells > 0 && n >= cfg->n_cells)) {
+            std::swap(d_cur, d_new);   // the next iteration starts from this one's result
+            list_fresh = false;
+            enqueue_iteration(iter + 1, par ^ 1u, true);
+            std::swap(d_cur, d_new);
+            ahead = true;
         }
-        if (control) next_attempt();
-        // a batch of replicates nearly always has one still retrying after
-        // two attempts: every attempt the controller may ask for (max_tries
-        // retries and the final re-run) is enqueued at once; those after the
-        // last replicate finished return at once
-        const bool all_attempts = direct && R > 1;
-        if (all_attempts && control)
-            for (int k = 0; k < cfg->max_tries_update_stepsize; ++k) next_attempt();
-        SCHK(hipGetLastError());
-        // the iteration's end (one kernel and, for large batches, the
-        // publication): the controller's end-of-iteration step, the columns
-        // the attempts leave alone (all columns of replicates that did not
-        // step), growth, the next iteration's dividing cells, the new state's
-        // packed segments (the next step_prepare brings the list's distances
-        // to them), then the division count and the status flags to the host.
-        // It does nothing while a replicate still retries; with the
-        // one-launch attempts (which read only the step's frozen sums) it is
-        // enqueued before the host knows that and enqueued again after the
-        // retries
-        const CtrlEnd ce{cs, kc, d_dt.p, d_active.p, d_pt, d_pdt, d_perr, d_pflags};
-        auto enqueue_end = [&] {
-            abi(c, iteration_tail_batched(c, d_cur, ld, d_n.p, n, c->d_rep, ce, Rr, Ldiv, d_new, d_count.p, d_idx.p,
-                                          d_cpending.p, true, d_pub, d_tick.p + 1));
-        };
-        if (direct) enqueue_end();   // ahead of the retry check
-        SCHK(hipStreamSynchronize(st));
-        if (!direct || h_pub->pending > 0) {
+        if (direct) SCHK(hipEventSynchronize(ev_end[par]));
+        else SCHK(hipStreamSynchronize(st));
+        if (!direct || !snap_h[par]->tail_ran) {   // a replicate still retries: the iteration ahead does nothing
+            ahead = false;
+            cur_control = iter % cfg->iter_update_stepsize == 0;
+            cur_gate = nullptr;
+            SCHK(hipStreamSynchronize(st));
             while (h_pub->pending > 0) {
                 next_attempt();
                 SCHK(hipGetLastError());
                 SCHK(hipStreamSynchronize(st));
             }
-            enqueue_end();
+            enqueue_end(par);
             SCHK(hipStreamSynchronize(st));
         }
-        if (h_pub->status) abi(c, bfm_sync(c));   // reports and clears the flags
+        const IterSnap* sn = snap_h[par];
+        if (sn->status) abi(c, bfm_sync(c));   // reports and clears the flags
+        ahead_ran = ahead && sn->spec;
+        if (ahead && !ahead_ran) SCHK(hipStreamSynchronize(st));   // its kernels found the gate closed
+        ndiv_next = sn->ndiv;
         std::swap(d_cur, d_new);
         list_fresh = false;
         }
@@ -1689,6 +1766,7 @@ void ensemble(bfm_ctx* c, const bfm_sim_config* cfg, int R, const int* sizes, co
                 if (act[r]) { t[r] += dt[r]; iters[r]++; }
         } else {
             take_iteration();
+            iter_par ^= 1u;
         }
         iter++;
         if (iter % cfg->iter_update_neighbors == 0) build();
