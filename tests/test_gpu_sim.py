@@ -70,6 +70,22 @@ def test_founder_growth_matches_oracle(orc, tableau, n_cells, max_iter, zb):
     assert parents[0] == -1 and np.all(parents[1:] < np.arange(1, len(parents)))
 
 
+def test_founder_growth_to_max_time_matches_oracle(orc):
+    """A run that ends on the simulated-time limit (t >= max_time,
+    simulation.hpp's loop condition): the iteration enqueued ahead of the
+    host's check tests the limit on the device, so the run must stop at the
+    oracle's iteration, with its time and state."""
+    from bfm import sim
+    from bfm.core import make_params
+    cfg = sim.sim_config(max_time=0.9, tableau="bogacki_shampine", daughter_angle_xy_bound=0.05, seed=42)
+    c0 = sim.founder_cell()
+    cap = 64
+    got, _, trace, stats = sim.simulate(_ctx(cap), cfg, c0, cap=cap)
+    want, trace_o, iters_o, steps_o = orc.simulate(cfg, make_params(), c0, cap=cap)
+    assert stats["t_end"] >= 0.9 and stats["t_end"] - trace[-1] < 0.9
+    _compare(got, want, trace, trace_o, stats, iters_o, steps_o)
+
+
 def test_tilted_daughters_controller_regime(orc):
     """Daughters tilted out of the plane relax toward n_z = 0, where the
     orientation flip of normalizeOrientations (mechanics.hpp:1946-1953) makes
