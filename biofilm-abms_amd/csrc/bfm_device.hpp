@@ -668,4 +668,45 @@ __device__ __forceinline__ void velocity_solve(const CellVel& c, bool assume_2d,
     }
 }
 
+// The packed 64-byte segment {r, n, half_l, 0} of cell c (k_pack_segs layout).
+__device__ __forceinline__ Seg load_packed_seg(const double* __restrict__ geom, int c)
+{
+    const double2* g = reinterpret_cast<const double2*>(geom + (int64_t)c * 8);
+    const double2 a = g[0], b = g[1], e = g[2], f = g[3];
+    Seg s;
+    s.r[0] = a.x; s.r[1] = a.y; s.r[2] = b.x;
+    s.n[0] = b.y; s.n[1] = e.x; s.n[2] = e.y;
+    s.hl = f.x;
+    return s;
+}
+
+// updateNeighborDistances (mechanics.hpp:604-650) fused into the first
+// kernel that reads the list (the pair forces, or the learned radius's
+// staging): each row's distance vector and contact coordinates recomputed
+// from the cells' packed segments and written back to the list.
+// geom == nullptr: the list's stored values are used.
+struct DistRefresh {
+    const double* geom;
+    int n;                  // rows naming a cell >= n keep their stored values
+    double *dx, *dy, *dz, *s, *t;
+    const int* gate;        // non-null: the kernel runs only if *gate != 0
+};
+
+// Row k's distance vector d and contact coordinates (s, t): refreshed from
+// the packed segments (and written back) or read from the list.
+__device__ __forceinline__ void row_geometry(const DistRefresh& rf, int k, int i, int j,
+                                             const double* __restrict__ pdx, const double* __restrict__ pdy,
+                                             const double* __restrict__ pdz, const double* __restrict__ ps,
+                                             const double* __restrict__ pt, double d[3], double& sv, double& tv)
+{
+    if (rf.geom && (unsigned)i < (unsigned)rf.n && (unsigned)j < (unsigned)rf.n) {
+        const Seg ci = load_packed_seg(rf.geom, i);
+        const Seg cj = load_packed_seg(rf.geom, j);
+        dist_between_cells(ci, cj, d, sv, tv);
+        rf.dx[k] = d[0]; rf.dy[k] = d[1]; rf.dz[k] = d[2]; rf.s[k] = sv; rf.t[k] = tv;
+    } else {
+        d[0] = pdx[k]; d[1] = pdy[k]; d[2] = pdz[k]; sv = ps ? ps[k] : 0.0; tv = pt ? pt[k] : 0.0;
+    }
+}
+
 }  // namespace bfm

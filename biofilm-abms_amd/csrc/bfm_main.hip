@@ -1679,16 +1679,6 @@ __global__ void k_pack_segs(int n, const double* __restrict__ cells, int ld, dou
     }
 }
 
-__device__ __forceinline__ Seg load_packed_seg(const double* __restrict__ geom, int c)
-{
-    const double2* g = reinterpret_cast<const double2*>(geom + (int64_t)c * 8);
-    const double2 a = g[0], b = g[1], e = g[2], f = g[3];
-    Seg s;
-    s.r[0] = a.x; s.r[1] = a.y; s.r[2] = b.x;
-    s.n[0] = b.y; s.n[1] = e.x; s.n[2] = e.y;
-    s.hl = f.x;
-    return s;
-}
 
 // updateNeighborDistances (mechanics.hpp:604-648) on packed segments.
 __global__ void k_update_distances_packed(const int* __restrict__ d_m, const double* __restrict__ geom, int n,
@@ -1744,16 +1734,6 @@ __global__ void k_sort_segments(int n, const int* __restrict__ off, int* __restr
 
 
 
-// updateNeighborDistances (mechanics.hpp:604-650) fused into the pair-force
-// kernel: each row's distance vector and contact coordinates recomputed from
-// the cells' packed segments (k_pack_segs layout) and written back to the
-// list before its force. geom == nullptr: the list's stored values are used.
-struct DistRefresh {
-    const double* geom;
-    int n;                  // rows naming a cell >= n keep their stored values
-    double *dx, *dy, *dz, *s, *t;
-    const int* gate;        // non-null: the kernel runs only if *gate != 0
-};
 
 // Per-pair force magnitudes (the scalar factors of cellCellInteractionForces,
 // mechanics.hpp:1209-1350 and :1390-1659), written with the pair geometry as
@@ -1796,18 +1776,9 @@ k_pair_forces(const int* __restrict__ d_m, const double* __restrict__ cells, int
             continue;
         }
         if (jcnt) atomicAdd(&jcnt[j], 1);
-        double dx, dy, dz, sv, tv;
-        if (rf.geom && (unsigned)i < (unsigned)rf.n && (unsigned)j < (unsigned)rf.n) {
-            // updateNeighborDistances of the row first (as k_update_distances_packed)
-            const Seg ci = load_packed_seg(rf.geom, i);
-            const Seg cj = load_packed_seg(rf.geom, j);
-            double d[3];
-            dist_between_cells(ci, cj, d, sv, tv);
-            dx = d[0]; dy = d[1]; dz = d[2];
-            rf.dx[k] = dx; rf.dy[k] = dy; rf.dz[k] = dz; rf.s[k] = sv; rf.t[k] = tv;
-        } else {
-            dx = pdx[k]; dy = pdy[k]; dz = pdz[k]; sv = ps[k]; tv = pt[k];
-        }
+        double d[3], sv, tv;   // updateNeighborDistances of the row first when rf asks
+        row_geometry(rf, k, i, j, pdx, pdy, pdz, ps, pt, d, sv, tv);
+        const double dx = d[0], dy = d[1], dz = d[2];
         const double dist = norm3(dx, dy, dz);
         const double overlap = 2 * fp.R - dist;
         double mm = 0, mh = 0, jkr = 0;
@@ -3882,14 +3853,14 @@ bool fused_forces_ok(const bfm_ctx* c)
 }
 
 // refresh: the list's distances are first brought to the cells' packed
-// segments in c->d_geom (updateNeighborDistances), inside the pair-force
-// kernel where nothing reads them before it.
+// segments in c->d_geom (updateNeighborDistances), inside the first kernel
+// that reads them (the pair forces, or the learned radius's staging).
 void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld, bool refresh = false,
                     const int* gate = nullptr)
 {
     const double* sur = nullptr;
     const bool sur_on = c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC && c->p.jkr_mode == BFM_JKR_SURROGATE;
-    if (refresh && (sur_on || c->p.adhesion_mode == BFM_ADHESION_JKR_ANISOTROPIC)) {
+    if (refresh && c->p.adhesion_mode == BFM_ADHESION_JKR_ANISOTROPIC) {
         if (n > 0)
             KLAUNCH(c, "update_distances", k_update_distances_packed, kPairGrid, 256, 0, c->d_m, c->d_geom, n,
                     c->pi, c->pj, c->pdx, c->pdy, c->pdz, c->ps, c->pt);
@@ -3897,7 +3868,13 @@ void compute_forces(bfm_ctx* c, const double* d_cells, int n, int ld, bool refre
     }
     if (sur_on) {
         require(c->sur != nullptr, BFM_ERR_ARG, "surrogate weights not set (bfm_set_surrogate)");
-        sur = bfm::surrogate_pair_radii(c, d_cells, ld, n);
+        // the learned radius's staging reads the list first and refreshes it;
+        // its iterations are never enqueued ahead (no gate)
+        require(gate == nullptr, BFM_ERR_ARG, "learned JKR radius with a gated iteration");
+        DistRefresh rf{};
+        if (refresh) rf = DistRefresh{c->d_geom, n, c->pdx, c->pdy, c->pdz, c->ps, c->pt, nullptr};
+        sur = bfm::surrogate_pair_radii(c, d_cells, ld, n, rf);
+        refresh = false;
     }
     if ((c->p.jkr_mode == BFM_JKR_TABLE_1D || c->p.jkr_mode == BFM_JKR_TABLE_2D) &&
         c->p.adhesion_mode == BFM_ADHESION_JKR_ISOTROPIC)

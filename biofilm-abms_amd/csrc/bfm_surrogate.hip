@@ -412,12 +412,14 @@ constexpr int kSurNetBlocks = 640;
 // (gamma = min of the two cells' > 0) and lie inside the envelope into its
 // staging slots [b C, b C + count) (row, capped overlap, gamma), in any order;
 // counts[b] = count. The test is the pair force kernel's, expression for
-// expression, so both kernels agree on which rows the network serves.
+// expression, so both kernels agree on which rows the network serves. rf.geom
+// set: each row's distances are first refreshed from the packed segments
+// (updateNeighborDistances) and written back for the pair force kernel.
 __global__ void __launch_bounds__(256)
 k_sur_stage(const int* __restrict__ d_m, const double* __restrict__ cells, int ld, const int* __restrict__ pi,
             const int* __restrict__ pj, const double* __restrict__ pdx, const double* __restrict__ pdy,
             const double* __restrict__ pdz, RadiusArgs ra, Envelope env, int* __restrict__ st_row,
-            double* __restrict__ st_delta, double* __restrict__ st_gamma, int* __restrict__ counts)
+            double* __restrict__ st_delta, double* __restrict__ st_gamma, int* __restrict__ counts, DistRefresh rf)
 {
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
@@ -433,7 +435,9 @@ k_sur_stage(const int* __restrict__ d_m, const double* __restrict__ cells, int l
         double delta = 0, gamma = 0;
         // rows of a build that overflowed its capacity (reported by status) may be stale
         if (k < hi && (unsigned)pi[k] < (unsigned)ld && (unsigned)pj[k] < (unsigned)ld) {
-            const double dist = norm3(pdx[k], pdy[k], pdz[k]);
+            double d[3], sv, tv;
+            row_geometry(rf, k, pi[k], pj[k], pdx, pdy, pdz, nullptr, nullptr, d, sv, tv);
+            const double dist = norm3(d[0], d[1], d[2]);
             const double overlap = 2 * ra.R - dist;
             if (overlap > 0) {
                 gamma = dmin(cells[(int64_t)ra.colidx_gamma * ld + pi[k]],
@@ -802,7 +806,7 @@ int fail(bfm_ctx* c, int code, const std::string& msg)
 
 namespace bfm {
 
-const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, int n)
+const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, int n, const DistRefresh& rf)
 {
     (void)n;
     SurrogateWeights* s = c->sur;
@@ -825,7 +829,7 @@ const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, in
     bfm::launch_timed(c, "sur_stage", [&] {
         hipLaunchKernelGGL(k_sur_stage, dim3(kSurStageBlocks), dim3(256), 0, c->stream, c->d_m, d_cells, ld, c->pi,
                            c->pj, c->pdx, c->pdy, c->pdz, ra, s->env, s->st_row, s->st_delta, s->st_gamma,
-                           s->st_count);
+                           s->st_count, rf);
     });
     bfm::launch_timed(c, "sur_network", [&] {
         hipLaunchKernelGGL(k_sur_network, dim3(kSurNetBlocks), dim3(256), 0, c->stream, c->d_m, s->st_count, s->st_row,

@@ -9,6 +9,8 @@
 
 namespace bfm {
 
+struct DistRefresh;   // bfm_device.hpp
+
 // Training envelope of the learned radius: overlap delta in [dlo R, dhi R]
 // and gamma in [glo, ghi] Pa um. Rows inside it take the network's radius;
 // the others the exact FP64 solve.
@@ -32,8 +34,10 @@ void launch_timed(bfm_ctx* c, const char* name, const std::function<void()>& lau
 // Network contact radius for the pairs of the context's neighbor list that
 // overlap, adhere and lie inside the training envelope (device array of
 // max_pairs doubles, indexed by row; other rows are not written: the pair
-// force kernel solves them exactly, by the same sur_inside test).
-const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, int n);
+// force kernel solves them exactly, by the same sur_inside test). rf.geom
+// set: the list's distances are refreshed from the packed segments first
+// (updateNeighborDistances), in the staging kernel.
+const double* surrogate_pair_radii(bfm_ctx* c, const double* d_cells, int ld, int n, const DistRefresh& rf);
 void surrogate_free(bfm_ctx* c);
 
 }  // namespace bfm
